@@ -1,0 +1,254 @@
+"""Host-side mirror of the fastPLS R API for ``backend = "hip"``.
+
+``pls()``, ``predict()``, ``fastsvd()``, ``pls_single_cv()`` and
+``optim_pls_cv()`` keep the reference's argument names and meanings
+(R/main.R:5860 ``pls``, :2577 ``predict.fastPLS``, :4818 ``fastsvd``,
+:4316 ``pls.single.cv``, :6399 ``optim.pls.cv``); the numerical work runs in
+the gfx950 library through the C ABI (``include/fastpls_hip.h``).  Returned
+models are dicts with the reference's field names plus a ``handle`` to the
+HBM-resident model.  Only ``backend="hip"`` exists here: anything else raises,
+and there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import as_ivec, as_mat, int_array
+
+SCALINGS = {"centering": 1, "autoscaling": 2, "none": 3}
+
+
+class FastPLS(dict):
+    """A fitted model: reference field names + the device-resident handle."""
+
+    def __init__(self, ctx: _lib.Context, handle: ctypes.c_void_p):
+        super().__init__()
+        self.ctx = ctx
+        self.handle = handle
+
+    def free(self):
+        if getattr(self, "handle", None) and self.ctx.h:
+            self.ctx.lib.fpls_model_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self) -> _lib.ModelInfo:
+        inf = _lib.ModelInfo()
+        self.ctx.check(self.ctx.lib.fpls_model_info_get(self.handle, ctypes.byref(inf)))
+        return inf
+
+    def read(self, field: str, shape) -> np.ndarray:
+        n = int(np.prod(shape))
+        out = np.empty(n, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.fpls_model_read(self.ctx.h, self.handle, _lib.FIELDS[field],
+                                                    out.ctypes.data, n))
+        return out.reshape(shape, order="F")
+
+    def lda_read(self, slice_idx: int, field: str, shape) -> np.ndarray:
+        n = int(np.prod(shape))
+        out = np.empty(n, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.fpls_lda_read(self.ctx.h, self.handle, slice_idx,
+                                                  _lib.LDA_FIELDS[field], out.ctypes.data, n))
+        return out.reshape(shape, order="F")
+
+
+def _is_factor(y) -> bool:
+    if type(y).__module__.startswith("torch"):
+        import torch
+        return y.dtype in (torch.int32, torch.int64)
+    a = np.asarray(y)
+    return a.dtype.kind in "iubOUS" and (a.ndim == 1 or (a.ndim == 2 and a.shape[1] == 1))
+
+
+def _encode_factor(y):
+    """Levels (sorted unique) and 1-based codes, as R's factor()."""
+    if type(y).__module__.startswith("torch"):
+        import torch
+        codes = y.to(torch.int32)
+        C = int(codes.max().item())
+        return list(range(1, C + 1)), codes, C
+    a = np.asarray(y).ravel()
+    lev, codes = np.unique(a, return_inverse=True)
+    return list(lev), (codes + 1).astype(np.int32), len(lev)
+
+
+def _check_backend(backend: str):
+    if backend != "hip":
+        raise ValueError("this package implements backend='hip' only")
+
+
+def _fetch_fields(model: FastPLS, fit: bool):
+    inf = model.info()
+    p, m, k, L = inf.p, inf.m, inf.kmax, inf.n_ncomp
+    model["p"], model["m"] = int(p), int(m)
+    model["ncomp"] = np.array(inf.ncomp[:L], dtype=np.int64)
+    model["R"] = model.read("R", (p, k))
+    model["Q"] = model.read("Q", (m, k))
+    model["mX"] = model.read("mX", (p,))
+    model["vX"] = model.read("vX", (p,))
+    model["mY"] = model.read("mY", (m,))
+    if inf.b_stored:
+        model["B"] = model.read("B", (p, m, L))
+    model["B_stored"] = bool(inf.b_stored)
+    model["compact_prediction"] = not bool(inf.b_stored)
+    if inf.method == _lib.METHOD_PLSSVD:
+        model["C_latent"] = model.read("C_latent", (k, k, L))
+        model["W_latent"] = model.read("W_latent", (k, m, L))
+        model["s"] = model.read("s", (k,))
+        model["pls_method"] = "plssvd"
+    else:
+        model["pls_method"] = "simpls"
+        model["n_components"] = int(inf.n_components)
+    if fit:
+        model["R2Y"] = model.read("R2Y", (L,))
+        model["Yfit"] = model.read("Yfit", (inf.n, m, L))
+    model["R_predict"] = model.read("R_predict", (p, k))
+    model["R_offset"] = model.read("R_offset", (k,))
+
+
+def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
+        method="simpls", svd_method="rsvd", classifier="argmax", lda_ridge=1e-8,
+        candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
+        fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
+        precision="fp32", label_aware=False, fetch=True, device=0, top=1) -> FastPLS:
+    """pls() (R/main.R:5860-6314) with backend = "hip"."""
+    _check_backend(backend)
+    ctx = _lib.context(device)
+    keep: list = []
+    X = as_mat(Xtrain, keep)
+    classification = _is_factor(Ytrain)
+    opts = _lib.default_opts(scaling=SCALINGS[scaling], seed=int(seed) & 0xFFFFFFFF,
+                             oversample=int(rsvd_oversample), power=int(rsvd_power),
+                             svd_method=_lib.SVD_EXACT if svd_method == "exact" else _lib.SVD_RSVD,
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32,
+                             fit=int(bool(fit)), label_aware=int(bool(label_aware)))
+    nc_arr, nc_ptr = int_array(np.atleast_1d(ncomp))
+    h = ctypes.c_void_p()
+    yv = None
+    Ym = None
+    lev = None
+    C = 0
+    if classification:
+        lev, codes, C = _encode_factor(Ytrain)
+        yv = as_ivec(codes, keep)
+    else:
+        Ym = as_mat(Ytrain, keep)
+    fn = ctx.lib.fpls_plssvd_fit if method == "plssvd" else ctx.lib.fpls_simpls_fit
+    if method not in ("plssvd", "simpls"):
+        raise ValueError("method must be 'plssvd' or 'simpls'")
+    ctx.check(fn(ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
+                 ctypes.byref(yv) if yv is not None else None, C, nc_ptr, nc_arr.size,
+                 ctypes.byref(opts), ctypes.byref(h)))
+    model = FastPLS(ctx, h)
+    model["classification"] = classification
+    model["lev"] = lev
+    model["xprod_mode"] = "label_aware_stream" if (classification and label_aware) else "dense"
+    model["predict_backend"] = "hip"
+    inf = model.info()
+    model["ncomp"] = np.array(inf.ncomp[:inf.n_ncomp], dtype=np.int64)
+    if fetch:
+        _fetch_fields(model, bool(fit))
+    if classification and classifier == "lda":
+        ncomp_eff = sorted(set(int(min(max(c, 1), inf.kmax)) for c in model["ncomp"]))
+        ks, kp = int_array(ncomp_eff)
+        ctx.check(ctx.lib.fpls_lda_train_prefix(ctx.h, h, None, ctypes.byref(yv), C, kp, ks.size,
+                                                float(lda_ridge)))
+        model["classification_rule"] = "lda_hip"
+        model["lda"] = {"ncomp": ncomp_eff, "ridge": float(lda_ridge)}
+    elif classification and classifier == "cknn":
+        ctx.check(ctx.lib.fpls_cknn_fit(ctx.h, h, None, ctypes.byref(yv), C, int(candidate_knn_k),
+                                        float(candidate_tau), float(candidate_alpha),
+                                        int(candidate_top_m)))
+        model["classification_rule"] = "candidate_knn_hip"
+    else:
+        model["classification_rule"] = "argmax"
+    if Xtest is not None:
+        model.update(predict(model, Xtest, Ytest, top=top))
+    return model
+
+
+def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
+    """predict.fastPLS (R/main.R:2577-2762) for a HIP model."""
+    ctx = model.ctx
+    keep: list = []
+    X = as_mat(newdata, keep)
+    w = X.rows
+    inf = model.info()
+    L = inf.n_ncomp
+    lev = model.get("lev")
+    rule = model.get("classification_rule", "argmax")
+    res: Dict = {}
+    if model.get("classification") and rule == "lda_hip":
+        labels = np.empty((w, L), dtype=np.int32, order="F")
+        ctx.check(ctx.lib.fpls_lda_predict(ctx.h, model.handle, ctypes.byref(X),
+                                           labels.ctypes.data, None))
+        res["Ypred_index"] = labels
+        res["Ypred"] = np.asarray(lev, dtype=object)[labels - 1]
+        return res
+    if model.get("classification") and rule == "candidate_knn_hip":
+        top = max(1, min(int(top), len(lev)))
+        ti = np.empty((w, top, L), dtype=np.int32, order="F")
+        ts = np.empty((w, top, L), dtype=np.float64, order="F")
+        ctx.check(ctx.lib.fpls_cknn_predict(ctx.h, model.handle, ctypes.byref(X), top,
+                                            ti.ctypes.data, ts.ctypes.data))
+        res["Ypred_index"] = ti[:, 0, :]
+        res["Ypred"] = np.asarray(lev, dtype=object)[ti[:, 0, :] - 1]
+        if top > 1:
+            res["Ypred_top"] = ti
+            res["Ypred_top_score"] = ts
+        return res
+    m = inf.m
+    yp = np.empty((w, m, L), dtype=np.float64, order="F")
+    ctx.check(ctx.lib.fpls_predict(ctx.h, model.handle, ctypes.byref(X), yp.ctypes.data))
+    if model.get("classification"):
+        idx = (np.argmax(yp, axis=1) + 1).astype(np.int32)  # first maximum
+        res["Ypred_index"] = idx
+        res["Ypred"] = np.asarray(lev, dtype=object)[idx - 1]
+        res["Yscore"] = yp
+    else:
+        res["Ypred"] = yp
+        if Ytest is not None:
+            yt = np.asarray(Ytest, dtype=np.float64).reshape(w, -1)
+            q2 = []
+            for a in range(L):
+                press = float(((yt - yp[:, :, a]) ** 2).sum())
+                tss = float(((yt - yt.mean(axis=0)) ** 2).sum())
+                q2.append(1.0 - press / tss)
+            res["Q2Y"] = np.array(q2)
+    return res
+
+
+def fastsvd(A, ncomp: int, method: str = "rsvd", rsvd_oversample: int = 10,
+            rsvd_power: int = 1, seed: int = 1, omega=None, backend: str = "hip",
+            device: int = 0) -> Dict:
+    """fastsvd (R/main.R:4818-4886): truncated SVD on the device."""
+    _check_backend(backend)
+    ctx = _lib.context(device)
+    keep: list = []
+    Am = as_mat(A, keep)
+    rows, cols = Am.rows, Am.cols
+    k = int(min(max(int(ncomp), 1), min(rows, cols)))
+    opts = _lib.default_opts(oversample=int(rsvd_oversample), power=int(rsvd_power),
+                             seed=int(seed) & 0xFFFFFFFF,
+                             svd_method=_lib.SVD_EXACT if method == "exact" else _lib.SVD_RSVD)
+    U = np.empty(rows * k)
+    s = np.empty(k)
+    Vt = np.empty(k * cols)
+    kout = ctypes.c_int32()
+    om = as_mat(np.asarray(omega, dtype=np.float64), keep) if omega is not None else None
+    ctx.check(ctx.lib.fpls_rsvd(ctx.h, ctypes.byref(Am), k, ctypes.byref(opts),
+                                ctypes.byref(om) if om is not None else None, U.ctypes.data,
+                                s.ctypes.data, Vt.ctypes.data, ctypes.byref(kout)))
+    kk = kout.value
+    U = U[: rows * kk].reshape((rows, kk), order="F")
+    Vt = Vt[: kk * cols].reshape((kk, cols), order="F")
+    return {"d": s[:kk], "u": U, "v": Vt.T.copy()}

@@ -1,0 +1,237 @@
+// Internal runtime of the gfx950 fastPLS library: context, HBM pool,
+// launch/timing wrapper, error plumbing and the kernel entry points shared
+// between translation units.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fastpls_hip.h"
+
+namespace fpls {
+
+// ---------------------------------------------------------------- errors
+struct Error : public std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+#define FPLS_HIP(call)                                                              \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      ::fpls::fail(FPLS_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define FPLS_REQUIRE(cond, code, msg)        \
+  do {                                       \
+    if (!(cond)) ::fpls::fail((code), (msg)); \
+  } while (0)
+
+// ---------------------------------------------------------------- pool
+// HBM pool: blocks are cached by size so that repeated fits allocate nothing
+// after the first call.
+struct Pool {
+  struct Block {
+    void* ptr;
+    size_t bytes;
+    bool used;
+  };
+  std::vector<Block> blocks;
+  void* get(size_t bytes);
+  void put(void* p);
+  void release_all();
+  size_t total_bytes() const;
+};
+
+struct Ctx;
+
+// Device buffer borrowed from the context pool (RAII).
+template <typename T>
+struct DBuf {
+  Ctx* ctx = nullptr;
+  T* ptr = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(Ctx* c, size_t count);
+  ~DBuf();
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : ctx(o.ctx), ptr(o.ptr), n(o.n) { o.ptr = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { reset(); ctx = o.ctx; ptr = o.ptr; n = o.n; o.ptr = nullptr; o.n = 0; }
+    return *this;
+  }
+  void reset();
+  T* get() const { return ptr; }
+};
+
+// ---------------------------------------------------------------- context
+struct KStat {
+  int64_t launches = 0;
+  double total_ms = 0.0;
+};
+
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  Pool pool;
+  bool timing = false;
+  std::map<std::string, KStat> kstats;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  std::vector<hipEvent_t> spare_events;
+  int num_cus = 256;
+  void* alloc(size_t bytes) { return pool.get(bytes); }
+  void free(void* p) { pool.put(p); }
+  hipEvent_t event();
+  void collect_timing();
+};
+
+template <typename T>
+DBuf<T>::DBuf(Ctx* c, size_t count) : ctx(c), n(count) {
+  ptr = count ? static_cast<T*>(c->alloc(count * sizeof(T))) : nullptr;
+}
+template <typename T>
+DBuf<T>::~DBuf() { reset(); }
+template <typename T>
+void DBuf<T>::reset() {
+  if (ptr && ctx) ctx->free(ptr);
+  ptr = nullptr;
+  n = 0;
+}
+
+// Launch wrapper: brackets the launch with HIP events when timing is on.
+template <typename Kernel, typename... Args>
+void launch(Ctx* ctx, const char* name, Kernel k, dim3 grid, dim3 block, size_t shmem,
+            Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ctx->timing) {
+    e0 = ctx->event();
+    e1 = ctx->event();
+    FPLS_HIP(hipEventRecord(e0, ctx->stream));
+  }
+  hipLaunchKernelGGL(k, grid, block, shmem, ctx->stream, args...);
+  FPLS_HIP(hipGetLastError());
+  if (ctx->timing) {
+    FPLS_HIP(hipEventRecord(e1, ctx->stream));
+    ctx->pending.push_back({name, {e0, e1}});
+  }
+}
+
+// ---------------------------------------------------------------- views
+// A device-resident column-major matrix (after staging).
+struct DMat {
+  const void* p = nullptr;
+  int dtype = FPLS_F64;
+  int64_t rows = 0, cols = 0, ld = 0;
+  const float* f() const { return static_cast<const float*>(p); }
+  const double* d() const { return static_cast<const double*>(p); }
+};
+
+// Stage a caller matrix on the device (host buffers are copied into `hold`).
+DMat stage(Ctx* ctx, const fpls_mat* m, DBuf<unsigned char>& hold, const char* what);
+// Stage labels; validates 1..C when check is set.
+const int32_t* stage_labels(Ctx* ctx, const fpls_ivec* y, int64_t n, DBuf<int32_t>& hold,
+                            const char* what);
+
+void to_host(Ctx* ctx, double* dst, const double* src, size_t n);
+void to_host_i32(Ctx* ctx, int32_t* dst, const int32_t* src, size_t n);
+void to_device(Ctx* ctx, double* dst, const double* src, size_t n);
+void sync(Ctx* ctx);
+
+// ---------------------------------------------------------------- kernels (k_blas.hip)
+void fill_f64(Ctx* ctx, double* x, double v, int64_t n);
+void fill_f32(Ctx* ctx, float* x, float v, int64_t n);
+void fill_i32(Ctx* ctx, int32_t* x, int32_t v, int64_t n);
+void copy_f64(Ctx* ctx, double* dst, const double* src, int64_t n);
+// Copy a rows x cols block between column-major buffers with different ld.
+void copy2d_f64(Ctx* ctx, double* dst, int64_t ldd, const double* src, int64_t lds,
+                int64_t rows, int64_t cols);
+// B (cols x rows, ldb) = A^T (A: rows x cols, lda)
+void transpose_f64(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols,
+                   double* B, int64_t ldb);
+void f64_to_f32(Ctx* ctx, float* dst, const double* src, int64_t n);
+void f32_to_f64(Ctx* ctx, double* dst, const float* src, int64_t n);
+
+// General FP64 GEMM: C = alpha op(A) op(B) + beta C, with optional per-stored-
+// column shifts subtracted from A and B on load (centering without writing
+// the centred matrix).  shiftA indexes the stored columns of A.
+void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t K, double alpha,
+              const double* A, int64_t lda, const double* shiftA, const double* B, int64_t ldb,
+              const double* shiftB, double beta, double* C, int64_t ldc,
+              const int32_t* labelsB = nullptr);
+
+// FP32-input contractions on MFMA.
+//  nn: C(M x N) = (A - 1 shiftA^T) B - 1 offset^T     A: M x K (ld lda), B: K x N FP32
+//      out FP32 (Cf) or FP64 (Cd), whichever is non-null.
+//  tn: C(M x N) = (A - 1 shiftA^T)^T (B - 1 shiftB^T)  A: K x M, B: K x N (K long)
+//      FP32 MFMA partials per K-slab, reduced in FP64 into C (FP64).
+void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                 const double* offset, float* Cf, double* Cd, int64_t ldc);
+//      labelsB (1-based codes) replaces B by the one-hot matrix onehot(y) (K x N)
+//      generated on the fly: the dense response is never formed.
+void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                 const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc);
+
+// Matrix-vector products on a (possibly centred) data matrix.
+//  y(M) = (A - 1 shift^T) x      (A: M x K col-major; FP32 or FP64)
+//  y(K) = (A - 1 shift^T)^T x
+void gemv_n(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y);
+void gemv_t(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y);
+
+// ---------------------------------------------------------------- kernels (k_stats.hip)
+// Column sums of X (FP64 accumulation, fixed order).
+void col_sums(Ctx* ctx, const DMat& X, double* sums);
+// Column sums of squared deviations from `center`.
+void col_ssd(Ctx* ctx, const DMat& X, const double* center, double* ssd);
+// Class counts from 1-based labels; *bad counts labels outside 1..C.
+void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* counts,
+                  int32_t* bad);
+
+// ---------------------------------------------------------------- kernels (k_dense.hip)
+// Symmetric eigen-decomposition (one workgroup, cyclic parallel Jacobi, FP64),
+// n <= kJacobiMax.  Eigenvalues descending in w, eigenvectors in V (ld n).
+constexpr int kJacobiMax = 128;
+void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
+// Cholesky-based solves of leading blocks: for each slice s, solve
+// (A[:k_s,:k_s] + diag_shift_s I) X = RHS, with LU fallback.  Used for the
+// PLSSVD latent coefficients and the LDA inverse covariance.
+//   mode 0: A_k X = diag(d)            mode 1: (A_k + lambda I) X = I (LDA ridge rule)
+//   mode 2: (A_k + 1e-10 mean(diag A_k) I) X = diag(d)   (label-aware PLSSVD)
+// d_k: device array of slice orders; status[s]: 0 Cholesky, 2 LU fallback, 1 singular.
+void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const int* d_k,
+                      int n_slices, int mode, const double* dvals, double ridge, double* X,
+                      int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status);
+void scale_cols(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols, const double* f,
+                bool divide);
+// Linv = chol(G + shift I)^{-1} (lower), with shift escalation on breakdown.
+void chol_inv_lower(Ctx* ctx, const double* G, int64_t ldg, int l, int64_t rows, double shift,
+                    double* Linv, int32_t* status);
+void scale_rows_div(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols,
+                    const double* f);
+
+// ---------------------------------------------------------------- RNG (host)
+// std::mt19937(seed) + std::normal_distribution<double>(0,1), column-major.
+void gaussian_host(double* out, int64_t n, uint32_t seed);
+
+}  // namespace fpls
+
+// The opaque ABI structs.
+struct fpls_ctx {
+  fpls::Ctx c;
+};

@@ -1,0 +1,439 @@
+// Small FP64 factorisations kept on the GPU (one workgroup each, matrix in LDS):
+//
+//  * sym_eig          -- cyclic Jacobi with the round-robin parallel ordering
+//                        (Brent & Luk): all n/2 disjoint rotations of a round
+//                        are applied at once, two-sided on A (LDS) and
+//                        one-sided on V (HBM).  Rotation angles follow the
+//                        symmetric Schur decomposition (Golub & Van Loan,
+//                        Algorithm 8.4.1).  Eigenvalues sorted descending.
+//  * spd_solve_slices -- per ncomp slice: Cholesky of the leading block (with
+//                        the reference's diagonal shift rules) and solves
+//                        against a diagonal / identity right-hand side; LU with
+//                        partial pivoting when the block is not SPD.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+
+#include "fpls_internal.h"
+
+namespace fpls {
+
+constexpr int JMAX = kJacobiMax;  // 128
+constexpr int JLD = JMAX + 1;
+
+__global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain, int64_t lda,
+                                                  int n, double* __restrict__ w,
+                                                  double* __restrict__ Vout,
+                                                  double* __restrict__ Vw, int max_sweeps) {
+  __shared__ double A[JMAX * JLD];
+  __shared__ double cs[JMAX / 2], sn[JMAX / 2];
+  __shared__ int pp[JMAX / 2], qq[JMAX / 2];
+  __shared__ double red[16];
+  __shared__ int done;
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int N = n + (n & 1);  // even order; index n (if any) is a zero dummy
+  for (int idx = t; idx < N * N; idx += nt) {
+    const int i = idx % N, j = idx / N;
+    A[i + j * JLD] = (i < n && j < n) ? 0.5 * (Ain[i + j * lda] + Ain[j + i * lda]) : 0.0;
+    Vw[i + (int64_t)j * N] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    // convergence test: off-diagonal mass relative to the total
+    double off = 0.0, tot = 0.0;
+    for (int idx = t; idx < N * N; idx += nt) {
+      const int i = idx % N, j = idx / N;
+      const double v = A[i + j * JLD];
+      tot += v * v;
+      if (i != j) off += v * v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_down(off, o, 64);
+      tot += __shfl_down(tot, o, 64);
+    }
+    if ((t & 63) == 0) {
+      red[t >> 6] = off;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double so = 0.0;
+      for (int k = 0; k < nt / 64; ++k) so += red[k];
+      red[15] = so;
+    }
+    __syncthreads();
+    const double off_all = red[15];
+    __syncthreads();
+    if ((t & 63) == 0) red[t >> 6] = tot;
+    __syncthreads();
+    if (t == 0) {
+      double st = 0.0;
+      for (int k = 0; k < nt / 64; ++k) st += red[k];
+      done = (off_all <= 1e-32 * st || off_all == 0.0) ? 1 : 0;
+    }
+    __syncthreads();
+    if (done) break;
+    for (int round = 0; round < N - 1; ++round) {
+      if (t < N / 2) {
+        auto pos = [&](int k) { return k == 0 ? 0 : ((k - 1 + round) % (N - 1)) + 1; };
+        int p = pos(t), q = pos(N - 1 - t);
+        if (p > q) { const int tmp = p; p = q; q = tmp; }
+        const double apq = A[p + q * JLD];
+        double c = 1.0, s = 0.0;
+        if (apq != 0.0) {
+          const double app = A[p + p * JLD], aqq = A[q + q * JLD];
+          const double zeta = (aqq - app) / (2.0 * apq);
+          const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          c = 1.0 / sqrt(1.0 + tt * tt);
+          s = tt * c;
+        }
+        pp[t] = p;
+        qq[t] = q;
+        cs[t] = c;
+        sn[t] = s;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int idx = t; idx < (N / 2) * N; idx += nt) {
+        const int k = idx % (N / 2), j = idx / (N / 2);
+        const int p = pp[k], q = qq[k];
+        const double c = cs[k], s = sn[k];
+        const double apj = A[p + j * JLD], aqj = A[q + j * JLD];
+        A[p + j * JLD] = c * apj - s * aqj;
+        A[q + j * JLD] = s * apj + c * aqj;
+      }
+      __syncthreads();
+      // columns: A <- A J ; V <- V J
+      for (int idx = t; idx < (N / 2) * N; idx += nt) {
+        const int k = idx / N, i = idx % N;
+        const int p = pp[k], q = qq[k];
+        const double c = cs[k], s = sn[k];
+        const double aip = A[i + p * JLD], aiq = A[i + q * JLD];
+        A[i + p * JLD] = c * aip - s * aiq;
+        A[i + q * JLD] = s * aip + c * aiq;
+        const double vip = Vw[i + (int64_t)p * N], viq = Vw[i + (int64_t)q * N];
+        Vw[i + (int64_t)p * N] = c * vip - s * viq;
+        Vw[i + (int64_t)q * N] = s * vip + c * viq;
+      }
+      __syncthreads();
+    }
+  }
+  // sort descending (stable on index), n <= 128
+  __shared__ int rank[JMAX];
+  if (t < n) {
+    const double wi = A[t + t * JLD];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double wj = A[j + j * JLD];
+      if (wj > wi || (wj == wi && j < t)) ++r;
+    }
+    rank[t] = r;
+    w[r] = wi;
+  }
+  __syncthreads();
+  for (int idx = t; idx < n * n; idx += nt) {
+    const int i = idx % n, j = idx / n;
+    Vout[i + (int64_t)rank[j] * n] = Vw[i + (int64_t)j * N];
+  }
+}
+
+void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
+  FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
+               "sym_eig: order must be in 1..128 for the one-workgroup solver");
+  const int N = n + (n & 1);
+  DBuf<double> Vw(ctx, (size_t)N * N);
+  launch(ctx, "sym_eig_jacobi", k_sym_eig, dim3(1), dim3(1024), 0, A, lda, n, w, V, Vw.get(), 60);
+}
+
+// --------------------------------------------------------------- slice solves
+// mode 0: (A_k) X = diag(d_k)                     -- PLSSVD latent coefficients
+// mode 1: (A_k + lambda I) X = I, lambda from the  -- LDA inverse covariance
+//         reference's ridge rule; lambda_out[s] = lambda
+// mode 2: (A_k + r I) X = diag(d_k), r = 1e-10 mean(diag A_k)  -- label-aware PLSSVD
+__global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
+                                                    const int* __restrict__ ks, int mode,
+                                                    const double* __restrict__ dvals,
+                                                    double ridge, double* __restrict__ X,
+                                                    int64_t ldx, int64_t slice_stride,
+                                                    double* __restrict__ lambda_out,
+                                                    int32_t* __restrict__ status) {
+  __shared__ double L[JMAX * JLD];
+  __shared__ int piv[JMAX];
+  __shared__ double shift;
+  __shared__ int chol_ok;
+  const int s = blockIdx.x;
+  const int k = ks[s];
+  const int t = threadIdx.x, nt = blockDim.x;
+  double* Xs = X + (int64_t)s * slice_stride;
+  if (t == 0) {
+    double sh = 0.0;
+    if (mode == 1) {
+      double tr = 0.0;
+      for (int i = 0; i < k; ++i) tr += Ain[i + (int64_t)i * lda];
+      double scale = tr / (double)(k > 1 ? k : 1);
+      if (!isfinite(scale) || scale <= 0.0) scale = 1.0;
+      double lam = ridge;
+      if (!isfinite(lam) || lam < 0.0) lam = 1e-8;
+      sh = lam * scale;
+      if (lambda_out) lambda_out[s] = sh;
+    } else if (mode == 2) {
+      double tr = 0.0;
+      for (int i = 0; i < k; ++i) tr += Ain[i + (int64_t)i * lda];
+      double r = 1e-10 * (tr / (double)k);
+      if (!isfinite(r) || r <= 0.0) r = 1e-10;
+      sh = r;
+    }
+    shift = sh;
+    chol_ok = 1;
+  }
+  __syncthreads();
+  for (int idx = t; idx < k * k; idx += nt) {
+    const int i = idx % k, j = idx / k;
+    double v = Ain[i + (int64_t)j * lda];
+    if (i == j) v += shift;
+    L[i + j * JLD] = v;
+  }
+  __syncthreads();
+  // right-looking Cholesky, lower triangle
+  for (int j = 0; j < k; ++j) {
+    if (t == 0) {
+      const double d = L[j + j * JLD];
+      if (!(d > 0.0) || !isfinite(d)) chol_ok = 0;
+      else L[j + j * JLD] = sqrt(d);
+    }
+    __syncthreads();
+    if (!chol_ok) break;
+    const double djj = L[j + j * JLD];
+    for (int i = j + 1 + t; i < k; i += nt) L[i + j * JLD] /= djj;
+    __syncthreads();
+    const int m = k - j - 1;
+    for (int idx = t; idx < m * m; idx += nt) {
+      const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
+      if (ii >= jj) L[ii + jj * JLD] -= L[ii + j * JLD] * L[jj + j * JLD];
+    }
+    __syncthreads();
+  }
+  if (chol_ok) {
+    // each thread solves one right-hand-side column c: L L^T x = rhs_c
+    for (int c = t; c < k; c += nt) {
+      double* x = Xs + (int64_t)c * ldx;
+      for (int i = 0; i < k; ++i) x[i] = 0.0;
+      x[c] = (mode == 1) ? 1.0 : dvals[c];
+      for (int i = c; i < k; ++i) {  // forward: rhs is zero above row c
+        double v = x[i];
+        for (int q = c; q < i; ++q) v -= L[i + q * JLD] * x[q];
+        x[i] = v / L[i + i * JLD];
+      }
+      for (int i = k - 1; i >= 0; --i) {  // backward with L^T
+        double v = x[i];
+        for (int q = i + 1; q < k; ++q) v -= L[q + i * JLD] * x[q];
+        x[i] = v / L[i + i * JLD];
+      }
+    }
+    if (t == 0 && status) status[s] = 0;
+    return;
+  }
+  // LU with partial pivoting (Doolittle, in place)
+  __syncthreads();
+  for (int idx = t; idx < k * k; idx += nt) {
+    const int i = idx % k, j = idx / k;
+    double v = Ain[i + (int64_t)j * lda];
+    if (i == j) v += shift;
+    L[i + j * JLD] = v;
+  }
+  __shared__ int singular;
+  if (t == 0) singular = 0;
+  __syncthreads();
+  for (int j = 0; j < k; ++j) {
+    if (t == 0) {
+      int pr = j;
+      double best = fabs(L[j + j * JLD]);
+      for (int i = j + 1; i < k; ++i) {
+        const double v = fabs(L[i + j * JLD]);
+        if (v > best) { best = v; pr = i; }
+      }
+      piv[j] = pr;
+      if (best == 0.0 || !isfinite(best)) singular = 1;
+    }
+    __syncthreads();
+    if (singular) break;
+    const int pr = piv[j];
+    if (pr != j) {
+      for (int c = t; c < k; c += nt) {
+        const double a = L[j + c * JLD];
+        L[j + c * JLD] = L[pr + c * JLD];
+        L[pr + c * JLD] = a;
+      }
+    }
+    __syncthreads();
+    const double d = L[j + j * JLD];
+    for (int i = j + 1 + t; i < k; i += nt) L[i + j * JLD] /= d;
+    __syncthreads();
+    const int m = k - j - 1;
+    for (int idx = t; idx < m * m; idx += nt) {
+      const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
+      L[ii + jj * JLD] -= L[ii + j * JLD] * L[j + jj * JLD];
+    }
+    __syncthreads();
+  }
+  if (singular) {
+    if (t == 0 && status) status[s] = 1;
+    return;
+  }
+  for (int c = t; c < k; c += nt) {
+    double* x = Xs + (int64_t)c * ldx;
+    for (int i = 0; i < k; ++i) x[i] = 0.0;
+    x[c] = (mode == 1) ? 1.0 : dvals[c];
+    for (int j = 0; j < k; ++j) {  // apply row swaps
+      const int pr = piv[j];
+      if (pr != j) { const double a = x[j]; x[j] = x[pr]; x[pr] = a; }
+    }
+    for (int i = 0; i < k; ++i) {
+      double v = x[i];
+      for (int q = 0; q < i; ++q) v -= L[i + q * JLD] * x[q];
+      x[i] = v;
+    }
+    for (int i = k - 1; i >= 0; --i) {
+      double v = x[i];
+      for (int q = i + 1; q < k; ++q) v -= L[i + q * JLD] * x[q];
+      x[i] = v / L[i + i * JLD];
+    }
+  }
+  if (t == 0 && status) status[s] = 2;  // solved through the LU fallback
+}
+
+void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const int* d_k,
+                      int n_slices, int mode, const double* dvals, double ridge, double* X,
+                      int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status) {
+  FPLS_REQUIRE(kmax >= 1 && kmax <= kJacobiMax, FPLS_ERR_ARG,
+               "spd_solve_slices: order must be in 1..128");
+  launch(ctx, "spd_solve", k_spd_solve, dim3(n_slices), dim3(1024), 0, A, lda, d_k, mode, dvals,
+         ridge, X, ldx, slice_stride, lambda_out, status);
+}
+
+// --------------------------------------------------------------- column scaling
+// X[:, j] *= f[j] (or /= f[j] when divide; zero divisors leave the column).
+__global__ void k_scale_cols(double* X, int64_t ld, int64_t rows, int64_t cols,
+                             const double* f, int divide) {
+  const int64_t n = rows * cols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % rows, j = t / rows;
+    const double g = f[j];
+    if (divide) {
+      if (g != 0.0) X[i + j * ld] /= g;
+    } else {
+      X[i + j * ld] *= g;
+    }
+  }
+}
+
+void scale_cols(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols, const double* f,
+                bool divide) {
+  if (rows <= 0 || cols <= 0) return;
+  const int64_t n = rows * cols;
+  launch(ctx, "scale_cols", k_scale_cols, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)),
+         dim3(256), 0, X, ld, rows, cols, f, divide ? 1 : 0);
+}
+
+// X[i, :] /= f[i] (row scaling; zero / non-finite f treated as 1)
+__global__ void k_scale_rows(double* X, int64_t ld, int64_t rows, int64_t cols, const double* f) {
+  const int64_t n = rows * cols;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % rows, j = t / rows;
+    double g = f[i];
+    if (!isfinite(g) || g == 0.0) g = 1.0;
+    X[i + j * ld] /= g;
+  }
+}
+
+void scale_rows_div(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols,
+                    const double* f) {
+  if (rows <= 0 || cols <= 0) return;
+  const int64_t n = rows * cols;
+  launch(ctx, "scale_rows", k_scale_rows, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)),
+         dim3(256), 0, X, ld, rows, cols, f);
+}
+
+}  // namespace fpls
+
+namespace fpls {
+
+// --------------------------------------------------------------- CholeskyQR step
+// Linv = L^{-1} where L L^T = G + shift I (G: l x l SPD Gram).  When the
+// factorisation breaks down the shift is raised (x100, up to 6 times) starting
+// from the shifted-CholeskyQR bound 11 (rows*l + l(l+1)) u tr(G) (Fukaya et al.,
+// SIAM J. Sci. Comput. 2020), so a numerically rank-deficient sample still
+// yields a usable orthogonalisation; status[0] records the retries.
+__global__ __launch_bounds__(1024) void k_chol_inv_lower(const double* __restrict__ G, int64_t ldg,
+                                                         int l, int64_t rows, double shift0,
+                                                         double* __restrict__ Linv,
+                                                         int32_t* __restrict__ status) {
+  __shared__ double L[JMAX * JLD];
+  __shared__ int ok;
+  __shared__ double shift;
+  const int t = threadIdx.x, nt = blockDim.x;
+  double tr = 0.0;
+  for (int i = 0; i < l; ++i) tr += G[i + (int64_t)i * ldg];
+  const double u = DBL_EPSILON * 0.5;
+  double boost = 11.0 * ((double)rows * l + (double)l * (l + 1)) * u * (tr > 0.0 ? tr : 1.0);
+  if (shift0 < 0.0) shift0 = boost;  // shifted first pass of CholeskyQR3
+  int attempt = 0;
+  for (;;) {
+    if (t == 0) {
+      shift = attempt == 0 ? shift0 : boost;
+      ok = 1;
+    }
+    __syncthreads();
+    for (int idx = t; idx < l * l; idx += nt) {
+      const int i = idx % l, j = idx / l;
+      double v = G[i + (int64_t)j * ldg];
+      if (i == j) v += shift;
+      L[i + j * JLD] = v;
+    }
+    __syncthreads();
+    for (int j = 0; j < l; ++j) {
+      if (t == 0) {
+        const double d = L[j + j * JLD];
+        if (!(d > 0.0) || !isfinite(d)) ok = 0;
+        else L[j + j * JLD] = sqrt(d);
+      }
+      __syncthreads();
+      if (!ok) break;
+      const double djj = L[j + j * JLD];
+      for (int i = j + 1 + t; i < l; i += nt) L[i + j * JLD] /= djj;
+      __syncthreads();
+      const int m = l - j - 1;
+      for (int idx = t; idx < m * m; idx += nt) {
+        const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
+        if (ii >= jj) L[ii + jj * JLD] -= L[ii + j * JLD] * L[jj + j * JLD];
+      }
+      __syncthreads();
+    }
+    if (ok || attempt >= 6) break;
+    if (attempt > 0) boost *= 100.0;
+    ++attempt;
+    __syncthreads();
+  }
+  if (t == 0 && status) status[0] = ok ? attempt : -1;
+  // forward substitution, one column of the identity per thread
+  for (int c = t; c < l; c += nt) {
+    double* x = Linv + (int64_t)c * l;
+    for (int i = 0; i < c; ++i) x[i] = 0.0;
+    for (int i = c; i < l; ++i) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int q = c; q < i; ++q) v -= L[i + q * JLD] * x[q];
+      x[i] = v / L[i + i * JLD];
+    }
+  }
+}
+
+void chol_inv_lower(Ctx* ctx, const double* G, int64_t ldg, int l, int64_t rows, double shift,
+                    double* Linv, int32_t* status) {
+  FPLS_REQUIRE(l >= 1 && l <= kJacobiMax, FPLS_ERR_ARG, "chol_inv_lower: order must be 1..128");
+  launch(ctx, "chol_inv", k_chol_inv_lower, dim3(1), dim3(1024), 0, G, ldg, l, rows, shift, Linv,
+         status);
+}
+
+}  // namespace fpls

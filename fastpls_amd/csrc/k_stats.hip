@@ -1,0 +1,109 @@
+// Streaming column statistics and label bookkeeping.
+//
+// Column-major X makes every column a contiguous run of n elements, so one
+// workgroup reduces one column with coalesced loads and a fixed-order FP64
+// tree (deterministic).  HBM-bound: 4 (or 8) bytes per element, one pass.
+#include <algorithm>
+
+#include "fpls_internal.h"
+
+namespace fpls {
+
+template <typename T>
+__device__ inline double block_sum_256(double v, double* red) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0) s = red[0] + red[1] + red[2] + red[3];
+  return s;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_col_sums(int64_t n, int64_t p, const T* __restrict__ X,
+                                                  int64_t ld, const double* __restrict__ center,
+                                                  int squared, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  if (j >= p) return;
+  const T* col = X + j * ld;
+  const double c = center ? center[j] : 0.0;
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += 256) {
+    const double v = static_cast<double>(col[i]) - c;
+    s += squared ? v * v : v;
+  }
+  s = block_sum_256<T>(s, red);
+  if (threadIdx.x == 0) out[j] = s;
+}
+
+void col_sums(Ctx* ctx, const DMat& X, double* sums) {
+  if (X.cols <= 0) return;
+  if (X.dtype == FPLS_F32)
+    launch(ctx, "col_sums", k_col_sums<float>, dim3((unsigned)X.cols), dim3(256), 0, X.rows,
+           X.cols, X.f(), X.ld, (const double*)nullptr, 0, sums);
+  else
+    launch(ctx, "col_sums", k_col_sums<double>, dim3((unsigned)X.cols), dim3(256), 0, X.rows,
+           X.cols, X.d(), X.ld, (const double*)nullptr, 0, sums);
+}
+
+void col_ssd(Ctx* ctx, const DMat& X, const double* center, double* ssd) {
+  if (X.cols <= 0) return;
+  if (X.dtype == FPLS_F32)
+    launch(ctx, "col_ssd", k_col_sums<float>, dim3((unsigned)X.cols), dim3(256), 0, X.rows,
+           X.cols, X.f(), X.ld, center, 1, ssd);
+  else
+    launch(ctx, "col_ssd", k_col_sums<double>, dim3((unsigned)X.cols), dim3(256), 0, X.rows,
+           X.cols, X.d(), X.ld, center, 1, ssd);
+}
+
+// Integer class counts: per-block LDS histogram then integer atomics (exact,
+// order-independent).  bad[0] counts codes outside 1..C.
+__global__ __launch_bounds__(256) void k_label_counts(const int32_t* __restrict__ y, int64_t n,
+                                                      int32_t C,
+                                                      unsigned long long* __restrict__ counts,
+                                                      int32_t* __restrict__ bad) {
+  extern __shared__ unsigned int hist[];
+  for (int c = threadIdx.x; c < C; c += blockDim.x) hist[c] = 0u;
+  __syncthreads();
+  int nbad = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = y[i];
+    if (c >= 1 && c <= C) atomicAdd(&hist[c - 1], 1u);
+    else ++nbad;
+  }
+  if (nbad) atomicAdd(bad, nbad);
+  __syncthreads();
+  for (int c = threadIdx.x; c < C; c += blockDim.x)
+    if (hist[c]) atomicAdd(&counts[c], (unsigned long long)hist[c]);
+}
+
+__global__ void k_label_counts_global(const int32_t* __restrict__ y, int64_t n, int32_t C,
+                                      unsigned long long* __restrict__ counts,
+                                      int32_t* __restrict__ bad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t c = y[i];
+    if (c >= 1 && c <= C) atomicAdd(&counts[c - 1], 1ull);
+    else atomicAdd(bad, 1);
+  }
+}
+
+void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* counts,
+                  int32_t* bad) {
+  fill_f64(ctx, reinterpret_cast<double*>(counts), 0.0, C);  // all-zero bit pattern
+  fill_i32(ctx, bad, 0, 1);
+  if (n <= 0) return;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  if (C <= 16384)
+    launch(ctx, "label_counts", k_label_counts, dim3(blocks), dim3(256), C * sizeof(unsigned),
+           y, n, C, reinterpret_cast<unsigned long long*>(counts), bad);
+  else
+    launch(ctx, "label_counts", k_label_counts_global, dim3(blocks), dim3(256), 0, y, n, C,
+           reinterpret_cast<unsigned long long*>(counts), bad);
+}
+
+}  // namespace fpls

@@ -1,0 +1,411 @@
+// LDA classifier head on the latent scores:
+//   training   lda_train_prefix_cpp      src/fastPLS.cpp:1817-1927
+//              lda_project_train_prefix_cpp                :1930-1957
+//   prediction lda_project_predict_labels_cpp              :2100-2147
+//              lda_labels_from_scores (strict '>', lowest class wins ties) :1718-1734
+//
+// Class sums of T come from the MFMA cross-product with the one-hot operand
+// generated from int32 labels; T^T T is the PLSSVD Gram already resident on
+// the model when the training rows are the fit's own rows.  Inverses,
+// linear terms and constants are FP64 on the device.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <set>
+#include <vector>
+
+#include "model.h"
+
+namespace fpls {
+
+void fit_scores_and_gram(Ctx* ctx, Model* M, const DMat& X);
+
+static unsigned g1(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+
+// means (C x k) = sums^T / counts, sums: k x C (ld k)
+__global__ void k_means_from_sums(const double* sums, int64_t k, int64_t C, const double* counts,
+                                  double* means) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < k * C;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = t % C, j = t / C;
+    means[c + j * C] = sums[j + c * k] / counts[c];
+  }
+}
+
+// pooled (k x k) = symmatu(G) (upper triangle mirrored), ld k
+__global__ void k_symmatu(const double* G, int64_t ldg, int64_t k, double* P) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < k * k;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % k, j = t / k;
+    P[i + j * k] = (i <= j) ? G[i + j * ldg] : G[j + i * ldg];
+  }
+}
+
+__global__ void k_scale_rows_by(double* X, int64_t ld, int64_t rows, int64_t cols, const double* f) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < rows * cols;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % rows, j = t / rows;
+    X[i + j * ld] *= f[i];
+  }
+}
+
+__global__ void k_scale_all(double* X, int64_t n, double f) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    X[t] *= f;
+}
+
+// constants[c] = -1/2 means_c . linear_c + log(max(count_c / n, DBL_MIN))
+__global__ void k_lda_constants(const double* means, int64_t ldm, const double* linear,
+                                int64_t ldl, int64_t C, int64_t k, const double* counts, double n,
+                                double* constants) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  double q = 0.0;
+  for (int64_t j = 0; j < k; ++j) q += means[c + j * ldm] * linear[c + j * ldl];
+  const double prior = fmax(counts[c] / n, DBL_MIN);
+  constants[c] = -0.5 * q + log(prior);
+}
+
+// labels[i] = 1 + argmax_c (scores[i,c] + cst[c]); first maximum wins.
+__global__ void k_argmax_rows(const double* __restrict__ scores, int64_t lds, int64_t n,
+                              int64_t C, const double* __restrict__ cst,
+                              int32_t* __restrict__ labels) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int best = 0;
+  double bv = scores[i] + cst[0];
+  for (int64_t c = 1; c < C; ++c) {
+    const double v = scores[i + c * lds] + cst[c];
+    if (v > bv) {
+      bv = v;
+      best = (int)c;
+    }
+  }
+  labels[i] = best + 1;
+}
+
+__global__ void k_add_consts(double* S, int64_t lds, int64_t n, int64_t C, const double* cst) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * C;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, c = t / n;
+    S[i + c * lds] += cst[c];
+  }
+}
+
+// Train prefix-LDA models for every k in ks on scores T (n x >= max ks).
+// G (optional): T^T T already computed (ld ldg).
+void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
+                    int32_t C, const std::vector<int>& ks, double ridge, int precision,
+                    DBuf<double>& counts_out, std::vector<LdaSlice>& slices) {
+  const int64_t n = T.rows;
+  FPLS_REQUIRE(n > 0 && T.cols > 0, FPLS_ERR_ARG, "LDA requires a non-empty score matrix");
+  FPLS_REQUIRE(C >= 2, FPLS_ERR_ARG, "LDA requires at least two classes");
+  const int kmax = *std::max_element(ks.begin(), ks.end());
+  FPLS_REQUIRE(kmax >= 1 && kmax <= T.cols, FPLS_ERR_ARG,
+               "LDA component counts must be in 1..ncol(Ttrain)");
+  FPLS_REQUIRE(kmax <= kJacobiMax, FPLS_ERR_ARG, "LDA on the device supports up to 128 components");
+  // counts + label validation (src/fastPLS.cpp:1853-1867)
+  DBuf<int64_t> cnt(ctx, C);
+  DBuf<int32_t> bad(ctx, 1);
+  label_counts(ctx, y, n, C, cnt.get(), bad.get());
+  int32_t hbad = 0;
+  to_host_i32(ctx, &hbad, bad.get(), 1);
+  FPLS_REQUIRE(hbad == 0, FPLS_ERR_LABEL_RANGE, "LDA labels must be encoded as 1..n_classes");
+  std::vector<int64_t> hc(C);
+  FPLS_HIP(hipMemcpyAsync(hc.data(), cnt.get(), C * sizeof(int64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+  sync(ctx);
+  std::vector<double> hcd(C);
+  for (int c = 0; c < C; ++c) {
+    FPLS_REQUIRE(hc[c] > 0, FPLS_ERR_EMPTY_CLASS, "LDA received an empty class");
+    hcd[c] = (double)hc[c];
+  }
+  counts_out = DBuf<double>(ctx, C);
+  to_device(ctx, counts_out.get(), hcd.data(), C);
+  // class sums of the leading kmax score columns: k x C
+  DBuf<double> sums(ctx, (size_t)kmax * C), means(ctx, (size_t)C * kmax);
+  if (T.dtype == FPLS_F32)
+    gemm_tn_f32(ctx, precision, kmax, C, n, T.f(), T.ld, nullptr, nullptr, 0, y, nullptr,
+                sums.get(), kmax);
+  else
+    gemm_f64(ctx, true, false, kmax, C, n, 1.0, T.d(), T.ld, nullptr, nullptr, n, nullptr, 0.0,
+             sums.get(), kmax, y);
+  launch(ctx, "lda_means", k_means_from_sums, dim3(g1(kmax * C)), dim3(256), 0, sums.get(),
+         (int64_t)kmax, (int64_t)C, counts_out.get(), means.get());
+  // pooled = (symmatu(T^T T) - sum_c n_c mu_c mu_c^T) / max(1, n - C)
+  DBuf<double> Gl, pooled(ctx, (size_t)kmax * kmax), msc(ctx, (size_t)C * kmax);
+  if (!G) {
+    Gl = DBuf<double>(ctx, (size_t)kmax * kmax);
+    if (T.dtype == FPLS_F32)
+      gemm_tn_f32(ctx, precision, kmax, kmax, n, T.f(), T.ld, nullptr, T.f(), T.ld, nullptr,
+                  nullptr, Gl.get(), kmax);
+    else
+      gemm_f64(ctx, true, false, kmax, kmax, n, 1.0, T.d(), T.ld, nullptr, T.d(), T.ld, nullptr,
+               0.0, Gl.get(), kmax);
+    G = Gl.get();
+    ldg = kmax;
+  }
+  launch(ctx, "symmatu", k_symmatu, dim3(g1(kmax * kmax)), dim3(256), 0, G, ldg, (int64_t)kmax,
+         pooled.get());
+  copy_f64(ctx, msc.get(), means.get(), (int64_t)C * kmax);
+  launch(ctx, "scale_rows", k_scale_rows_by, dim3(g1(C * kmax)), dim3(256), 0, msc.get(),
+         (int64_t)C, (int64_t)C, (int64_t)kmax, counts_out.get());
+  gemm_f64(ctx, true, false, kmax, kmax, C, -1.0, means.get(), C, nullptr, msc.get(), C, nullptr,
+           1.0, pooled.get(), kmax);
+  const double df = std::max(1.0, (double)n - (double)C);
+  launch(ctx, "scale", k_scale_all, dim3(g1(kmax * kmax)), dim3(256), 0, pooled.get(),
+         (int64_t)kmax * kmax, 1.0 / df);
+  // per k: ridge, inverse, linear, constants
+  slices.clear();
+  DBuf<int> dk(ctx, 1);
+  DBuf<double> lam(ctx, 1);
+  DBuf<int32_t> st(ctx, 1);
+  for (int k : ks) {
+    LdaSlice sl;
+    sl.k = k;
+    sl.means = DBuf<double>(ctx, (size_t)C * k);
+    copy2d_f64(ctx, sl.means.get(), C, means.get(), C, C, k);
+    sl.inv_cov = DBuf<double>(ctx, (size_t)k * k);
+    FPLS_HIP(hipMemcpyAsync(dk.get(), &k, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    spd_solve_slices(ctx, pooled.get(), kmax, k, dk.get(), 1, 1, nullptr, ridge, sl.inv_cov.get(),
+                     k, (int64_t)k * k, lam.get(), st.get());
+    sl.linear = DBuf<double>(ctx, (size_t)C * k);
+    gemm_f64(ctx, false, false, C, k, k, 1.0, sl.means.get(), C, nullptr, sl.inv_cov.get(), k,
+             nullptr, 0.0, sl.linear.get(), C);
+    sl.constants = DBuf<double>(ctx, C);
+    launch(ctx, "lda_constants", k_lda_constants, dim3((C + 255) / 256), dim3(256), 0,
+           sl.means.get(), (int64_t)C, sl.linear.get(), (int64_t)C, (int64_t)C, (int64_t)k,
+           counts_out.get(), (double)n, sl.constants.get());
+    to_host(ctx, &sl.ridge, lam.get(), 1);
+    int32_t hs = 0;
+    to_host_i32(ctx, &hs, st.get(), 1);
+    FPLS_REQUIRE(hs != 1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular");
+    sl.solve_status = hs;
+    slices.push_back(std::move(sl));
+  }
+}
+
+static std::vector<int> unique_sorted(const int32_t* ncomp, int32_t L, int kcap) {
+  FPLS_REQUIRE(ncomp && L >= 1, FPLS_ERR_ARG, "LDA requires at least one component count");
+  std::set<int> s;
+  for (int i = 0; i < L; ++i) s.insert(std::min(std::max<int>(ncomp[i], 1), kcap));
+  return std::vector<int>(s.begin(), s.end());
+}
+
+// Scores (w x C, FP64) for one LDA slice of kk components; T_pre holds the
+// projected test rows (w x >= kk) when the latent form is used.
+void lda_scores(Ctx* ctx, const Model* M, const LdaSlice& sl, const DMat& X, const DMat& Tpre,
+                double* scores) {
+  const int64_t w = X.rows, p = M->p, C = M->n_classes;
+  const int kk = sl.k;
+  const double latent_ops = (double)w * kk * ((double)p + (double)C);
+  const double direct_ops = (double)w * p * (double)C;
+  if (direct_ops < 0.5 * latent_ops) {
+    // direct form: (X - 1 mX^T) (R_predict lin^T)  (src/fastPLS.cpp:2128-2136)
+    DBuf<double> W(ctx, (size_t)p * C);
+    gemm_f64(ctx, false, true, p, C, kk, 1.0, M->R_predict.get(), p, nullptr, sl.linear.get(), C,
+             nullptr, 0.0, W.get(), p);
+    if (X.dtype == FPLS_F32) {
+      DBuf<float> Wf(ctx, (size_t)p * C);
+      f64_to_f32(ctx, Wf.get(), W.get(), p * C);
+      gemm_nn_f32(ctx, M->precision, w, C, p, X.f(), X.ld, M->mX.get(), Wf.get(), p, nullptr,
+                  nullptr, scores, w);
+    } else {
+      gemm_f64(ctx, false, false, w, C, p, 1.0, X.d(), X.ld, M->mX.get(), W.get(), p, nullptr,
+               0.0, scores, w);
+    }
+  } else {
+    // latent form: T lin^T with T = Xs R_predict (src/fastPLS.cpp:2139-2146)
+    if (Tpre.dtype == FPLS_F32) {
+      DBuf<double> linT(ctx, (size_t)kk * C);
+      transpose_f64(ctx, sl.linear.get(), C, C, kk, linT.get(), kk);
+      DBuf<float> linTf(ctx, (size_t)kk * C);
+      f64_to_f32(ctx, linTf.get(), linT.get(), (int64_t)kk * C);
+      gemm_nn_f32(ctx, M->precision, w, C, kk, Tpre.f(), Tpre.ld, nullptr, linTf.get(), kk,
+                  nullptr, nullptr, scores, w);
+    } else {
+      gemm_f64(ctx, false, true, w, C, kk, 1.0, Tpre.d(), Tpre.ld, nullptr, sl.linear.get(), C,
+               nullptr, 0.0, scores, w);
+    }
+  }
+}
+
+}  // namespace fpls
+
+using namespace fpls;
+
+extern "C" int fpls_lda_train_prefix(fpls_ctx* ctx, fpls_model* model, const fpls_mat* Xm,
+                                     const fpls_ivec* yv, int32_t n_classes,
+                                     const int32_t* ncomp, int32_t n_ncomp, double ridge) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    Model* M = &model->m;
+    std::vector<int> ks = unique_sorted(ncomp, n_ncomp, M->kmax);
+    DBuf<int32_t> hl;
+    const int32_t* y = stage_labels(c, yv, M->n, hl, "lda: y");
+    DBuf<unsigned char> hx;
+    DMat T;
+    const double* G = nullptr;
+    int64_t ldg = 0;
+    DBuf<float> Tf;
+    DBuf<double> Td;
+    if (Xm == nullptr) {
+      FPLS_REQUIRE(M->has_T(), FPLS_ERR_STATE, "model holds no training scores; pass Xtrain");
+      T = M->T();
+      if (M->G.get()) {
+        G = M->G.get();
+        ldg = M->kmax;
+      }
+    } else {
+      DMat X = stage(c, Xm, hx, "lda: Xtrain");
+      FPLS_REQUIRE(X.rows == M->n, FPLS_ERR_ARG, "LDA requires one class label per training row");
+      const int kmax = ks.back();
+      T.rows = X.rows;
+      T.cols = kmax;
+      T.ld = X.rows;
+      if (X.dtype == FPLS_F32) {
+        Tf = DBuf<float>(c, (size_t)X.rows * kmax);
+        project_scores(c, M, X, kmax, Tf.get(), nullptr, X.rows);
+        T.p = Tf.get();
+        T.dtype = FPLS_F32;
+      } else {
+        Td = DBuf<double>(c, (size_t)X.rows * kmax);
+        project_scores(c, M, X, kmax, nullptr, Td.get(), X.rows);
+        T.p = Td.get();
+        T.dtype = FPLS_F64;
+      }
+    }
+    M->n_classes = n_classes;
+    lda_train_core(c, T, G, ldg, y, n_classes, ks, ridge, M->precision, M->counts, M->lda);
+  });
+}
+
+extern "C" int fpls_lda_train_scores(fpls_ctx* ctx, const fpls_mat* Tm, const fpls_ivec* yv,
+                                     int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                                     double ridge, double* means, double* inv_cov,
+                                     double* linear, double* constants, double* priors,
+                                     double* ridge_out) {
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    DBuf<unsigned char> ht;
+    DMat T = stage(c, Tm, ht, "lda_train_scores: T");
+    FPLS_REQUIRE(ncomp && n_ncomp >= 1, FPLS_ERR_ARG, "LDA requires at least one component count");
+    std::vector<int> ks(ncomp, ncomp + n_ncomp);
+    for (int k : ks)
+      FPLS_REQUIRE(k >= 1 && k <= T.cols, FPLS_ERR_ARG,
+                   "LDA component counts must be in 1..ncol(Ttrain)");
+    DBuf<int32_t> hl;
+    const int32_t* y = stage_labels(c, yv, T.rows, hl, "lda_train_scores: y");
+    DBuf<double> counts;
+    std::vector<LdaSlice> sl;
+    lda_train_core(c, T, nullptr, 0, y, n_classes, ks, ridge, FPLS_PREC_FP32, counts, sl);
+    size_t om = 0, oi = 0, oc = 0;
+    for (size_t s = 0; s < sl.size(); ++s) {
+      const int k = sl[s].k;
+      if (means) to_host(c, means + om, sl[s].means.get(), (size_t)n_classes * k);
+      if (linear) to_host(c, linear + om, sl[s].linear.get(), (size_t)n_classes * k);
+      if (inv_cov) to_host(c, inv_cov + oi, sl[s].inv_cov.get(), (size_t)k * k);
+      if (constants) to_host(c, constants + oc, sl[s].constants.get(), n_classes);
+      if (ridge_out) ridge_out[s] = sl[s].ridge;
+      om += (size_t)n_classes * k;
+      oi += (size_t)k * k;
+      oc += n_classes;
+    }
+    if (priors) {
+      std::vector<double> h(n_classes);
+      to_host(c, h.data(), counts.get(), n_classes);
+      for (int i = 0; i < n_classes; ++i) priors[i] = h[i] / (double)T.rows;
+    }
+  });
+}
+
+extern "C" int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t slice, int32_t field,
+                             double* host_out, int64_t n_elem) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model& M = model->m;
+    FPLS_REQUIRE(slice >= 0 && slice < (int)M.lda.size(), FPLS_ERR_STATE, "no such LDA slice");
+    const LdaSlice& s = M.lda[slice];
+    const int64_t C = M.n_classes, k = s.k;
+    const double* src = nullptr;
+    int64_t need = 0;
+    std::vector<double> pri;
+    switch (field) {
+      case FPLS_LDA_MEANS: src = s.means.get(); need = C * k; break;
+      case FPLS_LDA_INV_COV: src = s.inv_cov.get(); need = k * k; break;
+      case FPLS_LDA_LINEAR: src = s.linear.get(); need = C * k; break;
+      case FPLS_LDA_CONSTANTS: src = s.constants.get(); need = C; break;
+      case FPLS_LDA_PRIORS: {
+        FPLS_REQUIRE(host_out && n_elem >= C, FPLS_ERR_ARG, "output buffer too small");
+        to_host(c, host_out, M.counts.get(), C);
+        for (int64_t i = 0; i < C; ++i) host_out[i] /= (double)M.n;
+        return;
+      }
+      case FPLS_LDA_RIDGE:
+        FPLS_REQUIRE(host_out && n_elem >= 1, FPLS_ERR_ARG, "output buffer too small");
+        host_out[0] = s.ridge;
+        return;
+      default: fail(FPLS_ERR_ARG, "unknown LDA field");
+    }
+    FPLS_REQUIRE(host_out && n_elem >= need, FPLS_ERR_ARG, "output buffer too small");
+    to_host(c, host_out, src, need);
+  });
+}
+
+extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                                int32_t* labels, double* scores_out) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model* M = &model->m;
+    FPLS_REQUIRE(!M->lda.empty(), FPLS_ERR_STATE,
+                 "this model does not contain fitted LDA classifier parameters");
+    DBuf<unsigned char> hx;
+    DMat X = stage(c, Xm, hx, "lda_predict: X");
+    FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "lda_predict: X has the wrong number of columns");
+    const int64_t w = X.rows, C = M->n_classes;
+    const int L = (int)M->ncomp.size();
+    const int lda_kmax = M->lda.back().k;
+    // projected test scores for the largest slice; prefixes serve the rest
+    DMat T;
+    T.rows = w;
+    T.cols = lda_kmax;
+    T.ld = w;
+    DBuf<float> Tf;
+    DBuf<double> Td;
+    if (X.dtype == FPLS_F32) {
+      Tf = DBuf<float>(c, (size_t)w * lda_kmax);
+      project_scores(c, M, X, lda_kmax, Tf.get(), nullptr, w);
+      T.p = Tf.get();
+      T.dtype = FPLS_F32;
+    } else {
+      Td = DBuf<double>(c, (size_t)w * lda_kmax);
+      project_scores(c, M, X, lda_kmax, nullptr, Td.get(), w);
+      T.p = Td.get();
+      T.dtype = FPLS_F64;
+    }
+    DBuf<double> sc(c, (size_t)w * C);
+    DBuf<int32_t> lab(c, (size_t)w * L);
+    for (int a = 0; a < L; ++a) {
+      const int kk = std::min(std::max(M->ncomp[a], 1), lda_kmax);
+      const LdaSlice* sl = nullptr;
+      for (const auto& s : M->lda)
+        if (s.k == kk) sl = &s;
+      FPLS_REQUIRE(sl != nullptr, FPLS_ERR_STATE, "no fitted LDA classifier for this ncomp");
+      lda_scores(c, M, *sl, X, T, sc.get());
+      launch(c, "lda_argmax", k_argmax_rows, dim3((unsigned)((w + 255) / 256)), dim3(256), 0,
+             sc.get(), w, w, C, sl->constants.get(), lab.get() + (size_t)a * w);
+      if (scores_out) {
+        launch(c, "lda_consts", k_add_consts, dim3(g1(w * C)), dim3(256), 0, sc.get(), w, w, C,
+               sl->constants.get());
+        to_host(c, scores_out + (size_t)a * w * C, sc.get(), (size_t)w * C);
+      }
+    }
+    to_host_i32(c, labels, lab.get(), (size_t)w * L);
+  });
+}

@@ -1,0 +1,116 @@
+// Device-resident fitted model shared by the fit, classifier and prediction
+// entry points.  Field names follow the reference's model list
+// (src/fastPLS.cpp:2759-2777, 4966-4980; R/main.R:1019-1041, 1236-1250, 858-871).
+#pragma once
+
+#include "fpls_internal.h"
+
+namespace fpls {
+
+struct LdaSlice {
+  int k = 0;                // ncomp of this LDA model
+  DBuf<double> means;       // C x k
+  DBuf<double> inv_cov;     // k x k
+  DBuf<double> linear;      // C x k
+  DBuf<double> constants;   // C
+  double ridge = 0.0;
+  int solve_status = 0;     // 0 Cholesky, 2 LU fallback
+};
+
+struct CknnSlice {
+  int k = 0;
+  DBuf<double> centroids;   // C x k (row-normalised)
+  DBuf<double> cmap;        // k x k score-space map (C_latent slice), or empty
+};
+
+struct Model {
+  Ctx* ctx = nullptr;
+  int method = FPLS_METHOD_PLSSVD;
+  int64_t n = 0, p = 0, m = 0;
+  int kmax = 0;                 // columns of R / Q
+  int n_components = 0;         // SIMPLS: components actually extracted
+  int in_dtype = FPLS_F64;      // dtype of the training X
+  int precision = FPLS_PREC_FP32;
+  int scaling = 1;
+  std::vector<int> ncomp;       // effective ncomp per slice
+  bool b_stored = false;
+  bool label_aware = false;
+
+  DBuf<double> R, Q, V, s, mX, vX, mY;
+  DBuf<double> C_latent, W_latent, B;
+  DBuf<double> R2Y, Yfit;
+  // projection cache (R/main.R:1019-1041): R_predict = R / vX, offset = mX R_predict
+  DBuf<double> R_predict, R_offset;
+  DBuf<float> R_predict_f32;    // for the MFMA projection of FP32 inputs
+  // training scores T = Xc R (n x kmax): FP32 for FP32 inputs, FP64 otherwise
+  DBuf<float> T_f32;
+  DBuf<double> T_f64;
+  DBuf<double> G;               // T^T T (kmax x kmax), reused by LDA
+
+  // LDA head
+  int n_classes = 0;
+  DBuf<double> counts;          // C (as double, like the reference)
+  std::vector<LdaSlice> lda;
+  // candidate-kNN head
+  std::vector<CknnSlice> cknn;
+  DBuf<int32_t> y_codes;        // n training labels (device)
+  DBuf<int32_t> class_offsets;  // C + 1 CSR offsets over rows sorted by class
+  DBuf<int32_t> class_rows;     // n row ids grouped by class (ascending inside a class)
+  std::vector<int64_t> class_offsets_host;
+  int knn_k = 10, top_m = 20;
+  double tau = 0.2, alpha = 0.75;
+
+  bool has_T() const { return T_f32.get() != nullptr || T_f64.get() != nullptr; }
+  DMat T() const {
+    DMat d;
+    d.rows = n;
+    d.cols = kmax;
+    d.ld = n;
+    if (T_f32.get()) { d.p = T_f32.get(); d.dtype = FPLS_F32; }
+    else { d.p = T_f64.get(); d.dtype = FPLS_F64; }
+    return d;
+  }
+};
+
+// R_predict / R_offset / FP32 copy from R, vX, mX (k columns).
+void build_projection_cache(Ctx* ctx, Model* M);
+// T = (X - 1 mX^T) R_predict[:, :k]  (FP32 MFMA for FP32 X, FP64 otherwise),
+// written into an n x k buffer of the input's dtype (Tf or Td).
+void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, double* Td,
+                    int64_t ldt);
+
+// Exact / randomized truncated SVD of a device FP64 matrix (rsvd.hip).
+struct SvdOut {
+  DBuf<double> U, s, Vt;        // U rows x k, s k, Vt k x cols
+  int k = 0;
+};
+void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
+                   SvdOut& out, bool left_only);
+void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
+              int oversample, int power, uint32_t seed, const double* omega_host,
+              bool left_only, bool use_full, SvdOut& out);
+// Orthonormal basis of the columns of Y (rows x l) by shifted CholeskyQR3, in place.
+void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l);
+
+}  // namespace fpls
+
+struct fpls_model {
+  fpls::Model m;
+};
+
+// Runs `body` and converts library errors into status codes + last error.
+template <typename F>
+int fpls_guard(fpls_ctx* ctx, F&& body) {
+  if (!ctx) return FPLS_ERR_ARG;
+  try {
+    FPLS_HIP(hipSetDevice(ctx->c.device));
+    body();
+    return FPLS_OK;
+  } catch (const fpls::Error& e) {
+    ctx->c.err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    ctx->c.err = e.what();
+    return FPLS_ERR_ARG;
+  }
+}

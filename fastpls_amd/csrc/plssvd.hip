@@ -1,0 +1,538 @@
+// PLSSVD fit (pls_model1, src/fastPLS.cpp:4780-4986, and the label-aware
+// stream model, R/main.R:513-655), model access, latent projection and
+// prediction (pls_predict, src/fastPLS.cpp:3211-3445).
+//
+// Data flow for an FP32 X (n x p) resident in HBM:
+//   col_sums          -> mX                               (1 pass over X)
+//   gemm_tn (labels)  -> S = Xc^T (onehot(y) - 1 mY^T)    (1 pass over X, MFMA,
+//                        one-hot generated from int32 labels, never stored)
+//   Gram + Jacobi     -> U, s, V of S                     (FP64, one workgroup)
+//   gemm_nn           -> T = Xc U                         (1 pass over X, MFMA)
+//   gemm_tn           -> G = T^T T                        (FP64 reduce)
+//   slice solves      -> C_k = G_k^{-1} diag(s_k), W_k = C_k V_k^T, B_k = U_k W_k
+// Centring is applied on load (X is never rewritten); T stays resident for the
+// classifier heads.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "model.h"
+
+namespace fpls {
+
+__global__ void k_div_scalar(double* x, int64_t n, double d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] /= d;
+}
+
+// sd from sums of squared deviations: sqrt(ssd / (n-1)); guard != 0 maps
+// zero / non-finite to 1 (label-aware rule, R/main.R:553-557).
+__global__ void k_sd_from_ssd(double* v, int64_t p, int64_t n, int guard) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < p;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    double s = sqrt(v[j] / (double)(n > 1 ? n - 1 : 1));
+    if (guard && (!isfinite(s) || s == 0.0)) s = 1.0;
+    v[j] = s;
+  }
+}
+
+__global__ void k_counts_to_prop(const unsigned long long* counts, double* mY, double* cnt_d,
+                                 int C, int64_t n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) {
+    const double v = (double)counts[c];
+    mY[c] = v / (double)n;
+    if (cnt_d) cnt_d[c] = v;
+  }
+}
+
+static unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
+
+void build_projection_cache(Ctx* ctx, Model* M) {
+  const int64_t p = M->p;
+  const int k = M->kmax;
+  M->R_predict = DBuf<double>(ctx, (size_t)p * k);
+  M->R_offset = DBuf<double>(ctx, k);
+  copy_f64(ctx, M->R_predict.get(), M->R.get(), p * k);
+  scale_rows_div(ctx, M->R_predict.get(), p, p, k, M->vX.get());
+  // offset = mX R_predict  (k)
+  gemm_f64(ctx, true, false, k, 1, p, 1.0, M->R_predict.get(), p, nullptr, M->mX.get(), p,
+           nullptr, 0.0, M->R_offset.get(), k);
+  M->R_predict_f32 = DBuf<float>(ctx, (size_t)p * k);
+  f64_to_f32(ctx, M->R_predict_f32.get(), M->R_predict.get(), p * k);
+}
+
+void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, double* Td,
+                    int64_t ldt) {
+  FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "projection: X has the wrong number of columns");
+  FPLS_REQUIRE(k >= 1 && k <= M->kmax, FPLS_ERR_ARG, "projection: k out of range");
+  // (X - 1 mX^T) R_predict == X R_predict - 1 offset^T (R/main.R:1003-1016);
+  // centring on load keeps FP32 accuracy for uncentred data.
+  if (X.dtype == FPLS_F32) {
+    gemm_nn_f32(ctx, M->precision, X.rows, k, X.cols, X.f(), X.ld, M->mX.get(),
+                M->R_predict_f32.get(), M->p, nullptr, Tf, Td, ldt);
+  } else {
+    FPLS_REQUIRE(Td != nullptr, FPLS_ERR_ARG, "projection: FP64 input needs an FP64 output");
+    gemm_f64(ctx, false, false, X.rows, k, X.cols, 1.0, X.d(), X.ld, M->mX.get(),
+             M->R_predict.get(), M->p, nullptr, 0.0, Td, ldt);
+  }
+}
+
+// mX, vX, mY (+ counts) for a fit; shared by PLSSVD and SIMPLS.
+void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int32_t* y,
+                    int32_t C, int scaling, bool guard_sd) {
+  const int64_t n = X.rows, p = X.cols;
+  M->mX = DBuf<double>(ctx, p);
+  M->vX = DBuf<double>(ctx, p);
+  if (scaling < 3) {
+    col_sums(ctx, X, M->mX.get());
+    launch(ctx, "div", k_div_scalar, dim3(g1(p)), dim3(256), 0, M->mX.get(), p, (double)n);
+  } else {
+    fill_f64(ctx, M->mX.get(), 0.0, p);
+  }
+  if (scaling == 2) {
+    col_ssd(ctx, X, M->mX.get(), M->vX.get());
+    launch(ctx, "sd", k_sd_from_ssd, dim3(g1(p)), dim3(256), 0, M->vX.get(), p, n,
+           guard_sd ? 1 : 0);
+  } else {
+    fill_f64(ctx, M->vX.get(), 1.0, p);
+  }
+  if (y) {
+    M->mY = DBuf<double>(ctx, C);
+    M->counts = DBuf<double>(ctx, C);
+    DBuf<int64_t> cnt(ctx, C);
+    DBuf<int32_t> bad(ctx, 1);
+    label_counts(ctx, y, n, C, cnt.get(), bad.get());
+    int32_t hbad = 0;
+    to_host_i32(ctx, &hbad, bad.get(), 1);
+    FPLS_REQUIRE(hbad == 0, FPLS_ERR_LABEL_RANGE, "labels must be encoded as 1..n_classes");
+    launch(ctx, "counts", k_counts_to_prop, dim3((C + 255) / 256), dim3(256), 0,
+           reinterpret_cast<const unsigned long long*>(cnt.get()), M->mY.get(), M->counts.get(),
+           C, n);
+  } else {
+    M->mY = DBuf<double>(ctx, Y->cols);
+    col_sums(ctx, *Y, M->mY.get());
+    launch(ctx, "div", k_div_scalar, dim3(g1(Y->cols)), dim3(256), 0, M->mY.get(), Y->cols,
+           (double)n);
+  }
+}
+
+// S = ((X - mX)/vX)^T (Y - mY)  (p x m, FP64); Y dense or labels.
+void cross_cov(Ctx* ctx, const Model* M, const DMat& X, const DMat* Y, const int32_t* y, int64_t m,
+               double* S, int precision) {
+  const int64_t n = X.rows, p = X.cols;
+  if (X.dtype == FPLS_F32) {
+    const float* Bp = nullptr;
+    int64_t ldb = 0;
+    DBuf<float> Yf;
+    if (!y) {
+      if (Y->dtype == FPLS_F32) {
+        Bp = Y->f();
+        ldb = Y->ld;
+      } else {
+        Yf = DBuf<float>(ctx, (size_t)n * m);
+        for (int64_t j = 0; j < m; ++j) f64_to_f32(ctx, Yf.get() + j * n, Y->d() + j * Y->ld, n);
+        Bp = Yf.get();
+        ldb = n;
+      }
+    }
+    gemm_tn_f32(ctx, precision, p, m, n, X.f(), X.ld, M->mX.get(), Bp, ldb, y, M->mY.get(), S, p);
+  } else {
+    DBuf<double> Yd;
+    const double* Bp = nullptr;
+    int64_t ldb = n;
+    if (!y) {
+      if (Y->dtype == FPLS_F64) {
+        Bp = Y->d();
+        ldb = Y->ld;
+      } else {
+        Yd = DBuf<double>(ctx, (size_t)n * m);
+        for (int64_t j = 0; j < m; ++j) f32_to_f64(ctx, Yd.get() + j * n, Y->f() + j * Y->ld, n);
+        Bp = Yd.get();
+      }
+    }
+    gemm_f64(ctx, true, false, p, m, n, 1.0, X.d(), X.ld, M->mX.get(), Bp, ldb, M->mY.get(), 0.0,
+             S, p, y);
+  }
+  if (M->scaling == 2) scale_rows_div(ctx, S, p, p, m, M->vX.get());
+}
+
+bool store_b_policy(int store_b, int64_t p, int64_t m, int n_slices) {
+  if (store_b == 0) return false;
+  if (store_b == 1) return true;
+  const double mb = (double)p * (double)m * (double)std::max(n_slices, 1) * 8.0 / (1024.0 * 1024.0);
+  return mb <= 256.0;  // FASTPLS_STORE_B_MAX_MB default (src/fastPLS.cpp:63-85)
+}
+
+// Compute T = Xs R (kmax columns) into the model, and G = T^T T.
+void fit_scores_and_gram(Ctx* ctx, Model* M, const DMat& X) {
+  const int64_t n = X.rows;
+  const int k = M->kmax;
+  if (X.dtype == FPLS_F32) {
+    M->T_f32 = DBuf<float>(ctx, (size_t)n * k);
+    project_scores(ctx, M, X, k, M->T_f32.get(), nullptr, n);
+  } else {
+    M->T_f64 = DBuf<double>(ctx, (size_t)n * k);
+    project_scores(ctx, M, X, k, nullptr, M->T_f64.get(), n);
+  }
+  M->G = DBuf<double>(ctx, (size_t)k * k);
+  DMat T = M->T();
+  if (T.dtype == FPLS_F32)
+    gemm_tn_f32(ctx, M->precision, k, k, n, T.f(), n, nullptr, T.f(), n, nullptr, nullptr,
+                M->G.get(), k);
+  else
+    gemm_f64(ctx, true, false, k, k, n, 1.0, T.d(), n, nullptr, T.d(), n, nullptr, 0.0,
+             M->G.get(), k);
+}
+
+// R2 = 1 - PRESS/TSS over all columns (RQ, src/fastPLS.cpp:1483-1500):
+// acc[0] += (y - yhat)^2, acc[1] += (y - mean_col(y))^2.  Y centred already.
+__global__ void k_press_tss(const double* __restrict__ Yc, int64_t ldy, const double* __restrict__ Yh,
+                            int64_t n, int64_t m, const double* __restrict__ ymean,
+                            double* __restrict__ acc) {
+  __shared__ double r0[4], r1[4];
+  double a = 0.0, b = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * m;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, j = t / n;
+    const double y = Yc[i + j * ldy], d = y - Yh[i + j * n], e = y - ymean[j];
+    a += d * d;
+    b += e * e;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    r0[threadIdx.x >> 6] = a;
+    r1[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[0], r0[0] + r0[1] + r0[2] + r0[3]);
+    atomicAdd(&acc[1], r1[0] + r1[1] + r1[2] + r1[3]);
+  }
+}
+
+__global__ void k_r2(const double* acc, double* out) { *out = 1.0 - acc[0] / acc[1]; }
+
+__global__ void k_add_row(double* X, int64_t ld, int64_t rows, int64_t cols, const double* v) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < rows * cols;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % rows, j = t / rows;
+    X[i + j * ld] += v[j];
+  }
+}
+
+void add_row(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols, const double* v) {
+  if (rows > 0 && cols > 0)
+    launch(ctx, "add_row", k_add_row, dim3(g1(rows * cols)), dim3(256), 0, X, ld, rows, cols, v);
+}
+
+// Centered response Yc (n x m, FP64) materialised for the fit statistics.
+DBuf<double> centered_response(Ctx* ctx, const Model* M, const DMat* Y, const int32_t* y,
+                               int64_t n, int64_t m);
+
+void r2_into(Ctx* ctx, const double* Yc, int64_t n, int64_t m, const double* Yhat, double* out) {
+  DBuf<double> acc(ctx, 2), ymean(ctx, m), ones(ctx, n);
+  fill_f64(ctx, acc.get(), 0.0, 2);
+  fill_f64(ctx, ones.get(), 1.0 / (double)n, n);
+  gemm_f64(ctx, true, false, m, 1, n, 1.0, Yc, n, nullptr, ones.get(), n, nullptr, 0.0,
+           ymean.get(), m);
+  launch(ctx, "r2", k_press_tss, dim3(g1(n * m)), dim3(256), 0, Yc, n, Yhat, n, m, ymean.get(),
+         acc.get());
+  launch(ctx, "r2", k_r2, dim3(1), dim3(1), 0, acc.get(), out);
+}
+
+__global__ void k_onehot_centered(const int32_t* y, int64_t n, int64_t C, const double* mY,
+                                  double* out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * C;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, c = t / n;
+    out[t] = (y[i] == c + 1 ? 1.0 : 0.0) - mY[c];
+  }
+}
+
+__global__ void k_center_cols(const double* Y, int64_t ldy, int64_t n, int64_t m, const double* mY,
+                              double* out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * m;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, j = t / n;
+    out[t] = Y[i + j * ldy] - mY[j];
+  }
+}
+__global__ void k_center_cols_f32(const float* Y, int64_t ldy, int64_t n, int64_t m,
+                                  const double* mY, double* out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * m;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t % n, j = t / n;
+    out[t] = (double)Y[i + j * ldy] - mY[j];
+  }
+}
+
+DBuf<double> centered_response(Ctx* ctx, const Model* M, const DMat* Y, const int32_t* y,
+                               int64_t n, int64_t m) {
+  DBuf<double> out(ctx, (size_t)n * m);
+  if (y)
+    launch(ctx, "onehot", k_onehot_centered, dim3(g1(n * m)), dim3(256), 0, y, n, m, M->mY.get(),
+           out.get());
+  else if (Y->dtype == FPLS_F64)
+    launch(ctx, "center", k_center_cols, dim3(g1(n * m)), dim3(256), 0, Y->d(), Y->ld, n, m,
+           M->mY.get(), out.get());
+  else
+    launch(ctx, "center", k_center_cols_f32, dim3(g1(n * m)), dim3(256), 0, Y->f(), Y->ld, n, m,
+           M->mY.get(), out.get());
+  return out;
+}
+
+}  // namespace fpls
+
+using namespace fpls;
+
+static std::vector<int> read_ncomp(const int32_t* ncomp, int32_t n_ncomp) {
+  FPLS_REQUIRE(ncomp != nullptr && n_ncomp >= 1, FPLS_ERR_ARG,
+               "ncomp must contain at least one value");
+  return std::vector<int>(ncomp, ncomp + n_ncomp);
+}
+
+extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym,
+                               const fpls_ivec* yv, int32_t n_classes, const int32_t* ncomp,
+                               int32_t n_ncomp, const fpls_opts* opts, fpls_model** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<fpls_model> mdl(new fpls_model());
+  int rc = fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    Model* M = &mdl->m;
+    M->ctx = c;
+    M->method = FPLS_METHOD_PLSSVD;
+    M->precision = o.precision;
+    M->scaling = o.scaling;
+    DBuf<unsigned char> hx, hy;
+    DBuf<int32_t> hl;
+    DMat X = stage(c, Xm, hx, "plssvd: X");
+    DMat Y;
+    const int32_t* y = nullptr;
+    int64_t m = 0;
+    if (yv && yv->ptr) {
+      FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "n_classes must be >= 1 with labels");
+      y = stage_labels(c, yv, X.rows, hl, "plssvd: y");
+      m = n_classes;
+    } else {
+      Y = stage(c, Ym, hy, "plssvd: Y");
+      FPLS_REQUIRE(Y.rows == X.rows, FPLS_ERR_ARG, "X and Y must have the same rows");
+      m = Y.cols;
+    }
+    const int64_t n = X.rows, p = X.cols;
+    FPLS_REQUIRE(n >= 1 && p >= 1 && m >= 1, FPLS_ERR_ARG, "empty input");
+    M->n = n;
+    M->p = p;
+    M->m = m;
+    M->in_dtype = X.dtype;
+    M->label_aware = (y != nullptr) && o.label_aware;
+    // ncomp capping (src/fastPLS.cpp:4801-4814; R/main.R:530-532)
+    const int max_rank = (int)std::min<int64_t>(n, std::min(p, m));
+    FPLS_REQUIRE(max_rank >= 1, FPLS_ERR_RANK, "plssvd effective rank is < 1");
+    std::vector<int> nc = read_ncomp(ncomp, n_ncomp);
+    for (int& v : nc) v = std::min(std::max(v, 1), max_rank);
+    int kmax = *std::max_element(nc.begin(), nc.end());
+    const int L = (int)nc.size();
+    fit_preprocess(c, M, X, y ? nullptr : &Y, y, (int32_t)m, o.scaling, M->label_aware);
+    if (y) M->n_classes = (int)m;
+    DBuf<double> S(c, (size_t)p * m);
+    cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision);
+    // truncated SVD of S (src/fastPLS.cpp:4843-4856, plssvd_use_small_exact_svd :318-324)
+    SvdOut sv;
+    const bool full = M->label_aware || o.svd_method == FPLS_SVD_EXACT || max_rank <= 32;
+    rsvd_dev(c, S.get(), p, p, m, kmax, o.oversample, o.power, o.seed, nullptr, false, full, sv);
+    kmax = std::min(kmax, sv.k);
+    FPLS_REQUIRE(kmax >= 1, FPLS_ERR_RANK, "plssvd effective rank is < 1 after SVD");
+    M->kmax = kmax;
+    M->R = DBuf<double>(c, (size_t)p * kmax);
+    copy_f64(c, M->R.get(), sv.U.get(), p * kmax);
+    M->s = DBuf<double>(c, kmax);
+    copy_f64(c, M->s.get(), sv.s.get(), kmax);
+    M->Q = DBuf<double>(c, (size_t)m * kmax);  // V = Vt^T (m x kmax)
+    transpose_f64(c, sv.Vt.get(), sv.k, kmax, m, M->Q.get(), m);
+    build_projection_cache(c, M);
+    fit_scores_and_gram(c, M, X);
+    // per-slice latent coefficients
+    for (int& v : nc) v = std::min(v, kmax);
+    M->ncomp = nc;
+    DBuf<int> dk(c, L);
+    FPLS_HIP(hipMemcpyAsync(dk.get(), nc.data(), L * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    M->C_latent = DBuf<double>(c, (size_t)kmax * kmax * L);
+    M->W_latent = DBuf<double>(c, (size_t)kmax * m * L);
+    fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
+    fill_f64(c, M->W_latent.get(), 0.0, (int64_t)kmax * m * L);
+    DBuf<int32_t> st(c, L);
+    spd_solve_slices(c, M->G.get(), kmax, kmax, dk.get(), L, M->label_aware ? 2 : 0, M->s.get(),
+                     0.0, M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st.get());
+    M->b_stored = !M->label_aware && store_b_policy(o.store_b, p, m, L);
+    if (M->b_stored) M->B = DBuf<double>(c, (size_t)p * m * L);
+    DBuf<double> Yc;
+    DBuf<double> Td;
+    if (o.fit) {
+      M->R2Y = DBuf<double>(c, L);
+      M->Yfit = DBuf<double>(c, (size_t)n * m * L);
+      Yc = centered_response(c, M, y ? nullptr : &Y, y, n, m);
+      if (M->T_f32.get()) {
+        Td = DBuf<double>(c, (size_t)n * kmax);
+        f32_to_f64(c, Td.get(), M->T_f32.get(), n * (int64_t)kmax);
+      }
+    }
+    for (int a = 0; a < L; ++a) {
+      const int mc = nc[a];
+      double* Ca = M->C_latent.get() + (size_t)a * kmax * kmax;
+      double* Wa = M->W_latent.get() + (size_t)a * kmax * m;
+      // W_a = C_a V_mc^T  (mc x m, ld kmax)
+      gemm_f64(c, false, true, mc, m, mc, 1.0, Ca, kmax, nullptr, M->Q.get(), m, nullptr, 0.0, Wa,
+               kmax);
+      if (M->b_stored)
+        gemm_f64(c, false, false, p, m, mc, 1.0, M->R.get(), p, nullptr, Wa, kmax, nullptr, 0.0,
+                 M->B.get() + (size_t)a * p * m, p);
+      if (o.fit) {
+        double* Yf = M->Yfit.get() + (size_t)a * n * m;
+        const double* Tp = M->T_f32.get() ? Td.get() : M->T_f64.get();
+        gemm_f64(c, false, false, n, m, mc, 1.0, Tp, n, nullptr, Wa, kmax, nullptr, 0.0, Yf, n);
+        r2_into(c, Yc.get(), n, m, Yf, M->R2Y.get() + a);
+        add_row(c, Yf, n, n, m, M->mY.get());
+      }
+    }
+    std::vector<int32_t> hst(L);
+    to_host_i32(c, hst.data(), st.get(), L);
+    for (int a = 0; a < L; ++a)
+      FPLS_REQUIRE(hst[a] != 1, FPLS_ERR_SOLVE, "plssvd latent solve failed");
+    M->n_components = kmax;
+    sync(c);
+  });
+  if (rc == FPLS_OK) *out = mdl.release();
+  return rc;
+}
+
+// ---------------------------------------------------------------- access
+extern "C" int fpls_model_info_get(const fpls_model* model, fpls_model_info* info) {
+  if (!model || !info) return FPLS_ERR_ARG;
+  const Model& M = model->m;
+  std::memset(info, 0, sizeof(*info));
+  info->method = M.method;
+  info->n = M.n;
+  info->p = M.p;
+  info->m = M.m;
+  info->kmax = M.kmax;
+  info->n_ncomp = (int32_t)M.ncomp.size();
+  info->n_components = M.n_components;
+  info->b_stored = M.b_stored ? 1 : 0;
+  info->has_lda = M.lda.empty() ? 0 : 1;
+  info->has_cknn = M.cknn.empty() ? 0 : 1;
+  info->n_classes = M.n_classes;
+  for (size_t i = 0; i < M.ncomp.size() && i < 64; ++i) info->ncomp[i] = M.ncomp[i];
+  return FPLS_OK;
+}
+
+extern "C" int fpls_model_read(fpls_ctx* ctx, const fpls_model* model, int32_t field,
+                               double* host_out, int64_t n_elem) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model& M = model->m;
+    const int64_t L = (int64_t)M.ncomp.size();
+    const double* src = nullptr;
+    int64_t need = 0;
+    DBuf<double> tmp;
+    switch (field) {
+      case FPLS_FIELD_R: src = M.R.get(); need = M.p * M.kmax; break;
+      case FPLS_FIELD_Q: src = M.Q.get(); need = M.m * M.kmax; break;
+      case FPLS_FIELD_MX: src = M.mX.get(); need = M.p; break;
+      case FPLS_FIELD_VX: src = M.vX.get(); need = M.p; break;
+      case FPLS_FIELD_MY: src = M.mY.get(); need = M.m; break;
+      case FPLS_FIELD_B: src = M.B.get(); need = M.p * M.m * L; break;
+      case FPLS_FIELD_C_LATENT: src = M.C_latent.get(); need = (int64_t)M.kmax * M.kmax * L; break;
+      case FPLS_FIELD_W_LATENT: src = M.W_latent.get(); need = (int64_t)M.kmax * M.m * L; break;
+      case FPLS_FIELD_S: src = M.s.get(); need = M.kmax; break;
+      case FPLS_FIELD_R2Y: src = M.R2Y.get(); need = L; break;
+      case FPLS_FIELD_YFIT: src = M.Yfit.get(); need = M.n * M.m * L; break;
+      case FPLS_FIELD_V: src = M.V.get(); need = M.p * M.kmax; break;
+      case FPLS_FIELD_R_PREDICT: src = M.R_predict.get(); need = M.p * M.kmax; break;
+      case FPLS_FIELD_R_OFFSET: src = M.R_offset.get(); need = M.kmax; break;
+      case FPLS_FIELD_TTRAIN:
+        need = M.n * M.kmax;
+        if (M.T_f64.get()) src = M.T_f64.get();
+        else if (M.T_f32.get()) {
+          tmp = DBuf<double>(c, need);
+          f32_to_f64(c, tmp.get(), M.T_f32.get(), need);
+          src = tmp.get();
+        }
+        break;
+      default: fail(FPLS_ERR_ARG, "unknown model field");
+    }
+    FPLS_REQUIRE(src != nullptr, FPLS_ERR_STATE, "model does not hold this field");
+    FPLS_REQUIRE(host_out != nullptr && n_elem >= need, FPLS_ERR_ARG,
+                 "output buffer too small for the model field");
+    to_host(c, host_out, src, need);
+  });
+}
+
+extern "C" void fpls_model_free(fpls_model* model) { delete model; }
+
+// ---------------------------------------------------------------- projection
+extern "C" int fpls_project(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm, int32_t k,
+                            double* T_out) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model* M = &model->m;
+    DBuf<unsigned char> hx;
+    DMat X = stage(c, Xm, hx, "project: X");
+    const int kk = std::min<int>(std::max<int>(k, 1), M->kmax);
+    DBuf<double> T(c, (size_t)X.rows * kk);
+    if (X.dtype == FPLS_F32) gemm_nn_f32(c, M->precision, X.rows, kk, X.cols, X.f(), X.ld,
+                                         M->mX.get(), M->R_predict_f32.get(), M->p, nullptr,
+                                         nullptr, T.get(), X.rows);
+    else project_scores(c, M, X, kk, nullptr, T.get(), X.rows);
+    to_host(c, T_out, T.get(), (size_t)X.rows * kk);
+  });
+}
+
+// ---------------------------------------------------------------- prediction
+extern "C" int fpls_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                            double* ypred) {
+  if (!model) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model* M = &model->m;
+    DBuf<unsigned char> hx;
+    DMat X = stage(c, Xm, hx, "predict: X");
+    FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "predict: Xtest has the wrong number of columns");
+    const int64_t w = X.rows, m = M->m;
+    const int L = (int)M->ncomp.size();
+    DBuf<double> out(c, (size_t)w * m * L);
+    // scores T = Xs R (kmax) once; every slice is a prefix
+    const int kmax = M->kmax;
+    DBuf<double> T(c, (size_t)w * kmax);
+    if (X.dtype == FPLS_F32)
+      gemm_nn_f32(c, M->precision, w, kmax, X.cols, X.f(), X.ld, M->mX.get(),
+                  M->R_predict_f32.get(), M->p, nullptr, nullptr, T.get(), w);
+    else
+      project_scores(c, M, X, kmax, nullptr, T.get(), w);
+    for (int a = 0; a < L; ++a) {
+      const int mc = M->ncomp[a];
+      double* Ya = out.get() + (size_t)a * w * m;
+      if (M->method == FPLS_METHOD_SIMPLS) {
+        // Xs R_mc Q_mc^T
+        gemm_f64(c, false, true, w, m, mc, 1.0, T.get(), w, nullptr, M->Q.get(), m, nullptr, 0.0,
+                 Ya, w);
+      } else {
+        gemm_f64(c, false, false, w, m, mc, 1.0, T.get(), w, nullptr,
+                 M->W_latent.get() + (size_t)a * kmax * m, kmax, nullptr, 0.0, Ya, w);
+      }
+      add_row(c, Ya, w, w, m, M->mY.get());
+    }
+    to_host(c, ypred, out.get(), (size_t)w * m * L);
+  });
+}

@@ -1,0 +1,213 @@
+// Truncated SVD on the device, FP64:
+//
+//  * svd_exact_dev -- thin SVD through the Gram matrix of the narrow side and
+//                     the one-workgroup Jacobi eigensolver (min dim <= 128);
+//                     stands in for svd_econ (src/svd_cpu_exact.cpp:7-38).
+//  * rsvd_dev      -- randomized range finder of src/svd_cpu_rsvd.cpp:95-124
+//                     with the finalisation of :26-93.  The sketch is the
+//                     reference's own mt19937 Gaussian stream (or a caller
+//                     sketch).  Householder QR is replaced by shifted
+//                     CholeskyQR3 (same column span; QR signs do not reach the
+//                     returned factors because Q enters only through Q Uhat).
+//  * fpls_rsvd / fpls_rsvd_xprod -- the C ABI (fastsvd and the matrix-free
+//                     crossprod rSVD of src/fastPLS.cpp:550-623).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+#include "model.h"
+
+namespace fpls {
+
+__global__ void k_sqrt_clamp(const double* __restrict__ w, double* __restrict__ s, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) s[i] = sqrt(fmax(w[i], 0.0));
+}
+
+void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l) {
+  DBuf<double> G(ctx, (size_t)l * l), Linv(ctx, (size_t)l * l), tmp(ctx, (size_t)rows * l);
+  DBuf<int32_t> st(ctx, 1);
+  for (int pass = 0; pass < 3; ++pass) {
+    gemm_f64(ctx, true, false, l, l, rows, 1.0, Y, rows, nullptr, Y, rows, nullptr, 0.0, G.get(),
+             l);
+    chol_inv_lower(ctx, G.get(), l, l, rows, pass == 0 ? -1.0 : 0.0, Linv.get(), st.get());
+    // Y <- Y L^{-T}
+    gemm_f64(ctx, false, true, rows, l, l, 1.0, Y, rows, nullptr, Linv.get(), l, nullptr, 0.0,
+             tmp.get(), rows);
+    copy_f64(ctx, Y, tmp.get(), rows * (int64_t)l);
+  }
+}
+
+void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
+                   SvdOut& out, bool left_only) {
+  const int64_t mn = std::min(rows, cols);
+  FPLS_REQUIRE(mn >= 1, FPLS_ERR_RANK, "svd: empty matrix");
+  FPLS_REQUIRE(mn <= kJacobiMax, FPLS_ERR_ARG,
+               "exact SVD on the device is limited to min(rows, cols) <= 128; use the rSVD path");
+  const int kk = (int)std::min<int64_t>(mn, std::max(k, 1));
+  const int g = (int)mn;
+  DBuf<double> Gm(ctx, (size_t)g * g), w(ctx, g), E(ctx, (size_t)g * g);
+  out.s = DBuf<double>(ctx, kk);
+  out.U = DBuf<double>(ctx, (size_t)rows * kk);
+  if (!left_only) out.Vt = DBuf<double>(ctx, (size_t)kk * cols);
+  out.k = kk;
+  if (cols <= rows) {
+    gemm_f64(ctx, true, false, cols, cols, rows, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
+             Gm.get(), g);
+    sym_eig(ctx, Gm.get(), g, g, w.get(), E.get());
+    launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
+           kk);
+    // U = A V / s
+    gemm_f64(ctx, false, false, rows, kk, cols, 1.0, A, lda, nullptr, E.get(), g, nullptr, 0.0,
+             out.U.get(), rows);
+    scale_cols(ctx, out.U.get(), rows, rows, kk, out.s.get(), true);
+    if (!left_only) transpose_f64(ctx, E.get(), g, cols, kk, out.Vt.get(), kk);
+  } else {
+    gemm_f64(ctx, false, true, rows, rows, cols, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
+             Gm.get(), g);
+    sym_eig(ctx, Gm.get(), g, g, w.get(), E.get());
+    launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
+           kk);
+    copy2d_f64(ctx, out.U.get(), rows, E.get(), g, rows, kk);
+    if (!left_only) {
+      // Vt = U^T A / s (row-wise)
+      gemm_f64(ctx, true, false, kk, cols, rows, 1.0, out.U.get(), rows, nullptr, A, lda,
+               nullptr, 0.0, out.Vt.get(), kk);
+      scale_rows_div(ctx, out.Vt.get(), kk, kk, cols, out.s.get());
+    }
+  }
+}
+
+void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
+              int oversample, int power, uint32_t seed, const double* omega_host,
+              bool left_only, bool use_full, SvdOut& out) {
+  FPLS_REQUIRE(k >= 1, FPLS_ERR_ARG, "truncated_svd: k must be >= 1");
+  const int64_t max_rank = std::min(rows, cols);
+  const int target = (int)std::min<int64_t>(max_rank, std::max(k, 1));
+  const int l = (int)std::min<int64_t>(max_rank, (int64_t)target + std::max(oversample, 0));
+  // dispatch rules of src/svd_iface.cpp:64-70 and src/svd_cpu_rsvd.cpp:100-105
+  if (use_full || max_rank < 6 || l >= max_rank) {
+    svd_exact_dev(ctx, A, lda, rows, cols, target, out, left_only);
+    return;
+  }
+  FPLS_REQUIRE(l <= kJacobiMax, FPLS_ERR_ARG, "rSVD sketch width k + oversample must be <= 128");
+  std::vector<double> om((size_t)cols * l);
+  if (omega_host) std::copy(omega_host, omega_host + om.size(), om.begin());
+  else gaussian_host(om.data(), (int64_t)om.size(), seed);
+  DBuf<double> Om(ctx, om.size()), Y(ctx, (size_t)rows * l), Z(ctx, (size_t)cols * l);
+  to_device(ctx, Om.get(), om.data(), om.size());
+  gemm_f64(ctx, false, false, rows, l, cols, 1.0, A, lda, nullptr, Om.get(), cols, nullptr, 0.0,
+           Y.get(), rows);
+  const int q = std::max(power, 0);
+  if (q == 1) {
+    gemm_f64(ctx, true, false, cols, l, rows, 1.0, A, lda, nullptr, Y.get(), rows, nullptr, 0.0,
+             Z.get(), cols);
+    gemm_f64(ctx, false, false, rows, l, cols, 1.0, A, lda, nullptr, Z.get(), cols, nullptr, 0.0,
+             Y.get(), rows);
+  } else {
+    for (int it = 0; it < q; ++it) {
+      gemm_f64(ctx, true, false, cols, l, rows, 1.0, A, lda, nullptr, Y.get(), rows, nullptr,
+               0.0, Z.get(), cols);
+      orth_cholqr(ctx, Z.get(), cols, l);
+      gemm_f64(ctx, false, false, rows, l, cols, 1.0, A, lda, nullptr, Z.get(), cols, nullptr,
+               0.0, Y.get(), rows);
+    }
+  }
+  orth_cholqr(ctx, Y.get(), rows, l);  // Y now holds Q
+  DBuf<double> Bm(ctx, (size_t)l * cols);
+  gemm_f64(ctx, true, false, l, cols, rows, 1.0, Y.get(), rows, nullptr, A, lda, nullptr, 0.0,
+           Bm.get(), l);
+  if (l <= cols) {
+    // eig(B B^T), descending; usable count as in src/svd_cpu_rsvd.cpp:45-56
+    DBuf<double> BB(ctx, (size_t)l * l), w(ctx, l), E(ctx, (size_t)l * l);
+    gemm_f64(ctx, false, true, l, l, cols, 1.0, Bm.get(), l, nullptr, Bm.get(), l, nullptr, 0.0,
+             BB.get(), l);
+    sym_eig(ctx, BB.get(), l, l, w.get(), E.get());
+    std::vector<double> wh(l);
+    to_host(ctx, wh.data(), w.get(), l);
+    const double tol = DBL_EPSILON * (double)std::max<int64_t>(l, cols) * std::max(wh[0], 1.0);
+    int usable = 0;
+    while (usable < l && wh[usable] > tol) ++usable;
+    usable = std::min(usable, target);
+    if (usable > 0) {
+      out.k = usable;
+      out.s = DBuf<double>(ctx, usable);
+      out.U = DBuf<double>(ctx, (size_t)rows * usable);
+      launch(ctx, "sqrt", k_sqrt_clamp, dim3((usable + 255) / 256), dim3(256), 0, w.get(),
+             out.s.get(), usable);
+      gemm_f64(ctx, false, false, rows, usable, l, 1.0, Y.get(), rows, nullptr, E.get(), l,
+               nullptr, 0.0, out.U.get(), rows);
+      if (!left_only) {
+        out.Vt = DBuf<double>(ctx, (size_t)usable * cols);
+        gemm_f64(ctx, true, false, usable, cols, l, 1.0, E.get(), l, nullptr, Bm.get(), l,
+                 nullptr, 0.0, out.Vt.get(), usable);
+        scale_rows_div(ctx, out.Vt.get(), usable, usable, cols, out.s.get());
+      }
+      return;
+    }
+  }
+  // svd_econ(B) fallback, then U = Q Uhat
+  SvdOut sb;
+  svd_exact_dev(ctx, Bm.get(), l, l, cols, target, sb, left_only);
+  out.k = sb.k;
+  out.s = std::move(sb.s);
+  out.U = DBuf<double>(ctx, (size_t)rows * sb.k);
+  gemm_f64(ctx, false, false, rows, sb.k, l, 1.0, Y.get(), rows, nullptr, sb.U.get(), l, nullptr,
+           0.0, out.U.get(), rows);
+  if (!left_only) out.Vt = std::move(sb.Vt);
+}
+
+}  // namespace fpls
+
+using namespace fpls;
+
+static DBuf<double> as_f64(Ctx* ctx, const DMat& A) {
+  DBuf<double> out(ctx, std::max<int64_t>(1, A.rows * A.cols));
+  if (A.dtype == FPLS_F64) {
+    copy2d_f64(ctx, out.get(), A.rows, A.d(), A.ld, A.rows, A.cols);
+  } else {
+    // convert column by column (ld may exceed rows)
+    for (int64_t j = 0; j < A.cols; ++j) f32_to_f64(ctx, out.get() + j * A.rows, A.f() + j * A.ld, A.rows);
+  }
+  return out;
+}
+
+extern "C" int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* Am, int32_t k, const fpls_opts* opts,
+                         const fpls_mat* omega, double* U, double* s, double* Vt,
+                         int32_t* k_out) {
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hold;
+    DMat A = stage(c, Am, hold, "fpls_rsvd: A");
+    FPLS_REQUIRE(k >= 1, FPLS_ERR_ARG, "truncated_svd: k must be >= 1");
+    DBuf<double> A64 = as_f64(c, A);
+    std::vector<double> om;
+    const double* omp = nullptr;
+    if (omega) {
+      FPLS_REQUIRE(omega->loc == FPLS_HOST && omega->dtype == FPLS_F64, FPLS_ERR_ARG,
+                   "fpls_rsvd: omega must be a host FP64 matrix");
+      om.resize((size_t)omega->rows * omega->cols);
+      const double* src = static_cast<const double*>(omega->ptr);
+      for (int64_t j = 0; j < omega->cols; ++j)
+        for (int64_t i = 0; i < omega->rows; ++i) om[i + j * omega->rows] = src[i + j * omega->ld];
+      const int64_t max_rank = std::min(A.rows, A.cols);
+      const int64_t tgt = std::min<int64_t>(max_rank, k);
+      const int64_t l = std::min<int64_t>(max_rank, tgt + std::max(o.oversample, 0));
+      FPLS_REQUIRE(omega->rows == A.cols && omega->cols == l, FPLS_ERR_ARG,
+                   "fpls_rsvd: omega must be cols(A) x (k + oversample)");
+      omp = om.data();
+    }
+    SvdOut r;
+    const bool full = o.svd_method == FPLS_SVD_EXACT;
+    rsvd_dev(c, A64.get(), A.rows, A.rows, A.cols, k, o.oversample, o.power, o.seed, omp,
+             Vt == nullptr, full, r);
+    if (k_out) *k_out = r.k;
+    if (U) to_host(c, U, r.U.get(), (size_t)A.rows * r.k);
+    if (s) to_host(c, s, r.s.get(), r.k);
+    if (Vt && r.Vt.get()) to_host(c, Vt, r.Vt.get(), (size_t)r.k * A.cols);
+  });
+}

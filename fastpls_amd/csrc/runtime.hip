@@ -1,0 +1,270 @@
+// Context, HBM pool, staging, timing and the small host utilities of the
+// gfx950 fastPLS library.
+#include <algorithm>
+#include <random>
+
+#include "fpls_internal.h"
+
+namespace fpls {
+
+// ---------------------------------------------------------------- pool
+void* Pool::get(size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (bytes == 0) bytes = 256;
+  // Smallest free block that fits without wasting more than half of it.
+  int best = -1;
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    const Block& b = blocks[i];
+    if (!b.used && b.bytes >= bytes && b.bytes <= 2 * bytes + (1 << 20)) {
+      if (best < 0 || b.bytes < blocks[best].bytes) best = static_cast<int>(i);
+    }
+  }
+  if (best >= 0) {
+    blocks[best].used = true;
+    return blocks[best].ptr;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    // Drop cached blocks and retry once.
+    (void)hipGetLastError();
+    for (auto it = blocks.begin(); it != blocks.end();) {
+      if (!it->used) {
+        (void)hipFree(it->ptr);
+        it = blocks.erase(it);
+      } else {
+        ++it;
+      }
+    }
+    FPLS_HIP(hipMalloc(&p, bytes));
+  }
+  blocks.push_back({p, bytes, true});
+  return p;
+}
+
+void Pool::put(void* p) {
+  for (auto& b : blocks) {
+    if (b.ptr == p) {
+      b.used = false;
+      return;
+    }
+  }
+}
+
+void Pool::release_all() {
+  for (auto& b : blocks) (void)hipFree(b.ptr);
+  blocks.clear();
+}
+
+size_t Pool::total_bytes() const {
+  size_t t = 0;
+  for (const auto& b : blocks) t += b.bytes;
+  return t;
+}
+
+// ---------------------------------------------------------------- timing
+hipEvent_t Ctx::event() {
+  if (!spare_events.empty()) {
+    hipEvent_t e = spare_events.back();
+    spare_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  FPLS_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void Ctx::collect_timing() {
+  if (pending.empty()) return;
+  FPLS_HIP(hipStreamSynchronize(stream));
+  for (auto& pe : pending) {
+    float ms = 0.f;
+    FPLS_HIP(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+    KStat& s = kstats[pe.first];
+    s.launches += 1;
+    s.total_ms += ms;
+    spare_events.push_back(pe.second.first);
+    spare_events.push_back(pe.second.second);
+  }
+  pending.clear();
+}
+
+// ---------------------------------------------------------------- staging
+DMat stage(Ctx* ctx, const fpls_mat* m, DBuf<unsigned char>& hold, const char* what) {
+  FPLS_REQUIRE(m != nullptr, FPLS_ERR_ARG, std::string(what) + ": matrix is NULL");
+  FPLS_REQUIRE(m->dtype == FPLS_F32 || m->dtype == FPLS_F64, FPLS_ERR_ARG,
+               std::string(what) + ": dtype must be FPLS_F32 or FPLS_F64");
+  FPLS_REQUIRE(m->rows >= 0 && m->cols >= 0 && m->ld >= std::max<int64_t>(m->rows, 1),
+               FPLS_ERR_ARG, std::string(what) + ": bad shape or leading dimension");
+  FPLS_REQUIRE(m->rows == 0 || m->cols == 0 || m->ptr != nullptr, FPLS_ERR_ARG,
+               std::string(what) + ": NULL data pointer");
+  DMat d;
+  d.dtype = m->dtype;
+  d.rows = m->rows;
+  d.cols = m->cols;
+  const size_t es = m->dtype == FPLS_F32 ? 4 : 8;
+  if (m->loc == FPLS_DEVICE) {
+    d.p = m->ptr;
+    d.ld = m->ld;
+    return d;
+  }
+  FPLS_REQUIRE(m->loc == FPLS_HOST, FPLS_ERR_ARG, std::string(what) + ": bad location");
+  hold = DBuf<unsigned char>(ctx, std::max<size_t>(1, size_t(m->rows) * m->cols * es));
+  if (m->rows > 0 && m->cols > 0) {
+    FPLS_HIP(hipMemcpy2DAsync(hold.get(), m->rows * es, m->ptr, m->ld * es, m->rows * es,
+                              m->cols, hipMemcpyHostToDevice, ctx->stream));
+  }
+  d.p = hold.get();
+  d.ld = std::max<int64_t>(m->rows, 1);
+  return d;
+}
+
+const int32_t* stage_labels(Ctx* ctx, const fpls_ivec* y, int64_t n, DBuf<int32_t>& hold,
+                            const char* what) {
+  FPLS_REQUIRE(y != nullptr && y->ptr != nullptr, FPLS_ERR_ARG,
+               std::string(what) + ": labels are NULL");
+  FPLS_REQUIRE(y->n == n, FPLS_ERR_ARG,
+               std::string(what) + ": one class label per training row is required");
+  if (y->loc == FPLS_DEVICE) return y->ptr;
+  hold = DBuf<int32_t>(ctx, std::max<int64_t>(n, 1));
+  FPLS_HIP(hipMemcpyAsync(hold.get(), y->ptr, n * sizeof(int32_t), hipMemcpyHostToDevice,
+                          ctx->stream));
+  return hold.get();
+}
+
+void to_host(Ctx* ctx, double* dst, const double* src, size_t n) {
+  if (!n) return;
+  FPLS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FPLS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void to_host_i32(Ctx* ctx, int32_t* dst, const int32_t* src, size_t n) {
+  if (!n) return;
+  FPLS_HIP(hipMemcpyAsync(dst, src, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FPLS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void to_device(Ctx* ctx, double* dst, const double* src, size_t n) {
+  if (!n) return;
+  FPLS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+}
+
+void sync(Ctx* ctx) { FPLS_HIP(hipStreamSynchronize(ctx->stream)); }
+
+// ---------------------------------------------------------------- RNG
+// Same generator and distribution as the reference's sketch
+// (src/svd_cpu_rsvd.cpp:10-22): the libstdc++ templates give identical draws.
+void gaussian_host(double* out, int64_t n, uint32_t seed) {
+  std::mt19937 rng(seed);
+  std::normal_distribution<double> norm(0.0, 1.0);
+  for (int64_t i = 0; i < n; ++i) out[i] = norm(rng);
+}
+
+}  // namespace fpls
+
+using namespace fpls;
+
+// ---------------------------------------------------------------- ABI
+extern "C" {
+
+void fpls_opts_default(fpls_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->scaling = 1;
+  o->svd_method = FPLS_SVD_RSVD;
+  o->oversample = 10;
+  o->power = 1;
+  o->seed = 1;
+  o->inc_iters = 2;
+  o->precision = FPLS_PREC_FP32;
+  o->fit = 0;
+  o->store_b = -1;
+}
+
+int fpls_available(void) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  for (int d = 0; d < count; ++d) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) == hipSuccess &&
+        std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+      return 1;
+  }
+  return 0;
+}
+
+int fpls_ctx_create(int device, fpls_ctx** out) {
+  if (!out) return FPLS_ERR_ARG;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    return FPLS_ERR_NO_DEVICE;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return FPLS_ERR_NO_DEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return FPLS_ERR_NO_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return FPLS_ERR_HIP;
+  fpls_ctx* c = new fpls_ctx();
+  c->c.device = device;
+  c->c.num_cus = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return FPLS_ERR_HIP;
+  }
+  *out = c;
+  return FPLS_OK;
+}
+
+void fpls_ctx_destroy(fpls_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  for (auto& pe : ctx->c.pending) {
+    (void)hipEventDestroy(pe.second.first);
+    (void)hipEventDestroy(pe.second.second);
+  }
+  for (auto e : ctx->c.spare_events) (void)hipEventDestroy(e);
+  ctx->c.pool.release_all();
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+const char* fpls_last_error(const fpls_ctx* ctx) { return ctx ? ctx->c.err.c_str() : ""; }
+
+int fpls_ctx_synchronize(fpls_ctx* ctx) {
+  if (!ctx) return FPLS_ERR_ARG;
+  return hipStreamSynchronize(ctx->c.stream) == hipSuccess ? FPLS_OK : FPLS_ERR_HIP;
+}
+
+int fpls_timing_enable(fpls_ctx* ctx, int enable) {
+  if (!ctx) return FPLS_ERR_ARG;
+  try {
+    ctx->c.collect_timing();
+  } catch (const Error& e) {
+    ctx->c.err = e.what();
+    return e.code;
+  }
+  ctx->c.timing = enable != 0;
+  ctx->c.kstats.clear();
+  return FPLS_OK;
+}
+
+int fpls_kernel_time(fpls_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms) {
+  if (!ctx || !kernel) return FPLS_ERR_ARG;
+  try {
+    ctx->c.collect_timing();
+  } catch (const Error& e) {
+    ctx->c.err = e.what();
+    return e.code;
+  }
+  auto it = ctx->c.kstats.find(kernel);
+  if (launches) *launches = it == ctx->c.kstats.end() ? 0 : it->second.launches;
+  if (total_ms) *total_ms = it == ctx->c.kstats.end() ? 0.0 : it->second.total_ms;
+  return FPLS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+/*
+ * fastpls_hip.h -- C ABI of the MI355X (gfx950) fastPLS hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status (FPLS_OK == 0, negative on error; the message is
+ * available from fpls_last_error()).  No C++ types or exceptions cross it.
+ *
+ * Matrices are column-major (R / Armadillo layout) and described by
+ * fpls_mat: a host or device pointer, an element type (FP32 / FP64), the
+ * shape and the leading dimension.  Device pointers must live on the
+ * context's device; host buffers are staged through the context's stream.
+ * Results are written to caller-allocated host buffers (FP64 / int32), the
+ * way an Rcpp wrapper hands R-allocated vectors to C.
+ *
+ * A fitted model (fpls_model) stays resident in HBM between fit, classifier
+ * training and prediction, so the latent scores of the training rows are
+ * computed once and reused (the reference recomputes them for LDA and
+ * candidate kNN: R/main.R:1216-1251, 826-836).
+ *
+ * Reference interfaces replaced (paths relative to the fastPLS 0.3 tree):
+ *   fpls_available            has_cuda()                 src/fastPLS.cpp:1607-1609
+ *   fpls_plssvd_fit           pls_model1                 src/fastPLS.cpp:4780-4986
+ *                             .plssvd_label_aware_stream_model (labels) R/main.R:513-655
+ *   fpls_simpls_fit           pls_model2_fast            src/fastPLS.cpp:2454-2778
+ *   fpls_model_read           the Rcpp::List fields returned by the fits
+ *   fpls_predict              pls_predict                src/fastPLS.cpp:3211-3445
+ *   fpls_project              .fastpls_latent_scores     R/main.R:965-1017
+ *   fpls_lda_train_prefix     lda_project_train_prefix_cpp src/fastPLS.cpp:1930-1957
+ *   fpls_lda_train_scores     lda_train_prefix_cpp       src/fastPLS.cpp:1817-1927
+ *   fpls_lda_read             the per-ncomp LDA list fields (means, linear, ...)
+ *   fpls_lda_predict          lda_project_predict_labels_cpp src/fastPLS.cpp:2100-2147
+ *   fpls_cknn_fit             .fit_candidate_knn         R/main.R:795-873
+ *   fpls_cknn_predict         .candidate_knn_predictions R/main.R:875-950
+ *   fpls_cknn_predict_scores  candidate_knn_predict_cpp  src/fastPLS.cpp:1627-1648
+ *   fpls_rsvd                 truncated_svd_cpu_rsvd / fastsvd  src/svd_cpu_rsvd.cpp:95-124
+ *   fpls_rsvd_xprod           truncated_rsvd_crossprod_double   src/fastPLS.cpp:550-623
+ *   fpls_class_sums           rowsum() class sums of the label-aware stream R/main.R:563-588
+ */
+#ifndef FASTPLS_HIP_H
+#define FASTPLS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define FPLS_OK                 0
+#define FPLS_ERR_ARG           -1   /* bad argument / shape                     */
+#define FPLS_ERR_HIP           -2   /* HIP runtime error                        */
+#define FPLS_ERR_EMPTY_CLASS   -3   /* LDA: a class has no rows (src/fastPLS.cpp:1766-1768) */
+#define FPLS_ERR_LABEL_RANGE   -4   /* labels outside 1..C (src/fastPLS.cpp:1757-1760) */
+#define FPLS_ERR_RANK          -5   /* effective rank < 1 (src/fastPLS.cpp:4812-4814) */
+#define FPLS_ERR_SOLVE         -6   /* singular system, no fallback left       */
+#define FPLS_ERR_NO_DEVICE     -7   /* no usable gfx950 device                  */
+#define FPLS_ERR_STATE         -8   /* model lacks the requested part          */
+
+/* ---- element types and locations --------------------------------------- */
+#define FPLS_F32     0
+#define FPLS_F64     1
+#define FPLS_HOST    0
+#define FPLS_DEVICE  1
+
+typedef struct fpls_mat {
+  const void* ptr;   /* first element, column-major                      */
+  int32_t dtype;     /* FPLS_F32 | FPLS_F64                              */
+  int32_t loc;       /* FPLS_HOST | FPLS_DEVICE                          */
+  int64_t rows;
+  int64_t cols;
+  int64_t ld;        /* leading dimension >= rows                       */
+} fpls_mat;
+
+typedef struct fpls_ivec {
+  const int32_t* ptr;  /* 1-based class codes                            */
+  int32_t loc;         /* FPLS_HOST | FPLS_DEVICE                         */
+  int64_t n;
+} fpls_ivec;
+
+/* ---- options ------------------------------------------------------------ */
+#define FPLS_SVD_EXACT   3   /* reference SVD method ids (src/svd_iface.h:39-44) */
+#define FPLS_SVD_RSVD    4
+#define FPLS_PREC_FP32   0   /* plain FP32 MFMA contractions                    */
+#define FPLS_PREC_BF16X3 1   /* split-BF16 x3 MFMA contractions                 */
+
+typedef struct fpls_opts {
+  int32_t scaling;     /* 1 centering, 2 autoscaling, 3 none (R/main.R:5890)  */
+  int32_t svd_method;  /* FPLS_SVD_EXACT | FPLS_SVD_RSVD                      */
+  int32_t oversample;  /* rSVD oversampling (default 10)                     */
+  int32_t power;       /* rSVD power iterations (default 1)                  */
+  uint32_t seed;       /* sketch seed (default 1)                            */
+  int32_t inc_iters;   /* SIMPLS refresh power iterations (default 2)        */
+  int32_t precision;   /* FPLS_PREC_*                                        */
+  int32_t fit;         /* compute Yfit / R2Y                                 */
+  int32_t store_b;     /* -1 auto (<= 256 MB), 0 never, 1 always             */
+  int32_t label_aware; /* labels + 1: .plssvd_label_aware_stream_model rules
+                          (ridge 1e-10 mean(diag G), guarded sd, no B)
+                          R/main.R:513-655                                    */
+  int32_t reserved[6];
+} fpls_opts;
+
+void fpls_opts_default(fpls_opts* opts);
+
+/* ---- context ------------------------------------------------------------ */
+typedef struct fpls_ctx fpls_ctx;
+typedef struct fpls_model fpls_model;
+
+int fpls_available(void);
+int fpls_ctx_create(int device, fpls_ctx** out);
+void fpls_ctx_destroy(fpls_ctx* ctx);
+const char* fpls_last_error(const fpls_ctx* ctx);
+int fpls_ctx_synchronize(fpls_ctx* ctx);
+/* Kernel timing: when enabled, every launch of a named kernel is bracketed
+   by HIP events on the context stream; fpls_kernel_time returns the count
+   and the summed milliseconds since the last reset. */
+int fpls_timing_enable(fpls_ctx* ctx, int enable);
+int fpls_kernel_time(fpls_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+
+/* ---- fits ---------------------------------------------------------------- */
+/* PLSSVD.  Give either a dense response Y (n x m) or 1-based labels y with
+   n_classes (the one-hot response is never formed: class sums are streamed
+   from the int32 labels). */
+int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
+                    int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                    const fpls_opts* opts, fpls_model** out);
+
+/* SIMPLS ("fast" incremental core).  Dense Y or labels (one-hot response). */
+int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
+                    int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                    const fpls_opts* opts, fpls_model** out);
+
+/* ---- model access -------------------------------------------------------- */
+#define FPLS_METHOD_PLSSVD 1
+#define FPLS_METHOD_SIMPLS 3
+
+typedef struct fpls_model_info {
+  int32_t method;      /* FPLS_METHOD_*                                      */
+  int64_t n, p, m;
+  int32_t kmax;        /* columns of R / Q                                   */
+  int32_t n_ncomp;     /* slices                                             */
+  int32_t n_components;/* components actually extracted (SIMPLS early stop)  */
+  int32_t b_stored;
+  int32_t has_lda;
+  int32_t has_cknn;
+  int32_t n_classes;
+  int32_t ncomp[64];   /* effective ncomp per slice (first 64)             */
+} fpls_model_info;
+
+/* Field ids for fpls_model_read; sizes in FP64 elements. */
+#define FPLS_FIELD_R        1  /* p x kmax                                  */
+#define FPLS_FIELD_Q        2  /* m x kmax                                  */
+#define FPLS_FIELD_MX       3  /* p                                         */
+#define FPLS_FIELD_VX       4  /* p                                         */
+#define FPLS_FIELD_MY       5  /* m                                         */
+#define FPLS_FIELD_B        6  /* p x m x n_ncomp (when b_stored)           */
+#define FPLS_FIELD_C_LATENT 7  /* kmax x kmax x n_ncomp (PLSSVD)            */
+#define FPLS_FIELD_W_LATENT 8  /* kmax x m x n_ncomp (PLSSVD)               */
+#define FPLS_FIELD_S        9  /* kmax singular values (PLSSVD)             */
+#define FPLS_FIELD_TTRAIN  10  /* n x kmax training scores                   */
+#define FPLS_FIELD_R2Y     11  /* n_ncomp (fit)                              */
+#define FPLS_FIELD_YFIT    12  /* n x m x n_ncomp (fit)                      */
+#define FPLS_FIELD_V       13  /* p x kmax SIMPLS V basis                    */
+#define FPLS_FIELD_R_PREDICT 14 /* p x kmax  R / vX                          */
+#define FPLS_FIELD_R_OFFSET  15 /* kmax      mX R_predict                    */
+
+int fpls_model_info_get(const fpls_model* model, fpls_model_info* info);
+int fpls_model_read(fpls_ctx* ctx, const fpls_model* model, int32_t field, double* host_out,
+                    int64_t n_elem);
+void fpls_model_free(fpls_model* model);
+
+/* ---- prediction ------------------------------------------------------------ */
+/* Ypred (w x m x n_ncomp) = ((Xtest - mX)/vX) B + mY, via B or the latent
+   factors. */
+int fpls_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest, double* ypred);
+/* Latent scores T = X R_predict[:, :k] - offset[:k] (n x k). */
+int fpls_project(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* X, int32_t k,
+                 double* T_out);
+
+/* ---- LDA head -------------------------------------------------------------- */
+/* Train the prefix LDA on the model's projected training rows (reusing the
+   resident training scores when X is the fit's own matrix: pass X = NULL). */
+int fpls_lda_train_prefix(fpls_ctx* ctx, fpls_model* model, const fpls_mat* Xtrain,
+                          const fpls_ivec* y, int32_t n_classes, const int32_t* ncomp,
+                          int32_t n_ncomp, double ridge);
+#define FPLS_LDA_MEANS     1  /* C x k                                        */
+#define FPLS_LDA_INV_COV   2  /* k x k                                        */
+#define FPLS_LDA_LINEAR    3  /* C x k                                        */
+#define FPLS_LDA_CONSTANTS 4  /* C                                            */
+#define FPLS_LDA_PRIORS    5  /* C                                            */
+#define FPLS_LDA_RIDGE     6  /* 1                                            */
+int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t slice, int32_t field,
+                  double* host_out, int64_t n_elem);
+/* Labels (w x n_ncomp, 1-based) and optionally the discriminant scores
+   (w x C x n_ncomp, may be NULL). */
+int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                     int32_t* labels, double* scores);
+/* Array-level trainer on given scores T (n x kmax). Outputs per slice s with
+   k_s = ncomp[s]: means/linear (C x k_s), inv_cov (k_s x k_s), constants (C),
+   packed one slice after another; priors (C) and ridge (n_ncomp). */
+int fpls_lda_train_scores(fpls_ctx* ctx, const fpls_mat* T, const fpls_ivec* y,
+                          int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                          double ridge, double* means, double* inv_cov, double* linear,
+                          double* constants, double* priors, double* ridge_out);
+
+/* ---- candidate kNN head ------------------------------------------------------- */
+int fpls_cknn_fit(fpls_ctx* ctx, fpls_model* model, const fpls_mat* Xtrain, const fpls_ivec* y,
+                  int32_t n_classes, int32_t knn_k, double tau, double alpha, int32_t top_m);
+/* top_index (w x top x n_ncomp, 1-based class), top_score (same, may be NULL). */
+int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                      int32_t top, int32_t* top_index, double* top_score);
+/* Array-level scorer on given normalised scores (candidate_knn_predict_cpp). */
+int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_norm,
+                             const fpls_mat* Ttrain_norm, const fpls_ivec* y_codes,
+                             const fpls_mat* centroids, const double* candidate_bias,
+                             int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
+                             double alpha, int32_t* top_index, double* top_score);
+
+/* ---- truncated SVD ---------------------------------------------------------- */
+/* rSVD (or exact, per opts->svd_method / the reference's dispatch rules) of A
+   (rows x cols).  U (rows x k), s (k), Vt (k x cols); *k_out = columns
+   produced.  omega (cols x l, FP64) replaces the Gaussian sketch when given
+   (the learned-sketch extension); NULL reproduces the reference's stream. */
+int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* A, int32_t k, const fpls_opts* opts,
+              const fpls_mat* omega, double* U, double* s, double* Vt, int32_t* k_out);
+/* Matrix-free rSVD of S = X^T Y without forming S. */
+int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, int32_t k,
+                    const fpls_opts* opts, double* U, double* s, double* Vt, int32_t* k_out);
+
+/* ---- streaming primitive ------------------------------------------------------ */
+/* Class sums (p x C, FP64) and column sums (p) of X in one pass from int32
+   labels; counts (C). */
+int fpls_class_sums(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y, int32_t n_classes,
+                    double* class_sums, double* col_sums, int64_t* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTPLS_HIP_H */

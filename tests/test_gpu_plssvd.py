@@ -1,0 +1,115 @@
+"""GPU parity: truncated SVD, PLSSVD and the LDA head against the FP64 oracle.
+
+Tolerances: FP64 inputs run FP64 kernels end to end (rel 1e-9).  FP32 inputs
+run FP32 MFMA contractions with FP64 factorisations (rel 1e-4 on
+coefficients); predicted labels must match exactly except rows whose FP64
+oracle top-2 margin is below 1e-3 of the score scale (documented near-ties).
+"""
+import numpy as np
+import pytest
+
+from oracle import fastpls_oracle as O
+from tests.helpers import align_signs, label_mismatches_excused, rel_err, sin_theta
+
+pytestmark = pytest.mark.gpu
+
+
+def _lowrank(rng, n, p, r, noise):
+    U, _ = np.linalg.qr(rng.standard_normal((n, r)))
+    V, _ = np.linalg.qr(rng.standard_normal((p, r)))
+    s = 10.0 * 0.8 ** np.arange(r)
+    return (U * s) @ V.T + noise * rng.standard_normal((n, p))
+
+
+@pytest.mark.parametrize("rows,cols,k,over,power", [
+    (300, 40, 10, 10, 1),   # rSVD, q == 1 (no re-orthonormalisation)
+    (300, 60, 8, 6, 2),     # rSVD, q == 2 (QR between passes)
+    (500, 30, 25, 10, 1),   # l >= max_rank -> exact
+    (50, 200, 5, 10, 0),    # wide, q == 0
+])
+def test_fastsvd_matches_oracle(ctx, rows, cols, k, over, power):
+    import fastpls_amd as F
+    rng = np.random.default_rng(7)
+    A = _lowrank(rng, rows, cols, min(12, min(rows, cols)), 1e-3)
+    got = F.fastsvd(A, k, rsvd_oversample=over, rsvd_power=power, seed=3)
+    ref = O.truncated_svd(A, k, "rsvd", over, power, 3)
+    kk = ref["U"].shape[1]
+    assert got["u"].shape[1] == kk
+    np.testing.assert_allclose(got["d"], ref["s"], rtol=1e-9, atol=1e-12 * ref["s"][0])
+    # leading singular vectors with a clear gap agree column-wise after sign alignment
+    kgap = min(kk, 8)
+    U = align_signs(got["u"][:, :kgap], ref["U"][:, :kgap])
+    assert rel_err(U, ref["U"][:, :kgap]) < 1e-7
+    assert sin_theta(got["u"], ref["U"]) < 1e-7
+
+
+def test_plssvd_regression_fp64(ctx):
+    import fastpls_amd as F
+    rng = np.random.default_rng(11)
+    X = _lowrank(rng, 150, 60, 15, 0.1)
+    Y = X @ rng.standard_normal((60, 8)) * 0.1 + 0.05 * rng.standard_normal((150, 8))
+    ncomp = [2, 5, 7]
+    m = F.pls(X, Y, method="plssvd", ncomp=ncomp, fit=True)
+    ref = O.pls_model1(X, Y, ncomp, 1, True, "rsvd", 10, 1, 1)
+    assert rel_err(m["mX"], ref["mX"]) < 1e-13
+    assert rel_err(m["mY"], ref["mY"]) < 1e-13
+    R = align_signs(m["R"], ref["R"])
+    assert rel_err(R, ref["R"]) < 1e-9
+    # B, Yfit and R2Y are sign invariant
+    assert rel_err(m["B"], ref["B"]) < 1e-9
+    assert rel_err(m["Yfit"], ref["Yfit"]) < 1e-9
+    np.testing.assert_allclose(m["R2Y"], ref["R2Y"], rtol=1e-9)
+    Xt = rng.standard_normal((20, 60))
+    yp = F.predict(m, Xt)["Ypred"]
+    assert rel_err(yp, O.pls_predict(ref, Xt)) < 1e-9
+
+
+def test_plssvd_autoscaling_fp64(ctx):
+    import fastpls_amd as F
+    rng = np.random.default_rng(12)
+    X = _lowrank(rng, 120, 40, 10, 0.2) * rng.uniform(0.5, 3.0, 40) + 2.0
+    Y = rng.standard_normal((120, 6))
+    m = F.pls(X, Y, method="plssvd", ncomp=[3, 6], scaling="autoscaling")
+    ref = O.pls_model1(X, Y, [3, 6], 2, False, "rsvd", 10, 1, 1)
+    assert rel_err(m["vX"], ref["vX"]) < 1e-12
+    assert rel_err(m["B"], ref["B"]) < 1e-9
+
+
+def test_plssvd_lda_fp32_labels(ctx):
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    Xtr, ytr, Xte, yte = synth.classes_on_sphere(n_classes=12, per_class_train=150,
+                                                 per_class_test=40, p=96, seed=5)
+    ncomp = 11
+    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=ncomp, classifier="lda")
+    ref = O.pls_plssvd_lda(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [ncomp])
+    # coefficients within the FP32 budget
+    assert rel_err(align_signs(m["R"], ref["R"]), ref["R"]) < 1e-4
+    assert rel_err(m["B"], ref["B"]) < 1e-4
+    lda_ref = ref["lda"][ncomp]
+    C = 12
+    lin = m.lda_read(0, "linear", (C, ncomp))
+    cst = m.lda_read(0, "constants", (C,))
+    # linear terms flip with the latent signs; constants are sign invariant
+    np.testing.assert_allclose(cst, lda_ref["constants"], rtol=1e-4, atol=1e-4)
+    assert rel_err(np.abs(lin), np.abs(lda_ref["linear"])) < 1e-3
+    labels = m["Ypred_index"][:, 0]
+    _, sc = O.lda_project_predict_labels(Xte.astype(np.float64), ref["R_predict"][:, :ncomp],
+                                         ref["R_offset"][:ncomp], lda_ref, return_scores=True)
+    scale = float(np.abs(sc).max())
+    bad, unexcused = label_mismatches_excused(labels, ref["Ypred_labels"][:, 0], sc, 1e-3 * scale)
+    assert not unexcused, f"label mismatches beyond ties: {unexcused}"
+
+
+def test_plssvd_label_aware_equals_dense_onehot(ctx):
+    """Label-aware S == Xc^T onehot(y) (dense); one-hot never formed."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(3)
+    n, p, C = 400, 30, 6
+    X = rng.standard_normal((n, p)) + 1.5
+    y = rng.integers(1, C + 1, n).astype(np.int32)
+    m_lab = F.pls(X, y, method="plssvd", ncomp=[4])
+    m_dense = F.pls(X, O.transformy(y), method="plssvd", ncomp=[4])
+    assert rel_err(m_lab["B"], m_dense["B"]) < 1e-11
+    ref = O.pls_model1(X, O.transformy(y), [4], 1, False, "rsvd", 10, 1, 1)
+    assert rel_err(m_lab["B"], ref["B"]) < 1e-9
