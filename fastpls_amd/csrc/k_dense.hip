@@ -21,71 +21,53 @@ namespace fpls {
 constexpr int JMAX = kJacobiMax;  // 128
 constexpr int JLD = JMAX + 1;
 
+// One round of the parallel ordering rotates n/2 disjoint pairs at once.
+// Every 2x2 block (rows of pair k, columns of pair l) is then updated
+// independently as J_k^T A_kl J_l, so a round is one parameter phase and one
+// update phase.  Rotations are skipped when |a_pq| <= 1e-15 sqrt(|a_pp a_qq|)
+// (the relative-accuracy criterion of Demmel & Veselic, SIAM J. Matrix Anal.
+// Appl. 13, 1992); a sweep without rotations ends the iteration.
+template <int NMAX, bool V_IN_LDS>
 __global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain, int64_t lda,
                                                   int n, double* __restrict__ w,
                                                   double* __restrict__ Vout,
-                                                  double* __restrict__ Vw, int max_sweeps) {
-  __shared__ double A[JMAX * JLD];
+                                                  double* __restrict__ Vg, int max_sweeps) {
+  // static LDS: A (NMAX x NMAX+1) and, when it fits beside A in the 160 KiB
+  // of a gfx950 CU, V (NMAX x NMAX)
+  __shared__ double sA[NMAX * (NMAX + 1)];
+  __shared__ double sV[V_IN_LDS ? NMAX * NMAX : 1];
   __shared__ double cs[JMAX / 2], sn[JMAX / 2];
   __shared__ int pp[JMAX / 2], qq[JMAX / 2];
-  __shared__ double red[16];
-  __shared__ int done;
+  __shared__ int rot_count;
+  __shared__ int rank[JMAX];
   const int t = threadIdx.x, nt = blockDim.x;
   const int N = n + (n & 1);  // even order; index n (if any) is a zero dummy
+  const int h = N / 2;
+  const int lda_s = N + 1;
+  double* A = sA;
+  double* V = V_IN_LDS ? sV : Vg;
   for (int idx = t; idx < N * N; idx += nt) {
     const int i = idx % N, j = idx / N;
-    A[i + j * JLD] = (i < n && j < n) ? 0.5 * (Ain[i + j * lda] + Ain[j + i * lda]) : 0.0;
-    Vw[i + (int64_t)j * N] = (i == j) ? 1.0 : 0.0;
+    A[i + j * lda_s] = (i < n && j < n) ? 0.5 * (Ain[i + j * lda] + Ain[j + i * lda]) : 0.0;
+    V[i + j * N] = (i == j) ? 1.0 : 0.0;
   }
-  __syncthreads();
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    // convergence test: off-diagonal mass relative to the total
-    double off = 0.0, tot = 0.0;
-    for (int idx = t; idx < N * N; idx += nt) {
-      const int i = idx % N, j = idx / N;
-      const double v = A[i + j * JLD];
-      tot += v * v;
-      if (i != j) off += v * v;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      off += __shfl_down(off, o, 64);
-      tot += __shfl_down(tot, o, 64);
-    }
-    if ((t & 63) == 0) {
-      red[t >> 6] = off;
-    }
+    if (t == 0) rot_count = 0;
     __syncthreads();
-    if (t == 0) {
-      double so = 0.0;
-      for (int k = 0; k < nt / 64; ++k) so += red[k];
-      red[15] = so;
-    }
-    __syncthreads();
-    const double off_all = red[15];
-    __syncthreads();
-    if ((t & 63) == 0) red[t >> 6] = tot;
-    __syncthreads();
-    if (t == 0) {
-      double st = 0.0;
-      for (int k = 0; k < nt / 64; ++k) st += red[k];
-      done = (off_all <= 1e-32 * st || off_all == 0.0) ? 1 : 0;
-    }
-    __syncthreads();
-    if (done) break;
     for (int round = 0; round < N - 1; ++round) {
-      if (t < N / 2) {
+      if (t < h) {
         auto pos = [&](int k) { return k == 0 ? 0 : ((k - 1 + round) % (N - 1)) + 1; };
         int p = pos(t), q = pos(N - 1 - t);
         if (p > q) { const int tmp = p; p = q; q = tmp; }
-        const double apq = A[p + q * JLD];
+        const double apq = A[p + q * lda_s];
+        const double app = A[p + p * lda_s], aqq = A[q + q * lda_s];
         double c = 1.0, s = 0.0;
-        if (apq != 0.0) {
-          const double app = A[p + p * JLD], aqq = A[q + q * JLD];
+        if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
           const double zeta = (aqq - app) / (2.0 * apq);
           const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
           c = 1.0 / sqrt(1.0 + tt * tt);
           s = tt * c;
+          atomicAdd(&rot_count, 1);
         }
         pp[t] = p;
         qq[t] = q;
@@ -93,38 +75,46 @@ __global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain
         sn[t] = s;
       }
       __syncthreads();
-      // rows: A <- J^T A
-      for (int idx = t; idx < (N / 2) * N; idx += nt) {
-        const int k = idx % (N / 2), j = idx / (N / 2);
-        const int p = pp[k], q = qq[k];
-        const double c = cs[k], s = sn[k];
-        const double apj = A[p + j * JLD], aqj = A[q + j * JLD];
-        A[p + j * JLD] = c * apj - s * aqj;
-        A[q + j * JLD] = s * apj + c * aqj;
+      for (int idx = t; idx < h * h; idx += nt) {
+        const int k = idx % h, l = idx / h;
+        const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
+        if (sk == 0.0 && sl == 0.0) continue;
+        const int pk = pp[k], qk = qq[k], pl = pp[l], ql = qq[l];
+        const double a = A[pk + pl * lda_s], b = A[pk + ql * lda_s];
+        const double c_ = A[qk + pl * lda_s], d = A[qk + ql * lda_s];
+        const double m00 = ck * a - sk * c_, m01 = ck * b - sk * d;
+        const double m10 = sk * a + ck * c_, m11 = sk * b + ck * d;
+        double n00 = m00 * cl - m01 * sl, n01 = m00 * sl + m01 * cl;
+        double n10 = m10 * cl - m11 * sl, n11 = m10 * sl + m11 * cl;
+        if (k == l && sk != 0.0) {
+          n01 = 0.0;
+          n10 = 0.0;
+        }
+        A[pk + pl * lda_s] = n00;
+        A[pk + ql * lda_s] = n01;
+        A[qk + pl * lda_s] = n10;
+        A[qk + ql * lda_s] = n11;
       }
-      __syncthreads();
-      // columns: A <- A J ; V <- V J
-      for (int idx = t; idx < (N / 2) * N; idx += nt) {
-        const int k = idx / N, i = idx % N;
-        const int p = pp[k], q = qq[k];
-        const double c = cs[k], s = sn[k];
-        const double aip = A[i + p * JLD], aiq = A[i + q * JLD];
-        A[i + p * JLD] = c * aip - s * aiq;
-        A[i + q * JLD] = s * aip + c * aiq;
-        const double vip = Vw[i + (int64_t)p * N], viq = Vw[i + (int64_t)q * N];
-        Vw[i + (int64_t)p * N] = c * vip - s * viq;
-        Vw[i + (int64_t)q * N] = s * vip + c * viq;
+      for (int idx = t; idx < N * h; idx += nt) {
+        const int i = idx % N, l = idx / N;
+        const double cl = cs[l], sl = sn[l];
+        if (sl == 0.0) continue;
+        const int pl = pp[l], ql = qq[l];
+        const double vip = V[i + pl * N], viq = V[i + ql * N];
+        V[i + pl * N] = cl * vip - sl * viq;
+        V[i + ql * N] = sl * vip + cl * viq;
       }
       __syncthreads();
     }
+    if (rot_count == 0) break;
+    __syncthreads();
   }
   // sort descending (stable on index), n <= 128
-  __shared__ int rank[JMAX];
   if (t < n) {
-    const double wi = A[t + t * JLD];
+    const double wi = A[t + t * lda_s];
     int r = 0;
     for (int j = 0; j < n; ++j) {
-      const double wj = A[j + j * JLD];
+      const double wj = A[j + j * lda_s];
       if (wj > wi || (wj == wi && j < t)) ++r;
     }
     rank[t] = r;
@@ -133,16 +123,22 @@ __global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain
   __syncthreads();
   for (int idx = t; idx < n * n; idx += nt) {
     const int i = idx % n, j = idx / n;
-    Vout[i + (int64_t)rank[j] * n] = Vw[i + (int64_t)j * N];
+    Vout[i + (int64_t)rank[j] * n] = V[i + j * N];
   }
 }
 
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
   FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
                "sym_eig: order must be in 1..128 for the one-workgroup solver");
-  const int N = n + (n & 1);
-  DBuf<double> Vw(ctx, (size_t)N * N);
-  launch(ctx, "sym_eig_jacobi", k_sym_eig, dim3(1), dim3(1024), 0, A, lda, n, w, V, Vw.get(), 60);
+  if (n <= 100) {
+    launch(ctx, "sym_eig_jacobi", k_sym_eig<100, true>, dim3(1), dim3(1024), 0, A, lda, n, w, V,
+           (double*)nullptr, 40);
+  } else {
+    const int N = n + (n & 1);
+    DBuf<double> Vw(ctx, (size_t)N * N);
+    launch(ctx, "sym_eig_jacobi", k_sym_eig<128, false>, dim3(1), dim3(1024), 0, A, lda, n, w, V,
+           Vw.get(), 40);
+  }
 }
 
 // --------------------------------------------------------------- slice solves
@@ -150,21 +146,30 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
 // mode 1: (A_k + lambda I) X = I, lambda from the  -- LDA inverse covariance
 //         reference's ridge rule; lambda_out[s] = lambda
 // mode 2: (A_k + r I) X = diag(d_k), r = 1e-10 mean(diag A_k)  -- label-aware PLSSVD
+// Cholesky and the inverse of the factor each take one barrier per step: the
+// trailing update of step j uses column j unscaled (A -= c c^T / d), and the
+// scaling of column j by 1/sqrt(d) is deferred to step j+1, when no thread
+// reads it any more.  The solution is then X = Z^T Z RHS with Z = L^{-1}.
+template <int KMAX, bool Z_IN_LDS>
 __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
                                                     const int* __restrict__ ks, int mode,
                                                     const double* __restrict__ dvals,
                                                     double ridge, double* __restrict__ X,
                                                     int64_t ldx, int64_t slice_stride,
                                                     double* __restrict__ lambda_out,
-                                                    int32_t* __restrict__ status) {
-  __shared__ double L[JMAX * JLD];
-  __shared__ int piv[JMAX];
+                                                    int32_t* __restrict__ status,
+                                                    double* __restrict__ Zg) {
+  __shared__ double L[KMAX * (KMAX + 1)];
+  __shared__ double sZ[Z_IN_LDS ? KMAX * (KMAX + 1) : 1];
+  __shared__ int piv[KMAX];
   __shared__ double shift;
   __shared__ int chol_ok;
+  constexpr int LD = KMAX + 1;
   const int s = blockIdx.x;
   const int k = ks[s];
   const int t = threadIdx.x, nt = blockDim.x;
   double* Xs = X + (int64_t)s * slice_stride;
+  double* Z = Z_IN_LDS ? sZ : Zg + (int64_t)s * KMAX * LD;
   if (t == 0) {
     double sh = 0.0;
     if (mode == 1) {
@@ -191,44 +196,61 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     const int i = idx % k, j = idx / k;
     double v = Ain[i + (int64_t)j * lda];
     if (i == j) v += shift;
-    L[i + j * JLD] = v;
+    L[i + j * LD] = v;
+    Z[i + j * LD] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  // right-looking Cholesky, lower triangle
+  // ---- Cholesky, lower triangle, one barrier per column
+  double prev_d = 1.0;  // pivot of the previous column (same in every thread)
   for (int j = 0; j < k; ++j) {
-    if (t == 0) {
-      const double d = L[j + j * JLD];
-      if (!(d > 0.0) || !isfinite(d)) chol_ok = 0;
-      else L[j + j * JLD] = sqrt(d);
+    const double d = L[j + j * LD];
+    if (!(d > 0.0) || !isfinite(d)) {
+      if (t == 0) chol_ok = 0;
+      break;
     }
-    __syncthreads();
-    if (!chol_ok) break;
-    const double djj = L[j + j * JLD];
-    for (int i = j + 1 + t; i < k; i += nt) L[i + j * JLD] /= djj;
-    __syncthreads();
+    if (j > 0) {  // deferred scaling of column j-1
+      const double r = 1.0 / sqrt(prev_d);
+      for (int i = j - 1 + t; i < k; i += nt) L[i + (j - 1) * LD] *= r;
+    }
+    prev_d = d;
     const int m = k - j - 1;
+    const double inv_d = 1.0 / d;
     for (int idx = t; idx < m * m; idx += nt) {
       const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
-      if (ii >= jj) L[ii + jj * JLD] -= L[ii + j * JLD] * L[jj + j * JLD];
+      if (ii >= jj) L[ii + jj * LD] -= L[ii + j * LD] * L[jj + j * LD] * inv_d;
     }
     __syncthreads();
   }
+  __syncthreads();
   if (chol_ok) {
-    // each thread solves one right-hand-side column c: L L^T x = rhs_c
-    for (int c = t; c < k; c += nt) {
-      double* x = Xs + (int64_t)c * ldx;
-      for (int i = 0; i < k; ++i) x[i] = 0.0;
-      x[c] = (mode == 1) ? 1.0 : dvals[c];
-      for (int i = c; i < k; ++i) {  // forward: rhs is zero above row c
-        double v = x[i];
-        for (int q = c; q < i; ++q) v -= L[i + q * JLD] * x[q];
-        x[i] = v / L[i + i * JLD];
+    if (t == 0) L[(k - 1) + (k - 1) * LD] = sqrt(L[(k - 1) + (k - 1) * LD]);
+    __syncthreads();
+    // ---- Z = L^{-1} by a column sweep, one barrier per step
+    for (int q = 0; q < k; ++q) {
+      const double lqq = L[q + q * LD];
+      if (q > 0) {  // deferred scaling of row q-1
+        const double r = 1.0 / L[(q - 1) + (q - 1) * LD];
+        for (int c = t; c < q; c += nt) Z[(q - 1) + c * LD] *= r;
       }
-      for (int i = k - 1; i >= 0; --i) {  // backward with L^T
-        double v = x[i];
-        for (int q = i + 1; q < k; ++q) v -= L[q + i * JLD] * x[q];
-        x[i] = v / L[i + i * JLD];
+      const int rows = k - q - 1, cols = q + 1;
+      for (int idx = t; idx < rows * cols; idx += nt) {
+        const int i = q + 1 + idx % rows, c = idx / rows;
+        Z[i + c * LD] -= L[i + q * LD] * (Z[q + c * LD] / lqq);
       }
+      __syncthreads();
+    }
+    {
+      const double r = 1.0 / L[(k - 1) + (k - 1) * LD];
+      for (int c = t; c < k; c += nt) Z[(k - 1) + c * LD] *= r;
+    }
+    __syncthreads();
+    // ---- X = Z^T Z diag(rhs)
+    for (int idx = t; idx < k * k; idx += nt) {
+      const int i = idx % k, j = idx / k;
+      const int r0 = i > j ? i : j;
+      double acc = 0.0;
+      for (int r = r0; r < k; ++r) acc = fma(Z[r + i * LD], Z[r + j * LD], acc);
+      Xs[i + (int64_t)j * ldx] = (mode == 1) ? acc : acc * dvals[j];
     }
     if (t == 0 && status) status[s] = 0;
     return;
@@ -239,7 +261,7 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     const int i = idx % k, j = idx / k;
     double v = Ain[i + (int64_t)j * lda];
     if (i == j) v += shift;
-    L[i + j * JLD] = v;
+    L[i + j * LD] = v;
   }
   __shared__ int singular;
   if (t == 0) singular = 0;
@@ -247,9 +269,9 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
   for (int j = 0; j < k; ++j) {
     if (t == 0) {
       int pr = j;
-      double best = fabs(L[j + j * JLD]);
+      double best = fabs(L[j + j * LD]);
       for (int i = j + 1; i < k; ++i) {
-        const double v = fabs(L[i + j * JLD]);
+        const double v = fabs(L[i + j * LD]);
         if (v > best) { best = v; pr = i; }
       }
       piv[j] = pr;
@@ -260,19 +282,19 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     const int pr = piv[j];
     if (pr != j) {
       for (int c = t; c < k; c += nt) {
-        const double a = L[j + c * JLD];
-        L[j + c * JLD] = L[pr + c * JLD];
-        L[pr + c * JLD] = a;
+        const double a = L[j + c * LD];
+        L[j + c * LD] = L[pr + c * LD];
+        L[pr + c * LD] = a;
       }
     }
     __syncthreads();
-    const double d = L[j + j * JLD];
-    for (int i = j + 1 + t; i < k; i += nt) L[i + j * JLD] /= d;
+    const double d = L[j + j * LD];
+    for (int i = j + 1 + t; i < k; i += nt) L[i + j * LD] /= d;
     __syncthreads();
     const int m = k - j - 1;
     for (int idx = t; idx < m * m; idx += nt) {
       const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
-      L[ii + jj * JLD] -= L[ii + j * JLD] * L[j + jj * JLD];
+      L[ii + jj * LD] -= L[ii + j * LD] * L[j + jj * LD];
     }
     __syncthreads();
   }
@@ -290,13 +312,13 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     }
     for (int i = 0; i < k; ++i) {
       double v = x[i];
-      for (int q = 0; q < i; ++q) v -= L[i + q * JLD] * x[q];
+      for (int q = 0; q < i; ++q) v -= L[i + q * LD] * x[q];
       x[i] = v;
     }
     for (int i = k - 1; i >= 0; --i) {
       double v = x[i];
-      for (int q = i + 1; q < k; ++q) v -= L[i + q * JLD] * x[q];
-      x[i] = v / L[i + i * JLD];
+      for (int q = i + 1; q < k; ++q) v -= L[i + q * LD] * x[q];
+      x[i] = v / L[i + i * LD];
     }
   }
   if (t == 0 && status) status[s] = 2;  // solved through the LU fallback
@@ -307,8 +329,14 @@ void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const in
                       int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status) {
   FPLS_REQUIRE(kmax >= 1 && kmax <= kJacobiMax, FPLS_ERR_ARG,
                "spd_solve_slices: order must be in 1..128");
-  launch(ctx, "spd_solve", k_spd_solve, dim3(n_slices), dim3(1024), 0, A, lda, d_k, mode, dvals,
-         ridge, X, ldx, slice_stride, lambda_out, status);
+  if (kmax <= 100) {
+    launch(ctx, "spd_solve", k_spd_solve<100, true>, dim3(n_slices), dim3(1024), 0, A, lda, d_k,
+           mode, dvals, ridge, X, ldx, slice_stride, lambda_out, status, (double*)nullptr);
+  } else {
+    DBuf<double> Zg(ctx, (size_t)n_slices * 128 * 129);
+    launch(ctx, "spd_solve", k_spd_solve<128, false>, dim3(n_slices), dim3(1024), 0, A, lda, d_k,
+           mode, dvals, ridge, X, ldx, slice_stride, lambda_out, status, Zg.get());
+  }
 }
 
 // --------------------------------------------------------------- column scaling
