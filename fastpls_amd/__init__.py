@@ -4,7 +4,7 @@ Public API (mirrors the fastPLS R package with backend = "hip"):
 ``pls``, ``predict``, ``fastsvd``, ``pls_single_cv``, ``optim_pls_cv``.
 """
 from ._lib import Context, FplsError, available, context, device_colmajor, load  # noqa: F401
-from .api import FastPLS, fastsvd, pls, predict  # noqa: F401
+from .api import FastPLS, candidate_knn_predict, fastsvd, pls, predict  # noqa: F401
 
 __all__ = ["pls", "predict", "fastsvd", "FastPLS", "Context", "FplsError", "available",
            "context", "device_colmajor", "load"]

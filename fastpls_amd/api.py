@@ -252,3 +252,29 @@ def fastsvd(A, ncomp: int, method: str = "rsvd", rsvd_oversample: int = 10,
     U = U[: rows * kk].reshape((rows, kk), order="F")
     Vt = Vt[: kk * cols].reshape((kk, cols), order="F")
     return {"d": s[:kk], "u": U, "v": Vt.T.copy()}
+
+
+def candidate_knn_predict(Ttest_norm, Ttrain_norm, y_codes, centroids, candidate_bias=None,
+                          top_k=1, top_m=20, knn_k=10, tau=0.2, alpha=0.75, device=0) -> Dict:
+    """candidate_knn_predict_cpp (src/fastPLS.cpp:1627-1648) on the device:
+    array-level candidate kNN on normalised scores."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Te = as_mat(Ttest_norm, keep)
+    Tr = as_mat(Ttrain_norm, keep)
+    Ce = as_mat(centroids, keep)
+    yv = as_ivec(y_codes, keep)
+    C = Ce.rows
+    top = max(1, min(int(top_k), C))
+    w = Te.rows
+    ti = np.empty((w, top), dtype=np.int32, order="F")
+    ts = np.empty((w, top), dtype=np.float64, order="F")
+    bias = None
+    if candidate_bias is not None and len(candidate_bias):
+        bias = np.ascontiguousarray(np.asarray(candidate_bias, dtype=np.float64))
+        keep.append(bias)
+    ctx.check(ctx.lib.fpls_cknn_predict_scores(
+        ctx.h, ctypes.byref(Te), ctypes.byref(Tr), ctypes.byref(yv), ctypes.byref(Ce),
+        bias.ctypes.data if bias is not None else None, top, int(top_m), int(knn_k), float(tau),
+        float(alpha), ti.ctypes.data, ts.ctypes.data))
+    return {"top_index": ti, "top_score": ts}

@@ -181,7 +181,11 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
 //      FP32 MFMA partials per K-slab, reduced in FP64 into C (FP64).
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
-                 const double* offset, float* Cf, double* Cd, int64_t ldc);
+                 const double* offset, float* Cf, double* Cd, int64_t ldc,
+                 bool c_rowmajor = false);
+// C (M x N col-major FP32, ld M) = A B^T, A: M x K and B: N x K both row-major.
+void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const float* B, int64_t ldb, float* C);
 //      labelsB (1-based codes) replaces B by the one-hot matrix onehot(y) (K x N)
 //      generated on the fly: the dense response is never formed.
 void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,

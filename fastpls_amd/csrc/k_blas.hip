@@ -268,7 +268,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_f32(int64_t M, int64_t N, int64
                                                      const float* __restrict__ B, int64_t ldb,
                                                      const double* __restrict__ off,
                                                      float* __restrict__ Cf,
-                                                     double* __restrict__ Cd, int64_t ldc) {
+                                                     double* __restrict__ Cd, int64_t ldc,
+                                                     int crow) {
   constexpr int BM = 64, BK = 16;
   constexpr int NB = BN / 32;  // 16-col blocks per wave
   __shared__ float As[BK][BM];
@@ -330,13 +331,15 @@ __global__ __launch_bounds__(256) void k_gemm_nn_f32(int64_t M, int64_t N, int64
       }
   __syncthreads();
   for (int idx = t; idx < BM * BN; idx += 256) {
-    const int i = idx % BM, j = idx / BM;
+    // column-major: rows fastest; row-major (crow): columns fastest
+    const int i = crow ? idx / BN : idx % BM, j = crow ? idx % BN : idx / BM;
     const int64_t gi = row0 + i, gj = col0 + j;
     if (gi < M && gj < N) {
       double v = Cs[j][i];
       if (off) v -= off[gj];
-      if (Cd) Cd[gi + gj * ldc] = v;
-      else Cf[gi + gj * ldc] = static_cast<float>(v);
+      const int64_t o = crow ? gi * ldc + gj : gi + gj * ldc;
+      if (Cd) Cd[o] = v;
+      else Cf[o] = static_cast<float>(v);
     }
   }
 }
@@ -357,7 +360,8 @@ __global__ __launch_bounds__(256) void k_gemm_nn_bf16x3(int64_t M, int64_t N, in
                                                         int64_t ldb,
                                                         const double* __restrict__ off,
                                                         float* __restrict__ Cf,
-                                                        double* __restrict__ Cd, int64_t ldc) {
+                                                        double* __restrict__ Cd, int64_t ldc,
+                                                        int crow) {
   constexpr int BM = 64, BK = 32;
   constexpr int NB = BN / 32;
   __shared__ float As[BM][BK + 1];   // row-major per tile row: lane reads 8 consecutive k
@@ -435,20 +439,23 @@ __global__ __launch_bounds__(256) void k_gemm_nn_bf16x3(int64_t M, int64_t N, in
       }
   __syncthreads();
   for (int idx = t; idx < BM * BN; idx += 256) {
-    const int i = idx % BM, j = idx / BM;
+    // column-major: rows fastest; row-major (crow): columns fastest
+    const int i = crow ? idx / BN : idx % BM, j = crow ? idx % BN : idx / BM;
     const int64_t gi = row0 + i, gj = col0 + j;
     if (gi < M && gj < N) {
       double v = Cs[j][i];
       if (off) v -= off[gj];
-      if (Cd) Cd[gi + gj * ldc] = v;
-      else Cf[gi + gj * ldc] = static_cast<float>(v);
+      const int64_t o = crow ? gi * ldc + gj : gi + gj * ldc;
+      if (Cd) Cd[o] = v;
+      else Cf[o] = static_cast<float>(v);
     }
   }
 }
 
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
-                 const double* offset, float* Cf, double* Cd, int64_t ldc) {
+                 const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
+  const int crow = c_rowmajor ? 1 : 0;
   if (M <= 0 || N <= 0) return;
   FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
   const bool wide = N > 64;
@@ -457,17 +464,17 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   if (precision == FPLS_PREC_BF16X3) {
     if (wide)
       launch(ctx, "gemm_nn_bf16x3", k_gemm_nn_bf16x3<128>, grid, dim3(256), 0, M, N, K, A, lda,
-             shiftA, B, ldb, offset, Cf, Cd, ldc);
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
     else
       launch(ctx, "gemm_nn_bf16x3", k_gemm_nn_bf16x3<64>, grid, dim3(256), 0, M, N, K, A, lda,
-             shiftA, B, ldb, offset, Cf, Cd, ldc);
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
   } else {
     if (wide)
       launch(ctx, "gemm_nn_f32", k_gemm_nn_f32<128>, grid, dim3(256), 0, M, N, K, A, lda, shiftA,
-             B, ldb, offset, Cf, Cd, ldc);
+             B, ldb, offset, Cf, Cd, ldc, crow);
     else
       launch(ctx, "gemm_nn_f32", k_gemm_nn_f32<64>, grid, dim3(256), 0, M, N, K, A, lda, shiftA,
-             B, ldb, offset, Cf, Cd, ldc);
+             B, ldb, offset, Cf, Cd, ldc, crow);
   }
 }
 
@@ -696,6 +703,21 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
            shiftA, B, ldb, labelsB, shiftB, part.get());
   launch(ctx, "gemm_tn_reduce", k_reduce_partials_f32, dim3(grid1d(M * N)), dim3(256), 0, M, N,
          (int)splits, part.get(), C, ldc);
+}
+
+// C (M x N, column-major FP32, ld M) = A B^T with A (M x K) and B (N x K) both
+// row-major (K contiguous): the TN kernel with a single K slab writes C directly.
+void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const float* B, int64_t ldb, float* C) {
+  if (M <= 0 || N <= 0) return;
+  const int64_t kchunk = ((std::max<int64_t>(K, 1) + 31) / 32) * 32;
+  dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), 1);
+  if (precision == FPLS_PREC_BF16X3)
+    launch(ctx, "gemm_rr_bf16x3", k_gemm_tn_bf16x3, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
+  else
+    launch(ctx, "gemm_rr_f32", k_gemm_tn_f32, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
 }
 
 // ---------------------------------------------------------------- GEMV pair

@@ -19,8 +19,9 @@ struct LdaSlice {
 
 struct CknnSlice {
   int k = 0;
-  DBuf<double> centroids;   // C x k (row-normalised)
-  DBuf<double> cmap;        // k x k score-space map (C_latent slice), or empty
+  DBuf<float> Tn;           // n x k row-major, row-normalised, rows sorted by class
+  DBuf<double> centroids;   // C x k row-major (row-normalised class means)
+  DBuf<float> centroids_f;  // FP32 copy for the MFMA base scores
 };
 
 struct Model {
@@ -56,7 +57,7 @@ struct Model {
   DBuf<int32_t> y_codes;        // n training labels (device)
   DBuf<int32_t> class_offsets;  // C + 1 CSR offsets over rows sorted by class
   DBuf<int32_t> class_rows;     // n row ids grouped by class (ascending inside a class)
-  std::vector<int64_t> class_offsets_host;
+  std::vector<int> class_offsets_host;
   int knn_k = 10, top_m = 20;
   double tau = 0.2, alpha = 0.75;
 

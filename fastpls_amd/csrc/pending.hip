@@ -10,19 +10,6 @@ static int not_built(fpls_ctx* ctx, const char* what) {
 }
 
 extern "C" {
-int fpls_cknn_fit(fpls_ctx* ctx, fpls_model*, const fpls_mat*, const fpls_ivec*, int32_t, int32_t,
-                  double, double, int32_t) {
-  return not_built(ctx, "fpls_cknn_fit");
-}
-int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model*, const fpls_mat*, int32_t, int32_t*,
-                      double*) {
-  return not_built(ctx, "fpls_cknn_predict");
-}
-int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat*, const fpls_mat*, const fpls_ivec*,
-                             const fpls_mat*, const double*, int32_t, int32_t, int32_t, double,
-                             double, int32_t*, double*) {
-  return not_built(ctx, "fpls_cknn_predict_scores");
-}
 int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat*, const fpls_mat*, int32_t, const fpls_opts*,
                     double*, double*, double*, int32_t*) {
   return not_built(ctx, "fpls_rsvd_xprod");
