@@ -4,7 +4,8 @@ Public API (mirrors the fastPLS R package with backend = "hip"):
 ``pls``, ``predict``, ``fastsvd``, ``pls_single_cv``, ``optim_pls_cv``.
 """
 from ._lib import Context, FplsError, available, context, device_colmajor, load  # noqa: F401
-from .api import FastPLS, candidate_knn_predict, fastsvd, pls, predict  # noqa: F401
+from .api import (FastPLS, candidate_knn_predict, class_sums, fastsvd, pls, predict,  # noqa: F401
+                  truncated_rsvd_crossprod, optim_pls_cv, pls_single_cv, pls_cv_predict)
 
-__all__ = ["pls", "predict", "fastsvd", "FastPLS", "Context", "FplsError", "available",
+__all__ = ["pls", "predict", "fastsvd", "optim_pls_cv", "pls_single_cv", "pls_cv_predict", "FastPLS", "Context", "FplsError", "available",
            "context", "device_colmajor", "load"]

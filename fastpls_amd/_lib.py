@@ -104,6 +104,11 @@ SIGNATURES = {
     "fpls_rsvd_xprod": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), ctypes.c_int32, _P(Opts), _vp, _vp,
                                        _vp, _P(ctypes.c_int32)]),
     "fpls_class_sums": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp, _vp]),
+    "fpls_cv_predict": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Opts),
+                                       ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int32, _vp, _vp,
+                                       ctypes.c_int32, _P(ctypes.c_int32), _vp, _vp]),
 }
 
 _lib = None

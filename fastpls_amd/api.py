@@ -254,6 +254,177 @@ def fastsvd(A, ncomp: int, method: str = "rsvd", rsvd_oversample: int = 10,
     return {"d": s[:kk], "u": U, "v": Vt.T.copy()}
 
 
+CLASSIFIERS = {"argmax": 0, "lda": 1, "cknn": 2}
+
+
+def _cv_metric(Ytrue: np.ndarray, Ypred: np.ndarray):
+    """.cv_metric_from_matrix (R/main.R:3699-3709): q2 for one response, rmsd otherwise."""
+    if Ytrue.shape[1] == 1:
+        press = float(((Ypred[:, 0] - Ytrue[:, 0]) ** 2).sum())
+        tss = float(((Ytrue[:, 0] - Ytrue[:, 0].mean()) ** 2).sum())
+        return "q2", (1.0 - press / tss) if tss > 0 else float("nan")
+    return "rmsd", float(np.sqrt(np.mean((Ypred - Ytrue) ** 2)))
+
+
+def _cv_best_index(names, values) -> int:
+    """.cv_best_index (R/main.R:6316-6330): max, or min for loss metrics."""
+    v = np.asarray(values, dtype=float)
+    fin = np.isfinite(v)
+    if not fin.any():
+        return 0
+    if any(nm in ("rmsd", "rmse", "mae", "mse") for nm, f in zip(names, fin) if f):
+        return int(np.argmin(np.where(fin, v, np.inf)))
+    return int(np.argmax(np.where(fin, v, -np.inf)))
+
+
+def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method="simpls",
+                   classifier="argmax", lda_ridge=1e-8, candidate_knn_k=10, candidate_tau=0.2,
+                   candidate_alpha=0.75, candidate_top_m=20, seed=1, rsvd_oversample=10,
+                   rsvd_power=1, precision="fp32", return_scores=False, device=0) -> Dict:
+    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151), leave-one-group-out
+    over the distinct values of ``constrain``."""
+    ctx = _lib.context(device)
+    keep: list = []
+    X = as_mat(Xdata, keep)
+    n = X.rows
+    classification = _is_factor(Ydata)
+    lev = None
+    yv = Ym = None
+    C = 0
+    if classification:
+        lev, codes, C = _encode_factor(Ydata)
+        yv = as_ivec(codes, keep)
+        m = C
+    else:
+        Ym = as_mat(Ydata, keep)
+        m = Ym.cols
+    cons = np.ascontiguousarray(np.asarray(constrain).ravel().astype(np.int32))
+    if cons.size != n:
+        raise ValueError("constrain must have one value for each sample")
+    keep.append(cons)
+    nc_arr, nc_ptr = int_array(np.atleast_1d(ncomp))
+    L = nc_arr.size
+    opts = _lib.default_opts(scaling=SCALINGS[scaling], seed=int(seed) & 0xFFFFFFFF,
+                             oversample=int(rsvd_oversample), power=int(rsvd_power),
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32)
+    ngroups = int(np.unique(cons).size)
+    fold = np.empty(n, dtype=np.int32)
+    status = np.zeros(max(ngroups, 1), dtype=np.int32)
+    nf = ctypes.c_int32()
+    cls = np.zeros((n, L), dtype=np.int32, order="F") if classification else None
+    want_scores = (not classification) or return_scores
+    yp = np.zeros((n, m, L), dtype=np.float64, order="F") if want_scores else None
+    meth = _lib.METHOD_PLSSVD if method == "plssvd" else _lib.METHOD_SIMPLS
+    ctx.check(ctx.lib.fpls_cv_predict(
+        ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
+        ctypes.byref(yv) if yv is not None else None, C, cons.ctypes.data, nc_ptr, L, -1, meth,
+        ctypes.byref(opts), CLASSIFIERS[classifier], float(lda_ridge), int(candidate_knn_k),
+        float(candidate_tau), float(candidate_alpha), int(candidate_top_m), fold.ctypes.data,
+        status.ctypes.data, status.size, ctypes.byref(nf),
+        cls.ctypes.data if cls is not None else None,
+        yp.ctypes.data if yp is not None else None))
+    out = {"fold": fold, "status": status[: nf.value], "ncomp": nc_arr.astype(np.int64),
+           "method": method, "backend": "hip", "classifier": classifier,
+           "classification": classification, "levels": lev}
+    names, values, preds = [], [], []
+    if classification:
+        out["class_pred"] = cls
+        truth = np.asarray(Ydata).ravel()
+        levs = np.asarray(lev, dtype=object)
+        for i in range(L):
+            ok = (cls[:, i] >= 1) & (cls[:, i] <= len(lev))
+            pred = np.full(n, None, dtype=object)
+            pred[ok] = levs[cls[ok, i] - 1]
+            preds.append(pred)
+            names.append("accuracy")
+            values.append(float(np.mean(pred == truth)))
+        if yp is not None:
+            out["Ypred"] = yp
+    else:
+        out["Ypred"] = yp
+        yt = np.asarray(Ydata, dtype=np.float64).reshape(n, -1)
+        for i in range(L):
+            preds.append(yp[:, :, i])
+            nm, v = _cv_metric(yt, yp[:, :, i])
+            names.append(nm)
+            values.append(v)
+    out["pred"] = preds
+    out["metrics"] = {"ncomp_index": list(range(1, L + 1)), "metric_name": names,
+                      "metric_value": values}
+    return out
+
+
+def optim_pls_cv(Xdata, Ydata, ncomp=2, constrain=None, scaling="centering", method="simpls",
+                 backend="hip", seed=1, classifier="argmax", **kw) -> Dict:
+    """optim.pls.cv (R/main.R:6399-6537) with leave-one-group-out folds given by
+    ``constrain`` (default: one group per sample, i.e. LOOCV)."""
+    _check_backend(backend)
+    n = np.asarray(Xdata).shape[0] if not type(Xdata).__module__.startswith("torch") else Xdata.shape[0]
+    if constrain is None:
+        constrain = np.arange(1, n + 1)
+    res = pls_cv_predict(Xdata, Ydata, constrain, ncomp, scaling, method, classifier, seed=seed,
+                         **kw)
+    vals = res["metrics"]["metric_value"]
+    bi = _cv_best_index(res["metrics"]["metric_name"], vals)
+    res["optim_comp"] = int(res["ncomp"][bi])
+    res["best_index"] = bi + 1
+    res["best_metric_name"] = res["metrics"]["metric_name"][bi]
+    res["best_metric_value"] = vals[bi]
+    res["Q2Y"] = vals
+    res["R2Y"] = vals
+    res["Ypred_optim"] = res["pred"][bi]
+    return res
+
+
+def pls_single_cv(Xdata, Ydata, constrain=None, ncomp=2, scaling="centering", method="simpls",
+                  backend="hip", seed=1, classifier="argmax", **kw) -> Dict:
+    """pls.single.cv (R/main.R:4316-4435): cross-validated predictions for the
+    given ncomp values with leave-one-group-out folds."""
+    return optim_pls_cv(Xdata, Ydata, ncomp, constrain, scaling, method, backend, seed,
+                        classifier, **kw)
+
+
+def truncated_rsvd_crossprod(X, Y, k: int, rsvd_oversample: int = 10, rsvd_power: int = 1,
+                             seed: int = 1, exact: bool = False, device: int = 0) -> Dict:
+    """truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623): rSVD of X^T Y
+    without forming it."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    Ym = as_mat(Y, keep)
+    p, m = Xm.cols, Ym.cols
+    kk = int(min(max(int(k), 1), min(p, m)))
+    opts = _lib.default_opts(oversample=int(rsvd_oversample), power=int(rsvd_power),
+                             seed=int(seed) & 0xFFFFFFFF,
+                             svd_method=_lib.SVD_EXACT if exact else _lib.SVD_RSVD)
+    U = np.empty(p * kk)
+    s = np.empty(kk)
+    Vt = np.empty(kk * m)
+    kout = ctypes.c_int32()
+    ctx.check(ctx.lib.fpls_rsvd_xprod(ctx.h, ctypes.byref(Xm), ctypes.byref(Ym), kk,
+                                      ctypes.byref(opts), U.ctypes.data, s.ctypes.data,
+                                      Vt.ctypes.data, ctypes.byref(kout)))
+    r = kout.value
+    return {"U": U[: p * r].reshape((p, r), order="F"), "s": s[:r],
+            "Vt": Vt[: r * m].reshape((r, m), order="F")}
+
+
+def class_sums(X, y, n_classes: int, device: int = 0) -> Dict:
+    """Per-class column sums (p x C), column sums (p) and class counts from
+    1-based int32 labels, streamed from HBM (R/main.R:563-588)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    yv = as_ivec(y, keep)
+    p = Xm.cols
+    S = np.empty((p, n_classes), order="F")
+    cs = np.empty(p)
+    cnt = np.empty(n_classes, dtype=np.int64)
+    ctx.check(ctx.lib.fpls_class_sums(ctx.h, ctypes.byref(Xm), ctypes.byref(yv), int(n_classes),
+                                      S.ctypes.data, cs.ctypes.data, cnt.ctypes.data))
+    return {"class_sums": S, "col_sums": cs, "counts": cnt}
+
+
 def candidate_knn_predict(Ttest_norm, Ttrain_norm, y_codes, centroids, candidate_bias=None,
                           top_k=1, top_m=20, knn_k=10, tau=0.2, alpha=0.75, device=0) -> Dict:
     """candidate_knn_predict_cpp (src/fastPLS.cpp:1627-1648) on the device:

@@ -421,12 +421,6 @@ __global__ void k_rerank(const int32_t* __restrict__ cand, const double* __restr
 }
 
 // ---------------------------------------------------------------- host core
-struct CkClassIndex {
-  std::vector<int> off;      // C + 1 offsets into the sorted rows
-  DBuf<int> d_off;
-  DBuf<int> d_rows;          // sorted position -> original row
-};
-
 void stable_class_sort(Ctx* ctx, const int32_t* y, int64_t n, int C, CkClassIndex& ci) {
   const int B = (int)((n + SORT_CHUNK - 1) / SORT_CHUNK);
   FPLS_REQUIRE(C <= 16384, FPLS_ERR_ARG, "candidate kNN supports up to 16384 classes");
@@ -450,8 +444,7 @@ void stable_class_sort(Ctx* ctx, const int32_t* y, int64_t n, int C, CkClassInde
   for (int c = 0; c < C; ++c) ci.off[c + 1] = ci.off[c] + ht[c];
   FPLS_REQUIRE(ci.off[C] == n, FPLS_ERR_LABEL_RANGE, "labels must be encoded as 1..n_classes");
   ci.d_off = DBuf<int>(ctx, C + 1);
-  FPLS_HIP(hipMemcpyAsync(ci.d_off.get(), ci.off.data(), (C + 1) * sizeof(int),
-                          hipMemcpyHostToDevice, ctx->stream));
+  h2d(ctx, ci.d_off.get(), ci.off.data(), (C + 1) * sizeof(int));
   launch(ctx, "class_sort", k_chunk_base, dim3((C + 255) / 256), dim3(256), 0, counts.get(), B, C,
          ci.d_off.get());
   ci.d_rows = DBuf<int>(ctx, std::max<int64_t>(n, 1));
@@ -473,10 +466,8 @@ void class_centroids(Ctx* ctx, const float* Tn, int k, const CkClassIndex& ci, i
   DBuf<int4> d_items(ctx, std::max<size_t>(items.size(), 1));
   DBuf<int> d_ib(ctx, C + 1);
   if (!items.empty())
-    FPLS_HIP(hipMemcpyAsync(d_items.get(), items.data(), items.size() * sizeof(int4),
-                            hipMemcpyHostToDevice, ctx->stream));
-  FPLS_HIP(hipMemcpyAsync(d_ib.get(), ib.data(), (C + 1) * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
+    h2d(ctx, d_items.get(), items.data(), items.size() * sizeof(int4));
+  h2d(ctx, d_ib.get(), ib.data(), (C + 1) * sizeof(int));
   DBuf<double> part(ctx, std::max<size_t>(items.size(), 1) * k);
   if (!items.empty())
     launch(ctx, "centroid_sums", k_class_partial_sums, dim3((unsigned)items.size()), dim3(256), 0,
@@ -543,8 +534,7 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
       }
   }
   FPLS_REQUIRE(ptotal < (int64_t(1) << 31), FPLS_ERR_ARG, "candidate kNN partial buffer too large");
-  FPLS_HIP(hipMemcpyAsync(cursor.get(), slot0.data(), C * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
+  h2d(ctx, cursor.get(), slot0.data(), C * sizeof(int));
   launch(ctx, "cknn_pairs", k_pair_scatter, dim3(g1(npairs)), dim3(256), 0,
          (const int32_t*)cand.get(), npairs, cursor.get(), bucket.get());
   std::vector<int> slot_class(std::max<int64_t>(npairs, 1), 0);
@@ -555,18 +545,13 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
       d_csize(ctx, C);
   DBuf<CkItem> d_items(ctx, std::max<size_t>(items.size(), 1));
   DBuf<float> part(ctx, std::max<int64_t>(ptotal, 1));
-  FPLS_HIP(hipMemcpyAsync(d_slot_class.get(), slot_class.data(), slot_class.size() * sizeof(int),
-                          hipMemcpyHostToDevice, ctx->stream));
-  FPLS_HIP(hipMemcpyAsync(d_nch.get(), nch.data(), C * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-  FPLS_HIP(hipMemcpyAsync(d_pbase.get(), pbase.data(), C * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
-  FPLS_HIP(hipMemcpyAsync(d_slot0.get(), slot0.data(), C * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
-  FPLS_HIP(hipMemcpyAsync(d_csize.get(), csize.data(), C * sizeof(int), hipMemcpyHostToDevice,
-                          ctx->stream));
+  h2d(ctx, d_slot_class.get(), slot_class.data(), slot_class.size() * sizeof(int));
+  h2d(ctx, d_nch.get(), nch.data(), C * sizeof(int));
+  h2d(ctx, d_pbase.get(), pbase.data(), C * sizeof(int));
+  h2d(ctx, d_slot0.get(), slot0.data(), C * sizeof(int));
+  h2d(ctx, d_csize.get(), csize.data(), C * sizeof(int));
   if (!items.empty()) {
-    FPLS_HIP(hipMemcpyAsync(d_items.get(), items.data(), items.size() * sizeof(CkItem),
-                            hipMemcpyHostToDevice, ctx->stream));
+    h2d(ctx, d_items.get(), items.data(), items.size() * sizeof(CkItem));
     if (k <= 128)
       launch(ctx, "cknn_partial", k_cand_partial<128>, dim3((unsigned)items.size()), dim3(256), 0,
              Ttn, k, Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k,
@@ -590,6 +575,12 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
          (const int32_t*)cand.get(), (const double*)score.get(), w, m, top_k, C, top_index,
          top_score, ld_out);
   sync(ctx);  // host vectors above must outlive the async copies
+}
+
+void rownorm_rows(Ctx* ctx, const float* S, int64_t n, int k, const int* rows, float* Tn) {
+  if (n > 0)
+    launch(ctx, "cknn_rownorm", k_rownorm_permute, dim3((unsigned)((n + 3) / 4)), dim3(256), 0, S, n,
+           k, rows, Tn);
 }
 
 // Score space of a model for k components (R/main.R:700-730): PLSSVD scores are
@@ -628,8 +619,7 @@ void score_space(Ctx* ctx, const Model* M, const DMat& T, int k, float* out) {
   } else {
     std::vector<float> I((size_t)k * k, 0.f);
     for (int j = 0; j < k; ++j) I[j + (size_t)j * k] = 1.f;
-    FPLS_HIP(hipMemcpyAsync(Cf.get(), I.data(), I.size() * sizeof(float), hipMemcpyHostToDevice,
-                            ctx->stream));
+    h2d(ctx, Cf.get(), I.data(), I.size() * sizeof(float));
     sync(ctx);
   }
   // out = T_k C_k in row-major layout
@@ -704,8 +694,7 @@ extern "C" int fpls_cknn_fit(fpls_ctx* ctx, fpls_model* model, const fpls_mat* X
       CkClassIndex cix;
       cix.off = ci.off;
       cix.d_off = DBuf<int>(c, n_classes + 1);
-      FPLS_HIP(hipMemcpyAsync(cix.d_off.get(), ci.off.data(), (n_classes + 1) * sizeof(int),
-                              hipMemcpyHostToDevice, c->stream));
+      h2d(c, cix.d_off.get(), ci.off.data(), (n_classes + 1) * sizeof(int));
       class_centroids(c, sl.Tn.get(), k, cix, n_classes, sl.centroids, sl.centroids_f);
       M->cknn.push_back(std::move(sl));
     }

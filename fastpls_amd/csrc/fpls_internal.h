@@ -147,6 +147,13 @@ DMat stage(Ctx* ctx, const fpls_mat* m, DBuf<unsigned char>& hold, const char* w
 const int32_t* stage_labels(Ctx* ctx, const fpls_ivec* y, int64_t n, DBuf<int32_t>& hold,
                             const char* what);
 
+// Host -> device copy that returns only when the source may be reused (the
+// small bookkeeping arrays copied this way live on the host stack).
+inline void h2d(Ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  FPLS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  FPLS_HIP(hipStreamSynchronize(ctx->stream));
+}
 void to_host(Ctx* ctx, double* dst, const double* src, size_t n);
 void to_host_i32(Ctx* ctx, int32_t* dst, const int32_t* src, size_t n);
 void to_device(Ctx* ctx, double* dst, const double* src, size_t n);

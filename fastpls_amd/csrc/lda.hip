@@ -95,6 +95,13 @@ __global__ void k_add_consts(double* S, int64_t lds, int64_t n, int64_t C, const
   }
 }
 
+void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t C,
+                 const double* cst, int32_t* labels) {
+  if (n > 0)
+    launch(ctx, "lda_argmax", k_argmax_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+           scores, lds, n, C, cst, labels);
+}
+
 // Train prefix-LDA models for every k in ks on scores T (n x >= max ks).
 // G (optional): T^T T already computed (ld ldg).
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
@@ -169,7 +176,7 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
     sl.means = DBuf<double>(ctx, (size_t)C * k);
     copy2d_f64(ctx, sl.means.get(), C, means.get(), C, C, k);
     sl.inv_cov = DBuf<double>(ctx, (size_t)k * k);
-    FPLS_HIP(hipMemcpyAsync(dk.get(), &k, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    h2d(ctx, dk.get(), &k, sizeof(int));
     spd_solve_slices(ctx, pooled.get(), kmax, k, dk.get(), 1, 1, nullptr, ridge, sl.inv_cov.get(),
                      k, (int64_t)k * k, lam.get(), st.get());
     sl.linear = DBuf<double>(ctx, (size_t)C * k);

@@ -93,6 +93,36 @@ void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols
 // Orthonormal basis of the columns of Y (rows x l) by shifted CholeskyQR3, in place.
 void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l);
 
+// Fits on device-resident inputs (plssvd.hip, simpls.hip).
+void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o);
+void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o);
+
+// LDA (lda.hip): prefix models for every k in ks on scores T; G = T^T T optional.
+void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
+                    int32_t C, const std::vector<int>& ks, double ridge, int precision,
+                    DBuf<double>& counts_out, std::vector<LdaSlice>& slices);
+// labels[i] = 1 + argmax_c(scores[i,c] + cst[c]) (first maximum wins)
+void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t C,
+                 const double* cst, int32_t* labels);
+
+// Candidate kNN building blocks (cknn.hip).
+struct CkClassIndex {
+  std::vector<int> off;  // C + 1 offsets into the class-sorted rows
+  DBuf<int> d_off;
+  DBuf<int> d_rows;      // sorted position -> original row
+};
+void stable_class_sort(Ctx* ctx, const int32_t* y, int64_t n, int C, CkClassIndex& ci);
+void class_centroids(Ctx* ctx, const float* Tn, int k, const CkClassIndex& ci, int C,
+                     DBuf<double>& cen, DBuf<float>& cen_f);
+void score_space(Ctx* ctx, const Model* M, const DMat& T, int k, float* out);
+void rownorm_rows(Ctx* ctx, const float* S, int64_t n, int k, const int* rows, float* Tn);
+void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int k,
+                       const float* Tn, const std::vector<int>& off, const float* cen_f, int C,
+                       const double* bias, int top, int top_m, int knn_k, double tau,
+                       double alpha, int32_t* top_index, double* top_score, int64_t ld_out);
+
 }  // namespace fpls
 
 struct fpls_model {

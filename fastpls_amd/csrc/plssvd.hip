@@ -297,37 +297,16 @@ static std::vector<int> read_ncomp(const int32_t* ncomp, int32_t n_ncomp) {
   return std::vector<int>(ncomp, ncomp + n_ncomp);
 }
 
-extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym,
-                               const fpls_ivec* yv, int32_t n_classes, const int32_t* ncomp,
-                               int32_t n_ncomp, const fpls_opts* opts, fpls_model** out) {
-  if (out) *out = nullptr;
-  std::unique_ptr<fpls_model> mdl(new fpls_model());
-  int rc = fpls_guard(ctx, [&] {
-    Ctx* c = &ctx->c;
-    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
-    fpls_opts o;
-    fpls_opts_default(&o);
-    if (opts) o = *opts;
-    Model* M = &mdl->m;
+namespace fpls {
+
+// pls_model1 on device-resident inputs (dense Y or labels with m classes).
+void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o) {
+  {
     M->ctx = c;
     M->method = FPLS_METHOD_PLSSVD;
     M->precision = o.precision;
     M->scaling = o.scaling;
-    DBuf<unsigned char> hx, hy;
-    DBuf<int32_t> hl;
-    DMat X = stage(c, Xm, hx, "plssvd: X");
-    DMat Y;
-    const int32_t* y = nullptr;
-    int64_t m = 0;
-    if (yv && yv->ptr) {
-      FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "n_classes must be >= 1 with labels");
-      y = stage_labels(c, yv, X.rows, hl, "plssvd: y");
-      m = n_classes;
-    } else {
-      Y = stage(c, Ym, hy, "plssvd: Y");
-      FPLS_REQUIRE(Y.rows == X.rows, FPLS_ERR_ARG, "X and Y must have the same rows");
-      m = Y.cols;
-    }
     const int64_t n = X.rows, p = X.cols;
     FPLS_REQUIRE(n >= 1 && p >= 1 && m >= 1, FPLS_ERR_ARG, "empty input");
     M->n = n;
@@ -365,7 +344,7 @@ extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
     for (int& v : nc) v = std::min(v, kmax);
     M->ncomp = nc;
     DBuf<int> dk(c, L);
-    FPLS_HIP(hipMemcpyAsync(dk.get(), nc.data(), L * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    h2d(c, dk.get(), nc.data(), L * sizeof(int));
     M->C_latent = DBuf<double>(c, (size_t)kmax * kmax * L);
     M->W_latent = DBuf<double>(c, (size_t)kmax * m * L);
     fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
@@ -410,6 +389,38 @@ extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       FPLS_REQUIRE(hst[a] != 1, FPLS_ERR_SOLVE, "plssvd latent solve failed");
     M->n_components = kmax;
     sync(c);
+  }
+}
+
+}  // namespace fpls
+
+extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym,
+                               const fpls_ivec* yv, int32_t n_classes, const int32_t* ncomp,
+                               int32_t n_ncomp, const fpls_opts* opts, fpls_model** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<fpls_model> mdl(new fpls_model());
+  int rc = fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hx, hy;
+    DBuf<int32_t> hl;
+    DMat X = stage(c, Xm, hx, "plssvd: X");
+    DMat Y;
+    const int32_t* y = nullptr;
+    int64_t m = 0;
+    if (yv && yv->ptr) {
+      FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "n_classes must be >= 1 with labels");
+      y = stage_labels(c, yv, X.rows, hl, "plssvd: y");
+      m = n_classes;
+    } else {
+      Y = stage(c, Ym, hy, "plssvd: Y");
+      FPLS_REQUIRE(Y.rows == X.rows, FPLS_ERR_ARG, "X and Y must have the same rows");
+      m = Y.cols;
+    }
+    plssvd_fit_dev(c, &mdl->m, X, Y, y, m, ncomp, n_ncomp, o);
   });
   if (rc == FPLS_OK) *out = mdl.release();
   return rc;

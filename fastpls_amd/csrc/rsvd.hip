@@ -211,3 +211,130 @@ extern "C" int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* Am, int32_t k, const fpl
     if (Vt && r.Vt.get()) to_host(c, Vt, r.Vt.get(), (size_t)r.k * A.cols);
   });
 }
+
+namespace fpls {
+
+// out (rows x l, FP64) = A M, A the data matrix (FP32 on MFMA or FP64), M cols x l
+static void data_times(Ctx* ctx, const DMat& A, const double* M, int l, double* out) {
+  if (A.dtype == FPLS_F32) {
+    DBuf<float> Mf(ctx, (size_t)A.cols * l);
+    f64_to_f32(ctx, Mf.get(), M, A.cols * (int64_t)l);
+    gemm_nn_f32(ctx, FPLS_PREC_FP32, A.rows, l, A.cols, A.f(), A.ld, nullptr, Mf.get(), A.cols,
+                nullptr, nullptr, out, A.rows);
+  } else {
+    gemm_f64(ctx, false, false, A.rows, l, A.cols, 1.0, A.d(), A.ld, nullptr, M, A.cols, nullptr,
+             0.0, out, A.rows);
+  }
+}
+
+// out (cols x l, FP64) = A^T N, N rows x l
+static void data_t_times(Ctx* ctx, const DMat& A, const double* N, int l, double* out) {
+  if (A.dtype == FPLS_F32) {
+    DBuf<float> Nf(ctx, (size_t)A.rows * l);
+    f64_to_f32(ctx, Nf.get(), N, A.rows * (int64_t)l);
+    gemm_tn_f32(ctx, FPLS_PREC_FP32, A.cols, l, A.rows, A.f(), A.ld, nullptr, Nf.get(), A.rows,
+                nullptr, nullptr, out, A.cols);
+  } else {
+    gemm_f64(ctx, true, false, A.cols, l, A.rows, 1.0, A.d(), A.ld, nullptr, N, A.rows, nullptr,
+             0.0, out, A.cols);
+  }
+}
+
+// S = X^T Y materialised in FP64 (p x m)
+static void cross(Ctx* ctx, const DMat& X, const DMat& Y, double* S) {
+  const int64_t p = X.cols, m = Y.cols;
+  DBuf<double> Yd(ctx, (size_t)Y.rows * m);
+  if (Y.dtype == FPLS_F64) copy2d_f64(ctx, Yd.get(), Y.rows, Y.d(), Y.ld, Y.rows, m);
+  else
+    for (int64_t j = 0; j < m; ++j) f32_to_f64(ctx, Yd.get() + j * Y.rows, Y.f() + j * Y.ld, Y.rows);
+  data_t_times(ctx, X, Yd.get(), (int)m, S);
+  (void)p;
+}
+
+}  // namespace fpls
+
+// truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623): rSVD of
+// S = X^T Y through a_times(M) = X^T (Y M) and at_times(M) = Y^T (X M).
+extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym, int32_t k,
+                               const fpls_opts* opts, double* U, double* s, double* Vt,
+                               int32_t* k_out) {
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hx, hy;
+    DMat X = stage(c, Xm, hx, "rsvd_xprod: X");
+    DMat Y = stage(c, Ym, hy, "rsvd_xprod: Y");
+    FPLS_REQUIRE(X.rows == Y.rows, FPLS_ERR_ARG, "rsvd_xprod: X and Y need the same rows");
+    const int64_t p = X.cols, m = Y.cols;
+    const int64_t max_rank = std::min(p, m);
+    FPLS_REQUIRE(max_rank >= 1, FPLS_ERR_RANK, "rsvd_xprod: empty cross-product");
+    const int target = (int)std::min<int64_t>(max_rank, std::max<int>(k, 1));
+    const int l = (int)std::min<int64_t>(max_rank, (int64_t)target + std::max(o.oversample, 0));
+    SvdOut r;
+    if (o.svd_method == FPLS_SVD_EXACT || l >= max_rank) {
+      DBuf<double> S(c, (size_t)p * m);
+      cross(c, X, Y, S.get());
+      rsvd_dev(c, S.get(), p, p, m, target, o.oversample, o.power, o.seed, nullptr, Vt == nullptr,
+               true, r);
+    } else {
+      FPLS_REQUIRE(l <= kJacobiMax, FPLS_ERR_ARG, "rSVD sketch width k + oversample must be <= 128");
+      const int64_t n = X.rows;
+      // Y and X operands in FP64 scratch for the small products
+      DBuf<double> Yd(c, (size_t)n * m);
+      if (Y.dtype == FPLS_F64) copy2d_f64(c, Yd.get(), n, Y.d(), Y.ld, n, m);
+      else
+        for (int64_t j = 0; j < m; ++j) f32_to_f64(c, Yd.get() + j * n, Y.f() + j * Y.ld, n);
+      DMat Yv;
+      Yv.p = Yd.get();
+      Yv.dtype = FPLS_F64;
+      Yv.rows = n;
+      Yv.cols = m;
+      Yv.ld = n;
+      std::vector<double> om((size_t)m * l);
+      gaussian_host(om.data(), (int64_t)om.size(), o.seed);
+      DBuf<double> Om(c, om.size()), Ys(c, (size_t)p * l), Z(c, (size_t)m * l), Nn(c, (size_t)n * l);
+      to_device(c, Om.get(), om.data(), om.size());
+      auto a_times = [&](const double* M, double* out) {  // X^T (Y M)
+        data_times(c, Yv, M, l, Nn.get());
+        data_t_times(c, X, Nn.get(), l, out);
+      };
+      auto at_times = [&](const double* M, double* out) {  // Y^T (X M)
+        data_times(c, X, M, l, Nn.get());
+        data_t_times(c, Yv, Nn.get(), l, out);
+      };
+      a_times(Om.get(), Ys.get());
+      const int q = std::max(o.power, 0);
+      if (q == 1) {
+        at_times(Ys.get(), Z.get());
+        a_times(Z.get(), Ys.get());
+      } else {
+        for (int it = 0; it < q; ++it) {
+          at_times(Ys.get(), Z.get());
+          orth_cholqr(c, Z.get(), m, l);
+          a_times(Z.get(), Ys.get());
+        }
+      }
+      orth_cholqr(c, Ys.get(), p, l);  // Q
+      // B = (X Q)^T Y  (l x m)
+      data_times(c, X, Ys.get(), l, Nn.get());
+      DBuf<double> Bm(c, (size_t)l * m);
+      gemm_f64(c, true, false, l, m, n, 1.0, Nn.get(), n, nullptr, Yd.get(), n, nullptr, 0.0,
+               Bm.get(), l);
+      // finalize_rsvd_from_q_b_double: svd_econ(B), U = Q Uhat
+      SvdOut sb;
+      svd_exact_dev(c, Bm.get(), l, l, m, target, sb, Vt == nullptr);
+      r.k = sb.k;
+      r.s = std::move(sb.s);
+      r.U = DBuf<double>(c, (size_t)p * sb.k);
+      gemm_f64(c, false, false, p, sb.k, l, 1.0, Ys.get(), p, nullptr, sb.U.get(), l, nullptr,
+               0.0, r.U.get(), p);
+      if (Vt) r.Vt = std::move(sb.Vt);
+    }
+    if (k_out) *k_out = r.k;
+    if (U) to_host(c, U, r.U.get(), (size_t)p * r.k);
+    if (s) to_host(c, s, r.s.get(), r.k);
+    if (Vt && r.Vt.get()) to_host(c, Vt, r.Vt.get(), (size_t)r.k * m);
+  });
+}

@@ -146,7 +146,7 @@ void to_host_i32(Ctx* ctx, int32_t* dst, const int32_t* src, size_t n) {
 
 void to_device(Ctx* ctx, double* dst, const double* src, size_t n) {
   if (!n) return;
-  FPLS_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  h2d(ctx, dst, src, n * sizeof(double));
 }
 
 void sync(Ctx* ctx) { FPLS_HIP(hipStreamSynchronize(ctx->stream)); }

@@ -76,38 +76,17 @@ static void norm2(Ctx* ctx, const double* x, int64_t n, double* out) {
 
 using namespace fpls;
 
-extern "C" int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym,
-                               const fpls_ivec* yv, int32_t n_classes, const int32_t* ncomp,
-                               int32_t n_ncomp, const fpls_opts* opts, fpls_model** out) {
-  if (out) *out = nullptr;
-  std::unique_ptr<fpls_model> mdl(new fpls_model());
-  int rc = fpls_guard(ctx, [&] {
-    Ctx* c = &ctx->c;
-    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
-    fpls_opts o;
-    fpls_opts_default(&o);
-    if (opts) o = *opts;
-    Model* M = &mdl->m;
+namespace fpls {
+
+// pls_model2_fast on device-resident inputs (dense Y or labels with m classes).
+void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o) {
+  {
     M->ctx = c;
     M->method = FPLS_METHOD_SIMPLS;
     M->precision = o.precision;
     M->scaling = o.scaling;
-    DBuf<unsigned char> hx, hy;
-    DBuf<int32_t> hl;
-    DMat X = stage(c, Xm, hx, "simpls: X");
-    DMat Y;
-    const int32_t* y = nullptr;
-    int64_t m = 0;
-    if (yv && yv->ptr) {
-      FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "n_classes must be >= 1 with labels");
-      y = stage_labels(c, yv, X.rows, hl, "simpls: y");
-      m = n_classes;
-      M->n_classes = n_classes;
-    } else {
-      Y = stage(c, Ym, hy, "simpls: Y");
-      FPLS_REQUIRE(Y.rows == X.rows, FPLS_ERR_ARG, "X and Y must have the same rows");
-      m = Y.cols;
-    }
+    if (y) M->n_classes = (int)m;
     const int64_t n = X.rows, p = X.cols;
     FPLS_REQUIRE(n >= 1 && p >= 1 && m >= 1, FPLS_ERR_ARG, "empty input");
     M->n = n;
@@ -285,6 +264,38 @@ extern "C" int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
     M->n_components = a;
     build_projection_cache(c, M);
     sync(c);
+  }
+}
+
+}  // namespace fpls
+
+extern "C" int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym,
+                               const fpls_ivec* yv, int32_t n_classes, const int32_t* ncomp,
+                               int32_t n_ncomp, const fpls_opts* opts, fpls_model** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<fpls_model> mdl(new fpls_model());
+  int rc = fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hx, hy;
+    DBuf<int32_t> hl;
+    DMat X = stage(c, Xm, hx, "simpls: X");
+    DMat Y;
+    const int32_t* y = nullptr;
+    int64_t m = 0;
+    if (yv && yv->ptr) {
+      FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "n_classes must be >= 1 with labels");
+      y = stage_labels(c, yv, X.rows, hl, "simpls: y");
+      m = n_classes;
+    } else {
+      Y = stage(c, Ym, hy, "simpls: Y");
+      FPLS_REQUIRE(Y.rows == X.rows, FPLS_ERR_ARG, "X and Y must have the same rows");
+      m = Y.cols;
+    }
+    simpls_fit_dev(c, &mdl->m, X, Y, y, m, ncomp, n_ncomp, o);
   });
   if (rc == FPLS_OK) *out = mdl.release();
   return rc;

@@ -226,6 +226,25 @@ int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* A, int32_t k, const fpls_opts* opts
 int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, int32_t k,
                     const fpls_opts* opts, double* U, double* s, double* Vt, int32_t* k_out);
 
+/* ---- cross-validation ----------------------------------------------------------- */
+/* pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out
+   folds: the distinct values of constrain (n, host) are the folds (kfold = -1,
+   or kfold >= number of groups).  Randomised fold assignment draws from R's
+   RNG and is rejected (FPLS_ERR_ARG).  method: FPLS_METHOD_PLSSVD or
+   FPLS_METHOD_SIMPLS; classifier: 0 argmax, 1 LDA, 2 candidate kNN (labels
+   required).  Each fold fits with seed + fold.  Outputs: fold_out (n, 1-based),
+   status_out (>= number of folds: 1 ok, 2 empty, 3 no training rows,
+   4 degenerate classification fold), *n_folds_out, class_pred (n x n_ncomp,
+   classification) and ypred (n x m x n_ncomp, regression or argmax scores;
+   may be NULL for classification). */
+int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
+                    int32_t n_classes, const int32_t* constrain, const int32_t* ncomp,
+                    int32_t n_ncomp, int32_t kfold, int32_t method, const fpls_opts* opts,
+                    int32_t classifier, double lda_ridge, int32_t knn_k, double tau,
+                    double alpha, int32_t top_m, int32_t* fold_out, int32_t* status_out,
+                    int32_t status_cap, int32_t* n_folds_out, int32_t* class_pred,
+                    double* ypred);
+
 /* ---- streaming primitive ------------------------------------------------------ */
 /* Class sums (p x C, FP64) and column sums (p) of X in one pass from int32
    labels; counts (C). */

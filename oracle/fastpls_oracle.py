@@ -838,6 +838,86 @@ def candidate_knn_predictions(model: Dict, Xtest, top: int = 1):
 # end-to-end pls() flows used by the tests and the CPU baseline (R/main.R:5860-6314)
 # ---------------------------------------------------------------------------
 
+def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False,
+                   n_classes=None, classifier="argmax", scaling=1, seed=1, lda_ridge=1e-8,
+                   knn_k=10, tau=0.2, alpha=0.75, top_m=20):
+    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out
+    folds (group_fold(j) = j over the sorted distinct constrain values, :6720-6736).
+    Y holds 1-based labels when ``classification``.  Returns fold (1-based),
+    status, class_pred (n x L) and Ypred (n x m x L, argmax / regression)."""
+    X = np.asarray(X, dtype=np.float64)
+    n = X.shape[0]
+    cons = np.asarray(constrain).ravel()
+    groups = np.unique(cons)
+    fold = np.searchsorted(groups, cons)
+    L = len(np.atleast_1d(ncomp))
+    nc = np.array(np.atleast_1d(ncomp), dtype=np.int64)
+    if classification:
+        y = np.asarray(Y, dtype=np.int64).ravel()
+        C = int(n_classes or y.max())
+        m = C
+        Yd = np.zeros((n, C))
+        Yd[np.arange(n), y - 1] = 1.0
+    else:
+        Yd = np.asarray(Y, dtype=np.float64).reshape(n, -1)
+        m = Yd.shape[1]
+    if method == "plssvd":
+        nc = np.minimum(np.maximum(nc, 1), min(n, X.shape[1], m))
+    class_pred = np.zeros((n, L), dtype=np.int32)
+    Ypred = np.zeros((n, m, L))
+    status = np.zeros(len(groups), dtype=np.int32)
+    for f in range(len(groups)):
+        te = np.nonzero(fold == f)[0]
+        tr = np.nonzero(fold != f)[0]
+        if te.size == 0:
+            status[f] = 2
+            continue
+        if tr.size == 0:
+            status[f] = 3
+            Ypred[te] = Yd[te][:, :, None]
+            if classification:
+                class_pred[te] = y[te][:, None]
+            continue
+        if classification:
+            active = np.unique(y[tr])
+            if active.size <= 1:
+                status[f] = 4
+                row = np.zeros(m)
+                if active.size == 1:
+                    row[active[0] - 1] = 1.0
+                Ypred[te] = row[None, :, None]
+                class_pred[te] = active[0] if active.size == 1 else 1
+                continue
+        fit = pls_model1 if method == "plssvd" else pls_model2_fast
+        mod = fit(X[tr], Yd[tr], nc, scaling, False, "rsvd", 10, 1, seed + f)
+        if classification and classifier in ("lda", "cknn"):
+            attach_latent_projection_cache(mod)
+            kmax = min(int(nc.max()), mod["R"].shape[1])
+            Ttr = latent_scores(mod, X[tr], kmax)
+            Tte = latent_scores(mod, X[te], kmax)
+            if classifier == "lda":
+                ks = [min(int(k), kmax) for k in nc]
+                lda = lda_train_prefix(Ttr, y[tr], C, ks, lda_ridge)
+                for s, k in enumerate(ks):
+                    class_pred[te, s] = lda_labels_from_scores(Tte[:, :k] @ lda[s]["linear"].T,
+                                                               lda[s]["constants"])
+            else:
+                for s, k in enumerate(nc):
+                    k = min(int(k), kmax)
+                    a = candidate_row_l2(candidate_score_space(mod, Ttr, k))
+                    b = candidate_row_l2(candidate_score_space(mod, Tte, k))
+                    cen = candidate_centroids(a, y[tr], C)
+                    r = candidate_knn_predict(b, a, y[tr], cen, None, 1, top_m, knn_k, tau, alpha)
+                    class_pred[te, s] = r["top_index"][:, 0]
+        else:
+            yp = pls_predict(mod, X[te])
+            Ypred[te] = yp
+            if classification:
+                class_pred[te] = (np.argmax(yp, axis=1) + 1).astype(np.int32)
+        status[f] = 1
+    return {"fold": fold + 1, "status": status, "class_pred": class_pred, "Ypred": Ypred}
+
+
 def pls_plssvd_lda(Xtrain, ytrain, Xtest, ncomp, lda_ridge=1e-8, scaling=1, seed=1,
                    oversample=10, power=1):
     """pls(method="plssvd", svd.method="rsvd", classifier="lda") then predict:

@@ -113,3 +113,35 @@ def test_plssvd_label_aware_equals_dense_onehot(ctx):
     assert rel_err(m_lab["B"], m_dense["B"]) < 1e-11
     ref = O.pls_model1(X, O.transformy(y), [4], 1, False, "rsvd", 10, 1, 1)
     assert rel_err(m_lab["B"], ref["B"]) < 1e-9
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("power", [0, 1, 2])
+def test_rsvd_crossprod_matches_oracle(ctx, dtype, power):
+    import fastpls_amd as F
+    rng = np.random.default_rng(40 + power)
+    n, p, m = 400, 150, 60
+    X = _lowrank(rng, n, p, 20, 0.05)
+    Y = X @ rng.standard_normal((p, m)) * 0.05 + 0.01 * rng.standard_normal((n, m))
+    Xd = X.astype(dtype)
+    Yd = Y.astype(dtype)
+    got = F.truncated_rsvd_crossprod(Xd, Yd, 8, rsvd_oversample=6, rsvd_power=power, seed=4)
+    ref = O.truncated_rsvd_crossprod(Xd.astype(np.float64), Yd.astype(np.float64), 8, 6, power, 4)
+    tol = 1e-9 if dtype == np.float64 else 2e-4
+    np.testing.assert_allclose(got["s"], ref["s"], rtol=tol)
+    assert sin_theta(got["U"][:, :6], ref["U"][:, :6]) < (1e-7 if dtype == np.float64 else 1e-3)
+
+
+def test_class_sums_exact_counts(ctx):
+    import fastpls_amd as F
+    rng = np.random.default_rng(50)
+    n, p, C = 20000, 33, 17
+    X = rng.standard_normal((n, p)).astype(np.float32)
+    y = rng.integers(1, C + 1, n).astype(np.int32)
+    r = F.class_sums(X, y, C)
+    np.testing.assert_array_equal(r["counts"], np.bincount(y - 1, minlength=C))
+    ref = O.class_sums(X.astype(np.float64), y, C)
+    np.testing.assert_allclose(r["class_sums"], ref, rtol=1e-5, atol=1e-3)
+    np.testing.assert_allclose(r["col_sums"], X.astype(np.float64).sum(0), rtol=1e-9, atol=1e-6)
+    with pytest.raises(F.FplsError, match="LABEL_RANGE"):
+        F.class_sums(X, np.where(y == 3, C + 1, y).astype(np.int32), C)
