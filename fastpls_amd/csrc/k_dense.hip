@@ -75,11 +75,17 @@ __global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain
         sn[t] = s;
       }
       __syncthreads();
-      for (int idx = t; idx < h * h; idx += nt) {
-        const int k = idx % h, l = idx / h;
-        const double ck = cs[k], sk = sn[k], cl = cs[l], sl = sn[l];
+      // thread -> (row pair k fixed, column pairs l strided): the row pair's
+      // rotation stays in registers for the whole round
+      const int lg_count = nt / h;
+      const int kq = t % h, lg = t / h;
+      const double ck = cs[kq], sk = sn[kq];
+      const int pk = pp[kq], qk = qq[kq];
+      for (int l = lg; lg < lg_count && l < h; l += lg_count) {
+        const int k = kq;
+        const double cl = cs[l], sl = sn[l];
         if (sk == 0.0 && sl == 0.0) continue;
-        const int pk = pp[k], qk = qq[k], pl = pp[l], ql = qq[l];
+        const int pl = pp[l], ql = qq[l];
         const double a = A[pk + pl * lda_s], b = A[pk + ql * lda_s];
         const double c_ = A[qk + pl * lda_s], d = A[qk + ql * lda_s];
         const double m00 = ck * a - sk * c_, m01 = ck * b - sk * d;
@@ -95,14 +101,18 @@ __global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain
         A[qk + pl * lda_s] = n10;
         A[qk + ql * lda_s] = n11;
       }
-      for (int idx = t; idx < N * h; idx += nt) {
-        const int i = idx % N, l = idx / N;
+      {
+        // thread -> (column pair l fixed, rows strided)
+        const int l = t % h, ig = t / h;
         const double cl = cs[l], sl = sn[l];
-        if (sl == 0.0) continue;
         const int pl = pp[l], ql = qq[l];
-        const double vip = V[i + pl * N], viq = V[i + ql * N];
-        V[i + pl * N] = cl * vip - sl * viq;
-        V[i + ql * N] = sl * vip + cl * viq;
+        if (sl != 0.0 && ig < lg_count) {
+          for (int i = ig; i < N; i += lg_count) {
+            const double vip = V[i + pl * N], viq = V[i + ql * N];
+            V[i + pl * N] = cl * vip - sl * viq;
+            V[i + ql * N] = sl * vip + cl * viq;
+          }
+        }
       }
       __syncthreads();
     }

@@ -101,6 +101,42 @@ def test_plssvd_lda_fp32_labels(ctx):
     assert not unexcused, f"label mismatches beyond ties: {unexcused}"
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_precision_modes_subspace_error(ctx, precision):
+    """Both contraction modes keep the latent subspace within sin(theta) <= 1e-4
+    of the FP64 oracle and labels equal up to near-ties."""
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    Xtr, ytr, Xte, yte = synth.classes_on_sphere(n_classes=20, per_class_train=200,
+                                                 per_class_test=30, p=256, seed=17)
+    k = 19
+    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=k, classifier="lda", precision=precision)
+    ref = O.pls_plssvd_lda(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [k])
+    assert sin_theta(m["R"], ref["R"]) < 1e-4
+    assert rel_err(m["B"], ref["B"]) < (1e-4 if precision == "fp32" else 5e-4)
+    lda_ref = ref["lda"][k]
+    _, sc = O.lda_project_predict_labels(Xte.astype(np.float64), ref["R_predict"][:, :k],
+                                         ref["R_offset"][:k], lda_ref, return_scores=True)
+    bad, unexcused = label_mismatches_excused(m["Ypred_index"][:, 0], ref["Ypred_labels"][:, 0],
+                                              sc, 1e-3 * float(np.abs(sc).max()))
+    assert not unexcused
+
+
+def test_bf16x3_gemm_accuracy(ctx):
+    """Projection X R through the split-BF16x3 MFMA path: relative error well
+    below 1e-4 (three products of 8-bit-mantissa halves, FP32 accumulation)."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(18)
+    X = rng.standard_normal((3000, 300)).astype(np.float32)
+    y = rng.standard_normal((3000, 5)).astype(np.float32)
+    m32 = F.pls(X, y, method="plssvd", ncomp=5, precision="fp32")
+    m16 = F.pls(X, y, method="plssvd", ncomp=5, precision="bf16x3")
+    ref = O.pls_model1(X.astype(np.float64), y.astype(np.float64), [5])
+    e32 = rel_err(m32["B"], ref["B"])
+    e16 = rel_err(m16["B"], ref["B"])
+    assert e32 < 1e-5 and e16 < 1e-4, (e32, e16)
+
+
 def test_plssvd_label_aware_equals_dense_onehot(ctx):
     """Label-aware S == Xc^T onehot(y) (dense); one-hot never formed."""
     import fastpls_amd as F
