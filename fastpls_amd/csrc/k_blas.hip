@@ -2,17 +2,10 @@
 //
 //  * gemm_f64     -- LDS-tiled FP64 GEMM (64x64 tiles, split-K for long K)
 //                    for the small/medium FP64 products of the factorisations.
-//  * gemm_nn_f32  -- X (n x p, FP32) times a skinny FP64 factor on MFMA
-//                    (v_mfma_f32_16x16x4f32), centring applied on load and an
-//                    offset epilogue: T = (X - 1 mX^T) U - 1 o^T.
-//  * gemm_tn_f32  -- cross products reduced over the long row dimension
-//                    (X^T Y, T^T T) on MFMA with FP32 K-slab partials reduced
-//                    in FP64 in a fixed order (deterministic).
+//  (the FP32-data MFMA contractions live in k_gemm2.hip)
 //  * gemv_n/_t    -- the SIMPLS matrix-vector pair t = Xc r, p = Xc^T t.
 //
-// Layouts are column-major throughout.  MFMA 16x16x4 f32 lane layout
-// (CDNA3/CDNA4): lane l supplies A[l%16][l/16] and B[l/16][l%16]; the four
-// accumulator registers hold D[4*(l/16)+r][l%16], r = 0..3.
+// Layouts are column-major throughout.
 #include <algorithm>
 #include <cmath>
 
@@ -20,9 +13,6 @@
 
 namespace fpls {
 
-typedef float v4f __attribute__((ext_vector_type(4)));
-typedef short v8s __attribute__((ext_vector_type(8)));
-typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------- elementwise
 __global__ void k_fill_f64(double* x, double v, int64_t n) {
@@ -251,473 +241,6 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
            N, (int)splits, part.get(), alpha, beta, C, ldc);
   }
-}
-
-// ---------------------------------------------------------------- MFMA helpers
-__device__ inline v4f mfma4(float a, float b, v4f c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-
-// ---------------------------------------------------------------- NN on MFMA
-// C(M x N) = (A - 1 shA^T) B - 1 off^T.  BM = 64, BK = 16, BN in {64, 128}.
-// 4 waves: wave w covers rows [32*(w%2), +32) and cols [(BN/2)*(w/2), +BN/2).
-template <int BN>
-__global__ __launch_bounds__(256) void k_gemm_nn_f32(int64_t M, int64_t N, int64_t K,
-                                                     const float* __restrict__ A, int64_t lda,
-                                                     const double* __restrict__ shA,
-                                                     const float* __restrict__ B, int64_t ldb,
-                                                     const double* __restrict__ off,
-                                                     float* __restrict__ Cf,
-                                                     double* __restrict__ Cd, int64_t ldc,
-                                                     int crow) {
-  constexpr int BM = 64, BK = 16;
-  constexpr int NB = BN / 32;  // 16-col blocks per wave
-  __shared__ float As[BK][BM];
-  __shared__ float Bs[BK][BN];
-  __shared__ float Cs[BN][BM + 1];
-  __shared__ float shs[BK];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  const int64_t row0 = (int64_t)blockIdx.x * BM;
-  const int64_t col0 = (int64_t)blockIdx.y * BN;
-  v4f acc[2][NB];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    if (t < BK) shs[t] = (shA && k0 + t < K) ? static_cast<float>(shA[k0 + t]) : 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = t % 64, k = t / 64 + 4 * r;
-      const int64_t gi = row0 + i, gk = k0 + k;
-      float v = 0.f;
-      if (gi < M && gk < K) v = A[gi + gk * lda] - shs[k];
-      As[k][i] = v;
-    }
-#pragma unroll
-    for (int r = 0; r < BN / 16; ++r) {
-      const int k = t % 16, j = t / 16 + 16 * r;
-      const int64_t gj = col0 + j, gk = k0 + k;
-      Bs[k][j] = (gj < N && gk < K) ? B[gk + gj * ldb] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      float af[2], bf[NB];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[kk + lane / 16][wm * 32 + a * 16 + lane % 16];
-#pragma unroll
-      for (int b = 0; b < NB; ++b) bf[b] = Bs[kk + lane / 16][wn * (BN / 2) + b * 16 + lane % 16];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = mfma4(af[a], bf[b], acc[a][b]);
-    }
-    __syncthreads();
-  }
-  // stage the tile in LDS (column-major) and write coalesced columns
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wm * 32 + a * 16 + 4 * (lane / 16) + r;
-        const int j = wn * (BN / 2) + b * 16 + lane % 16;
-        Cs[j][i] = acc[a][b][r];
-      }
-  __syncthreads();
-  for (int idx = t; idx < BM * BN; idx += 256) {
-    // column-major: rows fastest; row-major (crow): columns fastest
-    const int i = crow ? idx / BN : idx % BM, j = crow ? idx % BN : idx / BM;
-    const int64_t gi = row0 + i, gj = col0 + j;
-    if (gi < M && gj < N) {
-      double v = Cs[j][i];
-      if (off) v -= off[gj];
-      const int64_t o = crow ? gi * ldc + gj : gi + gj * ldc;
-      if (Cd) Cd[o] = v;
-      else Cf[o] = static_cast<float>(v);
-    }
-  }
-}
-
-// Split-BF16x3 variant: a = ah + al, b = bh + bl (bf16 parts), and
-// a b ~= ah bh + ah bl + al bh on v_mfma_f32_16x16x32_bf16.
-// Operands are staged as FP32 in LDS and split in registers.
-__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
-  hi = static_cast<__bf16>(x);
-  lo = static_cast<__bf16>(x - static_cast<float>(hi));
-}
-
-template <int BN>
-__global__ __launch_bounds__(256) void k_gemm_nn_bf16x3(int64_t M, int64_t N, int64_t K,
-                                                        const float* __restrict__ A, int64_t lda,
-                                                        const double* __restrict__ shA,
-                                                        const float* __restrict__ B,
-                                                        int64_t ldb,
-                                                        const double* __restrict__ off,
-                                                        float* __restrict__ Cf,
-                                                        double* __restrict__ Cd, int64_t ldc,
-                                                        int crow) {
-  constexpr int BM = 64, BK = 32;
-  constexpr int NB = BN / 32;
-  __shared__ float As[BM][BK + 1];   // row-major per tile row: lane reads 8 consecutive k
-  __shared__ float Bs[BN][BK + 1];
-  __shared__ float Cs[BN][BM + 1];
-  __shared__ float shs[BK];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  const int64_t row0 = (int64_t)blockIdx.x * BM;
-  const int64_t col0 = (int64_t)blockIdx.y * BN;
-  v4f acc[2][NB];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-
-  for (int64_t k0 = 0; k0 < K; k0 += BK) {
-    if (t < BK) shs[t] = (shA && k0 + t < K) ? static_cast<float>(shA[k0 + t]) : 0.f;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int i = t % 64, k = t / 64 + 4 * r;
-      const int64_t gi = row0 + i, gk = k0 + k;
-      As[i][k] = (gi < M && gk < K) ? A[gi + gk * lda] - shs[k] : 0.f;
-    }
-#pragma unroll
-    for (int r = 0; r < BN / 8; ++r) {
-      const int k = t % 32, j = t / 32 + 8 * r;
-      const int64_t gj = col0 + j, gk = k0 + k;
-      Bs[j][k] = (gj < N && gk < K) ? B[gk + gj * ldb] : 0.f;
-    }
-    __syncthreads();
-    v8bf ah[2], al[2], bh[NB], bl[NB];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int i = wm * 32 + a * 16 + lane % 16;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 h, l;
-        split_bf16(As[i][8 * (lane / 16) + e], h, l);
-        ah[a][e] = h;
-        al[a][e] = l;
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int j = wn * (BN / 2) + b * 16 + lane % 16;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 h, l;
-        split_bf16(Bs[j][8 * (lane / 16) + e], h, l);
-        bh[b][e] = h;
-        bl[b][e] = l;
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < NB; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-      }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wm * 32 + a * 16 + 4 * (lane / 16) + r;
-        const int j = wn * (BN / 2) + b * 16 + lane % 16;
-        Cs[j][i] = acc[a][b][r];
-      }
-  __syncthreads();
-  for (int idx = t; idx < BM * BN; idx += 256) {
-    // column-major: rows fastest; row-major (crow): columns fastest
-    const int i = crow ? idx / BN : idx % BM, j = crow ? idx % BN : idx / BM;
-    const int64_t gi = row0 + i, gj = col0 + j;
-    if (gi < M && gj < N) {
-      double v = Cs[j][i];
-      if (off) v -= off[gj];
-      const int64_t o = crow ? gi * ldc + gj : gi + gj * ldc;
-      if (Cd) Cd[o] = v;
-      else Cf[o] = static_cast<float>(v);
-    }
-  }
-}
-
-void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
-                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
-                 const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
-  const int crow = c_rowmajor ? 1 : 0;
-  if (M <= 0 || N <= 0) return;
-  FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
-  const bool wide = N > 64;
-  const int BN = wide ? 128 : 64;
-  dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + BN - 1) / BN));
-  if (precision == FPLS_PREC_BF16X3) {
-    if (wide)
-      launch(ctx, "gemm_nn_bf16x3", k_gemm_nn_bf16x3<128>, grid, dim3(256), 0, M, N, K, A, lda,
-             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
-    else
-      launch(ctx, "gemm_nn_bf16x3", k_gemm_nn_bf16x3<64>, grid, dim3(256), 0, M, N, K, A, lda,
-             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
-  } else {
-    if (wide)
-      launch(ctx, "gemm_nn_f32", k_gemm_nn_f32<128>, grid, dim3(256), 0, M, N, K, A, lda, shiftA,
-             B, ldb, offset, Cf, Cd, ldc, crow);
-    else
-      launch(ctx, "gemm_nn_f32", k_gemm_nn_f32<64>, grid, dim3(256), 0, M, N, K, A, lda, shiftA,
-             B, ldb, offset, Cf, Cd, ldc, crow);
-  }
-}
-
-// ---------------------------------------------------------------- TN on MFMA
-// partial[z](M x N) = sum over rows k in slab z of (A[k,m]-shA[m]) (B[k,n]-shB[n]).
-// A: K x M (lda), B: K x N (ldb).  BM = BN = 64, BK = 32.
-__global__ __launch_bounds__(256) void k_gemm_tn_f32(int64_t M, int64_t N, int64_t K,
-                                                     int64_t kchunk, const float* __restrict__ A,
-                                                     int64_t lda, const double* __restrict__ shA,
-                                                     const float* __restrict__ B, int64_t ldb,
-                                                     const int32_t* __restrict__ labB,
-                                                     const double* __restrict__ shB,
-                                                     float* __restrict__ partial) {
-  constexpr int BM = 64, BN = 64, BK = 32;
-  __shared__ float As[BM][BK + 1];
-  __shared__ float Bs[BN][BK + 1];
-  __shared__ float Cs[BN][BM + 1];
-  __shared__ float sha[BM], shb[BN];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
-  const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
-  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
-  if (t < BM) sha[t] = (shA && m0 + t < M) ? static_cast<float>(shA[m0 + t]) : 0.f;
-  else if (t < BM + BN) {
-    const int j = t - BM;
-    shb[j] = (shB && n0 + j < N) ? static_cast<float>(shB[n0 + j]) : 0.f;
-  }
-  v4f acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  __syncthreads();
-  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int k = t % 32, m = t / 32 + 8 * r;
-      const int64_t gk = k0 + k, gm = m0 + m;
-      As[m][k] = (gk < kend && gm < M) ? A[gk + gm * lda] - sha[m] : 0.f;
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int k = t % 32, n = t / 32 + 8 * r;
-      const int64_t gk = k0 + k, gn = n0 + n;
-      float bv = 0.f;
-      if (gk < kend && gn < N) {
-        const float raw = labB ? (labB[gk] == gn + 1 ? 1.f : 0.f) : B[gk + gn * ldb];
-        bv = raw - shb[n];
-      }
-      Bs[n][k] = bv;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 4) {
-      float af[2], bf[2];
-#pragma unroll
-      for (int a = 0; a < 2; ++a) af[a] = As[wm * 32 + a * 16 + lane % 16][kk + lane / 16];
-#pragma unroll
-      for (int b = 0; b < 2; ++b) bf[b] = Bs[wn * 32 + b * 16 + lane % 16][kk + lane / 16];
-#pragma unroll
-      for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(af[a], bf[b], acc[a][b]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wm * 32 + a * 16 + 4 * (lane / 16) + r;
-        const int j = wn * 32 + b * 16 + lane % 16;
-        Cs[j][i] = acc[a][b][r];
-      }
-  __syncthreads();
-  float* P = partial + (int64_t)blockIdx.z * M * N;
-  for (int idx = t; idx < BM * BN; idx += 256) {
-    const int i = idx % BM, j = idx / BM;
-    const int64_t gm = m0 + i, gn = n0 + j;
-    if (gm < M && gn < N) P[gm + gn * M] = Cs[j][i];
-  }
-}
-
-__global__ __launch_bounds__(256) void k_gemm_tn_bf16x3(int64_t M, int64_t N, int64_t K,
-                                                        int64_t kchunk,
-                                                        const float* __restrict__ A, int64_t lda,
-                                                        const double* __restrict__ shA,
-                                                        const float* __restrict__ B, int64_t ldb,
-                                                        const int32_t* __restrict__ labB,
-                                                        const double* __restrict__ shB,
-                                                        float* __restrict__ partial) {
-  constexpr int BM = 64, BN = 64, BK = 32;
-  __shared__ float As[BM][BK + 1];
-  __shared__ float Bs[BN][BK + 1];
-  __shared__ float Cs[BN][BM + 1];
-  __shared__ float sha[BM], shb[BN];
-  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wm = w & 1, wn = w >> 1;
-  const int64_t m0 = (int64_t)blockIdx.x * BM;
-  const int64_t n0 = (int64_t)blockIdx.y * BN;
-  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
-  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
-  if (t < BM) sha[t] = (shA && m0 + t < M) ? static_cast<float>(shA[m0 + t]) : 0.f;
-  else if (t < BM + BN) {
-    const int j = t - BM;
-    shb[j] = (shB && n0 + j < N) ? static_cast<float>(shB[n0 + j]) : 0.f;
-  }
-  v4f acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  __syncthreads();
-  for (int64_t k0 = kbeg; k0 < kend; k0 += BK) {
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int k = t % 32, m = t / 32 + 8 * r;
-      const int64_t gk = k0 + k, gm = m0 + m;
-      As[m][k] = (gk < kend && gm < M) ? A[gk + gm * lda] - sha[m] : 0.f;
-    }
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int k = t % 32, n = t / 32 + 8 * r;
-      const int64_t gk = k0 + k, gn = n0 + n;
-      float bv = 0.f;
-      if (gk < kend && gn < N) {
-        const float raw = labB ? (labB[gk] == gn + 1 ? 1.f : 0.f) : B[gk + gn * ldb];
-        bv = raw - shb[n];
-      }
-      Bs[n][k] = bv;
-    }
-    __syncthreads();
-    v8bf ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a) {
-      const int i = wm * 32 + a * 16 + lane % 16;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 h, l;
-        split_bf16(As[i][8 * (lane / 16) + e], h, l);
-        ah[a][e] = h;
-        al[a][e] = l;
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int j = wn * 32 + b * 16 + lane % 16;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 h, l;
-        split_bf16(Bs[j][8 * (lane / 16) + e], h, l);
-        bh[b][e] = h;
-        bl[b][e] = l;
-      }
-    }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-      }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = wm * 32 + a * 16 + 4 * (lane / 16) + r;
-        const int j = wn * 32 + b * 16 + lane % 16;
-        Cs[j][i] = acc[a][b][r];
-      }
-  __syncthreads();
-  float* P = partial + (int64_t)blockIdx.z * M * N;
-  for (int idx = t; idx < BM * BN; idx += 256) {
-    const int i = idx % BM, j = idx / BM;
-    const int64_t gm = m0 + i, gn = n0 + j;
-    if (gm < M && gn < N) P[gm + gn * M] = Cs[j][i];
-  }
-}
-
-__global__ void k_reduce_partials_f32(int64_t M, int64_t N, int splits,
-                                      const float* __restrict__ partial,
-                                      double* __restrict__ C, int64_t ldc) {
-  const int64_t n = M * N;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int z = 0; z < splits; ++z) s += static_cast<double>(partial[(int64_t)z * n + t]);
-    const int64_t i = t % M, j = t / M;
-    C[i + j * ldc] = s;
-  }
-}
-
-void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
-                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
-                 const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc) {
-  if (M <= 0 || N <= 0) return;
-  if (K <= 0) {
-    for (int64_t j = 0; j < N; ++j) fill_f64(ctx, C + j * ldc, 0.0, M);
-    return;
-  }
-  const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
-  const int64_t want = 4 * ctx->num_cus;
-  int64_t splits = std::max<int64_t>(1, (want + tiles - 1) / tiles);
-  splits = std::min<int64_t>(splits, (K + 255) / 256);  // at least 256 rows per slab
-  const int64_t cap = std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N));
-  splits = std::max<int64_t>(1, std::min(splits, cap));
-  int64_t kchunk = (K + splits - 1) / splits;
-  kchunk = (kchunk + 31) / 32 * 32;
-  splits = (K + kchunk - 1) / kchunk;
-  DBuf<float> part(ctx, (size_t)splits * M * N);
-  dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)splits);
-  if (precision == FPLS_PREC_BF16X3)
-    launch(ctx, "gemm_tn_bf16x3", k_gemm_tn_bf16x3, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
-           shiftA, B, ldb, labelsB, shiftB, part.get());
-  else
-    launch(ctx, "gemm_tn_f32", k_gemm_tn_f32, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
-           shiftA, B, ldb, labelsB, shiftB, part.get());
-  launch(ctx, "gemm_tn_reduce", k_reduce_partials_f32, dim3(grid1d(M * N)), dim3(256), 0, M, N,
-         (int)splits, part.get(), C, ldc);
-}
-
-// C (M x N, column-major FP32, ld M) = A B^T with A (M x K) and B (N x K) both
-// row-major (K contiguous): the TN kernel with a single K slab writes C directly.
-void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
-                 int64_t lda, const float* B, int64_t ldb, float* C) {
-  if (M <= 0 || N <= 0) return;
-  const int64_t kchunk = ((std::max<int64_t>(K, 1) + 31) / 32) * 32;
-  dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), 1);
-  if (precision == FPLS_PREC_BF16X3)
-    launch(ctx, "gemm_rr_bf16x3", k_gemm_tn_bf16x3, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
-           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
-  else
-    launch(ctx, "gemm_rr_f32", k_gemm_tn_f32, grid, dim3(256), 0, M, N, K, kchunk, A, lda,
-           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
 }
 
 // ---------------------------------------------------------------- GEMV pair

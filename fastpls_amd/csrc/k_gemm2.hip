@@ -1,0 +1,499 @@
+// Pipelined MFMA contractions for FP32 data (gfx950).
+//
+//   NN:  C (M x N) = (A - 1 shA^T) B - 1 off^T      A: M x K col-major (X), B: K x N
+//   TN:  P_z (M x N) = sum_{k in slab z} (A[k,:] - shA)^T (B[k,:] - shB)
+//        A: K x M col-major (X), B: K x N col-major or the one-hot of int32 labels
+//
+// 128 x 128 output tiles, BK = 32, 256 threads = 4 waves in a 2 x 2 grid, each
+// wave 64 x 64 = 4 x 4 blocks of 16 x 16.  The next K tile is fetched into
+// registers (float4 loads) while the MFMAs of the current tile run; one barrier
+// per tile.  FP32 path: v_mfma_f32_16x16x4f32.  Split-BF16x3 path: each element
+// is split once, at LDS-store time, into bf16 hi + lo stored k-contiguous, and
+// three v_mfma_f32_16x16x32_bf16 (lo*hi + hi*lo + hi*hi) replace eight FP32
+// MFMAs.  Lane layouts: 16x16x4 f32 -- lane l supplies A[l%16][l/16],
+// B[l/16][l%16]; 16x16x32 bf16 -- lane l supplies A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16]; both accumulate D[4(l/16)+r][l%16].
+#include <algorithm>
+
+#include "fpls_internal.h"
+
+namespace fpls {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int G2_BM = 128, G2_BN = 128, G2_BK = 32, G2_NT = 256;
+constexpr int G2_PADF = 4;   // FP32 LDS row padding (floats)
+constexpr int G2_PADH = 8;   // bf16 LDS row padding (elements)
+
+__device__ inline void split_bf(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// 4 consecutive floats p[0..3] with bounds (count valid), vectorised when aligned
+__device__ inline float4 load4(const float* __restrict__ p, int valid, bool aligned) {
+  if (valid >= 4 && aligned) return *reinterpret_cast<const float4*>(p);
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid > 0) v.x = p[0];
+  if (valid > 1) v.y = p[1];
+  if (valid > 2) v.z = p[2];
+  if (valid > 3) v.w = p[3];
+  return v;
+}
+
+// LDS layout shared by both kernels, as raw floats.
+//  FP32:  As[2][BK][BM + PADF] (k-major, m contiguous), Bs[2][BK][BN + PADF]
+//  BF16:  Ah/Al[2][BM][BK + PADH] (m-major, k contiguous), Bh/Bl[2][BN][BK + PADH]
+constexpr int G2_F32_TILE = G2_BK * (G2_BM + G2_PADF);              // floats per buffer
+constexpr int G2_BF_TILE = G2_BM * (G2_BK + G2_PADH);               // bf16 per buffer
+constexpr int G2_SMEM_F32 = 2 * 2 * G2_F32_TILE;                    // floats
+constexpr int G2_SMEM_BF = (4 * 2 * G2_BF_TILE) / 2;                // floats
+constexpr int G2_SMEM_C = G2_BN * (G2_BM + 1);                      // epilogue tile
+template <bool BF>
+struct G2Smem {
+  static constexpr int tiles = BF ? G2_SMEM_BF : G2_SMEM_F32;
+  static constexpr int value = tiles > G2_SMEM_C ? tiles : G2_SMEM_C;
+};
+
+// The register-staged tile of one thread: A part (4 float4) and B part (4 float4)
+struct G2Regs {
+  float4 a[4];
+  float4 b[4];
+  double sa[4];   // NN: centring shift of the four A columns
+  int4 lab[4];    // TN with labels: codes of the four rows
+};
+
+// Store the staged registers into LDS buffer `buf`.
+//  amaj: for NN, A regs hold 4 rows (m..m+3) of one column k; for TN, 4 k of one m.
+template <bool BF, bool NN>
+__device__ inline void g2_store(float* smem, int buf, const G2Regs& r, int t) {
+  if (!BF) {
+    float* As = smem + buf * G2_F32_TILE;
+    float* Bs = smem + 2 * G2_F32_TILE + buf * G2_F32_TILE;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (NN) {  // A: rows m4..m4+3 of column k
+        const int m4 = (t % 32) * 4, k = t / 32 + 8 * j;
+        float* d = As + k * (G2_BM + G2_PADF) + m4;
+        *reinterpret_cast<float4*>(d) = r.a[j];
+      } else {   // A: k4..k4+3 of column m
+        const int k4 = (t % 8) * 4, m = t / 8 + 32 * j;
+        As[(k4 + 0) * (G2_BM + G2_PADF) + m] = r.a[j].x;
+        As[(k4 + 1) * (G2_BM + G2_PADF) + m] = r.a[j].y;
+        As[(k4 + 2) * (G2_BM + G2_PADF) + m] = r.a[j].z;
+        As[(k4 + 3) * (G2_BM + G2_PADF) + m] = r.a[j].w;
+      }
+      const int k4 = (t % 8) * 4, n = t / 8 + 32 * j;  // B: k4..k4+3 of column n
+      Bs[(k4 + 0) * (G2_BN + G2_PADF) + n] = r.b[j].x;
+      Bs[(k4 + 1) * (G2_BN + G2_PADF) + n] = r.b[j].y;
+      Bs[(k4 + 2) * (G2_BN + G2_PADF) + n] = r.b[j].z;
+      Bs[(k4 + 3) * (G2_BN + G2_PADF) + n] = r.b[j].w;
+    }
+  } else {
+    __bf16* base = reinterpret_cast<__bf16*>(smem);
+    __bf16* Ah = base + (0 * 2 + buf) * G2_BF_TILE;
+    __bf16* Al = base + (1 * 2 + buf) * G2_BF_TILE;
+    __bf16* Bh = base + (2 * 2 + buf) * G2_BF_TILE;
+    __bf16* Bl = base + (3 * 2 + buf) * G2_BF_TILE;
+    constexpr int LD = G2_BK + G2_PADH;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float av[4] = {r.a[j].x, r.a[j].y, r.a[j].z, r.a[j].w};
+      if (NN) {
+        const int m4 = (t % 32) * 4, k = t / 32 + 8 * j;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          __bf16 h, l;
+          split_bf(av[e], h, l);
+          Ah[(m4 + e) * LD + k] = h;
+          Al[(m4 + e) * LD + k] = l;
+        }
+      } else {
+        const int k4 = (t % 8) * 4, m = t / 8 + 32 * j;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          __bf16 h, l;
+          split_bf(av[e], h, l);
+          Ah[m * LD + k4 + e] = h;
+          Al[m * LD + k4 + e] = l;
+        }
+      }
+      const float bv[4] = {r.b[j].x, r.b[j].y, r.b[j].z, r.b[j].w};
+      const int k4 = (t % 8) * 4, n = t / 8 + 32 * j;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        __bf16 h, l;
+        split_bf(bv[e], h, l);
+        Bh[n * LD + k4 + e] = h;
+        Bl[n * LD + k4 + e] = l;
+      }
+    }
+  }
+}
+
+// MFMAs of one K tile from LDS buffer `buf` into the 4 x 4 accumulators.
+template <bool BF>
+__device__ inline void g2_compute(const float* smem, int buf, v4f (&acc)[4][4], int lane, int wm,
+                                  int wn) {
+  if (!BF) {
+    const float* As = smem + buf * G2_F32_TILE;
+    const float* Bs = smem + 2 * G2_F32_TILE + buf * G2_F32_TILE;
+#pragma unroll
+    for (int kk = 0; kk < G2_BK; kk += 4) {
+      float af[4], bf[4];
+      const int kr = kk + lane / 16;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) af[a] = As[kr * (G2_BM + G2_PADF) + wm * 64 + a * 16 + lane % 16];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) bf[b] = Bs[kr * (G2_BN + G2_PADF) + wn * 64 + b * 16 + lane % 16];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bf[b], acc[a][b], 0, 0, 0);
+    }
+  } else {
+    const __bf16* base = reinterpret_cast<const __bf16*>(smem);
+    const __bf16* Ah = base + (0 * 2 + buf) * G2_BF_TILE;
+    const __bf16* Al = base + (1 * 2 + buf) * G2_BF_TILE;
+    const __bf16* Bh = base + (2 * 2 + buf) * G2_BF_TILE;
+    const __bf16* Bl = base + (3 * 2 + buf) * G2_BF_TILE;
+    constexpr int LD = G2_BK + G2_PADH;
+    const int ko = 8 * (lane / 16);
+    v8bf ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int m = wm * 64 + a * 16 + lane % 16;
+      ah[a] = *reinterpret_cast<const v8bf*>(Ah + m * LD + ko);
+      al[a] = *reinterpret_cast<const v8bf*>(Al + m * LD + ko);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int n = wn * 64 + b * 16 + lane % 16;
+      bh[b] = *reinterpret_cast<const v8bf*>(Bh + n * LD + ko);
+      bl[b] = *reinterpret_cast<const v8bf*>(Bl + n * LD + ko);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+      }
+  }
+}
+
+// Accumulators -> LDS tile Cs[n][m] (ld BM + 1).
+__device__ inline void g2_stage_c(float* smem, const v4f (&acc)[4][4], int lane, int wm, int wn) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = wm * 64 + a * 16 + 4 * (lane / 16) + r;
+        const int j = wn * 64 + b * 16 + lane % 16;
+        smem[j * (G2_BM + 1) + i] = acc[a][b][r];
+      }
+}
+
+// ---------------------------------------------------------------- NN
+// B: zero-padded to Kp = ceil(K/32)*32 rows and Np = ceil(N/128)*128 columns,
+// ldb = Kp (multiple of 4), 16-byte aligned.
+template <bool BF>
+__global__ __launch_bounds__(G2_NT) void k_g2_nn(int64_t M, int64_t N, int64_t K,
+                                                 const float* __restrict__ A, int64_t lda,
+                                                 const double* __restrict__ shA,
+                                                 const float* __restrict__ B, int64_t ldb,
+                                                 const double* __restrict__ off,
+                                                 float* __restrict__ Cf, double* __restrict__ Cd,
+                                                 int64_t ldc, int crow) {
+  __shared__ __align__(16) float smem[G2Smem<BF>::value];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int64_t row0 = (int64_t)blockIdx.x * G2_BM, col0 = (int64_t)blockIdx.y * G2_BN;
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // Addresses are clamped into the matrix (rows >= M and columns >= K read valid
+  // memory); B is zero-padded to whole tiles, so clamped columns contribute 0 and
+  // clamped rows are never written.  No per-element branches in the loop.
+  const bool vecA = ((lda & 3) == 0) && ((M & 3) == 0) &&
+                    ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  auto fetch = [&](int64_t k0, G2Regs& r) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int m4 = (t % 32) * 4, k = t / 32 + 8 * j;
+      const int64_t gk = std::min<int64_t>(k0 + k, K - 1);
+      const float* col = A + gk * lda;
+      float4 a;
+      if (vecA) {
+        a = *reinterpret_cast<const float4*>(col + std::min<int64_t>(row0 + m4, M - 4));
+      } else {
+        a.x = col[std::min<int64_t>(row0 + m4 + 0, M - 1)];
+        a.y = col[std::min<int64_t>(row0 + m4 + 1, M - 1)];
+        a.z = col[std::min<int64_t>(row0 + m4 + 2, M - 1)];
+        a.w = col[std::min<int64_t>(row0 + m4 + 3, M - 1)];
+      }
+      r.sa[j] = shA ? shA[gk] : 0.0;
+      r.a[j] = a;
+      const int kq = (t % 8) * 4, n = t / 8 + 32 * j;
+      r.b[j] = *reinterpret_cast<const float4*>(B + (k0 + kq) + (col0 + n) * ldb);
+    }
+  };
+  // arithmetic on the fetched registers runs after the MFMAs of the current tile
+  auto finish = [&](G2Regs& r) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float sv = static_cast<float>(r.sa[j]);
+      r.a[j].x -= sv;
+      r.a[j].y -= sv;
+      r.a[j].z -= sv;
+      r.a[j].w -= sv;
+    }
+  };
+  const int nk = (int)((K + G2_BK - 1) / G2_BK);
+  G2Regs r;
+  fetch(0, r);
+  finish(r);
+  g2_store<BF, true>(smem, 0, r, t);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) fetch((int64_t)(kt + 1) * G2_BK, r);
+    g2_compute<BF>(smem, cur, acc, lane, wm, wn);
+    if (more) {
+      finish(r);
+      g2_store<BF, true>(smem, cur ^ 1, r, t);
+    }
+    __syncthreads();
+  }
+  g2_stage_c(smem, acc, lane, wm, wn);
+  __syncthreads();
+  for (int idx = t; idx < G2_BM * G2_BN; idx += G2_NT) {
+    const int i = crow ? idx / G2_BN : idx % G2_BM;
+    const int j = crow ? idx % G2_BN : idx / G2_BM;
+    const int64_t gi = row0 + i, gj = col0 + j;
+    if (gi < M && gj < N) {
+      double v = smem[j * (G2_BM + 1) + i];
+      if (off) v -= off[gj];
+      const int64_t o = crow ? gi * ldc + gj : gi + gj * ldc;
+      if (Cd) Cd[o] = v;
+      else Cf[o] = static_cast<float>(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- TN (split-K partials)
+template <bool BF>
+__global__ __launch_bounds__(G2_NT) void k_g2_tn(int64_t M, int64_t N, int64_t K, int64_t kchunk,
+                                                 const float* __restrict__ A, int64_t lda,
+                                                 const double* __restrict__ shA,
+                                                 const float* __restrict__ B, int64_t ldb,
+                                                 const int32_t* __restrict__ labB,
+                                                 const double* __restrict__ shB,
+                                                 float* __restrict__ partial) {
+  __shared__ __align__(16) float smem[G2Smem<BF>::value];
+  __shared__ float sha[G2_BM], shb[G2_BN];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int64_t m0 = (int64_t)blockIdx.x * G2_BM, n0 = (int64_t)blockIdx.y * G2_BN;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
+  const bool a_al = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                    ((kbeg & 3) == 0);
+  const bool b_al = B && ((ldb & 3) == 0) && ((reinterpret_cast<uintptr_t>(B) & 15) == 0) &&
+                    ((kbeg & 3) == 0);
+  if (t < G2_BM) sha[t] = (shA && m0 + t < M) ? static_cast<float>(shA[m0 + t]) : 0.f;
+  else shb[t - G2_BM] = (shB && n0 + t - G2_BM < N) ? static_cast<float>(shB[n0 + t - G2_BM]) : 0.f;
+  __syncthreads();
+  v4f acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // Clamped addresses, then masks (selects, not branches) zero everything outside
+  // the slab / the matrix.
+  auto ld4 = [&](const float* base, int64_t gk, bool vec) -> float4 {
+    if (vec && gk + 4 <= K) return *reinterpret_cast<const float4*>(base + gk);
+    float4 v;
+    v.x = base[std::min<int64_t>(gk + 0, K - 1)];
+    v.y = base[std::min<int64_t>(gk + 1, K - 1)];
+    v.z = base[std::min<int64_t>(gk + 2, K - 1)];
+    v.w = base[std::min<int64_t>(gk + 3, K - 1)];
+    return v;
+  };
+  auto fetch = [&](int64_t k0, G2Regs& r) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kq = (t % 8) * 4, c = t / 8 + 32 * j;
+      const int64_t gk = k0 + kq;
+      r.a[j] = ld4(A + std::min<int64_t>(m0 + c, M - 1) * lda, gk, a_al);
+      if (labB) {
+        r.lab[j].x = labB[std::min<int64_t>(gk + 0, K - 1)];
+        r.lab[j].y = labB[std::min<int64_t>(gk + 1, K - 1)];
+        r.lab[j].z = labB[std::min<int64_t>(gk + 2, K - 1)];
+        r.lab[j].w = labB[std::min<int64_t>(gk + 3, K - 1)];
+      } else {
+        r.b[j] = ld4(B + std::min<int64_t>(n0 + c, N - 1) * ldb, gk, b_al);
+      }
+    }
+  };
+  // centring, one-hot generation and slab masks, applied after the MFMAs
+  auto finish = [&](int64_t k0, G2Regs& r) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kq = (t % 8) * 4, c = t / 8 + 32 * j;
+      const int64_t gk = k0 + kq;
+      const bool in0 = gk + 0 < kend, in1 = gk + 1 < kend, in2 = gk + 2 < kend, in3 = gk + 3 < kend;
+      const bool mok = m0 + c < M, nok = n0 + c < N;
+      const float s = sha[c];
+      float4 a = r.a[j];
+      a.x = (in0 && mok) ? a.x - s : 0.f;
+      a.y = (in1 && mok) ? a.y - s : 0.f;
+      a.z = (in2 && mok) ? a.z - s : 0.f;
+      a.w = (in3 && mok) ? a.w - s : 0.f;
+      r.a[j] = a;
+      float4 b;
+      if (labB) {
+        const int lab_n = (int)(n0 + c) + 1;
+        b.x = (r.lab[j].x == lab_n ? 1.f : 0.f);
+        b.y = (r.lab[j].y == lab_n ? 1.f : 0.f);
+        b.z = (r.lab[j].z == lab_n ? 1.f : 0.f);
+        b.w = (r.lab[j].w == lab_n ? 1.f : 0.f);
+      } else {
+        b = r.b[j];
+      }
+      const float sb = shb[c];
+      b.x = (in0 && nok) ? b.x - sb : 0.f;
+      b.y = (in1 && nok) ? b.y - sb : 0.f;
+      b.z = (in2 && nok) ? b.z - sb : 0.f;
+      b.w = (in3 && nok) ? b.w - sb : 0.f;
+      r.b[j] = b;
+    }
+  };
+  const int nk = (int)((kend - kbeg + G2_BK - 1) / G2_BK);
+  G2Regs r;
+  if (nk > 0) {
+    fetch(kbeg, r);
+    finish(kbeg, r);
+    g2_store<BF, false>(smem, 0, r, t);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    const int64_t knext = kbeg + (int64_t)(kt + 1) * G2_BK;
+    if (more) fetch(knext, r);
+    g2_compute<BF>(smem, cur, acc, lane, wm, wn);
+    if (more) {
+      finish(knext, r);
+      g2_store<BF, false>(smem, cur ^ 1, r, t);
+    }
+    __syncthreads();
+  }
+  g2_stage_c(smem, acc, lane, wm, wn);
+  __syncthreads();
+  float* P = partial + (int64_t)blockIdx.z * M * N;
+  for (int idx = t; idx < G2_BM * G2_BN; idx += G2_NT) {
+    const int i = idx % G2_BM, j = idx / G2_BM;
+    const int64_t gm = m0 + i, gn = n0 + j;
+    if (gm < M && gn < N) P[gm + gn * M] = smem[j * (G2_BM + 1) + i];
+  }
+}
+
+static unsigned g1(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+
+__global__ void k_g2_pad(const float* __restrict__ B, int64_t ldb, int64_t K, int64_t N,
+                         float* __restrict__ out, int64_t Kp, int64_t Np) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < Kp * Np;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = t % Kp, n = t / Kp;
+    out[t] = (k < K && n < N) ? B[k + n * ldb] : 0.f;
+  }
+}
+
+__global__ void k_g2_reduce(int64_t M, int64_t N, int splits, const float* __restrict__ partial,
+                            double* __restrict__ C, int64_t ldc) {
+  const int64_t n = M * N;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int z = 0; z < splits; ++z) s += static_cast<double>(partial[(int64_t)z * n + t]);
+    C[(t % M) + (t / M) * ldc] = s;
+  }
+}
+
+void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                 const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
+  if (M <= 0 || N <= 0) return;
+  FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
+  dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN));
+  const int crow = c_rowmajor ? 1 : 0;
+  // zero-padded copy of the (small) B operand: Kp x Np, ld Kp
+  const int64_t Kp = (K + G2_BK - 1) / G2_BK * G2_BK, Np = (N + G2_BN - 1) / G2_BN * G2_BN;
+  DBuf<float> Bp(ctx, (size_t)Kp * Np);
+  launch(ctx, "pad_b", k_g2_pad, dim3(g1(Kp * Np)), dim3(256), 0, B, ldb, K, N, Bp.get(), Kp, Np);
+  B = Bp.get();
+  ldb = Kp;
+  if (precision == FPLS_PREC_BF16X3)
+    launch(ctx, "gemm_nn_bf16x3", k_g2_nn<true>, grid, dim3(G2_NT), 0, M, N, K, A, lda, shiftA, B,
+           ldb, offset, Cf, Cd, ldc, crow);
+  else
+    launch(ctx, "gemm_nn_f32", k_g2_nn<false>, grid, dim3(G2_NT), 0, M, N, K, A, lda, shiftA, B,
+           ldb, offset, Cf, Cd, ldc, crow);
+}
+
+void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                 const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc) {
+  if (M <= 0 || N <= 0) return;
+  if (K <= 0) {
+    for (int64_t j = 0; j < N; ++j) fill_f64(ctx, C + j * ldc, 0.0, M);
+    return;
+  }
+  const int64_t tiles = ((M + G2_BM - 1) / G2_BM) * ((N + G2_BN - 1) / G2_BN);
+  const int64_t want = 2 * ctx->num_cus;
+  int64_t splits = std::max<int64_t>(1, (want + tiles - 1) / tiles);
+  splits = std::min<int64_t>(splits, (K + 511) / 512);  // at least 512 rows per slab
+  const int64_t cap = std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N));
+  splits = std::max<int64_t>(1, std::min(splits, cap));
+  int64_t kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + G2_BK - 1) / G2_BK * G2_BK;
+  splits = (K + kchunk - 1) / kchunk;
+  DBuf<float> part(ctx, (size_t)splits * M * N);
+  dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN),
+            (unsigned)splits);
+  if (precision == FPLS_PREC_BF16X3)
+    launch(ctx, "gemm_tn_bf16x3", k_g2_tn<true>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
+           shiftA, B, ldb, labelsB, shiftB, part.get());
+  else
+    launch(ctx, "gemm_tn_f32", k_g2_tn<false>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
+           shiftA, B, ldb, labelsB, shiftB, part.get());
+  launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3(g1(M * N)), dim3(256), 0, M, N, (int)splits,
+         (const float*)part.get(), C, ldc);
+}
+
+// C (M x N col-major FP32, ld M) = A B^T, A: M x K and B: N x K row-major.
+void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                 int64_t lda, const float* B, int64_t ldb, float* C) {
+  if (M <= 0 || N <= 0) return;
+  const int64_t kchunk = ((std::max<int64_t>(K, 1) + G2_BK - 1) / G2_BK) * G2_BK;
+  dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN), 1);
+  if (precision == FPLS_PREC_BF16X3)
+    launch(ctx, "gemm_rr_bf16x3", k_g2_tn<true>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
+  else
+    launch(ctx, "gemm_rr_f32", k_g2_tn<false>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
+}
+
+}  // namespace fpls
