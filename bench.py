@@ -35,9 +35,39 @@ SIN_THETA_BOUND = 1e-4
 
 # Algorithmic work of the projection kernel per step (useful FLOPs of the
 # reference algorithm): T = Xc U for the training rows and Ttest = Xtest_c U,
-# 2 n p K each.  Bytes: the FP32 X streamed once per launch.
+# 2 n p K each (two launches).  Bytes: the FP32 X read once and the FP32 scores
+# T written once, 4 n (p + K) per launch.
 PROJ_FLOPS = 2.0 * N_TRAIN * P * NCOMP + 2.0 * N_TEST * P * NCOMP
-PROJ_BYTES = 4.0 * (N_TRAIN + N_TEST) * P
+PROJ_BYTES = 4.0 * (N_TRAIN + N_TEST) * (P + NCOMP)
+# Split-BF16x3 issues three BF16 MFMA products per useful product, so its peak
+# for useful FLOPs is a third of the BF16 MFMA peak.
+USEFUL_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0}
+
+
+def measured_traffic(kname, launches):
+    """HBM bytes per launch of ``kname`` from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (profiles/hbm_traffic.json, written from those counter
+    runs and scaled by a plain copy of known size), or None when absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return float(t["kernels"][kname]["hbm_bytes_per_launch"])
+    except (OSError, KeyError, ValueError, TypeError):
+        return None
+
+
+def calibration_copy(mib):
+    """One plain device-to-device copy of ``mib`` MiB: the reference dispatch
+    that scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
 
 
 def sin_theta(U, Uref):
@@ -80,7 +110,8 @@ def run_gpu(args, data):
         for _ in range(tsteps):
             step()
         kern = {}
-        for nm in ("gemm_nn_f32", "gemm_nn_bf16x3", "gemm_tn_f32", "gemm_tn_bf16x3",
+        for nm in ("project_f32", "project_bf16x3", "gemm_nn_f32", "gemm_nn_bf16x3",
+                   "gemm_tn_f32", "gemm_tn_bf16x3",
                    "sym_eig_jacobi", "spd_solve", "col_sums", "lda_argmax", "gemm_f64",
                    "gemm_tn_reduce", "label_counts"):
             n, ms = ctx.kernel_time(nm)
@@ -115,6 +146,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -122,6 +155,8 @@ def main():
     from fastpls_amd import synth
     data = synth.classes_on_sphere(n_classes=C, per_class_train=N_TRAIN // C,
                                    per_class_test=N_TEST // C, p=P)
+    if args.copy_mib > 0:
+        calibration_copy(args.copy_mib)
     res = run_gpu(args, data)
     errors = {}
     cpu = None
@@ -149,15 +184,30 @@ def main():
                "oracle_accuracy": float(np.mean(ref["Ypred_labels"][:, 0] == data[3]))}
     chosen = "bf16x3" if ok["bf16x3"] else "fp32"
     r = res[chosen]
-    kname = "gemm_nn_bf16x3" if chosen == "bf16x3" else "gemm_nn_f32"
+    kname = "project_bf16x3" if chosen == "bf16x3" else "project_f32"
     roof = None
     if kname in r["kernels"]:
-        ms = r["kernels"][kname]["ms_per_step"]
-        achieved = PROJ_FLOPS / (ms * 1e-3) / 1e12
-        peak = PEAK_TFLOPS[chosen]
-        roof = {"bound": "mfma", "kernel": kname, "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "hbm_GBps": PROJ_BYTES / (ms * 1e-3) / 1e9}
+        kt = r["kernels"][kname]
+        launches = kt["launches_per_step"]
+        avg_s = kt["ms_per_step"] * 1e-3 / launches
+        # arithmetic intensity vs the ridge point of the form's useful-FLOP peak
+        ai = PROJ_FLOPS / PROJ_BYTES
+        ridge = USEFUL_PEAK_TFLOPS[chosen] * 1e12 / (HBM_PEAK_GBS * 1e9)
+        if ai < ridge:
+            bound, unit, peak = "hbm", "GB/s", HBM_PEAK_GBS
+            achieved = PROJ_BYTES / launches / avg_s / 1e9
+        else:
+            bound, unit, peak = "mfma", "TFLOP/s", PEAK_TFLOPS[chosen]
+            achieved = PROJ_FLOPS / launches / avg_s / 1e12
+        roof = {"bound": bound, "kernel": kname, "achieved": achieved, "peak": peak,
+                "unit": unit, "frac": achieved / peak,
+                "traffic": measured_traffic(kname, launches),
+                "arith_intensity": ai, "ridge": ridge,
+                "tflops": PROJ_FLOPS / launches / avg_s / 1e12,
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": launches,
+                "algorithmic_flops_per_launch": PROJ_FLOPS / launches,
+                "algorithmic_bytes_per_launch": PROJ_BYTES / launches,
+                "hbm_GBps": PROJ_BYTES / launches / avg_s / 1e9}
     out = {
         "metric": METRIC, "value": r["fits_per_s"], "unit": "fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],

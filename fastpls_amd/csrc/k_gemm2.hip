@@ -201,8 +201,9 @@ __device__ inline void g2_stage_c(float* smem, const v4f (&acc)[4][4], int lane,
 
 // ---------------------------------------------------------------- NN
 // B: zero-padded to Kp = ceil(K/32)*32 rows and Np = ceil(N/128)*128 columns,
-// ldb = Kp (multiple of 4), 16-byte aligned.
-template <bool BF>
+// ldb = Kp (multiple of 4), 16-byte aligned.  CEN: A is centred on load by shA
+// (the latent projections of raw X); otherwise A is used as stored.
+template <bool BF, bool CEN>
 __global__ __launch_bounds__(G2_NT) void k_g2_nn(int64_t M, int64_t N, int64_t K,
                                                  const float* __restrict__ A, int64_t lda,
                                                  const double* __restrict__ shA,
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(G2_NT) void k_g2_nn(int64_t M, int64_t N, int64_t K
         a.z = col[std::min<int64_t>(row0 + m4 + 2, M - 1)];
         a.w = col[std::min<int64_t>(row0 + m4 + 3, M - 1)];
       }
-      r.sa[j] = shA ? shA[gk] : 0.0;
+      if (CEN) r.sa[j] = shA[gk];
       r.a[j] = a;
       const int kq = (t % 8) * 4, n = t / 8 + 32 * j;
       r.b[j] = *reinterpret_cast<const float4*>(B + (k0 + kq) + (col0 + n) * ldb);
@@ -247,6 +248,7 @@ __global__ __launch_bounds__(G2_NT) void k_g2_nn(int64_t M, int64_t N, int64_t K
   };
   // arithmetic on the fetched registers runs after the MFMAs of the current tile
   auto finish = [&](G2Regs& r) {
+    if (!CEN) return;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const float sv = static_cast<float>(r.sa[j]);
@@ -436,6 +438,7 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
                  const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
   if (M <= 0 || N <= 0) return;
   FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
+  FPLS_REQUIRE(K >= 1, FPLS_ERR_ARG, "gemm_nn_f32: inner dimension must be positive");
   dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN));
   const int crow = c_rowmajor ? 1 : 0;
   // zero-padded copy of the (small) B operand: Kp x Np, ld Kp
@@ -444,12 +447,23 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   launch(ctx, "pad_b", k_g2_pad, dim3(g1(Kp * Np)), dim3(256), 0, B, ldb, K, N, Bp.get(), Kp, Np);
   B = Bp.get();
   ldb = Kp;
-  if (precision == FPLS_PREC_BF16X3)
-    launch(ctx, "gemm_nn_bf16x3", k_g2_nn<true>, grid, dim3(G2_NT), 0, M, N, K, A, lda, shiftA, B,
-           ldb, offset, Cf, Cd, ldc, crow);
-  else
-    launch(ctx, "gemm_nn_f32", k_g2_nn<false>, grid, dim3(G2_NT), 0, M, N, K, A, lda, shiftA, B,
-           ldb, offset, Cf, Cd, ldc, crow);
+  // centred contractions (projections of X) are timed as "project_*"
+  const bool bf = precision == FPLS_PREC_BF16X3;
+  if (shiftA) {
+    if (bf)
+      launch(ctx, "project_bf16x3", k_g2_nn<true, true>, grid, dim3(G2_NT), 0, M, N, K, A, lda,
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
+    else
+      launch(ctx, "project_f32", k_g2_nn<false, true>, grid, dim3(G2_NT), 0, M, N, K, A, lda,
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
+  } else {
+    if (bf)
+      launch(ctx, "gemm_nn_bf16x3", k_g2_nn<true, false>, grid, dim3(G2_NT), 0, M, N, K, A, lda,
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
+    else
+      launch(ctx, "gemm_nn_f32", k_g2_nn<false, false>, grid, dim3(G2_NT), 0, M, N, K, A, lda,
+             shiftA, B, ldb, offset, Cf, Cd, ldc, crow);
+  }
 }
 
 void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
