@@ -112,7 +112,7 @@ def run_gpu(args, data):
         kern = {}
         for nm in ("project_f32", "project_bf16x3", "gemm_nn_f32", "gemm_nn_bf16x3",
                    "gemm_tn_f32", "gemm_tn_bf16x3",
-                   "sym_eig_jacobi", "spd_solve", "col_sums", "lda_argmax", "gemm_f64",
+                   "sym_eig_dc", "spd_solve", "col_sums", "lda_argmax", "gemm_f64",
                    "gemm_tn_reduce", "label_counts"):
             n, ms = ctx.kernel_time(nm)
             if n:

@@ -1,11 +1,6 @@
-// Small FP64 factorisations kept on the GPU (one workgroup each, matrix in LDS):
+// Small FP64 factorisations kept on the GPU (one workgroup each, matrix in LDS;
+// the symmetric eigensolver lives in k_eig.hip):
 //
-//  * sym_eig          -- cyclic Jacobi with the round-robin parallel ordering
-//                        (Brent & Luk): all n/2 disjoint rotations of a round
-//                        are applied at once, two-sided on A (LDS) and
-//                        one-sided on V (HBM).  Rotation angles follow the
-//                        symmetric Schur decomposition (Golub & Van Loan,
-//                        Algorithm 8.4.1).  Eigenvalues sorted descending.
 //  * spd_solve_slices -- per ncomp slice: Cholesky of the leading block (with
 //                        the reference's diagonal shift rules) and solves
 //                        against a diagonal / identity right-hand side; LU with
@@ -20,136 +15,6 @@ namespace fpls {
 
 constexpr int JMAX = kJacobiMax;  // 128
 constexpr int JLD = JMAX + 1;
-
-// One round of the parallel ordering rotates n/2 disjoint pairs at once.
-// Every 2x2 block (rows of pair k, columns of pair l) is then updated
-// independently as J_k^T A_kl J_l, so a round is one parameter phase and one
-// update phase.  Rotations are skipped when |a_pq| <= 1e-15 sqrt(|a_pp a_qq|)
-// (the relative-accuracy criterion of Demmel & Veselic, SIAM J. Matrix Anal.
-// Appl. 13, 1992); a sweep without rotations ends the iteration.
-template <int NMAX, bool V_IN_LDS>
-__global__ __launch_bounds__(1024) void k_sym_eig(const double* __restrict__ Ain, int64_t lda,
-                                                  int n, double* __restrict__ w,
-                                                  double* __restrict__ Vout,
-                                                  double* __restrict__ Vg, int max_sweeps) {
-  // static LDS: A (NMAX x NMAX+1) and, when it fits beside A in the 160 KiB
-  // of a gfx950 CU, V (NMAX x NMAX)
-  __shared__ double sA[NMAX * (NMAX + 1)];
-  __shared__ double sV[V_IN_LDS ? NMAX * NMAX : 1];
-  __shared__ double cs[JMAX / 2], sn[JMAX / 2];
-  __shared__ int pp[JMAX / 2], qq[JMAX / 2];
-  __shared__ int rot_count;
-  __shared__ int rank[JMAX];
-  const int t = threadIdx.x, nt = blockDim.x;
-  const int N = n + (n & 1);  // even order; index n (if any) is a zero dummy
-  const int h = N / 2;
-  const int lda_s = N + 1;
-  double* A = sA;
-  double* V = V_IN_LDS ? sV : Vg;
-  for (int idx = t; idx < N * N; idx += nt) {
-    const int i = idx % N, j = idx / N;
-    A[i + j * lda_s] = (i < n && j < n) ? 0.5 * (Ain[i + j * lda] + Ain[j + i * lda]) : 0.0;
-    V[i + j * N] = (i == j) ? 1.0 : 0.0;
-  }
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    if (t == 0) rot_count = 0;
-    __syncthreads();
-    for (int round = 0; round < N - 1; ++round) {
-      if (t < h) {
-        auto pos = [&](int k) { return k == 0 ? 0 : ((k - 1 + round) % (N - 1)) + 1; };
-        int p = pos(t), q = pos(N - 1 - t);
-        if (p > q) { const int tmp = p; p = q; q = tmp; }
-        const double apq = A[p + q * lda_s];
-        const double app = A[p + p * lda_s], aqq = A[q + q * lda_s];
-        double c = 1.0, s = 0.0;
-        if (fabs(apq) > 1e-15 * sqrt(fabs(app * aqq)) && fabs(apq) > 1e-300) {
-          const double zeta = (aqq - app) / (2.0 * apq);
-          const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-          c = 1.0 / sqrt(1.0 + tt * tt);
-          s = tt * c;
-          atomicAdd(&rot_count, 1);
-        }
-        pp[t] = p;
-        qq[t] = q;
-        cs[t] = c;
-        sn[t] = s;
-      }
-      __syncthreads();
-      // thread -> (row pair k fixed, column pairs l strided): the row pair's
-      // rotation stays in registers for the whole round
-      const int lg_count = nt / h;
-      const int kq = t % h, lg = t / h;
-      const double ck = cs[kq], sk = sn[kq];
-      const int pk = pp[kq], qk = qq[kq];
-      for (int l = lg; lg < lg_count && l < h; l += lg_count) {
-        const int k = kq;
-        const double cl = cs[l], sl = sn[l];
-        if (sk == 0.0 && sl == 0.0) continue;
-        const int pl = pp[l], ql = qq[l];
-        const double a = A[pk + pl * lda_s], b = A[pk + ql * lda_s];
-        const double c_ = A[qk + pl * lda_s], d = A[qk + ql * lda_s];
-        const double m00 = ck * a - sk * c_, m01 = ck * b - sk * d;
-        const double m10 = sk * a + ck * c_, m11 = sk * b + ck * d;
-        double n00 = m00 * cl - m01 * sl, n01 = m00 * sl + m01 * cl;
-        double n10 = m10 * cl - m11 * sl, n11 = m10 * sl + m11 * cl;
-        if (k == l && sk != 0.0) {
-          n01 = 0.0;
-          n10 = 0.0;
-        }
-        A[pk + pl * lda_s] = n00;
-        A[pk + ql * lda_s] = n01;
-        A[qk + pl * lda_s] = n10;
-        A[qk + ql * lda_s] = n11;
-      }
-      {
-        // thread -> (column pair l fixed, rows strided)
-        const int l = t % h, ig = t / h;
-        const double cl = cs[l], sl = sn[l];
-        const int pl = pp[l], ql = qq[l];
-        if (sl != 0.0 && ig < lg_count) {
-          for (int i = ig; i < N; i += lg_count) {
-            const double vip = V[i + pl * N], viq = V[i + ql * N];
-            V[i + pl * N] = cl * vip - sl * viq;
-            V[i + ql * N] = sl * vip + cl * viq;
-          }
-        }
-      }
-      __syncthreads();
-    }
-    if (rot_count == 0) break;
-    __syncthreads();
-  }
-  // sort descending (stable on index), n <= 128
-  if (t < n) {
-    const double wi = A[t + t * lda_s];
-    int r = 0;
-    for (int j = 0; j < n; ++j) {
-      const double wj = A[j + j * lda_s];
-      if (wj > wi || (wj == wi && j < t)) ++r;
-    }
-    rank[t] = r;
-    w[r] = wi;
-  }
-  __syncthreads();
-  for (int idx = t; idx < n * n; idx += nt) {
-    const int i = idx % n, j = idx / n;
-    Vout[i + (int64_t)rank[j] * n] = V[i + j * N];
-  }
-}
-
-void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
-  FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
-               "sym_eig: order must be in 1..128 for the one-workgroup solver");
-  if (n <= 100) {
-    launch(ctx, "sym_eig_jacobi", k_sym_eig<100, true>, dim3(1), dim3(1024), 0, A, lda, n, w, V,
-           (double*)nullptr, 40);
-  } else {
-    const int N = n + (n & 1);
-    DBuf<double> Vw(ctx, (size_t)N * N);
-    launch(ctx, "sym_eig_jacobi", k_sym_eig<128, false>, dim3(1), dim3(1024), 0, A, lda, n, w, V,
-           Vw.get(), 40);
-  }
-}
 
 // --------------------------------------------------------------- slice solves
 // mode 0: (A_k) X = diag(d_k)                     -- PLSSVD latent coefficients
