@@ -129,7 +129,10 @@ __global__ __launch_bounds__(256) void k_gemm_f64(bool transA, bool transB, int6
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[r][c] = 0.0;
 
-  for (int64_t k0 = kbeg; k0 < kend; k0 += G64_BK) {
+  // register prefetch: the loads of tile k0 + BK are in flight during the
+  // FMAs of tile k0
+  double ra[4], rb[4];
+  auto fetch = [&](int64_t k0) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int i, k;
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(256) void k_gemm_f64(bool transA, bool transB, int6
         if (!transA) v = A[gi + gk * lda] - (shA ? shA[gk] : 0.0);
         else v = A[gk + gi * lda] - (shA ? shA[gi] : 0.0);
       }
-      As[k][i] = v;
+      ra[r] = v;
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -155,9 +158,20 @@ __global__ __launch_bounds__(256) void k_gemm_f64(bool transA, bool transB, int6
         else if (!transB) v = B[gk + gj * ldb] - (shB ? shB[gj] : 0.0);
         else v = B[gj + gk * ldb] - (shB ? shB[gk] : 0.0);
       }
-      Bs[k][j] = v;
+      rb[r] = v;
+    }
+  };
+  if (kbeg < kend) fetch(kbeg);
+  for (int64_t k0 = kbeg; k0 < kend; k0 += G64_BK) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (!transA) As[t / 64 + 4 * r][t % 64] = ra[r];
+      else As[t % 16][t / 16 + 16 * r] = ra[r];
+      if (!transB) Bs[t % 16][t / 16 + 16 * r] = rb[r];
+      else Bs[t / 64 + 4 * r][t % 64] = rb[r];
     }
     __syncthreads();
+    if (k0 + G64_BK < kend) fetch(k0 + G64_BK);
 #pragma unroll
     for (int k = 0; k < G64_BK; ++k) {
       double a[4], b[4];
@@ -221,8 +235,10 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
   const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
   int64_t splits = 1;
   const int64_t want = 2 * ctx->num_cus;
-  if (tiles < want && K > 256) {
-    splits = std::min<int64_t>((want + tiles - 1) / tiles, (K + 255) / 256);
+  if (tiles < want && K > 2 * G64_BK) {
+    // small outputs: spread K over many workgroups (two tiles of K each at
+    // least), latency rather than FLOPs bounds these products
+    splits = std::min<int64_t>((want + tiles - 1) / tiles, (K + 2 * G64_BK - 1) / (2 * G64_BK));
     // keep the partial buffer modest
     const int64_t cap = std::max<int64_t>(1, (int64_t(64) << 20) / std::max<int64_t>(1, M * N));
     splits = std::max<int64_t>(1, std::min(splits, cap));

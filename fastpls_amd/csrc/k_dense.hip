@@ -67,12 +67,15 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     chol_ok = 1;
   }
   __syncthreads();
-  for (int idx = t; idx < k * k; idx += nt) {
-    const int i = idx % k, j = idx / k;
-    double v = Ain[i + (int64_t)j * lda];
-    if (i == j) v += shift;
-    L[i + j * LD] = v;
-    Z[i + j * LD] = (i == j) ? 1.0 : 0.0;
+  // 2-D thread map (no integer division in the loops): tx runs down a column
+  const int tx = t & 31, ty = t >> 5;
+  for (int j = ty; j < k; j += 32) {
+    for (int i = tx; i < k; i += 32) {
+      double v = Ain[i + (int64_t)j * lda];
+      if (i == j) v += shift;
+      L[i + j * LD] = v;
+      Z[i + j * LD] = (i == j) ? 1.0 : 0.0;
+    }
   }
   __syncthreads();
   // ---- Cholesky, lower triangle, one barrier per column
@@ -88,11 +91,10 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
       for (int i = j - 1 + t; i < k; i += nt) L[i + (j - 1) * LD] *= r;
     }
     prev_d = d;
-    const int m = k - j - 1;
     const double inv_d = 1.0 / d;
-    for (int idx = t; idx < m * m; idx += nt) {
-      const int ii = j + 1 + idx % m, jj = j + 1 + idx / m;
-      if (ii >= jj) L[ii + jj * LD] -= L[ii + j * LD] * L[jj + j * LD] * inv_d;
+    for (int jj = j + 1 + ty; jj < k; jj += 32) {
+      const double cj = L[jj + j * LD] * inv_d;
+      for (int ii = jj + tx; ii < k; ii += 32) L[ii + jj * LD] -= L[ii + j * LD] * cj;
     }
     __syncthreads();
   }
@@ -107,10 +109,10 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
         const double r = 1.0 / L[(q - 1) + (q - 1) * LD];
         for (int c = t; c < q; c += nt) Z[(q - 1) + c * LD] *= r;
       }
-      const int rows = k - q - 1, cols = q + 1;
-      for (int idx = t; idx < rows * cols; idx += nt) {
-        const int i = q + 1 + idx % rows, c = idx / rows;
-        Z[i + c * LD] -= L[i + q * LD] * (Z[q + c * LD] / lqq);
+      const double ilqq = 1.0 / lqq;
+      for (int c = ty; c <= q; c += 32) {
+        const double zq = Z[q + c * LD] * ilqq;
+        for (int i = q + 1 + tx; i < k; i += 32) Z[i + c * LD] -= L[i + q * LD] * zq;
       }
       __syncthreads();
     }
@@ -119,13 +121,42 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
       for (int c = t; c < k; c += nt) Z[(k - 1) + c * LD] *= r;
     }
     __syncthreads();
-    // ---- X = Z^T Z diag(rhs)
-    for (int idx = t; idx < k * k; idx += nt) {
-      const int i = idx % k, j = idx / k;
-      const int r0 = i > j ? i : j;
-      double acc = 0.0;
-      for (int r = r0; r < k; ++r) acc = fma(Z[r + i * LD], Z[r + j * LD], acc);
-      Xs[i + (int64_t)j * ldx] = (mode == 1) ? acc : acc * dvals[j];
+    // ---- X = Z^T Z diag(rhs), Z lower triangular.  Each wave owns a 32 x 32
+    // output block as an 8 x 8 grid of 4 x 4 register tiles, so a step over r
+    // reads 64 distinct Z values per wave (the rest are LDS broadcasts).
+    {
+      const int wv = t >> 6, ln = t & 63;
+      const int i0 = (wv & 3) * 32 + (ln & 7) * 4, j0 = (wv >> 2) * 32 + (ln >> 3) * 4;
+      const int wi0 = (wv & 3) * 32, wj0 = (wv >> 2) * 32;
+      if (wi0 < k && wj0 < k) {
+        double acc[4][4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
+        // rows below max(i, j) vanish in both factors
+        for (int r = wi0 > wj0 ? wi0 : wj0; r < k; ++r) {
+          double zi[4], zj[4];
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            zi[x] = i0 + x < k ? Z[r + (i0 + x) * LD] : 0.0;
+            zj[x] = j0 + x < k ? Z[r + (j0 + x) * LD] : 0.0;
+          }
+#pragma unroll
+          for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] = fma(zi[x], zj[y], acc[x][y]);
+        }
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          const int j = j0 + y;
+          if (j >= k) continue;
+          const double sc = (mode == 1) ? 1.0 : dvals[j];
+#pragma unroll
+          for (int x = 0; x < 4; ++x)
+            if (i0 + x < k) Xs[(i0 + x) + (int64_t)j * ldx] = acc[x][y] * sc;
+        }
+      }
     }
     if (t == 0 && status) status[s] = 0;
     return;

@@ -21,6 +21,13 @@
 #include <cmath>
 
 #include "fpls_internal.h"
+#include "fpls_wave.h"
+
+// Timing hook for stage-level measurements; compiled out unless a probe build
+// defines it (it then records a clock value from thread 0 at each call site).
+#ifndef FPLS_EIG_STAMP
+#define FPLS_EIG_STAMP(slot)
+#endif
 
 namespace fpls {
 
@@ -31,34 +38,20 @@ constexpr int ELD = EN + 1;       // LDS leading dimension of A in stage 1
 constexpr int ENT = 1024;         // threads
 constexpr double EEPS = DBL_EPSILON;
 
-__device__ __forceinline__ double sum8(double v) {
-  // butterfly over the 8-lane group: every step adds the same two operands on
-  // both partners, so all 8 lanes end with bitwise identical sums
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
+__device__ __forceinline__ double sum8(double v) { return grp8_sum(v); }
+// 1/d from the hardware reciprocal and two Newton steps (a few ulp; used only
+// where d is bounded away from zero: secular-equation terms and their ratios)
+__device__ __forceinline__ double rcp_nr(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
 }
-__device__ __forceinline__ double prod8(double v) {
-  v *= __shfl_xor(v, 1, 64);
-  v *= __shfl_xor(v, 2, 64);
-  v *= __shfl_xor(v, 4, 64);
-  return v;
-}
-__device__ __forceinline__ int isum8(int v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  v += __shfl_xor(v, 4, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
-}
-__device__ __forceinline__ double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-  return v;
-}
+__device__ __forceinline__ double prod8(double v) { return grp8_prod(v); }
+__device__ __forceinline__ int isum8(int v) { return grp8_isum(v); }
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_d(v); }
+__device__ __forceinline__ double wave_max(double v) { return wave_max_d(v); }
 
 // first index in sorted x[lo, hi) with x >= v (strict = false) or x > v (strict = true)
 __device__ __forceinline__ int bound(const double* x, int lo, int hi, double v, bool strict) {
@@ -83,9 +76,9 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
   __shared__ double big[EN * ELD];  // stage 1: A; stage 2: W (ld n); stage 3: reflectors (ld n)
   __shared__ double dv[EN], ev[EN], tauv[EN], pv[EN];
   __shared__ double lam[EN], Ds[EN], zs[EN], taus[EN], zh[EN], invn[EN], val[EN];
-  __shared__ double rc[EN], rs[EN];
+  __shared__ double rc[EN], rs[EN], dnc[EN], znc[EN], dorgv[EN];
   __shared__ int perm[EN], defl[EN], ndidx[EN], rootidx[EN], orig[EN], colpos[EN];
-  __shared__ int rp[EN], rq[EN];
+  __shared__ int rp[EN], rq[EN], iperm[EN];
   __shared__ int nrot[EN / 2], Kb[EN / 2];
   __shared__ double red[ENT / 64];
   __shared__ double s_scale;
@@ -116,61 +109,87 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
   if (t < EN) {
     tauv[t] = 0.0;
     perm[t] = t;  // every index array starts in range
+    iperm[t] = 0;
     colpos[t] = 0;
   }
   __syncthreads();
 
+  FPLS_EIG_STAMP(0);
   // ------------------------------------------------ stage 1: tridiagonalisation
+  // Only the lower triangle of the trailing block is kept current.  Wave 0
+  // updates the first trailing column itself and goes straight on to the next
+  // reflector, so a column costs two barriers.
+  auto householder = [&](int j) {  // wave 0 only
+    const int i0 = j + 1 + lane, i1 = j + 1 + lane + 64;
+    const double x0 = (i0 < n && i0 > j + 1) ? A[i0 + j * ELD] : 0.0;
+    const double x1 = (i1 < n) ? A[i1 + j * ELD] : 0.0;
+    const double xn2 = wave_sum(x0 * x0 + x1 * x1);
+    const double alpha = A[(j + 1) + j * ELD];
+    double beta = alpha, tau = 0.0, sv = 0.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
+      tau = (beta - alpha) / beta;
+      sv = 1.0 / (alpha - beta);
+    }
+    if (i0 < n && i0 > j + 1) A[i0 + j * ELD] = x0 * sv;
+    if (i1 < n) A[i1 + j * ELD] = x1 * sv;
+    if (lane == 0) {
+      dv[j] = A[j + j * ELD];
+      ev[j] = beta;
+      tauv[j] = tau;
+      A[(j + 1) + j * ELD] = 1.0;
+    }
+  };
+  if (wv == 0 && n >= 3) householder(0);
+  __syncthreads();
   for (int j = 0; j + 2 < n; ++j) {
-    const int m = n - 1 - j;
-    if (wv == 0) {
-      const int i0 = j + 1 + lane, i1 = j + 1 + lane + 64;
-      const double x0 = (i0 < n && i0 > j + 1) ? A[i0 + j * ELD] : 0.0;
-      const double x1 = (i1 < n) ? A[i1 + j * ELD] : 0.0;
-      const double xn2 = wave_sum(x0 * x0 + x1 * x1);
-      const double alpha = A[(j + 1) + j * ELD];
-      double beta = alpha, tau = 0.0, sv = 0.0;
-      if (xn2 > 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + xn2), alpha);
-        tau = (beta - alpha) / beta;
-        sv = 1.0 / (alpha - beta);
-      }
-      if (i0 < n && i0 > j + 1) A[i0 + j * ELD] = x0 * sv;
-      if (i1 < n) A[i1 + j * ELD] = x1 * sv;
-      if (lane == 0) {
-        dv[j] = A[j + j * ELD];
-        ev[j] = beta;
-        tauv[j] = tau;
-        A[(j + 1) + j * ELD] = 1.0;
-      }
-    }
-    __syncthreads();
     const double tau = tauv[j];
-    if (tau == 0.0) continue;  // H_j = I: A22 unchanged (uniform branch)
-    // p = tau A22 v, 8 lanes per row, rows read as columns (A22 symmetric)
-    {
-      const int i = j + 1 + (t >> 3), g = t & 7;
-      double s = 0.0;
-      if (i < n) {
-        for (int k = j + 1 + g; k < n; k += 8) s += A[k + i * ELD] * A[k + j * ELD];
+    if (tau != 0.0) {  // uniform branch; H_j = I leaves A22 unchanged
+      // p = tau A22 v, 8 lanes per row i: A22(i, k) is A(i, k) for k <= i
+      // (row i) and A(k, i) for k > i (column i)
+      {
+        const int i = j + 1 + (t >> 3), g = t & 7;
+        double s = 0.0;
+        if (i < n) {
+          int k = j + 1 + g;
+          for (; k <= i; k += 8) s += A[i + k * ELD] * A[k + j * ELD];
+          for (; k < n; k += 8) s += A[k + i * ELD] * A[k + j * ELD];
+        }
+        s = sum8(s);
+        if (i < n && g == 0) pv[i] = tau * s;
       }
-      s = sum8(s);
-      if (i < n && g == 0) pv[i] = tau * s;
-    }
-    __syncthreads();
-    {
-      // K = tau/2 p^T v (every wave computes the same butterfly sum)
+      __syncthreads();
+      // K = tau/2 p^T v (every wave computes the same sum)
       const int i0 = j + 1 + lane, i1 = j + 1 + lane + 64;
       double s = 0.0;
       if (i0 < n) s += pv[i0] * A[i0 + j * ELD];
       if (i1 < n) s += pv[i1] * A[i1 + j * ELD];
       const double K = 0.5 * tau * wave_sum(s);
-      for (int idx = t; idx < m * m; idx += ENT) {
-        const int i = j + 1 + idx % m, k = j + 1 + idx / m;
-        const double vi = A[i + j * ELD], vk = A[k + j * ELD];
-        const double wi = pv[i] - K * vi, wk = pv[k] - K * vk;
-        A[i + k * ELD] -= vi * wk + wi * vk;
+      if (wv == 0) {
+        // first trailing column (kk = j + 1), rows ii >= kk
+        const int kk = j + 1;
+        const double vk = 1.0, wk = pv[kk] - K;
+        for (int ii = kk + lane; ii < n; ii += 64) {
+          const double vi = A[ii + j * ELD], wi = pv[ii] - K * vi;
+          A[ii + kk * ELD] -= vi * wk + wi * vk;
+        }
+      } else {
+        // remaining columns kk >= j + 2: 30 groups of 32 lanes
+        const int u = t - 64, gx = u & 31, gy = u >> 5;
+        for (int kk = j + 2 + gy; kk < n; kk += 30) {
+          const double vk = A[kk + j * ELD], wk = pv[kk] - K * vk;
+          for (int ii = kk + gx; ii < n; ii += 32) {
+            const double vi = A[ii + j * ELD], wi = pv[ii] - K * vi;
+            A[ii + kk * ELD] -= vi * wk + wi * vk;
+          }
+        }
       }
+    }
+    if (wv == 0 && j + 3 < n) {
+      // the wave's own column writes must land before it reads them back
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      householder(j + 1);
     }
     __syncthreads();
   }
@@ -182,14 +201,18 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
     dv[n - 1] = A[(n - 1) + (n - 1) * ELD];
   }
   // reflectors to the workspace (stage 2 reuses the LDS)
-  for (int idx = t; idx < n * n; idx += ENT) {
-    const int i = idx % n, j = idx / n;
-    Vr[idx] = (j + 2 < n && i > j) ? A[i + j * ELD] : 0.0;
-    Qa[idx] = (i == j) ? 1.0 : 0.0;
-    Qb[idx] = 0.0;
+  const int tx = t & 31, ty = t >> 5;
+  for (int j = ty; j < n; j += 32) {
+    for (int i = tx; i < n; i += 32) {
+      const int idx = i + j * n;
+      Vr[idx] = (j + 2 < n && i > j) ? A[i + j * ELD] : 0.0;
+      Qa[idx] = (i == j) ? 1.0 : 0.0;
+      Qb[idx] = 0.0;
+    }
   }
   __syncthreads();
 
+  FPLS_EIG_STAMP(1);
   // ------------------------------------------------ stage 2: divide and conquer
   // 1x1 leaves after splitting T at every off-diagonal: T = blockdiag + sum of
   // |e_s| v_s v_s^T with v_s = e_s + sign(e_s) e_{s+1}
@@ -203,12 +226,12 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
   double* W = big;
   double* Qs = Qa;
   double* Qd = Qb;
-  for (int size = 1; size < n; size *= 2) {
-    const int bsz = 2 * size;
-    const int nblk = (n + bsz - 1) / bsz;
+  for (int lg = 1; (1 << (lg - 1)) < n; ++lg) {
+    const int size = 1 << (lg - 1), bsz = 1 << lg;
+    const int nblk = (n + bsz - 1) >> lg;
     // (a) sorted merge of the two halves, z from the boundary rows of Q
     if (t < n) {
-      const int bi = t / bsz, a = bi * bsz, m = a + size, b = min(a + bsz, n);
+      const int a = (t >> lg) << lg, m = a + size, b = min(a + bsz, n);
       if (m < n) {
         const double beta = ev[m - 1];
         const double sg = beta >= 0.0 ? 1.0 : -1.0;
@@ -225,279 +248,274 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
         Ds[a + rank] = v;
         zs[a + rank] = z * 0.70710678118654752440;
         perm[a + rank] = t;
+        iperm[t] = rank;
       }
     }
     __syncthreads();
-    // (b) deflation, one thread per merge (serial scan in sorted order)
+    FPLS_EIG_STAMP(2);
+    // (b) deflation, one thread per merge (serial scan in sorted order);
+    //     the K surviving poles and weights are packed into dnc / znc
     if (t < nblk) {
-      const int a = t * bsz, m = a + size, b = min(a + bsz, n), mm = b - a;
+      const int a = t << lg, m = a + size, b = min(a + bsz, n), mm = b - a;
       int K = 0, nr = 0;
       if (m < n) {
         const double rho2 = 2.0 * fabs(ev[m - 1]);
         double dmax = 0.0;
         for (int q = 0; q < mm; ++q) dmax = fmax(dmax, fabs(Ds[a + q]));
         const double tol = 8.0 * EEPS * fmax(dmax, rho2);
+        // the state of the current candidate (prev) stays in registers; entries
+        // q are read before anything writes them, so the loads pipeline
         int prev = -1;
+        double dprev = 0.0, zprev = 0.0;
         for (int q = 0; q < mm; ++q) {
+          const double dq = Ds[a + q], zq = zs[a + q];
           defl[a + q] = 0;
-          if (rho2 * fabs(zs[a + q]) <= tol) {
+          if (rho2 * fabs(zq) <= tol) {
             defl[a + q] = 1;
             continue;
           }
-          if (prev < 0) {
-            prev = q;
-            continue;
-          }
-          double s = zs[a + prev], c = zs[a + q];
-          const double tt = hypot(s, c), dd = Ds[a + q] - Ds[a + prev];
-          c /= tt;
-          s = -s / tt;
-          if (fabs(dd * c * s) <= tol) {
-            zs[a + q] = tt;
-            zs[a + prev] = 0.0;
-            const double dp = Ds[a + prev], dq = Ds[a + q];
-            Ds[a + prev] = c * c * dp + s * s * dq;
-            Ds[a + q] = s * s * dp + c * c * dq;
-            defl[a + prev] = 1;
-            rp[a + nr] = prev;
-            rq[a + nr] = q;
-            rc[a + nr] = c;
-            rs[a + nr] = s;
-            ++nr;
-          }
-          prev = q;
-        }
-        for (int q = 0; q < mm; ++q) {
-          if (!defl[a + q]) {
-            ndidx[a + K] = q;
-            rootidx[a + q] = K;
+          if (prev >= 0) {
+            const double tt = sqrt(zprev * zprev + zq * zq), it = 1.0 / tt;
+            const double c = zq * it, sn = -zprev * it;
+            if (fabs((dq - dprev) * c * sn) <= tol) {
+              // rotate (prev, q): z_prev -> 0, z_q -> tt; prev deflates
+              const double dp2 = c * c * dprev + sn * sn * dq;
+              const double dq2 = sn * sn * dprev + c * c * dq;
+              Ds[a + prev] = dp2;
+              zs[a + prev] = 0.0;
+              defl[a + prev] = 1;
+              rp[a + nr] = prev;
+              rq[a + nr] = q;
+              rc[a + nr] = c;
+              rs[a + nr] = sn;
+              ++nr;
+              Ds[a + q] = dq2;
+              zs[a + q] = tt;
+              prev = q;
+              dprev = dq2;
+              zprev = tt;
+              continue;
+            }
+            // prev survives
+            ndidx[a + K] = prev;
+            rootidx[a + prev] = K;
+            dnc[a + K] = dprev;
+            znc[a + K] = zprev;
             ++K;
           }
+          prev = q;
+          dprev = dq;
+          zprev = zq;
+        }
+        if (prev >= 0) {
+          ndidx[a + K] = prev;
+          rootidx[a + prev] = K;
+          dnc[a + K] = dprev;
+          znc[a + K] = zprev;
+          ++K;
         }
       }
       Kb[t] = K;
       nrot[t] = nr;
     }
     __syncthreads();
+    const int slot = t >> 3, g = t & 7;
+    const int sbi = slot >> lg, sa = sbi << lg, sm = sa + size;
+    const int sj = slot - sa;
+    const bool spair = slot < n && sm < n;
+    const int sK = spair ? Kb[sbi] : 0;
+    const double* Dn = dnc + sa;
+    const double* Zn = znc + sa;
+    FPLS_EIG_STAMP(3);
     // (c) secular roots, 8 lanes per root
-    {
-      const int slot = t >> 3, g = t & 7;
-      const int bi = slot / bsz, a = bi * bsz, m = a + size;
-      const int j = slot - a;
-      const bool act = slot < n && m < n && j < Kb[bi];
-      if (act) {
-        const int K = Kb[bi];
-        const double rho2 = 2.0 * fabs(ev[m - 1]);
-        auto Dn = [&](int k) { return Ds[a + ndidx[a + k]]; };
-        auto Zn = [&](int k) { return zs[a + ndidx[a + k]]; };
-        int o;
-        double lo, hi;
-        const double dj = Dn(j);
-        if (j < K - 1) {
-          const double delta = 0.5 * (Dn(j + 1) - dj);
-          double s = 0.0;
-          for (int k = g; k < K; k += 8) {
-            const double zk = Zn(k);
-            s += zk * zk / ((Dn(k) - dj) - delta);
-          }
-          const double f = 1.0 + rho2 * sum8(s);
-          if (f >= 0.0) {
-            o = j;
-            lo = 0.0;
-            hi = delta;
-          } else {
-            o = j + 1;
-            lo = -delta;
-            hi = 0.0;
-          }
-        } else {
-          double s = 0.0;
-          for (int k = g; k < K; k += 8) {
-            const double zk = Zn(k);
-            s += zk * zk;
-          }
+    if (spair && sj < sK) {
+      const int K = sK, j = sj;
+      const double rho2 = 2.0 * fabs(ev[sm - 1]);
+      int o;
+      double lo, hi;
+      const double dj = Dn[j];
+      if (j < K - 1) {
+        const double delta = 0.5 * (Dn[j + 1] - dj);
+        double s = 0.0;
+        for (int k = g; k < K; k += 8) s += Zn[k] * Zn[k] * rcp_nr((Dn[k] - dj) - delta);
+        const double f = 1.0 + rho2 * sum8(s);
+        if (f >= 0.0) {
           o = j;
           lo = 0.0;
-          hi = rho2 * sum8(s);
+          hi = delta;
+        } else {
+          o = j + 1;
+          lo = -delta;
+          hi = 0.0;
         }
-        const double dorg = Dn(o);
-        double x = 0.5 * (lo + hi);
-        for (int it = 0; it < 200; ++it) {
-          double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, eab = 0.0;
-          for (int k = g; k < K; k += 8) {
-            const double zk = Zn(k);
-            const double den = (Dn(k) - dorg) - x;
-            const double r = zk / den;
-            const double zr = zk * r, rr = r * r;
-            if (k <= j) {
-              psi += zr;
-              dpsi += rr;
-            } else {
-              phi += zr;
-              dphi += rr;
-            }
-            eab += fabs(zr);
-          }
-          psi = rho2 * sum8(psi);
-          dpsi = rho2 * sum8(dpsi);
-          phi = rho2 * sum8(phi);
-          dphi = rho2 * sum8(dphi);
-          eab = rho2 * sum8(eab);
-          const double f = 1.0 + psi + phi;
-          if (fabs(f) <= 2.0 * EEPS * (1.0 + eab)) break;
-          if (f < 0.0) lo = x;
-          else hi = x;
-          if (hi - lo <= 2.0 * EEPS * fmax(fabs(lo), fabs(hi))) break;
-          // rational model: psi ~ a1 + b1/(dl - h), phi ~ a2 + b2/(dr - h)
-          const double dl = (dj - dorg) - x;
-          const double b1 = dpsi * dl * dl, a1 = psi - b1 / dl;
-          double h;
-          if (j < K - 1) {
-            const double dr = (Dn(j + 1) - dorg) - x;
-            const double b2 = dphi * dr * dr, a2 = phi - b2 / dr;
-            const double c = 1.0 + a1 + a2;
-            const double qb = -(c * (dl + dr) + b1 + b2);
-            const double qc = c * dl * dr + b1 * dr + b2 * dl;
-            if (c == 0.0) {
-              h = qb != 0.0 ? -qc / qb : NAN;
-            } else {
-              const double disc = fmax(qb * qb - 4.0 * c * qc, 0.0);
-              const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
-              const double h1 = q / c, h2 = q != 0.0 ? qc / q : NAN;
-              h = (h1 > dl && h1 < dr) ? h1 : h2;
-            }
-          } else {
-            const double c = 1.0 + a1 + phi;
-            h = c != 0.0 ? dl + b1 / c : NAN;
-          }
-          double xn = x + h;
-          if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-          x = xn;
-        }
-        if (g == 0) {
-          orig[a + j] = o;
-          taus[a + j] = x;
-        }
+      } else {
+        double s = 0.0;
+        for (int k = g; k < K; k += 8) s += Zn[k] * Zn[k];
+        o = j;
+        lo = 0.0;
+        hi = rho2 * sum8(s);
       }
-    }
-    __syncthreads();
-    // (d) Gu-Eisenstat z-hat, 8 lanes per non-deflated index
-    {
-      const int slot = t >> 3, g = t & 7;
-      const int bi = slot / bsz, a = bi * bsz, m = a + size;
-      const int i = slot - a;
-      const bool act = slot < n && m < n && i < Kb[bi];
-      if (act) {
-        const int K = Kb[bi];
-        const double rho2 = 2.0 * fabs(ev[m - 1]);
-        const double di = Ds[a + ndidx[a + i]];
-        auto Dn = [&](int k) { return Ds[a + ndidx[a + k]]; };
-        // -delta(i, j) = lam_j - d_i = (D_o(j) - d_i) + tau_j
-        auto lmd = [&](int jj) { return (Dn(orig[a + jj]) - di) + taus[a + jj]; };
-        double pr = 1.0;
+      const double dorg = Dn[o];
+      double x = 0.5 * (lo + hi);
+      for (int it = 0; it < 200; ++it) {
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0, eab = 0.0;
         for (int k = g; k < K; k += 8) {
-          if (k < i) pr *= lmd(k) / (Dn(k) - di);
-          else if (k > i) pr *= lmd(k - 1) / (Dn(k) - di);
+          const double zk = Zn[k];
+          const double r = zk * rcp_nr((Dn[k] - dorg) - x);
+          const double zr = zk * r, rr = r * r;
+          if (k <= j) {
+            psi += zr;
+            dpsi += rr;
+          } else {
+            phi += zr;
+            dphi += rr;
+          }
+          eab += fabs(zr);
         }
-        pr = prod8(pr) * (lmd(K - 1) / rho2);
-        if (g == 0) zh[a + i] = copysign(sqrt(fmax(pr, 0.0)), zs[a + ndidx[a + i]]);
+        psi = rho2 * sum8(psi);
+        dpsi = rho2 * sum8(dpsi);
+        phi = rho2 * sum8(phi);
+        dphi = rho2 * sum8(dphi);
+        eab = rho2 * sum8(eab);
+        const double f = 1.0 + psi + phi;
+        if (fabs(f) <= 2.0 * EEPS * (1.0 + eab)) break;
+        if (f < 0.0) lo = x;
+        else hi = x;
+        if (hi - lo <= 2.0 * EEPS * fmax(fabs(lo), fabs(hi))) break;
+        // rational model: psi ~ a1 + b1/(dl - h), phi ~ a2 + b2/(dr - h)
+        const double dl = (dj - dorg) - x;
+        const double b1 = dpsi * dl * dl, a1 = psi - b1 / dl;
+        double h;
+        if (j < K - 1) {
+          const double dr = (Dn[j + 1] - dorg) - x;
+          const double b2 = dphi * dr * dr, a2 = phi - b2 / dr;
+          const double c = 1.0 + a1 + a2;
+          const double qb = -(c * (dl + dr) + b1 + b2);
+          const double qc = c * dl * dr + b1 * dr + b2 * dl;
+          if (c == 0.0) {
+            h = qb != 0.0 ? -qc / qb : NAN;
+          } else {
+            const double disc = fmax(qb * qb - 4.0 * c * qc, 0.0);
+            const double q = -0.5 * (qb + copysign(sqrt(disc), qb));
+            const double h1 = q / c, h2 = q != 0.0 ? qc / q : NAN;
+            h = (h1 > dl && h1 < dr) ? h1 : h2;
+          }
+        } else {
+          const double c = 1.0 + a1 + phi;
+          h = c != 0.0 ? dl + b1 / c : NAN;
+        }
+        double xn = x + h;
+        if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+        x = xn;
+      }
+      if (g == 0) {
+        orig[sa + j] = o;
+        taus[sa + j] = x;
+        dorgv[sa + j] = dorg;
       }
     }
     __syncthreads();
-    // (e) eigenvector norms and output eigenvalues (sorted-input index space)
-    {
-      const int slot = t >> 3, g = t & 7;
-      const int bi = slot / bsz, a = bi * bsz, m = a + size, b = min(a + bsz, n);
-      const int jr = slot - a;
-      if (slot < n && m < n) {
-        const int K = Kb[bi];
-        if (jr < K) {
-          const double dorg = Ds[a + ndidx[a + orig[a + jr]]];
-          const double tj = taus[a + jr];
-          double s = 0.0;
-          for (int i = g; i < K; i += 8) {
-            const double del = (Ds[a + ndidx[a + i]] - dorg) - tj;
-            const double u = zh[a + i] / del;
-            s += u * u;
-          }
-          s = sum8(s);
-          if (g == 0) {
-            invn[a + jr] = 1.0 / sqrt(s);
-            val[a + ndidx[a + jr]] = dorg + tj;
-          }
-        }
-        if (g == 0 && jr < b - a && defl[a + jr]) val[a + jr] = Ds[a + jr];
+    FPLS_EIG_STAMP(4);
+    // (d) Gu-Eisenstat z-hat, 8 lanes per surviving pole i:
+    //     zhat_i^2 = prod_j (lam_j - d_i) / (rho prod_{k != i} (d_k - d_i))
+    if (spair && sj < sK) {
+      const int K = sK, i = sj;
+      const double rho2 = 2.0 * fabs(ev[sm - 1]);
+      const double di = Dn[i];
+      // lam_j - d_i = (D_o(j) - d_i) + tau_j
+      double pr = 1.0;
+      for (int k = g; k < K; k += 8) {
+        if (k < i) pr *= ((dorgv[sa + k] - di) + taus[sa + k]) * rcp_nr(Dn[k] - di);
+        else if (k > i) pr *= ((dorgv[sa + k - 1] - di) + taus[sa + k - 1]) * rcp_nr(Dn[k] - di);
       }
+      pr = prod8(pr) * (((dorgv[sa + K - 1] - di) + taus[sa + K - 1]) / rho2);
+      if (g == 0) zh[sa + i] = copysign(sqrt(fmax(pr, 0.0)), Zn[i]);
+    }
+    __syncthreads();
+    FPLS_EIG_STAMP(5);
+    // (e) eigenvector norms and output eigenvalues (sorted-input index space)
+    if (spair) {
+      if (sj < sK) {
+        const double dorg = dorgv[sa + sj], tj = taus[sa + sj];
+        double s = 0.0;
+        for (int i = g; i < sK; i += 8) {
+          const double u = zh[sa + i] * rcp_nr((Dn[i] - dorg) - tj);
+          s += u * u;
+        }
+        s = sum8(s);
+        if (g == 0) {
+          invn[sa + sj] = 1.0 / sqrt(s);
+          val[sa + ndidx[sa + sj]] = dorg + tj;
+        }
+      }
+      if (g == 0 && sa + sj < n && defl[sa + sj]) val[sa + sj] = Ds[sa + sj];
     }
     __syncthreads();
     // (f) output order (ascending, ties by index) and the small matrix W
-    {
-      const int slot = t >> 3, g = t & 7;
-      const int bi = slot / bsz, a = bi * bsz, m = a + size, b = min(a + bsz, n);
-      const int c = slot - a, mm = b - a;
-      if (slot < n && m < n) {
-        const double vc = val[a + c];
-        int r = 0;
-        for (int k = g; k < mm; k += 8) {
-          const double vk = val[a + k];
-          r += (vk < vc || (vk == vc && k < c)) ? 1 : 0;
-        }
-        r = isum8(r);
-        if (g == 0) colpos[a + c] = r;
+    if (spair) {
+      const int mm = min(sa + bsz, n) - sa;
+      const double vc = val[slot];
+      int r = 0;
+      for (int k = g; k < mm; k += 8) {
+        const double vk = val[sa + k];
+        r += (vk < vc || (vk == vc && k < sj)) ? 1 : 0;
       }
+      r = isum8(r);
+      if (g == 0) colpos[slot] = r;
     }
-    for (int idx = t; idx < n * n; idx += ENT) {
-      const int q = idx % n, c = idx / n;  // W(q, c), block-local indices stored at (a+q, a+c)
-      const int bi = q / bsz;
-      if (c / bsz != bi) continue;
-      const int a = bi * bsz, m = a + size;
-      if (m >= n) continue;
-      const int ql = q - a, cl = c - a;
+    for (int idx = t; idx < (n << lg); idx += ENT) {
+      const int c = idx >> lg, ql = idx & (bsz - 1);
+      const int a = (c >> lg) << lg, q = a + ql;
+      if (q >= n || a + size >= n) continue;
+      const int cl = c - a;
       double wqc;
-      if (defl[a + cl]) {
+      if (defl[c]) {
         wqc = (ql == cl) ? 1.0 : 0.0;
-      } else if (defl[a + ql]) {
+      } else if (defl[q]) {
         wqc = 0.0;
       } else {
-        const int jr = rootidx[a + cl], i = rootidx[a + ql];
-        const double dorg = Ds[a + ndidx[a + orig[a + jr]]];
-        const double del = (Ds[a + ql] - dorg) - taus[a + jr];
-        wqc = zh[a + i] / del * invn[a + jr];
+        const int jr = rootidx[c], i = rootidx[q];
+        wqc = zh[a + i] * rcp_nr((Ds[q] - dorgv[a + jr]) - taus[a + jr]) * invn[a + jr];
       }
       W[q + c * n] = wqc;
     }
     __syncthreads();
+    FPLS_EIG_STAMP(6);
     // (g) deflation rotations folded into W: W <- G_1 (G_2 (... W)), rows p, q
     {
       int maxr = 0;
       for (int k = 0; k < nblk; ++k) maxr = max(maxr, nrot[k]);
       for (int r = maxr - 1; r >= 0; --r) {
-        for (int idx = t; idx < n; idx += ENT) {
-          const int bi = idx / bsz, a = bi * bsz, m = a + size;
-          if (m >= n || r >= nrot[bi]) continue;
-          const int c = idx;  // column of W
-          const int p = a + rp[a + r], q = a + rq[a + r];
-          const double cs = rc[a + r], sn = rs[a + r];
-          const double wp = W[p + c * n], wq = W[q + c * n];
-          W[p + c * n] = cs * wp - sn * wq;
-          W[q + c * n] = sn * wp + cs * wq;
+        if (t < n) {
+          const int bi = t >> lg, a = bi << lg;
+          if (a + size < n && r < nrot[bi]) {
+            const int c = t;  // column of W
+            const int p = a + rp[a + r], q = a + rq[a + r];
+            const double cs = rc[a + r], sn = rs[a + r];
+            const double wp = W[p + c * n], wq = W[q + c * n];
+            W[p + c * n] = cs * wp - sn * wq;
+            W[q + c * n] = sn * wp + cs * wq;
+          }
         }
         __syncthreads();
       }
     }
+    FPLS_EIG_STAMP(7);
     // (h) Q_new(:, a + colpos(c)) = Q(:, perm) W(:, c) over the rows of the block;
-    //     unpaired trailing blocks are copied
+    //     unpaired trailing blocks are copied.  Two inner steps per batch of
+    //     global loads keep several L2 requests in flight per thread.
     {
-      const int tpd = (bsz + 3) / 4;  // 4x4 tiles per dimension of a full block
+      const int tpd = (bsz + 3) >> 2;  // 4x4 tiles per dimension of a full block
       const int ntile = nblk * tpd * tpd;
       for (int tile = t; tile < ntile; tile += ENT) {
-        const int bi = tile / (tpd * tpd), tt = tile % (tpd * tpd);
-        const int a = bi * bsz, m = a + size, b = min(a + bsz, n), mm = b - a;
+        const int bi = tile / (tpd * tpd), tt = tile - bi * (tpd * tpd);
+        const int a = bi << lg, m = a + size, b = min(a + bsz, n), mm = b - a;
         const int r0 = (tt % tpd) * 4, c0 = (tt / tpd) * 4;
         if (r0 >= mm || c0 >= mm) continue;
+        const int rlim = min(4, mm - r0), clim = min(4, mm - c0);
         if (m >= n) {
-          for (int cc = c0; cc < min(c0 + 4, mm); ++cc)
-            for (int rr = r0; rr < min(r0 + 4, mm); ++rr)
+          for (int cc = c0; cc < c0 + clim; ++cc)
+            for (int rr = r0; rr < r0 + rlim; ++rr)
               Qd[(a + rr) + (int64_t)(a + cc) * n] = Qs[(a + rr) + (int64_t)(a + cc) * n];
           continue;
         }
@@ -506,18 +524,30 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
         for (int x = 0; x < 4; ++x)
 #pragma unroll
           for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
-        const int rlim = min(4, mm - r0), clim = min(4, mm - c0);
-        for (int q = 0; q < mm; ++q) {
-          const double* col = Qs + (int64_t)perm[a + q] * n + a + r0;
-          double qv[4], wvv[4];
+        // Q is block diagonal over the two halves: rows of one half only meet
+        // the columns of that half.  Column k of Q pairs with row iperm(k) of W.
+        const int ra = a + r0, rz = a + r0 + rlim;
+        const int kb = ra >= m ? m : a, ke = rz <= m ? m : b;
+        for (int k0 = kb; k0 < ke; k0 += 2) {
+          double qv[2][4];
 #pragma unroll
-          for (int x = 0; x < 4; ++x) qv[x] = x < rlim ? col[x] : 0.0;
+          for (int u = 0; u < 2; ++u) {
+            const int k = min(k0 + u, ke - 1);
+            const double* col = Qs + (int64_t)k * n + ra;
 #pragma unroll
-          for (int y = 0; y < 4; ++y) wvv[y] = y < clim ? W[(a + q) + (a + c0 + y) * n] : 0.0;
+            for (int x = 0; x < 4; ++x) qv[u][x] = (x < rlim && k0 + u < ke) ? col[x] : 0.0;
+          }
 #pragma unroll
-          for (int x = 0; x < 4; ++x)
+          for (int u = 0; u < 2; ++u) {
+            const int q = a + iperm[min(k0 + u, ke - 1)];
+            double wvv[4];
 #pragma unroll
-            for (int y = 0; y < 4; ++y) acc[x][y] += qv[x] * wvv[y];
+            for (int y = 0; y < 4; ++y) wvv[y] = y < clim ? W[q + (a + c0 + y) * n] : 0.0;
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+              for (int y = 0; y < 4; ++y) acc[x][y] += qv[u][x] * wvv[y];
+          }
         }
         for (int y = 0; y < clim; ++y) {
           const int oc = a + colpos[a + c0 + y];
@@ -528,8 +558,8 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
     __syncthreads();
     // eigenvalues of the merged blocks in ascending order
     if (t < n) {
-      const int bi = t / bsz, a = bi * bsz, m = a + size;
-      if (m < n) lam[a + colpos[t]] = val[t];
+      const int a = (t >> lg) << lg;
+      if (a + size < n) lam[a + colpos[t]] = val[t];
     }
     {
       double* tmp = Qs;
@@ -537,8 +567,10 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
       Qd = tmp;
     }
     __syncthreads();
+    FPLS_EIG_STAMP(8);
   }
 
+  FPLS_EIG_STAMP(9);
   // ------------------------------------------------ stage 3: back-transformation
   double* R = big;
   for (int idx = t; idx < n * n; idx += ENT) R[idx] = Vr[idx];
@@ -556,21 +588,23 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
         z[ii] = r < n ? Qs[r + (int64_t)c * n] : 0.0;
       }
     }
+    // reflector rows <= j are stored as zeros, so only rows >= n are masked
+    bool ok[RPL];
+#pragma unroll
+    for (int ii = 0; ii < RPL; ++ii) ok[ii] = g + 8 * ii < n;
     for (int j = n - 3; j >= 0; --j) {
       const double tj = tauv[j];
       if (tj == 0.0) continue;
-      double s = 0.0;
+      const double* Rj = R + j * n + g;
+      double rv[RPL];
 #pragma unroll
-      for (int ii = 0; ii < RPL; ++ii) {
-        const int r = g + 8 * ii;
-        if (r > j && r < n) s += R[r + j * n] * z[ii];
-      }
-      s = tj * sum8(s);
+      for (int ii = 0; ii < RPL; ++ii) rv[ii] = ok[ii] ? Rj[8 * ii] : 0.0;
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int ii = 0; ii < RPL; ++ii) {
-        const int r = g + 8 * ii;
-        if (r > j && r < n) z[ii] -= R[r + j * n] * s;
-      }
+      for (int ii = 0; ii < RPL; ++ii) s4[ii & 3] = fma(rv[ii], z[ii], s4[ii & 3]);
+      const double s = tj * sum8((s4[0] + s4[1]) + (s4[2] + s4[3]));
+#pragma unroll
+      for (int ii = 0; ii < RPL; ++ii) z[ii] = fma(-rv[ii], s, z[ii]);
     }
     if (c < n) {
       const int oc = n - 1 - c;
@@ -582,6 +616,7 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
       if (g == 0) w[oc] = lam[c] * scale;
     }
   }
+  FPLS_EIG_STAMP(10);
 }
 
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
