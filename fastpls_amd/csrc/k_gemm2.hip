@@ -433,6 +433,191 @@ __global__ void k_g2_reduce(int64_t M, int64_t N, int splits, const float* __res
   }
 }
 
+// ---------------------------------------------------------------- skinny NN
+// T (M x N, N <= 128) = (X - 1 shA^T) B with X streamed straight from HBM into
+// MFMA operand registers (no LDS for the big operand).  Workgroup = 4 waves x
+// 32 rows; each wave holds 2 x ceil(N/16) accumulator blocks.  B (K x N, the
+// zero-padded copy) goes through LDS one 32-deep K chunk at a time (double
+// buffered, one barrier per chunk); the X operand of chunk c+1 is loaded into
+// registers while the MFMAs of chunk c run.
+constexpr int P_BM = 128, P_BK = 32, P_NT = 256;
+constexpr int P_LDB = P_BK + 8;  // bf16 row length of the [n][k] B tiles
+template <bool BF>
+struct PSmem {
+  static constexpr int value = BF ? (2 * 2 * 128 * P_LDB) / 2 : 2 * P_BK * (128 + 4);
+};
+
+template <bool BF, bool CEN>
+__global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
+                                               const float* A, int64_t lda,
+                                               const float* shA,
+                                               const float* B, int64_t ldb,
+                                               const double* __restrict__ off,
+                                               float* __restrict__ Cf, double* __restrict__ Cd,
+                                               int64_t ldc, int crow) {
+  __shared__ __align__(16) float smem[PSmem<BF>::value];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t row0 = (int64_t)blockIdx.x * P_BM + w * 32;
+  const int nb = (N + 15) >> 4;
+  v4f acc[2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  const float* arow0 = A + std::min<int64_t>(row0 + lane % 16, M - 1);
+  const float* arow1 = A + std::min<int64_t>(row0 + 16 + lane % 16, M - 1);
+  // k offset (within a chunk) of operand element j of this lane
+  auto kof = [&](int j) { return BF ? 8 * (lane / 16) + j : 4 * j + lane / 16; };
+  // the centring shifts arrive as FP32 (converted once per call): any
+  // arithmetic on a value at load time would wait on every outstanding load
+  float ac0[8], ac1[8], an0[8], an1[8], mc[8], mn[8];
+  auto fetchA = [&](int64_t k0, float (&x0)[8], float (&x1)[8], float (&mx)[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int64_t gk = std::min<int64_t>(k0 + kof(j), K - 1);
+      x0[j] = arow0[gk * lda];
+      x1[j] = arow1[gk * lda];
+      mx[j] = CEN ? shA[gk] : 0.f;
+    }
+  };
+  // B chunk staging: thread -> column n = t / 2, 16 consecutive k
+  const int bn = t >> 1, bk = (t & 1) * 16;
+  float4 bq[4];
+  auto fetchB = [&](int64_t k0) {
+    const float* src = B + (k0 + bk) + (int64_t)bn * ldb;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = *reinterpret_cast<const float4*>(src + 4 * q);
+  };
+  auto storeB = [&](int buf) {
+    if (BF) {
+      __bf16* base = reinterpret_cast<__bf16*>(smem);
+      __bf16* Bh = base + (0 * 2 + buf) * 128 * P_LDB;
+      __bf16* Bl = base + (1 * 2 + buf) * 128 * P_LDB;
+      v8bf h0, l0, h1, l1;
+      const float v[16] = {bq[0].x, bq[0].y, bq[0].z, bq[0].w, bq[1].x, bq[1].y, bq[1].z, bq[1].w,
+                           bq[2].x, bq[2].y, bq[2].z, bq[2].w, bq[3].x, bq[3].y, bq[3].z, bq[3].w};
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 hh, ll;
+        split_bf(v[e], hh, ll);
+        h0[e] = hh;
+        l0[e] = ll;
+        split_bf(v[8 + e], hh, ll);
+        h1[e] = hh;
+        l1[e] = ll;
+      }
+      *reinterpret_cast<v8bf*>(Bh + bn * P_LDB + bk) = h0;
+      *reinterpret_cast<v8bf*>(Bh + bn * P_LDB + bk + 8) = h1;
+      *reinterpret_cast<v8bf*>(Bl + bn * P_LDB + bk) = l0;
+      *reinterpret_cast<v8bf*>(Bl + bn * P_LDB + bk + 8) = l1;
+    } else {
+      float* Bs = smem + buf * P_BK * (128 + 4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        Bs[(bk + 4 * q + 0) * (128 + 4) + bn] = bq[q].x;
+        Bs[(bk + 4 * q + 1) * (128 + 4) + bn] = bq[q].y;
+        Bs[(bk + 4 * q + 2) * (128 + 4) + bn] = bq[q].z;
+        Bs[(bk + 4 * q + 3) * (128 + 4) + bn] = bq[q].w;
+      }
+    }
+  };
+  auto compute = [&](int buf) {
+    if (BF) {
+      const __bf16* base = reinterpret_cast<const __bf16*>(smem);
+      const __bf16* Bh = base + (0 * 2 + buf) * 128 * P_LDB;
+      const __bf16* Bl = base + (1 * 2 + buf) * 128 * P_LDB;
+      v8bf ah0, al0, ah1, al1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 hh, ll;
+        const float me = mc[e];
+        split_bf(ac0[e] - me, hh, ll);
+        ah0[e] = hh;
+        al0[e] = ll;
+        split_bf(ac1[e] - me, hh, ll);
+        ah1[e] = hh;
+        al1[e] = ll;
+      }
+      const int ko = 8 * (lane / 16);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b < nb) {
+          const int n = b * 16 + lane % 16;
+          const v8bf bh = *reinterpret_cast<const v8bf*>(Bh + n * P_LDB + ko);
+          const v8bf bl = *reinterpret_cast<const v8bf*>(Bl + n * P_LDB + ko);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al0, bh, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah0, bl, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah0, bh, acc[0][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al1, bh, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah1, bl, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah1, bh, acc[1][b], 0, 0, 0);
+        }
+      }
+    } else {
+      const float* Bs = smem + buf * P_BK * (128 + 4);
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const float ms = mc[s];
+        const float x0 = ac0[s] - ms, x1 = ac1[s] - ms;
+        const int kr = 4 * s + lane / 16;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (b < nb) {
+            const float bv = Bs[kr * (128 + 4) + b * 16 + lane % 16];
+            acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, bv, acc[0][b], 0, 0, 0);
+            acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, bv, acc[1][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  const int nk = (int)((K + P_BK - 1) / P_BK);
+  fetchA(0, ac0, ac1, mc);
+  fetchB(0);
+  storeB(0);
+  __syncthreads();
+  for (int c = 0; c < nk; ++c) {
+    const bool more = c + 1 < nk;
+    if (more) {
+      fetchA((int64_t)(c + 1) * P_BK, an0, an1, mn);
+      fetchB((int64_t)(c + 1) * P_BK);
+    }
+    // keep the next chunk's loads ahead of this chunk's MFMAs
+    __builtin_amdgcn_sched_barrier(0);
+    compute(c & 1);
+    if (more) {
+      storeB((c + 1) & 1);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        ac0[j] = an0[j];
+        ac1[j] = an1[j];
+        mc[j] = mn[j];
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue: lane holds D[4(l/16) + r][l%16] of each 16 x 16 block
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int col = b * 16 + lane % 16;
+      if (b >= nb || col >= N) continue;
+      const double o = off ? off[col] : 0.0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = rbase + r;
+        if (row >= M) continue;
+        const double v = static_cast<double>(acc[a][b][r]) - o;
+        const int64_t idx = crow ? row * ldc + col : row + (int64_t)col * ldc;
+        if (Cd) Cd[idx] = v;
+        else Cf[idx] = static_cast<float>(v);
+      }
+    }
+  }
+}
+
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
@@ -449,6 +634,28 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   ldb = Kp;
   // centred contractions (projections of X) are timed as "project_*"
   const bool bf = precision == FPLS_PREC_BF16X3;
+  if (N <= 128) {
+    dim3 pgrid((unsigned)((M + P_BM - 1) / P_BM));
+    const int n32 = (int)N;
+    if (shiftA) {
+      DBuf<float> shf(ctx, (size_t)K);
+      f64_to_f32(ctx, shf.get(), shiftA, K);
+      if (bf)
+        launch(ctx, "project_bf16x3", k_proj<true, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow);
+      else
+        launch(ctx, "project_f32", k_proj<false, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow);
+    } else {
+      if (bf)
+        launch(ctx, "gemm_nn_bf16x3", k_proj<true, false>, pgrid, dim3(P_NT), 0, M, n32, K, A,
+               lda, (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow);
+      else
+        launch(ctx, "gemm_nn_f32", k_proj<false, false>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+               (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow);
+    }
+    return;
+  }
   if (shiftA) {
     if (bf)
       launch(ctx, "project_bf16x3", k_g2_nn<true, true>, grid, dim3(G2_NT), 0, M, N, K, A, lda,
@@ -466,12 +673,211 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   }
 }
 
+// ---------------------------------------------------------------- TN, one-hot B
+// P_z (M x N, N <= 128) = sum_{k in slab z} (A[k, m] - shA[m]) [y_k == n + 1]:
+// the class sums of the centred columns of A (K x M col-major) from int32 labels.
+// For column-major A the MFMA A operand (8 consecutive k of one m per lane) is
+// two contiguous float4 loads, and the one-hot B operand is generated in
+// registers from 8 labels, so the kernel needs neither LDS staging nor barriers
+// in its main loop.  Labels outside 1..N (and rows past the slab, read as 0)
+// fall in no class.  Workgroup: 32 columns m (2 blocks of 16) x all N; its 4
+// waves take interleaved 32-deep K steps of the slab and are summed through LDS
+// at the end.  Split-BF16x3 splits A three ways (hi + mid + lo), which with the
+// exact 0/1 operand reproduces the FP32 product; FP32 mode uses
+// v_mfma_f32_16x16x4f32 with lane-private k (a summation index may be assigned
+// to lanes in any order shared by both operands).
+constexpr int TL_BM = 32, TL_BK = 32, TL_NT = 256;
+
+// C(m, n) = sum_z P_z(m, n) - shB(n) sum_z P_z(m, N)   (second term with shB)
+__global__ void k_tl_reduce(int64_t M, int64_t N, int splits, const float* __restrict__ partial,
+                            const double* __restrict__ shB, double* __restrict__ C, int64_t ldc) {
+  const int64_t NC = N + (shB ? 1 : 0);
+  const int64_t stride = M * NC;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < M * N;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = t % M, n = t / M;
+    double s = 0.0, u = 0.0;
+    for (int z = 0; z < splits; ++z) {
+      s += static_cast<double>(partial[z * stride + m + n * M]);
+      if (shB) u += static_cast<double>(partial[z * stride + m + N * M]);
+    }
+    if (shB) s -= u * shB[n];
+    C[m + n * ldc] = s;
+  }
+}
+
+template <bool BF>
+__global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, int64_t kchunk,
+                                                   const float* A, int64_t lda,
+                                                   const double* shA, const int32_t* lab,
+                                                   int ones, float* partial) {
+  __shared__ float red[4][TL_BM][128 + 1];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * TL_BM;
+  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
+  // with `ones`, column N is the all-ones column: P(:, N) = column sums of
+  // the centred A over the slab (for the one-hot centring correction)
+  const int NC = N + (ones ? 1 : 0);
+  const int nb = (NC + 15) >> 4;
+  const int64_t mr0 = std::min<int64_t>(m0 + lane % 16, M - 1);
+  const int64_t mr1 = std::min<int64_t>(m0 + 16 + lane % 16, M - 1);
+  const float* col0 = A + mr0 * lda;
+  const float* col1 = A + mr1 * lda;
+  const float s0 = shA ? static_cast<float>(shA[mr0]) : 0.f;
+  const float s1 = shA ? static_cast<float>(shA[mr1]) : 0.f;
+  const bool vec = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                   ((reinterpret_cast<uintptr_t>(lab) & 15) == 0) && ((kbeg & 3) == 0);
+  v4f acc[2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int ko = 8 * (lane / 16);
+  float x0[8], x1[8], y0[8], y1[8];
+  int lb[8], lc[8];
+  auto fetch = [&](int64_t k0, float (&a0)[8], float (&a1)[8], int (&l8)[8]) {
+    const int64_t kb = k0 + ko;
+    if (vec && kb + 8 <= kend) {
+      const float4 p = *reinterpret_cast<const float4*>(col0 + kb);
+      const float4 q = *reinterpret_cast<const float4*>(col0 + kb + 4);
+      const float4 u = *reinterpret_cast<const float4*>(col1 + kb);
+      const float4 v = *reinterpret_cast<const float4*>(col1 + kb + 4);
+      const int4 la = *reinterpret_cast<const int4*>(lab + kb);
+      const int4 lz = *reinterpret_cast<const int4*>(lab + kb + 4);
+      a0[0] = p.x; a0[1] = p.y; a0[2] = p.z; a0[3] = p.w;
+      a0[4] = q.x; a0[5] = q.y; a0[6] = q.z; a0[7] = q.w;
+      a1[0] = u.x; a1[1] = u.y; a1[2] = u.z; a1[3] = u.w;
+      a1[4] = v.x; a1[5] = v.y; a1[6] = v.z; a1[7] = v.w;
+      l8[0] = la.x; l8[1] = la.y; l8[2] = la.z; l8[3] = la.w;
+      l8[4] = lz.x; l8[5] = lz.y; l8[6] = lz.z; l8[7] = lz.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t k = kb + j;
+        const int64_t kc = std::min<int64_t>(k, K - 1);
+        a0[j] = col0[kc];
+        a1[j] = col1[kc];
+        l8[j] = k < kend ? lab[kc] : 0;  // label 0 matches no class
+      }
+    }
+  };
+  auto compute = [&](const float (&a0)[8], const float (&a1)[8], const int (&l8)[8]) {
+    if (BF) {
+      v8bf h0, m0v, l0, h1, m1v, l1;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float c0 = a0[j] - s0, c1 = a1[j] - s1;
+        const __bf16 hh0 = static_cast<__bf16>(c0), hh1 = static_cast<__bf16>(c1);
+        const float r0 = c0 - static_cast<float>(hh0), r1 = c1 - static_cast<float>(hh1);
+        const __bf16 mm0 = static_cast<__bf16>(r0), mm1 = static_cast<__bf16>(r1);
+        h0[j] = hh0;
+        m0v[j] = mm0;
+        l0[j] = static_cast<__bf16>(r0 - static_cast<float>(mm0));
+        h1[j] = hh1;
+        m1v[j] = mm1;
+        l1[j] = static_cast<__bf16>(r1 - static_cast<float>(mm1));
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b < nb) {
+          const int cls = b * 16 + lane % 16 + 1;
+          const bool all = ones && cls == N + 1;
+          v8bf oh;
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            oh[j] = static_cast<__bf16>((l8[j] == cls || (all && l8[j] > 0)) ? 1.f : 0.f);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, oh, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m0v, oh, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0, oh, acc[0][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1, oh, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(m1v, oh, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1, oh, acc[1][b], 0, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float c0 = a0[j] - s0, c1 = a1[j] - s1;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (b < nb) {
+            const int cls = b * 16 + lane % 16 + 1;
+            const bool all = ones && cls == N + 1;
+            const float oh = (l8[j] == cls || (all && l8[j] > 0)) ? 1.f : 0.f;
+            acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0, oh, acc[0][b], 0, 0, 0);
+            acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(c1, oh, acc[1][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  // wave w takes K steps w, w + 4, w + 8, ... of the slab; the loads of step
+  // i + 1 are issued before the MFMAs of step i
+  int64_t k0 = kbeg + (int64_t)w * TL_BK;
+  if (k0 < kend) fetch(k0, x0, x1, lb);
+  while (k0 < kend) {
+    const int64_t kn = k0 + 4 * TL_BK;
+    const bool more = kn < kend;
+    if (more) fetch(kn, y0, y1, lc);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(x0, x1, lb);
+    if (more) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        x0[j] = y0[j];
+        x1[j] = y1[j];
+        lb[j] = lc[j];
+      }
+    }
+    k0 = kn;
+  }
+  // sum the four waves: lane holds D[4(l/16) + r][l%16] of each 16 x 16 block
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) red[w][a * 16 + 4 * (lane / 16) + r][b * 16 + lane % 16] = acc[a][b][r];
+  __syncthreads();
+  float* P = partial + (int64_t)blockIdx.y * M * NC;
+  for (int idx = t; idx < TL_BM * 128; idx += TL_NT) {
+    const int mi = idx % TL_BM, n = idx / TL_BM;
+    const int64_t m = m0 + mi;
+    if (m < M && n < NC)
+      P[m + (int64_t)n * M] = (red[0][mi][n] + red[1][mi][n]) + (red[2][mi][n] + red[3][mi][n]);
+  }
+}
+
 void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0) {
     for (int64_t j = 0; j < N; ++j) fill_f64(ctx, C + j * ldc, 0.0, M);
+    return;
+  }
+  if (labelsB && N + (shiftB ? 1 : 0) <= 128) {
+    // class sums from the labels; the centring of the one-hot by shiftB uses
+    // an extra all-ones column (the column sums of the centred A)
+    const int64_t mt = (M + TL_BM - 1) / TL_BM;
+    const int64_t NC = N + (shiftB ? 1 : 0);
+    int64_t splits = std::max<int64_t>(1, (2 * ctx->num_cus + mt - 1) / mt);
+    splits = std::min<int64_t>(splits, (K + 4 * TL_BK * 4 - 1) / (4 * TL_BK * 4));
+    int64_t kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + TL_BK - 1) / TL_BK * TL_BK;
+    splits = (K + kchunk - 1) / kchunk;
+    DBuf<float> part(ctx, (size_t)splits * M * NC);
+    dim3 grid((unsigned)mt, (unsigned)splits);
+    const int n32 = (int)N, ones = shiftB ? 1 : 0;
+    if (precision == FPLS_PREC_BF16X3)
+      launch(ctx, "gemm_tn_bf16x3", k_tns_lab<true>, grid, dim3(TL_NT), 0, M, n32, K, kchunk, A,
+             lda, shiftA, labelsB, ones, part.get());
+    else
+      launch(ctx, "gemm_tn_f32", k_tns_lab<false>, grid, dim3(TL_NT), 0, M, n32, K, kchunk, A,
+             lda, shiftA, labelsB, ones, part.get());
+    launch(ctx, "gemm_tn_reduce", k_tl_reduce, dim3(g1(M * N)), dim3(256), 0, M, N, (int)splits,
+           (const float*)part.get(), shiftB, C, ldc);
     return;
   }
   const int64_t tiles = ((M + G2_BM - 1) / G2_BM) * ((N + G2_BN - 1) / G2_BN);
