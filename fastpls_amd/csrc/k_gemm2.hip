@@ -422,14 +422,28 @@ __global__ void k_g2_pad(const float* __restrict__ B, int64_t ldb, int64_t K, in
   }
 }
 
-__global__ void k_g2_reduce(int64_t M, int64_t N, int splits, const float* __restrict__ partial,
-                            double* __restrict__ C, int64_t ldc) {
+// C(m, n) = sum_z P_z(m, n) in FP64, fixed order.  Block: 32 outputs x 8
+// split groups; group g sums splits g, g + 8, ... and the 8 group sums are
+// added in order, so many loads are in flight per output.
+__global__ __launch_bounds__(256) void k_g2_reduce(int64_t M, int64_t N, int splits,
+                                                   const float* __restrict__ partial,
+                                                   double* __restrict__ C, int64_t ldc) {
+  __shared__ double red[8][32];
   const int64_t n = M * N;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int z = 0; z < splits; ++z) s += static_cast<double>(partial[(int64_t)z * n + t]);
-    C[(t % M) + (t / M) * ldc] = s;
+  const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t t = (int64_t)blockIdx.x * 32 + o;
+  double s = 0.0;
+  if (t < n) {
+#pragma unroll 4
+    for (int z = g; z < splits; z += 8) s += static_cast<double>(partial[(int64_t)z * n + t]);
+  }
+  red[g][o] = s;
+  __syncthreads();
+  if (g == 0 && t < n) {
+    double r = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) r += red[q][o];
+    C[(t % M) + (t / M) * ldc] = r;
   }
 }
 
@@ -689,20 +703,36 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
 constexpr int TL_BM = 32, TL_BK = 32, TL_NT = 256;
 
 // C(m, n) = sum_z P_z(m, n) - shB(n) sum_z P_z(m, N)   (second term with shB)
-__global__ void k_tl_reduce(int64_t M, int64_t N, int splits, const float* __restrict__ partial,
-                            const double* __restrict__ shB, double* __restrict__ C, int64_t ldc) {
+__global__ __launch_bounds__(256) void k_tl_reduce(int64_t M, int64_t N, int splits,
+                                                   const float* __restrict__ partial,
+                                                   const double* __restrict__ shB,
+                                                   double* __restrict__ C, int64_t ldc) {
+  __shared__ double red[8][32][2];
   const int64_t NC = N + (shB ? 1 : 0);
   const int64_t stride = M * NC;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < M * N;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = t % M, n = t / M;
-    double s = 0.0, u = 0.0;
-    for (int z = 0; z < splits; ++z) {
-      s += static_cast<double>(partial[z * stride + m + n * M]);
-      if (shB) u += static_cast<double>(partial[z * stride + m + N * M]);
+  const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t t = (int64_t)blockIdx.x * 32 + o;
+  const int64_t m = t % M, n = t / M;
+  double s = 0.0, u = 0.0;
+  if (t < M * N) {
+#pragma unroll 4
+    for (int z = g; z < splits; z += 8) {
+      s += static_cast<double>(partial[(int64_t)z * stride + m + n * M]);
+      if (shB) u += static_cast<double>(partial[(int64_t)z * stride + m + N * M]);
     }
-    if (shB) s -= u * shB[n];
-    C[m + n * ldc] = s;
+  }
+  red[g][o][0] = s;
+  red[g][o][1] = u;
+  __syncthreads();
+  if (g == 0 && t < M * N) {
+    double r = 0.0, v = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      r += red[q][o][0];
+      v += red[q][o][1];
+    }
+    if (shB) r -= v * shB[n];
+    C[m + n * ldc] = r;
   }
 }
 
@@ -849,12 +879,188 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
   }
 }
 
+// ---------------------------------------------------------------- Gram, M <= 128
+// P_z (M x M) = sum_{k in slab z} (A[k, :] - sh)^T (A[k, :] - sh) for a tall
+// A (K x M col-major).  One 32-deep K step of A is staged once in LDS in the
+// [m][k] layout that serves both MFMA operands (hi/lo bf16 for split-BF16x3,
+// FP32 [k][m] for the plain path), double buffered with one barrier per step;
+// wave w owns output rows 32w .. 32w + 31.
+template <bool BF>
+struct GSmem {
+  static constexpr int value = BF ? (2 * 2 * 128 * P_LDB) / 2 : 2 * P_BK * (128 + 4);
+};
+
+template <bool BF>
+__global__ __launch_bounds__(P_NT) void k_gram(int64_t M, int64_t K, int64_t kchunk,
+                                               const float* A, int64_t lda, const double* sh,
+                                               float* partial) {
+  __shared__ __align__(16) float smem[GSmem<BF>::value];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t kbeg = (int64_t)blockIdx.x * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
+  const int nb = (int)((M + 15) >> 4);
+  v4f acc[2][8];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  // staging: thread -> column m = t / 2, 16 consecutive k
+  const int sm = t >> 1, sk = (t & 1) * 16;
+  const bool mok = sm < M;
+  const float* scol = A + std::min<int64_t>(sm, M - 1) * lda;
+  const float shv = (sh && mok) ? static_cast<float>(sh[sm]) : 0.f;
+  const bool vec = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                   ((kbeg & 3) == 0);
+  float v[16];
+  auto fetch = [&](int64_t k0) {
+    const int64_t kb = k0 + sk;
+    if (vec && kb + 16 <= kend) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 f = *reinterpret_cast<const float4*>(scol + kb + 4 * q);
+        v[4 * q + 0] = f.x;
+        v[4 * q + 1] = f.y;
+        v[4 * q + 2] = f.z;
+        v[4 * q + 3] = f.w;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; ++e) v[e] = scol[std::min<int64_t>(kb + e, K - 1)];
+    }
+  };
+  auto store = [&](int64_t k0, int buf) {
+    const int64_t kb = k0 + sk;
+    float c[16];
+#pragma unroll
+    for (int e = 0; e < 16; ++e) c[e] = (mok && kb + e < kend) ? v[e] - shv : 0.f;
+    if (BF) {
+      __bf16* base = reinterpret_cast<__bf16*>(smem);
+      __bf16* Th = base + (0 * 2 + buf) * 128 * P_LDB;
+      __bf16* Tl = base + (1 * 2 + buf) * 128 * P_LDB;
+      v8bf h0, l0, h1, l1;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        __bf16 hh, ll;
+        split_bf(c[e], hh, ll);
+        h0[e] = hh;
+        l0[e] = ll;
+        split_bf(c[8 + e], hh, ll);
+        h1[e] = hh;
+        l1[e] = ll;
+      }
+      *reinterpret_cast<v8bf*>(Th + sm * P_LDB + sk) = h0;
+      *reinterpret_cast<v8bf*>(Th + sm * P_LDB + sk + 8) = h1;
+      *reinterpret_cast<v8bf*>(Tl + sm * P_LDB + sk) = l0;
+      *reinterpret_cast<v8bf*>(Tl + sm * P_LDB + sk + 8) = l1;
+    } else {
+      float* Ts = smem + buf * P_BK * (128 + 4);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) Ts[(sk + e) * (128 + 4) + sm] = c[e];
+    }
+  };
+  auto compute = [&](int buf) {
+    if (BF) {
+      const __bf16* base = reinterpret_cast<const __bf16*>(smem);
+      const __bf16* Th = base + (0 * 2 + buf) * 128 * P_LDB;
+      const __bf16* Tl = base + (1 * 2 + buf) * 128 * P_LDB;
+      const int ko = 8 * (lane / 16);
+      v8bf ah[2], al[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const int m = w * 32 + a * 16 + lane % 16;
+        ah[a] = *reinterpret_cast<const v8bf*>(Th + m * P_LDB + ko);
+        al[a] = *reinterpret_cast<const v8bf*>(Tl + m * P_LDB + ko);
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        if (b < nb) {
+          const int n = b * 16 + lane % 16;
+          const v8bf bh = *reinterpret_cast<const v8bf*>(Th + n * P_LDB + ko);
+          const v8bf bl = *reinterpret_cast<const v8bf*>(Tl + n * P_LDB + ko);
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+          }
+        }
+      }
+    } else {
+      const float* Ts = smem + buf * P_BK * (128 + 4);
+#pragma unroll
+      for (int s4 = 0; s4 < 8; ++s4) {
+        const int kr = 4 * s4 + lane / 16;
+        float af[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) af[a] = Ts[kr * (128 + 4) + w * 32 + a * 16 + lane % 16];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          if (b < nb) {
+            const float bv = Ts[kr * (128 + 4) + b * 16 + lane % 16];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+              acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a], bv, acc[a][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  const int nk = (int)((kend - kbeg + P_BK - 1) / P_BK);
+  if (nk > 0) {
+    fetch(kbeg);
+    store(kbeg, 0);
+  }
+  __syncthreads();
+  for (int c = 0; c < nk; ++c) {
+    const bool more = c + 1 < nk;
+    const int64_t kn = kbeg + (int64_t)(c + 1) * P_BK;
+    if (more) fetch(kn);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(c & 1);
+    if (more) store(kn, (c + 1) & 1);
+    __syncthreads();
+  }
+  float* P = partial + (int64_t)blockIdx.x * M * M;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int col = b * 16 + lane % 16;
+      if (b >= nb || col >= M) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = w * 32 + a * 16 + 4 * (lane / 16) + r;
+        if (row < M) P[row + (int64_t)col * M] = acc[a][b][r];
+      }
+    }
+  }
+}
+
 void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc) {
   if (M <= 0 || N <= 0) return;
   if (K <= 0) {
     for (int64_t j = 0; j < N; ++j) fill_f64(ctx, C + j * ldc, 0.0, M);
+    return;
+  }
+  if (!labelsB && A == B && lda == ldb && shiftA == shiftB && M == N && M <= 128) {
+    // Gram matrix of a tall A
+    int64_t splits = std::max<int64_t>(1, std::min<int64_t>(2 * ctx->num_cus, (K + 255) / 256));
+    const int64_t cap = std::max<int64_t>(1, (int64_t(32) << 20) / (4 * M * M));
+    splits = std::min(splits, cap);
+    int64_t kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
+    splits = (K + kchunk - 1) / kchunk;
+    DBuf<float> part(ctx, (size_t)splits * M * M);
+    if (precision == FPLS_PREC_BF16X3)
+      launch(ctx, "gemm_tn_bf16x3", k_gram<true>, dim3((unsigned)splits), dim3(P_NT), 0, M, K,
+             kchunk, A, lda, shiftA, part.get());
+    else
+      launch(ctx, "gemm_tn_f32", k_gram<false>, dim3((unsigned)splits), dim3(P_NT), 0, M, K,
+             kchunk, A, lda, shiftA, part.get());
+    launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
+           (const float*)part.get(), C, ldc);
     return;
   }
   if (labelsB && N + (shiftB ? 1 : 0) <= 128) {
@@ -876,7 +1082,7 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
     else
       launch(ctx, "gemm_tn_f32", k_tns_lab<false>, grid, dim3(TL_NT), 0, M, n32, K, kchunk, A,
              lda, shiftA, labelsB, ones, part.get());
-    launch(ctx, "gemm_tn_reduce", k_tl_reduce, dim3(g1(M * N)), dim3(256), 0, M, N, (int)splits,
+    launch(ctx, "gemm_tn_reduce", k_tl_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
            (const float*)part.get(), shiftB, C, ldc);
     return;
   }
@@ -898,7 +1104,7 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   else
     launch(ctx, "gemm_tn_f32", k_g2_tn<false>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
            shiftA, B, ldb, labelsB, shiftB, part.get());
-  launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3(g1(M * N)), dim3(256), 0, M, N, (int)splits,
+  launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
          (const float*)part.get(), C, ldc);
 }
 

@@ -30,11 +30,36 @@ __global__ __launch_bounds__(256) void k_col_sums(int64_t n, int64_t p, const T*
   if (j >= p) return;
   const T* col = X + j * ld;
   const double c = center ? center[j] : 0.0;
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += 256) {
-    const double v = static_cast<double>(col[i]) - c;
-    s += squared ? v * v : v;
+  auto term = [&](T x) {
+    const double v = static_cast<double>(x) - c;
+    return squared ? v * v : v;
+  };
+  // four independent partial sums over 16-byte vectors keep several loads in
+  // flight per thread
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  constexpr int VW = 16 / sizeof(T);
+  int64_t head = 0;
+  if ((reinterpret_cast<uintptr_t>(col) & 15) == 0) {
+    typedef T vT __attribute__((ext_vector_type(VW)));
+    const vT* cv = reinterpret_cast<const vT*>(col);
+    const int64_t nv = n / VW;
+#pragma unroll 4
+    for (int64_t i = threadIdx.x; i < nv; i += 256) {
+      const vT v = cv[i];
+      if constexpr (VW == 4) {
+        s0 += term(v[0]);
+        s1 += term(v[1]);
+        s2 += term(v[2]);
+        s3 += term(v[3]);
+      } else {
+        s0 += term(v[0]);
+        s1 += term(v[1]);
+      }
+    }
+    head = nv * VW;
   }
+  for (int64_t i = head + threadIdx.x; i < n; i += 256) s0 += term(col[i]);
+  double s = (s0 + s1) + (s2 + s3);
   s = block_sum_256<T>(s, red);
   if (threadIdx.x == 0) out[j] = s;
 }
