@@ -90,7 +90,7 @@ def run_gpu(args, data):
     for prec in ("fp32", "bf16x3"):
         def step(fetch=False):
             m = F.pls(dXtr, dy, dXte, method="plssvd", ncomp=NCOMP, classifier="lda",
-                      precision=prec, fetch=fetch)
+                      precision=prec, fetch=fetch, n_classes=C)
             res = (m["Ypred_index"], m.get("R"), m.get("B"))
             m.free()
             return res

@@ -69,12 +69,14 @@ def _is_factor(y) -> bool:
     return a.dtype.kind in "iubOUS" and (a.ndim == 1 or (a.ndim == 2 and a.shape[1] == 1))
 
 
-def _encode_factor(y):
-    """Levels (sorted unique) and 1-based codes, as R's factor()."""
+def _encode_factor(y, n_classes=None):
+    """Levels (sorted unique) and 1-based codes, as R's factor().  Device
+    (torch) labels are taken as codes 1..C; C is ``n_classes`` when given (a
+    factor's level count), else a device reduction."""
     if type(y).__module__.startswith("torch"):
         import torch
-        codes = y.to(torch.int32)
-        C = int(codes.max().item())
+        codes = y if y.dtype == torch.int32 else y.to(torch.int32)
+        C = int(n_classes) if n_classes is not None else int(codes.max().item())
         return list(range(1, C + 1)), codes, C
     a = np.asarray(y).ravel()
     lev, codes = np.unique(a, return_inverse=True)
@@ -119,8 +121,10 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         method="simpls", svd_method="rsvd", classifier="argmax", lda_ridge=1e-8,
         candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
         fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
-        precision="fp32", label_aware=False, fetch=True, device=0, top=1) -> FastPLS:
-    """pls() (R/main.R:5860-6314) with backend = "hip"."""
+        precision="fp32", label_aware=False, fetch=True, device=0, top=1,
+        n_classes=None) -> FastPLS:
+    """pls() (R/main.R:5860-6314) with backend = "hip".  ``n_classes``: level
+    count of device-resident integer labels (optional)."""
     _check_backend(backend)
     ctx = _lib.context(device)
     keep: list = []
@@ -138,7 +142,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     lev = None
     C = 0
     if classification:
-        lev, codes, C = _encode_factor(Ytrain)
+        lev, codes, C = _encode_factor(Ytrain, n_classes)
         yv = as_ivec(codes, keep)
     else:
         Ym = as_mat(Ytrain, keep)

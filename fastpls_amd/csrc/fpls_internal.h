@@ -93,6 +93,23 @@ struct Ctx {
   std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
   std::vector<hipEvent_t> spare_events;
   int num_cus = 256;
+  // Deferred checks: kernels record failure counts in device flag slots, and
+  // one read-back at the end of an entry point raises the first failing check,
+  // so a fit makes no host round trip half way through.
+  struct Check {
+    int slot, count, code, fail_value;  // fail_value 0: fails when != 0
+    std::string msg;
+  };
+  int32_t* flags_dev = nullptr;
+  int32_t* flags_host = nullptr;
+  int nflags = 0;
+  std::vector<Check> checks;
+  int32_t* defer_check(int count, int code, const std::string& msg, int fail_value = 0);
+  void flush_checks();
+  void drop_checks() {
+    checks.clear();
+    nflags = 0;
+  }
   void* alloc(size_t bytes) { return pool.get(bytes); }
   void free(void* p) { pool.put(p); }
   hipEvent_t event();
@@ -163,6 +180,7 @@ void sync(Ctx* ctx);
 void fill_f64(Ctx* ctx, double* x, double v, int64_t n);
 void fill_f32(Ctx* ctx, float* x, float v, int64_t n);
 void fill_i32(Ctx* ctx, int32_t* x, int32_t v, int64_t n);
+void copy_i32(Ctx* ctx, int32_t* d, const int32_t* s, int64_t n);
 void copy_f64(Ctx* ctx, double* dst, const double* src, int64_t n);
 // Copy a rows x cols block between column-major buffers with different ld.
 void copy2d_f64(Ctx* ctx, double* dst, int64_t ldd, const double* src, int64_t lds,
@@ -225,7 +243,7 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
 //   mode 0: A_k X = diag(d)            mode 1: (A_k + lambda I) X = I (LDA ridge rule)
 //   mode 2: (A_k + 1e-10 mean(diag A_k) I) X = diag(d)   (label-aware PLSSVD)
 // d_k: device array of slice orders; status[s]: 0 Cholesky, 2 LU fallback, 1 singular.
-void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const int* d_k,
+void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
                       int n_slices, int mode, const double* dvals, double ridge, double* X,
                       int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status);
 void scale_cols(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols, const double* f,

@@ -79,6 +79,14 @@ void fill_f32(Ctx* ctx, float* x, float v, int64_t n) {
 void fill_i32(Ctx* ctx, int32_t* x, int32_t v, int64_t n) {
   if (n > 0) launch(ctx, "fill", k_fill_i32, dim3(grid1d(n)), dim3(256), 0, x, v, n);
 }
+__global__ void k_copy_i32(int32_t* __restrict__ d, const int32_t* __restrict__ s, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x)
+    d[t] = s[t];
+}
+void copy_i32(Ctx* ctx, int32_t* d, const int32_t* s, int64_t n) {
+  if (n > 0) launch(ctx, "copy", k_copy_i32, dim3(grid1d(n)), dim3(256), 0, d, s, n);
+}
 void copy_f64(Ctx* ctx, double* d, const double* s, int64_t n) {
   if (n > 0) launch(ctx, "copy", k_copy_f64, dim3(grid1d(n)), dim3(256), 0, d, s, n);
 }

@@ -14,6 +14,12 @@
 namespace fpls {
 
 constexpr int JMAX = kJacobiMax;  // 128
+
+// orders of the ncomp slices of one launch, passed by value
+constexpr int kSliceMax = 64;
+struct SliceOrders {
+  int k[kSliceMax];
+};
 constexpr int JLD = JMAX + 1;
 
 // --------------------------------------------------------------- slice solves
@@ -27,7 +33,7 @@ constexpr int JLD = JMAX + 1;
 // reads it any more.  The solution is then X = Z^T Z RHS with Z = L^{-1}.
 template <int KMAX, bool Z_IN_LDS>
 __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
-                                                    const int* __restrict__ ks, int mode,
+                                                    SliceOrders ks, int mode,
                                                     const double* __restrict__ dvals,
                                                     double ridge, double* __restrict__ X,
                                                     int64_t ldx, int64_t slice_stride,
@@ -41,7 +47,7 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
   __shared__ int chol_ok;
   constexpr int LD = KMAX + 1;
   const int s = blockIdx.x;
-  const int k = ks[s];
+  const int k = ks.k[s];
   const int t = threadIdx.x, nt = blockDim.x;
   double* Xs = X + (int64_t)s * slice_stride;
   double* Z = Z_IN_LDS ? sZ : Zg + (int64_t)s * KMAX * LD;
@@ -230,18 +236,32 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
   if (t == 0 && status) status[s] = 2;  // solved through the LU fallback
 }
 
-void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const int* d_k,
+void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
                       int n_slices, int mode, const double* dvals, double ridge, double* X,
                       int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status) {
   FPLS_REQUIRE(kmax >= 1 && kmax <= kJacobiMax, FPLS_ERR_ARG,
                "spd_solve_slices: order must be in 1..128");
-  if (kmax <= 100) {
-    launch(ctx, "spd_solve", k_spd_solve<100, true>, dim3(n_slices), dim3(1024), 0, A, lda, d_k,
-           mode, dvals, ridge, X, ldx, slice_stride, lambda_out, status, (double*)nullptr);
-  } else {
-    DBuf<double> Zg(ctx, (size_t)n_slices * 128 * 129);
-    launch(ctx, "spd_solve", k_spd_solve<128, false>, dim3(n_slices), dim3(1024), 0, A, lda, d_k,
-           mode, dvals, ridge, X, ldx, slice_stride, lambda_out, status, Zg.get());
+  FPLS_REQUIRE((int)ks.size() >= n_slices, FPLS_ERR_ARG, "spd_solve_slices: missing orders");
+  // slice orders travel in the kernel arguments, kSliceMax per launch
+  for (int c0 = 0; c0 < n_slices; c0 += kSliceMax) {
+    const int cnt = std::min(kSliceMax, n_slices - c0);
+    SliceOrders so{};
+    for (int i = 0; i < cnt; ++i) {
+      FPLS_REQUIRE(ks[c0 + i] >= 1 && ks[c0 + i] <= kmax, FPLS_ERR_ARG,
+                   "spd_solve_slices: order out of range");
+      so.k[i] = ks[c0 + i];
+    }
+    double* Xc = X + (int64_t)c0 * slice_stride;
+    double* lc = lambda_out ? lambda_out + c0 : nullptr;
+    int32_t* sc = status ? status + c0 : nullptr;
+    if (kmax <= 100) {
+      launch(ctx, "spd_solve", k_spd_solve<100, true>, dim3(cnt), dim3(1024), 0, A, lda, so,
+             mode, dvals, ridge, Xc, ldx, slice_stride, lc, sc, (double*)nullptr);
+    } else {
+      DBuf<double> Zg(ctx, (size_t)cnt * 128 * 129);
+      launch(ctx, "spd_solve", k_spd_solve<128, false>, dim3(cnt), dim3(1024), 0, A, lda, so,
+             mode, dvals, ridge, Xc, ldx, slice_stride, lc, sc, Zg.get());
+    }
   }
 }
 

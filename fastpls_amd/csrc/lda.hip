@@ -102,6 +102,16 @@ void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t
            scores, lds, n, C, cst, labels);
 }
 
+// FP64 class counts; empty[0] = number of empty classes
+__global__ void k_counts_checked(const unsigned long long* __restrict__ cnt, int C,
+                                 double* __restrict__ out, int32_t* __restrict__ empty) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < C) {
+    out[c] = (double)cnt[c];
+    if (cnt[c] == 0ull) atomicAdd(empty, 1);
+  }
+}
+
 // Train prefix-LDA models for every k in ks on scores T (n x >= max ks).
 // G (optional): T^T T already computed (ld ldg).
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
@@ -114,24 +124,17 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
   FPLS_REQUIRE(kmax >= 1 && kmax <= T.cols, FPLS_ERR_ARG,
                "LDA component counts must be in 1..ncol(Ttrain)");
   FPLS_REQUIRE(kmax <= kJacobiMax, FPLS_ERR_ARG, "LDA on the device supports up to 128 components");
-  // counts + label validation (src/fastPLS.cpp:1853-1867)
+  // counts + label validation (src/fastPLS.cpp:1853-1867): codes outside 1..C
+  // and empty classes are counted on the device and checked when the entry
+  // point returns (kernels below count such rows in no class)
   DBuf<int64_t> cnt(ctx, C);
-  DBuf<int32_t> bad(ctx, 1);
-  label_counts(ctx, y, n, C, cnt.get(), bad.get());
-  int32_t hbad = 0;
-  to_host_i32(ctx, &hbad, bad.get(), 1);
-  FPLS_REQUIRE(hbad == 0, FPLS_ERR_LABEL_RANGE, "LDA labels must be encoded as 1..n_classes");
-  std::vector<int64_t> hc(C);
-  FPLS_HIP(hipMemcpyAsync(hc.data(), cnt.get(), C * sizeof(int64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-  sync(ctx);
-  std::vector<double> hcd(C);
-  for (int c = 0; c < C; ++c) {
-    FPLS_REQUIRE(hc[c] > 0, FPLS_ERR_EMPTY_CLASS, "LDA received an empty class");
-    hcd[c] = (double)hc[c];
-  }
+  int32_t* bad =
+      ctx->defer_check(1, FPLS_ERR_LABEL_RANGE, "LDA labels must be encoded as 1..n_classes");
+  label_counts(ctx, y, n, C, cnt.get(), bad);
   counts_out = DBuf<double>(ctx, C);
-  to_device(ctx, counts_out.get(), hcd.data(), C);
+  int32_t* empty = ctx->defer_check(1, FPLS_ERR_EMPTY_CLASS, "LDA received an empty class");
+  launch(ctx, "lda_counts", k_counts_checked, dim3((C + 255) / 256), dim3(256), 0,
+         reinterpret_cast<const unsigned long long*>(cnt.get()), C, counts_out.get(), empty);
   // class sums of the leading kmax score columns: k x C
   DBuf<double> sums(ctx, (size_t)kmax * C), means(ctx, (size_t)C * kmax);
   if (T.dtype == FPLS_F32)
@@ -167,18 +170,18 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
          (int64_t)kmax * kmax, 1.0 / df);
   // per k: ridge, inverse, linear, constants
   slices.clear();
-  DBuf<int> dk(ctx, 1);
-  DBuf<double> lam(ctx, 1);
-  DBuf<int32_t> st(ctx, 1);
   for (int k : ks) {
     LdaSlice sl;
     sl.k = k;
     sl.means = DBuf<double>(ctx, (size_t)C * k);
     copy2d_f64(ctx, sl.means.get(), C, means.get(), C, C, k);
     sl.inv_cov = DBuf<double>(ctx, (size_t)k * k);
-    h2d(ctx, dk.get(), &k, sizeof(int));
-    spd_solve_slices(ctx, pooled.get(), kmax, k, dk.get(), 1, 1, nullptr, ridge, sl.inv_cov.get(),
-                     k, (int64_t)k * k, lam.get(), st.get());
+    sl.ridge_d = DBuf<double>(ctx, 1);
+    sl.status_d = DBuf<int32_t>(ctx, 1);
+    spd_solve_slices(ctx, pooled.get(), kmax, k, std::vector<int>{k}, 1, 1, nullptr, ridge,
+                     sl.inv_cov.get(), k, (int64_t)k * k, sl.ridge_d.get(), sl.status_d.get());
+    int32_t* sing = ctx->defer_check(1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular", 1);
+    copy_i32(ctx, sing, sl.status_d.get(), 1);
     sl.linear = DBuf<double>(ctx, (size_t)C * k);
     gemm_f64(ctx, false, false, C, k, k, 1.0, sl.means.get(), C, nullptr, sl.inv_cov.get(), k,
              nullptr, 0.0, sl.linear.get(), C);
@@ -186,11 +189,6 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
     launch(ctx, "lda_constants", k_lda_constants, dim3((C + 255) / 256), dim3(256), 0,
            sl.means.get(), (int64_t)C, sl.linear.get(), (int64_t)C, (int64_t)C, (int64_t)k,
            counts_out.get(), (double)n, sl.constants.get());
-    to_host(ctx, &sl.ridge, lam.get(), 1);
-    int32_t hs = 0;
-    to_host_i32(ctx, &hs, st.get(), 1);
-    FPLS_REQUIRE(hs != 1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular");
-    sl.solve_status = hs;
     slices.push_back(std::move(sl));
   }
 }
@@ -317,7 +315,7 @@ extern "C" int fpls_lda_train_scores(fpls_ctx* ctx, const fpls_mat* Tm, const fp
       if (linear) to_host(c, linear + om, sl[s].linear.get(), (size_t)n_classes * k);
       if (inv_cov) to_host(c, inv_cov + oi, sl[s].inv_cov.get(), (size_t)k * k);
       if (constants) to_host(c, constants + oc, sl[s].constants.get(), n_classes);
-      if (ridge_out) ridge_out[s] = sl[s].ridge;
+      if (ridge_out) to_host(c, ridge_out + s, sl[s].ridge_d.get(), 1);
       om += (size_t)n_classes * k;
       oi += (size_t)k * k;
       oc += n_classes;
@@ -355,7 +353,7 @@ extern "C" int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t sli
       }
       case FPLS_LDA_RIDGE:
         FPLS_REQUIRE(host_out && n_elem >= 1, FPLS_ERR_ARG, "output buffer too small");
-        host_out[0] = s.ridge;
+        to_host(c, host_out, s.ridge_d.get(), 1);
         return;
       default: fail(FPLS_ERR_ARG, "unknown LDA field");
     }

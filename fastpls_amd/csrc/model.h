@@ -13,8 +13,8 @@ struct LdaSlice {
   DBuf<double> inv_cov;     // k x k
   DBuf<double> linear;      // C x k
   DBuf<double> constants;   // C
-  double ridge = 0.0;
-  int solve_status = 0;     // 0 Cholesky, 2 LU fallback
+  DBuf<double> ridge_d;     // applied ridge (1), read back on request
+  DBuf<int32_t> status_d;   // 0 Cholesky, 1 singular, 2 LU fallback (1)
 };
 
 struct CknnSlice {
@@ -136,11 +136,14 @@ int fpls_guard(fpls_ctx* ctx, F&& body) {
   try {
     FPLS_HIP(hipSetDevice(ctx->c.device));
     body();
+    ctx->c.flush_checks();
     return FPLS_OK;
   } catch (const fpls::Error& e) {
+    ctx->c.drop_checks();
     ctx->c.err = e.what();
     return e.code;
   } catch (const std::exception& e) {
+    ctx->c.drop_checks();
     ctx->c.err = e.what();
     return FPLS_ERR_ARG;
   }

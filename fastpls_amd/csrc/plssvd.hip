@@ -102,11 +102,11 @@ void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int3
     M->mY = DBuf<double>(ctx, C);
     M->counts = DBuf<double>(ctx, C);
     DBuf<int64_t> cnt(ctx, C);
-    DBuf<int32_t> bad(ctx, 1);
-    label_counts(ctx, y, n, C, cnt.get(), bad.get());
-    int32_t hbad = 0;
-    to_host_i32(ctx, &hbad, bad.get(), 1);
-    FPLS_REQUIRE(hbad == 0, FPLS_ERR_LABEL_RANGE, "labels must be encoded as 1..n_classes");
+    // rows with codes outside 1..C are counted in no class by every kernel of
+    // the fit; the count is checked when the entry point returns
+    int32_t* bad =
+        ctx->defer_check(1, FPLS_ERR_LABEL_RANGE, "labels must be encoded as 1..n_classes");
+    label_counts(ctx, y, n, C, cnt.get(), bad);
     launch(ctx, "counts", k_counts_to_prop, dim3((C + 255) / 256), dim3(256), 0,
            reinterpret_cast<const unsigned long long*>(cnt.get()), M->mY.get(), M->counts.get(),
            C, n);
@@ -343,15 +343,13 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     // per-slice latent coefficients
     for (int& v : nc) v = std::min(v, kmax);
     M->ncomp = nc;
-    DBuf<int> dk(c, L);
-    h2d(c, dk.get(), nc.data(), L * sizeof(int));
     M->C_latent = DBuf<double>(c, (size_t)kmax * kmax * L);
     M->W_latent = DBuf<double>(c, (size_t)kmax * m * L);
     fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
     fill_f64(c, M->W_latent.get(), 0.0, (int64_t)kmax * m * L);
-    DBuf<int32_t> st(c, L);
-    spd_solve_slices(c, M->G.get(), kmax, kmax, dk.get(), L, M->label_aware ? 2 : 0, M->s.get(),
-                     0.0, M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st.get());
+    int32_t* st = c->defer_check(L, FPLS_ERR_SOLVE, "plssvd latent solve failed", 1);
+    spd_solve_slices(c, M->G.get(), kmax, kmax, nc, L, M->label_aware ? 2 : 0, M->s.get(), 0.0,
+                     M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st);
     M->b_stored = !M->label_aware && store_b_policy(o.store_b, p, m, L);
     if (M->b_stored) M->B = DBuf<double>(c, (size_t)p * m * L);
     DBuf<double> Yc;
@@ -383,12 +381,7 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
         add_row(c, Yf, n, n, m, M->mY.get());
       }
     }
-    std::vector<int32_t> hst(L);
-    to_host_i32(c, hst.data(), st.get(), L);
-    for (int a = 0; a < L; ++a)
-      FPLS_REQUIRE(hst[a] != 1, FPLS_ERR_SOLVE, "plssvd latent solve failed");
     M->n_components = kmax;
-    sync(c);
   }
 }
 

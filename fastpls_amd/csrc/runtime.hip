@@ -62,6 +62,42 @@ size_t Pool::total_bytes() const {
   return t;
 }
 
+// ---------------------------------------------------------------- deferred checks
+constexpr int kFlagSlots = 1024;
+
+int32_t* Ctx::defer_check(int count, int code, const std::string& msg, int fail_value) {
+  FPLS_REQUIRE(count >= 1 && count <= kFlagSlots, FPLS_ERR_ARG, "too many deferred checks");
+  if (!flags_dev) {
+    FPLS_HIP(hipMalloc(&flags_dev, kFlagSlots * sizeof(int32_t)));
+    FPLS_HIP(hipHostMalloc(&flags_host, kFlagSlots * sizeof(int32_t), hipHostMallocDefault));
+  }
+  if (nflags + count > kFlagSlots) flush_checks();
+  int32_t* p = flags_dev + nflags;
+  fill_i32(this, p, 0, count);
+  checks.push_back(Check{nflags, count, code, fail_value, msg});
+  nflags += count;
+  return p;
+}
+
+void Ctx::flush_checks() {
+  if (checks.empty()) {
+    nflags = 0;
+    return;
+  }
+  FPLS_HIP(hipMemcpyAsync(flags_host, flags_dev, nflags * sizeof(int32_t), hipMemcpyDeviceToHost,
+                          stream));
+  FPLS_HIP(hipStreamSynchronize(stream));
+  std::vector<Check> cs;
+  cs.swap(checks);
+  nflags = 0;
+  for (const Check& c : cs) {
+    for (int i = 0; i < c.count; ++i) {
+      const int32_t v = flags_host[c.slot + i];
+      if (c.fail_value == 0 ? v != 0 : v == c.fail_value) fail(c.code, c.msg);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- timing
 hipEvent_t Ctx::event() {
   if (!spare_events.empty()) {
@@ -228,6 +264,8 @@ void fpls_ctx_destroy(fpls_ctx* ctx) {
     (void)hipEventDestroy(pe.second.second);
   }
   for (auto e : ctx->c.spare_events) (void)hipEventDestroy(e);
+  if (ctx->c.flags_dev) (void)hipFree(ctx->c.flags_dev);
+  if (ctx->c.flags_host) (void)hipHostFree(ctx->c.flags_host);
   ctx->c.pool.release_all();
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;
