@@ -27,21 +27,17 @@ constexpr int JLD = JMAX + 1;
 // mode 1: (A_k + lambda I) X = I, lambda from the  -- LDA inverse covariance
 //         reference's ridge rule; lambda_out[s] = lambda
 // mode 2: (A_k + r I) X = diag(d_k), r = 1e-10 mean(diag A_k)  -- label-aware PLSSVD
-// Cholesky and the inverse of the factor each take one barrier per step: the
-// trailing update of step j uses column j unscaled (A -= c c^T / d), and the
-// scaling of column j by 1/sqrt(d) is deferred to step j+1, when no thread
-// reads it any more.  The solution is then X = Z^T Z RHS with Z = L^{-1}.
-template <int KMAX, bool Z_IN_LDS>
+// The SPD inverse is a symmetric sweep held in registers (one barrier per
+// pivot); X = A^{-1} RHS.
+template <int KMAX>
 __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
                                                     SliceOrders ks, int mode,
                                                     const double* __restrict__ dvals,
                                                     double ridge, double* __restrict__ X,
                                                     int64_t ldx, int64_t slice_stride,
                                                     double* __restrict__ lambda_out,
-                                                    int32_t* __restrict__ status,
-                                                    double* __restrict__ Zg) {
+                                                    int32_t* __restrict__ status) {
   __shared__ double L[KMAX * (KMAX + 1)];
-  __shared__ double sZ[Z_IN_LDS ? KMAX * (KMAX + 1) : 1];
   __shared__ int piv[KMAX];
   __shared__ double shift;
   __shared__ int chol_ok;
@@ -50,7 +46,6 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
   const int k = ks.k[s];
   const int t = threadIdx.x, nt = blockDim.x;
   double* Xs = X + (int64_t)s * slice_stride;
-  double* Z = Z_IN_LDS ? sZ : Zg + (int64_t)s * KMAX * LD;
   if (t == 0) {
     double sh = 0.0;
     if (mode == 1) {
@@ -73,99 +68,123 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
     chol_ok = 1;
   }
   __syncthreads();
-  // 2-D thread map (no integer division in the loops): tx runs down a column
-  const int tx = t & 31, ty = t >> 5;
-  for (int j = ty; j < k; j += 32) {
-    for (int i = tx; i < k; i += 32) {
-      double v = Ain[i + (int64_t)j * lda];
-      if (i == j) v += shift;
-      L[i + j * LD] = v;
-      Z[i + j * LD] = (i == j) ? 1.0 : 0.0;
-    }
-  }
-  __syncthreads();
-  // ---- Cholesky, lower triangle, one barrier per column
-  double prev_d = 1.0;  // pivot of the previous column (same in every thread)
-  for (int j = 0; j < k; ++j) {
-    const double d = L[j + j * LD];
-    if (!(d > 0.0) || !isfinite(d)) {
-      if (t == 0) chol_ok = 0;
-      break;
-    }
-    if (j > 0) {  // deferred scaling of column j-1
-      const double r = 1.0 / sqrt(prev_d);
-      for (int i = j - 1 + t; i < k; i += nt) L[i + (j - 1) * LD] *= r;
-    }
-    prev_d = d;
-    const double inv_d = 1.0 / d;
-    for (int jj = j + 1 + ty; jj < k; jj += 32) {
-      const double cj = L[jj + j * LD] * inv_d;
-      for (int ii = jj + tx; ii < k; ii += 32) L[ii + jj * LD] -= L[ii + j * LD] * cj;
-    }
-    __syncthreads();
-  }
-  __syncthreads();
-  if (chol_ok) {
-    if (t == 0) L[(k - 1) + (k - 1) * LD] = sqrt(L[(k - 1) + (k - 1) * LD]);
-    __syncthreads();
-    // ---- Z = L^{-1} by a column sweep, one barrier per step
-    for (int q = 0; q < k; ++q) {
-      const double lqq = L[q + q * LD];
-      if (q > 0) {  // deferred scaling of row q-1
-        const double r = 1.0 / L[(q - 1) + (q - 1) * LD];
-        for (int c = t; c < q; c += nt) Z[(q - 1) + c * LD] *= r;
+  // ---- symmetric sweep (Gauss-Jordan without pivoting; Goodnight 1979).
+  // Thread (tr, tc) of a 32 x 32 grid holds the lower-triangle entries among
+  // (tr + 32a, tc + 32b), a, b < 4, in registers.  Sweeping pivot j:
+  //   d = A(j,j); A(j,j) = -1/d; A(i,j) = A(i,j)/d; A(i,c) -= A(i,j) A(c,j)/d,
+  // and after all k pivots A = -A_in^{-1}.  Column j of the symmetric matrix
+  // is read from a double-buffered LDS snapshot written by its owners (column
+  // j below the diagonal, row j left of it) during the previous pivot, so each
+  // pivot costs one barrier.  A non-positive pivot is the Cholesky failure (a
+  // leading minor is not positive definite) and sends the slice to the LU
+  // fallback, as the reference's solve(likely_sympd) does.
+  {
+    constexpr int G = 32, R = 4;  // grid side, register blocks per side
+    __shared__ double snap[2][128];
+    const int tr = t / G, tc = t % G;
+    const bool active = t < G * G;
+    auto lower = [&](int a, int b) { return a > b || (a == b && tr >= tc); };
+    double v[R][R];
+#pragma unroll
+    for (int a = 0; a < R; ++a)
+#pragma unroll
+      for (int b = 0; b < R; ++b) {
+        const int i = tr + G * a, c = tc + G * b;
+        double x = 0.0;
+        if (active && i < k && c < k && lower(a, b)) {
+          x = Ain[i + (int64_t)c * lda];
+          if (i == c) x += shift;
+        }
+        v[a][b] = x;
       }
-      const double ilqq = 1.0 / lqq;
-      for (int c = ty; c <= q; c += 32) {
-        const double zq = Z[q + c * LD] * ilqq;
-        for (int i = q + 1 + tx; i < k; i += 32) Z[i + c * LD] -= L[i + q * LD] * zq;
+    // snapshot of column jn: owners of (i, jn), i >= jn, and of (jn, c), c < jn
+    auto publish = [&](int jn, double* dst) {
+      if (!active) return;
+      if (tc == jn % G) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b) {
+            const int i = tr + G * a;
+            if (b == jn / G && lower(a, b) && i < k) dst[i] = v[a][b];
+          }
       }
+      if (tr == jn % G) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b) {
+            const int c = tc + G * b;
+            if (a == jn / G && lower(a, b) && c < jn) dst[c] = v[a][b];
+          }
+      }
+    };
+    if (t < 256) (&snap[0][0])[t] = 0.0;  // 2 x 128 entries
+    __syncthreads();
+    publish(0, snap[0]);
+    __syncthreads();
+    for (int j = 0; j < k; ++j) {
+      const double* cur = snap[j & 1];
+      // all snapshot reads issued together (entries past k are never used
+      // for outputs, so no bounds test is needed on the 128-entry buffer)
+      const double d = cur[j];
+      double ns[R], sc[R];
+#pragma unroll
+      for (int a = 0; a < R; ++a) ns[a] = -cur[(tr + G * a) & 127];
+#pragma unroll
+      for (int b = 0; b < R; ++b) sc[b] = cur[(tc + G * b) & 127];
+      if (!(d > 0.0) || !isfinite(d)) {  // uniform: every thread reads the same pivot
+        if (t == 0) chol_ok = 0;
+        break;
+      }
+      const double rd = 1.0 / d;
+#pragma unroll
+      for (int b = 0; b < R; ++b) sc[b] *= rd;
+#pragma unroll
+      for (int a = 0; a < R; ++a)
+#pragma unroll
+        for (int b = 0; b < R; ++b)
+          if (lower(a, b)) v[a][b] = fma(ns[a], sc[b], v[a][b]);
+      // row j and column j of the swept matrix
+      if (tr == j % G) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b) {
+            const int c = tc + G * b;
+            if (a == j / G && lower(a, b) && c <= j) v[a][b] = (c == j) ? -rd : sc[b];
+          }
+      }
+      if (tc == j % G) {
+#pragma unroll
+        for (int a = 0; a < R; ++a)
+#pragma unroll
+          for (int b = 0; b < R; ++b) {
+            const int i = tr + G * a;
+            if (b == j / G && lower(a, b) && i > j) v[a][b] = -ns[a] * rd;
+          }
+      }
+      if (j + 1 < k) publish(j + 1, snap[(j + 1) & 1]);
       __syncthreads();
     }
-    {
-      const double r = 1.0 / L[(k - 1) + (k - 1) * LD];
-      for (int c = t; c < k; c += nt) Z[(k - 1) + c * LD] *= r;
-    }
     __syncthreads();
-    // ---- X = Z^T Z diag(rhs), Z lower triangular.  Each wave owns a 32 x 32
-    // output block as an 8 x 8 grid of 4 x 4 register tiles, so a step over r
-    // reads 64 distinct Z values per wave (the rest are LDS broadcasts).
-    {
-      const int wv = t >> 6, ln = t & 63;
-      const int i0 = (wv & 3) * 32 + (ln & 7) * 4, j0 = (wv >> 2) * 32 + (ln >> 3) * 4;
-      const int wi0 = (wv & 3) * 32, wj0 = (wv >> 2) * 32;
-      if (wi0 < k && wj0 < k) {
-        double acc[4][4];
+    if (chol_ok) {
+      // X = A^{-1} diag(rhs) = -A_swept diag(rhs), both triangles from the lower
+      if (active) {
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int a = 0; a < R; ++a)
 #pragma unroll
-          for (int y = 0; y < 4; ++y) acc[x][y] = 0.0;
-        // rows below max(i, j) vanish in both factors
-        for (int r = wi0 > wj0 ? wi0 : wj0; r < k; ++r) {
-          double zi[4], zj[4];
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            zi[x] = i0 + x < k ? Z[r + (i0 + x) * LD] : 0.0;
-            zj[x] = j0 + x < k ? Z[r + (j0 + x) * LD] : 0.0;
+          for (int b = 0; b < R; ++b) {
+            const int i = tr + G * a, c = tc + G * b;
+            if (!lower(a, b) || i >= k || c >= k) continue;
+            const double x = -v[a][b];
+            Xs[i + (int64_t)c * ldx] = (mode == 1) ? x : x * dvals[c];
+            if (i != c) Xs[c + (int64_t)i * ldx] = (mode == 1) ? x : x * dvals[i];
           }
-#pragma unroll
-          for (int x = 0; x < 4; ++x)
-#pragma unroll
-            for (int y = 0; y < 4; ++y) acc[x][y] = fma(zi[x], zj[y], acc[x][y]);
-        }
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          const int j = j0 + y;
-          if (j >= k) continue;
-          const double sc = (mode == 1) ? 1.0 : dvals[j];
-#pragma unroll
-          for (int x = 0; x < 4; ++x)
-            if (i0 + x < k) Xs[(i0 + x) + (int64_t)j * ldx] = acc[x][y] * sc;
-        }
       }
+      if (t == 0 && status) status[s] = 0;
+      return;
     }
-    if (t == 0 && status) status[s] = 0;
-    return;
   }
   // LU with partial pivoting (Doolittle, in place)
   __syncthreads();
@@ -254,14 +273,8 @@ void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const st
     double* Xc = X + (int64_t)c0 * slice_stride;
     double* lc = lambda_out ? lambda_out + c0 : nullptr;
     int32_t* sc = status ? status + c0 : nullptr;
-    if (kmax <= 100) {
-      launch(ctx, "spd_solve", k_spd_solve<100, true>, dim3(cnt), dim3(1024), 0, A, lda, so,
-             mode, dvals, ridge, Xc, ldx, slice_stride, lc, sc, (double*)nullptr);
-    } else {
-      DBuf<double> Zg(ctx, (size_t)cnt * 128 * 129);
-      launch(ctx, "spd_solve", k_spd_solve<128, false>, dim3(cnt), dim3(1024), 0, A, lda, so,
-             mode, dvals, ridge, Xc, ldx, slice_stride, lc, sc, Zg.get());
-    }
+    launch(ctx, "spd_solve", k_spd_solve<128>, dim3(cnt), dim3(1024), 0, A, lda, so, mode, dvals,
+           ridge, Xc, ldx, slice_stride, lc, sc);
   }
 }
 
