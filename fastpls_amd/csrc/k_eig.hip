@@ -79,7 +79,8 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
   __shared__ double rc[EN], rs[EN], dnc[EN], znc[EN], dorgv[EN];
   __shared__ int perm[EN], defl[EN], ndidx[EN], rootidx[EN], orig[EN], colpos[EN];
   __shared__ int rp[EN], rq[EN], iperm[EN];
-  __shared__ int nrot[EN / 2], Kb[EN / 2];
+  __shared__ int nrot[EN / 2], Kb[EN / 2], needser[EN / 2];
+  __shared__ unsigned long long bmaxv[EN / 2];
   __shared__ double red[ENT / 64];
   __shared__ double s_scale;
 
@@ -253,16 +254,49 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
     }
     __syncthreads();
     FPLS_EIG_STAMP(2);
-    // (b) deflation, one thread per merge (serial scan in sorted order);
-    //     the K surviving poles and weights are packed into dnc / znc
+    // (b) deflation.  A parallel pre-pass finds the block max |d| (for the
+    //     tolerance), the tiny-z flags, and tests every surviving z against its
+    //     surviving predecessor with the unrotated values.  When no pair of a
+    //     merge passes that test the serial scan would rotate nothing, so the
+    //     survivors are packed in parallel; otherwise one thread runs the
+    //     serial scan (dlaed2's order).  K survivors go to dnc / znc.
+    if (t < nblk) {
+      bmaxv[t] = 0ull;
+      needser[t] = 0;
+    }
+    __syncthreads();
+    if (t < n && ((t >> lg) << lg) + size < n)
+      atomicMax(&bmaxv[t >> lg], (unsigned long long)__double_as_longlong(fabs(Ds[t])));
+    __syncthreads();
+    auto tol_of = [&](int bi) {
+      const int m = (bi << lg) + size;
+      const double rho2 = 2.0 * fabs(ev[m - 1]);
+      return 8.0 * EEPS * fmax(__longlong_as_double((long long)bmaxv[bi]), rho2);
+    };
+    if (t < n) {
+      const int bi = t >> lg, m = (bi << lg) + size;
+      if (m < n) defl[t] = (2.0 * fabs(ev[m - 1]) * fabs(zs[t]) <= tol_of(bi)) ? 1 : 0;
+    }
+    __syncthreads();
+    if (t < n) {
+      const int bi = t >> lg, a = bi << lg, m = a + size;
+      if (m < n && !defl[t]) {
+        int pv = t - 1;
+        while (pv >= a && defl[pv]) --pv;
+        if (pv >= a) {
+          const double zp = zs[pv], zq = zs[t];
+          const double it = 1.0 / sqrt(zp * zp + zq * zq);
+          if (fabs((Ds[t] - Ds[pv]) * (zq * it) * (zp * it)) <= tol_of(bi)) needser[bi] = 1;
+        }
+      }
+    }
+    __syncthreads();
     if (t < nblk) {
       const int a = t << lg, m = a + size, b = min(a + bsz, n), mm = b - a;
       int K = 0, nr = 0;
-      if (m < n) {
+      if (m < n && needser[t]) {
+        const double tol = tol_of(t);
         const double rho2 = 2.0 * fabs(ev[m - 1]);
-        double dmax = 0.0;
-        for (int q = 0; q < mm; ++q) dmax = fmax(dmax, fabs(Ds[a + q]));
-        const double tol = 8.0 * EEPS * fmax(dmax, rho2);
         // the state of the current candidate (prev) stays in registers; entries
         // q are read before anything writes them, so the loads pipeline
         int prev = -1;
@@ -314,9 +348,22 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
           znc[a + K] = zprev;
           ++K;
         }
+      } else if (m < n) {
+        for (int q = 0; q < mm; ++q) K += defl[a + q] ? 0 : 1;
       }
       Kb[t] = K;
       nrot[t] = nr;
+    }
+    if (t < n) {
+      const int bi = t >> lg, a = bi << lg, m = a + size;
+      if (m < n && !needser[bi] && !defl[t]) {
+        int r = 0;
+        for (int q = a; q < t; ++q) r += defl[q] ? 0 : 1;
+        ndidx[a + r] = t - a;
+        rootidx[t] = r;
+        dnc[a + r] = Ds[t];
+        znc[a + r] = zs[t];
+      }
     }
     __syncthreads();
     const int slot = t >> 3, g = t & 7;
