@@ -1,11 +1,15 @@
 """fastPLS hot path on MI355X (gfx950): HIP kernels behind a C ABI.
 
 Public API (mirrors the fastPLS R package with backend = "hip"):
-``pls``, ``predict``, ``fastsvd``, ``pls_single_cv``, ``optim_pls_cv``.
+``pls``, ``predict``, ``fastsvd``, ``pls_single_cv``, ``optim_pls_cv``,
+``pls_double_cv``; ``cv_batched`` runs many cross-validation fits as one
+batched launch.
 """
 from ._lib import Context, FplsError, available, context, device_colmajor, load  # noqa: F401
-from .api import (FastPLS, candidate_knn_predict, class_sums, fastsvd, pls, predict,  # noqa: F401
-                  truncated_rsvd_crossprod, optim_pls_cv, pls_single_cv, pls_cv_predict)
+from .api import (FastPLS, candidate_knn_predict, class_sums, cv_batched, fastsvd, pls,  # noqa: F401
+                  predict, truncated_rsvd_crossprod, optim_pls_cv, pls_double_cv, pls_single_cv,
+                  pls_cv_predict)
 
-__all__ = ["pls", "predict", "fastsvd", "optim_pls_cv", "pls_single_cv", "pls_cv_predict", "FastPLS", "Context", "FplsError", "available",
+__all__ = ["pls", "predict", "fastsvd", "optim_pls_cv", "pls_single_cv", "pls_cv_predict",
+           "pls_double_cv", "cv_batched", "FastPLS", "Context", "FplsError", "available",
            "context", "device_colmajor", "load"]

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from typing import Optional, Sequence
 
 import numpy as np
@@ -109,6 +110,10 @@ SIGNATURES = {
                                        ctypes.c_int32, ctypes.c_double, ctypes.c_int32,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_int32, _vp, _vp,
                                        ctypes.c_int32, _P(ctypes.c_int32), _vp, _vp]),
+    "fpls_cv_batched": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp,
+                                       ctypes.c_int32, _vp, _vp, ctypes.c_int32, _P(Opts), _vp,
+                                       _vp, _vp, _vp]),
+    "fpls_gaussian_streams": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp]),
 }
 
 _lib = None
@@ -208,6 +213,11 @@ class Context:
     """One HIP stream + HBM pool on one device (fpls_ctx)."""
 
     def __init__(self, device: int = 0):
+        # torch bundles its own HIP runtime, which cannot initialise after this
+        # library's has: when torch is in use, let it go first
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_available():
+            torch.cuda.init()
         lib = load()
         h = ctypes.c_void_p()
         rc = lib.fpls_ctx_create(device, ctypes.byref(h))

@@ -281,12 +281,102 @@ def _cv_best_index(names, values) -> int:
     return int(np.argmax(np.where(fin, v, -np.inf)))
 
 
+def cv_batched(Xdata, Ydata, masks, ncomp=2, seeds=None, scaling="centering", seed=1,
+               precision="fp32", n_classes=None, return_class_pred=True, return_scores=False,
+               device=0) -> Dict:
+    """All fits of a cross-validation (or of every inner loop of a double
+    cross-validation) as one batched SIMPLS-fast call (``fpls_cv_batched``).
+
+    ``masks`` is n x F (0 unused, 1 train, 2 test row of fit f); ``seeds`` the
+    F seeds of the fits (default ``seed + f``, the CV driver's rule,
+    src/fastPLS.cpp:6906).  Returns ``status`` (F), and for the test rows of
+    each fit ``class_pred`` (n x L x F), ``n_correct`` (L x F) and, when asked
+    or for regression, ``Ypred`` (n x m x L x F)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    X = as_mat(Xdata, keep)
+    n = X.rows
+    M = np.asfortranarray(np.asarray(masks, dtype=np.int8))
+    if M.ndim != 2 or M.shape[0] != n:
+        raise ValueError("masks must be n x n_fits")
+    keep.append(M)
+    F = M.shape[1]
+    classification = _is_factor(Ydata)
+    yv = Ym = None
+    C = 0
+    lev = None
+    if classification:
+        lev, codes, C = _encode_factor(Ydata, n_classes)
+        yv = as_ivec(codes, keep)
+        m = C
+    else:
+        Ym = as_mat(Ydata, keep)
+        m = Ym.cols
+    nc_arr, nc_ptr = int_array(np.atleast_1d(ncomp))
+    L = nc_arr.size
+    opts = _lib.default_opts(scaling=SCALINGS[scaling], seed=int(seed) & 0xFFFFFFFF,
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32)
+    sd = None
+    if seeds is not None:
+        sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64).ravel() & 0xFFFFFFFF,
+                                  dtype=np.uint32)
+        if sd.size != F:
+            raise ValueError("seeds must have one value per fit")
+        keep.append(sd)
+    status = np.zeros(F, dtype=np.int32)
+    cls = np.zeros((n, L, F), dtype=np.int32, order="F") if classification and return_class_pred else None
+    ncor = np.zeros((L, F), dtype=np.int32, order="F") if classification else None
+    want_scores = (not classification) or return_scores
+    yp = np.zeros((n, m, L, F), dtype=np.float64, order="F") if want_scores else None
+    ctx.check(ctx.lib.fpls_cv_batched(
+        ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
+        ctypes.byref(yv) if yv is not None else None, C, M.ctypes.data, F,
+        sd.ctypes.data if sd is not None else None, nc_ptr, L, ctypes.byref(opts),
+        status.ctypes.data, cls.ctypes.data if cls is not None else None,
+        ncor.ctypes.data if ncor is not None else None,
+        yp.ctypes.data if yp is not None else None))
+    return {"status": status, "class_pred": cls, "n_correct": ncor, "Ypred": yp,
+            "ncomp": nc_arr.astype(np.int64), "levels": lev, "classification": classification,
+            "n_test": (M == 2).sum(axis=0)}
+
+
+def _batched_eligible(Xdata, method, classifier, scaling, ngroups) -> bool:
+    if method != "simpls" or classifier != "argmax" or scaling == "autoscaling":
+        return False
+    if not 1 <= ngroups <= 128:
+        return False
+    if type(Xdata).__module__.startswith("torch"):
+        import torch
+        ok = Xdata.dtype == torch.float32
+        shape = Xdata.shape
+    else:
+        ok = np.asarray(Xdata).dtype == np.float32
+        shape = np.asarray(Xdata).shape
+    return ok and len(shape) == 2 and shape[1] > 5
+
+
 def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method="simpls",
                    classifier="argmax", lda_ridge=1e-8, candidate_knn_k=10, candidate_tau=0.2,
                    candidate_alpha=0.75, candidate_top_m=20, seed=1, rsvd_oversample=10,
-                   rsvd_power=1, precision="fp32", return_scores=False, device=0) -> Dict:
+                   rsvd_power=1, precision="fp32", return_scores=False, device=0,
+                   engine="auto") -> Dict:
     """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151), leave-one-group-out
-    over the distinct values of ``constrain``."""
+    over the distinct values of ``constrain``.
+
+    ``engine``: "folds" fits the folds one after another (every method and
+    classifier); "batched" runs all folds as one batched call (SIMPLS, argmax
+    classifier, FP32 X, scaling centering / none); "auto" takes "batched"
+    when it applies.  Both run on the GPU and give the same predictions up to
+    rounding."""
+    cons0 = np.asarray(constrain).ravel()
+    ngroups = int(np.unique(cons0).size)
+    if engine not in ("auto", "batched", "folds"):
+        raise ValueError("engine must be 'auto', 'batched' or 'folds'")
+    use_batched = engine == "batched" or (
+        engine == "auto" and _batched_eligible(Xdata, method, classifier, scaling, ngroups))
+    if use_batched:
+        return _pls_cv_predict_batched(Xdata, Ydata, cons0, ncomp, scaling, seed, precision,
+                                       return_scores, device, method, classifier)
     ctx = _lib.context(device)
     keep: list = []
     X = as_mat(Xdata, keep)
@@ -356,6 +446,133 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     out["metrics"] = {"ncomp_index": list(range(1, L + 1)), "metric_name": names,
                       "metric_value": values}
     return out
+
+
+def _cv_summary(out, Ydata, cls, yp, classification, lev, n, L):
+    names, values, preds = [], [], []
+    if classification:
+        out["class_pred"] = cls
+        truth = np.asarray(Ydata).ravel()
+        levs = np.asarray(lev, dtype=object)
+        for i in range(L):
+            ok = (cls[:, i] >= 1) & (cls[:, i] <= len(lev))
+            pred = np.full(n, None, dtype=object)
+            pred[ok] = levs[cls[ok, i] - 1]
+            preds.append(pred)
+            names.append("accuracy")
+            values.append(float(np.mean(pred == truth)))
+        if yp is not None:
+            out["Ypred"] = yp
+    else:
+        out["Ypred"] = yp
+        yt = np.asarray(Ydata, dtype=np.float64).reshape(n, -1)
+        for i in range(L):
+            preds.append(yp[:, :, i])
+            nm, v = _cv_metric(yt, yp[:, :, i])
+            names.append(nm)
+            values.append(v)
+    out["pred"] = preds
+    out["metrics"] = {"ncomp_index": list(range(1, L + 1)), "metric_name": names,
+                      "metric_value": values}
+    return out
+
+
+def _pls_cv_predict_batched(Xdata, Ydata, cons, ncomp, scaling, seed, precision, return_scores,
+                            device, method, classifier):
+    if method != "simpls" or classifier != "argmax":
+        raise ValueError("the batched engine runs SIMPLS with the argmax classifier")
+    groups = np.unique(cons)  # sorted, as arma::unique / std::set
+    fold = np.searchsorted(groups, cons).astype(np.int32)
+    n = cons.size
+    F = groups.size
+    masks = np.ones((n, F), dtype=np.int8, order="F")
+    masks[np.arange(n), fold] = 2
+    res = cv_batched(Xdata, Ydata, masks, ncomp, None, scaling, seed, precision,
+                     return_class_pred=True, return_scores=return_scores, device=device)
+    classification = res["classification"]
+    L = res["ncomp"].size
+    cls = yp = None
+    if classification:
+        cls = np.asfortranarray(res["class_pred"].sum(axis=2, dtype=np.int64).astype(np.int32))
+    if res["Ypred"] is not None:
+        yp = np.asfortranarray(res["Ypred"].sum(axis=3))
+    out = {"fold": fold + 1, "status": res["status"], "ncomp": res["ncomp"],
+           "method": method, "backend": "hip", "classifier": classifier,
+           "classification": classification, "levels": res["levels"], "engine": "batched"}
+    return _cv_summary(out, Ydata, cls, yp, classification, res["levels"], n, L)
+
+
+def pls_double_cv(Xdata, Ydata, ncomp=2, outer_fold=None, inner_fold=None, scaling="centering",
+                  seed=1, run=1, precision="fp32", device=0) -> Dict:
+    """One run of pls.double.cv (R/main.R:6586-6894) for SIMPLS with the argmax
+    classifier, on given folds (the reference draws them from R's RNG).
+
+    ``outer_fold`` (n) assigns rows to outer folds; ``inner_fold`` (n x
+    n_outer) assigns, for outer fold j, the rows outside it to inner folds
+    (entries of rows inside outer fold j are ignored).  Every inner fit of
+    every outer fold runs in one batched call, seeded
+    seed + 1000 run + f + inner fold (R/main.R:6708 and src/fastPLS.cpp:6906;
+    f counts outer folds from 1); the optimal ncomp of each outer fold
+    maximises inner accuracy (first maximum, R/main.R:6527-6536); the outer
+    fits (seed + 2000 run + f, R/main.R:6738) run as a second batched call."""
+    keep_shape = np.asarray(Xdata).shape if not type(Xdata).__module__.startswith("torch") \
+        else tuple(Xdata.shape)
+    n = keep_shape[0]
+    if not _is_factor(Ydata):
+        raise ValueError("pls_double_cv here covers classification (factor Ydata)")
+    lev, codes, C = _encode_factor(Ydata)
+    ofold = np.asarray(outer_fold).ravel()
+    outer_vals = np.unique(ofold)
+    J = outer_vals.size
+    inner = np.asarray(inner_fold).reshape(n, J)
+    nc = np.atleast_1d(np.asarray(ncomp, dtype=np.int64))
+    cols, seeds, owner = [], [], []
+    for j, ov in enumerate(outer_vals):
+        tr = ofold != ov
+        ivals = np.unique(inner[tr, j])
+        for fi, iv in enumerate(ivals):
+            mk = np.zeros(n, dtype=np.int8)
+            mk[tr] = 1
+            mk[tr & (inner[:, j] == iv)] = 2
+            cols.append(mk)
+            seeds.append(int(seed) + 1000 * int(run) + (j + 1) + fi)
+            owner.append(j)
+    masks = np.stack(cols, axis=1)
+    owner = np.asarray(owner)
+    inner_res = cv_batched(Xdata, np.asarray(Ydata), masks, nc, seeds, scaling, seed, precision,
+                           return_class_pred=False, device=device)
+    ncor = inner_res["n_correct"].astype(np.int64)  # L x F_inner
+    best = np.zeros(J, dtype=np.int64)
+    acc_inner = np.zeros((J, nc.size))
+    for j in range(J):
+        sel = owner == j
+        ntr = (ofold != outer_vals[j]).sum()
+        acc = ncor[:, sel].sum(axis=1) / max(int(ntr), 1)
+        acc_inner[j] = acc
+        best[j] = nc[int(np.argmax(acc))]
+    # outer fits: train on the outer-train rows, predict the outer fold, one
+    # slice per distinct optimal ncomp
+    omask = np.ones((n, J), dtype=np.int8, order="F")
+    for j, ov in enumerate(outer_vals):
+        omask[ofold == ov, j] = 2
+    grid = np.unique(best)
+    oseeds = [int(seed) + 2000 * int(run) + (j + 1) for j in range(J)]
+    outer_res = cv_batched(Xdata, np.asarray(Ydata), omask, grid, oseeds, scaling, seed,
+                           precision, return_class_pred=True, device=device)
+    pred_codes = np.zeros(n, dtype=np.int32)
+    for j in range(J):
+        s = int(np.searchsorted(grid, best[j]))
+        rows = omask[:, j] == 2
+        pred_codes[rows] = outer_res["class_pred"][rows, s, j]
+    levs = np.asarray(lev, dtype=object)
+    pred = np.full(n, None, dtype=object)
+    ok = pred_codes >= 1
+    pred[ok] = levs[pred_codes[ok] - 1]
+    truth = np.asarray(Ydata).ravel()
+    return {"Ypred": pred, "optim_comp": best, "inner_accuracy": acc_inner,
+            "inner_status": inner_res["status"], "outer_status": outer_res["status"],
+            "metric_name": "accuracy", "metric_value": float(np.mean(pred == truth)),
+            "fold": ofold, "backend": "hip", "method": "simpls"}
 
 
 def optim_pls_cv(Xdata, Ydata, ncomp=2, constrain=None, scaling="centering", method="simpls",

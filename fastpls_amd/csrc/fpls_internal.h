@@ -104,6 +104,9 @@ struct Ctx {
   int32_t* flags_host = nullptr;
   int nflags = 0;
   std::vector<Check> checks;
+  // Captured launch plan of the batched cross-validation (cvb.hip), kept while
+  // the shapes and X stay the same.
+  std::shared_ptr<void> cv_plan;
   int32_t* defer_check(int count, int code, const std::string& msg, int fail_value = 0);
   void flush_checks();
   void drop_checks() {
@@ -217,6 +220,20 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc);
 
+// The dense-B TN product with an explicit split-K count (gemm_tn_f32 picks
+// tn_default_splits), and the split-K skinny NN slabs of the batched fits.
+int64_t tn_default_splits(const Ctx* ctx, int64_t M, int64_t N, int64_t K);
+void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                   int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                   const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc,
+                   int64_t splits);
+void proj_splitk(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float* A,
+                 int64_t lda, const float* shA32, const float* Bpad, int64_t ldb, int64_t kchunk,
+                 int splits, float* part);
+// Device Gaussian streams (k_rng.hip): column s of out (len values, ld) is the
+// std::mt19937(seeds[s]) + std::normal_distribution<double>(0, 1) stream.
+void gaussian_dev(Ctx* ctx, const uint32_t* seeds_dev, int n_streams, int64_t len, double* out,
+                  int64_t ld);
 // Matrix-vector products on a (possibly centred) data matrix.
 //  y(M) = (A - 1 shift^T) x      (A: M x K col-major; FP32 or FP64)
 //  y(K) = (A - 1 shift^T)^T x

@@ -468,10 +468,15 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
                                                const float* B, int64_t ldb,
                                                const double* __restrict__ off,
                                                float* __restrict__ Cf, double* __restrict__ Cd,
-                                               int64_t ldc, int crow) {
+                                               int64_t ldc, int crow, int64_t kchunk,
+                                               float* __restrict__ part) {
   __shared__ __align__(16) float smem[PSmem<BF>::value];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t row0 = (int64_t)blockIdx.x * P_BM + w * 32;
+  // K slab of this workgroup (kchunk is a multiple of P_BK; B's zero rows pad
+  // the last slab up to a whole chunk)
+  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
   const int nb = (N + 15) >> 4;
   v4f acc[2][8];
 #pragma unroll
@@ -585,16 +590,16 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       }
     }
   };
-  const int nk = (int)((K + P_BK - 1) / P_BK);
-  fetchA(0, ac0, ac1, mc);
-  fetchB(0);
+  const int nk = (int)((kend - kbeg + P_BK - 1) / P_BK);
+  fetchA(kbeg, ac0, ac1, mc);
+  fetchB(kbeg);
   storeB(0);
   __syncthreads();
   for (int c = 0; c < nk; ++c) {
     const bool more = c + 1 < nk;
     if (more) {
-      fetchA((int64_t)(c + 1) * P_BK, an0, an1, mn);
-      fetchB((int64_t)(c + 1) * P_BK);
+      fetchA(kbeg + (int64_t)(c + 1) * P_BK, an0, an1, mn);
+      fetchB(kbeg + (int64_t)(c + 1) * P_BK);
     }
     // keep the next chunk's loads ahead of this chunk's MFMAs
     __builtin_amdgcn_sched_barrier(0);
@@ -611,6 +616,22 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
     __syncthreads();
   }
   // epilogue: lane holds D[4(l/16) + r][l%16] of each 16 x 16 block
+  if (part) {  // split-K: raw FP32 slab partial, M x N column-major
+    float* P = part + (int64_t)blockIdx.y * M * N;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int col = b * 16 + lane % 16;
+        if (b >= nb || col >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (rbase + r < M) P[rbase + r + (int64_t)col * M] = acc[a][b][r];
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < 2; ++a) {
     const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
@@ -656,17 +677,17 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       f64_to_f32(ctx, shf.get(), shiftA, K);
       if (bf)
         launch(ctx, "project_bf16x3", k_proj<true, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
-               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow);
+               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
       else
         launch(ctx, "project_f32", k_proj<false, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
-               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow);
+               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
     } else {
       if (bf)
         launch(ctx, "gemm_nn_bf16x3", k_proj<true, false>, pgrid, dim3(P_NT), 0, M, n32, K, A,
-               lda, (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow);
+               lda, (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
       else
         launch(ctx, "gemm_nn_f32", k_proj<false, false>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
-               (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow);
+               (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
     }
     return;
   }
@@ -1086,10 +1107,22 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
            (const float*)part.get(), shiftB, C, ldc);
     return;
   }
+  gemm_tn_dense(ctx, precision, M, N, K, A, lda, shiftA, B, ldb, labelsB, shiftB, C, ldc,
+                tn_default_splits(ctx, M, N, K));
+}
+
+int64_t tn_default_splits(const Ctx* ctx, int64_t M, int64_t N, int64_t K) {
   const int64_t tiles = ((M + G2_BM - 1) / G2_BM) * ((N + G2_BN - 1) / G2_BN);
   const int64_t want = 2 * ctx->num_cus;
   int64_t splits = std::max<int64_t>(1, (want + tiles - 1) / tiles);
-  splits = std::min<int64_t>(splits, (K + 511) / 512);  // at least 512 rows per slab
+  return std::min<int64_t>(splits, (K + 511) / 512);  // at least 512 rows per slab
+}
+
+void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
+                   int64_t lda, const double* shiftA, const float* B, int64_t ldb,
+                   const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc,
+                   int64_t splits) {
+  if (M <= 0 || N <= 0) return;
   const int64_t cap = std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N));
   splits = std::max<int64_t>(1, std::min(splits, cap));
   int64_t kchunk = (K + splits - 1) / splits;
@@ -1106,6 +1139,39 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
            shiftA, B, ldb, labelsB, shiftB, part.get());
   launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
          (const float*)part.get(), C, ldc);
+}
+
+// Split-K skinny NN (N <= 128): part[z] (M x N FP32, column-major) is slab z
+// (rows z kchunk ... of B) of (A - 1 shA^T) B.  B is the zero-padded FP32
+// operand (at least ceil(K / 32) 32 rows and 128 columns, ld ldb), so the
+// last slab needs no masking.
+void proj_splitk(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float* A,
+                 int64_t lda, const float* shA32, const float* Bpad, int64_t ldb, int64_t kchunk,
+                 int splits, float* part) {
+  FPLS_REQUIRE(N >= 1 && N <= 128 && kchunk % P_BK == 0 && splits >= 1 &&
+                   (int64_t)(splits - 1) * kchunk < K && (int64_t)splits * kchunk >= K,
+               FPLS_ERR_ARG, "proj_splitk: bad shape");
+  dim3 grid((unsigned)((M + P_BM - 1) / P_BM), (unsigned)splits);
+  const bool bf = precision == FPLS_PREC_BF16X3;
+  if (shA32) {
+    if (bf)
+      launch(ctx, "cv_project_bf16x3", k_proj<true, true>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+             shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+             (int64_t)0, 0, kchunk, part);
+    else
+      launch(ctx, "cv_project_f32", k_proj<false, true>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+             shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+             (int64_t)0, 0, kchunk, part);
+  } else {
+    if (bf)
+      launch(ctx, "cv_project_bf16x3", k_proj<true, false>, grid, dim3(P_NT), 0, M, N, K, A,
+             lda, (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
+             (double*)nullptr, (int64_t)0, 0, kchunk, part);
+    else
+      launch(ctx, "cv_project_f32", k_proj<false, false>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+             (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
+             (double*)nullptr, (int64_t)0, 0, kchunk, part);
+  }
 }
 
 // C (M x N col-major FP32, ld M) = A B^T, A: M x K and B: N x K row-major.

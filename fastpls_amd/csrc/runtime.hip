@@ -266,6 +266,7 @@ void fpls_ctx_destroy(fpls_ctx* ctx) {
   for (auto e : ctx->c.spare_events) (void)hipEventDestroy(e);
   if (ctx->c.flags_dev) (void)hipFree(ctx->c.flags_dev);
   if (ctx->c.flags_host) (void)hipHostFree(ctx->c.flags_host);
+  ctx->c.cv_plan.reset();
   ctx->c.pool.release_all();
   (void)hipStreamDestroy(ctx->c.stream);
   delete ctx;

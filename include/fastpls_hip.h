@@ -35,6 +35,9 @@
  *   fpls_rsvd                 truncated_svd_cpu_rsvd / fastsvd  src/svd_cpu_rsvd.cpp:95-124
  *   fpls_rsvd_xprod           truncated_rsvd_crossprod_double   src/fastPLS.cpp:550-623
  *   fpls_class_sums           rowsum() class sums of the label-aware stream R/main.R:563-588
+ *   fpls_cv_batched           the fold loops of pls_cv_predict_compiled src/fastPLS.cpp:6777-7142
+ *                             and pls.double.cv R/main.R:6676-6764
+ *   fpls_gaussian_streams     gaussian_matrix_local      src/fastPLS.cpp:99-115
  */
 #ifndef FASTPLS_HIP_H
 #define FASTPLS_HIP_H
@@ -244,6 +247,38 @@ int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const f
                     double alpha, int32_t top_m, int32_t* fold_out, int32_t* status_out,
                     int32_t status_cap, int32_t* n_folds_out, int32_t* class_pred,
                     double* ypred);
+
+/* Batched cross-validation: n_fits SIMPLS-fast fits (pls_model2_fast,
+   src/fastPLS.cpp:2454-2778) over one resident FP32 X, each defined by a row
+   mask, advanced together and replayed as one captured HIP graph.  Replaces
+   the per-fold loop of pls_cv_predict_compiled (src/fastPLS.cpp:6777-7142,
+   argmax classifier or regression) and, with the fits of every outer fold in
+   one batch, the inner loops of pls.double.cv (R/main.R:6676-6764).
+     masks   n x n_fits (column-major, host): 0 unused, 1 train, 2 test row
+     seeds   n_fits seeds of the Gaussian starts (NULL: opts->seed + fit)
+     y / Y   labels 1..n_classes (classification) or responses (regression)
+   Outputs: status_out (n_fits: 1 ok, 2 no test rows, 3 no training rows,
+   4 fewer than two training classes), and for the test rows of every fit
+   class_pred (n x n_ncomp x n_fits, 0 elsewhere), n_correct (n_ncomp x
+   n_fits, the accuracy numerators of optim.pls.cv, R/main.R:3744-3763) and
+   ypred (n x m x n_ncomp x n_fits: predictions, or argmax scores for
+   classification); each output may be NULL.  Limits: p > 5, n_fits <= 128,
+   m <= 512, ncomp <= 512, scaling 1 or 3. */
+int fpls_cv_batched(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
+                    int32_t n_classes, const int8_t* masks, int32_t n_fits,
+                    const uint32_t* seeds, const int32_t* ncomp, int32_t n_ncomp,
+                    const fpls_opts* opts, int32_t* status_out, int32_t* class_pred,
+                    int32_t* n_correct, double* ypred);
+
+/* ---- Gaussian sketch streams ---------------------------------------------------- */
+/* gaussian_matrix_local (src/fastPLS.cpp:99-115) drawn on the device: column s
+   of out (n_streams x len doubles, column-major, host) is the
+   std::mt19937(seeds[s]) + std::normal_distribution<double>(0, 1) stream.
+   Exact integer and IEEE-rounded arithmetic throughout; log is correctly
+   rounded, so draws may differ from glibc's by an ulp or two where glibc's
+   log is not (about 0.1% of pairs). */
+int fpls_gaussian_streams(fpls_ctx* ctx, const uint32_t* seeds, int32_t n_streams, int64_t len,
+                          double* out);
 
 /* ---- streaming primitive ------------------------------------------------------ */
 /* Class sums (p x C, FP64) and column sums (p) of X in one pass from int32

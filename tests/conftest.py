@@ -22,6 +22,11 @@ def _gpu_ok():
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch ships its own HIP runtime, which only initialises before the
+    # library's (tests hand torch device tensors to the library)
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.init()
     import fastpls_amd
     if not _gpu_ok():
         pytest.fail("GPU test requested but no gfx950 device / library is available")
