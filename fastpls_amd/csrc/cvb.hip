@@ -41,6 +41,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -83,8 +84,14 @@ struct CvArgs {
   double* npart;                           // F x nch
   double* cpart;                           // F x nch
   const float* projpart;                   // nsplit x n x F
-  double* T64;                             // n x F
-  float* Tw32;                             // n x F
+  double* T64;                             // n x F (t before division by |t|)
+  int bf;                                  // split-BF16x3 operands for the TN product
+  int64_t npad;                            // rows of the TN operand (n rounded up to 32)
+  float* Tw32;                             // npad x F: masked t (FP32 mode)
+  __bf16* Twh;                             // npad x F: masked t, bf16 hi (split mode)
+  __bf16* Twl;                             // npad x F: masked t, bf16 lo
+  int nblk;                                // row blocks of k_cvb_treduce
+  double* tpart;                           // F x nblk x (m + 2) partial sums
   double* q;                               // F x m
   double* stv;                             // F
   double* tnv;                             // F
@@ -336,72 +343,76 @@ __global__ __launch_bounds__(CV_NT) void k_cvb_rr(CvArgs A) {
 }
 
 // ---------------------------------------------------------------- append
-// One workgroup per fit: t = (slab sum of T) - d.r, |t| over the training
-// rows, the stop test, t /= |t| and r /= |t|, the masked FP32 copy for the
-// TN product, 1^T w t and q = Yc^T t over the training rows.
-__global__ __launch_bounds__(1024) void k_cvb_tstats(CvArgs A) {
-  __shared__ double red[16];
-  const int f = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int64_t n = A.n, p = A.p, F = A.F;
-  const int8_t* mk = A.mask + (int64_t)f * n;
-  double* T = A.T64 + (int64_t)f * n;
-  float* Tw = A.Tw32 + (int64_t)f * n;
+// t before its division by |t| (grid: row blocks x fits): slab sums of the NN
+// product minus d.r, the masked operand of the TN product (FP32, or split into
+// bf16 hi / lo), and per block the training-row sums of each class's t (or of
+// t (Y_c - gy_c)), of t^2 and of t.  The TN product and q are linear in t, so
+// the division by |t| is applied to their results.
+__device__ inline void cv_split_bf(float x, __bf16& hi, __bf16& lo) {
+  hi = static_cast<__bf16>(x);
+  lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+__global__ __launch_bounds__(CV_NT) void k_cvb_treduce(CvArgs A) {
+  __shared__ double red[CV_NT / 64][CV_MAX_M + 2];
+  const int f = blockIdx.y, t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t n = A.n, F = A.F;
+  const int64_t i = (int64_t)blockIdx.x * CV_NT + t;
+  const int64_t oi = i + (int64_t)f * A.npad;
   if (!A.act[f]) {
-    for (int64_t i = t; i < n; i += 1024) Tw[i] = 0.f;
+    if (i < n) {
+      if (A.bf) {
+        A.Twh[oi] = static_cast<__bf16>(0.f);
+        A.Twl[oi] = static_cast<__bf16>(0.f);
+      } else {
+        A.Tw32[oi] = 0.f;
+      }
+    }
     return;
   }
   double cf = 0.0;
   for (int q = 0; q < A.nch; ++q) cf += A.cpart[(int64_t)f * A.nch + q];
-  double s2 = 0.0;
-  for (int64_t i = t; i < n; i += 1024) {
-    double v = 0.0;
+  double v = 0.0;
+  bool tr = false;
+  if (i < n) {
     for (int z = 0; z < A.nsplit; ++z) v += (double)A.projpart[(int64_t)z * n * F + i + (int64_t)f * n];
     v -= cf;
-    T[i] = v;
-    if (mk[i] == 1) s2 = fma(v, v, s2);
-  }
-  const double tn = sqrt(cv_block_sum(s2, red));
-  if (!(isfinite(tn) && tn > 0.0)) {
-    if (t == 0) A.act[f] = 0;
-    for (int64_t i = t; i < n; i += 1024) Tw[i] = 0.f;
-    return;
-  }
-  double st = 0.0;
-  for (int64_t i = t; i < n; i += 1024) {
-    const double v = T[i] / tn;
-    T[i] = v;
-    const bool tr = mk[i] == 1;
-    Tw[i] = tr ? (float)v : 0.f;
-    if (tr) st += v;
-  }
-  for (int64_t j = t; j < p; j += 1024) A.rr[(int64_t)f * p + j] /= tn;
-  st = cv_block_sum(st, red);  // its barriers also publish T to the block
-  const int m = A.m;
-  for (int c = w; c < m; c += 16) {
-    double s = 0.0;
-    if (A.classification) {
-      for (int64_t i = lane; i < n; i += 64)
-        if (mk[i] == 1 && A.y[i] == c + 1) s += T[i];
+    A.T64[i + (int64_t)f * n] = v;
+    tr = A.mask[i + (int64_t)f * n] == 1;
+    const float tw = tr ? (float)v : 0.f;
+    if (A.bf) {
+      __bf16 hh, ll;
+      cv_split_bf(tw, hh, ll);
+      A.Twh[oi] = hh;
+      A.Twl[oi] = ll;
     } else {
-      const double g = A.gy[c];
-      for (int64_t i = lane; i < n; i += 64)
-        if (mk[i] == 1) s = fma(T[i], cv_yval(A, i, c) - g, s);
-    }
-    s = wave_sum_d(s);
-    if (lane == 0) {
-      const double mu = A.classification ? A.mY[(int64_t)f * m + c]
-                                         : A.mY[(int64_t)f * m + c] - A.gy[c];
-      A.q[(int64_t)f * m + c] = s - mu * st;
+      A.Tw32[oi] = tw;
     }
   }
-  if (t == 0) {
-    A.stv[f] = st;
-    A.tnv[f] = tn;
+  const int m = A.m;
+  const double vt = tr ? v : 0.0;
+  for (int c = 0; c < m; ++c) {
+    double x;
+    if (A.classification) x = (tr && A.y[i] == c + 1) ? v : 0.0;
+    else x = tr ? v * (cv_yval(A, i, c) - A.gy[c]) : 0.0;
+    x = wave_sum_d(x);
+    if (lane == 0) red[w][c] = x;
   }
+  const double s2 = wave_sum_d(vt * vt), s1 = wave_sum_d(vt);
+  if (lane == 0) {
+    red[w][m] = s2;
+    red[w][m + 1] = s1;
+  }
+  __syncthreads();
+  for (int q = t; q < m + 2; q += CV_NT)
+    A.tpart[((int64_t)f * A.nblk + blockIdx.x) * (m + 2) + q] =
+        (red[0][q] + red[1][q]) + (red[2][q] + red[3][q]);
 }
 
-// p_f = P_f - d_f (1^T w t) and the partial sums of V_f^T p_f (or of |p|^2
-// for the first component).
+// |t| from the row-block sums, the stop test (src/fastPLS.cpp:2580-2583), and
+// (first chunk) q = Yc^T t, 1^T w t and |t| for later kernels; then
+// p_f = (P_f - d_f 1^T w T) / |t|, r /= |t|, and the partial sums of V_f^T p_f
+// (or of |p|^2 for the first component).
 __global__ __launch_bounds__(CV_NT) void k_cvb_vproj(CvArgs A, int a) {
   __shared__ double vec[CV_CH];
   __shared__ double red[CV_NT / 64];
@@ -409,19 +420,46 @@ __global__ __launch_bounds__(CV_NT) void k_cvb_vproj(CvArgs A, int a) {
   if (!A.act[f]) return;
   const int64_t p = A.p, j0 = (int64_t)ch * CV_CH;
   const int rows = (int)std::min<int64_t>(CV_CH, p - j0);
-  const double st = A.stv[f];
-  double s2 = 0.0;
+  const int m = A.m, m2 = A.m + 2;
+  const double* tp = A.tpart + (int64_t)f * A.nblk * m2;
+  double s2 = 0.0, s1 = 0.0;
+  for (int b = 0; b < A.nblk; ++b) {
+    s2 += tp[(int64_t)b * m2 + m];
+    s1 += tp[(int64_t)b * m2 + m + 1];
+  }
+  const double tn = sqrt(s2);
+  if (!(isfinite(tn) && tn > 0.0)) {
+    // every workgroup of the fit computes the same |t| and stops here
+    if (ch == 0 && t == 0) A.act[f] = 0;
+    return;
+  }
+  const double st = s1 / tn;
+  if (ch == 0) {
+    for (int c = t; c < m; c += CV_NT) {
+      double cs = 0.0;
+      for (int b = 0; b < A.nblk; ++b) cs += tp[(int64_t)b * m2 + c];
+      const double mu = A.classification ? A.mY[(int64_t)f * m + c]
+                                         : A.mY[(int64_t)f * m + c] - A.gy[c];
+      A.q[(int64_t)f * m + c] = (cs - mu * s1) / tn;
+    }
+    if (t == 0) {
+      A.stv[f] = st;
+      A.tnv[f] = tn;
+    }
+  }
+  double pp = 0.0;
   for (int r = t; r < rows; r += CV_NT) {
     const int64_t j = (int64_t)f * p + j0 + r;
-    const double pj = A.P[j] - A.d[j] * st;
+    A.rr[j] /= tn;
+    const double pj = A.P[j] / tn - A.d[j] * st;
     vec[r] = pj;
     A.vbuf[j] = pj;
-    s2 = fma(pj, pj, s2);
+    pp = fma(pj, pj, pp);
   }
   __syncthreads();
   if (a == 0) {
-    s2 = cv_block_sum(s2, red);
-    if (t == 0) A.v2part[(int64_t)f * A.nch + ch] = s2;
+    pp = cv_block_sum(pp, red);
+    if (t == 0) A.v2part[(int64_t)f * A.nch + ch] = pp;
   } else {
     cv_chunk_colsums(A.V + (int64_t)f * A.K * p + j0, p, rows, a, vec,
                      A.vpart + ((int64_t)f * A.nch + ch) * A.K, 1);
@@ -494,7 +532,7 @@ __global__ __launch_bounds__(CV_NT) void k_cvb_pred(CvArgs A, const int* sl, int
   double* Y = A.Yacc + (int64_t)f * m * n + i;
   const double* mu = A.mY + (int64_t)f * m;
   if (A.act[f]) {
-    const double tv = A.T64[i + (int64_t)f * n];
+    const double tv = A.T64[i + (int64_t)f * n] / A.tnv[f];
     const double* qf = A.q + (int64_t)f * m;
     for (int c = 0; c < m; ++c) Y[(int64_t)c * n] = fma(tv, qf[c], Y[(int64_t)c * n]);
   }
@@ -614,10 +652,11 @@ static void cv_record(Ctx* c, CvPlan& pl) {
     proj_splitk(c, pl.precision, pl.n, pl.F, pl.p, (const float*)pl.X, pl.ldx,
                 pl.scaling == 3 ? nullptr : pl.g32, pl.A.RR32, pl.Kp, pl.nn_kchunk, pl.nsplit,
                 pl.projw);
-    launch(c, "cv_tstats", k_cvb_tstats, dim3((unsigned)pl.F), dim3(1024), 0, A);
-    gemm_tn_dense(c, pl.precision, pl.p, pl.F, pl.n, (const float*)pl.X, pl.ldx,
-                  pl.scaling == 3 ? nullptr : pl.g, A.Tw32, pl.n, nullptr, nullptr, pl.Pw, pl.p,
-                  pl.tn_splits);
+    launch(c, "cv_treduce", k_cvb_treduce, dim3((unsigned)A.nblk, (unsigned)pl.F), dim3(CV_NT),
+           0, A);
+    tn_skinny(c, pl.precision, pl.p, pl.F, pl.n, (const float*)pl.X, pl.ldx,
+              pl.scaling == 3 ? nullptr : pl.g, A.Tw32, A.Twh, A.Twl, A.npad, pl.tn_splits, pl.Pw,
+              pl.p);
     launch(c, "cv_vproj", k_cvb_vproj, chunks, dim3(CV_NT), 0, A, a);
     if (a > 0) launch(c, "cv_vorth", k_cvb_vorth, chunks, dim3(CV_NT), 0, A, a);
     launch(c, "cv_vnorm", k_cvb_vnorm, chunks, dim3(CV_NT), 0, A, a);
@@ -648,6 +687,9 @@ static double cv_time(Ctx* c, Fn&& run) {
 static void cv_autotune(Ctx* c, CvPlan& pl) {
   const bool timing = c->timing;
   c->timing = false;
+  // FPLS_TUNE_LOG=1 prints every candidate's time to stderr
+  const char* lg = getenv("FPLS_TUNE_LOG");
+  const bool log = lg && lg[0] == '1';
   // NN: slabs of at least 256 rows of p, at most 32 slabs
   double best = 1e30;
   int best_s = 1;
@@ -660,6 +702,7 @@ static void cv_autotune(Ctx* c, CvPlan& pl) {
       proj_splitk(c, pl.precision, pl.n, pl.F, pl.p, (const float*)pl.X, pl.ldx, pl.g32,
                   pl.A.RR32, pl.Kp, kchunk, s, pl.projw);
     });
+    if (log) fprintf(stderr, "cv autotune: NN splits %d  %.1f us\n", s, 1e3 * ms);
     if (ms < best) {
       best = ms;
       best_s = s;
@@ -667,16 +710,16 @@ static void cv_autotune(Ctx* c, CvPlan& pl) {
   }
   pl.nsplit = best_s;
   pl.nn_kchunk = ((pl.p + best_s - 1) / best_s + 31) / 32 * 32;
-  // TN: split counts around the default
-  const int64_t def = tn_default_splits(c, pl.p, pl.F, pl.n);
+  // TN: slabs of at least 256 rows of n
   best = 1e30;
-  int64_t best_t = def;
-  for (int64_t s : {def / 4, def / 2, def, def * 2, def * 4}) {
-    if (s < 1 || s > std::max<int64_t>(1, pl.n / 256)) continue;
+  int64_t best_t = 1;
+  for (int64_t s : {1, 2, 4, 8, 16, 32}) {
+    if (s > 1 && pl.n / s < 256) continue;
     const double ms = cv_time(c, [&] {
-      gemm_tn_dense(c, pl.precision, pl.p, pl.F, pl.n, (const float*)pl.X, pl.ldx, pl.g,
-                    pl.A.Tw32, pl.n, nullptr, nullptr, pl.Pw, pl.p, s);
+      tn_skinny(c, pl.precision, pl.p, pl.F, pl.n, (const float*)pl.X, pl.ldx, pl.g, pl.A.Tw32,
+                pl.A.Twh, pl.A.Twl, pl.A.npad, s, pl.Pw, pl.p);
     });
+    if (log) fprintf(stderr, "cv autotune: TN splits %ld  %.1f us\n", (long)s, 1e3 * ms);
     if (ms < best) {
       best = ms;
       best_t = s;
@@ -768,7 +811,16 @@ static std::shared_ptr<CvPlan> cv_build_plan(Ctx* c, const DMat& X, const DMat* 
   P.projw = P.dalloc<float>((size_t)32 * n * F);
   A.projpart = P.projw;
   A.T64 = P.dalloc<double>((size_t)n * F);
-  A.Tw32 = P.dalloc<float>((size_t)n * F);
+  A.bf = o.precision == FPLS_PREC_BF16X3 ? 1 : 0;
+  A.npad = (n + 31) / 32 * 32;
+  if (A.bf) {
+    A.Twh = P.dalloc<__bf16>((size_t)A.npad * F);
+    A.Twl = P.dalloc<__bf16>((size_t)A.npad * F);
+  } else {
+    A.Tw32 = P.dalloc<float>((size_t)A.npad * F);
+  }
+  A.nblk = (int)((n + CV_NT - 1) / CV_NT);
+  A.tpart = P.dalloc<double>((size_t)F * A.nblk * (m + 2));
   A.q = P.dalloc<double>((size_t)F * m);
   A.stv = P.dalloc<double>(F);
   A.tnv = P.dalloc<double>(F);
@@ -791,7 +843,14 @@ static std::shared_ptr<CvPlan> cv_build_plan(Ctx* c, const DMat& X, const DMat* 
   fill_f32(c, A.RR32, 0.f, pl->Kp * 128);
   fill_f64(c, P.g, 0.0, p);
   fill_f32(c, P.g32, 0.f, p);
-  fill_f32(c, A.Tw32, 0.f, n * F);
+  // the TN operand's padding rows stay zero (bf16 zero is the all-zero pattern;
+  // npad F is even)
+  if (A.bf) {
+    fill_i32(c, reinterpret_cast<int32_t*>(A.Twh), 0, A.npad * F / 2);
+    fill_i32(c, reinterpret_cast<int32_t*>(A.Twl), 0, A.npad * F / 2);
+  } else {
+    fill_f32(c, A.Tw32, 0.f, A.npad * F);
+  }
   {
     PoolSwap sw(c, pl.get());
     cv_autotune(c, P);

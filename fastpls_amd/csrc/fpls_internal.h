@@ -230,6 +230,11 @@ void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, con
 void proj_splitk(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float* A,
                  int64_t lda, const float* shA32, const float* Bpad, int64_t ldb, int64_t kchunk,
                  int splits, float* part);
+// Skinny TN (N <= 128) with a dense zero-padded B: FP32 (Bf) or split into
+// bf16 hi / lo planes (Bh, Bl; [n][k], ld ldb >= K rounded up to 32).
+void tn_skinny(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float* A, int64_t lda,
+               const double* shA, const float* Bf, const __bf16* Bh, const __bf16* Bl,
+               int64_t ldb, int64_t splits, double* C, int64_t ldc);
 // Device Gaussian streams (k_rng.hip): column s of out (len values, ld) is the
 // std::mt19937(seeds[s]) + std::normal_distribution<double>(0, 1) stream.
 void gaussian_dev(Ctx* ctx, const uint32_t* seeds_dev, int n_streams, int64_t len, double* out,

@@ -450,18 +450,23 @@ __global__ __launch_bounds__(256) void k_g2_reduce(int64_t M, int64_t N, int spl
 // ---------------------------------------------------------------- skinny NN
 // T (M x N, N <= 128) = (X - 1 shA^T) B with X streamed straight from HBM into
 // MFMA operand registers (no LDS for the big operand).  Workgroup = 4 waves x
-// 32 rows; each wave holds 2 x ceil(N/16) accumulator blocks.  B (K x N, the
-// zero-padded copy) goes through LDS one 32-deep K chunk at a time (double
-// buffered, one barrier per chunk); the X operand of chunk c+1 is loaded into
-// registers while the MFMAs of chunk c run.
+// 32 rows; each wave holds 2 x NB accumulator blocks (NB 16-column blocks of
+// B staged: 4 for N <= 64, else 8).  B (K x N, the zero-padded copy) goes
+// through LDS one 32-deep K chunk at a time (double buffered, one barrier per
+// chunk); the X operand of chunk c+1 is loaded into registers while the MFMAs
+// of chunk c run.  With split-K (part != NULL) each workgroup writes its
+// slab's FP32 partial.  The kernel is latency bound: loading two consecutive
+// rows per lane as one vector costs registers and, at the lower occupancy,
+// time (measured), so rows go one per lane.
 constexpr int P_BM = 128, P_BK = 32, P_NT = 256;
 constexpr int P_LDB = P_BK + 8;  // bf16 row length of the [n][k] B tiles
-template <bool BF>
+template <bool BF, int NB>
 struct PSmem {
-  static constexpr int value = BF ? (2 * 2 * 128 * P_LDB) / 2 : 2 * P_BK * (128 + 4);
+  static constexpr int value = BF ? (2 * 2 * 16 * NB * P_LDB) / 2 : 2 * P_BK * (16 * NB + 4);
 };
 
-template <bool BF, bool CEN>
+// NB: 16-column blocks of B staged and multiplied (N <= 16 NB)
+template <bool BF, bool CEN, int NB>
 __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
                                                const float* A, int64_t lda,
                                                const float* shA,
@@ -470,7 +475,10 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
                                                float* __restrict__ Cf, double* __restrict__ Cd,
                                                int64_t ldc, int crow, int64_t kchunk,
                                                float* __restrict__ part) {
-  __shared__ __align__(16) float smem[PSmem<BF>::value];
+  __shared__ __align__(16) float smem[PSmem<BF, NB>::value];
+  constexpr int BW = 16 * NB;              // staged B columns
+  constexpr int TPC = P_NT / BW;           // staging threads per column
+  constexpr int KPT = P_BK / TPC;          // k values per staging thread (8 or 16)
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t row0 = (int64_t)blockIdx.x * P_BM + w * 32;
   // K slab of this workgroup (kchunk is a multiple of P_BK; B's zero rows pad
@@ -478,11 +486,11 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
   const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
   const int nb = (N + 15) >> 4;
-  v4f acc[2][8];
+  v4f acc[2][NB];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 8; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
   const float* arow0 = A + std::min<int64_t>(row0 + lane % 16, M - 1);
   const float* arow1 = A + std::min<int64_t>(row0 + 16 + lane % 16, M - 1);
   // k offset (within a chunk) of operand element j of this lane
@@ -499,52 +507,50 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       mx[j] = CEN ? shA[gk] : 0.f;
     }
   };
-  // B chunk staging: thread -> column n = t / 2, 16 consecutive k
-  const int bn = t >> 1, bk = (t & 1) * 16;
-  float4 bq[4];
+  // B chunk staging: thread -> column n = t / TPC, KPT consecutive k
+  const int bn = t / TPC, bk = (t % TPC) * KPT;
+  float4 bq[KPT / 4];
   auto fetchB = [&](int64_t k0) {
     const float* src = B + (k0 + bk) + (int64_t)bn * ldb;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) bq[q] = *reinterpret_cast<const float4*>(src + 4 * q);
+    for (int q = 0; q < KPT / 4; ++q) bq[q] = *reinterpret_cast<const float4*>(src + 4 * q);
   };
   auto storeB = [&](int buf) {
     if (BF) {
       __bf16* base = reinterpret_cast<__bf16*>(smem);
-      __bf16* Bh = base + (0 * 2 + buf) * 128 * P_LDB;
-      __bf16* Bl = base + (1 * 2 + buf) * 128 * P_LDB;
-      v8bf h0, l0, h1, l1;
-      const float v[16] = {bq[0].x, bq[0].y, bq[0].z, bq[0].w, bq[1].x, bq[1].y, bq[1].z, bq[1].w,
-                           bq[2].x, bq[2].y, bq[2].z, bq[2].w, bq[3].x, bq[3].y, bq[3].z, bq[3].w};
+      __bf16* Bh = base + (0 * 2 + buf) * BW * P_LDB;
+      __bf16* Bl = base + (1 * 2 + buf) * BW * P_LDB;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        __bf16 hh, ll;
-        split_bf(v[e], hh, ll);
-        h0[e] = hh;
-        l0[e] = ll;
-        split_bf(v[8 + e], hh, ll);
-        h1[e] = hh;
-        l1[e] = ll;
+      for (int h = 0; h < KPT / 8; ++h) {
+        const float v[8] = {bq[2 * h].x, bq[2 * h].y, bq[2 * h].z, bq[2 * h].w,
+                            bq[2 * h + 1].x, bq[2 * h + 1].y, bq[2 * h + 1].z, bq[2 * h + 1].w};
+        v8bf hv, lv;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          __bf16 hh, ll;
+          split_bf(v[e], hh, ll);
+          hv[e] = hh;
+          lv[e] = ll;
+        }
+        *reinterpret_cast<v8bf*>(Bh + bn * P_LDB + bk + 8 * h) = hv;
+        *reinterpret_cast<v8bf*>(Bl + bn * P_LDB + bk + 8 * h) = lv;
       }
-      *reinterpret_cast<v8bf*>(Bh + bn * P_LDB + bk) = h0;
-      *reinterpret_cast<v8bf*>(Bh + bn * P_LDB + bk + 8) = h1;
-      *reinterpret_cast<v8bf*>(Bl + bn * P_LDB + bk) = l0;
-      *reinterpret_cast<v8bf*>(Bl + bn * P_LDB + bk + 8) = l1;
     } else {
-      float* Bs = smem + buf * P_BK * (128 + 4);
+      float* Bs = smem + buf * P_BK * (BW + 4);
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        Bs[(bk + 4 * q + 0) * (128 + 4) + bn] = bq[q].x;
-        Bs[(bk + 4 * q + 1) * (128 + 4) + bn] = bq[q].y;
-        Bs[(bk + 4 * q + 2) * (128 + 4) + bn] = bq[q].z;
-        Bs[(bk + 4 * q + 3) * (128 + 4) + bn] = bq[q].w;
+      for (int q = 0; q < KPT / 4; ++q) {
+        Bs[(bk + 4 * q + 0) * (BW + 4) + bn] = bq[q].x;
+        Bs[(bk + 4 * q + 1) * (BW + 4) + bn] = bq[q].y;
+        Bs[(bk + 4 * q + 2) * (BW + 4) + bn] = bq[q].z;
+        Bs[(bk + 4 * q + 3) * (BW + 4) + bn] = bq[q].w;
       }
     }
   };
   auto compute = [&](int buf) {
     if (BF) {
       const __bf16* base = reinterpret_cast<const __bf16*>(smem);
-      const __bf16* Bh = base + (0 * 2 + buf) * 128 * P_LDB;
-      const __bf16* Bl = base + (1 * 2 + buf) * 128 * P_LDB;
+      const __bf16* Bh = base + (0 * 2 + buf) * BW * P_LDB;
+      const __bf16* Bl = base + (1 * 2 + buf) * BW * P_LDB;
       v8bf ah0, al0, ah1, al1;
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
@@ -559,7 +565,7 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       }
       const int ko = 8 * (lane / 16);
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < NB; ++b) {
         if (b < nb) {
           const int n = b * 16 + lane % 16;
           const v8bf bh = *reinterpret_cast<const v8bf*>(Bh + n * P_LDB + ko);
@@ -573,16 +579,16 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
         }
       }
     } else {
-      const float* Bs = smem + buf * P_BK * (128 + 4);
+      const float* Bs = smem + buf * P_BK * (BW + 4);
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
         const float ms = mc[s];
         const float x0 = ac0[s] - ms, x1 = ac1[s] - ms;
         const int kr = 4 * s + lane / 16;
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
+        for (int b = 0; b < NB; ++b) {
           if (b < nb) {
-            const float bv = Bs[kr * (128 + 4) + b * 16 + lane % 16];
+            const float bv = Bs[kr * (BW + 4) + b * 16 + lane % 16];
             acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, bv, acc[0][b], 0, 0, 0);
             acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, bv, acc[1][b], 0, 0, 0);
           }
@@ -622,7 +628,7 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
     for (int a = 0; a < 2; ++a) {
       const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
 #pragma unroll
-      for (int b = 0; b < 8; ++b) {
+      for (int b = 0; b < NB; ++b) {
         const int col = b * 16 + lane % 16;
         if (b >= nb || col >= N) continue;
 #pragma unroll
@@ -636,7 +642,7 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   for (int a = 0; a < 2; ++a) {
     const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < NB; ++b) {
       const int col = b * 16 + lane % 16;
       if (b >= nb || col >= N) continue;
       const double o = off ? off[col] : 0.0;
@@ -651,6 +657,15 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       }
     }
   }
+}
+
+// k_proj with the column-block count picked from N.
+template <bool BF, bool CEN, typename... Args>
+static void launch_proj(Ctx* ctx, const char* name, int N, dim3 grid, Args... args) {
+  if (N <= 64)
+    launch(ctx, name, k_proj<BF, CEN, 4>, grid, dim3(P_NT), 0, args...);
+  else
+    launch(ctx, name, k_proj<BF, CEN, 8>, grid, dim3(P_NT), 0, args...);
 }
 
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
@@ -676,17 +691,17 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       DBuf<float> shf(ctx, (size_t)K);
       f64_to_f32(ctx, shf.get(), shiftA, K);
       if (bf)
-        launch(ctx, "project_bf16x3", k_proj<true, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+        launch_proj<true, true>(ctx, "project_bf16x3", n32, pgrid, M, n32, K, A, lda,
                (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
       else
-        launch(ctx, "project_f32", k_proj<false, true>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+        launch_proj<false, true>(ctx, "project_f32", n32, pgrid, M, n32, K, A, lda,
                (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
     } else {
       if (bf)
-        launch(ctx, "gemm_nn_bf16x3", k_proj<true, false>, pgrid, dim3(P_NT), 0, M, n32, K, A,
+        launch_proj<true, false>(ctx, "gemm_nn_bf16x3", n32, pgrid, M, n32, K, A,
                lda, (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
       else
-        launch(ctx, "gemm_nn_f32", k_proj<false, false>, pgrid, dim3(P_NT), 0, M, n32, K, A, lda,
+        launch_proj<false, false>(ctx, "gemm_nn_f32", n32, pgrid, M, n32, K, A, lda,
                (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
     }
     return;
@@ -898,6 +913,239 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
     if (m < M && n < NC)
       P[m + (int64_t)n * M] = (red[0][mi][n] + red[1][mi][n]) + (red[2][mi][n] + red[3][mi][n]);
   }
+}
+
+// ---------------------------------------------------------------- TN, skinny dense B
+// P_z (M x N, N <= 16 NB) = sum_{k in slab z} (A[k, m] - shA[m]) B[k, n] for a
+// dense, zero-padded B (K x N): the batched fits' X^T T.  Workgroup = 4 waves
+// x 32 columns m (128 per workgroup), all on the same 32-deep K step: A goes
+// straight from HBM into operand registers (two float4 per lane and column,
+// next step prefetched), and the step's B tile is loaded once per workgroup
+// into LDS (double buffered, one barrier per step), so the L2 traffic of B
+// is a quarter of one tile per wave.  The B tile is stored [n][k] (k
+// contiguous), from which each lane reads its operand with 16-byte LDS loads.
+// Split-BF16x3 takes B pre-split into bf16 hi / lo planes ([n][k], ld ldb >=
+// K rounded up to 32, zero rows past K), FP32 takes B itself (column-major,
+// same padding).  kchunk is a multiple of 32, so only the last slab runs past
+// K, into B's zero rows.  Each wave owns its columns: no cross-wave sum.
+constexpr int TS_LDK = TL_BK + 8;  // LDS row length (elements) of the [n][k] B tile
+
+template <bool BF, int NB>
+__global__ __launch_bounds__(TL_NT) void k_tns_dense(int64_t M, int N, int64_t K, int64_t kchunk,
+                                                     const float* A, int64_t lda,
+                                                     const double* shA, const float* Bf,
+                                                     const __bf16* Bh, const __bf16* Bl,
+                                                     int64_t ldb, float* partial) {
+  // per buffer: BF -> hi and lo [16 NB][TS_LDK] bf16; FP32 -> [16 NB][TS_LDK] float
+  constexpr int TILE_F = 16 * NB * TS_LDK;  // floats per buffer (FP32) = bf16 pairs (BF)
+  __shared__ __align__(16) float smem[2 * TILE_F];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int64_t m0 = (int64_t)blockIdx.x * (4 * TL_BM) + w * TL_BM;
+  const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
+  const int nb = (N + 15) >> 4;
+  const int64_t mr0 = std::min<int64_t>(m0 + lane % 16, M - 1);
+  const int64_t mr1 = std::min<int64_t>(m0 + 16 + lane % 16, M - 1);
+  const float* col0 = A + mr0 * lda;
+  const float* col1 = A + mr1 * lda;
+  const float s0 = shA ? static_cast<float>(shA[mr0]) : 0.f;
+  const float s1 = shA ? static_cast<float>(shA[mr1]) : 0.f;
+  const bool vec = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
+                   ((kbeg & 3) == 0);
+  v4f acc[2][NB];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+  const int ko = 8 * (lane / 16);
+  float x0[8], x1[8], y0[8], y1[8];
+  auto fetchA = [&](int64_t k0, float (&a0)[8], float (&a1)[8]) {
+    const int64_t kb = k0 + ko;
+    if (vec && kb + 8 <= K) {
+      const float4 p = *reinterpret_cast<const float4*>(col0 + kb);
+      const float4 q = *reinterpret_cast<const float4*>(col0 + kb + 4);
+      const float4 u = *reinterpret_cast<const float4*>(col1 + kb);
+      const float4 v = *reinterpret_cast<const float4*>(col1 + kb + 4);
+      a0[0] = p.x; a0[1] = p.y; a0[2] = p.z; a0[3] = p.w;
+      a0[4] = q.x; a0[5] = q.y; a0[6] = q.z; a0[7] = q.w;
+      a1[0] = u.x; a1[1] = u.y; a1[2] = u.z; a1[3] = u.w;
+      a1[4] = v.x; a1[5] = v.y; a1[6] = v.z; a1[7] = v.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t kc = std::min<int64_t>(kb + j, K - 1);
+        a0[j] = col0[kc];
+        a1[j] = col1[kc];
+      }
+    }
+  };
+  // B staging: thread t -> column bn = t / 4 + 64 pass (NB / 4 passes; clamped
+  // to N - 1: columns >= N only feed outputs that are never stored), 8
+  // consecutive k at 8 (t % 4)
+  constexpr int NPASS = NB / 4;
+  const int bk = 8 * (t & 3);
+  int64_t bsrc[NPASS];
+#pragma unroll
+  for (int q = 0; q < NPASS; ++q)
+    bsrc[q] = (int64_t)std::min((t >> 2) + 64 * q, N - 1) * ldb + bk;
+  float4 bq0[NPASS], bq1[NPASS];  // FP32: 8 floats; BF: hi (bq0) / lo (bq1), 8 bf16 each
+  auto fetchB = [&](int64_t k0) {
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      if constexpr (BF) {
+        bq0[q] = *reinterpret_cast<const float4*>(Bh + bsrc[q] + k0);
+        bq1[q] = *reinterpret_cast<const float4*>(Bl + bsrc[q] + k0);
+      } else {
+        bq0[q] = *reinterpret_cast<const float4*>(Bf + bsrc[q] + k0);
+        bq1[q] = *reinterpret_cast<const float4*>(Bf + bsrc[q] + k0 + 4);
+      }
+    }
+  };
+  auto storeB = [&](int buf) {
+    float* base = smem + buf * TILE_F;
+#pragma unroll
+    for (int q = 0; q < NPASS; ++q) {
+      const int bn = (t >> 2) + 64 * q;
+      if constexpr (BF) {
+        __bf16* hb = reinterpret_cast<__bf16*>(base);
+        __bf16* lb = hb + 16 * NB * TS_LDK;
+        *reinterpret_cast<float4*>(hb + bn * TS_LDK + bk) = bq0[q];
+        *reinterpret_cast<float4*>(lb + bn * TS_LDK + bk) = bq1[q];
+      } else {
+        *reinterpret_cast<float4*>(base + bn * TS_LDK + bk) = bq0[q];
+        *reinterpret_cast<float4*>(base + bn * TS_LDK + bk + 4) = bq1[q];
+      }
+    }
+  };
+  auto compute = [&](int buf, const float (&a0)[8], const float (&a1)[8]) {
+    const float* base = smem + buf * TILE_F;
+    if constexpr (BF) {
+      const __bf16* hb = reinterpret_cast<const __bf16*>(base);
+      const __bf16* lb = hb + 16 * NB * TS_LDK;
+      v8bf h0, l0, h1, l1;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        __bf16 hh, ll;
+        split_bf(a0[j] - s0, hh, ll);
+        h0[j] = hh;
+        l0[j] = ll;
+        split_bf(a1[j] - s1, hh, ll);
+        h1[j] = hh;
+        l1[j] = ll;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (b < nb) {
+          const int o = (b * 16 + lane % 16) * TS_LDK + ko;
+          const v8bf bh = *reinterpret_cast<const v8bf*>(hb + o);
+          const v8bf bl = *reinterpret_cast<const v8bf*>(lb + o);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l0, bh, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0, bl, acc[0][b], 0, 0, 0);
+          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h0, bh, acc[0][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(l1, bh, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1, bl, acc[1][b], 0, 0, 0);
+          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(h1, bh, acc[1][b], 0, 0, 0);
+        }
+      }
+    } else {
+      float c0[8], c1[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        c0[j] = a0[j] - s0;
+        c1[j] = a1[j] - s1;
+      }
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        if (b < nb) {
+          const float* bc = base + (b * 16 + lane % 16) * TS_LDK + ko;
+          const float4 p = *reinterpret_cast<const float4*>(bc);
+          const float4 q = *reinterpret_cast<const float4*>(bc + 4);
+          const float bv[8] = {p.x, p.y, p.z, p.w, q.x, q.y, q.z, q.w};
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(c0[j], bv[j], acc[0][b], 0, 0, 0);
+            acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(c1[j], bv[j], acc[1][b], 0, 0, 0);
+          }
+        }
+      }
+    }
+  };
+  const int nk = (int)((kend - kbeg + TL_BK - 1) / TL_BK);
+  if (nk > 0) {
+    fetchA(kbeg, x0, x1);
+    fetchB(kbeg);
+    storeB(0);
+  }
+  __syncthreads();
+  for (int c = 0; c < nk; ++c) {
+    const bool more = c + 1 < nk;
+    const int64_t kn = kbeg + (int64_t)(c + 1) * TL_BK;
+    if (more) {
+      fetchA(kn, y0, y1);
+      fetchB(kn);
+    }
+    // keep the next step's loads ahead of this step's MFMAs
+    __builtin_amdgcn_sched_barrier(0);
+    compute(c & 1, x0, x1);
+    if (more) {
+      storeB((c + 1) & 1);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        x0[j] = y0[j];
+        x1[j] = y1[j];
+      }
+    }
+    __syncthreads();
+  }
+  // lane holds D[4(l/16) + r][l%16] of each 16 x 16 block: rows are columns m
+  // of A, columns are n
+  float* P = partial + (int64_t)blockIdx.y * M * N;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t mb = m0 + 16 * a + 4 * (lane / 16);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int n = b * 16 + lane % 16;
+      if (b >= nb || n >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (mb + r < M) P[mb + r + (int64_t)n * M] = acc[a][b][r];
+    }
+  }
+}
+
+// C (M x N FP64, ldc) = (A - 1 shA^T)^T B for a skinny dense B (N <= 128)
+// given as FP32 (Bf) or as bf16 hi / lo planes (Bh, Bl), see k_tns_dense.
+void tn_skinny(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float* A, int64_t lda,
+               const double* shA, const float* Bf, const __bf16* Bh, const __bf16* Bl,
+               int64_t ldb, int64_t splits, double* C, int64_t ldc) {
+  const bool bf = precision == FPLS_PREC_BF16X3;
+  FPLS_REQUIRE(N >= 1 && N <= 128 && ldb >= (K + TL_BK - 1) / TL_BK * TL_BK &&
+                   (bf ? (Bh && Bl) : Bf != nullptr),
+               FPLS_ERR_ARG, "tn_skinny: bad operands");
+  splits = std::max<int64_t>(1, std::min<int64_t>(splits, (K + TL_BK - 1) / TL_BK));
+  int64_t kchunk = (K + splits - 1) / splits;
+  kchunk = (kchunk + TL_BK - 1) / TL_BK * TL_BK;
+  splits = (K + kchunk - 1) / kchunk;
+  DBuf<float> part(ctx, (size_t)splits * M * N);
+  dim3 grid((unsigned)((M + 4 * TL_BM - 1) / (4 * TL_BM)), (unsigned)splits);
+  if (N <= 64) {
+    if (bf)
+      launch(ctx, "tn_skinny_bf16x3", k_tns_dense<true, 4>, grid, dim3(TL_NT), 0, M, N, K,
+             kchunk, A, lda, shA, Bf, Bh, Bl, ldb, part.get());
+    else
+      launch(ctx, "tn_skinny_f32", k_tns_dense<false, 4>, grid, dim3(TL_NT), 0, M, N, K, kchunk,
+             A, lda, shA, Bf, Bh, Bl, ldb, part.get());
+  } else {
+    if (bf)
+      launch(ctx, "tn_skinny_bf16x3", k_tns_dense<true, 8>, grid, dim3(TL_NT), 0, M, N, K,
+             kchunk, A, lda, shA, Bf, Bh, Bl, ldb, part.get());
+    else
+      launch(ctx, "tn_skinny_f32", k_tns_dense<false, 8>, grid, dim3(TL_NT), 0, M, N, K, kchunk,
+             A, lda, shA, Bf, Bh, Bl, ldb, part.get());
+  }
+  launch(ctx, "tn_skinny_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0,
+         M, (int64_t)N, (int)splits, (const float*)part.get(), C, ldc);
 }
 
 // ---------------------------------------------------------------- Gram, M <= 128
@@ -1155,20 +1403,20 @@ void proj_splitk(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const flo
   const bool bf = precision == FPLS_PREC_BF16X3;
   if (shA32) {
     if (bf)
-      launch(ctx, "cv_project_bf16x3", k_proj<true, true>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+      launch_proj<true, true>(ctx, "cv_project_bf16x3", N, grid, M, N, K, A, lda,
              shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
              (int64_t)0, 0, kchunk, part);
     else
-      launch(ctx, "cv_project_f32", k_proj<false, true>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+      launch_proj<false, true>(ctx, "cv_project_f32", N, grid, M, N, K, A, lda,
              shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
              (int64_t)0, 0, kchunk, part);
   } else {
     if (bf)
-      launch(ctx, "cv_project_bf16x3", k_proj<true, false>, grid, dim3(P_NT), 0, M, N, K, A,
+      launch_proj<true, false>(ctx, "cv_project_bf16x3", N, grid, M, N, K, A,
              lda, (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
              (double*)nullptr, (int64_t)0, 0, kchunk, part);
     else
-      launch(ctx, "cv_project_f32", k_proj<false, false>, grid, dim3(P_NT), 0, M, N, K, A, lda,
+      launch_proj<false, false>(ctx, "cv_project_f32", N, grid, M, N, K, A, lda,
              (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
              (double*)nullptr, (int64_t)0, 0, kchunk, part);
   }

@@ -49,3 +49,41 @@ def classes_on_sphere(n_classes=100, per_class_train=500, per_class_test=100, p=
     Xtr, ytr = draw(per_class_train)
     Xte, yte = draw(per_class_test)
     return Xtr, ytr, Xte, yte
+
+
+def _dirichlet_folds(rng, rows, k, alpha=5.0):
+    """Assign `rows` (index array) to k folds with Dirichlet(alpha) sizes,
+    rounded, at least one row each; returns fold ids 1..k per row."""
+    sizes = np.maximum(1, np.round(rng.dirichlet(np.full(k, alpha)) * rows.size).astype(int))
+    while sizes.sum() > rows.size:
+        sizes[np.argmax(sizes)] -= 1
+    sizes[-1] += rows.size - sizes.sum()
+    ids = np.repeat(np.arange(1, k + 1), sizes)
+    rng.shuffle(ids)
+    return ids
+
+
+def nested_cv_lognormal(n=5000, p=20000, block=200, rho=0.6, n_outer=10, n_inner=5,
+                        props=(0.4, 0.25, 0.15, 0.12, 0.08), seed=SEED_BASE + 4):
+    """Double cross-validation workload: X = exp(Z), Z ~ N(0,1) with blocks of
+    `block` columns sharing a factor (within-block correlation rho), FP32
+    column-major; labels 1..5 with the given proportions; outer folds of
+    Dirichlet(5) sizes and, for each outer fold j, inner folds (Dirichlet(5)
+    sizes) of the rows outside fold j.  Returns X, y, outer (n), inner (n x
+    n_outer; entries of rows inside outer fold j are 0)."""
+    rng = np.random.default_rng(seed)
+    nb = p // block
+    Z = rng.standard_normal((n, p), dtype=np.float32)
+    Z *= np.float32(np.sqrt(1.0 - rho))
+    shared = rng.standard_normal((n, nb), dtype=np.float32) * np.float32(np.sqrt(rho))
+    Z += np.repeat(shared, block, axis=1)
+    np.exp(Z, out=Z)
+    X = np.asfortranarray(Z)
+    del Z
+    y = rng.choice(np.arange(1, len(props) + 1), n, p=np.asarray(props)).astype(np.int32)
+    outer = _dirichlet_folds(rng, np.arange(n), n_outer)
+    inner = np.zeros((n, n_outer), dtype=np.int64)
+    for j in range(n_outer):
+        tr = np.nonzero(outer != j + 1)[0]
+        inner[tr, j] = _dirichlet_folds(rng, tr, n_inner)
+    return X, y, outer, inner
