@@ -259,6 +259,20 @@ void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* cou
 // n <= kJacobiMax.  Eigenvalues descending in w, eigenvectors in V (ld n).
 constexpr int kJacobiMax = 128;
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
+// count independent problems (order ns_dev[b] <= nmax, A + b strideA with ld
+// lda), one workgroup each; eigenvalues descending at w + b strideW,
+// eigenvectors at V + b strideV (ld = the problem's order).
+void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
+                     int nmax, int count, double* w, int64_t strideW, double* V,
+                     int64_t strideV);
+// Any order: block Jacobi over pairs of <= 64-row blocks, each pair
+// diagonalised by the one-workgroup solver (k_jacobi.hip).  Same outputs as
+// sym_eig (eigenvalues descending in w, eigenvectors in V, ld n).
+void sym_eig_large(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
+// spd_solve_slices for orders above 128, through the eigendecomposition.
+void spd_solve_large(Ctx* ctx, const double* A, int64_t lda, const std::vector<int>& ks,
+                     int n_slices, int mode, const double* dvals, double ridge, double* X,
+                     int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status);
 // Cholesky-based solves of leading blocks: for each slice s, solve
 // (A[:k_s,:k_s] + diag_shift_s I) X = RHS, with LU fallback.  Used for the
 // PLSSVD latent coefficients and the LDA inverse covariance.

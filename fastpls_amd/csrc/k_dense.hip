@@ -258,9 +258,14 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
 void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
                       int n_slices, int mode, const double* dvals, double ridge, double* X,
                       int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status) {
-  FPLS_REQUIRE(kmax >= 1 && kmax <= kJacobiMax, FPLS_ERR_ARG,
-               "spd_solve_slices: order must be in 1..128");
+  FPLS_REQUIRE(kmax >= 1, FPLS_ERR_ARG, "spd_solve_slices: order must be positive");
   FPLS_REQUIRE((int)ks.size() >= n_slices, FPLS_ERR_ARG, "spd_solve_slices: missing orders");
+  if (kmax > kJacobiMax) {
+    // orders above the one-workgroup sweep go through the eigendecomposition
+    spd_solve_large(ctx, A, lda, ks, n_slices, mode, dvals, ridge, X, ldx, slice_stride,
+                    lambda_out, status);
+    return;
+  }
   // slice orders travel in the kernel arguments, kSliceMax per launch
   for (int c0 = 0; c0 < n_slices; c0 += kSliceMax) {
     const int cnt = std::min(kSliceMax, n_slices - c0);

@@ -67,12 +67,23 @@ __device__ __forceinline__ int bound(const double* x, int lo, int hi, double v, 
 }  // namespace
 
 // Qa, Qb, Vr: n x n FP64 workspace (ld n).
-__global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ Ain, int64_t lda,
-                                                    int n, double* __restrict__ w,
-                                                    double* __restrict__ Vout,
-                                                    double* __restrict__ Qa,
-                                                    double* __restrict__ Qb,
-                                                    double* __restrict__ Vr) {
+// Batched: workgroup b solves problem b (A + b strideA, order ns[b] or n0,
+// results at w + b strideW and V + b strideV with ld = its order, workspace of
+// 3 nmax^2 doubles at ws + b strideWs).
+__global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ Ain0, int64_t lda,
+                                                    int64_t strideA, const int* __restrict__ ns,
+                                                    int n0, int nmax, double* __restrict__ w0,
+                                                    int64_t strideW, double* __restrict__ Vout0,
+                                                    int64_t strideV, double* __restrict__ ws,
+                                                    int64_t strideWs) {
+  const int64_t bi = blockIdx.x;
+  const int n = ns ? ns[bi] : n0;
+  const double* __restrict__ Ain = Ain0 + bi * strideA;
+  double* __restrict__ w = w0 + bi * strideW;
+  double* __restrict__ Vout = Vout0 + bi * strideV;
+  double* __restrict__ Qa = ws + bi * strideWs;
+  double* __restrict__ Qb = Qa + (int64_t)nmax * nmax;
+  double* __restrict__ Vr = Qb + (int64_t)nmax * nmax;
   __shared__ double big[EN * ELD];  // stage 1: A; stage 2: W (ld n); stage 3: reflectors (ld n)
   __shared__ double dv[EN], ev[EN], tauv[EN], pv[EN];
   __shared__ double lam[EN], Ds[EN], zs[EN], taus[EN], zh[EN], invn[EN], val[EN];
@@ -670,10 +681,18 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
   FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
                "sym_eig: order must be in 1..128 for the one-workgroup solver");
   DBuf<double> ws(ctx, (size_t)3 * n * n);
-  double* Qa = ws.get();
-  double* Qb = Qa + (size_t)n * n;
-  double* Vr = Qb + (size_t)n * n;
-  launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, n, w, V, Qa, Qb, Vr);
+  launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
+         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0);
+}
+
+void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
+                     int nmax, int count, double* w, int64_t strideW, double* V,
+                     int64_t strideV) {
+  FPLS_REQUIRE(nmax >= 1 && nmax <= kJacobiMax && count >= 1, FPLS_ERR_ARG,
+               "sym_eig_batched: orders must be in 1..128");
+  DBuf<double> ws(ctx, (size_t)3 * nmax * nmax * count);
+  launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda, strideA,
+         ns_dev, nmax, nmax, w, strideW, V, strideV, ws.get(), (int64_t)3 * nmax * nmax);
 }
 
 }  // namespace fpls

@@ -1,7 +1,8 @@
 // Truncated SVD on the device, FP64:
 //
 //  * svd_exact_dev -- thin SVD through the Gram matrix of the narrow side and
-//                     the one-workgroup Jacobi eigensolver (min dim <= 128);
+//                     the symmetric eigensolver (one workgroup up to order
+//                     128, block Jacobi above);
 //                     stands in for svd_econ (src/svd_cpu_exact.cpp:7-38).
 //  * rsvd_dev      -- randomized range finder of src/svd_cpu_rsvd.cpp:95-124
 //                     with the finalisation of :26-93.  The sketch is the
@@ -43,8 +44,7 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
                    SvdOut& out, bool left_only) {
   const int64_t mn = std::min(rows, cols);
   FPLS_REQUIRE(mn >= 1, FPLS_ERR_RANK, "svd: empty matrix");
-  FPLS_REQUIRE(mn <= kJacobiMax, FPLS_ERR_ARG,
-               "exact SVD on the device is limited to min(rows, cols) <= 128; use the rSVD path");
+  // orders above 128 take the block-Jacobi eigensolver (k_jacobi.hip)
   const int kk = (int)std::min<int64_t>(mn, std::max(k, 1));
   const int g = (int)mn;
   DBuf<double> Gm(ctx, (size_t)g * g), w(ctx, g), E(ctx, (size_t)g * g);
@@ -55,7 +55,7 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
   if (cols <= rows) {
     gemm_f64(ctx, true, false, cols, cols, rows, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);
-    sym_eig(ctx, Gm.get(), g, g, w.get(), E.get());
+    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get());
     launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
            kk);
     // U = A V / s
@@ -66,7 +66,7 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
   } else {
     gemm_f64(ctx, false, true, rows, rows, cols, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);
-    sym_eig(ctx, Gm.get(), g, g, w.get(), E.get());
+    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get());
     launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
            kk);
     copy2d_f64(ctx, out.U.get(), rows, E.get(), g, rows, kk);

@@ -87,3 +87,35 @@ def nested_cv_lognormal(n=5000, p=20000, block=200, rho=0.6, n_outer=10, n_inner
         tr = np.nonzero(outer != j + 1)[0]
         inner[tr, j] = _dirichlet_folds(rng, tr, n_inner)
     return X, y, outer, inner
+
+
+def zipf_classes_large(n=1_200_000, n_test=50_000, p=1024, n_classes=1000, zipf=1.1,
+                       radius=4.0, noise=0.6, seed=SEED_BASE + 3, chunk=100_000):
+    """Large-class-count PLS-DA workload: class sizes proportional to
+    Zipf(zipf) (every class present in the training set), class means
+    uniform on the radius-`radius` sphere in R^p, rows = mean + N(0, noise^2 I),
+    FP32 column-major.  Test rows drawn the same way."""
+    rng = np.random.default_rng(seed)
+    w = 1.0 / np.arange(1, n_classes + 1) ** zipf
+    w /= w.sum()
+    mu = rng.standard_normal((n_classes, p))
+    mu *= radius / np.linalg.norm(mu, axis=1, keepdims=True)
+    mu = mu.astype(np.float32)
+
+    def draw(rows, cover):
+        y = rng.choice(np.arange(1, n_classes + 1), rows, p=w).astype(np.int32)
+        if cover:
+            y[:n_classes] = np.arange(1, n_classes + 1)
+            rng.shuffle(y)
+        X = np.empty((rows, p), dtype=np.float32, order="F")
+        for a in range(0, rows, chunk):
+            b = min(rows, a + chunk)
+            blk = rng.standard_normal((b - a, p), dtype=np.float32)
+            blk *= np.float32(noise)
+            blk += mu[y[a:b] - 1]
+            X[a:b] = blk
+        return X, y
+
+    Xtr, ytr = draw(n, True)
+    Xte, yte = draw(n_test, False)
+    return Xtr, ytr, Xte, yte

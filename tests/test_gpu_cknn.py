@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from oracle import fastpls_oracle as O
+from tests.helpers import sin_theta
 
 pytestmark = pytest.mark.gpu
 
@@ -82,3 +83,32 @@ def test_pls_cknn_pipeline(ctx):
     rr = O.candidate_knn_predictions(ref, Xte.astype(np.float64), top=5)[0]
     _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-4)
     assert np.mean(res["Ypred_index"][:, 0] == yte) > 0.9
+
+
+def test_label_aware_cknn_many_classes_large_ncomp(ctx):
+    """The large-class-count label-aware PLSSVD + candidate-kNN path at reduced
+    size: Zipf class sizes, ncomp above 128 (exact SVD and the latent solve
+    take the block-Jacobi eigensolver).  Against the oracle's label-aware
+    model and candidate-kNN predictions."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(41)
+    C, p, n, nt = 180, 256, 9000, 400
+    w = 1.0 / np.arange(1, C + 1) ** 1.1
+    w /= w.sum()
+    mu = rng.standard_normal((C, p))
+    mu *= 4.0 / np.linalg.norm(mu, axis=1, keepdims=True)
+    ytr = np.concatenate([np.arange(1, C + 1), rng.choice(np.arange(1, C + 1), n - C, p=w)])
+    yte = rng.choice(np.arange(1, C + 1), nt, p=w)
+    Xtr = (mu[ytr - 1] + 0.6 * rng.standard_normal((n, p))).astype(np.float32)
+    Xte = (mu[yte - 1] + 0.6 * rng.standard_normal((nt, p))).astype(np.float32)
+    ytr = ytr.astype(np.int32)
+    ncomp = 150
+    m = F.pls(Xtr, ytr, method="plssvd", ncomp=ncomp, classifier="cknn", label_aware=True,
+              candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20)
+    res = F.predict(m, Xte, top=5)
+    ref = O.plssvd_label_aware(Xtr.astype(np.float64), ytr, [ncomp])
+    assert sin_theta(m["R"][:, :ncomp], ref["R"][:, :ncomp]) < 1e-4
+    ref["lev"] = list(range(1, C + 1))
+    O.fit_candidate_knn(ref, Xtr.astype(np.float64), ytr, 10, 0.2, 0.75, 20)
+    rr = O.candidate_knn_predictions(ref, Xte.astype(np.float64), top=5)[0]
+    _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-3)

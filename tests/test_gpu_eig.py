@@ -74,3 +74,42 @@ def test_eig_wide_and_tall(ctx):
     for rows, cols in [(50, 120), (128, 128), (700, 96)]:
         A = rng.standard_normal((rows, cols)) @ np.diag(np.linspace(3, 1, cols))
         _check(A, min(rows, cols), [(0, 4)])
+
+
+# ---- orders above 128: block Jacobi over pairs of <= 64-row blocks, each pair
+# diagonalised by the one-workgroup solver
+
+@pytest.mark.parametrize("n", [129, 200, 333])
+def test_eig_large_random_orders(ctx, n):
+    rng = np.random.default_rng(300 + n)
+    A = rng.standard_normal((2 * n, n))
+    _check(A, n, [(0, 5), (n // 2, n // 2 + 3)])
+
+
+def test_eig_large_repeated_and_rank_deficient(ctx):
+    rng = np.random.default_rng(9)
+    s = np.concatenate([np.repeat([8.0, 5.0, 3.0, 2.0], 40), np.linspace(1.5, 0.5, 60),
+                        np.zeros(40)])
+    A = _with_spectrum(rng, 400, s)
+    import fastpls_amd as F
+    got = F.fastsvd(A, 220, method="exact")
+    U0, s0, _ = np.linalg.svd(A, full_matrices=False)
+    np.testing.assert_allclose(got["d"], s0[:220], rtol=1e-10, atol=1e-12 * s0[0])
+    for lo, hi in [(0, 40), (40, 80), (80, 120), (120, 160)]:
+        assert sin_theta(got["u"][:, lo:hi], U0[:, lo:hi]) < 1e-8, (lo, hi)
+    V = got["v"]
+    assert np.abs(V.T @ V - np.eye(220)).max() < 1e-11
+
+
+def test_eig_large_wide_cross_covariance(ctx):
+    """The label-aware shape at reduced size: a wide p x C class-sum matrix
+    whose columns sum to zero (rank C - 1)."""
+    rng = np.random.default_rng(10)
+    p, C = 300, 260
+    S = rng.standard_normal((p, C))
+    S -= S.mean(axis=1, keepdims=True)
+    import fastpls_amd as F
+    got = F.fastsvd(S, 150, method="exact")
+    U0, s0, _ = np.linalg.svd(S, full_matrices=False)
+    np.testing.assert_allclose(got["d"], s0[:150], rtol=1e-10)
+    assert sin_theta(got["u"][:, :150], U0[:, :150]) < 1e-8
