@@ -102,8 +102,13 @@ SIGNATURES = {
                                                 ctypes.c_double, ctypes.c_double, _vp, _vp]),
     "fpls_rsvd": (ctypes.c_int, [_vp, _P(Mat), ctypes.c_int32, _P(Opts), _P(Mat), _vp, _vp, _vp,
                                  _P(ctypes.c_int32)]),
-    "fpls_rsvd_xprod": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), ctypes.c_int32, _P(Opts), _vp, _vp,
-                                       _vp, _P(ctypes.c_int32)]),
+    "fpls_rsvd_xprod": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), ctypes.c_int32, _P(Opts), _P(Mat),
+                                       _vp, _vp, _vp, _P(ctypes.c_int32)]),
+    "fpls_sketch_train": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _vp, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _P(Opts),
+                                         _vp, _vp]),
+    "fpls_sketch_eval": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _vp, ctypes.c_int32, ctypes.c_int32,
+                                        _vp, ctypes.c_int32, ctypes.c_int32, _P(Opts), _vp]),
     "fpls_class_sums": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp, _vp]),
     "fpls_cv_predict": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp,
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Opts),

@@ -606,9 +606,11 @@ def pls_single_cv(Xdata, Ydata, constrain=None, ncomp=2, scaling="centering", me
 
 
 def truncated_rsvd_crossprod(X, Y, k: int, rsvd_oversample: int = 10, rsvd_power: int = 1,
-                             seed: int = 1, exact: bool = False, device: int = 0) -> Dict:
+                             seed: int = 1, exact: bool = False, omega=None,
+                             fetch: bool = True, device: int = 0) -> Dict:
     """truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623): rSVD of X^T Y
-    without forming it."""
+    without forming it.  ``omega`` (cols(Y) x (k + oversample)) replaces the
+    Gaussian sketch (a learned sketch from ``sketch_train``)."""
     ctx = _lib.context(device)
     keep: list = []
     Xm = as_mat(X, keep)
@@ -618,16 +620,70 @@ def truncated_rsvd_crossprod(X, Y, k: int, rsvd_oversample: int = 10, rsvd_power
     opts = _lib.default_opts(oversample=int(rsvd_oversample), power=int(rsvd_power),
                              seed=int(seed) & 0xFFFFFFFF,
                              svd_method=_lib.SVD_EXACT if exact else _lib.SVD_RSVD)
-    U = np.empty(p * kk)
+    Om = as_mat(np.asarray(omega, dtype=np.float64), keep) if omega is not None else None
+    U = np.empty(p * kk) if fetch else None
     s = np.empty(kk)
-    Vt = np.empty(kk * m)
+    Vt = np.empty(kk * m) if fetch else None
     kout = ctypes.c_int32()
     ctx.check(ctx.lib.fpls_rsvd_xprod(ctx.h, ctypes.byref(Xm), ctypes.byref(Ym), kk,
-                                      ctypes.byref(opts), U.ctypes.data, s.ctypes.data,
-                                      Vt.ctypes.data, ctypes.byref(kout)))
+                                      ctypes.byref(opts),
+                                      ctypes.byref(Om) if Om is not None else None,
+                                      U.ctypes.data if U is not None else None, s.ctypes.data,
+                                      Vt.ctypes.data if Vt is not None else None,
+                                      ctypes.byref(kout)))
+    if not fetch:
+        return {"s": s[: kout.value]}
     r = kout.value
     return {"U": U[: p * r].reshape((p, r), order="F"), "s": s[:r],
             "Vt": Vt[: r * m].reshape((r, m), order="F")}
+
+
+def sketch_train(X, Y, masks, k: int, l: int, iters: int = 200, lr: float = 0.02, seed: int = 1,
+                 precision: str = "fp32", device: int = 0) -> Dict:
+    """Learn a sketch Omega (cols(Y) x l) for the rSVD range finder of Xc^T Yc
+    from the cross-covariances of the row-fold draws in ``masks`` (n x D,
+    nonzero = row in the draw); ``fpls_sketch_train``.  Returns ``omega`` and
+    the per-step mean loss (squared Frobenius sine of the top-k subspace)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    Ym = as_mat(Y, keep)
+    M = np.asfortranarray(np.asarray(masks, dtype=np.int8))
+    keep.append(M)
+    m = Ym.cols
+    opts = _lib.default_opts(seed=int(seed) & 0xFFFFFFFF,
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32)
+    om = np.zeros((m, l), order="F")
+    trace = np.zeros(int(iters) + 1)
+    ctx.check(ctx.lib.fpls_sketch_train(ctx.h, ctypes.byref(Xm), ctypes.byref(Ym), M.ctypes.data,
+                                        M.shape[1], int(k), int(l), int(iters), float(lr),
+                                        ctypes.byref(opts), om.ctypes.data, trace.ctypes.data))
+    return {"omega": om, "loss": trace}
+
+
+def sketch_eval(X, Y, masks, k: int, omega=None, l: int = None, power: int = 0, seed: int = 1,
+                precision: str = "fp32", device: int = 0) -> np.ndarray:
+    """sin(theta) per draw between the range finder's subspace of S_d (``omega``
+    or the Gaussian stream of ``seed``, ``power`` iterations) and the exact
+    top-k left singular subspace of S_d; ``fpls_sketch_eval``."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    Ym = as_mat(Y, keep)
+    M = np.asfortranarray(np.asarray(masks, dtype=np.int8))
+    keep.append(M)
+    om = None
+    if omega is not None:
+        om = np.asfortranarray(np.asarray(omega, dtype=np.float64))
+        keep.append(om)
+        l = om.shape[1]
+    opts = _lib.default_opts(seed=int(seed) & 0xFFFFFFFF,
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32)
+    out = np.zeros(M.shape[1])
+    ctx.check(ctx.lib.fpls_sketch_eval(ctx.h, ctypes.byref(Xm), ctypes.byref(Ym), M.ctypes.data,
+                                       M.shape[1], int(k), om.ctypes.data if om is not None else None,
+                                       int(l), int(power), ctypes.byref(opts), out.ctypes.data))
+    return out
 
 
 def class_sums(X, y, n_classes: int, device: int = 0) -> Dict:

@@ -256,8 +256,8 @@ static void cross(Ctx* ctx, const DMat& X, const DMat& Y, double* S) {
 // truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623): rSVD of
 // S = X^T Y through a_times(M) = X^T (Y M) and at_times(M) = Y^T (X M).
 extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat* Ym, int32_t k,
-                               const fpls_opts* opts, double* U, double* s, double* Vt,
-                               int32_t* k_out) {
+                               const fpls_opts* opts, const fpls_mat* omega, double* U, double* s,
+                               double* Vt, int32_t* k_out) {
   return fpls_guard(ctx, [&] {
     Ctx* c = &ctx->c;
     fpls_opts o;
@@ -293,7 +293,16 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       Yv.cols = m;
       Yv.ld = n;
       std::vector<double> om((size_t)m * l);
-      gaussian_host(om.data(), (int64_t)om.size(), o.seed);
+      if (omega) {  // a caller sketch (the learned-sketch extension), m x l host FP64
+        FPLS_REQUIRE(omega->loc == FPLS_HOST && omega->dtype == FPLS_F64 && omega->rows == m &&
+                         omega->cols == l,
+                     FPLS_ERR_ARG, "rsvd_xprod: omega must be a host FP64 cols(Y) x (k + oversample)");
+        const double* src = static_cast<const double*>(omega->ptr);
+        for (int64_t j = 0; j < l; ++j)
+          for (int64_t i = 0; i < m; ++i) om[i + j * m] = src[i + j * omega->ld];
+      } else {
+        gaussian_host(om.data(), (int64_t)om.size(), o.seed);
+      }
       DBuf<double> Om(c, om.size()), Ys(c, (size_t)p * l), Z(c, (size_t)m * l), Nn(c, (size_t)n * l);
       to_device(c, Om.get(), om.data(), om.size());
       auto a_times = [&](const double* M, double* out) {  // X^T (Y M)

@@ -38,6 +38,8 @@
  *   fpls_cv_batched           the fold loops of pls_cv_predict_compiled src/fastPLS.cpp:6777-7142
  *                             and pls.double.cv R/main.R:6676-6764
  *   fpls_gaussian_streams     gaussian_matrix_local      src/fastPLS.cpp:99-115
+ *   fpls_sketch_train / fpls_sketch_eval  the learned-sketch extension of the
+ *                             Gaussian range finder (no reference counterpart)
  */
 #ifndef FASTPLS_HIP_H
 #define FASTPLS_HIP_H
@@ -225,9 +227,36 @@ int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_norm,
    (the learned-sketch extension); NULL reproduces the reference's stream. */
 int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* A, int32_t k, const fpls_opts* opts,
               const fpls_mat* omega, double* U, double* s, double* Vt, int32_t* k_out);
-/* Matrix-free rSVD of S = X^T Y without forming S. */
+/* Matrix-free rSVD of S = X^T Y without forming S.  omega (cols(Y) x l,
+   host FP64) replaces the Gaussian sketch when given (a learned sketch from
+   fpls_sketch_train); NULL reproduces the reference's stream. */
 int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, int32_t k,
-                    const fpls_opts* opts, double* U, double* s, double* Vt, int32_t* k_out);
+                    const fpls_opts* opts, const fpls_mat* omega, double* U, double* s,
+                    double* Vt, int32_t* k_out);
+
+/* ---- learned sketch ------------------------------------------------------------ */
+/* Trains a dense sketch Omega (cols(Y) x l) for the rSVD range finder of
+   S = Xc^T Yc on the cross-covariances of row-fold draws of (X, Y): masks is
+   n x n_draws (host int8, nonzero = row in the draw), each draw centred on its
+   own rows.  Loss: the squared Frobenius sine between range(S_d Omega) and the
+   exact top-k left singular subspace of S_d, averaged over draws (exact, from
+   the eigenpairs of S_d^T S_d); iters Adam steps at learning rate lr from the
+   orthonormalised Gaussian stream of opts->seed, Omega re-orthonormalised
+   after every step.  omega (cols(Y) x l, host) receives the result,
+   loss_trace (iters + 1, optional) the loss before each step and at the end.
+   X FP32; 1 <= k <= l <= cols(Y) <= 128.  No counterpart in the reference: the
+   learned-sketch extension of the Gaussian range finder
+   (src/svd_cpu_rsvd.cpp:95-124, src/fastPLS.cpp:550-623). */
+int fpls_sketch_train(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const int8_t* masks,
+                      int32_t n_draws, int32_t k, int32_t l, int32_t iters, double lr,
+                      const fpls_opts* opts, double* omega, double* loss_trace);
+/* Per draw of masks: sin(theta) (spectral) between the range finder's
+   l-dimensional subspace of S_d (omega, or the Gaussian stream of opts->seed
+   when NULL; power iterations with the reference's schedule) and S_d's exact
+   top-k left singular subspace. */
+int fpls_sketch_eval(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const int8_t* masks,
+                     int32_t n_draws, int32_t k, const double* omega, int32_t l, int32_t power,
+                     const fpls_opts* opts, double* sin_theta);
 
 /* ---- cross-validation ----------------------------------------------------------- */
 /* pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out

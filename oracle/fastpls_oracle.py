@@ -246,8 +246,9 @@ def finalize_rsvd_from_q_b(Q: np.ndarray, B: np.ndarray, k: int, left_only: bool
 
 def truncated_rsvd_crossprod(X: np.ndarray, Y: np.ndarray, k: int, oversample: int = 10,
                              power: int = 1, seed: int = 1, left_only: bool = False,
-                             use_full_svd: bool = False):
-    """Matrix-free rSVD of S = X^T Y (src/fastPLS.cpp:550-623)."""
+                             use_full_svd: bool = False, omega: np.ndarray = None):
+    """Matrix-free rSVD of S = X^T Y (src/fastPLS.cpp:550-623).  ``omega``
+    (m x l) replaces the Gaussian sketch of ``seed`` (a learned sketch)."""
     p, m = X.shape[1], Y.shape[1]
     max_rank = min(p, m)
     target = min(max_rank, max(k, 1))
@@ -261,7 +262,7 @@ def truncated_rsvd_crossprod(X: np.ndarray, Y: np.ndarray, k: int, oversample: i
     def at_times(M):
         return Y.T @ (X @ M)
 
-    Omega = gaussian_matrix(m, l, seed)
+    Omega = gaussian_matrix(m, l, seed) if omega is None else np.asarray(omega, np.float64)
     Ys = a_times(Omega)
     q = max(power, 0)
     if q == 1:
