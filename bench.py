@@ -346,15 +346,212 @@ def main_cv(args):
     print(json.dumps(res))
 
 
+# rSVD of S = X^T Y with a learned vs a Gaussian sketch: X 20,000 x 50,000
+# FP32 (rank-80 part with s_i = 50 * 0.93^i on random orthonormal factors,
+# plus N(0, 1)), Y = Y0c G / 10 (20,000 x 100) from Y0 = X B + N(0, 1)
+# (2,000 columns, B ~ N(0, 1) / sqrt(p)), k = 50, oversampling 10.
+C5_N, C5_P, C5_RANK, C5_M0, C5_M, C5_K, C5_OS = 20_000, 50_000, 80, 2_000, 100, 50, 10
+C5_TRAIN, C5_EVAL, C5_FRAC, C5_ITERS, C5_LR = 200, 50, 0.8, 300, 0.02
+
+
+def c5_data(seed):
+    """The workload's X (column-major FP32 on the device) and Y, generated on
+    the GPU from one seeded torch generator (orthonormal factors from numpy)."""
+    import torch
+    dev = "cuda:0"
+    rng = np.random.default_rng(seed)
+    U = np.linalg.qr(rng.standard_normal((C5_N, C5_RANK)))[0]
+    V = np.linalg.qr(rng.standard_normal((C5_P, C5_RANK)))[0]
+    s = 50.0 * 0.93 ** np.arange(C5_RANK)
+    Us = torch.from_numpy(U * s).to(dev)
+    Vd = torch.from_numpy(V).to(dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    Xt = torch.randn(C5_P, C5_N, generator=g, device=dev, dtype=torch.float32)
+    for j0 in range(0, C5_P, 5000):  # rows of X^T = columns of X
+        Xt[j0:j0 + 5000] += (Vd[j0:j0 + 5000] @ Us.T).float()
+    X = Xt.t()  # n x p, column-major
+    B = torch.randn(C5_P, C5_M0, generator=g, device=dev, dtype=torch.float32) / C5_P ** 0.5
+    Y0 = X @ B + torch.randn(C5_N, C5_M0, generator=g, device=dev, dtype=torch.float32)
+    Y0 -= Y0.mean(0)
+    G = torch.randn(C5_M0, C5_M, generator=g, device=dev, dtype=torch.float32)
+    Y = ((Y0 @ G) / 10.0).t().contiguous().t()
+    del B, Y0, G, Us, Vd
+    torch.cuda.synchronize()
+    return X, Y
+
+
+def main_sketch(args):
+    """Matrix-free rSVD with Gaussian sketches (q = 0..3: the reference
+    algorithm, q = 1 its default) against the learned sketch and against S
+    formed once; sin(theta) of every form against the FP64 SVD of S, the
+    learned sketch trained on fold draws and judged on fresh draws."""
+    import torch
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    X, Y = c5_data(synth.SEED_BASE + 5)
+    n, p, m, k = C5_N, C5_P, C5_M, C5_K
+    l = k + C5_OS
+    # FP64 judge: S = X^T Y in FP64 and its top-k left singular subspace
+    X64 = X.double()
+    S64 = X64.t() @ Y.double()
+    del X64
+    S64 = S64.cpu().numpy()
+    lam, E = np.linalg.eigh(S64.T @ S64)
+    order = np.argsort(-lam)
+    sig = np.sqrt(np.maximum(lam[order], 0.0))
+    Uex = np.linalg.qr((S64 @ E[:, order[:k]]) / sig[:k])[0]
+    del S64
+
+    def sin_dev(U):
+        Q = np.linalg.qr(U)[0]
+        R = Q - Uex @ (Uex.T @ Q)
+        return float(np.sqrt(max(np.linalg.eigvalsh(R.T @ R).max(), 0.0)))
+
+    rng = np.random.default_rng(synth.SEED_BASE + 6)
+    train = (rng.random((n, C5_TRAIN)) < C5_FRAC).astype(np.int8)
+    fresh = (rng.random((n, C5_EVAL)) < C5_FRAC).astype(np.int8)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tr = F.sketch_train(X, Y, train, k, l, iters=C5_ITERS, lr=C5_LR, seed=1)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    om = tr["omega"]
+    draws = {}
+    for q in range(4):
+        st = F.sketch_eval(X, Y, fresh, k, l=l, power=q, seed=1)
+        draws[f"gaussian_q{q}"] = {"median": float(np.median(st)), "max": float(st.max())}
+    for q in range(2):
+        st = F.sketch_eval(X, Y, fresh, k, omega=om, power=q)
+        draws[f"learned_q{q}"] = {"median": float(np.median(st)), "max": float(st.max())}
+
+    ctx = F.context(0)
+    forms = {}
+    kern_names = ("tn_skinny_f32", "tn_skinny_bf16x3", "tn_skinny_reduce", "gemm_nn_f32",
+                  "gemm_nn_bf16x3", "gemm_nn_reduce", "pad_b", "gemm_tn_f32", "gemm_tn_bf16x3",
+                  "gemm_tn_reduce", "gemm_f64", "gemm_f64_reduce", "sym_eig_dc", "convert",
+                  "copy", "fill", "chol_inv", "scale_cols", "scale_rows", "transpose", "symmatu",
+                  "div", "norm2", "jacobi_cols")
+
+    def run_form(name, **kw):
+        def step(fetch=False):
+            return F.truncated_rsvd_crossprod(X, Y, k, rsvd_oversample=C5_OS, seed=1, fetch=fetch,
+                                              **kw)
+        res = step(fetch=True)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        ctx.timing(True)
+        step()
+        kern = {}
+        for nm in kern_names:
+            c_, ms = ctx.kernel_time(nm)
+            if c_:
+                kern[nm] = {"launches_per_step": c_, "ms_per_step": ms}
+        ctx.timing(False)
+        forms[name] = {"ms": 1e3 * dt, "fits_per_s": 1.0 / dt, "sin_theta": sin_dev(res["U"]),
+                       "kernels": kern}
+
+    for q in range(4):
+        run_form(f"matrix_free_gaussian_q{q}_fp32", rsvd_power=q)
+    run_form("matrix_free_learned_q0_fp32", rsvd_power=0, omega=om)
+    run_form("matrix_free_learned_q1_fp32", rsvd_power=1, omega=om)
+    run_form("formed_S_gaussian_q1_fp32", rsvd_power=1, route="materialise")
+    run_form("formed_S_learned_q0_fp32", rsvd_power=0, omega=om, route="materialise")
+    run_form("formed_S_exact_fp32", exact=True)
+    run_form("formed_S_exact_bf16x3", exact=True, precision="bf16x3")
+
+    # SIMPLS ncomp = 50 fit of the same data
+    def simpls():
+        return F.pls(X, Y, ncomp=k, method="simpls", fetch=False)
+    simpls()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        simpls()
+    torch.cuda.synchronize()
+    simpls_ms = 1e3 * (time.perf_counter() - t0) / 2
+
+    plain = forms["matrix_free_gaussian_q1_fp32"]
+    ok = [f for f in forms if forms[f]["sin_theta"] <= SIN_THETA_BOUND]
+    chosen = max(ok, key=lambda f: forms[f]["fits_per_s"]) if ok else \
+        min(forms, key=lambda f: forms[f]["sin_theta"])
+    r = forms[chosen]
+    kname = "tn_skinny_bf16x3" if "bf16x3" in chosen else "tn_skinny_f32"
+    roof = None
+    if kname in r["kernels"]:
+        kt = r["kernels"][kname]
+        launches = kt["launches_per_step"]
+        avg_s = kt["ms_per_step"] * 1e-3 / launches
+        # the S pass: X read once, the FP32 Y read, the split-K FP32 partials of S
+        bytes_per = 4.0 * n * p + 4.0 * n * m
+        achieved = bytes_per / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": measured_traffic(kname + "_c5", launches),
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": launches,
+                "algorithmic_bytes_per_launch": bytes_per,
+                "tflops": 2.0 * n * p * m / avg_s / 1e12}
+    out = {
+        "metric": METRIC, "value": r["fits_per_s"], "unit": "rSVD/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3" if "bf16x3" in chosen else "f32",
+        "data": "synthetic (seeded, generated on the GPU; rank-80 signal s_i = 50*0.93^i "
+                "plus N(0,1); Y = Y0c G / 10 from Y0 = X B + N(0,1))",
+        "config": {"workload": f"rsvd of X^T Y, X {n}x{p} FP32, Y {n}x{m}, k={k}, l={l}",
+                   "chosen_form": chosen, "inputs": "resident in HBM",
+                   "sin_theta_bound": SIN_THETA_BOUND},
+        "plain_form": "matrix_free_gaussian_q1_fp32",
+        "plain_fits_per_s": plain["fits_per_s"], "plain_sin_theta": plain["sin_theta"],
+        "chosen_vs_plain": r["fits_per_s"] / plain["fits_per_s"],
+        "forms": forms,
+        "sketch": {"train_draws": C5_TRAIN, "eval_draws": C5_EVAL, "row_fraction": C5_FRAC,
+                   "iters": C5_ITERS, "lr": C5_LR, "train_seconds": train_s,
+                   "loss_first": float(tr["loss"][0]), "loss_last": float(tr["loss"][-1]),
+                   "fresh_draw_sin_theta": draws},
+        "spectrum": {"s1": float(sig[0]), "s50": float(sig[k - 1]), "s51": float(sig[k]),
+                     "s61": float(sig[l]), "s100": float(sig[-1])},
+        "simpls_ncomp50_ms": simpls_ms,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = c5_cpu_baseline(X, Y)
+    print(json.dumps(out))
+
+
+def c5_cpu_baseline(X, Y):
+    """The oracle's matrix-free rSVD (FP64, q = 1: the reference default) of
+    the same S = X^T Y on the host cores: one rSVD is the sample."""
+    from threadpoolctl import threadpool_limits
+    from oracle import fastpls_oracle as O
+    cores = min(16, os.cpu_count() or 1)
+    Xh = X.cpu().numpy().astype(np.float64)
+    Yh = Y.cpu().numpy().astype(np.float64)
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        O.truncated_rsvd_crossprod(Xh, Yh, C5_K, C5_OS, 1, 1)
+        dt = time.perf_counter() - t0
+    return {"value": 1.0 / dt, "unit": "rSVD/s", "cores": cores, "kind": "port",
+            "sample": "one matrix-free rSVD (q=1, FP64 numpy oracle) of the full workload",
+            "seconds": dt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plssvd_lda", "cv"], default="plssvd_lda",
+    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch"], default="plssvd_lda",
                     help="plssvd_lda: PLSSVD + LDA fit/predict (default); cv: batched nested "
-                         "cross-validation of SIMPLS")
+                         "cross-validation of SIMPLS; sketch: rSVD of X^T Y with learned vs "
+                         "Gaussian sketches")
     ap.add_argument("--forms", choices=["all", "batched"], default="all",
                     help="cv workload: 'batched' skips the fold-by-fold plain path (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -367,6 +564,9 @@ def main():
         calibration_copy(args.copy_mib)
     if args.workload == "cv":
         main_cv(args)
+        return
+    if args.workload == "sketch":
+        main_sketch(args)
         return
     from fastpls_amd import synth
     data = synth.classes_on_sphere(n_classes=C, per_class_train=N_TRAIN // C,

@@ -56,7 +56,7 @@ class Opts(ctypes.Structure):
                 ("seed", ctypes.c_uint32), ("inc_iters", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("fit", ctypes.c_int32),
                 ("store_b", ctypes.c_int32), ("label_aware", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("xprod_route", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class ModelInfo(ctypes.Structure):

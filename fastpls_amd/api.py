@@ -607,10 +607,16 @@ def pls_single_cv(Xdata, Ydata, constrain=None, ncomp=2, scaling="centering", me
 
 def truncated_rsvd_crossprod(X, Y, k: int, rsvd_oversample: int = 10, rsvd_power: int = 1,
                              seed: int = 1, exact: bool = False, omega=None,
-                             fetch: bool = True, device: int = 0) -> Dict:
+                             fetch: bool = True, precision: str = "fp32",
+                             route: str = "matrix_free", device: int = 0) -> Dict:
     """truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623): rSVD of X^T Y
     without forming it.  ``omega`` (cols(Y) x (k + oversample)) replaces the
-    Gaussian sketch (a learned sketch from ``sketch_train``)."""
+    Gaussian sketch (a learned sketch from ``sketch_train``).  ``route =
+    "materialise"`` forms S = X^T Y once (one pass over X) and runs the same
+    range finder on S; ``precision`` selects the FP32 or split-BF16x3 passes
+    over an FP32 X."""
+    if route not in ("matrix_free", "materialise"):
+        raise ValueError("route must be 'matrix_free' or 'materialise'")
     ctx = _lib.context(device)
     keep: list = []
     Xm = as_mat(X, keep)
@@ -619,7 +625,9 @@ def truncated_rsvd_crossprod(X, Y, k: int, rsvd_oversample: int = 10, rsvd_power
     kk = int(min(max(int(k), 1), min(p, m)))
     opts = _lib.default_opts(oversample=int(rsvd_oversample), power=int(rsvd_power),
                              seed=int(seed) & 0xFFFFFFFF,
-                             svd_method=_lib.SVD_EXACT if exact else _lib.SVD_RSVD)
+                             svd_method=_lib.SVD_EXACT if exact else _lib.SVD_RSVD,
+                             precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32,
+                             xprod_route=1 if route == "materialise" else 0)
     Om = as_mat(np.asarray(omega, dtype=np.float64), keep) if omega is not None else None
     U = np.empty(p * kk) if fetch else None
     s = np.empty(kk)

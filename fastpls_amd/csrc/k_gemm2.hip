@@ -685,8 +685,47 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   // centred contractions (projections of X) are timed as "project_*"
   const bool bf = precision == FPLS_PREC_BF16X3;
   if (N <= 128) {
-    dim3 pgrid((unsigned)((M + P_BM - 1) / P_BM));
+    const int64_t tiles = (M + P_BM - 1) / P_BM;
     const int n32 = (int)N;
+    if (Cd && !offset && !c_rowmajor && tiles < 4 * (int64_t)ctx->num_cus && K >= 512) {
+      // few row tiles: split K into slabs (FP32 partials, summed in FP64)
+      int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
+      splits = std::min<int64_t>(splits, K / 256);
+      splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N)));
+      int64_t kchunk = (K + splits - 1) / splits;
+      kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
+      splits = (K + kchunk - 1) / kchunk;
+      DBuf<float> part(ctx, (size_t)splits * M * N), shf;
+      if (shiftA) {
+        shf = DBuf<float>(ctx, (size_t)K);
+        f64_to_f32(ctx, shf.get(), shiftA, K);
+      }
+      dim3 sgrid((unsigned)tiles, (unsigned)splits);
+      const float* sh32 = shf.get();
+      if (shiftA) {
+        if (bf)
+          launch_proj<true, true>(ctx, "project_bf16x3", n32, sgrid, M, n32, K, A, lda, sh32, B,
+                                  ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+                                  (int64_t)0, 0, kchunk, part.get());
+        else
+          launch_proj<false, true>(ctx, "project_f32", n32, sgrid, M, n32, K, A, lda, sh32, B,
+                                   ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+                                   (int64_t)0, 0, kchunk, part.get());
+      } else {
+        if (bf)
+          launch_proj<true, false>(ctx, "gemm_nn_bf16x3", n32, sgrid, M, n32, K, A, lda, sh32, B,
+                                   ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+                                   (int64_t)0, 0, kchunk, part.get());
+        else
+          launch_proj<false, false>(ctx, "gemm_nn_f32", n32, sgrid, M, n32, K, A, lda, sh32, B,
+                                    ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
+                                    (int64_t)0, 0, kchunk, part.get());
+      }
+      launch(ctx, "gemm_nn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0,
+             M, N, (int)splits, (const float*)part.get(), Cd, ldc);
+      return;
+    }
+    dim3 pgrid((unsigned)tiles);
     if (shiftA) {
       DBuf<float> shf(ctx, (size_t)K);
       f64_to_f32(ctx, shf.get(), shiftA, K);
@@ -931,7 +970,7 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
 constexpr int TS_LDK = TL_BK + 8;  // LDS row length (elements) of the [n][k] B tile
 
 template <bool BF, int NB>
-__global__ __launch_bounds__(TL_NT) void k_tns_dense(int64_t M, int N, int64_t K, int64_t kchunk,
+__global__ __launch_bounds__(TL_NT) __attribute__((amdgpu_waves_per_eu(BF && NB == 8 ? 2 : 3))) void k_tns_dense(int64_t M, int N, int64_t K, int64_t kchunk,
                                                      const float* A, int64_t lda,
                                                      const double* shA, const float* Bf,
                                                      const __bf16* Bh, const __bf16* Bl,
@@ -982,40 +1021,42 @@ __global__ __launch_bounds__(TL_NT) void k_tns_dense(int64_t M, int N, int64_t K
   // B staging: thread t -> column bn = t / 4 + 64 pass (NB / 4 passes; clamped
   // to N - 1: columns >= N only feed outputs that are never stored), 8
   // consecutive k at 8 (t % 4)
+  // (named registers, not arrays: arrays captured by the lambdas below were
+  // left in scratch memory)
   constexpr int NPASS = NB / 4;
+  static_assert(NPASS == 1 || NPASS == 2, "k_tns_dense: NB must be 4 or 8");
   const int bk = 8 * (t & 3);
-  int64_t bsrc[NPASS];
-#pragma unroll
-  for (int q = 0; q < NPASS; ++q)
-    bsrc[q] = (int64_t)std::min((t >> 2) + 64 * q, N - 1) * ldb + bk;
-  float4 bq0[NPASS], bq1[NPASS];  // FP32: 8 floats; BF: hi (bq0) / lo (bq1), 8 bf16 each
+  const int64_t bsA = (int64_t)std::min(t >> 2, N - 1) * ldb + bk;
+  const int64_t bsB = (int64_t)std::min((t >> 2) + 64, N - 1) * ldb + bk;
+  float4 bA0, bA1, bB0, bB1;  // FP32: 8 floats; BF: hi (x0) / lo (x1), 8 bf16 each
+  auto fetch1 = [&](int64_t src, float4& q0, float4& q1) {
+    if constexpr (BF) {
+      q0 = *reinterpret_cast<const float4*>(Bh + src);
+      q1 = *reinterpret_cast<const float4*>(Bl + src);
+    } else {
+      q0 = *reinterpret_cast<const float4*>(Bf + src);
+      q1 = *reinterpret_cast<const float4*>(Bf + src + 4);
+    }
+  };
   auto fetchB = [&](int64_t k0) {
-#pragma unroll
-    for (int q = 0; q < NPASS; ++q) {
-      if constexpr (BF) {
-        bq0[q] = *reinterpret_cast<const float4*>(Bh + bsrc[q] + k0);
-        bq1[q] = *reinterpret_cast<const float4*>(Bl + bsrc[q] + k0);
-      } else {
-        bq0[q] = *reinterpret_cast<const float4*>(Bf + bsrc[q] + k0);
-        bq1[q] = *reinterpret_cast<const float4*>(Bf + bsrc[q] + k0 + 4);
-      }
+    fetch1(bsA + k0, bA0, bA1);
+    if constexpr (NPASS == 2) fetch1(bsB + k0, bB0, bB1);
+  };
+  auto store1 = [&](float* base, int bn, const float4& q0, const float4& q1) {
+    if constexpr (BF) {
+      __bf16* hb = reinterpret_cast<__bf16*>(base);
+      __bf16* lb = hb + 16 * NB * TS_LDK;
+      *reinterpret_cast<float4*>(hb + bn * TS_LDK + bk) = q0;
+      *reinterpret_cast<float4*>(lb + bn * TS_LDK + bk) = q1;
+    } else {
+      *reinterpret_cast<float4*>(base + bn * TS_LDK + bk) = q0;
+      *reinterpret_cast<float4*>(base + bn * TS_LDK + bk + 4) = q1;
     }
   };
   auto storeB = [&](int buf) {
     float* base = smem + buf * TILE_F;
-#pragma unroll
-    for (int q = 0; q < NPASS; ++q) {
-      const int bn = (t >> 2) + 64 * q;
-      if constexpr (BF) {
-        __bf16* hb = reinterpret_cast<__bf16*>(base);
-        __bf16* lb = hb + 16 * NB * TS_LDK;
-        *reinterpret_cast<float4*>(hb + bn * TS_LDK + bk) = bq0[q];
-        *reinterpret_cast<float4*>(lb + bn * TS_LDK + bk) = bq1[q];
-      } else {
-        *reinterpret_cast<float4*>(base + bn * TS_LDK + bk) = bq0[q];
-        *reinterpret_cast<float4*>(base + bn * TS_LDK + bk + 4) = bq1[q];
-      }
-    }
+    store1(base, t >> 2, bA0, bA1);
+    if constexpr (NPASS == 2) store1(base, (t >> 2) + 64, bB0, bB1);
   };
   auto compute = [&](int buf, const float (&a0)[8], const float (&a1)[8]) {
     const float* base = smem + buf * TILE_F;
@@ -1070,32 +1111,33 @@ __global__ __launch_bounds__(TL_NT) void k_tns_dense(int64_t M, int N, int64_t K
       }
     }
   };
+  // A runs two steps ahead of the MFMAs in three register sets (x, y, z)
+  // whose roles rotate through a 3-way unrolled loop, so no set is copied.
+  // Each step fetches the next B tile before the A operand two steps ahead:
+  // loads complete in issue order, so the wait for B before its LDS store
+  // leaves the younger A loads in flight across the barrier.
   const int nk = (int)((kend - kbeg + TL_BK - 1) / TL_BK);
+  float z0[8], z1[8];
   if (nk > 0) {
-    fetchA(kbeg, x0, x1);
     fetchB(kbeg);
+    fetchA(kbeg, x0, x1);
+    if (nk > 1) fetchA(kbeg + TL_BK, y0, y1);
     storeB(0);
   }
   __syncthreads();
-  for (int c = 0; c < nk; ++c) {
-    const bool more = c + 1 < nk;
-    const int64_t kn = kbeg + (int64_t)(c + 1) * TL_BK;
-    if (more) {
-      fetchA(kn, y0, y1);
-      fetchB(kn);
-    }
-    // keep the next step's loads ahead of this step's MFMAs
+  auto step = [&](int c, const float (&c0)[8], const float (&c1)[8], float (&d0)[8],
+                  float (&d1)[8]) {
+    if (c + 1 < nk) fetchB(kbeg + (int64_t)(c + 1) * TL_BK);
+    if (c + 2 < nk) fetchA(kbeg + (int64_t)(c + 2) * TL_BK, d0, d1);
     __builtin_amdgcn_sched_barrier(0);
-    compute(c & 1, x0, x1);
-    if (more) {
-      storeB((c + 1) & 1);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        x0[j] = y0[j];
-        x1[j] = y1[j];
-      }
-    }
+    compute(c & 1, c0, c1);
+    if (c + 1 < nk) storeB((c + 1) & 1);
     __syncthreads();
+  };
+  for (int c = 0; c < nk; c += 3) {
+    step(c, x0, x1, z0, z1);
+    if (c + 1 < nk) step(c + 1, y0, y1, x0, x1);
+    if (c + 2 < nk) step(c + 2, z0, z1, y0, y1);
   }
   // lane holds D[4(l/16) + r][l%16] of each 16 x 16 block: rows are columns m
   // of A, columns are n
@@ -1146,6 +1188,28 @@ void tn_skinny(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float
   }
   launch(ctx, "tn_skinny_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0,
          M, (int64_t)N, (int)splits, (const float*)part.get(), C, ldc);
+}
+
+// The B operand of k_tns_dense from a column-major B (K x N, ld ldb): [n][k]
+// with ld Kp (K rounded up to 32) and zero rows K .. Kp - 1, as FP32 (Bf) or
+// as bf16 hi / lo planes (Bh, Bl).
+__global__ void k_g2_bsplit(const float* __restrict__ B, int64_t ldb, int64_t K, int N,
+                            int64_t Kp, float* __restrict__ Bf, __bf16* __restrict__ Bh,
+                            __bf16* __restrict__ Bl) {
+  const int64_t tot = Kp * N;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = e % Kp, n = e / Kp;
+    const float v = k < K ? B[k + n * ldb] : 0.f;
+    if (Bf) {
+      Bf[e] = v;
+    } else {
+      __bf16 hi, lo;
+      split_bf(v, hi, lo);
+      Bh[e] = hi;
+      Bl[e] = lo;
+    }
+  }
 }
 
 // ---------------------------------------------------------------- Gram, M <= 128
@@ -1353,6 +1417,30 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
              lda, shiftA, labelsB, ones, part.get());
     launch(ctx, "gemm_tn_reduce", k_tl_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
            (const float*)part.get(), shiftB, C, ldc);
+    return;
+  }
+  if (!labelsB && !shiftB && N <= 128) {
+    // skinny dense B: the B tile of each K step is shared through LDS by the
+    // workgroup's four waves (k_tns_dense); split K so that the grid holds
+    // about five workgroups per CU
+    const bool bf = precision == FPLS_PREC_BF16X3;
+    const int64_t Kp = (K + TL_BK - 1) / TL_BK * TL_BK;
+    DBuf<float> Bf;
+    DBuf<__bf16> Bh, Bl;
+    if (bf) {
+      Bh = DBuf<__bf16>(ctx, (size_t)Kp * N);
+      Bl = DBuf<__bf16>(ctx, (size_t)Kp * N);
+    } else {
+      Bf = DBuf<float>(ctx, (size_t)Kp * N);
+    }
+    launch(ctx, "pad_b", k_g2_bsplit, dim3(g1(Kp * N)), dim3(256), 0, B, ldb, K, (int)N, Kp,
+           Bf.get(), Bh.get(), Bl.get());
+    const int64_t tiles = (M + 4 * TL_BM - 1) / (4 * TL_BM);
+    int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, K / 256));
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N)));
+    tn_skinny(ctx, precision, M, (int)N, K, A, lda, shiftA, Bf.get(), Bh.get(), Bl.get(), Kp,
+              splits, C, ldc);
     return;
   }
   gemm_tn_dense(ctx, precision, M, N, K, A, lda, shiftA, B, ldb, labelsB, shiftB, C, ldc,

@@ -215,11 +215,12 @@ extern "C" int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* Am, int32_t k, const fpl
 namespace fpls {
 
 // out (rows x l, FP64) = A M, A the data matrix (FP32 on MFMA or FP64), M cols x l
-static void data_times(Ctx* ctx, const DMat& A, const double* M, int l, double* out) {
+static void data_times(Ctx* ctx, const DMat& A, const double* M, int l, double* out,
+                       int precision) {
   if (A.dtype == FPLS_F32) {
     DBuf<float> Mf(ctx, (size_t)A.cols * l);
     f64_to_f32(ctx, Mf.get(), M, A.cols * (int64_t)l);
-    gemm_nn_f32(ctx, FPLS_PREC_FP32, A.rows, l, A.cols, A.f(), A.ld, nullptr, Mf.get(), A.cols,
+    gemm_nn_f32(ctx, precision, A.rows, l, A.cols, A.f(), A.ld, nullptr, Mf.get(), A.cols,
                 nullptr, nullptr, out, A.rows);
   } else {
     gemm_f64(ctx, false, false, A.rows, l, A.cols, 1.0, A.d(), A.ld, nullptr, M, A.cols, nullptr,
@@ -228,11 +229,12 @@ static void data_times(Ctx* ctx, const DMat& A, const double* M, int l, double* 
 }
 
 // out (cols x l, FP64) = A^T N, N rows x l
-static void data_t_times(Ctx* ctx, const DMat& A, const double* N, int l, double* out) {
+static void data_t_times(Ctx* ctx, const DMat& A, const double* N, int l, double* out,
+                         int precision) {
   if (A.dtype == FPLS_F32) {
     DBuf<float> Nf(ctx, (size_t)A.rows * l);
     f64_to_f32(ctx, Nf.get(), N, A.rows * (int64_t)l);
-    gemm_tn_f32(ctx, FPLS_PREC_FP32, A.cols, l, A.rows, A.f(), A.ld, nullptr, Nf.get(), A.rows,
+    gemm_tn_f32(ctx, precision, A.cols, l, A.rows, A.f(), A.ld, nullptr, Nf.get(), A.rows,
                 nullptr, nullptr, out, A.cols);
   } else {
     gemm_f64(ctx, true, false, A.cols, l, A.rows, 1.0, A.d(), A.ld, nullptr, N, A.rows, nullptr,
@@ -241,13 +243,18 @@ static void data_t_times(Ctx* ctx, const DMat& A, const double* N, int l, double
 }
 
 // S = X^T Y materialised in FP64 (p x m)
-static void cross(Ctx* ctx, const DMat& X, const DMat& Y, double* S) {
+static void cross(Ctx* ctx, const DMat& X, const DMat& Y, double* S, int precision) {
   const int64_t p = X.cols, m = Y.cols;
+  if (X.dtype == FPLS_F32 && Y.dtype == FPLS_F32) {  // both FP32: Y is the MFMA operand as is
+    gemm_tn_f32(ctx, precision, p, m, X.rows, X.f(), X.ld, nullptr, Y.f(), Y.ld, nullptr, nullptr,
+                S, p);
+    return;
+  }
   DBuf<double> Yd(ctx, (size_t)Y.rows * m);
   if (Y.dtype == FPLS_F64) copy2d_f64(ctx, Yd.get(), Y.rows, Y.d(), Y.ld, Y.rows, m);
   else
     for (int64_t j = 0; j < m; ++j) f32_to_f64(ctx, Yd.get() + j * Y.rows, Y.f() + j * Y.ld, Y.rows);
-  data_t_times(ctx, X, Yd.get(), (int)m, S);
+  data_t_times(ctx, X, Yd.get(), (int)m, S, precision);
   (void)p;
 }
 
@@ -273,17 +280,32 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
     const int target = (int)std::min<int64_t>(max_rank, std::max<int>(k, 1));
     const int l = (int)std::min<int64_t>(max_rank, (int64_t)target + std::max(o.oversample, 0));
     SvdOut r;
-    if (o.svd_method == FPLS_SVD_EXACT || l >= max_rank) {
+    std::vector<double> om((size_t)m * l);
+    if (omega) {  // a caller sketch (the learned-sketch extension), m x l host FP64
+      FPLS_REQUIRE(omega->loc == FPLS_HOST && omega->dtype == FPLS_F64 && omega->rows == m &&
+                       omega->cols == l,
+                   FPLS_ERR_ARG, "rsvd_xprod: omega must be a host FP64 cols(Y) x (k + oversample)");
+      const double* src = static_cast<const double*>(omega->ptr);
+      for (int64_t j = 0; j < l; ++j)
+        for (int64_t i = 0; i < m; ++i) om[i + j * m] = src[i + j * omega->ld];
+    } else {
+      gaussian_host(om.data(), (int64_t)om.size(), o.seed);
+    }
+    const bool exact = o.svd_method == FPLS_SVD_EXACT || l >= max_rank;
+    if (exact || o.xprod_route == 1) {
+      // S formed once (one pass over X), then the exact SVD or the range
+      // finder on S with the same sketch and power schedule
       DBuf<double> S(c, (size_t)p * m);
-      cross(c, X, Y, S.get());
-      rsvd_dev(c, S.get(), p, p, m, target, o.oversample, o.power, o.seed, nullptr, Vt == nullptr,
-               true, r);
+      cross(c, X, Y, S.get(), o.precision);
+      rsvd_dev(c, S.get(), p, p, m, target, o.oversample, o.power, o.seed, om.data(),
+               Vt == nullptr, exact, r);
     } else {
       FPLS_REQUIRE(l <= kJacobiMax, FPLS_ERR_ARG, "rSVD sketch width k + oversample must be <= 128");
       const int64_t n = X.rows;
       // Y and X operands in FP64 scratch for the small products
       DBuf<double> Yd(c, (size_t)n * m);
       if (Y.dtype == FPLS_F64) copy2d_f64(c, Yd.get(), n, Y.d(), Y.ld, n, m);
+      else if (Y.ld == n) f32_to_f64(c, Yd.get(), Y.f(), n * m);
       else
         for (int64_t j = 0; j < m; ++j) f32_to_f64(c, Yd.get() + j * n, Y.f() + j * Y.ld, n);
       DMat Yv;
@@ -292,26 +314,15 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       Yv.rows = n;
       Yv.cols = m;
       Yv.ld = n;
-      std::vector<double> om((size_t)m * l);
-      if (omega) {  // a caller sketch (the learned-sketch extension), m x l host FP64
-        FPLS_REQUIRE(omega->loc == FPLS_HOST && omega->dtype == FPLS_F64 && omega->rows == m &&
-                         omega->cols == l,
-                     FPLS_ERR_ARG, "rsvd_xprod: omega must be a host FP64 cols(Y) x (k + oversample)");
-        const double* src = static_cast<const double*>(omega->ptr);
-        for (int64_t j = 0; j < l; ++j)
-          for (int64_t i = 0; i < m; ++i) om[i + j * m] = src[i + j * omega->ld];
-      } else {
-        gaussian_host(om.data(), (int64_t)om.size(), o.seed);
-      }
       DBuf<double> Om(c, om.size()), Ys(c, (size_t)p * l), Z(c, (size_t)m * l), Nn(c, (size_t)n * l);
       to_device(c, Om.get(), om.data(), om.size());
       auto a_times = [&](const double* M, double* out) {  // X^T (Y M)
-        data_times(c, Yv, M, l, Nn.get());
-        data_t_times(c, X, Nn.get(), l, out);
+        data_times(c, Yv, M, l, Nn.get(), o.precision);
+        data_t_times(c, X, Nn.get(), l, out, o.precision);
       };
       auto at_times = [&](const double* M, double* out) {  // Y^T (X M)
-        data_times(c, X, M, l, Nn.get());
-        data_t_times(c, Yv, Nn.get(), l, out);
+        data_times(c, X, M, l, Nn.get(), o.precision);
+        data_t_times(c, Yv, Nn.get(), l, out, o.precision);
       };
       a_times(Om.get(), Ys.get());
       const int q = std::max(o.power, 0);
@@ -327,7 +338,7 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       }
       orth_cholqr(c, Ys.get(), p, l);  // Q
       // B = (X Q)^T Y  (l x m)
-      data_times(c, X, Ys.get(), l, Nn.get());
+      data_times(c, X, Ys.get(), l, Nn.get(), o.precision);
       DBuf<double> Bm(c, (size_t)l * m);
       gemm_f64(c, true, false, l, m, n, 1.0, Nn.get(), n, nullptr, Yd.get(), n, nullptr, 0.0,
                Bm.get(), l);

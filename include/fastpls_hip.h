@@ -101,7 +101,10 @@ typedef struct fpls_opts {
   int32_t label_aware; /* labels + 1: .plssvd_label_aware_stream_model rules
                           (ridge 1e-10 mean(diag G), guarded sd, no B)
                           R/main.R:513-655                                    */
-  int32_t reserved[6];
+  int32_t xprod_route; /* fpls_rsvd_xprod: 0 matrix free (a_times/at_times,
+                          src/fastPLS.cpp:550-623), 1 form S = X^T Y once
+                          and sketch S (one pass over X)                      */
+  int32_t reserved[5];
 } fpls_opts;
 
 void fpls_opts_default(fpls_opts* opts);

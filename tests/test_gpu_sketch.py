@@ -12,6 +12,7 @@ import pytest
 import scipy.linalg
 
 from oracle import fastpls_oracle as O
+from tests.helpers import sin_theta
 
 pytestmark = pytest.mark.gpu
 
@@ -161,6 +162,14 @@ def test_xprod_with_sketch_argument(ctx):
         ref = O.truncated_rsvd_crossprod(X.astype(np.float64), Y.astype(np.float64), k, os_, power,
                                          1, omega=om)
         np.testing.assert_allclose(got["s"], ref["s"], rtol=2e-4)
+        # S formed once, the same range finder on S
+        mat = F.truncated_rsvd_crossprod(X, Y, k, rsvd_oversample=os_, rsvd_power=power, omega=om,
+                                         route="materialise")
+        np.testing.assert_allclose(mat["s"], ref["s"], rtol=2e-4)
+        assert sin_theta(mat["U"], ref["U"]) < 1e-3
+        b3 = F.truncated_rsvd_crossprod(X, Y, k, rsvd_oversample=os_, rsvd_power=power, omega=om,
+                                        route="materialise", precision="bf16x3")
+        np.testing.assert_allclose(b3["s"], ref["s"], rtol=1e-3)
     with pytest.raises(F.FplsError):
         F.truncated_rsvd_crossprod(X, Y, k, rsvd_oversample=os_, omega=om[:, :-1])
 
