@@ -287,6 +287,52 @@ __global__ __launch_bounds__(256) void k_gemv_n(int64_t M, int64_t K, int64_t kc
   ypart[(int64_t)blockIdx.y * M + i] = s;
 }
 
+// The same partials with VW = 16 / sizeof(T) consecutive rows per lane (one
+// 16-byte load per column) and 8 columns loaded before any is used, so each
+// lane keeps 128 bytes of X in flight; needs M and lda multiples of VW and a
+// 16-byte aligned A.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_nv(int64_t M, int64_t K, int64_t kchunk,
+                                                 const T* __restrict__ A, int64_t lda,
+                                                 const double* __restrict__ sh,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ ypart) {
+  constexpr int VW = 16 / sizeof(T);
+  constexpr int KU = 8;
+  struct alignas(16) Vec {
+    T v[VW];
+  };
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * VW;
+  const int64_t kb = (int64_t)blockIdx.y * kchunk;
+  const int64_t ke = std::min<int64_t>(K, kb + kchunk);
+  if (i0 >= M) return;
+  double s[VW];
+#pragma unroll
+  for (int r = 0; r < VW; ++r) s[r] = 0.0;
+  const T* a0 = A + i0;
+  int64_t k = kb;
+  for (; k + KU <= ke; k += KU) {
+    Vec a[KU];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) a[u] = *reinterpret_cast<const Vec*>(a0 + (k + u) * lda);
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      const double xv = x[k + u], sv = sh ? sh[k + u] : 0.0;
+#pragma unroll
+      for (int r = 0; r < VW; ++r) s[r] = fma(static_cast<double>(a[u].v[r]) - sv, xv, s[r]);
+    }
+  }
+  for (; k < ke; ++k) {
+    const Vec a = *reinterpret_cast<const Vec*>(a0 + k * lda);
+    const double xv = x[k], sv = sh ? sh[k] : 0.0;
+#pragma unroll
+    for (int r = 0; r < VW; ++r) s[r] = fma(static_cast<double>(a.v[r]) - sv, xv, s[r]);
+  }
+  double* out = ypart + (int64_t)blockIdx.y * M + i0;
+#pragma unroll
+  for (int r = 0; r < VW; ++r) out[r] = s[r];
+}
+
 __global__ void k_sum_parts(int64_t M, int parts, const double* __restrict__ part,
                             double* __restrict__ y) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < M;
@@ -315,9 +361,74 @@ __global__ __launch_bounds__(256) void k_gemv_t(int64_t M, int64_t K, const T* _
   if (lane == 0) y[k] = s;
 }
 
+// y(K) with VW = 16 / sizeof(T) consecutive rows per lane (one 16-byte load)
+// and four independent partial sums, so four loads per lane are in flight;
+// same layout conditions as k_gemv_nv.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_tv(int64_t M, int64_t K, const T* __restrict__ A,
+                                                 int64_t lda, const double* __restrict__ sh,
+                                                 const double* __restrict__ x,
+                                                 double* __restrict__ y) {
+  constexpr int VW = 16 / sizeof(T);
+  constexpr int U = 4;
+  struct alignas(16) Vec {
+    T v[VW];
+  };
+  const int lane = threadIdx.x & 63;
+  const int64_t k = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= K) return;
+  const double s0 = sh ? sh[k] : 0.0;
+  const T* col = A + k * lda;
+  constexpr int64_t STEP = 64 * VW;
+  double s[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) s[u] = 0.0;
+  int64_t i = (int64_t)lane * VW;
+  // i, M multiples of VW: i + (U - 1) STEP < M keeps all U vectors in range
+  for (; i + (U - 1) * STEP < M; i += U * STEP) {
+    Vec a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = *reinterpret_cast<const Vec*>(col + i + u * STEP);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < VW; ++r)
+        s[u] = fma(static_cast<double>(a[u].v[r]) - s0, x[i + u * STEP + r], s[u]);
+  }
+  for (; i < M; i += STEP) {
+    const Vec a = *reinterpret_cast<const Vec*>(col + i);
+#pragma unroll
+    for (int r = 0; r < VW; ++r) s[0] = fma(static_cast<double>(a.v[r]) - s0, x[i + r], s[0]);
+  }
+  double t = (s[0] + s[1]) + (s[2] + s[3]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+  if (lane == 0) y[k] = t;
+}
+
 void gemv_n(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y) {
   const int64_t M = A.rows, K = A.cols;
   if (M <= 0) return;
+  const int vw = A.dtype == FPLS_F32 ? 4 : 2;
+  if (K > 0 && M % vw == 0 && A.ld % vw == 0 && (reinterpret_cast<uintptr_t>(A.p) & 15) == 0) {
+    // vector path: about four 256-thread workgroups per CU, K slabs of >= 64
+    const int64_t rb = (M / vw + 255) / 256;
+    int64_t parts = std::max<int64_t>(1, std::min<int64_t>((4 * ctx->num_cus + rb - 1) / rb,
+                                                           (K + 63) / 64));
+    const int64_t kchunk = (K + parts - 1) / parts;
+    parts = std::max<int64_t>(1, (K + kchunk - 1) / kchunk);
+    DBuf<double> part(ctx, (size_t)parts * M);
+    dim3 grid((unsigned)rb, (unsigned)parts);
+    if (A.dtype == FPLS_F32)
+      launch(ctx, "gemv_n", k_gemv_nv<float>, grid, dim3(256), 0, M, K, kchunk, A.f(), A.ld,
+             shift, x, part.get());
+    else
+      launch(ctx, "gemv_n", k_gemv_nv<double>, grid, dim3(256), 0, M, K, kchunk, A.d(), A.ld,
+             shift, x, part.get());
+    launch(ctx, "gemv_n_reduce", k_sum_parts, dim3(grid1d(M)), dim3(256), 0, M, (int)parts,
+           part.get(), y);
+    return;
+  }
   const int64_t rb = (M + 255) / 256;
   int64_t parts = std::max<int64_t>(1, std::min<int64_t>((2 * ctx->num_cus + rb - 1) / rb,
                                                          (K + 63) / 64));
@@ -339,6 +450,15 @@ void gemv_t(Ctx* ctx, const DMat& A, const double* shift, const double* x, doubl
   const int64_t M = A.rows, K = A.cols;
   if (K <= 0) return;
   dim3 grid((unsigned)((K + 3) / 4));
+  const int vw = A.dtype == FPLS_F32 ? 4 : 2;
+  if (M > 0 && M % vw == 0 && A.ld % vw == 0 && (reinterpret_cast<uintptr_t>(A.p) & 15) == 0) {
+    if (A.dtype == FPLS_F32)
+      launch(ctx, "gemv_t", k_gemv_tv<float>, grid, dim3(256), 0, M, K, A.f(), A.ld, shift, x, y);
+    else
+      launch(ctx, "gemv_t", k_gemv_tv<double>, grid, dim3(256), 0, M, K, A.d(), A.ld, shift, x,
+             y);
+    return;
+  }
   if (A.dtype == FPLS_F32)
     launch(ctx, "gemv_t", k_gemv_t<float>, grid, dim3(256), 0, M, K, A.f(), A.ld, shift, x, y);
   else
