@@ -9,8 +9,8 @@
 // a class), so every class is one contiguous block of rows.  Prediction:
 //   base = Ttest_n C^T (MFMA), top-m candidate classes per row (wave argmax,
 //   ties -> lower class), (row, candidate) pairs bucketed by class, then one
-//   work item per (class, 32 pairs, 4096-row chunk) computes the 32 x chunk
-//   similarity block on MFMA and keeps running top-k lists; partial lists are
+//   work item per (class, 128 pairs, 4096-row chunk) computes the 128 x chunk
+//   similarity block as an MFMA GEMM and keeps running top-k lists; partial lists are
 //   merged per pair, the temperature-smoothed score is formed in FP64 and the
 //   candidates are re-ranked with the reference's strict '>' insertion.
 #include <algorithm>
@@ -25,8 +25,7 @@ namespace fpls {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-constexpr int CK_PT = 32;       // pairs per work item
-constexpr int CK_RT = 64;       // training rows per MFMA tile
+constexpr int CK_PT = 128;      // pairs per work item (CKG_P)
 constexpr int CK_CHUNK = 4096;  // training rows per work item
 constexpr int CK_KNN_MAX = 32;  // largest knn_k (same clamp as the CUDA kernel of the reference)
 constexpr int SORT_CHUNK = 4096;
@@ -232,7 +231,7 @@ __global__ void k_pair_scatter(const int32_t* __restrict__ cand, int64_t npairs,
 // ---------------------------------------------------------------- candidate kNN partials
 struct CkItem {
   int cls;        // 0-based class
-  int pair_beg;   // first bucket slot of the 32 pairs
+  int pair_beg;   // first bucket slot of the item's pairs
   int pair_end;
   int row_beg;    // sorted training rows of this chunk
   int row_end;
@@ -241,93 +240,146 @@ struct CkItem {
   int part_base;  // float offset of this class's partial lists
 };
 
-template <int KMAXT>
-__global__ __launch_bounds__(256) void k_cand_partial(const float* __restrict__ Ttest, int k,
-                                                      const float* __restrict__ Tn,
-                                                      const int* __restrict__ bucket, int m,
-                                                      const CkItem* __restrict__ items,
-                                                      int knn_k, float* __restrict__ part) {
-  __shared__ float At[CK_PT][KMAXT + 4];
-  __shared__ float Bt[CK_RT][17];
-  __shared__ float Sim[CK_PT][CK_RT + 1];
-  __shared__ float Lst[2 * CK_PT][CK_KNN_MAX];
+// GEMM form of the same partial lists: work item = class c x up to CKG_P
+// pairs (test rows whose candidates include c) x one chunk of c's training
+// rows.  Sim = A B^T for A the pairs' normalised test rows and B a tile of
+// CKG_R normalised training rows, K streamed through LDS in CKG_KS-deep
+// steps (double buffered, one barrier per step).  4 waves, each a 64 x 64
+// block of Sim: 4 x 4 v_mfma_f32_16x16x4f32 accumulators; within a step lane
+// l takes k = 4 (l / 16) + s in sub-step s (any bijection of k shared by both
+// operands gives the same sum), so each operand fragment is one 16-byte LDS
+// read.  After each tile Sim goes through LDS (aliasing the operand buffers)
+// and thread t keeps a register top-knn_k list of pair t / 2 over half the
+// tile's columns; lists are written in the layout k_cand_merge reads.
+constexpr int CKG_P = CK_PT, CKG_R = 128, CKG_KS = 16, CKG_LD = CKG_KS + 4;
+constexpr int CKG_OPS = 2 * (CKG_P + CKG_R) * CKG_LD;  // floats, both buffers
+constexpr int CKG_SIM = CKG_P * (CKG_R + 1);
+constexpr int CKG_SMEM = CKG_OPS > CKG_SIM ? CKG_OPS : CKG_SIM;
+
+__global__ __launch_bounds__(256) void k_cand_gemm(const float* __restrict__ Ttest, int k,
+                                                   const float* __restrict__ Tn,
+                                                   const int* __restrict__ bucket, int m,
+                                                   const CkItem* __restrict__ items,
+                                                   int knn_k, float* __restrict__ part) {
+  __shared__ __align__(16) float smem[CKG_SMEM];
   const CkItem it = items[blockIdx.x];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int np = it.pair_end - it.pair_beg;
-  // test rows of the pairs (row-major, k contiguous)
-  for (int idx = t; idx < CK_PT * k; idx += 256) {
-    const int p = idx / k, j = idx % k;
-    float v = 0.f;
-    if (p < np) {
-      const int pair = bucket[it.pair_beg + p];
-      const int64_t row = pair / m;
-      v = Ttest[row * k + j];
-    }
-    At[p][j] = v;
-  }
-  for (int idx = t; idx < 2 * CK_PT * CK_KNN_MAX; idx += 256) Lst[idx / CK_KNN_MAX][idx % CK_KNN_MAX] = -INFINITY;
-  const int rb = w & 1;          // 16-row block of pairs
-  const int cb0 = (w >> 1) * 2;  // two 16-column blocks of training rows
-  for (int r0 = it.row_beg; r0 < it.row_end; r0 += CK_RT) {
-    v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < k; k0 += 16) {
-      __syncthreads();
-      {
-        const int r = t >> 2, e0 = (t & 3) * 4;
-        const int64_t row = (int64_t)r0 + r;
+  const int ph = w & 1, th = w >> 1;  // 64-row block of pairs, 64-column block of rows
+  // staging: thread -> operand row t / 2, 8 consecutive k at 8 (t % 2)
+  const int srow = t >> 1, sk = (t & 1) * 8;
+  const float* arow = nullptr;
+  if (srow < np) arow = Ttest + (int64_t)(bucket[it.pair_beg + srow] / m) * k;
+  const bool vec = (k & 3) == 0 && ((reinterpret_cast<uintptr_t>(Ttest) |
+                                     reinterpret_cast<uintptr_t>(Tn)) & 15) == 0;
+  float qa[8], qb[8];
+  auto load8 = [&](const float* src, int k0, float (&q)[8]) {
+    const int j0 = k0 + sk;
+    if (src && vec && j0 + 8 <= k) {
+      const float4 u = *reinterpret_cast<const float4*>(src + j0);
+      const float4 v = *reinterpret_cast<const float4*>(src + j0 + 4);
+      q[0] = u.x; q[1] = u.y; q[2] = u.z; q[3] = u.w;
+      q[4] = v.x; q[5] = v.y; q[6] = v.z; q[7] = v.w;
+    } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int j = k0 + e0 + e;
-          Bt[r][e0 + e] = (row < it.row_end && j < k) ? Tn[row * k + j] : 0.f;
+      for (int e = 0; e < 8; ++e) q[e] = (src && j0 + e < k) ? src[j0 + e] : 0.f;
+    }
+  };
+  auto store8 = [&](float* dst, const float (&q)[8]) {
+    *reinterpret_cast<float4*>(dst) = make_float4(q[0], q[1], q[2], q[3]);
+    *reinterpret_cast<float4*>(dst + 4) = make_float4(q[4], q[5], q[6], q[7]);
+  };
+  auto opA = [&](int buf) { return smem + buf * (CKG_P + CKG_R) * CKG_LD; };
+  auto opB = [&](int buf) { return smem + buf * (CKG_P + CKG_R) * CKG_LD + CKG_P * CKG_LD; };
+  // list entries are only ever indexed by compile-time constants (a runtime
+  // index would put the array in scratch memory); thr = L[knn_k - 1]
+  float L[CK_KNN_MAX];
+#pragma unroll
+  for (int j = 0; j < CK_KNN_MAX; ++j) L[j] = -INFINITY;
+  float thr = -INFINITY;
+  const int nks = (k + CKG_KS - 1) / CKG_KS;
+  for (int r0 = it.row_beg; r0 < it.row_end; r0 += CKG_R) {
+    const float* brow = (r0 + srow < it.row_end) ? Tn + (int64_t)(r0 + srow) * k : nullptr;
+    v4f acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    load8(arow, 0, qa);
+    load8(brow, 0, qb);
+    __syncthreads();  // the previous tile's Sim reads are done
+    store8(opA(0) + srow * CKG_LD + sk, qa);
+    store8(opB(0) + srow * CKG_LD + sk, qb);
+    __syncthreads();
+    const int q4 = 4 * (lane >> 4), li = lane & 15;
+    for (int st = 0; st < nks; ++st) {
+      const bool more = st + 1 < nks;
+      if (more) {
+        load8(arow, (st + 1) * CKG_KS, qa);
+        load8(brow, (st + 1) * CKG_KS, qb);
+      }
+      const float* A = opA(st & 1);
+      const float* B = opB(st & 1);
+      float4 af[4], bf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        af[a] = *reinterpret_cast<const float4*>(A + (ph * 64 + a * 16 + li) * CKG_LD + q4);
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        bf[b] = *reinterpret_cast<const float4*>(B + (th * 64 + b * 16 + li) * CKG_LD + q4);
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a].x, bf[b].x, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a].y, bf[b].y, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a].z, bf[b].z, acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a].w, bf[b].w, acc[a][b], 0, 0, 0);
         }
+      if (more) {
+        store8(opA((st + 1) & 1) + srow * CKG_LD + sk, qa);
+        store8(opB((st + 1) & 1) + srow * CKG_LD + sk, qb);
       }
       __syncthreads();
-#pragma unroll
-      for (int kk = 0; kk < 16; kk += 4) {
-        const int j = k0 + kk + lane / 16;
-        const float a = (j < k) ? At[rb * 16 + lane % 16][j] : 0.f;
-        const float b0 = Bt[cb0 * 16 + lane % 16][kk + lane / 16];
-        const float b1 = Bt[(cb0 + 1) * 16 + lane % 16][kk + lane / 16];
-        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b1, acc1, 0, 0, 0);
-      }
     }
-    __syncthreads();
+    // Sim tile through LDS: lane holds D[4 (l / 16) + r][l % 16] of each block
+    float* Sim = smem;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int pi = rb * 16 + 4 * (lane / 16) + r;
-      Sim[pi][cb0 * 16 + lane % 16] = acc0[r];
-      Sim[pi][(cb0 + 1) * 16 + lane % 16] = acc1[r];
-    }
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Sim[(ph * 64 + a * 16 + q4 + r) * (CKG_R + 1) + th * 64 + b * 16 + li] = acc[a][b][r];
     __syncthreads();
-    if (t < 2 * CK_PT) {
-      const int p = t >> 1, h = t & 1;
-      if (p < np) {
-        float* L = Lst[t];
-        for (int c = h * 32; c < h * 32 + 32; ++c) {
-          if (r0 + c >= it.row_end) break;
-          const float v = Sim[p][c];
-          if (v > L[knn_k - 1]) {  // strict '>' insertion (src/fastPLS.cpp:1249-1267)
-            int pos = knn_k - 1;
-            while (pos > 0 && v > L[pos - 1]) {
-              L[pos] = L[pos - 1];
-              --pos;
+    if (srow < np) {
+      const int cend = min(CKG_R, it.row_end - r0);
+      const float* srow_p = Sim + srow * (CKG_R + 1);
+      for (int c = (t & 1) * 64; c < (t & 1) * 64 + 64 && c < cend; ++c) {
+        float v = srow_p[c];
+        if (v > thr) {  // a value the list keeps: bubble it into place
+#pragma unroll
+          for (int j = 0; j < CK_KNN_MAX; ++j) {
+            if (j < knn_k && v > L[j]) {
+              const float o = L[j];
+              L[j] = v;
+              v = o;
             }
-            L[pos] = v;
           }
+#pragma unroll
+          for (int j = 0; j < CK_KNN_MAX; ++j)
+            if (j == knn_k - 1) thr = L[j];
         }
       }
     }
   }
-  __syncthreads();
-  if (t < 2 * CK_PT) {
-    const int p = t >> 1, h = t & 1;
-    if (p < np) {
-      const int slot = it.pair_beg + p;  // bucket slot
-      float* dst = part + (int64_t)it.part_base +
-                   ((int64_t)(slot) * it.nchunks + it.chunk) * 2 * knn_k + h * knn_k;
-      for (int j = 0; j < knn_k; ++j) dst[j] = Lst[t][j];
-    }
+  if (srow < np) {
+    const int slot = it.pair_beg + srow;
+    float* dst = part + (int64_t)it.part_base +
+                 ((int64_t)slot * it.nchunks + it.chunk) * 2 * knn_k + (t & 1) * knn_k;
+#pragma unroll
+    for (int j = 0; j < CK_KNN_MAX; ++j)
+      if (j < knn_k) dst[j] = L[j];
   }
 }
 
@@ -552,18 +604,8 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
   h2d(ctx, d_csize.get(), csize.data(), C * sizeof(int));
   if (!items.empty()) {
     h2d(ctx, d_items.get(), items.data(), items.size() * sizeof(CkItem));
-    if (k <= 128)
-      launch(ctx, "cknn_partial", k_cand_partial<128>, dim3((unsigned)items.size()), dim3(256), 0,
-             Ttn, k, Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k,
-             part.get());
-    else if (k <= 512)
-      launch(ctx, "cknn_partial", k_cand_partial<512>, dim3((unsigned)items.size()), dim3(256), 0,
-             Ttn, k, Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k,
-             part.get());
-    else
-      launch(ctx, "cknn_partial", k_cand_partial<1024>, dim3((unsigned)items.size()), dim3(256), 0,
-             Ttn, k, Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k,
-             part.get());
+    launch(ctx, "cknn_partial", k_cand_gemm, dim3((unsigned)items.size()), dim3(256), 0, Ttn, k,
+           Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k, part.get());
   }
   if (nslots > 0)
     launch(ctx, "cknn_merge", k_cand_merge, dim3((nslots + 255) / 256), dim3(256), 0,

@@ -29,8 +29,10 @@ constexpr int JLD = JMAX + 1;
 // mode 2: (A_k + r I) X = diag(d_k), r = 1e-10 mean(diag A_k)  -- label-aware PLSSVD
 // The SPD inverse is a symmetric sweep held in registers (one barrier per
 // pivot); X = A^{-1} RHS.
-template <int KMAX>
-__global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
+// GS: side of the thread grid of the sweep (GS x GS threads, each holding
+// (KMAX / GS)^2 entries in registers).
+template <int KMAX, int GS>
+__global__ __launch_bounds__(GS * GS) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
                                                     SliceOrders ks, int mode,
                                                     const double* __restrict__ dvals,
                                                     double ridge, double* __restrict__ X,
@@ -79,7 +81,8 @@ __global__ __launch_bounds__(1024) void k_spd_solve(const double* __restrict__ A
   // leading minor is not positive definite) and sends the slice to the LU
   // fallback, as the reference's solve(likely_sympd) does.
   {
-    constexpr int G = 32, R = 4;  // grid side, register blocks per side
+    constexpr int G = GS, R = KMAX / GS;  // grid side, register blocks per side
+    static_assert(KMAX == 128 && G * R == KMAX && G * G >= 256, "k_spd_solve: grid shape");
     __shared__ double snap[2][128];
     const int tr = t / G, tc = t % G;
     const bool active = t < G * G;
@@ -278,8 +281,8 @@ void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const st
     double* Xc = X + (int64_t)c0 * slice_stride;
     double* lc = lambda_out ? lambda_out + c0 : nullptr;
     int32_t* sc = status ? status + c0 : nullptr;
-    launch(ctx, "spd_solve", k_spd_solve<128>, dim3(cnt), dim3(1024), 0, A, lda, so, mode, dvals,
-           ridge, Xc, ldx, slice_stride, lc, sc);
+    launch(ctx, "spd_solve", k_spd_solve<128, 32>, dim3(cnt), dim3(1024), 0, A, lda, so, mode,
+           dvals, ridge, Xc, ldx, slice_stride, lc, sc);
   }
 }
 
