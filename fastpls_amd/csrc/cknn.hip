@@ -571,8 +571,11 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
     FPLS_REQUIRE(nch[c] <= 64, FPLS_ERR_ARG, "a class exceeds 262144 training rows");
     pbase[c] = (int)ptotal;
     ptotal += (int64_t)hc[c] * nch[c] * 2 * knn_k;
-    for (int pb = 0; pb < hc[c]; pb += CK_PT)
-      for (int ch = 0; ch < nch[c]; ++ch) {
+    // chunk-major order: the pair groups of one chunk run together, so the
+    // chunk's training rows are fetched from HBM once and then served by the
+    // caches to the other groups
+    for (int ch = 0; ch < nch[c]; ++ch)
+      for (int pb = 0; pb < hc[c]; pb += CK_PT) {
         CkItem it;
         it.cls = c;
         it.pair_beg = slot0[c] + pb;
