@@ -122,10 +122,31 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
         fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
         precision="fp32", label_aware=False, fetch=True, device=0, top=1,
-        n_classes=None) -> FastPLS:
+        n_classes=None, gaussian_y=False, gaussian_y_dim=None) -> FastPLS:
     """pls() (R/main.R:5860-6314) with backend = "hip".  ``n_classes``: level
-    count of device-resident integer labels (optional)."""
+    count of device-resident integer labels (optional).  ``gaussian_y``: fit a
+    Gaussian compression of the response and decode the predictions
+    (.prepare_gaussian_y, R/main.R:6057-6068; argmax classifier)."""
     _check_backend(backend)
+    if gaussian_y:
+        if classifier != "argmax":
+            raise ValueError("gaussian_y supports the argmax classifier")
+        yprep = prepare_gaussian_y(Ytrain, Xtrain, True, gaussian_y_dim, seed, device)
+        model = pls(Xtrain, yprep["Ytrain"], None, None, ncomp, scaling, method, svd_method,
+                    "argmax", lda_ridge, candidate_knn_k, candidate_tau, candidate_alpha,
+                    candidate_top_m, fit, backend, seed, rsvd_oversample, rsvd_power,
+                    precision, False, fetch, device, top)
+        _attach_gaussian_y(model, yprep)
+        if fit and fetch and not yprep["classification"] and "Yfit" in model:
+            Yf = decode_gaussian_y_cube(model["gaussian_y"], model["Yfit"], device)
+            model["Yfit"] = Yf
+            Yo = np.asarray(Ytrain, dtype=np.float64).reshape(Yf.shape[0], -1)
+            tss = float(((Yo - Yo.mean(axis=0)) ** 2).sum())
+            model["R2Y"] = np.array([1.0 - float(((Yo - Yf[:, :, a]) ** 2).sum()) / tss
+                                     for a in range(Yf.shape[2])])
+        if Xtest is not None:
+            model.update(predict(model, Xtest, Ytest, top=top))
+        return model
     ctx = _lib.context(device)
     keep: list = []
     X = as_mat(Xtrain, keep)
@@ -213,6 +234,9 @@ def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
     m = inf.m
     yp = np.empty((w, m, L), dtype=np.float64, order="F")
     ctx.check(ctx.lib.fpls_predict(ctx.h, model.handle, ctypes.byref(X), yp.ctypes.data))
+    if model.get("gaussian_y"):
+        # .decode_gaussian_y_cube (R/main.R:2099-2118) after the prediction
+        yp = decode_gaussian_y_cube(model["gaussian_y"], yp, ctx.device)
     if model.get("classification"):
         idx = (np.argmax(yp, axis=1) + 1).astype(np.int32)  # first maximum
         res["Ypred_index"] = idx
@@ -229,6 +253,115 @@ def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
                 q2.append(1.0 - press / tss)
             res["Q2Y"] = np.array(q2)
     return res
+
+
+def gaussian_y_default_dim(Xtrain, gaussian_y_dim=None) -> int:
+    """.gaussian_y_default_dim (R/main.R:1945-1954): min(ncol(X), 100)."""
+    if gaussian_y_dim is None:
+        p = Xtrain.shape[1] if hasattr(Xtrain, "shape") and len(Xtrain.shape) > 1 else 1
+        return max(1, min(int(p), 100))
+    d = int(np.atleast_1d(gaussian_y_dim)[0])
+    if d < 1:
+        raise ValueError("gaussian_y_dim must be a positive integer")
+    return d
+
+
+def prepare_gaussian_y(Ytrain, Xtrain, gaussian_y: bool = False, gaussian_y_dim=None,
+                       seed: int = 1, device: int = 0) -> Dict:
+    """.prepare_gaussian_y (R/main.R:1997-2052): the compressed response Z
+    (host FP64, or a device FP64 column-major tensor for a device Y) and the
+    decoding spec.  The projection / codebook come from this library's
+    seeded Gaussian stream (fpls_gaussian_y_encode / fpls_gaussian_y_codes)."""
+    classification = _is_factor(Ytrain)
+    if not gaussian_y:
+        return {"Ytrain": Ytrain, "classification": classification, "lev": None,
+                "gaussian": None}
+    dim = gaussian_y_default_dim(Xtrain, gaussian_y_dim)
+    ctx = _lib.context(device)
+    keep: list = []
+    seed32 = int(seed) & 0xFFFFFFFF
+    if classification:
+        lev, codes_i, C = _encode_factor(Ytrain, None)
+        yv = as_ivec(codes_i, keep)
+        n = yv.n
+        codes = np.empty((C, dim), order="F")
+        Z, Zm = _out_matrix(n, dim, Ytrain, keep)
+        ctx.check(ctx.lib.fpls_gaussian_y_codes(ctx.h, ctypes.byref(yv), C, dim, seed32,
+                                                ctypes.byref(Zm), codes.ctypes.data))
+        return {"Ytrain": Z, "classification": True, "lev": lev,
+                "gaussian": {"enabled": True, "task": "classification", "dim": dim,
+                             "original_dim": C, "codes": codes}}
+    Ym = as_mat(Ytrain, keep)
+    n, m = Ym.rows, Ym.cols
+    center = np.empty(m)
+    decoder = np.empty((dim, m), order="F")
+    Z, Zm = _out_matrix(n, dim, Ytrain, keep)
+    ctx.check(ctx.lib.fpls_gaussian_y_encode(ctx.h, ctypes.byref(Ym), dim, seed32,
+                                             ctypes.byref(Zm), center.ctypes.data,
+                                             decoder.ctypes.data, None))
+    return {"Ytrain": Z, "classification": False, "lev": None,
+            "gaussian": {"enabled": True, "task": "regression", "dim": dim, "original_dim": m,
+                         "y_center": center, "decoder": decoder}}
+
+
+def _out_matrix(rows: int, cols: int, like, keep: list):
+    """An FP64 output matrix on the side of ``like``: a device column-major
+    tensor for a torch input, else a host Fortran array."""
+    if _lib._is_torch(like):
+        import torch
+        t = torch.empty((cols, rows), dtype=torch.float64, device=like.device).t()
+        keep.append(t)
+        return t, _lib.Mat(t.data_ptr(), _lib.F64, _lib.DEVICE, rows, cols, max(rows, 1))
+    a = np.empty((rows, cols), order="F")
+    keep.append(a)
+    return a, _lib.Mat(a.ctypes.data, _lib.F64, _lib.HOST, rows, cols, max(rows, 1))
+
+
+def _attach_gaussian_y(model: "FastPLS", yprep: Dict) -> None:
+    """.attach_gaussian_y (R/main.R:2054-2075)."""
+    spec = yprep["gaussian"]
+    model["gaussian_y"] = spec
+    model["gaussian_y_task"] = spec["task"]
+    model["gaussian_y_dim"] = spec["dim"]
+    model["gaussian_y_original_dim"] = spec["original_dim"]
+    model["classification"] = yprep["classification"]
+    model["lev"] = yprep["lev"]
+    model["classification_rule"] = "argmax"
+
+
+def decode_gaussian_y_matrix(spec: Dict, Z, device: int = 0) -> np.ndarray:
+    """.decode_gaussian_y_matrix (R/main.R:2077-2096): regression predictions
+    (n x m) or class scores 2 Z codes^T - ||code||^2 (n x C)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Zm = as_mat(np.asarray(Z, dtype=np.float64) if not _lib._is_torch(Z) else Z, keep)
+    n = Zm.rows
+    if spec["task"] == "classification":
+        codes = np.asfortranarray(spec["codes"], dtype=np.float64)
+        C = codes.shape[0]
+        labels = np.empty(n, dtype=np.int32)
+        scores = np.empty((n, C), order="F")
+        ctx.check(ctx.lib.fpls_gaussian_y_classes(ctx.h, ctypes.byref(Zm), codes.ctypes.data, C,
+                                                  labels.ctypes.data, scores.ctypes.data))
+        return scores
+    dec = np.asfortranarray(spec["decoder"], dtype=np.float64)
+    cen = np.ascontiguousarray(spec["y_center"], dtype=np.float64)
+    m = dec.shape[1]
+    Y = np.empty((n, m), order="F")
+    Ym = _lib.Mat(Y.ctypes.data, _lib.F64, _lib.HOST, n, m, max(n, 1))
+    ctx.check(ctx.lib.fpls_gaussian_y_decode(ctx.h, ctypes.byref(Zm), dec.ctypes.data, m,
+                                             cen.ctypes.data, ctypes.byref(Ym)))
+    return Y
+
+
+def decode_gaussian_y_cube(spec: Dict, Zcube, device: int = 0) -> np.ndarray:
+    """.decode_gaussian_y_cube (R/main.R:2099-2118): slice by slice."""
+    Zc = np.asarray(Zcube, dtype=np.float64)
+    if Zc.ndim == 2:
+        return decode_gaussian_y_matrix(spec, Zc, device)
+    out = [decode_gaussian_y_matrix(spec, np.asfortranarray(Zc[:, :, i]), device)
+           for i in range(Zc.shape[2])]
+    return np.stack(out, axis=2)
 
 
 def fastsvd(A, ncomp: int, method: str = "rsvd", rsvd_oversample: int = 10,

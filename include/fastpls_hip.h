@@ -40,6 +40,11 @@
  *   fpls_gaussian_streams     gaussian_matrix_local      src/fastPLS.cpp:99-115
  *   fpls_sketch_train / fpls_sketch_eval  the learned-sketch extension of the
  *                             Gaussian range finder (no reference counterpart)
+ *   fpls_gaussian_y_encode    .prepare_gaussian_y (regression) R/main.R:1997-2052,
+ *                             .gaussian_y_ridge_decoder        R/main.R:1967-1980
+ *   fpls_gaussian_y_codes     .gaussian_y_class_codes          R/main.R:1982-1995
+ *   fpls_gaussian_y_decode    .decode_gaussian_y_matrix (regr.) R/main.R:2077-2096
+ *   fpls_gaussian_y_classes   nearest_code_classes       src/fastPLS.cpp:4203-4215
  */
 #ifndef FASTPLS_HIP_H
 #define FASTPLS_HIP_H
@@ -260,6 +265,36 @@ int fpls_sketch_train(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const
 int fpls_sketch_eval(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const int8_t* masks,
                      int32_t n_draws, int32_t k, const double* omega, int32_t l, int32_t power,
                      const fpls_opts* opts, double* sin_theta);
+
+/* ---- Gaussian response compression (gaussian_y) --------------------------- */
+/* Output matrices (Z_out, Y_out) are caller buffers described by fpls_mat
+   (host or device, FPLS_F32 or FPLS_F64); the library writes through ptr.
+   The random projection and codebook come from the std::mt19937 +
+   normal_distribution<double> stream of `seed` (the reference draws them
+   from R's RNG), scaled by 1/sqrt(dim), column-major.  */
+
+/* Regression (.prepare_gaussian_y, R/main.R:2031-2052): center (m) = column
+   means of Y, Yc = Y - center, P (m x dim), Z_out (n x dim) = Yc P,
+   decoder (dim x m) = (Z^T Z + ridge I)^{-1} Z^T Yc with ridge = 1e-8
+   mean(diag Z^T Z) (.gaussian_y_ridge_decoder, R/main.R:1967-1980);
+   projection (m x dim) is optional.  center, decoder, projection: host FP64. */
+int fpls_gaussian_y_encode(fpls_ctx* ctx, const fpls_mat* Y, int32_t dim, uint32_t seed,
+                           const fpls_mat* Z_out, double* center, double* decoder,
+                           double* projection);
+/* Classification codebook (.gaussian_y_class_codes, R/main.R:1982-1995):
+   codes (C x dim, host FP64) with centred unit-norm columns and unit-norm rows;
+   when Z_out is given, Z_out (n x dim) = codes[y, ] (R/main.R:2019-2021). */
+int fpls_gaussian_y_codes(fpls_ctx* ctx, const fpls_ivec* y, int32_t n_classes, int32_t dim,
+                          uint32_t seed, const fpls_mat* Z_out, double* codes);
+/* Regression decode (.decode_gaussian_y_matrix, R/main.R:2091-2096):
+   Y_out (n x m) = Z decoder + 1 center^T. */
+int fpls_gaussian_y_decode(fpls_ctx* ctx, const fpls_mat* Z, const double* decoder, int32_t m,
+                           const double* center, const fpls_mat* Y_out);
+/* Class decode (nearest_code_classes, src/fastPLS.cpp:4203-4215;
+   R/main.R:2081-2087): scores = 2 Z codes^T - ||code_c||^2, labels (n, host,
+   1-based) = first maximum; scores (n x C, host) optional. */
+int fpls_gaussian_y_classes(fpls_ctx* ctx, const fpls_mat* Z, const double* codes,
+                            int32_t n_classes, int32_t* labels, double* scores);
 
 /* ---- cross-validation ----------------------------------------------------------- */
 /* pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out

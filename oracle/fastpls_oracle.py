@@ -836,6 +836,78 @@ def candidate_knn_predictions(model: Dict, Xtest, top: int = 1):
 
 
 # ---------------------------------------------------------------------------
+# Gaussian response compression (R/main.R:1945-2142, src/fastPLS.cpp:4203-4215)
+# The reference draws the projection and the codebook from R's RNG
+# (set.seed + rnorm); here they come from gaussian_matrix(seed) -- that draw is
+# "parity unpinned" (no R here); everything after the draw follows the
+# reference and is pinned in tests/test_oracle_gaussian_y.py.
+# ---------------------------------------------------------------------------
+
+def gaussian_y_default_dim(p: int, gaussian_y_dim=None) -> int:
+    """.gaussian_y_default_dim (R/main.R:1945-1954)."""
+    if gaussian_y_dim is None:
+        return max(1, min(int(p), 100))
+    d = int(gaussian_y_dim)
+    if d < 1:
+        raise ValueError("gaussian_y_dim must be a positive integer")
+    return d
+
+
+def gaussian_y_ridge_decoder(Z: np.ndarray, Yc: np.ndarray) -> np.ndarray:
+    """.gaussian_y_ridge_decoder (R/main.R:1967-1980): solve(Z'Z + ridge I, Z'Yc)."""
+    ztz = Z.T @ Z
+    ridge = 1e-8 * float(np.mean(np.diag(ztz)))
+    if not math.isfinite(ridge) or ridge <= 0:
+        ridge = 1e-8
+    return np.linalg.solve(ztz + ridge * np.eye(ztz.shape[0]), Z.T @ Yc)
+
+
+def gaussian_y_class_codes(n_classes: int, dim: int, seed: int) -> np.ndarray:
+    """.gaussian_y_class_codes (R/main.R:1982-1995); draw from gaussian_matrix."""
+    codes = gaussian_matrix(n_classes, dim, seed) / math.sqrt(dim)
+    codes = codes - codes.mean(axis=0)
+    sds = np.sqrt((codes * codes).sum(axis=0))
+    sds[~np.isfinite(sds) | (sds == 0)] = 1.0
+    codes = codes / sds
+    rn = np.sqrt((codes * codes).sum(axis=1))
+    rn[~np.isfinite(rn) | (rn == 0)] = 1.0
+    return codes / rn[:, None]
+
+
+def prepare_gaussian_y_regression(Y: np.ndarray, dim: int, seed: int) -> Dict:
+    """.prepare_gaussian_y, regression branch (R/main.R:2031-2052)."""
+    Y = np.asarray(Y, dtype=np.float64)
+    center = Y.mean(axis=0)
+    Yc = Y - center
+    P = gaussian_matrix(Y.shape[1], dim, seed) / math.sqrt(dim)
+    Z = Yc @ P
+    return {"Z": Z, "y_center": center, "decoder": gaussian_y_ridge_decoder(Z, Yc),
+            "projection": P}
+
+
+def prepare_gaussian_y_classes(y_codes, n_classes: int, dim: int, seed: int) -> Dict:
+    """.prepare_gaussian_y, classification branch (R/main.R:2015-2030)."""
+    codes = gaussian_y_class_codes(n_classes, dim, seed)
+    y = np.asarray(y_codes, dtype=np.int64)
+    return {"Z": codes[y - 1], "codes": codes}
+
+
+def decode_gaussian_y_regression(Z, decoder, center) -> np.ndarray:
+    """.decode_gaussian_y_matrix, regression (R/main.R:2089-2096)."""
+    return np.asarray(Z, dtype=np.float64) @ decoder + np.asarray(center)[None, :]
+
+
+def gaussian_y_class_scores(Z, codes) -> np.ndarray:
+    """.decode_gaussian_y_matrix, classification (R/main.R:2081-2087)."""
+    return 2.0 * (np.asarray(Z, dtype=np.float64) @ codes.T) - (codes * codes).sum(axis=1)[None, :]
+
+
+def nearest_code_classes(Z, codes) -> np.ndarray:
+    """nearest_code_classes (src/fastPLS.cpp:4203-4215): 1-based, first maximum."""
+    return (np.argmax(gaussian_y_class_scores(Z, codes), axis=1) + 1).astype(np.int32)
+
+
+# ---------------------------------------------------------------------------
 # end-to-end pls() flows used by the tests and the CPU baseline (R/main.R:5860-6314)
 # ---------------------------------------------------------------------------
 
