@@ -10,6 +10,13 @@
 #include <cmath>
 
 #include "fpls_internal.h"
+#include "fpls_wave.h"
+
+// Timing hook for stage-level measurements of the slice solve; compiled out
+// unless a probe build defines it.
+#ifndef FPLS_SPD_STAMP
+#define FPLS_SPD_STAMP(slot)
+#endif
 
 namespace fpls {
 
@@ -29,17 +36,49 @@ constexpr int JLD = JMAX + 1;
 // mode 2: (A_k + r I) X = diag(d_k), r = 1e-10 mean(diag A_k)  -- label-aware PLSSVD
 // The SPD inverse is a symmetric sweep held in registers (one barrier per
 // pivot); X = A^{-1} RHS.
-// GS: side of the thread grid of the sweep (GS x GS threads, each holding
-// (KMAX / GS)^2 entries in registers).
-template <int KMAX, int GS>
-__global__ __launch_bounds__(GS * GS) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
-                                                    SliceOrders ks, int mode,
-                                                    const double* __restrict__ dvals,
-                                                    double ridge, double* __restrict__ X,
-                                                    int64_t ldx, int64_t slice_stride,
-                                                    double* __restrict__ lambda_out,
-                                                    int32_t* __restrict__ status) {
-  __shared__ double L[KMAX * (KMAX + 1)];
+// Blocked symmetric sweep (Gauss-Jordan without pivoting on an SPD matrix;
+// Goodnight 1979) over pivot blocks of SPD_B = 16: with D = A_JJ,
+//   A_JJ <- -D^{-1},  A_iJ <- A_iJ D^{-1},  A_ij <- A_ij - A_iJ D^{-1} A_Jj,
+// and after every block A = -A_in^{-1}.  256 threads hold the (padded)
+// 128 x 128 matrix as a 16 x 16 grid of 8 x 8 FP64 register blocks.  Per
+// block: the owners publish column panel J to LDS; one wave sweeps the
+// 16 x 16 diagonal block in registers with cross-lane reads (no barrier); all
+// threads form W = P D^{-1} (128 x 16) and apply the rank-16 update with FP64
+// FMAs -- three barriers per block instead of one per pivot.  Orders that are
+// not multiples of 16 are padded with identity rows.  A non-positive pivot
+// inside a diagonal block is the Cholesky failure (a leading minor is not
+// positive definite) and sends the slice to the LU fallback below, as the
+// reference's solve(likely_sympd) does.
+constexpr int SPD_B = 16, SPD_T = 256, SPD_R = 8, SPD_PLD = SPD_B + 2;  // 16-byte rows
+
+// 1/d from the hardware reciprocal and two Newton steps (a few ulp; d is a
+// positive pivot here)
+__device__ __forceinline__ double spd_rcp(double d) {
+  double y = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-d, y, 1.0);
+  return fma(y, e, y);
+}
+
+// a double of lane `src` (a compile-time lane) in every lane
+__device__ __forceinline__ double spd_bcast(double v, int src) {
+  const unsigned long long u = __double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readlane((unsigned)(u & 0xffffffffu), src);
+  const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), src);
+  return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
+template <int KMAX>
+__global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
+                                                     SliceOrders ks, int mode,
+                                                     const double* __restrict__ dvals,
+                                                     double ridge, double* __restrict__ X,
+                                                     int64_t ldx, int64_t slice_stride,
+                                                     double* __restrict__ lambda_out,
+                                                     int32_t* __restrict__ status) {
+  static_assert(KMAX == 128, "k_spd_solve: 16 x 16 grid of 8 x 8 blocks");
+  __shared__ __align__(16) double L[KMAX * (KMAX + 1)];  // LU fallback; panels of the sweep
   __shared__ int piv[KMAX];
   __shared__ double shift;
   __shared__ int chol_ok;
@@ -48,11 +87,23 @@ __global__ __launch_bounds__(GS * GS) void k_spd_solve(const double* __restrict_
   const int k = ks.k[s];
   const int t = threadIdx.x, nt = blockDim.x;
   double* Xs = X + (int64_t)s * slice_stride;
+  // trace of A_k for the shift rules: one diagonal entry per thread, summed in
+  // a fixed order (a serial loop of dependent global loads in one thread
+  // costs about a microsecond per entry)
+  __shared__ double trpart[SPD_T / 64];
+  {
+    double dv = 0.0;
+    for (int i = t; i < k; i += nt) dv += Ain[i + (int64_t)i * lda];
+    dv = wave_sum_d(dv);
+    if ((t & 63) == 0) trpart[t >> 6] = dv;
+  }
+  __syncthreads();
   if (t == 0) {
     double sh = 0.0;
+    double trace = 0.0;
+    for (int w = 0; w < SPD_T / 64; ++w) trace += trpart[w];
     if (mode == 1) {
-      double tr = 0.0;
-      for (int i = 0; i < k; ++i) tr += Ain[i + (int64_t)i * lda];
+      const double tr = trace;
       double scale = tr / (double)(k > 1 ? k : 1);
       if (!isfinite(scale) || scale <= 0.0) scale = 1.0;
       double lam = ridge;
@@ -60,8 +111,7 @@ __global__ __launch_bounds__(GS * GS) void k_spd_solve(const double* __restrict_
       sh = lam * scale;
       if (lambda_out) lambda_out[s] = sh;
     } else if (mode == 2) {
-      double tr = 0.0;
-      for (int i = 0; i < k; ++i) tr += Ain[i + (int64_t)i * lda];
+      const double tr = trace;
       double r = 1e-10 * (tr / (double)k);
       if (!isfinite(r) || r <= 0.0) r = 1e-10;
       sh = r;
@@ -70,121 +120,155 @@ __global__ __launch_bounds__(GS * GS) void k_spd_solve(const double* __restrict_
     chol_ok = 1;
   }
   __syncthreads();
-  // ---- symmetric sweep (Gauss-Jordan without pivoting; Goodnight 1979).
-  // Thread (tr, tc) of a 32 x 32 grid holds the lower-triangle entries among
-  // (tr + 32a, tc + 32b), a, b < 4, in registers.  Sweeping pivot j:
-  //   d = A(j,j); A(j,j) = -1/d; A(i,j) = A(i,j)/d; A(i,c) -= A(i,j) A(c,j)/d,
-  // and after all k pivots A = -A_in^{-1}.  Column j of the symmetric matrix
-  // is read from a double-buffered LDS snapshot written by its owners (column
-  // j below the diagonal, row j left of it) during the previous pivot, so each
-  // pivot costs one barrier.  A non-positive pivot is the Cholesky failure (a
-  // leading minor is not positive definite) and sends the slice to the LU
-  // fallback, as the reference's solve(likely_sympd) does.
+  FPLS_SPD_STAMP(0);
   {
-    constexpr int G = GS, R = KMAX / GS;  // grid side, register blocks per side
-    static_assert(KMAX == 128 && G * R == KMAX && G * G >= 256, "k_spd_solve: grid shape");
-    __shared__ double snap[2][128];
-    const int tr = t / G, tc = t % G;
-    const bool active = t < G * G;
-    auto lower = [&](int a, int b) { return a > b || (a == b && tr >= tc); };
-    double v[R][R];
+    const int tr = t >> 4, tc = t & 15;
+    const int kp = (k + SPD_B - 1) / SPD_B * SPD_B;  // padded order
+    const bool live = tr * SPD_R < kp && tc * SPD_R < kp;
+    double a[SPD_R][SPD_R];
 #pragma unroll
-    for (int a = 0; a < R; ++a)
+    for (int r = 0; r < SPD_R; ++r)
 #pragma unroll
-      for (int b = 0; b < R; ++b) {
-        const int i = tr + G * a, c = tc + G * b;
-        double x = 0.0;
-        if (active && i < k && c < k && lower(a, b)) {
-          x = Ain[i + (int64_t)c * lda];
-          if (i == c) x += shift;
+      for (int c = 0; c < SPD_R; ++c) {
+        const int i = tr * SPD_R + r, j = tc * SPD_R + c;
+        double x = (i == j) ? 1.0 : 0.0;
+        if (i < k && j < k) {
+          x = Ain[i + (int64_t)j * lda];
+          if (i == j) x += shift;
         }
-        v[a][b] = x;
+        a[r][c] = x;
       }
-    // snapshot of column jn: owners of (i, jn), i >= jn, and of (jn, c), c < jn
-    auto publish = [&](int jn, double* dst) {
-      if (!active) return;
-      if (tc == jn % G) {
+    double* P = L;                         // column panel, 128 x SPD_PLD
+    double* W = L + KMAX * SPD_PLD;        // P D^{-1}, 128 x SPD_PLD
+    double* Dm = L + 2 * KMAX * SPD_PLD;   // swept diagonal block, 16 x SPD_PLD
+    FPLS_SPD_STAMP(1);
+    for (int J = 0; J < kp / SPD_B; ++J) {
+      const int j0 = J * SPD_B;
+      if ((tc >> 1) == J) {
+        const int q0 = (tc & 1) * SPD_R;
 #pragma unroll
-        for (int a = 0; a < R; ++a)
+        for (int r = 0; r < SPD_R; ++r)
 #pragma unroll
-          for (int b = 0; b < R; ++b) {
-            const int i = tr + G * a;
-            if (b == jn / G && lower(a, b) && i < k) dst[i] = v[a][b];
-          }
+          for (int c = 0; c < SPD_R; ++c) P[(tr * SPD_R + r) * SPD_PLD + q0 + c] = a[r][c];
       }
-      if (tr == jn % G) {
-#pragma unroll
-        for (int a = 0; a < R; ++a)
-#pragma unroll
-          for (int b = 0; b < R; ++b) {
-            const int c = tc + G * b;
-            if (a == jn / G && lower(a, b) && c < jn) dst[c] = v[a][b];
-          }
-      }
-    };
-    if (t < 256) (&snap[0][0])[t] = 0.0;  // 2 x 128 entries
-    __syncthreads();
-    publish(0, snap[0]);
-    __syncthreads();
-    for (int j = 0; j < k; ++j) {
-      const double* cur = snap[j & 1];
-      // all snapshot reads issued together (entries past k are never used
-      // for outputs, so no bounds test is needed on the 128-entry buffer)
-      const double d = cur[j];
-      double ns[R], sc[R];
-#pragma unroll
-      for (int a = 0; a < R; ++a) ns[a] = -cur[(tr + G * a) & 127];
-#pragma unroll
-      for (int b = 0; b < R; ++b) sc[b] = cur[(tc + G * b) & 127];
-      if (!(d > 0.0) || !isfinite(d)) {  // uniform: every thread reads the same pivot
-        if (t == 0) chol_ok = 0;
-        break;
-      }
-      const double rd = 1.0 / d;
-#pragma unroll
-      for (int b = 0; b < R; ++b) sc[b] *= rd;
-#pragma unroll
-      for (int a = 0; a < R; ++a)
-#pragma unroll
-        for (int b = 0; b < R; ++b)
-          if (lower(a, b)) v[a][b] = fma(ns[a], sc[b], v[a][b]);
-      // row j and column j of the swept matrix
-      if (tr == j % G) {
-#pragma unroll
-        for (int a = 0; a < R; ++a)
-#pragma unroll
-          for (int b = 0; b < R; ++b) {
-            const int c = tc + G * b;
-            if (a == j / G && lower(a, b) && c <= j) v[a][b] = (c == j) ? -rd : sc[b];
-          }
-      }
-      if (tc == j % G) {
-#pragma unroll
-        for (int a = 0; a < R; ++a)
-#pragma unroll
-          for (int b = 0; b < R; ++b) {
-            const int i = tr + G * a;
-            if (b == j / G && lower(a, b) && i > j) v[a][b] = -ns[a] * rd;
-          }
-      }
-      if (j + 1 < k) publish(j + 1, snap[(j + 1) & 1]);
       __syncthreads();
+      if (t < 64) {
+        // lane (i = t % 16, cg = t / 16) holds D[i][4 cg .. 4 cg + 3]; per pivot
+        // the pivot comes by readlane, the pivot row's entries from lane p of
+        // the same 16-lane segment and D[i][p] from lane i of segment p / 4
+        const int i = t & 15, cg = t >> 4;
+        double d[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) d[e] = P[(j0 + i) * SPD_PLD + 4 * cg + e];
+        bool ok = true;
+#pragma unroll
+        for (int p = 0; p < SPD_B; ++p) {
+          const double pv = spd_bcast(d[p & 3], p + 16 * (p >> 2));
+          double rowp[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) rowp[e] = __shfl(d[e], p, 16);
+          const double colp = __shfl(d[p & 3], i + 16 * (p >> 2), 64);
+          if (!(pv > 0.0) || !isfinite(pv)) {  // uniform
+            ok = false;
+            break;
+          }
+          const double rd = spd_rcp(pv);
+          const double f = -colp * rd;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int j = 4 * cg + e;
+            const double upd = fma(f, rowp[e], d[e]);
+            const double rowv = (j == p) ? -rd : rowp[e] * rd;
+            const double colv = colp * rd;
+            d[e] = (i == p) ? rowv : ((j == p) ? colv : upd);
+          }
+        }
+        if (!ok && t == 0) chol_ok = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) Dm[i * SPD_PLD + 4 * cg + e] = d[e];
+      }
+      __syncthreads();
+      FPLS_SPD_STAMP(2);
+      if (!chol_ok) break;
+      {
+        // W = P D^{-1} = -P Dm: thread -> row t / 2, 8 columns at 8 (t % 2)
+        const int i = t >> 1, q0 = (t & 1) * SPD_R;
+        double w[SPD_R];
+#pragma unroll
+        for (int c = 0; c < SPD_R; ++c) w[c] = 0.0;
+#pragma unroll
+        for (int q = 0; q < SPD_B; ++q) {
+          const double pv = P[i * SPD_PLD + q];
+#pragma unroll
+          for (int c = 0; c < SPD_R; ++c) w[c] = fma(pv, Dm[q * SPD_PLD + q0 + c], w[c]);
+        }
+#pragma unroll
+        for (int c = 0; c < SPD_R; ++c) W[i * SPD_PLD + q0 + c] = -w[c];
+      }
+      __syncthreads();
+      FPLS_SPD_STAMP(3);
+      if (live) {
+        const bool rowJ = (tr >> 1) == J, colJ = (tc >> 1) == J;
+        if (rowJ && colJ) {
+#pragma unroll
+          for (int r = 0; r < SPD_R; ++r)
+#pragma unroll
+            for (int c = 0; c < SPD_R; ++c)
+              a[r][c] = Dm[((tr & 1) * SPD_R + r) * SPD_PLD + (tc & 1) * SPD_R + c];
+        } else if (colJ) {
+#pragma unroll
+          for (int r = 0; r < SPD_R; ++r)
+#pragma unroll
+            for (int c = 0; c < SPD_R; ++c)
+              a[r][c] = W[(tr * SPD_R + r) * SPD_PLD + (tc & 1) * SPD_R + c];
+        } else if (rowJ) {
+#pragma unroll
+          for (int r = 0; r < SPD_R; ++r)
+#pragma unroll
+            for (int c = 0; c < SPD_R; ++c)
+              a[r][c] = W[(tc * SPD_R + c) * SPD_PLD + (tr & 1) * SPD_R + r];
+        } else {
+#pragma unroll 2
+          for (int q = 0; q < SPD_B; q += 2) {
+            double2 wv[SPD_R], pv[SPD_R];
+#pragma unroll
+            for (int r = 0; r < SPD_R; ++r)
+              wv[r] = *reinterpret_cast<const double2*>(W + (tr * SPD_R + r) * SPD_PLD + q);
+#pragma unroll
+            for (int c = 0; c < SPD_R; ++c)
+              pv[c] = *reinterpret_cast<const double2*>(P + (tc * SPD_R + c) * SPD_PLD + q);
+#pragma unroll
+            for (int r = 0; r < SPD_R; ++r)
+#pragma unroll
+              for (int c = 0; c < SPD_R; ++c)
+                a[r][c] = fma(-wv[r].y, pv[c].y, fma(-wv[r].x, pv[c].x, a[r][c]));
+          }
+        }
+      }
+      __syncthreads();  // the next panel overwrites P, W and Dm
+      FPLS_SPD_STAMP(4);
     }
-    __syncthreads();
     if (chol_ok) {
-      // X = A^{-1} diag(rhs) = -A_swept diag(rhs), both triangles from the lower
-      if (active) {
+      // X = A^{-1} diag(rhs) = -A_swept diag(rhs): both triangles from the
+      // lower one, staged in LDS so that the global writes are coalesced
+      if (live && tr >= tc) {
 #pragma unroll
-        for (int a = 0; a < R; ++a)
+        for (int r = 0; r < SPD_R; ++r)
 #pragma unroll
-          for (int b = 0; b < R; ++b) {
-            const int i = tr + G * a, c = tc + G * b;
-            if (!lower(a, b) || i >= k || c >= k) continue;
-            const double x = -v[a][b];
-            Xs[i + (int64_t)c * ldx] = (mode == 1) ? x : x * dvals[c];
-            if (i != c) Xs[c + (int64_t)i * ldx] = (mode == 1) ? x : x * dvals[i];
+          for (int c = 0; c < SPD_R; ++c) {
+            const int i = tr * SPD_R + r, j = tc * SPD_R + c;
+            if (i < j || i >= k || j >= k) continue;
+            L[i + j * LD] = -a[r][c];
+            L[j + i * LD] = -a[r][c];
           }
       }
+      __syncthreads();
+      for (int idx = t; idx < k * k; idx += nt) {
+        const int i = idx % k, j = idx / k;
+        const double x = L[i + j * LD];
+        Xs[i + (int64_t)j * ldx] = (mode == 1) ? x : x * dvals[j];
+      }
+      FPLS_SPD_STAMP(5);
       if (t == 0 && status) status[s] = 0;
       return;
     }
@@ -281,7 +365,7 @@ void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const st
     double* Xc = X + (int64_t)c0 * slice_stride;
     double* lc = lambda_out ? lambda_out + c0 : nullptr;
     int32_t* sc = status ? status + c0 : nullptr;
-    launch(ctx, "spd_solve", k_spd_solve<128, 32>, dim3(cnt), dim3(1024), 0, A, lda, so, mode,
+    launch(ctx, "spd_solve", k_spd_solve<128>, dim3(cnt), dim3(SPD_T), 0, A, lda, so, mode,
            dvals, ridge, Xc, ldx, slice_stride, lc, sc);
   }
 }
@@ -348,9 +432,18 @@ __global__ __launch_bounds__(1024) void k_chol_inv_lower(const double* __restric
   __shared__ double L[JMAX * JLD];
   __shared__ int ok;
   __shared__ double shift;
+  __shared__ double trpart[1024 / 64];
   const int t = threadIdx.x, nt = blockDim.x;
+  // trace: one diagonal entry per thread, fixed-order sum of the wave sums
+  {
+    double dv = 0.0;
+    for (int i = t; i < l; i += nt) dv += G[i + (int64_t)i * ldg];
+    dv = wave_sum_d(dv);
+    if ((t & 63) == 0) trpart[t >> 6] = dv;
+  }
+  __syncthreads();
   double tr = 0.0;
-  for (int i = 0; i < l; ++i) tr += G[i + (int64_t)i * ldg];
+  for (int w = 0; w < (nt >> 6); ++w) tr += trpart[w];
   const double u = DBL_EPSILON * 0.5;
   double boost = 11.0 * ((double)rows * l + (double)l * (l + 1)) * u * (tr > 0.0 ? tr : 1.0);
   if (shift0 < 0.0) shift0 = boost;  // shifted first pass of CholeskyQR3
@@ -392,15 +485,25 @@ __global__ __launch_bounds__(1024) void k_chol_inv_lower(const double* __restric
     __syncthreads();
   }
   if (t == 0 && status) status[0] = ok ? attempt : -1;
-  // forward substitution, one column of the identity per thread
-  for (int c = t; c < l; c += nt) {
-    double* x = Linv + (int64_t)c * l;
-    for (int i = 0; i < c; ++i) x[i] = 0.0;
-    for (int i = c; i < l; ++i) {
-      double v = (i == c) ? 1.0 : 0.0;
-      for (int q = c; q < i; ++q) v -= L[i + q * JLD] * x[q];
-      x[i] = v / L[i + i * JLD];
+  // L^{-1} in place, columns from the right: with columns > j already
+  // inverted, Linv(i, j) = -(sum_{j < q <= i} Linv(i, q) L(q, j)) / L(j, j)
+  // for i > j (one row per thread) and Linv(j, j) = 1 / L(j, j)
+  __syncthreads();
+  for (int j = l - 1; j >= 0; --j) {
+    const double rjj = 1.0 / L[j + j * JLD];
+    double v = 0.0;
+    const int i = j + 1 + t;
+    if (i < l) {
+      for (int q = j + 1; q <= i; ++q) v = fma(L[i + q * JLD], L[q + j * JLD], v);
     }
+    __syncthreads();  // every read of column j is done
+    if (i < l) L[i + j * JLD] = -v * rjj;
+    if (t == 0) L[j + j * JLD] = rjj;
+    __syncthreads();
+  }
+  for (int idx = t; idx < l * l; idx += nt) {
+    const int i = idx % l, c = idx / l;
+    Linv[i + (int64_t)c * l] = i >= c ? L[i + c * JLD] : 0.0;
   }
 }
 
