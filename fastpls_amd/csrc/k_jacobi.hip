@@ -24,6 +24,12 @@
 
 #include "fpls_internal.h"
 
+// Convergence hook (sweep, off-diagonal Frobenius^2, total Frobenius^2);
+// compiled out unless a probe build defines it.
+#ifndef FPLS_JAC_SWEEP
+#define FPLS_JAC_SWEEP(sweep, off2, tot2)
+#endif
+
 namespace fpls {
 
 namespace {
@@ -102,33 +108,36 @@ __global__ __launch_bounds__(256) void k_jac_cols(int n, const double* __restric
 // entry is positive.  Block Jacobi converges only when the block rotations
 // stay near the identity (sorting the pair's eigenvalues instead makes them
 // permutation-like and the sweeps stall; checked in a float64 model).
-__global__ __launch_bounds__(128) void k_jac_nearid(const int* __restrict__ ns,
+//
+// Fast path: when the largest-|entry| rows of the columns (ties -> lower row)
+// are pairwise distinct -- the usual case once the sweeps approach
+// convergence -- every column takes its own argmax row whatever the order,
+// which is exactly the greedy result; the greedy loop runs only on a
+// conflict.  Column maxima and argmaxes: one wave per column, 2 rows per lane.
+__global__ __launch_bounds__(256) void k_jac_nearid(const int* __restrict__ ns,
                                                     const double* __restrict__ Z,
                                                     double* __restrict__ Zo) {
+  __shared__ double Zs[JB_PAIR * JB_PAIR];  // the pair's eigenvectors (ld s)
   __shared__ double cmax[JB_PAIR];
-  __shared__ int order[JB_PAIR], taken[JB_PAIR], dest[JB_PAIR];
-  __shared__ double wv[2];
-  __shared__ int wi[2];
+  __shared__ int dest[JB_PAIR], amax[JB_PAIR], claim[JB_PAIR];
+  __shared__ int conflict;
   const int p = blockIdx.x, s = ns[p], t = threadIdx.x, lane = t & 63, w = t >> 6;
   const double* Zp = Z + (int64_t)p * JB_PAIR * JB_PAIR;
   double* Zq = Zo + (int64_t)p * JB_PAIR * JB_PAIR;
-  if (t < s) {
-    double m = 0.0;
-    for (int i = 0; i < s; ++i) m = fmax(m, fabs(Zp[i + t * s]));
-    cmax[t] = m;
-    taken[t] = 0;
-  }
+  for (int e = t; e < s * s; e += 256) Zs[e] = Zp[e];
+  if (t < JB_PAIR) claim[t] = 0;
+  if (t == 0) conflict = 0;
   __syncthreads();
-  if (t < s) {
-    int r = 0;
-    for (int j = 0; j < s; ++j) r += (cmax[j] > cmax[t] || (cmax[j] == cmax[t] && j < t)) ? 1 : 0;
-    order[r] = t;
-  }
-  __syncthreads();
-  for (int step = 0; step < s; ++step) {
-    const int j = order[step];
-    double v = (t < s && !taken[t]) ? fabs(Zp[t + j * s]) : -1.0;
-    int ix = t;
+  for (int j = w; j < s; j += 4) {
+    double v = -1.0;
+    int ix = 0;
+    for (int i = lane; i < s; i += 64) {
+      const double a = fabs(Zs[i + j * s]);
+      if (a > v) {  // rows visited in increasing order: ties keep the lower row
+        v = a;
+        ix = i;
+      }
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       const double ov = __shfl_down(v, o, 64);
@@ -139,21 +148,92 @@ __global__ __launch_bounds__(128) void k_jac_nearid(const int* __restrict__ ns,
       }
     }
     if (lane == 0) {
-      wv[w] = v;
-      wi[w] = ix;
+      cmax[j] = v;
+      amax[j] = ix;
     }
-    __syncthreads();
-    if (t == 0) {
-      const int best = (wv[1] > wv[0] || (wv[1] == wv[0] && wi[1] < wi[0])) ? wi[1] : wi[0];
-      taken[best] = 1;
-      dest[j] = best;
-    }
-    __syncthreads();
   }
-  for (int e = t; e < s * s; e += 128) {
+  __syncthreads();
+  if (t < s && atomicAdd(&claim[amax[t]], 1) != 0) conflict = 1;
+  __syncthreads();
+  // columns whose argmax row no other column wants keep it
+  if (t < s) {
+    const bool mine = claim[amax[t]] == 1;
+    dest[t] = mine ? amax[t] : -1;
+    if (!mine) conflict = 1;
+  }
+  __syncthreads();
+  if (conflict && w == 0) {
+    // the other columns, greedily in one wave: by decreasing maximum (ties ->
+    // lower column), each taking its largest row not yet held; lane owns rows
+    // lane and lane + 64, their held flags in registers
+    const int r0 = lane, r1 = lane + 64;
+    bool tk0 = r0 < s && claim[r0] == 1, tk1 = r1 < s && claim[r1] == 1;
+    int todo = 0;
+    for (int j = lane; j < s; j += 64) {
+      if (dest[j] >= 0) cmax[j] = -3.0;  // placed
+      else ++todo;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) todo += __shfl_xor(todo, o, 64);
+    int done = 0;
+    while (done < todo) {
+      // next column in the order: the largest cmax among columns not yet placed
+      double cv = -2.0;
+      int cj = JB_PAIR;
+      for (int j = lane; j < s; j += 64) {
+        const double c = cmax[j];
+        if (c > cv || (c == cv && j < cj)) {
+          cv = c;
+          cj = j;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(cv, o, 64);
+        const int oj = __shfl_xor(cj, o, 64);
+        if (ov > cv || (ov == cv && oj < cj)) {
+          cv = ov;
+          cj = oj;
+        }
+      }
+      const int j = cj;
+      double v = -1.0;
+      int ix = JB_PAIR;
+      if (r0 < s && !tk0) {
+        v = fabs(Zs[r0 + j * s]);
+        ix = r0;
+      }
+      if (r1 < s && !tk1) {
+        const double a = fabs(Zs[r1 + j * s]);
+        if (a > v) {
+          v = a;
+          ix = r1;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(v, o, 64);
+        const int oi = __shfl_xor(ix, o, 64);
+        if (ov > v || (ov == v && oi < ix)) {
+          v = ov;
+          ix = oi;
+        }
+      }
+      if (ix == r0) tk0 = true;
+      if (ix == r1) tk1 = true;
+      if (lane == 0) {
+        dest[j] = ix;
+        cmax[j] = -3.0;  // placed
+      }
+      __builtin_amdgcn_wave_barrier();
+      ++done;
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < s * s; e += 256) {
     const int i = e % s, j = e / s, d = dest[j];
-    const double sg = Zp[d + j * s] >= 0.0 ? 1.0 : -1.0;
-    Zq[i + d * s] = sg * Zp[i + j * s];
+    const double sg = Zs[d + j * s] >= 0.0 ? 1.0 : -1.0;
+    Zq[i + d * s] = sg * Zs[i + j * s];
   }
 }
 
@@ -270,7 +350,7 @@ void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, dou
              (const double*)Aw.get(), (int64_t)n, ix, nsr, M.get());
       sym_eig_batched(ctx, M.get(), JB_PAIR, (int64_t)JB_PAIR * JB_PAIR, nsr, JB_PAIR, q,
                       wb.get(), JB_PAIR, Z.get(), (int64_t)JB_PAIR * JB_PAIR);
-      launch(ctx, "jacobi_nearid", k_jac_nearid, dim3((unsigned)q), dim3(128), 0, nsr,
+      launch(ctx, "jacobi_nearid", k_jac_nearid, dim3((unsigned)q), dim3(256), 0, nsr,
              (const double*)Z.get(), Zn.get());
       // B = A J; A <- B^T J; V <- V J
       launch(ctx, "jacobi_cols", k_jac_cols, cgrid, dim3(256), 0, n, (const double*)Aw.get(),
@@ -287,6 +367,7 @@ void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, dou
     double h[2];
     to_host(ctx, h, norms.get(), 2);
     FPLS_REQUIRE(std::isfinite(h[1]), FPLS_ERR_ARG, "sym_eig_large: non-finite input");
+    FPLS_JAC_SWEEP(sweep, h[0], h[1]);
     if (h[0] <= 1e-28 * h[1]) break;
   }
   // eigenvalues descending (stable on ties by index)
