@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "model.h"
+#include "fpls_wave.h"
 
 namespace fpls {
 
@@ -58,15 +59,20 @@ __global__ void k_scale_all(double* X, int64_t n, double f) {
 }
 
 // constants[c] = -1/2 means_c . linear_c + log(max(count_c / n, DBL_MIN))
+// one wave per class: q = sum_j means[c, j] linear[c, j] (fixed-order wave sum)
 __global__ void k_lda_constants(const double* means, int64_t ldm, const double* linear,
                                 int64_t ldl, int64_t C, int64_t k, const double* counts, double n,
                                 double* constants) {
-  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t c = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (c >= C) return;
   double q = 0.0;
-  for (int64_t j = 0; j < k; ++j) q += means[c + j * ldm] * linear[c + j * ldl];
-  const double prior = fmax(counts[c] / n, DBL_MIN);
-  constants[c] = -0.5 * q + log(prior);
+  for (int64_t j = lane; j < k; j += 64) q = fma(means[c + j * ldm], linear[c + j * ldl], q);
+  q = wave_sum_d(q);
+  if (lane == 0) {
+    const double prior = fmax(counts[c] / n, DBL_MIN);
+    constants[c] = -0.5 * q + log(prior);
+  }
 }
 
 // labels[i] = 1 + argmax_c (scores[i,c] + cst[c]); first maximum wins.
@@ -77,7 +83,20 @@ __global__ void k_argmax_rows(const double* __restrict__ scores, int64_t lds, in
   if (i >= n) return;
   int best = 0;
   double bv = scores[i] + cst[0];
-  for (int64_t c = 1; c < C; ++c) {
+  int64_t c = 1;
+  // eight loads in flight, then the comparisons in column order (first max)
+  for (; c + 8 <= C; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = scores[i + (c + u) * lds] + cst[c + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (v[u] > bv) {
+        bv = v[u];
+        best = (int)(c + u);
+      }
+  }
+  for (; c < C; ++c) {
     const double v = scores[i + c * lds] + cst[c];
     if (v > bv) {
       bv = v;
@@ -186,7 +205,7 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
     gemm_f64(ctx, false, false, C, k, k, 1.0, sl.means.get(), C, nullptr, sl.inv_cov.get(), k,
              nullptr, 0.0, sl.linear.get(), C);
     sl.constants = DBuf<double>(ctx, C);
-    launch(ctx, "lda_constants", k_lda_constants, dim3((C + 255) / 256), dim3(256), 0,
+    launch(ctx, "lda_constants", k_lda_constants, dim3((C + 3) / 4), dim3(256), 0,
            sl.means.get(), (int64_t)C, sl.linear.get(), (int64_t)C, (int64_t)C, (int64_t)k,
            counts_out.get(), (double)n, sl.constants.get());
     slices.push_back(std::move(sl));
