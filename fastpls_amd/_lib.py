@@ -116,6 +116,10 @@ SIGNATURES = {
                                              ctypes.c_uint32, _P(Mat), _vp]),
     "fpls_gaussian_y_decode": (ctypes.c_int, [_vp, _P(Mat), _vp, ctypes.c_int32, _vp, _P(Mat)]),
     "fpls_gaussian_y_classes": (ctypes.c_int, [_vp, _P(Mat), _vp, ctypes.c_int32, _vp, _vp]),
+    "fpls_class_predict_topk": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, ctypes.c_int32,
+                                               ctypes.c_int32, _vp, _vp]),
+    "fpls_class_topk_scores": (ctypes.c_int, [_vp, _P(Mat), ctypes.c_int32, ctypes.c_int32, _vp,
+                                              ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "fpls_cv_predict": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp,
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Opts),
                                        ctypes.c_int32, ctypes.c_double, ctypes.c_int32,

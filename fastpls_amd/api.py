@@ -201,9 +201,128 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     return model
 
 
-def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
-    """predict.fastPLS (R/main.R:2577-2762) for a HIP model."""
+def resolve_top_k(top=1, top5: bool = False) -> int:
+    """.resolve_top_k (R/main.R:368-375)."""
+    t = np.atleast_1d(top)[0] if top is not None else None
+    if t is None or not np.isfinite(float(t)) or int(t) < 1:
+        raise ValueError("top must be a positive integer")
+    t = int(t)
+    return max(t, 5) if top5 else t
+
+
+def class_bias_matrix(class_bias, lev, n_slices: int) -> np.ndarray:
+    """.class_bias_matrix (R/main.R:377-415): an m x n_slices FP64 offset
+    matrix from None (zeros), a per-class vector (a dict is matched to the
+    levels by name) or a matrix with one column or one column per slice
+    (transposed when it is n_slices x m)."""
+    C = len(lev)
+    if class_bias is None:
+        return np.zeros((C, n_slices), order="F")
+    if isinstance(class_bias, dict):
+        try:
+            vals = [float(class_bias[l]) for l in lev]
+        except KeyError:
+            raise ValueError("class_bias must have one numeric value per class") from None
+        class_bias = np.asarray(vals)
+    b = np.asarray(class_bias, dtype=np.float64)
+    if b.ndim <= 1:
+        b = b.reshape(-1)
+        if b.size != C or np.isnan(b).any():
+            raise ValueError("class_bias must have one numeric value per class")
+        return np.asfortranarray(np.repeat(b[:, None], n_slices, axis=1))
+    if b.shape[0] != C and b.shape[1] == C:
+        b = b.T
+    if b.shape[0] != C or np.isnan(b).any():
+        raise ValueError("class_bias must have one row per class")
+    if b.shape[1] == 1 and n_slices > 1:
+        b = np.repeat(b, n_slices, axis=1)
+    if b.shape[1] != n_slices:
+        raise ValueError("class_bias must have one column or one column per ncomp")
+    return np.asfortranarray(b)
+
+
+def _topk_result(ti: np.ndarray, ts: np.ndarray, lev) -> Dict:
+    """.class_topk_to_labels (R/main.R:417-451): labels of rank 1 per slice,
+    their indices, and for top > 1 the ranked index and score cubes."""
+    labs = np.asarray(lev, dtype=object)
+    res = {"Ypred_index": np.ascontiguousarray(ti[:, 0, :]),
+           "Ypred": labs[ti[:, 0, :] - 1]}
+    if ti.shape[1] > 1:
+        res["Ypred_top"] = ti
+        res["Ypred_top_score"] = ts
+        res["Ypred_top_label"] = np.where(ti > 0, labs[np.maximum(ti, 1) - 1], None)
+    return res
+
+
+def class_predict_topk(model: FastPLS, newdata, class_bias=None, top: int = 1) -> Dict:
+    """.class_bias_predict (R/main.R:476-492) -> pls_class_predict_topk
+    (src/fastPLS.cpp:3954-4113): the argmax head's ranked classes with
+    per-class offsets, computed on the device from the compact factors."""
     ctx = model.ctx
+    keep: list = []
+    X = as_mat(newdata, keep)
+    inf = model.info()
+    L, m = inf.n_ncomp, inf.m
+    lev = model.get("lev") or [str(i + 1) for i in range(m)]
+    bias = class_bias_matrix(class_bias, lev, L)
+    k = max(1, min(int(top), m))
+    ti = np.empty((X.rows, k, L), dtype=np.int32, order="F")
+    ts = np.empty((X.rows, k, L), dtype=np.float64, order="F")
+    ctx.check(ctx.lib.fpls_class_predict_topk(ctx.h, model.handle, ctypes.byref(X),
+                                              bias.ctypes.data, L, int(top), ti.ctypes.data,
+                                              ts.ctypes.data))
+    return _topk_result(ti, ts, lev)
+
+
+def class_topk_from_score_cube(score_cube, lev, class_bias=None, top: int = 1,
+                               device: int = 0) -> Dict:
+    """.class_topk_from_score_cube (R/main.R:453-474) on the device
+    (fpls_class_topk_scores): score_cube is w x C x L (host array, or a device
+    FP32/FP64 tensor of that shape)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    if hasattr(score_cube, "is_cuda"):
+        w, C = int(score_cube.shape[0]), int(score_cube.shape[1])
+        L = int(score_cube.shape[2]) if score_cube.dim() == 3 else 1
+        flat = score_cube.reshape(w, C, L).permute(2, 1, 0).contiguous().reshape(L * C, w).t()
+        S = as_mat(flat, keep)
+    else:
+        cube = np.asarray(score_cube, dtype=np.float64)
+        if cube.ndim == 2:
+            cube = cube[:, :, None]
+        w, C, L = cube.shape
+        S = as_mat(np.asfortranarray(cube).reshape(w, C * L, order="F"), keep)
+    bias = class_bias_matrix(class_bias, lev, L)
+    k = max(1, min(int(top), C))
+    ti = np.empty((w, k, L), dtype=np.int32, order="F")
+    ts = np.empty((w, k, L), dtype=np.float64, order="F")
+    ctx.check(ctx.lib.fpls_class_topk_scores(ctx.h, ctypes.byref(S), C, L, bias.ctypes.data, L,
+                                             int(top), ti.ctypes.data, ts.ctypes.data))
+    return _topk_result(ti, ts, lev)
+
+
+def _q2_classes(Ytest, lev, cube: np.ndarray) -> np.ndarray:
+    """Q2Y of a classification prediction (R/main.R:2694-2719): the one-hot
+    Ytest against each slice of the score cube, RQ (src/fastPLS.cpp:1483)."""
+    yt = np.asarray(Ytest).astype(str)
+    onehot = (yt[:, None] == np.asarray(lev, dtype=str)[None, :]).astype(np.float64)
+    tss = float(((onehot - onehot.mean(axis=0)) ** 2).sum())
+    return np.array([1.0 - float(((onehot - cube[:, :, a]) ** 2).sum()) / tss
+                     for a in range(cube.shape[2])])
+
+
+def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 1,
+            top5: bool = False, raw_scores: bool = False) -> Dict:
+    """predict.fastPLS (R/main.R:2577-2762) for a HIP model.
+
+    Classification results follow .class_topk_to_labels: ``Ypred`` (labels,
+    w x L), ``Ypred_index`` (1-based), and for top > 1 ``Ypred_top`` (ranked
+    class indices, w x top x L), ``Ypred_top_score`` and ``Ypred_top_label``.
+    The argmax head without Ytest / raw_scores ranks on the device straight
+    from the compact factors (fpls_class_predict_topk); otherwise the score
+    cube is formed and ranked (fpls_class_topk_scores)."""
+    ctx = model.ctx
+    top = resolve_top_k(top, top5)
     keep: list = []
     X = as_mat(newdata, keep)
     w = X.rows
@@ -212,24 +331,46 @@ def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
     lev = model.get("lev")
     rule = model.get("classification_rule", "argmax")
     res: Dict = {}
+    if proj:
+        T = np.empty((w, inf.kmax), dtype=np.float64, order="F")
+        ctx.check(ctx.lib.fpls_project(ctx.h, model.handle, ctypes.byref(X), inf.kmax,
+                                       T.ctypes.data))
+        res["Ttest"] = T
     if model.get("classification") and rule == "lda_hip":
+        C = len(lev)
         labels = np.empty((w, L), dtype=np.int32, order="F")
+        want = raw_scores or top > 1
+        sc = np.empty((w, C, L), dtype=np.float64, order="F") if want else None
         ctx.check(ctx.lib.fpls_lda_predict(ctx.h, model.handle, ctypes.byref(X),
-                                           labels.ctypes.data, None))
+                                           labels.ctypes.data,
+                                           sc.ctypes.data if want else None))
         res["Ypred_index"] = labels
         res["Ypred"] = np.asarray(lev, dtype=object)[labels - 1]
+        if raw_scores:
+            res["LDA_scores"] = sc
+        if top > 1:
+            res.update(class_topk_from_score_cube(sc, lev, None, top, ctx.device))
+        if Ytest is not None:
+            res["Q2Y"] = _q2_classes(Ytest, lev, np.stack(
+                [(res["Ypred_index"][:, a][:, None] == np.arange(1, C + 1)[None, :])
+                 .astype(np.float64) for a in range(L)], axis=2))
         return res
     if model.get("classification") and rule == "candidate_knn_hip":
-        top = max(1, min(int(top), len(lev)))
-        ti = np.empty((w, top, L), dtype=np.int32, order="F")
-        ts = np.empty((w, top, L), dtype=np.float64, order="F")
-        ctx.check(ctx.lib.fpls_cknn_predict(ctx.h, model.handle, ctypes.byref(X), top,
+        k = max(1, min(int(top), len(lev)))
+        ti = np.empty((w, k, L), dtype=np.int32, order="F")
+        ts = np.empty((w, k, L), dtype=np.float64, order="F")
+        ctx.check(ctx.lib.fpls_cknn_predict(ctx.h, model.handle, ctypes.byref(X), k,
                                             ti.ctypes.data, ts.ctypes.data))
-        res["Ypred_index"] = ti[:, 0, :]
-        res["Ypred"] = np.asarray(lev, dtype=object)[ti[:, 0, :] - 1]
-        if top > 1:
-            res["Ypred_top"] = ti
-            res["Ypred_top_score"] = ts
+        res.update(_topk_result(ti, ts, lev))
+        if Ytest is not None:
+            C = len(lev)
+            res["Q2Y"] = _q2_classes(Ytest, lev, np.stack(
+                [(res["Ypred_index"][:, a][:, None] == np.arange(1, C + 1)[None, :])
+                 .astype(np.float64) for a in range(L)], axis=2))
+        return res
+    if (model.get("classification") and Ytest is None and not raw_scores
+            and not model.get("gaussian_y")):
+        res.update(class_predict_topk(model, newdata, None, top))
         return res
     m = inf.m
     yp = np.empty((w, m, L), dtype=np.float64, order="F")
@@ -238,10 +379,11 @@ def predict(model: FastPLS, newdata, Ytest=None, top: int = 1) -> Dict:
         # .decode_gaussian_y_cube (R/main.R:2099-2118) after the prediction
         yp = decode_gaussian_y_cube(model["gaussian_y"], yp, ctx.device)
     if model.get("classification"):
-        idx = (np.argmax(yp, axis=1) + 1).astype(np.int32)  # first maximum
-        res["Ypred_index"] = idx
-        res["Ypred"] = np.asarray(lev, dtype=object)[idx - 1]
-        res["Yscore"] = yp
+        if Ytest is not None:
+            res["Q2Y"] = _q2_classes(Ytest, lev, yp)
+        if raw_scores:
+            res["Yscore"] = yp
+        res.update(class_topk_from_score_cube(yp, lev, None, top, ctx.device))
     else:
         res["Ypred"] = yp
         if Ytest is not None:

@@ -45,6 +45,11 @@
  *   fpls_gaussian_y_codes     .gaussian_y_class_codes          R/main.R:1982-1995
  *   fpls_gaussian_y_decode    .decode_gaussian_y_matrix (regr.) R/main.R:2077-2096
  *   fpls_gaussian_y_classes   nearest_code_classes       src/fastPLS.cpp:4203-4215
+ *   fpls_class_predict_topk   pls_class_predict_topk_cpp src/fastPLS.cpp:3954-4113
+ *                             (compact_prediction_weights :3737, class_bias_offsets
+ *                             :3838, fill_topk_from_yblock :3863)
+ *   fpls_class_topk_scores    class_bias_topk_from_cube  src/fastPLS.cpp:3908-3952,
+ *                             .class_topk_from_score_cube R/main.R:453-474
  */
 #ifndef FASTPLS_HIP_H
 #define FASTPLS_HIP_H
@@ -227,6 +232,25 @@ int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_norm,
                              const fpls_mat* centroids, const double* candidate_bias,
                              int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
                              double alpha, int32_t* top_index, double* top_score);
+
+/* ---- argmax-head class ranking ------------------------------------------------ */
+/* Top-k classes of the argmax head with per-class offsets: value(i, j) =
+   (Xs R[:, :mc] W_a)(i, j) + mY[j] + class_bias[j, a], ranked per row in the
+   order of a stable descending sort (larger value first, ties to the smaller
+   class index; NaN and -inf never ranked, empty slots hold class 0 and score
+   -inf).  k = max(1, min(top_k, m)); top_index (w x k x n_ncomp, 1-based
+   class) and top_score (same, may be NULL) are host buffers, column-major.
+   class_bias: host FP64 m x bias_cols (column-major), bias_cols 0 (NULL,
+   zero offsets), 1 (broadcast over the slices) or n_ncomp. */
+int fpls_class_predict_topk(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                            const double* class_bias, int32_t bias_cols, int32_t top_k,
+                            int32_t* top_index, double* top_score);
+/* The same ranking over a given score cube: scores is w x (m * n_slices)
+   (element (i, j, a) at column j + m a), host or device, FP32 or FP64;
+   offsets are class_bias alone. */
+int fpls_class_topk_scores(fpls_ctx* ctx, const fpls_mat* scores, int32_t m, int32_t n_slices,
+                           const double* class_bias, int32_t bias_cols, int32_t top_k,
+                           int32_t* top_index, double* top_score);
 
 /* ---- truncated SVD ---------------------------------------------------------- */
 /* rSVD (or exact, per opts->svd_method / the reference's dispatch rules) of A

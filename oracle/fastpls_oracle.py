@@ -1012,3 +1012,141 @@ def pls_plssvd_lda(Xtrain, ytrain, Xtest, ncomp, lda_ridge=1e-8, scaling=1, seed
                                                 model["lda"][kk]))
     model["Ypred_labels"] = np.stack(preds, axis=1)
     return model
+
+
+# ---------------------------------------------------------------------------
+# Argmax-head class ranking with per-class offsets
+# (src/fastPLS.cpp:3737-4113; R/main.R:377-492)
+# ---------------------------------------------------------------------------
+
+def compact_prediction_weights(model: Dict) -> np.ndarray:
+    """compact_prediction_weights (src/fastPLS.cpp:3737-3836): Wflash
+    (kmax x m x L); slice a holds Q[:, :mc]^T (SIMPLS) or W_latent[:, :, a]
+    (PLSSVD) in its first mc rows."""
+    R = model["R"]
+    kmax = R.shape[1]
+    m = int(model["m"])
+    ncomp = np.asarray(model["ncomp"])
+    W = np.zeros((kmax, m, ncomp.size))
+    for a in range(ncomp.size):
+        mc = int(ncomp[a])
+        if mc < 1 or mc > kmax:
+            raise ValueError("ncomp exceeds latent rank for compact class prediction")
+        if model["pls_method"] == "simpls":
+            W[:mc, :, a] = model["Q"][:, :mc].T
+        else:
+            W[:, :, a] = model["W_latent"][:, :, a]
+    return W
+
+
+def class_bias_offsets(mY, class_bias, L: int) -> np.ndarray:
+    """class_bias_offsets (src/fastPLS.cpp:3838-3861): m x L, mY in every
+    column plus class_bias (empty, one column broadcast, or one per slice)."""
+    mY = np.asarray(mY, dtype=np.float64).reshape(-1)
+    m = mY.size
+    off = np.repeat(mY[:, None], L, axis=1)
+    if class_bias is None or np.size(class_bias) == 0:
+        return off
+    b = np.asarray(class_bias, dtype=np.float64)
+    if b.ndim == 1:
+        b = b[:, None]
+    if b.shape[0] != m:
+        raise ValueError("class_bias must have one row per class")
+    if b.shape[1] == 1:
+        return off + b[:, :1]
+    if b.shape[1] == L:
+        return off + b
+    raise ValueError("class_bias must have one column or one column per ncomp")
+
+
+def fill_topk_from_yblock(yblock, offset, top_k: int):
+    """fill_topk_from_yblock (src/fastPLS.cpp:3863-3906), loop for loop: per
+    row, value = yblock[i, j] + offset[j] inserted with a strict '>' into a
+    list initialised to (-inf, class 0).  Returns (index (rows x k, 1-based),
+    score (rows x k)) with k = max(1, min(top_k, m))."""
+    yblock = np.asarray(yblock, dtype=np.float64)
+    offset = np.asarray(offset, dtype=np.float64).reshape(-1)
+    n, m = yblock.shape
+    k = max(1, min(int(top_k), m))
+    idx = np.zeros((n, k), dtype=np.int32)
+    sc = np.full((n, k), -np.inf)
+    for i in range(n):
+        bs = [-math.inf] * k
+        bi = [0] * k
+        row = yblock[i]
+        for j in range(m):
+            v = float(row[j]) + float(offset[j])
+            for r in range(k):
+                if v > bs[r]:
+                    bs[r + 1:] = bs[r:k - 1]
+                    bi[r + 1:] = bi[r:k - 1]
+                    bs[r] = v
+                    bi[r] = j + 1
+                    break
+        idx[i] = bi
+        sc[i] = bs
+    return idx, sc
+
+
+def class_bias_topk_from_cube(Ypred, class_bias, top_k: int):
+    """class_bias_topk_from_cube (src/fastPLS.cpp:3908-3952): ranks of a given
+    score cube (w x m x L) with class_bias alone as the offsets.  Returns
+    top_index / top_score cubes (w x k x L)."""
+    Y = np.asarray(Ypred, dtype=np.float64)
+    if Y.ndim == 2:
+        Y = Y[:, :, None]
+    w, m, L = Y.shape
+    off = class_bias_offsets(np.zeros(m), class_bias, L)
+    k = max(1, min(int(top_k), m))
+    ti = np.zeros((w, k, L), dtype=np.int32)
+    ts = np.zeros((w, k, L))
+    for a in range(L):
+        ti[:, :, a], ts[:, :, a] = fill_topk_from_yblock(Y[:, :, a], off[:, a], k)
+    return ti, ts
+
+
+def pls_class_predict_topk(model: Dict, Xtest, class_bias=None, top_k: int = 1,
+                           block_size: int = 4096):
+    """pls_class_predict_topk_cpp (src/fastPLS.cpp:3954-4113), compact path:
+    scores = ((Xtest - mX)/vX) R per row block, yblock = scores[:, :mc] Wflash_a
+    and fill_topk_from_yblock with offsets mY + class_bias."""
+    X = (np.asarray(Xtest, dtype=np.float64) - model["mX"]) / model["vX"]
+    ncomp = np.asarray(model["ncomp"])
+    L = ncomp.size
+    m = int(model["m"])
+    k = max(1, min(int(top_k), m))
+    off = class_bias_offsets(model["mY"], class_bias, L)
+    W = compact_prediction_weights(model)
+    R = model["R"]
+    w = X.shape[0]
+    ti = np.zeros((w, k, L), dtype=np.int32)
+    ts = np.zeros((w, k, L))
+    bs = max(1, min(int(block_size), max(w, 1)))
+    for s in range(0, w, bs):
+        sc = X[s:s + bs] @ R
+        for a in range(L):
+            mc = int(ncomp[a])
+            yb = sc[:, :mc] @ W[:mc, :, a]
+            ti[s:s + bs, :, a], ts[s:s + bs, :, a] = fill_topk_from_yblock(yb, off[:, a], k)
+    return ti, ts
+
+
+def class_topk_from_score_cube(score_cube, class_bias=None, top: int = 1):
+    """.class_topk_from_score_cube (R/main.R:453-474): score + bias per slice;
+    top == 1 takes max.col(ties.method = "first"), top > 1 the first `top` of
+    order(decreasing = TRUE), R's stable order (ties by class index).
+    Returns (top_index, top_score) cubes (w x top x L)."""
+    Y = np.asarray(score_cube, dtype=np.float64)
+    if Y.ndim == 2:
+        Y = Y[:, :, None]
+    w, m, L = Y.shape
+    top = min(int(top), m)
+    off = class_bias_offsets(np.zeros(m), class_bias, L)
+    ti = np.zeros((w, top, L), dtype=np.int32)
+    ts = np.zeros((w, top, L))
+    for a in range(L):
+        S = Y[:, :, a] + off[:, a]
+        order = np.argsort(-S, axis=1, kind="stable")[:, :top]
+        ti[:, :, a] = order + 1
+        ts[:, :, a] = np.take_along_axis(S, order, axis=1)
+    return ti, ts
