@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 
 N_TRAIN, N_TEST, P, C, NCOMP = 50_000, 10_000, 768, 100, 99
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp64": 78.6}
 METRIC = "PLS fits/s at fixed FP64-oracle error, chosen vs plain HIP; % of HBM/MFMA peak"
 SIN_THETA_BOUND = 1e-4
 
@@ -673,6 +673,136 @@ def main_cknn(args):
     print(json.dumps(out))
 
 
+# Argmax-head class ranking on the same large-class-count data: the label-aware
+# PLSSVD model (ncomp 500, 1000 classes) fitted once, then one step ranks the
+# 50,000 test rows (top-5 with per-class offsets mY, no class bias).
+TK_SAMPLE = 30000
+
+
+def main_topk(args):
+    """predict(top = 5) of the argmax head (pls_class_predict_topk) on the
+    C = 1000 workload; the fit is outside the timed region."""
+    import torch
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    Xtr, ytr, Xte, yte = synth.zipf_classes_large()
+    dX = F.device_colmajor(Xtr)
+    dXt = F.device_colmajor(Xte)
+    dy = torch.from_numpy(ytr).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = F.context(0)
+    m = F.pls(dX, dy, method="plssvd", ncomp=C3_K, label_aware=True, n_classes=C3_C,
+              fetch=False)
+    del dX
+    torch.cuda.empty_cache()
+    names = ("project_f32", "project_bf16x3", "gemm_nn_f32", "gemm_nn_reduce", "gemm_f64",
+             "gemm_f64_reduce", "class_topk", "class_topk_rank", "class_topk_offsets")
+
+    # one step: the C-ABI call an R binding makes (outputs into host buffers
+    # allocated once); label decoration on the host is not part of it
+    import ctypes
+    from fastpls_amd._lib import as_mat
+    keep = []
+    Xm = as_mat(dXt, keep)
+    n_test = Xte.shape[0]
+    ti = np.empty((n_test, C3_TOP, 1), dtype=np.int32, order="F")
+    ts = np.empty((n_test, C3_TOP, 1), dtype=np.float64, order="F")
+
+    def step():
+        ctx.check(ctx.lib.fpls_class_predict_topk(ctx.h, m.handle, ctypes.byref(Xm), None, 0,
+                                                  C3_TOP, ti.ctypes.data, ts.ctypes.data))
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    r = {"Ypred_top": ti, "Ypred_top_score": ts}
+    ctx.timing(True)
+    step()
+    kern = {}
+    for nm in names:
+        c_, ms = ctx.kernel_time(nm)
+        if c_:
+            kern[nm] = {"launches_per_step": c_, "ms_per_step": ms}
+    ctx.timing(False)
+    dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    kt = kern[dom]
+    avg_s = kt["ms_per_step"] * 1e-3 / kt["launches_per_step"]
+    if dom.startswith("gemm_f64"):
+        flops = 2.0 * n_test * C3_K * C3_C / kt["launches_per_step"]
+        achieved = flops / avg_s / 1e12
+        roof = {"bound": "fp64", "kernel": dom, "achieved": achieved, "peak": PEAK_TFLOPS["fp64"],
+                "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"]}
+    elif dom.startswith("project"):
+        flops = 2.0 * n_test * Xte.shape[1] * C3_K / kt["launches_per_step"]
+        achieved = flops / avg_s / 1e12
+        roof = {"bound": "mfma", "kernel": dom, "achieved": achieved,
+                "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK_TFLOPS["fp32"]}
+    else:
+        byts = (8.0 * n_test * C3_C + 12.0 * n_test * C3_TOP) / kt["launches_per_step"]
+        achieved = byts / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS}
+    roof.update({"traffic": measured_traffic(dom + "_topk", kt["launches_per_step"]),
+                 "avg_launch_us": avg_s * 1e6, "launches_per_step": kt["launches_per_step"]})
+    errors, cpu = {}, None
+    if not args.no_cpu_baseline:
+        errors, cpu = topk_oracle_check(m, Xte, r)
+    out = {
+        "metric": "argmax-head top-5 ranked test rows/s", "value": n_test / dt, "unit": "rows/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded; Zipf(1.1) class sizes, means on the radius-4 sphere, "
+                "N(0, 0.6^2) noise)",
+        "config": {"workload": "class-bias top-5 prediction ntest=50000 p=1024 C=1000 ncomp=500 "
+                               "(label-aware plssvd model, FP32 inputs)",
+                   "inputs": "resident in HBM", "outputs": "host (top index + score)"},
+        "accuracy": float(np.mean(r["Ypred_top"][:, 0, 0] == yte)),
+        "roofline": roof, "oracle_errors": errors, "kernels": kern,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    m.free()
+    print(json.dumps(out))
+
+
+def topk_oracle_check(m, Xte, r):
+    """The oracle's pls_class_predict_topk on the device model's fields for the
+    first TK_SAMPLE test rows: timed (the CPU baseline, per row) and compared."""
+    from threadpoolctl import threadpool_limits
+    from oracle import fastpls_oracle as O
+    inf = m.info()
+    p, k, C, L = Xte.shape[1], inf.kmax, inf.m, inf.n_ncomp
+    mod = {"R": m.read("R", (p, k)), "W_latent": m.read("W_latent", (k, C, L)),
+           "mX": m.read("mX", (p,)), "vX": m.read("vX", (p,)), "mY": m.read("mY", (C,)),
+           "ncomp": np.array([inf.ncomp[i] for i in range(L)]), "m": C, "pls_method": "plssvd"}
+    cores = min(16, os.cpu_count() or 1)
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        ri, rs = O.pls_class_predict_topk(mod, Xte[:TK_SAMPLE].astype(np.float64), None, C3_TOP)
+        t = time.perf_counter() - t0
+    ti, ts = r["Ypred_top"][:TK_SAMPLE, :, 0], r["Ypred_top_score"][:TK_SAMPLE, :, 0]
+    ri, rs = ri[:, :, 0], rs[:, :, 0]
+    bad = 0
+    for i in range(ri.shape[0]):
+        if not np.array_equal(ti[i], ri[i]):
+            gaps = np.abs(np.diff(rs[i]))
+            if not (gaps.size and np.min(gaps) < 1e-4 * np.abs(rs[i]).max()):
+                bad += 1
+    errors = {"top5_equal_rows": float(np.mean(np.all(ti == ri, axis=1))),
+              "top5_mismatch_non_tie": bad,
+              "max_abs_score_err": float(np.max(np.abs(ts - rs))),
+              "max_abs_score": float(np.max(np.abs(rs)))}
+    cpu = {"value": TK_SAMPLE / t, "unit": "rows/s", "cores": cores, "kind": "port",
+           "sample": f"FP64 numpy oracle pls_class_predict_topk on the first {TK_SAMPLE} test rows "
+                     "(NumPy BLAS products + the insertion loop)", "seconds": t}
+    return errors, cpu
+
+
 def c3_oracle_check(Xtr, ytr, Xte, res):
     """FP64 oracle: full fit on the training rows, prediction of the first
     C3_SAMPLE test rows (timed: the CPU baseline, prediction extrapolated to
@@ -720,7 +850,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch", "cknn"],
+    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch", "cknn", "topk"],
                     default="plssvd_lda",
                     help="plssvd_lda: PLSSVD + LDA fit/predict (default); cv: batched nested "
                          "cross-validation of SIMPLS; sketch: rSVD of X^T Y with learned vs "
@@ -744,6 +874,9 @@ def main():
         return
     if args.workload == "cknn":
         main_cknn(args)
+        return
+    if args.workload == "topk":
+        main_topk(args)
         return
     from fastpls_amd import synth
     data = synth.classes_on_sphere(n_classes=C, per_class_train=N_TRAIN // C,

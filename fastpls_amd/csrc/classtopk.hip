@@ -85,16 +85,30 @@ __global__ __launch_bounds__(CTK_T) void k_ctk_reg(const TS* __restrict__ S, int
   for (int q = 0; q < KM; ++q) { bv[q] = -INFINITY; bi[q] = 0; }
 
   if (live) {
-    const TS* s = S + i;
-    int j = j0;
-    for (; j + CTK_U <= j1; j += CTK_U) {
-      double v[CTK_U];
+    // two register sets of CTK_U scores: the loads of the next set are in
+    // flight while the current one is ranked
+    const TS* s = S + (int64_t)j0 * ldS + i;
+    const int n = j1 > j0 ? j1 - j0 : 0;
+    const int nfull = n / CTK_U * CTK_U;
+    TS va[CTK_U], vb[CTK_U];
+    if (nfull > 0) {
 #pragma unroll
-      for (int u = 0; u < CTK_U; ++u) v[u] = (double)s[(int64_t)(j + u) * ldS];
-#pragma unroll
-      for (int u = 0; u < CTK_U; ++u) ctk_push<KM>(bv, bi, v[u] + off[j + u], j + u + 1);
+      for (int u = 0; u < CTK_U; ++u) va[u] = s[(int64_t)u * ldS];
     }
-    for (; j < j1; ++j) ctk_push<KM>(bv, bi, (double)s[(int64_t)j * ldS] + off[j], j + 1);
+    for (int c = 0; c < nfull; c += CTK_U) {
+      const bool more = c + CTK_U < nfull;
+      if (more) {
+#pragma unroll
+        for (int u = 0; u < CTK_U; ++u) vb[u] = s[(int64_t)(c + CTK_U + u) * ldS];
+      }
+#pragma unroll
+      for (int u = 0; u < CTK_U; ++u)
+        ctk_push<KM>(bv, bi, (double)va[u] + off[j0 + c + u], j0 + c + u + 1);
+#pragma unroll
+      for (int u = 0; u < CTK_U; ++u) va[u] = vb[u];
+    }
+    for (int c = nfull; c < n; ++c)
+      ctk_push<KM>(bv, bi, (double)s[(int64_t)c * ldS] + off[j0 + c], j0 + c + 1);
   }
 
   if (P > 1) {
@@ -162,7 +176,7 @@ __global__ __launch_bounds__(CTK_T) void k_ctk_rank(const TS* __restrict__ S, in
 }
 
 int ctk_kmax(int k) {
-  return k <= 1 ? 1 : k <= 4 ? 4 : k <= 16 ? 16 : k <= CTK_MAX_REG ? CTK_MAX_REG : 0;
+  return k <= 1 ? 1 : k <= 4 ? 4 : k <= 8 ? 8 : k <= 16 ? 16 : k <= CTK_MAX_REG ? CTK_MAX_REG : 0;
 }
 
 template <int KM, typename TS>
@@ -176,11 +190,13 @@ void ctk_reg_launch(Ctx* c, const TS* S, int64_t ldS, int64_t rows, int m, int P
 }
 
 // Segments per row: enough workgroups to cover the chip, each segment at
-// least 16 classes, and the merge staging under 64 KiB of LDS.
+// least 16 classes, at least 64 rows per workgroup (a wave reads one
+// segment, so its offsets are uniform loads), and the merge staging under
+// 64 KiB of LDS.
 int ctk_segments(Ctx* c, int64_t rows, int m, int KM) {
   int P = 1;
   const int64_t want = 4LL * std::max(1, c->num_cus);
-  while (P < 16) {
+  while (P < CTK_T / 64) {
     const int P2 = P * 2;
     const int RB = CTK_T / P2;
     const size_t sm = (size_t)(P2 - 1) * RB * KM * (sizeof(double) + sizeof(int));
@@ -212,6 +228,7 @@ void ctk_dispatch(Ctx* c, const TS* S, int64_t ldS, int64_t rows, int m, const d
   switch (KM) {
     case 1: ctk_reg_launch<1>(c, S, ldS, rows, m, P, off, k, ldo, oi, os); break;
     case 4: ctk_reg_launch<4>(c, S, ldS, rows, m, P, off, k, ldo, oi, os); break;
+    case 8: ctk_reg_launch<8>(c, S, ldS, rows, m, P, off, k, ldo, oi, os); break;
     case 16: ctk_reg_launch<16>(c, S, ldS, rows, m, P, off, k, ldo, oi, os); break;
     default: ctk_reg_launch<CTK_MAX_REG>(c, S, ldS, rows, m, P, off, k, ldo, oi, os); break;
   }
