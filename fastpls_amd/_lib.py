@@ -120,6 +120,15 @@ SIGNATURES = {
                                                ctypes.c_int32, _vp, _vp]),
     "fpls_class_topk_scores": (ctypes.c_int, [_vp, _P(Mat), ctypes.c_int32, ctypes.c_int32, _vp,
                                               ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
+    "fpls_opls_filter": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_int32, _P(Mat), _vp, _vp, _vp,
+                                        _vp, _vp]),
+    "fpls_opls_apply": (ctypes.c_int, [_vp, _P(Mat), _vp, _vp, _vp, _vp, ctypes.c_int32,
+                                       _P(Mat)]),
+    "fpls_kernel_matrix": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), ctypes.c_int32, ctypes.c_double,
+                                          ctypes.c_int32, ctypes.c_double, _P(Mat)]),
+    "fpls_center_kernel_train": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _vp, _vp]),
+    "fpls_center_kernel_test": (ctypes.c_int, [_vp, _P(Mat), _vp, ctypes.c_double, _P(Mat)]),
     "fpls_cv_predict": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp, _vp,
                                        ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _P(Opts),
                                        ctypes.c_int32, ctypes.c_double, ctypes.c_int32,

@@ -122,12 +122,32 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
         fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
         precision="fp32", label_aware=False, fetch=True, device=0, top=1,
-        n_classes=None, gaussian_y=False, gaussian_y_dim=None) -> FastPLS:
+        n_classes=None, gaussian_y=False, gaussian_y_dim=None, north=1, kernel="linear",
+        gamma=None, degree=3, coef0=1.0) -> FastPLS:
     """pls() (R/main.R:5860-6314) with backend = "hip".  ``n_classes``: level
     count of device-resident integer labels (optional).  ``gaussian_y``: fit a
     Gaussian compression of the response and decode the predictions
-    (.prepare_gaussian_y, R/main.R:6057-6068; argmax classifier)."""
+    (.prepare_gaussian_y, R/main.R:6057-6068; argmax classifier).
+    ``method="opls"`` (``north`` orthogonal components) and
+    ``method="kernelpls"`` (``kernel``, ``gamma``, ``degree``, ``coef0``) fit
+    SIMPLS on the filtered X / the centred training kernel (R/main.R:5956-6006)."""
     _check_backend(backend)
+    if method in ("opls", "kernelpls"):
+        inner_kw = dict(ncomp=ncomp, classifier=classifier, lda_ridge=lda_ridge,
+                        candidate_knn_k=candidate_knn_k, candidate_tau=candidate_tau,
+                        candidate_alpha=candidate_alpha, candidate_top_m=candidate_top_m,
+                        fit=fit, seed=seed, rsvd_oversample=rsvd_oversample,
+                        rsvd_power=rsvd_power, precision=precision, fetch=fetch, device=device,
+                        n_classes=n_classes, gaussian_y=gaussian_y,
+                        gaussian_y_dim=gaussian_y_dim)
+        if method == "opls":
+            model = _opls_fit(Xtrain, Ytrain, int(north), scaling, inner_kw)
+        else:
+            model = _kernel_pls_fit(Xtrain, Ytrain, scaling, kernel, gamma, degree, coef0,
+                                    inner_kw)
+        if Xtest is not None:
+            model.update(predict(model, Xtest, Ytest, top=top))
+        return model
     if gaussian_y:
         if classifier != "argmax":
             raise ValueError("gaussian_y supports the argmax classifier")
@@ -323,6 +343,11 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
     cube is formed and ranked (fpls_class_topk_scores)."""
     ctx = model.ctx
     top = resolve_top_k(top, top5)
+    if model.get("opls") is not None:
+        newdata = opls_apply_filter(newdata, **model["opls"], device=ctx.device,
+                                    out="device")
+    elif model.get("kernel_pls") is not None:
+        newdata = _kernel_test_matrix(model["kernel_pls"], newdata, ctx.device)
     keep: list = []
     X = as_mat(newdata, keep)
     w = X.rows
@@ -395,6 +420,201 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
                 q2.append(1.0 - press / tss)
             res["Q2Y"] = np.array(q2)
     return res
+
+
+def _device_f64(rows: int, cols: int, device: int):
+    import torch
+    return torch.empty((cols, rows), dtype=torch.float64, device=f"cuda:{device}").t()
+
+
+def _f64_out(rows: int, cols: int, out: str, device: int):
+    if out == "device":
+        return _device_f64(rows, cols, device)
+    return np.empty((rows, cols), dtype=np.float64, order="F")
+
+
+def _mat_out(a, keep: list) -> _lib.Mat:
+    if hasattr(a, "is_cuda"):
+        return as_mat(a, keep)
+    keep.append(a)
+    return _lib.Mat(a.ctypes.data, _lib.F64, _lib.HOST, a.shape[0], a.shape[1],
+                    max(a.shape[0], 1))
+
+
+def _shape(x):
+    return (int(x.shape[0]), int(x.shape[1]) if len(x.shape) > 1 else 1)
+
+
+def opls_filter(X, Y, north: int = 1, scaling="centering", n_classes=None, device: int = 0,
+                out: str = "host") -> Dict:
+    """opls_filter_cpp (src/fastPLS.cpp:4377-4461) on the device.  Y: a dense
+    response, or class labels (their one-hot, .supervised_response_matrix).
+    ``out="device"`` leaves the filtered X in HBM (a column-major FP64 tensor)."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    n, p = Xm.rows, Xm.cols
+    sc = SCALINGS[scaling] if isinstance(scaling, str) else int(scaling)
+    Ym = yv = None
+    C = 0
+    if _is_factor(Y):
+        _, codes, C = _encode_factor(Y, n_classes)
+        yv = as_ivec(codes, keep)
+    else:
+        Ym = as_mat(Y, keep)
+    Xo = _f64_out(n, p, out, ctx.device)
+    Om = _mat_out(Xo, keep)
+    mX = np.empty(p)
+    vX = np.empty(p)
+    Wo = np.zeros((p, max(int(north), 0)), order="F")
+    Po = np.zeros((p, max(int(north), 0)), order="F")
+    used = ctypes.c_int32(0)
+    ctx.check(ctx.lib.fpls_opls_filter(ctx.h, ctypes.byref(Xm),
+                                       ctypes.byref(Ym) if Ym is not None else None,
+                                       ctypes.byref(yv) if yv is not None else None, C,
+                                       int(north), sc, ctypes.byref(Om), mX.ctypes.data,
+                                       vX.ctypes.data, Wo.ctypes.data if north > 0 else None,
+                                       Po.ctypes.data if north > 0 else None,
+                                       ctypes.byref(used)))
+    k = used.value
+    return {"X": Xo, "mX": mX, "vX": vX, "W_orth": np.asfortranarray(Wo[:, :k]),
+            "P_orth": np.asfortranarray(Po[:, :k]), "north": k}
+
+
+def opls_apply_filter(X, mX, vX, W_orth, P_orth, north=None, device: int = 0,
+                      out: str = "host"):
+    """opls_apply_filter_cpp (src/fastPLS.cpp:4463-4483) on the device."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Xm = as_mat(X, keep)
+    n, p = Xm.rows, Xm.cols
+    mX = np.ascontiguousarray(mX, dtype=np.float64).reshape(-1)
+    vX = np.ascontiguousarray(vX, dtype=np.float64).reshape(-1)
+    Wo = np.asfortranarray(W_orth, dtype=np.float64).reshape(p, -1, order="F")
+    Po = np.asfortranarray(P_orth, dtype=np.float64).reshape(p, -1, order="F")
+    if mX.size != p or vX.size != p:
+        raise ValueError("X columns must match stored OPLS preprocessing")
+    if Wo.shape != Po.shape:
+        raise ValueError("Invalid OPLS orthogonal filter dimensions")
+    k = Wo.shape[1]
+    Xo = _f64_out(n, p, out, ctx.device)
+    Om = _mat_out(Xo, keep)
+    ctx.check(ctx.lib.fpls_opls_apply(ctx.h, ctypes.byref(Xm), mX.ctypes.data, vX.ctypes.data,
+                                      Wo.ctypes.data if k else None,
+                                      Po.ctypes.data if k else None, k, ctypes.byref(Om)))
+    return Xo
+
+
+KERNELS = {"linear": 1, "rbf": 2, "poly": 3}
+
+
+def kernel_matrix(X1, X2, kernel="rbf", gamma=None, degree: int = 3, coef0: float = 1.0,
+                  device: int = 0, out: str = "host"):
+    """kernel_matrix_cpp (src/fastPLS.cpp:4314-4341) on the device (FP64)."""
+    ctx = _lib.context(device)
+    kid = KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    keep: list = []
+    A = as_mat(X1, keep)
+    B = as_mat(X2, keep)
+    if gamma is None:
+        gamma = 1.0 / max(1, A.cols)
+    K = _f64_out(A.rows, B.rows, out, ctx.device)
+    Km = _mat_out(K, keep)
+    ctx.check(ctx.lib.fpls_kernel_matrix(ctx.h, ctypes.byref(A), ctypes.byref(B), kid,
+                                         float(gamma), int(degree), float(coef0),
+                                         ctypes.byref(Km)))
+    return K
+
+
+def center_kernel_train(K, device: int = 0, out: str = "host", in_place: bool = False) -> Dict:
+    """center_kernel_train_cpp (src/fastPLS.cpp:4344-4357) on the device."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Km = as_mat(K, keep)
+    Ko = K if in_place else _f64_out(Km.rows, Km.cols, out, ctx.device)
+    Om = _mat_out(Ko, keep) if not in_place else Km
+    cm = np.empty(Km.cols)
+    gm = ctypes.c_double(0.0)
+    ctx.check(ctx.lib.fpls_center_kernel_train(ctx.h, ctypes.byref(Km), ctypes.byref(Om),
+                                               cm.ctypes.data, ctypes.byref(gm)))
+    return {"K": Ko, "col_means": cm, "grand_mean": gm.value}
+
+
+def center_kernel_test(Ktest, col_means, grand_mean: float, device: int = 0,
+                       out: str = "host"):
+    """center_kernel_test_cpp (src/fastPLS.cpp:4360-4374) on the device."""
+    ctx = _lib.context(device)
+    keep: list = []
+    Km = as_mat(Ktest, keep)
+    cm = np.ascontiguousarray(col_means, dtype=np.float64).reshape(-1)
+    if cm.size != Km.cols:
+        raise ValueError("Ktest columns must match the training kernel size")
+    Ko = _f64_out(Km.rows, Km.cols, out, ctx.device)
+    Om = _mat_out(Ko, keep)
+    ctx.check(ctx.lib.fpls_center_kernel_test(ctx.h, ctypes.byref(Km), cm.ctypes.data,
+                                              float(grand_mean), ctypes.byref(Om)))
+    return Ko
+
+
+def _opls_fit(Xtrain, Ytrain, north: int, scaling, inner_kw: Dict) -> FastPLS:
+    """.opls_fit (R/main.R:3047-3091): the filter on the device, SIMPLS on the
+    filtered X (HBM-resident) with scaling "none"."""
+    filt = opls_filter(Xtrain, Ytrain, north, scaling, inner_kw.get("n_classes"),
+                       inner_kw["device"], out="device")
+    model = pls(filt["X"], Ytrain, None, None, method="simpls", scaling="none", **inner_kw)
+    model["opls"] = {k: filt[k] for k in ("mX", "vX", "W_orth", "P_orth")}
+    model["north"] = filt["north"]
+    model["pls_method"] = "simpls"
+    model["opls_engine"] = "hip"
+    model["class"] = ["fastPLSOpls", "fastPLS"]
+    return model
+
+
+def _kernel_pls_fit(Xtrain, Ytrain, scaling, kernel, gamma, degree, coef0,
+                    inner_kw: Dict) -> FastPLS:
+    """.kernel_pls_fit (R/main.R:2826-2905): the linear kernel fits SIMPLS on X
+    directly; rbf / poly preprocess X, build and centre the training kernel in
+    HBM and fit SIMPLS on it with scaling "none"."""
+    if kernel not in KERNELS:
+        raise ValueError("kernel must be 'linear', 'rbf' or 'poly'")
+    dev = inner_kw["device"]
+    if kernel == "linear":
+        model = pls(Xtrain, Ytrain, None, None, method="simpls", scaling=scaling, **inner_kw)
+        model["kernel"] = "linear"
+        model["kernel_engine"] = "hip_direct"
+        model["kernel_linear_direct"] = True
+        model["pls_method"] = "simpls"
+        return model
+    prep = opls_filter(Xtrain, np.zeros((_shape(Xtrain)[0], 1)), 0, scaling, device=dev,
+                       out="device")
+    p = prep["X"].shape[1]
+    if gamma is None:
+        gamma = 1.0 / max(1, p)
+    gamma = float(np.atleast_1d(gamma)[0])
+    if not np.isfinite(gamma) or gamma <= 0:
+        raise ValueError("gamma must be a finite positive number")
+    K = kernel_matrix(prep["X"], prep["X"], kernel, gamma, degree, coef0, dev, out="device")
+    kc = center_kernel_train(K, dev, in_place=True)
+    model = pls(kc["K"], Ytrain, None, None, method="simpls", scaling="none", **inner_kw)
+    model["kernel_pls"] = {"Xref": prep["X"], "mX": prep["mX"], "vX": prep["vX"],
+                           "kernel": kernel, "kernel_id": KERNELS[kernel], "gamma": gamma,
+                           "degree": int(degree), "coef0": float(coef0),
+                           "col_means": kc["col_means"], "grand_mean": kc["grand_mean"]}
+    model["kernel"] = kernel
+    model["kernel_engine"] = "hip"
+    model["pls_method"] = "simpls"
+    model["class"] = ["fastPLSKernel", "fastPLS"]
+    return model
+
+
+def _kernel_test_matrix(spec: Dict, newdata, device: int):
+    """predict.fastPLSKernel (R/main.R:3031-3045): the centred test kernel."""
+    p = spec["Xref"].shape[1]
+    Xn = opls_apply_filter(newdata, spec["mX"], spec["vX"], np.zeros((p, 0)), np.zeros((p, 0)),
+                           device=device, out="device")
+    Kt = kernel_matrix(Xn, spec["Xref"], spec["kernel"], spec["gamma"], spec["degree"],
+                       spec["coef0"], device, out="device")
+    return center_kernel_test(Kt, spec["col_means"], spec["grand_mean"], device, out="device")
 
 
 def gaussian_y_default_dim(Xtrain, gaussian_y_dim=None) -> int:

@@ -50,6 +50,11 @@
  *                             :3838, fill_topk_from_yblock :3863)
  *   fpls_class_topk_scores    class_bias_topk_from_cube  src/fastPLS.cpp:3908-3952,
  *                             .class_topk_from_score_cube R/main.R:453-474
+ *   fpls_opls_filter          opls_filter_cpp            src/fastPLS.cpp:4377-4461
+ *   fpls_opls_apply           opls_apply_filter_cpp      src/fastPLS.cpp:4463-4483
+ *   fpls_kernel_matrix        kernel_matrix_cpp          src/fastPLS.cpp:4314-4341
+ *   fpls_center_kernel_train  center_kernel_train_cpp    src/fastPLS.cpp:4344-4357
+ *   fpls_center_kernel_test   center_kernel_test_cpp     src/fastPLS.cpp:4360-4374
  */
 #ifndef FASTPLS_HIP_H
 #define FASTPLS_HIP_H
@@ -251,6 +256,40 @@ int fpls_class_predict_topk(fpls_ctx* ctx, const fpls_model* model, const fpls_m
 int fpls_class_topk_scores(fpls_ctx* ctx, const fpls_mat* scores, int32_t m, int32_t n_slices,
                            const double* class_bias, int32_t bias_cols, int32_t top_k,
                            int32_t* top_index, double* top_score);
+
+/* ---- OPLS filter and kernel matrices (pre-stages of the SIMPLS fit) ---------- */
+/* Orthogonal-signal filter: X is centred (scaling 1), centred and divided by
+   its column sd (2, zero or non-finite sd -> 1) or left (3); then up to north
+   orthogonal components are removed (early stop when a norm is zero or not
+   finite).  The response is Y (n x m, dense) or the one-hot of y (1-based
+   labels, n_classes columns): give exactly one.  Outputs: X_out (n x p FP64,
+   host or device; the filtered X), mX, vX (p, host), W_orth, P_orth (p x
+   north_used, host, column-major, may be NULL), north_used.  north = 0 is the
+   plain preprocessing. */
+int fpls_opls_filter(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
+                     int32_t n_classes, int32_t north, int32_t scaling, const fpls_mat* X_out,
+                     double* mX, double* vX, double* W_orth, double* P_orth,
+                     int32_t* north_used);
+/* X_out = the stored filter applied to new rows: (X - mX) / vX, then
+   X -= (X w_a) p_a^T for a = 1..north (mX, vX, W_orth, P_orth host). */
+int fpls_opls_apply(fpls_ctx* ctx, const fpls_mat* X, const double* mX, const double* vX,
+                    const double* W_orth, const double* P_orth, int32_t north,
+                    const fpls_mat* X_out);
+#define FPLS_KERNEL_LINEAR 1
+#define FPLS_KERNEL_RBF    2
+#define FPLS_KERNEL_POLY   3
+/* K (n1 x n2 FP64, host or device) = X1 X2^T (linear),
+   exp(-gamma (|x1|^2 + |x2|^2 - 2 x1'x2)) with (-1e-10, 0) clamped to 0 (rbf),
+   or (gamma x1'x2 + coef0)^degree (poly). */
+int fpls_kernel_matrix(fpls_ctx* ctx, const fpls_mat* X1, const fpls_mat* X2, int32_t kernel,
+                       double gamma, int32_t degree, double coef0, const fpls_mat* K_out);
+/* Kc = K - 1 colmeans^T - rowmeans 1^T + grand_mean; K_out may alias K.
+   col_means (n2) and grand_mean are host outputs (may be NULL). */
+int fpls_center_kernel_train(fpls_ctx* ctx, const fpls_mat* K, const fpls_mat* K_out,
+                             double* col_means, double* grand_mean);
+/* The test-side centring with the training col_means / grand_mean (host). */
+int fpls_center_kernel_test(fpls_ctx* ctx, const fpls_mat* K, const double* col_means,
+                            double grand_mean, const fpls_mat* K_out);
 
 /* ---- truncated SVD ---------------------------------------------------------- */
 /* rSVD (or exact, per opts->svd_method / the reference's dispatch rules) of A

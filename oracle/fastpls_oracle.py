@@ -1150,3 +1150,150 @@ def class_topk_from_score_cube(score_cube, class_bias=None, top: int = 1):
         ti[:, :, a] = order + 1
         ts[:, :, a] = np.take_along_axis(S, order, axis=1)
     return ti, ts
+
+
+# ---------------------------------------------------------------------------
+# OPLS filter and kernel matrices (src/fastPLS.cpp:4314-4483; R/main.R:2764-3249)
+# ---------------------------------------------------------------------------
+
+def kernel_matrix(X1, X2, kernel: int, gamma: float, degree: int = 3, coef0: float = 1.0):
+    """kernel_matrix_cpp (src/fastPLS.cpp:4314-4341): 1 linear, 2 rbf, 3 poly."""
+    X1 = np.asarray(X1, dtype=np.float64)
+    X2 = np.asarray(X2, dtype=np.float64)
+    if X1.shape[1] != X2.shape[1]:
+        raise ValueError("X1 and X2 must have the same number of columns")
+    dots = X1 @ X2.T
+    if kernel == 1:
+        return dots
+    if kernel == 3:
+        return np.power(gamma * dots + coef0, float(degree))
+    if kernel != 2:
+        raise ValueError("Unknown kernel id")
+    n1 = np.sum(X1 * X1, axis=1)
+    n2 = np.sum(X2 * X2, axis=1)
+    d2 = (n1[:, None] + n2[None, :]) - 2.0 * dots
+    d2 = np.where((d2 < 0.0) & (d2 > -1e-10), 0.0, d2)
+    return np.exp(-gamma * d2)
+
+
+def center_kernel_train(K):
+    """center_kernel_train_cpp (src/fastPLS.cpp:4344-4357)."""
+    K = np.asarray(K, dtype=np.float64)
+    cm = K.mean(axis=0)
+    rm = K.mean(axis=1)
+    gm = float(cm.mean())
+    return {"K": ((K - cm[None, :]) - rm[:, None]) + gm, "col_means": cm, "grand_mean": gm}
+
+
+def center_kernel_test(Ktest, col_means, grand_mean: float):
+    """center_kernel_test_cpp (src/fastPLS.cpp:4360-4374)."""
+    K = np.asarray(Ktest, dtype=np.float64)
+    cm = np.asarray(col_means, dtype=np.float64).reshape(-1)
+    if K.shape[1] != cm.size:
+        raise ValueError("Ktest columns must match the training kernel size")
+    rm = K.mean(axis=1)
+    return ((K - cm[None, :]) - rm[:, None]) + grand_mean
+
+
+def preprocess_train(X, scaling: int):
+    """.fastpls_preprocess_train (R/main.R:2764-2779) == opls_filter_cpp's
+    preprocessing: centre for scaling 1/2, divide by the column sd (n - 1)
+    for 2 (zero / non-finite sd -> 1)."""
+    X = np.asarray(X, dtype=np.float64)
+    mX = np.zeros(X.shape[1])
+    vX = np.ones(X.shape[1])
+    if scaling < 3:
+        mX = X.mean(axis=0)
+        X = X - mX
+    if scaling == 2:
+        vX = X.std(axis=0, ddof=1) if X.shape[0] > 1 else np.zeros(X.shape[1])
+        vX = np.where(np.isfinite(vX) & (vX != 0.0), vX, 1.0)
+        X = X / vX
+    return X, mX, vX
+
+
+def opls_filter(X, Y, north: int, scaling: int = 1) -> Dict:
+    """opls_filter_cpp (src/fastPLS.cpp:4377-4461)."""
+    if north < 0:
+        raise ValueError("north must be >= 0")
+    X, mX, vX = preprocess_train(X, scaling)
+    Y = np.asarray(Y, dtype=np.float64)
+    if Y.ndim == 1:
+        Y = Y[:, None]
+    if X.shape[0] != Y.shape[0]:
+        raise ValueError("X and Y must have the same number of rows")
+    Y = Y - Y.mean(axis=0)
+    Wo, Po = [], []
+    for _ in range(north):
+        S = X.T @ Y
+        U = np.linalg.svd(S, full_matrices=False)[0]
+        if U.shape[1] == 0:
+            break
+        w = U[:, 0]
+        wn = np.linalg.norm(w)
+        if not np.isfinite(wn) or wn <= 0.0:
+            break
+        w = w / wn
+        t = X @ w
+        tss = float(t @ t)
+        if not np.isfinite(tss) or tss <= 0.0:
+            break
+        p = X.T @ t / tss
+        wo = p - w * (float(w @ p) / float(w @ w))
+        won = np.linalg.norm(wo)
+        if not np.isfinite(won) or won <= 0.0:
+            break
+        wo = wo / won
+        to = X @ wo
+        toss = float(to @ to)
+        if not np.isfinite(toss) or toss <= 0.0:
+            break
+        po = X.T @ to / toss
+        X = X - np.outer(to, po)
+        Wo.append(wo)
+        Po.append(po)
+    p_ = X.shape[1]
+    return {"X": X, "mX": mX, "vX": vX,
+            "W_orth": np.array(Wo).T if Wo else np.zeros((p_, 0)),
+            "P_orth": np.array(Po).T if Po else np.zeros((p_, 0)), "north": len(Wo)}
+
+
+def opls_apply_filter(X, mX, vX, W_orth, P_orth):
+    """opls_apply_filter_cpp (src/fastPLS.cpp:4463-4483)."""
+    X = (np.asarray(X, dtype=np.float64) - mX) / vX
+    for a in range(W_orth.shape[1]):
+        t = X @ W_orth[:, a]
+        X = X - np.outer(t, P_orth[:, a])
+    return X
+
+
+def opls_pls(Xtrain, Ytrain, Xtest, ncomp, north: int = 1, scaling: int = 1, classes=None,
+             oversample=10, power=1, seed=1):
+    """.opls_fit (R/main.R:3047-3091) with the SIMPLS inner fit on the filtered
+    X (scaling "none") and predict.fastPLSOpls (R/main.R:3230-3242).  classes:
+    the number of classes when Ytrain holds 1-based labels (the filter sees
+    their one-hot, .supervised_response_matrix)."""
+    Yf = transformy(Ytrain) if classes else np.asarray(Ytrain, dtype=np.float64)
+    filt = opls_filter(Xtrain, Yf, north, scaling)
+    mod = pls_model2_fast(filt["X"], Yf, ncomp, 3, False, "rsvd", oversample, power, seed)
+    Xt = opls_apply_filter(Xtest, filt["mX"], filt["vX"], filt["W_orth"], filt["P_orth"])
+    return {"filter": filt, "inner": mod, "Ypred": pls_predict(mod, Xt)}
+
+
+def kernel_pls(Xtrain, Ytrain, Xtest, ncomp, kernel: int = 2, gamma=None, degree: int = 3,
+               coef0: float = 1.0, scaling: int = 1, classes=None, oversample=10, power=1,
+               seed=1):
+    """.kernel_pls_fit (R/main.R:2826-2905) for the rbf / poly kernels and
+    predict.fastPLSKernel (R/main.R:3031-3045): SIMPLS on the centred training
+    kernel (scaling "none"), prediction from the centred test kernel."""
+    Xp, mX, vX = preprocess_train(Xtrain, scaling)
+    if gamma is None:
+        gamma = 1.0 / max(1, Xp.shape[1])
+    K = kernel_matrix(Xp, Xp, kernel, gamma, degree, coef0)
+    kc = center_kernel_train(K)
+    Yf = transformy(Ytrain) if classes else np.asarray(Ytrain, dtype=np.float64)
+    mod = pls_model2_fast(kc["K"], Yf, ncomp, 3, False, "rsvd", oversample, power, seed)
+    Xn = (np.asarray(Xtest, dtype=np.float64) - mX) / vX
+    Kt = center_kernel_test(kernel_matrix(Xn, Xp, kernel, gamma, degree, coef0),
+                            kc["col_means"], kc["grand_mean"])
+    return {"Kc": kc, "inner": mod, "gamma": gamma, "Ypred": pls_predict(mod, Kt), "Ktest": Kt}
