@@ -803,6 +803,113 @@ def topk_oracle_check(m, Xte, r):
     return errors, cpu
 
 
+# Kernel PLS (rbf, gamma = 1/p) on 100 classes: 10,000 training rows x 768
+# (FP32, classes on the sphere), 2,000 test rows, ncomp 20.  One step = the
+# training kernel (10,000 x 10,000 FP64) built and centred in HBM, the SIMPLS
+# fit on it, the test kernel and the prediction.  The OPLS filter (north 1)
+# + SIMPLS + prediction on the 50,000-row classification data is timed beside.
+KP_C, KP_TRAIN, KP_TEST, KP_P, KP_K = 100, 100, 20, 768, 20
+
+
+def main_kernel(args):
+    import torch
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    Xtr, ytr, Xte, yte = synth.classes_on_sphere(n_classes=KP_C, per_class_train=KP_TRAIN,
+                                                 per_class_test=KP_TEST, p=KP_P, seed=71)
+    dX, dXt = F.device_colmajor(Xtr), F.device_colmajor(Xte)
+    torch.cuda.synchronize()
+    ctx = F.context(0)
+    n, nt = Xtr.shape[0], Xte.shape[0]
+
+    def kstep():
+        m = F.pls(dX, ytr, dXt, method="kernelpls", kernel="rbf", ncomp=KP_K, fetch=False)
+        lab = m["Ypred_index"]
+        m.free()
+        return lab
+    for _ in range(args.warmup):
+        kstep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        lab = kstep()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    ctx.timing(True)
+    kstep()
+    names = ("kernel_gemm", "kernel_epilogue", "row_sumsq", "row_means", "kernel_center",
+             "kernel_colmeans", "col_sums", "opls_prep", "gemm_f64", "gemm_f64_reduce", "gemv_n",
+             "gemv_t", "class_topk")
+    kern = {}
+    for nm in names:
+        c_, ms = ctx.kernel_time(nm)
+        if c_:
+            kern[nm] = {"launches_per_step": c_, "ms_per_step": ms}
+    ctx.timing(False)
+    kt = kern["kernel_gemm"]
+    avg_s = kt["ms_per_step"] * 1e-3 / kt["launches_per_step"]
+    flops = 2.0 * (n * n + nt * n) * KP_P / kt["launches_per_step"]
+    achieved = flops / avg_s / 1e12
+    roof = {"bound": "fp64", "kernel": "kernel_gemm", "achieved": achieved,
+            "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"],
+            "traffic": measured_traffic("kernel_gemm_kp", kt["launches_per_step"]),
+            "avg_launch_us": avg_s * 1e6, "launches_per_step": kt["launches_per_step"],
+            "algorithmic_flops_per_launch": flops}
+    # OPLS on the 50,000-row classification data
+    data = synth.classes_on_sphere(n_classes=C, per_class_train=N_TRAIN // C,
+                                   per_class_test=N_TEST // C, p=P)
+    oX, oXt = F.device_colmajor(data[0]), F.device_colmajor(data[2])
+
+    def ostep():
+        m = F.pls(oX, data[1], oXt, method="opls", north=1, ncomp=KP_K, fetch=False)
+        lab = m["Ypred_index"]
+        m.free()
+        return lab
+    ostep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        olab = ostep()
+    torch.cuda.synchronize()
+    odt = (time.perf_counter() - t0) / args.steps
+    errors, cpu = {}, None
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+        from oracle import fastpls_oracle as O
+        cores = min(16, os.cpu_count() or 1)
+        with threadpool_limits(limits=cores):
+            t0 = time.perf_counter()
+            ref = O.kernel_pls(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [KP_K],
+                               kernel=2, classes=KP_C)
+            t = time.perf_counter() - t0
+        rl = np.argmax(ref["Ypred"][:, :, 0], axis=1) + 1
+        s = np.sort(ref["Ypred"][:, :, 0], axis=1)
+        margin = s[:, -1] - s[:, -2]
+        diff = lab[:, 0] != rl
+        errors = {"label_agreement": float(1.0 - diff.mean()),
+                  "label_mismatch_non_tie": int((diff & (margin > 1e-6)).sum()),
+                  "accuracy": float(np.mean(lab[:, 0] == yte)),
+                  "oracle_accuracy": float(np.mean(rl == yte))}
+        cpu = {"value": 1.0 / t, "unit": "fits/s", "cores": cores, "kind": "port",
+               "sample": "one full FP64 kernel-PLS fit + predict of the same workload (numpy "
+                         "oracle)", "seconds": t}
+    out = {
+        "metric": "kernel PLS (rbf) fits/s", "value": 1.0 / dt, "unit": "fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded; classes on the sphere)",
+        "config": {"workload": f"kernelpls rbf n={n} ntest={nt} p={KP_P} C={KP_C} ncomp={KP_K}",
+                   "inputs": "resident in HBM (FP32)"},
+        "opls": {"workload": f"opls north=1 n={N_TRAIN} ntest={N_TEST} p={P} C={C} ncomp={KP_K}",
+                 "fits_per_s": 1.0 / odt, "ms_per_step": 1e3 * odt,
+                 "accuracy": float(np.mean(olab[:, 0] == data[3]))},
+        "roofline": roof, "oracle_errors": errors, "kernels": kern,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out))
+
+
 def c3_oracle_check(Xtr, ytr, Xte, res):
     """FP64 oracle: full fit on the training rows, prediction of the first
     C3_SAMPLE test rows (timed: the CPU baseline, prediction extrapolated to
@@ -850,7 +957,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch", "cknn", "topk"],
+    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch", "cknn", "topk",
+                                           "kernelpls"],
                     default="plssvd_lda",
                     help="plssvd_lda: PLSSVD + LDA fit/predict (default); cv: batched nested "
                          "cross-validation of SIMPLS; sketch: rSVD of X^T Y with learned vs "
@@ -877,6 +985,9 @@ def main():
         return
     if args.workload == "topk":
         main_topk(args)
+        return
+    if args.workload == "kernelpls":
+        main_kernel(args)
         return
     from fastpls_amd import synth
     data = synth.classes_on_sphere(n_classes=C, per_class_train=N_TRAIN // C,

@@ -200,7 +200,7 @@ void f32_to_f64(Ctx* ctx, double* dst, const float* src, int64_t n);
 void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, const double* shiftA, const double* B, int64_t ldb,
               const double* shiftB, double beta, double* C, int64_t ldc,
-              const int32_t* labelsB = nullptr);
+              const int32_t* labelsB = nullptr, const char* name = nullptr);
 
 // FP32-input contractions on MFMA.
 //  nn: C(M x N) = (A - 1 shiftA^T) B - 1 offset^T     A: M x K (ld lda), B: K x N FP32

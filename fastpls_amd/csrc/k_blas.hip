@@ -230,8 +230,9 @@ __global__ void k_reduce_partials_f64(int64_t M, int64_t N, int splits,
 void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, const double* shiftA, const double* B, int64_t ldb,
               const double* shiftB, double beta, double* C, int64_t ldc,
-              const int32_t* labelsB) {
+              const int32_t* labelsB, const char* name) {
   if (M <= 0 || N <= 0) return;
+  const char* kname = name ? name : "gemm_f64";
   if (K <= 0) {
     // C = beta C
     DBuf<double> zero(ctx, 1);
@@ -256,11 +257,11 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
   splits = (K + kchunk - 1) / kchunk;
   dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)splits);
   if (splits == 1) {
-    launch(ctx, "gemm_f64", k_gemm_f64, grid, dim3(256), 0, transA, transB, M, N, K, kchunk,
+    launch(ctx, kname, k_gemm_f64, grid, dim3(256), 0, transA, transB, M, N, K, kchunk,
            alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, (double*)nullptr);
   } else {
     DBuf<double> part(ctx, (size_t)splits * M * N);
-    launch(ctx, "gemm_f64", k_gemm_f64, grid, dim3(256), 0, transA, transB, M, N, K, kchunk,
+    launch(ctx, kname, k_gemm_f64, grid, dim3(256), 0, transA, transB, M, N, K, kchunk,
            alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, part.get());
     launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
            N, (int)splits, part.get(), alpha, beta, C, ldc);

@@ -150,27 +150,50 @@ __global__ void k_op_deflate(double* __restrict__ X, int64_t ld, int64_t rows, i
   }
 }
 
-// r[i] = sum_j X[i, j]^2 (one thread per row, columns in order)
-__global__ void k_op_row_sq(const double* __restrict__ X, int64_t ld, int64_t rows, int64_t cols,
-                            double* __restrict__ r) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  double s = 0.0;
-  for (int64_t j = 0; j < cols; ++j) {
-    const double v = X[i + j * ld];
-    s += v * v;
+// r[i] = sum_j f(X[i, j]) / div, f = square (SQ) or identity.  A
+// workgroup takes 64 rows (one per lane, coalesced down each column) and
+// splits the columns over its 16 waves; the 16 partial sums of a row are
+// added in wave order through LDS.
+constexpr int OP_RR_SEG = 16;
+template <bool SQ>
+__global__ __launch_bounds__(64 * OP_RR_SEG) void k_op_row_reduce(const double* __restrict__ X,
+                                                                  int64_t ld, int64_t rows,
+                                                                  int64_t cols, double div,
+                                                                  double* __restrict__ r) {
+  __shared__ double part[OP_RR_SEG][64];
+  const int lane = threadIdx.x & 63, seg = threadIdx.x >> 6;
+  const int64_t i = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t per = (cols + OP_RR_SEG - 1) / OP_RR_SEG;
+  const int64_t j0 = seg * per, j1 = std::min<int64_t>(cols, j0 + per);
+  double s0 = 0.0, s1 = 0.0;
+  if (i < rows) {
+    const double* x = X + i;
+    int64_t j = j0;
+    for (; j + 1 < j1; j += 2) {
+      const double a = x[j * ld], b = x[(j + 1) * ld];
+      s0 += SQ ? a * a : a;
+      s1 += SQ ? b * b : b;
+    }
+    if (j < j1) {
+      const double a = x[j * ld];
+      s0 += SQ ? a * a : a;
+    }
   }
-  r[i] = s;
+  part[seg][lane] = s0 + s1;
+  __syncthreads();
+  if (seg == 0 && i < rows) {
+    double s = 0.0;
+    for (int q = 0; q < OP_RR_SEG; ++q) s += part[q][lane];
+    r[i] = s / div;
+  }
 }
 
-// r[i] = sum_j K[i, j] / cols
-__global__ void k_op_row_means(const double* __restrict__ K, int64_t ld, int64_t rows,
-                               int64_t cols, double* __restrict__ r) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  double s = 0.0;
-  for (int64_t j = 0; j < cols; ++j) s += K[i + j * ld];
-  r[i] = s / (double)cols;
+void row_reduce(Ctx* c, bool sq, const double* X, int64_t ld, int64_t rows, int64_t cols,
+                double div, double* r) {
+  if (rows <= 0) return;
+  const dim3 g((unsigned)((rows + 63) / 64)), b(64 * OP_RR_SEG);
+  if (sq) launch(c, "row_sumsq", k_op_row_reduce<true>, g, b, 0, X, ld, rows, cols, div, r);
+  else launch(c, "row_means", k_op_row_reduce<false>, g, b, 0, X, ld, rows, cols, div, r);
 }
 
 // kernel epilogue on D = X1 X2^T in place: kind 2 rbf, 3 poly
@@ -474,14 +497,12 @@ extern "C" int fpls_kernel_matrix(fpls_ctx* ctx, const fpls_mat* X1m, const fpls
     }
     if (n1 > 0 && n2 > 0) {
       gemm_f64(c, false, true, n1, n2, p, 1.0, a1, std::max<int64_t>(l1, 1), nullptr, a2,
-               std::max<int64_t>(l2, 1), nullptr, 0.0, K.dev, K.ld);
+               std::max<int64_t>(l2, 1), nullptr, 0.0, K.dev, K.ld, nullptr, "kernel_gemm");
       if (kernel != 1) {
         DBuf<double> q1(c, n1), q2(c, n2);
         if (kernel == 2) {
-          launch(c, "kernel_rowsq", k_op_row_sq, dim3((unsigned)((n1 + 255) / 256)), dim3(256), 0,
-                 a1, l1, n1, p, q1.get());
-          launch(c, "kernel_rowsq", k_op_row_sq, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0,
-                 a2, l2, n2, p, q2.get());
+          row_reduce(c, true, a1, l1, n1, p, 1.0, q1.get());
+          row_reduce(c, true, a2, l2, n2, p, 1.0, q2.get());
         }
         launch(c, "kernel_epilogue", k_op_kernel, dim3(opg(n1 * n2)), dim3(256), 0, K.dev, K.ld,
                n1, n2, (int)kernel, gamma, (double)degree, coef0, (const double*)q1.get(),
@@ -510,8 +531,7 @@ static void center_kernel(Ctx* c, const fpls_mat* Km, const fpls_mat* K_out, con
            gm.get());
   }
   if (r > 0 && q > 0) {
-    launch(c, "kernel_rowmeans", k_op_row_means, dim3((unsigned)((r + 255) / 256)), dim3(256), 0,
-           K.d(), K.ld, r, q, rm.get());
+    row_reduce(c, false, K.d(), K.ld, r, q, (double)q, rm.get());
     launch(c, "kernel_center", k_op_center, dim3(opg(r * q)), dim3(256), 0, K.d(), K.ld, r, q,
            (const double*)cm.get(), (const double*)rm.get(), (const double*)gm.get(), O.dev,
            O.ld);
