@@ -212,6 +212,159 @@ __global__ __launch_bounds__(256) void k_gemm_f64(bool transA, bool transB, int6
   }
 }
 
+// Large FP64 products: 128 x 128 tiles, 256 threads each holding an 8 x 8
+// register block (rows ty*4 + {0..3} and 64 + ty*4 + {0..3}, columns likewise
+// with tx), 16-deep K steps through double-buffered LDS (k-major, 2 doubles of
+// padding per row) with the next step's global loads in registers during the
+// FMAs; one barrier per step.  Operand fragments are 16-byte LDS reads; the
+// A fragment is a broadcast across the 16 lanes of a row group.
+constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_LD = GB_M + 2;
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void gb_fetch(int t, int64_t i0, int64_t j0, int64_t k0, int64_t kend,
+                                         int64_t M, int64_t N, const double* __restrict__ A,
+                                         int64_t lda, const double* __restrict__ shA,
+                                         const double* __restrict__ B, int64_t ldb,
+                                         const double* __restrict__ shB, double (&ra)[8],
+                                         double (&rb)[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = TA ? (t >> 4) + 16 * r : (t & 127);
+    const int k = TA ? (t & 15) : (t >> 7) + 2 * r;
+    const int64_t gi = i0 + i, gk = k0 + k;
+    double v = 0.0;
+    if (gi < M && gk < kend) {
+      v = TA ? A[gk + gi * lda] : A[gi + gk * lda];
+      if (shA) v -= TA ? shA[gi] : shA[gk];
+    }
+    ra[r] = v;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int j = TB ? (t & 127) : (t >> 4) + 16 * r;
+    const int k = TB ? (t >> 7) + 2 * r : (t & 15);
+    const int64_t gj = j0 + j, gk = k0 + k;
+    double v = 0.0;
+    if (gj < N && gk < kend) {
+      v = TB ? B[gj + gk * ldb] : B[gk + gj * ldb];
+      if (shB) v -= TB ? shB[gk] : shB[gj];
+    }
+    rb[r] = v;
+  }
+}
+
+template <bool TA, bool TB>
+__device__ __forceinline__ void gb_store(int t, double (*As)[GB_LD], double (*Bs)[GB_LD],
+                                         const double (&ra)[8], const double (&rb)[8]) {
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int i = TA ? (t >> 4) + 16 * r : (t & 127);
+    const int k = TA ? (t & 15) : (t >> 7) + 2 * r;
+    As[k][i] = ra[r];
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int j = TB ? (t & 127) : (t >> 4) + 16 * r;
+    const int k = TB ? (t >> 7) + 2 * r : (t & 15);
+    Bs[k][j] = rb[r];
+  }
+}
+
+// grid (ceil(M/128), ceil(N/128), splits); splits == 1: C = alpha acc + beta C,
+// else partial[z] = acc.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void k_gemm_f64_big(int64_t M, int64_t N, int64_t K,
+                                                        int64_t kchunk, double alpha,
+                                                        const double* __restrict__ A, int64_t lda,
+                                                        const double* __restrict__ shA,
+                                                        const double* __restrict__ B, int64_t ldb,
+                                                        const double* __restrict__ shB,
+                                                        double beta, double* __restrict__ C,
+                                                        int64_t ldc, double* __restrict__ partial) {
+  __shared__ __align__(16) double As[2][GB_K][GB_LD];
+  __shared__ __align__(16) double Bs[2][GB_K][GB_LD];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int64_t i0 = (int64_t)blockIdx.x * GB_M, j0 = (int64_t)blockIdx.y * GB_N;
+  const int64_t kbeg = (int64_t)blockIdx.z * kchunk;
+  const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
+  double acc[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[r][c] = 0.0;
+  double ra[8], rb[8];
+  int buf = 0;
+  if (kbeg < kend) {
+    gb_fetch<TA, TB>(t, i0, j0, kbeg, kend, M, N, A, lda, shA, B, ldb, shB, ra, rb);
+    gb_store<TA, TB>(t, As[0], Bs[0], ra, rb);
+  }
+  __syncthreads();
+  for (int64_t k0 = kbeg; k0 < kend; k0 += GB_K) {
+    const bool more = k0 + GB_K < kend;
+    if (more) gb_fetch<TA, TB>(t, i0, j0, k0 + GB_K, kend, M, N, A, lda, shA, B, ldb, shB, ra, rb);
+#pragma unroll 2
+    for (int k = 0; k < GB_K; ++k) {
+      const double2 a0 = *reinterpret_cast<const double2*>(&As[buf][k][ty * 4]);
+      const double2 a1 = *reinterpret_cast<const double2*>(&As[buf][k][ty * 4 + 2]);
+      const double2 a2 = *reinterpret_cast<const double2*>(&As[buf][k][64 + ty * 4]);
+      const double2 a3 = *reinterpret_cast<const double2*>(&As[buf][k][64 + ty * 4 + 2]);
+      const double2 b0 = *reinterpret_cast<const double2*>(&Bs[buf][k][tx * 4]);
+      const double2 b1 = *reinterpret_cast<const double2*>(&Bs[buf][k][tx * 4 + 2]);
+      const double2 b2 = *reinterpret_cast<const double2*>(&Bs[buf][k][64 + tx * 4]);
+      const double2 b3 = *reinterpret_cast<const double2*>(&Bs[buf][k][64 + tx * 4 + 2]);
+      const double a[8] = {a0.x, a0.y, a1.x, a1.y, a2.x, a2.y, a3.x, a3.y};
+      const double b[8] = {b0.x, b0.y, b1.x, b1.y, b2.x, b2.y, b3.x, b3.y};
+#pragma unroll
+      for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+    }
+    if (more) gb_store<TA, TB>(t, As[buf ^ 1], Bs[buf ^ 1], ra, rb);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const int64_t gi = i0 + (r < 4 ? ty * 4 + r : 64 + ty * 4 + (r - 4));
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int64_t gj = j0 + (c < 4 ? tx * 4 + c : 64 + tx * 4 + (c - 4));
+      if (gi < M && gj < N) {
+        if (partial) {
+          partial[(int64_t)blockIdx.z * M * N + gi + gj * M] = acc[r][c];
+        } else {
+          double v = alpha * acc[r][c];
+          if (beta != 0.0) v += beta * C[gi + gj * ldc];
+          C[gi + gj * ldc] = v;
+        }
+      }
+    }
+  }
+}
+
+// Deep split-K reductions: one wave per output, lanes over the splits, then a
+// fixed butterfly (deterministic order).
+__global__ __launch_bounds__(256) void k_reduce_partials_wave(int64_t M, int64_t N, int splits,
+                                                             const double* __restrict__ partial,
+                                                             double alpha, double beta,
+                                                             double* __restrict__ C,
+                                                             int64_t ldc) {
+  const int64_t n = M * N;
+  const int lane = threadIdx.x & 63;
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= n) return;
+  double s = 0.0;
+  for (int z = lane; z < splits; z += 64) s += partial[(int64_t)z * n + t];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) {
+    const int64_t i = t % M, j = t / M;
+    double v = alpha * s;
+    if (beta != 0.0) v += beta * C[i + j * ldc];
+    C[i + j * ldc] = v;
+  }
+}
+
 __global__ void k_reduce_partials_f64(int64_t M, int64_t N, int splits,
                                       const double* __restrict__ partial, double alpha,
                                       double beta, double* __restrict__ C, int64_t ldc) {
@@ -241,6 +394,40 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
            N, 0, zero.get(), alpha, beta, C, ldc);
     return;
   }
+  if (!labelsB && M >= 128 && N >= 64 && K >= 64) {
+    const int64_t tiles = ((M + GB_M - 1) / GB_M) * ((N + GB_N - 1) / GB_N);
+    int64_t splits = 1;
+    const int64_t want = 2 * ctx->num_cus;
+    if (tiles < want && K >= 512) {
+      splits = std::min<int64_t>((want + tiles - 1) / tiles, K / 256);
+      const int64_t cap = std::max<int64_t>(1, (int64_t(64) << 20) / std::max<int64_t>(1, M * N));
+      splits = std::max<int64_t>(1, std::min(splits, cap));
+    }
+    int64_t kchunk = (K + splits - 1) / splits;
+    kchunk = (kchunk + GB_K - 1) / GB_K * GB_K;
+    splits = (K + kchunk - 1) / kchunk;
+    dim3 grid((unsigned)((M + GB_M - 1) / GB_M), (unsigned)((N + GB_N - 1) / GB_N),
+              (unsigned)splits);
+    DBuf<double> part;
+    if (splits > 1) part = DBuf<double>(ctx, (size_t)splits * M * N);
+    double* pp = splits > 1 ? part.get() : nullptr;
+    if (!transA && !transB)
+      launch(ctx, kname, k_gemm_f64_big<false, false>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+    else if (!transA && transB)
+      launch(ctx, kname, k_gemm_f64_big<false, true>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+    else if (transA && !transB)
+      launch(ctx, kname, k_gemm_f64_big<true, false>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+    else
+      launch(ctx, kname, k_gemm_f64_big<true, true>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+    if (splits > 1)
+      launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
+             N, (int)splits, (const double*)part.get(), alpha, beta, C, ldc);
+    return;
+  }
   const int64_t tiles = ((M + 63) / 64) * ((N + 63) / 64);
   int64_t splits = 1;
   const int64_t want = 2 * ctx->num_cus;
@@ -263,8 +450,13 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     DBuf<double> part(ctx, (size_t)splits * M * N);
     launch(ctx, kname, k_gemm_f64, grid, dim3(256), 0, transA, transB, M, N, K, kchunk,
            alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, part.get());
-    launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
-           N, (int)splits, part.get(), alpha, beta, C, ldc);
+    if (splits >= 16)
+      launch(ctx, "gemm_f64_reduce", k_reduce_partials_wave,
+             dim3((unsigned)((M * N + 3) / 4)), dim3(256), 0, M, N, (int)splits,
+             (const double*)part.get(), alpha, beta, C, ldc);
+    else
+      launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0,
+             M, N, (int)splits, (const double*)part.get(), alpha, beta, C, ldc);
   }
 }
 

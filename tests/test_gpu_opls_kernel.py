@@ -143,3 +143,16 @@ def test_pls_kernelpls_like_reference_tests(ctx):
     assert fl["kernel_linear_direct"] and fl["Ypred"].shape == (12, 2, 1)
     with pytest.raises(ValueError):
         F.pls(X[tr], Yr[tr], ncomp=2, method="kernelpls", kernel="rbf", gamma=-1.0)
+
+
+@pytest.mark.parametrize("n1,n2,p", [(1000, 700, 300), (333, 129, 2000), (130, 64, 64)])
+def test_large_fp64_products_through_the_linear_kernel(ctx, n1, n2, p):
+    """X1 X2^T through the 128 x 128-tile FP64 GEMM (ragged edges; the middle
+    shape takes split-K partials), against numpy in FP64."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(n1 + n2 + p)
+    A = rng.standard_normal((n1, p))
+    B = rng.standard_normal((n2, p))
+    K = F.kernel_matrix(A, B, "linear")
+    ref = A @ B.T
+    assert np.abs(K - ref).max() <= 1e-13 * np.sqrt(p) * np.abs(A).max() * np.abs(B).max()
