@@ -220,11 +220,12 @@ __global__ __launch_bounds__(256) void k_gemm_f64(bool transA, bool transB, int6
 // A fragment is a broadcast across the 16 lanes of a row group.
 constexpr int GB_M = 128, GB_N = 128, GB_K = 16, GB_LD = GB_M + 2;
 
-template <bool TA, bool TB>
+template <bool TA, bool TB, bool LAB>
 __device__ __forceinline__ void gb_fetch(int t, int64_t i0, int64_t j0, int64_t k0, int64_t kend,
                                          int64_t M, int64_t N, const double* __restrict__ A,
                                          int64_t lda, const double* __restrict__ shA,
                                          const double* __restrict__ B, int64_t ldb,
+                                         const int32_t* __restrict__ labB,
                                          const double* __restrict__ shB, double (&ra)[8],
                                          double (&rb)[8]) {
 #pragma unroll
@@ -246,7 +247,9 @@ __device__ __forceinline__ void gb_fetch(int t, int64_t i0, int64_t j0, int64_t 
     const int64_t gj = j0 + j, gk = k0 + k;
     double v = 0.0;
     if (gj < N && gk < kend) {
-      v = TB ? B[gj + gk * ldb] : B[gk + gj * ldb];
+      // labels (non-transposed B only): B(k, j) = [lab_k == j + 1]
+      if (LAB) v = labB[gk] == gj + 1 ? 1.0 : 0.0;
+      else v = TB ? B[gj + gk * ldb] : B[gk + gj * ldb];
       if (shB) v -= TB ? shB[gk] : shB[gj];
     }
     rb[r] = v;
@@ -271,13 +274,14 @@ __device__ __forceinline__ void gb_store(int t, double (*As)[GB_LD], double (*Bs
 }
 
 // grid (ceil(M/128), ceil(N/128), splits); splits == 1: C = alpha acc + beta C,
-// else partial[z] = acc.
-template <bool TA, bool TB>
+// else partial[z] = acc.  LAB: B is the one-hot of labB (not transposed).
+template <bool TA, bool TB, bool LAB>
 __global__ __launch_bounds__(256, 2) void k_gemm_f64_big(int64_t M, int64_t N, int64_t K,
                                                         int64_t kchunk, double alpha,
                                                         const double* __restrict__ A, int64_t lda,
                                                         const double* __restrict__ shA,
                                                         const double* __restrict__ B, int64_t ldb,
+                                                        const int32_t* __restrict__ labB,
                                                         const double* __restrict__ shB,
                                                         double beta, double* __restrict__ C,
                                                         int64_t ldc, double* __restrict__ partial) {
@@ -295,13 +299,15 @@ __global__ __launch_bounds__(256, 2) void k_gemm_f64_big(int64_t M, int64_t N, i
   double ra[8], rb[8];
   int buf = 0;
   if (kbeg < kend) {
-    gb_fetch<TA, TB>(t, i0, j0, kbeg, kend, M, N, A, lda, shA, B, ldb, shB, ra, rb);
+    gb_fetch<TA, TB, LAB>(t, i0, j0, kbeg, kend, M, N, A, lda, shA, B, ldb, labB, shB, ra, rb);
     gb_store<TA, TB>(t, As[0], Bs[0], ra, rb);
   }
   __syncthreads();
   for (int64_t k0 = kbeg; k0 < kend; k0 += GB_K) {
     const bool more = k0 + GB_K < kend;
-    if (more) gb_fetch<TA, TB>(t, i0, j0, k0 + GB_K, kend, M, N, A, lda, shA, B, ldb, shB, ra, rb);
+    if (more)
+      gb_fetch<TA, TB, LAB>(t, i0, j0, k0 + GB_K, kend, M, N, A, lda, shA, B, ldb, labB, shB, ra,
+                            rb);
 #pragma unroll 2
     for (int k = 0; k < GB_K; ++k) {
       const double2 a0 = *reinterpret_cast<const double2*>(&As[buf][k][ty * 4]);
@@ -394,7 +400,7 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
            N, 0, zero.get(), alpha, beta, C, ldc);
     return;
   }
-  if (!labelsB && M >= 128 && N >= 64 && K >= 64) {
+  if ((!labelsB || !transB) && M >= 128 && N >= 64 && K >= 64) {
     const int64_t tiles = ((M + GB_M - 1) / GB_M) * ((N + GB_N - 1) / GB_N);
     int64_t splits = 1;
     const int64_t want = 2 * ctx->num_cus;
@@ -411,18 +417,24 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     DBuf<double> part;
     if (splits > 1) part = DBuf<double>(ctx, (size_t)splits * M * N);
     double* pp = splits > 1 ? part.get() : nullptr;
-    if (!transA && !transB)
-      launch(ctx, kname, k_gemm_f64_big<false, false>, grid, dim3(256), 0, M, N, K, kchunk,
-             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+    if (labelsB && !transA)
+      launch(ctx, kname, k_gemm_f64_big<false, false, true>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
+    else if (labelsB)
+      launch(ctx, kname, k_gemm_f64_big<true, false, true>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
+    else if (!transA && !transB)
+      launch(ctx, kname, k_gemm_f64_big<false, false, false>, grid, dim3(256), 0, M, N, K,
+             kchunk, alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
     else if (!transA && transB)
-      launch(ctx, kname, k_gemm_f64_big<false, true>, grid, dim3(256), 0, M, N, K, kchunk,
-             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+      launch(ctx, kname, k_gemm_f64_big<false, true, false>, grid, dim3(256), 0, M, N, K,
+             kchunk, alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
     else if (transA && !transB)
-      launch(ctx, kname, k_gemm_f64_big<true, false>, grid, dim3(256), 0, M, N, K, kchunk,
-             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+      launch(ctx, kname, k_gemm_f64_big<true, false, false>, grid, dim3(256), 0, M, N, K,
+             kchunk, alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
     else
-      launch(ctx, kname, k_gemm_f64_big<true, true>, grid, dim3(256), 0, M, N, K, kchunk,
-             alpha, A, lda, shiftA, B, ldb, shiftB, beta, C, ldc, pp);
+      launch(ctx, kname, k_gemm_f64_big<true, true, false>, grid, dim3(256), 0, M, N, K, kchunk,
+             alpha, A, lda, shiftA, B, ldb, labelsB, shiftB, beta, C, ldc, pp);
     if (splits > 1)
       launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
              N, (int)splits, (const double*)part.get(), alpha, beta, C, ldc);
