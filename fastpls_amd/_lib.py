@@ -67,6 +67,15 @@ class ModelInfo(ctypes.Structure):
                 ("n_classes", ctypes.c_int32), ("ncomp", ctypes.c_int32 * 64)]
 
 
+class ModelFields(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int32), ("kmax", ctypes.c_int32), ("p", ctypes.c_int64),
+                ("m", ctypes.c_int64), ("n_ncomp", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("ncomp", ctypes.c_void_p), ("R", ctypes.c_void_p), ("Q", ctypes.c_void_p),
+                ("C_latent", ctypes.c_void_p), ("W_latent", ctypes.c_void_p),
+                ("B", ctypes.c_void_p), ("mX", ctypes.c_void_p), ("vX", ctypes.c_void_p),
+                ("mY", ctypes.c_void_p)]
+
+
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
 SIGNATURES = {
@@ -83,6 +92,7 @@ SIGNATURES = {
     "fpls_simpls_fit": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                        ctypes.c_int32, _P(Opts), _P(_vp)]),
     "fpls_model_info_get": (ctypes.c_int, [_vp, _P(ModelInfo)]),
+    "fpls_model_import": (ctypes.c_int, [_vp, _P(ModelFields), _P(_vp)]),
     "fpls_model_read": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int64]),
     "fpls_model_free": (None, [_vp]),
     "fpls_predict": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp]),

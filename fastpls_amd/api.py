@@ -617,6 +617,78 @@ def _kernel_test_matrix(spec: Dict, newdata, device: int):
     return center_kernel_test(Kt, spec["col_means"], spec["grand_mean"], device, out="device")
 
 
+def model_from_fields(fields: Dict, device: int = 0) -> FastPLS:
+    """A device model from the reference's model-list fields (R, Q, C_latent,
+    W_latent, B, mX, vX, mY, ncomp, pls_method; optional classification /
+    lev), e.g. a model fitted by the reference's CPU path: predict() then runs
+    on the device (fpls_model_import; compact_prediction_weights,
+    src/fastPLS.cpp:3737-3836)."""
+    ctx = _lib.context(device)
+    method = fields.get("pls_method", "simpls")
+    if method not in ("simpls", "simpls_fast", "plssvd"):
+        raise ValueError("pls_method must be 'simpls' or 'plssvd'")
+    keep: list = []
+
+    def arr(name, shape=None, required=False):
+        v = fields.get(name)
+        if v is None:
+            if required:
+                raise ValueError(f"model field {name} is required")
+            return None
+        a = np.asfortranarray(np.asarray(v, dtype=np.float64))
+        if shape is not None:
+            a = np.asfortranarray(a.reshape(shape, order="F"))
+        keep.append(a)
+        return a
+    R = arr("R", required=True)
+    if R.ndim != 2:
+        raise ValueError("Model `R` must be a p x kmax matrix")
+    p, kmax = R.shape
+    nc = np.ascontiguousarray(np.atleast_1d(fields["ncomp"]), dtype=np.int32)
+    keep.append(nc)
+    L = nc.size
+    mY = arr("mY", (-1,), True)
+    m = mY.size
+    f = _lib.ModelFields()
+    f.method = _lib.METHOD_PLSSVD if method == "plssvd" else _lib.METHOD_SIMPLS
+    f.kmax, f.p, f.m, f.n_ncomp = kmax, p, m, L
+    f.ncomp = nc.ctypes.data
+    f.R = R.ctypes.data
+    for name, shape in (("Q", (m, kmax)), ("C_latent", (kmax, kmax, L)),
+                        ("W_latent", (kmax, m, L)), ("B", (p, m, L))):
+        v = fields.get(name)
+        if v is not None and np.size(v):
+            a = np.asarray(v, dtype=np.float64)
+            if name == "W_latent" and a.ndim == 3 and a.shape[2] > L:
+                a = a[:, :, :L]          # W_dim[2] >= length(ncomp) in the reference
+            setattr(f, name, arr_from(a, shape, keep).ctypes.data)
+    f.mX = arr("mX", (-1,), True).ctypes.data
+    f.vX = arr("vX", (-1,), True).ctypes.data
+    f.mY = mY.ctypes.data
+    h = ctypes.c_void_p()
+    ctx.check(ctx.lib.fpls_model_import(ctx.h, ctypes.byref(f), ctypes.byref(h)))
+    model = FastPLS(ctx, h)
+    model["classification"] = bool(fields.get("classification", False))
+    model["lev"] = fields.get("lev")
+    if model["classification"] and model["lev"] is None:
+        model["lev"] = list(range(1, m + 1))
+    model["classification_rule"] = "argmax"
+    model["pls_method"] = "plssvd" if method == "plssvd" else "simpls"
+    model["predict_backend"] = "hip"
+    model["ncomp"] = nc.astype(np.int64)
+    model["m"], model["p"] = m, p
+    inf = model.info()
+    model["B_stored"] = bool(inf.b_stored)
+    model["compact_prediction"] = not bool(inf.b_stored)
+    return model
+
+
+def arr_from(a: np.ndarray, shape, keep: list) -> np.ndarray:
+    a = np.asfortranarray(a.reshape(shape, order="F"))
+    keep.append(a)
+    return a
+
+
 def gaussian_y_default_dim(Xtrain, gaussian_y_dim=None) -> int:
     """.gaussian_y_default_dim (R/main.R:1945-1954): min(ncol(X), 100)."""
     if gaussian_y_dim is None:

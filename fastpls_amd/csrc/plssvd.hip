@@ -13,6 +13,9 @@
 // Centring is applied on load (X is never rewritten); T stays resident for the
 // classifier heads.
 #include <algorithm>
+#include <cctype>
+#include <cstdlib>
+#include <string>
 #include <cmath>
 #include <vector>
 
@@ -158,11 +161,28 @@ void cross_cov(Ctx* ctx, const Model* M, const DMat& X, const DMat* Y, const int
   if (M->scaling == 2) scale_rows_div(ctx, S, p, p, m, M->vX.get());
 }
 
+// Whether a fit keeps the dense coefficient cube B (p x m x slices) beside the
+// compact factors, should_store_coefficients (src/fastPLS.cpp:63-85): the
+// caller's choice, else FASTPLS_STORE_B (always / 1 / true / yes, never / 0 /
+// false / no), else B fits in FASTPLS_STORE_B_MAX_MB (default 256, clamped to
+// 0..1048576; an unparsable value keeps the default).
 bool store_b_policy(int store_b, int64_t p, int64_t m, int n_slices) {
   if (store_b == 0) return false;
   if (store_b == 1) return true;
+  if (const char* mode = std::getenv("FASTPLS_STORE_B")) {
+    std::string v(mode);
+    for (char& ch : v) ch = (char)std::tolower((unsigned char)ch);
+    if (v == "always" || v == "1" || v == "true" || v == "yes") return true;
+    if (v == "never" || v == "0" || v == "false" || v == "no") return false;
+  }
+  long max_mb = 256;
+  if (const char* raw = std::getenv("FASTPLS_STORE_B_MAX_MB")) {
+    char* end = nullptr;
+    const long v = std::strtol(raw, &end, 10);
+    if (end != raw) max_mb = std::min<long>(1048576, std::max<long>(0, v));
+  }
   const double mb = (double)p * (double)m * (double)std::max(n_slices, 1) * 8.0 / (1024.0 * 1024.0);
-  return mb <= 256.0;  // FASTPLS_STORE_B_MAX_MB default (src/fastPLS.cpp:63-85)
+  return mb <= (double)max_mb;
 }
 
 // Compute T = Xs R (kmax columns) into the model, and G = T^T T.

@@ -23,6 +23,9 @@
  *                             .plssvd_label_aware_stream_model (labels) R/main.R:513-655
  *   fpls_simpls_fit           pls_model2_fast            src/fastPLS.cpp:2454-2778
  *   fpls_model_read           the Rcpp::List fields returned by the fits
+ *   fpls_model_import         those fields back into a device model (predict()
+ *                             on a model fitted by the reference; the B_stored
+ *                             policy src/fastPLS.cpp:63-90)
  *   fpls_predict              pls_predict                src/fastPLS.cpp:3211-3445
  *   fpls_project              .fastpls_latent_scores     R/main.R:965-1017
  *   fpls_lda_train_prefix     lda_project_train_prefix_cpp src/fastPLS.cpp:1930-1957
@@ -187,6 +190,30 @@ typedef struct fpls_model_info {
 #define FPLS_FIELD_R_OFFSET  15 /* kmax      mX R_predict                    */
 
 int fpls_model_info_get(const fpls_model* model, fpls_model_info* info);
+
+/* A fitted model in the reference's list format (host FP64, column-major),
+   for fpls_model_import. */
+typedef struct fpls_model_fields {
+  int32_t method;          /* FPLS_METHOD_SIMPLS | FPLS_METHOD_PLSSVD          */
+  int32_t kmax;            /* columns of R                                     */
+  int64_t p, m;
+  int32_t n_ncomp;
+  int32_t reserved;
+  const int32_t* ncomp;    /* n_ncomp slice orders, 1..kmax                    */
+  const double* R;         /* p x kmax                                         */
+  const double* Q;         /* m x kmax (SIMPLS required; PLSSVD with C_latent) */
+  const double* C_latent;  /* kmax x kmax x n_ncomp (PLSSVD) or NULL          */
+  const double* W_latent;  /* kmax x m x n_ncomp (PLSSVD) or NULL             */
+  const double* B;         /* p x m x n_ncomp or NULL                          */
+  const double* mX;        /* p                                                */
+  const double* vX;        /* p                                                */
+  const double* mY;        /* m                                                */
+} fpls_model_fields;
+/* Device model from those fields: fpls_predict, fpls_project and
+   fpls_class_predict_topk then work on it (the LDA / cKNN heads are not part
+   of the fields).  Errors follow compact_prediction_weights: SIMPLS needs Q,
+   PLSSVD W_latent or C_latent + Q, and every ncomp within 1..kmax. */
+int fpls_model_import(fpls_ctx* ctx, const fpls_model_fields* fields, fpls_model** out);
 int fpls_model_read(fpls_ctx* ctx, const fpls_model* model, int32_t field, double* host_out,
                     int64_t n_elem);
 void fpls_model_free(fpls_model* model);
