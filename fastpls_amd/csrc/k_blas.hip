@@ -400,8 +400,12 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
            N, 0, zero.get(), alpha, beta, C, ldc);
     return;
   }
-  if ((!labelsB || !transB) && M >= 128 && N >= 64 && K >= 64) {
-    const int64_t tiles = ((M + GB_M - 1) / GB_M) * ((N + GB_N - 1) / GB_N);
+  // the large tile pays when there are enough of them (or a long K to split);
+  // small factor products stay on the 64 x 64 tiles below
+  const int64_t big_tiles = ((M + GB_M - 1) / GB_M) * ((N + GB_N - 1) / GB_N);
+  if ((!labelsB || !transB) && M >= 128 && N >= 64 && K >= 64 &&
+      (big_tiles >= ctx->num_cus / 2 || K >= 4096)) {
+    const int64_t tiles = big_tiles;
     int64_t splits = 1;
     const int64_t want = 2 * ctx->num_cus;
     if (tiles < want && K >= 512) {
