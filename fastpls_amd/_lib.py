@@ -26,7 +26,7 @@ F32, F64 = 0, 1
 HOST, DEVICE = 0, 1
 SVD_EXACT, SVD_RSVD = 3, 4
 PREC_FP32, PREC_BF16X3 = 0, 1
-METHOD_PLSSVD, METHOD_SIMPLS = 1, 3
+METHOD_PLSSVD, METHOD_SIMPLS, METHOD_OPLS, METHOD_KERNELPLS = 1, 3, 4, 5
 
 FIELDS = {
     "R": 1, "Q": 2, "mX": 3, "vX": 4, "mY": 5, "B": 6, "C_latent": 7, "W_latent": 8,
@@ -56,7 +56,8 @@ class Opts(ctypes.Structure):
                 ("seed", ctypes.c_uint32), ("inc_iters", ctypes.c_int32),
                 ("precision", ctypes.c_int32), ("fit", ctypes.c_int32),
                 ("store_b", ctypes.c_int32), ("label_aware", ctypes.c_int32),
-                ("xprod_route", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("xprod_route", ctypes.c_int32), ("opls_north", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class ModelInfo(ctypes.Structure):

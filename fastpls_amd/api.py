@@ -926,9 +926,12 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
                    classifier="argmax", lda_ridge=1e-8, candidate_knn_k=10, candidate_tau=0.2,
                    candidate_alpha=0.75, candidate_top_m=20, seed=1, rsvd_oversample=10,
                    rsvd_power=1, precision="fp32", return_scores=False, device=0,
-                   engine="auto") -> Dict:
+                   engine="auto", north=1) -> Dict:
     """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151), leave-one-group-out
-    over the distinct values of ``constrain``.
+    over the distinct values of ``constrain``.  ``method``: "plssvd",
+    "simpls", "opls" (the filter with ``north`` components per training fold,
+    then SIMPLS with scaling "none") or "kernelpls" (the direct SIMPLS core,
+    as the compiled CV fits it).
 
     ``engine``: "folds" fits the folds one after another (every method and
     classifier); "batched" runs all folds as one batched call (SIMPLS, argmax
@@ -968,6 +971,7 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     opts = _lib.default_opts(scaling=SCALINGS[scaling], seed=int(seed) & 0xFFFFFFFF,
                              oversample=int(rsvd_oversample), power=int(rsvd_power),
                              precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32)
+    opts.opls_north = int(north)
     ngroups = int(np.unique(cons).size)
     fold = np.empty(n, dtype=np.int32)
     status = np.zeros(max(ngroups, 1), dtype=np.int32)
@@ -975,7 +979,12 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     cls = np.zeros((n, L), dtype=np.int32, order="F") if classification else None
     want_scores = (not classification) or return_scores
     yp = np.zeros((n, m, L), dtype=np.float64, order="F") if want_scores else None
-    meth = _lib.METHOD_PLSSVD if method == "plssvd" else _lib.METHOD_SIMPLS
+    methods = {"plssvd": _lib.METHOD_PLSSVD, "simpls": _lib.METHOD_SIMPLS,
+               "simpls_fast": _lib.METHOD_SIMPLS, "opls": _lib.METHOD_OPLS,
+               "kernelpls": _lib.METHOD_KERNELPLS}
+    if method not in methods:
+        raise ValueError("method must be 'plssvd', 'simpls', 'opls' or 'kernelpls'")
+    meth = methods[method]
     ctx.check(ctx.lib.fpls_cv_predict(
         ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
         ctypes.byref(yv) if yv is not None else None, C, cons.ctypes.data, nc_ptr, L, -1, meth,

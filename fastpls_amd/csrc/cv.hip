@@ -93,8 +93,12 @@ extern "C" int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
     fpls_opts o;
     fpls_opts_default(&o);
     if (opts) o = *opts;
-    FPLS_REQUIRE(method == FPLS_METHOD_PLSSVD || method == FPLS_METHOD_SIMPLS, FPLS_ERR_ARG,
-                 "method must be 1 = plssvd or 3 = simpls_fast");
+    FPLS_REQUIRE(method == FPLS_METHOD_PLSSVD || method == FPLS_METHOD_SIMPLS ||
+                     method == FPLS_METHOD_OPLS || method == FPLS_METHOD_KERNELPLS,
+                 FPLS_ERR_ARG, "method must be 1=plssvd, 3=simpls_fast, 4=opls, or 5=kernelpls");
+    // kernel PLS folds fit the direct SIMPLS core (src/fastPLS.cpp:6887-6891)
+    const bool opls = method == FPLS_METHOD_OPLS;
+    if (method == FPLS_METHOD_KERNELPLS) method = FPLS_METHOD_SIMPLS;
     DBuf<unsigned char> hx, hy;
     DMat X = stage(c, Xm, hx, "cv: X");
     const int64_t n = X.rows;
@@ -234,6 +238,26 @@ extern "C" int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       of.seed = o.seed + (uint32_t)f;
       of.fit = 0;
       of.store_b = 0;
+      // OPLS: filter the training fold, apply it to the held-out rows, fit
+      // SIMPLS with scaling "none" (src/fastPLS.cpp:6876-6886)
+      DBuf<double> fX, fXte, fmX, fvX, fWo, fPo;
+      if (opls) {
+        int used = 0;
+        fX = DBuf<double>(c, (size_t)std::max<int64_t>(Xtr.rows * Xtr.cols, 1));
+        opls_filter_dev(c, Xtr, classification ? nullptr : &Ytr, ytr, m, std::max(o.opls_north, 0),
+                        o.scaling, fX.get(), std::max<int64_t>(Xtr.rows, 1), fmX, fvX, fWo, fPo,
+                        used);
+        fXte = DBuf<double>(c, (size_t)std::max<int64_t>(Xte.rows * Xte.cols, 1));
+        opls_apply_dev(c, Xte, fmX.get(), fvX.get(), fWo.get(), fPo.get(), used, fXte.get(),
+                       std::max<int64_t>(Xte.rows, 1));
+        Xtr.p = fX.get();
+        Xtr.dtype = FPLS_F64;
+        Xtr.ld = std::max<int64_t>(Xtr.rows, 1);
+        Xte.p = fXte.get();
+        Xte.dtype = FPLS_F64;
+        Xte.ld = std::max<int64_t>(Xte.rows, 1);
+        of.scaling = 3;
+      }
       if (method == FPLS_METHOD_PLSSVD)
         plssvd_fit_dev(c, &M, Xtr, Ytr, ytr, m, nc.data(), L, of);
       else

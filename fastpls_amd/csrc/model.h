@@ -93,6 +93,16 @@ void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols
 // Orthonormal basis of the columns of Y (rows x l) by shifted CholeskyQR3, in place.
 void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l);
 
+// OPLS filter on a device matrix (opls.hip): Xout (FP64, ld ldo) = X
+// preprocessed and deflated; mX, vX, Wo, Po (p x north, `used` columns valid)
+// on the device.  The response is *Y (dense) or the one-hot of lab (m classes).
+void opls_filter_dev(Ctx* c, const DMat& X, const DMat* Y, const int32_t* lab, int64_t m,
+                     int north, int scaling, double* Xout, int64_t ldo, DBuf<double>& mX,
+                     DBuf<double>& vX, DBuf<double>& Wo, DBuf<double>& Po, int& used);
+// The stored filter (device arrays) applied to X into Xout.
+void opls_apply_dev(Ctx* c, const DMat& X, const double* mX, const double* vX, const double* Wo,
+                    const double* Po, int north, double* Xout, int64_t ldo);
+
 // Fits on device-resident inputs (plssvd.hip, simpls.hip).
 void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
                     const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o);

@@ -309,6 +309,116 @@ DMat dview(const double* p, int64_t rows, int64_t cols, int64_t ld) {
 
 }  // namespace
 
+// The filter on a device matrix: X (n x p, FP32 / FP64) preprocessed and
+// deflated into Xout (FP64, ld ldo); the response is Y (dense) or the one-hot
+// of the labels lab (m classes).  Outputs mX, vX (p), Wo, Po (p x north,
+// columns 0..used-1 valid) on the device.
+void opls_filter_dev(Ctx* c, const DMat& X, const DMat* Y, const int32_t* lab, int64_t m,
+                     int north, int scaling, double* Xout, int64_t ldo, DBuf<double>& mX,
+                     DBuf<double>& vX, DBuf<double>& Wo, DBuf<double>& Po, int& used) {
+  const int64_t n = X.rows, p = X.cols;
+  // preprocessing (scaling 1 centre, 2 centre + sd, 3 none)
+  mX = DBuf<double>(c, std::max<int64_t>(p, 1));
+  vX = DBuf<double>(c, std::max<int64_t>(p, 1));
+  fill_f64(c, mX.get(), 0.0, p);
+  fill_f64(c, vX.get(), 1.0, p);
+  if (scaling < 3 && n > 0) {
+    DBuf<double> sums(c, std::max<int64_t>(p, 1)), ssd(c, std::max<int64_t>(p, 1));
+    col_sums(c, X, sums.get());
+    launch(c, "opls_moments", k_op_moments, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
+           sums.get(), (const double*)nullptr, n, p, 1, 0, mX.get(), vX.get());
+    if (scaling == 2) {
+      col_ssd(c, X, mX.get(), ssd.get());
+      launch(c, "opls_moments", k_op_moments, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
+             sums.get(), ssd.get(), n, p, 0, 1, mX.get(), vX.get());
+    }
+    sync(c);
+  }
+  prep_into(c, X, scaling < 3 ? mX.get() : nullptr, scaling == 2 ? vX.get() : nullptr, Xout, ldo);
+  // centred response: dense Y (FP64 copy) or the one-hot of the labels
+  DBuf<double> Yd, mY(c, std::max<int64_t>(m, 1)), ysum(c, std::max<int64_t>(m, 1));
+  fill_f64(c, mY.get(), 0.0, m);
+  if (n > 0) {
+    if (Y) {
+      Yd = DBuf<double>(c, (size_t)std::max<int64_t>(n * m, 1));
+      prep_into(c, *Y, nullptr, nullptr, Yd.get(), n);
+      col_sums(c, dview(Yd.get(), n, m, n), ysum.get());
+    } else {
+      // the one-hot column sums are the class counts
+      DBuf<double> ones(c, n);
+      fill_f64(c, ones.get(), 1.0, n);
+      gemm_f64(c, true, false, 1, m, n, 1.0, ones.get(), n, nullptr, nullptr, 0, nullptr, 0.0,
+               ysum.get(), 1, lab);
+      sync(c);
+    }
+    // mY = column sums / n
+    launch(c, "opls_moments", k_op_moments, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
+           (const double*)ysum.get(), (const double*)nullptr, n, m, 1, 0, mY.get(),
+           (double*)nullptr);
+  }
+  DBuf<double> S(c, (size_t)std::max<int64_t>(p * m, 1));
+  DBuf<double> w(c, std::max<int64_t>(p, 1)), pv(c, std::max<int64_t>(p, 1)),
+      t(c, std::max<int64_t>(n, 1)), ss(c, 1);
+  Wo = DBuf<double>(c, (size_t)std::max<int64_t>(p * north, 1));
+  Po = DBuf<double>(c, (size_t)std::max<int64_t>(p * north, 1));
+  DBuf<int32_t> flag(c, 1);
+  const DMat Xw = dview(Xout, n, p, ldo);
+  used = 0;
+  for (int a = 0; a < north && n > 0 && p > 0 && m > 0; ++a) {
+    fill_i32(c, flag.get(), 0, 1);
+    // S = X^T (Y - 1 mY^T)
+    if (Y)
+      gemm_f64(c, true, false, p, m, n, 1.0, Xout, ldo, nullptr, Yd.get(), n, mY.get(), 0.0,
+               S.get(), p);
+    else
+      gemm_f64(c, true, false, p, m, n, 1.0, Xout, ldo, nullptr, nullptr, 0, mY.get(), 0.0,
+               S.get(), p, lab);
+    SvdOut sv;
+    svd_exact_dev(c, S.get(), p, p, m, 1, sv, true);
+    copy_f64(c, w.get(), sv.U.get(), p);
+    launch(c, "opls_unit", k_op_unit, dim3(1), dim3(OP_T), 0, w.get(), p, flag.get());
+    // t = X w, p = X^T t / t't
+    gemv_n(c, Xw, nullptr, w.get(), t.get());
+    gemv_t(c, dview(t.get(), n, 1, n), nullptr, t.get(), ss.get());
+    gemv_t(c, Xw, nullptr, t.get(), pv.get());
+    launch(c, "opls_div", k_op_div, dim3(opg(p)), dim3(256), 0, pv.get(), p,
+           (const double*)ss.get(), flag.get());
+    double* wo = Wo.get() + (size_t)used * p;
+    double* po = Po.get() + (size_t)used * p;
+    launch(c, "opls_orth", k_op_orth, dim3(1), dim3(OP_T), 0, (const double*)w.get(),
+           (const double*)pv.get(), p, wo, flag.get());
+    // t_o = X w_o, p_o = X^T t_o / t_o't_o, X -= t_o p_o^T
+    gemv_n(c, Xw, nullptr, wo, t.get());
+    gemv_t(c, dview(t.get(), n, 1, n), nullptr, t.get(), ss.get());
+    gemv_t(c, Xw, nullptr, t.get(), po);
+    launch(c, "opls_div", k_op_div, dim3(opg(p)), dim3(256), 0, po, p, (const double*)ss.get(),
+           flag.get());
+    launch(c, "opls_deflate", k_op_deflate, dim3(opg(n * p)), dim3(256), 0, Xout, ldo, n, p,
+           (const double*)t.get(), (const double*)po, (const int32_t*)flag.get());
+    int32_t stop = 0;
+    to_host_i32(c, &stop, flag.get(), 1);
+    if (stop) break;
+    ++used;
+  }
+}
+
+// Xout (FP64, ld ldo) = the stored filter (device mX, vX, Wo, Po) on X.
+void opls_apply_dev(Ctx* c, const DMat& X, const double* mX, const double* vX, const double* Wo,
+                    const double* Po, int north, double* Xout, int64_t ldo) {
+  const int64_t n = X.rows, p = X.cols;
+  prep_into(c, X, mX, vX, Xout, ldo);
+  if (north > 0 && n > 0) {
+    DBuf<double> t(c, n);
+    const DMat Xw = dview(Xout, n, p, ldo);
+    for (int a = 0; a < north; ++a) {
+      gemv_n(c, Xw, nullptr, Wo + (size_t)a * p, t.get());
+      launch(c, "opls_deflate", k_op_deflate, dim3(opg(n * p)), dim3(256), 0, Xout, ldo, n, p,
+             (const double*)t.get(), Po + (size_t)a * p, (const int32_t*)nullptr);
+    }
+    sync(c);
+  }
+}
+
 }  // namespace fpls
 
 using namespace fpls;
@@ -341,88 +451,10 @@ extern "C" int fpls_opls_filter(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_ma
       m = n_classes;
     }
     OutMat W = out_mat(c, X_out, n, p, "opls_filter: X_out");
-    // preprocessing (scaling 1 centre, 2 centre + sd, 3 none)
-    DBuf<double> mX(c, std::max<int64_t>(p, 1)), vX(c, std::max<int64_t>(p, 1));
-    fill_f64(c, mX.get(), 0.0, p);
-    fill_f64(c, vX.get(), 1.0, p);
-    if (scaling < 3 && n > 0) {
-      DBuf<double> sums(c, std::max<int64_t>(p, 1)), ssd(c, std::max<int64_t>(p, 1));
-      col_sums(c, X, sums.get());
-      launch(c, "opls_moments", k_op_moments, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
-             sums.get(), (const double*)nullptr, n, p, 1, 0, mX.get(), vX.get());
-      if (scaling == 2) {
-        col_ssd(c, X, mX.get(), ssd.get());
-        launch(c, "opls_moments", k_op_moments, dim3((unsigned)((p + 255) / 256)), dim3(256), 0,
-               sums.get(), ssd.get(), n, p, 0, 1, mX.get(), vX.get());
-      }
-    }
-    prep_into(c, X, scaling < 3 ? mX.get() : nullptr, scaling == 2 ? vX.get() : nullptr, W.dev,
-              W.ld);
-    // centred response: dense Y (FP64 copy) or the one-hot of the labels
-    DBuf<double> Yd, mY(c, std::max<int64_t>(m, 1)), ysum(c, std::max<int64_t>(m, 1));
-    fill_f64(c, mY.get(), 0.0, m);
-    if (n > 0) {
-      if (Ym) {
-        Yd = DBuf<double>(c, (size_t)std::max<int64_t>(n * m, 1));
-        prep_into(c, Y, nullptr, nullptr, Yd.get(), n);
-        col_sums(c, dview(Yd.get(), n, m, n), ysum.get());
-      } else {
-        // the one-hot column sums are the class counts
-        DBuf<double> ones(c, n);
-        fill_f64(c, ones.get(), 1.0, n);
-        gemm_f64(c, true, false, 1, m, n, 1.0, ones.get(), n, nullptr, nullptr, 0, nullptr, 0.0,
-                 ysum.get(), 1, lab);
-        sync(c);
-      }
-      // mY = column sums / n
-      launch(c, "opls_moments", k_op_moments, dim3((unsigned)((m + 255) / 256)), dim3(256), 0,
-             (const double*)ysum.get(), (const double*)nullptr, n, m, 1, 0, mY.get(),
-             (double*)nullptr);
-    }
-    DBuf<double> S(c, (size_t)std::max<int64_t>(p * m, 1));
-    DBuf<double> w(c, std::max<int64_t>(p, 1)), pv(c, std::max<int64_t>(p, 1)),
-        t(c, std::max<int64_t>(n, 1)), ss(c, 1);
-    DBuf<double> Wo(c, (size_t)std::max<int64_t>(p * north, 1)),
-        Po(c, (size_t)std::max<int64_t>(p * north, 1));
-    DBuf<int32_t> flag(c, 1);
-    const DMat Xw = dview(W.dev, n, p, W.ld);
+    DBuf<double> mX, vX, Wo, Po;
     int used = 0;
-    for (int a = 0; a < north && n > 0 && p > 0 && m > 0; ++a) {
-      fill_i32(c, flag.get(), 0, 1);
-      // S = X^T (Y - 1 mY^T)
-      if (Ym)
-        gemm_f64(c, true, false, p, m, n, 1.0, W.dev, W.ld, nullptr, Yd.get(), n, mY.get(), 0.0,
-                 S.get(), p);
-      else
-        gemm_f64(c, true, false, p, m, n, 1.0, W.dev, W.ld, nullptr, nullptr, 0, mY.get(), 0.0,
-                 S.get(), p, lab);
-      SvdOut sv;
-      svd_exact_dev(c, S.get(), p, p, m, 1, sv, true);
-      copy_f64(c, w.get(), sv.U.get(), p);
-      launch(c, "opls_unit", k_op_unit, dim3(1), dim3(OP_T), 0, w.get(), p, flag.get());
-      // t = X w, p = X^T t / t't
-      gemv_n(c, Xw, nullptr, w.get(), t.get());
-      gemv_t(c, dview(t.get(), n, 1, n), nullptr, t.get(), ss.get());
-      gemv_t(c, Xw, nullptr, t.get(), pv.get());
-      launch(c, "opls_div", k_op_div, dim3(opg(p)), dim3(256), 0, pv.get(), p,
-             (const double*)ss.get(), flag.get());
-      double* wo = Wo.get() + (size_t)used * p;
-      double* po = Po.get() + (size_t)used * p;
-      launch(c, "opls_orth", k_op_orth, dim3(1), dim3(OP_T), 0, (const double*)w.get(),
-             (const double*)pv.get(), p, wo, flag.get());
-      // t_o = X w_o, p_o = X^T t_o / t_o't_o, X -= t_o p_o^T
-      gemv_n(c, Xw, nullptr, wo, t.get());
-      gemv_t(c, dview(t.get(), n, 1, n), nullptr, t.get(), ss.get());
-      gemv_t(c, Xw, nullptr, t.get(), po);
-      launch(c, "opls_div", k_op_div, dim3(opg(p)), dim3(256), 0, po, p, (const double*)ss.get(),
-             flag.get());
-      launch(c, "opls_deflate", k_op_deflate, dim3(opg(n * p)), dim3(256), 0, W.dev, W.ld, n, p,
-             (const double*)t.get(), (const double*)po, (const int32_t*)flag.get());
-      int32_t stop = 0;
-      to_host_i32(c, &stop, flag.get(), 1);
-      if (stop) break;
-      ++used;
-    }
+    opls_filter_dev(c, X, Ym ? &Y : nullptr, lab, m, north, scaling, W.dev, W.ld, mX, vX, Wo, Po,
+                    used);
     if (north_used) *north_used = used;
     if (mX_out) to_host(c, mX_out, mX.get(), p);
     if (vX_out) to_host(c, vX_out, vX.get(), p);
@@ -447,19 +479,14 @@ extern "C" int fpls_opls_apply(fpls_ctx* ctx, const fpls_mat* Xm, const double* 
     DBuf<double> dm(c, std::max<int64_t>(p, 1)), dv(c, std::max<int64_t>(p, 1));
     h2d(c, dm.get(), mX, sizeof(double) * p);
     h2d(c, dv.get(), vX, sizeof(double) * p);
-    prep_into(c, X, dm.get(), dv.get(), W.dev, W.ld);
-    if (north > 0 && n > 0) {
-      DBuf<double> Wo(c, (size_t)p * north), Po(c, (size_t)p * north), t(c, n);
+    DBuf<double> Wo, Po;
+    if (north > 0) {
+      Wo = DBuf<double>(c, (size_t)p * north);
+      Po = DBuf<double>(c, (size_t)p * north);
       h2d(c, Wo.get(), W_orth, sizeof(double) * p * north);
       h2d(c, Po.get(), P_orth, sizeof(double) * p * north);
-      const DMat Xw = dview(W.dev, n, p, W.ld);
-      for (int a = 0; a < north; ++a) {
-        gemv_n(c, Xw, nullptr, Wo.get() + (size_t)a * p, t.get());
-        launch(c, "opls_deflate", k_op_deflate, dim3(opg(n * p)), dim3(256), 0, W.dev, W.ld, n,
-               p, (const double*)t.get(), (const double*)(Po.get() + (size_t)a * p),
-               (const int32_t*)nullptr);
-      }
     }
+    opls_apply_dev(c, X, dm.get(), dv.get(), Wo.get(), Po.get(), north, W.dev, W.ld);
     finish_out(c, W, n, p);
   });
 }

@@ -215,6 +215,7 @@ void fpls_opts_default(fpls_opts* o) {
   o->precision = FPLS_PREC_FP32;
   o->fit = 0;
   o->store_b = -1;
+  o->opls_north = 1;
 }
 
 int fpls_available(void) {

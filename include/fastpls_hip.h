@@ -122,7 +122,9 @@ typedef struct fpls_opts {
   int32_t xprod_route; /* fpls_rsvd_xprod: 0 matrix free (a_times/at_times,
                           src/fastPLS.cpp:550-623), 1 form S = X^T Y once
                           and sketch S (one pass over X)                      */
-  int32_t reserved[5];
+  int32_t opls_north;  /* fpls_cv_predict with FPLS_METHOD_OPLS: orthogonal
+                          components removed per training fold (default 1)   */
+  int32_t reserved[4];
 } fpls_opts;
 
 void fpls_opts_default(fpls_opts* opts);
@@ -158,6 +160,11 @@ int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const f
 /* ---- model access -------------------------------------------------------- */
 #define FPLS_METHOD_PLSSVD 1
 #define FPLS_METHOD_SIMPLS 3
+/* fpls_cv_predict only (the reference's method ids 4 and 5,
+   src/fastPLS.cpp:6664-6670): OPLS filter per training fold, then SIMPLS with
+   scaling "none"; kernel PLS, whose compiled CV fits the direct SIMPLS core. */
+#define FPLS_METHOD_OPLS 4
+#define FPLS_METHOD_KERNELPLS 5
 
 typedef struct fpls_model_info {
   int32_t method;      /* FPLS_METHOD_*                                      */

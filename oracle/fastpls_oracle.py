@@ -913,11 +913,15 @@ def nearest_code_classes(Z, codes) -> np.ndarray:
 
 def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False,
                    n_classes=None, classifier="argmax", scaling=1, seed=1, lda_ridge=1e-8,
-                   knn_k=10, tau=0.2, alpha=0.75, top_m=20):
+                   knn_k=10, tau=0.2, alpha=0.75, top_m=20, north=1):
     """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out
     folds (group_fold(j) = j over the sorted distinct constrain values, :6720-6736).
     Y holds 1-based labels when ``classification``.  Returns fold (1-based),
-    status, class_pred (n x L) and Ypred (n x m x L, argmax / regression)."""
+    status, class_pred (n x L) and Ypred (n x m x L, argmax / regression).
+    method "opls" filters each training fold (opls_filter_cpp with ``north``
+    components) and applies the filter to the held-out rows before a SIMPLS
+    fit with scaling "none"; "kernelpls" is the SIMPLS fit (its linear kernel,
+    :6874-6891)."""
     X = np.asarray(X, dtype=np.float64)
     n = X.shape[0]
     cons = np.asarray(constrain).ravel()
@@ -961,13 +965,19 @@ def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False
                 Ypred[te] = row[None, :, None]
                 class_pred[te] = active[0] if active.size == 1 else 1
                 continue
+        Xtr, Xte, fit_scaling = X[tr], X[te], scaling
+        if method == "opls":
+            filt = opls_filter(Xtr, Yd[tr], max(int(north), 0), scaling)
+            Xtr = filt["X"]
+            Xte = opls_apply_filter(Xte, filt["mX"], filt["vX"], filt["W_orth"], filt["P_orth"])
+            fit_scaling = 3
         fit = pls_model1 if method == "plssvd" else pls_model2_fast
-        mod = fit(X[tr], Yd[tr], nc, scaling, False, "rsvd", 10, 1, seed + f)
+        mod = fit(Xtr, Yd[tr], nc, fit_scaling, False, "rsvd", 10, 1, seed + f)
         if classification and classifier in ("lda", "cknn"):
             attach_latent_projection_cache(mod)
             kmax = min(int(nc.max()), mod["R"].shape[1])
-            Ttr = latent_scores(mod, X[tr], kmax)
-            Tte = latent_scores(mod, X[te], kmax)
+            Ttr = latent_scores(mod, Xtr, kmax)
+            Tte = latent_scores(mod, Xte, kmax)
             if classifier == "lda":
                 ks = [min(int(k), kmax) for k in nc]
                 lda = lda_train_prefix(Ttr, y[tr], C, ks, lda_ridge)
@@ -983,7 +993,7 @@ def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False
                     r = candidate_knn_predict(b, a, y[tr], cen, None, 1, top_m, knn_k, tau, alpha)
                     class_pred[te, s] = r["top_index"][:, 0]
         else:
-            yp = pls_predict(mod, X[te])
+            yp = pls_predict(mod, Xte)
             Ypred[te] = yp
             if classification:
                 class_pred[te] = (np.argmax(yp, axis=1) + 1).astype(np.int32)

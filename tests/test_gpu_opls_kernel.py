@@ -156,3 +156,30 @@ def test_large_fp64_products_through_the_linear_kernel(ctx, n1, n2, p):
     K = F.kernel_matrix(A, B, "linear")
     ref = A @ B.T
     assert np.abs(K - ref).max() <= 1e-13 * np.sqrt(p) * np.abs(A).max() * np.abs(B).max()
+
+
+@pytest.mark.parametrize("method", ["opls", "kernelpls"])
+def test_cv_with_opls_and_kernelpls_methods(ctx, method):
+    """pls_cv_predict_compiled methods 4 (OPLS filter per training fold, then
+    SIMPLS with scaling none) and 5 (the direct SIMPLS core), leave-one-group-
+    out, FP64 regression against the oracle's fold loop; and the LDA head on
+    OPLS-filtered folds."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(41)
+    n, p, m = 150, 30, 2
+    X = rng.standard_normal((n, p)) + 0.5
+    Y = X[:, :4] @ rng.standard_normal((4, m)) + 0.2 * rng.standard_normal((n, m))
+    g = np.repeat(np.arange(5), 30)
+    rng.shuffle(g)
+    res = F.pls_cv_predict(X, Y, g, ncomp=[1, 2, 3], method=method, north=2)
+    ref = O.pls_cv_predict(X, Y, g, [1, 2, 3], method=method, north=2)
+    np.testing.assert_array_equal(res["status"], ref["status"])
+    assert rel(res["Ypred"], ref["Ypred"]) < 1e-8
+    if method == "opls":
+        y = rng.integers(1, 4, n).astype(np.int32)
+        X[:, 0] += y
+        rc = F.pls_cv_predict(X, y, g, ncomp=[2, 3], method="opls", north=1, classifier="lda")
+        rr = O.pls_cv_predict(X, y, g, [2, 3], method="opls", north=1, classification=True,
+                              n_classes=3, classifier="lda")
+        np.testing.assert_array_equal(rc["status"], rr["status"])
+        assert np.mean(rc["class_pred"] == rr["class_pred"]) > 0.99
