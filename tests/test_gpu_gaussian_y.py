@@ -118,3 +118,28 @@ def test_gaussian_y_argument_errors(ctx):
         F.prepare_gaussian_y(np.ones((10, 3)), X, True, 0)
     with pytest.raises(ValueError):
         F.pls(X, np.array(list("ab") * 5), ncomp=1, gaussian_y=True, classifier="lda")
+
+
+def test_gaussian_y_through_opls_and_kernelpls(ctx):
+    """tests/testthat/test-gaussian-y.R:89-122 with this library's draws: the
+    compressed response through the OPLS filter and kernel PLS; the OPLS
+    prediction equals the oracle pipeline (filter on the original response,
+    SIMPLS on the encoded one, decode)."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(122)
+    X = rng.standard_normal((55, 10))
+    Y = rng.standard_normal((55, 18))
+    Xt = rng.standard_normal((8, 10))
+    Yt = rng.standard_normal((8, 18))
+    of = F.pls(X, Y, Xt, Yt, ncomp=2, method="opls", gaussian_y=True, gaussian_y_dim=6, seed=3)
+    kf = F.pls(X, Y, Xt, Yt, ncomp=2, method="kernelpls", kernel="linear", gaussian_y=True,
+               gaussian_y_dim=6, seed=3)
+    assert of["Ypred"].shape == (8, 18, 1) and kf["Ypred"].shape == (8, 18, 1)
+    assert of["gaussian_y"] and kf["gaussian_y"]
+    enc = O.prepare_gaussian_y_regression(Y, 6, 3)
+    filt = O.opls_filter(X, Y, 1, 1)
+    mod = O.pls_model2_fast(filt["X"], enc["Z"], [2], 3, False, "rsvd", 10, 1, 3)
+    zt = O.pls_predict(mod, O.opls_apply_filter(Xt, filt["mX"], filt["vX"], filt["W_orth"],
+                                                filt["P_orth"]))
+    ref = O.decode_gaussian_y_regression(zt[:, :, 0], enc["decoder"], enc["y_center"])
+    np.testing.assert_allclose(of["Ypred"][:, :, 0], ref, rtol=1e-7, atol=1e-8)
