@@ -398,7 +398,7 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
         res.update(class_predict_topk(model, newdata, None, top))
         return res
     m = inf.m
-    yp = np.empty((w, m, L), dtype=np.float64, order="F")
+    yp = _host_f64(w, m, L)
     ctx.check(ctx.lib.fpls_predict(ctx.h, model.handle, ctypes.byref(X), yp.ctypes.data))
     if model.get("gaussian_y"):
         # .decode_gaussian_y_cube (R/main.R:2099-2118) after the prediction
@@ -420,6 +420,21 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
                 q2.append(1.0 - press / tss)
             res["Q2Y"] = np.array(q2)
     return res
+
+
+def _host_f64(*shape: int) -> np.ndarray:
+    """A Fortran-order FP64 host array for a large result copied from HBM:
+    pinned (page-locked) when torch is loaded, so the copy runs at PCIe speed
+    instead of through the runtime's pageable staging."""
+    if int(np.prod(shape)) * 8 >= (4 << 20):
+        try:
+            import torch
+            if torch.cuda.is_available():
+                t = torch.empty(tuple(reversed(shape)), dtype=torch.float64, pin_memory=True)
+                return t.numpy().T
+        except (ImportError, RuntimeError):
+            pass
+    return np.empty(shape, dtype=np.float64, order="F")
 
 
 def _device_f64(rows: int, cols: int, device: int):
@@ -763,9 +778,10 @@ def _attach_gaussian_y(model: "FastPLS", yprep: Dict) -> None:
     model["classification_rule"] = "argmax"
 
 
-def decode_gaussian_y_matrix(spec: Dict, Z, device: int = 0) -> np.ndarray:
+def decode_gaussian_y_matrix(spec: Dict, Z, device: int = 0, out=None) -> np.ndarray:
     """.decode_gaussian_y_matrix (R/main.R:2077-2096): regression predictions
-    (n x m) or class scores 2 Z codes^T - ||code||^2 (n x C)."""
+    (n x m) or class scores 2 Z codes^T - ||code||^2 (n x C).  ``out``: an
+    n x m column-major FP64 host array to decode into (regression)."""
     ctx = _lib.context(device)
     keep: list = []
     Zm = as_mat(np.asarray(Z, dtype=np.float64) if not _lib._is_torch(Z) else Z, keep)
@@ -781,7 +797,9 @@ def decode_gaussian_y_matrix(spec: Dict, Z, device: int = 0) -> np.ndarray:
     dec = np.asfortranarray(spec["decoder"], dtype=np.float64)
     cen = np.ascontiguousarray(spec["y_center"], dtype=np.float64)
     m = dec.shape[1]
-    Y = np.empty((n, m), order="F")
+    Y = _host_f64(n, m) if out is None else out
+    if Y.shape != (n, m) or not Y.flags.f_contiguous:
+        raise ValueError("out must be a column-major n x m array")
     Ym = _lib.Mat(Y.ctypes.data, _lib.F64, _lib.HOST, n, m, max(n, 1))
     ctx.check(ctx.lib.fpls_gaussian_y_decode(ctx.h, ctypes.byref(Zm), dec.ctypes.data, m,
                                              cen.ctypes.data, ctypes.byref(Ym)))
@@ -793,9 +811,15 @@ def decode_gaussian_y_cube(spec: Dict, Zcube, device: int = 0) -> np.ndarray:
     Zc = np.asarray(Zcube, dtype=np.float64)
     if Zc.ndim == 2:
         return decode_gaussian_y_matrix(spec, Zc, device)
-    out = [decode_gaussian_y_matrix(spec, np.asfortranarray(Zc[:, :, i]), device)
-           for i in range(Zc.shape[2])]
-    return np.stack(out, axis=2)
+    if spec["task"] == "classification":
+        out = [decode_gaussian_y_matrix(spec, np.asfortranarray(Zc[:, :, i]), device)
+               for i in range(Zc.shape[2])]
+        return np.stack(out, axis=2)
+    n, L = Zc.shape[0], Zc.shape[2]
+    cube = _host_f64(n, np.asarray(spec["decoder"]).shape[1], L)
+    for i in range(L):
+        decode_gaussian_y_matrix(spec, np.asfortranarray(Zc[:, :, i]), device, out=cube[:, :, i])
+    return cube
 
 
 def fastsvd(A, ncomp: int, method: str = "rsvd", rsvd_oversample: int = 10,

@@ -151,12 +151,12 @@ extern "C" int fpls_gaussian_y_encode(fpls_ctx* ctx, const fpls_mat* Ym, int32_t
     to_device(c, Pd.get(), P.data(), P.size());
     // Z = (Y - 1 center^T) P
     gemm_f64(c, false, false, n, dim, m, 1.0, Yd.get(), n, cs.get(), Pd.get(), m, nullptr, 0.0,
-             Z.get(), n);
+             Z.get(), n, nullptr, "gy_encode");
     // decoder = (Z^T Z + ridge I)^{-1} Z^T Yc
     gemm_f64(c, true, false, dim, dim, n, 1.0, Z.get(), n, nullptr, Z.get(), n, nullptr, 0.0,
              G.get(), dim);
     gemm_f64(c, true, false, dim, m, n, 1.0, Z.get(), n, nullptr, Yd.get(), n, cs.get(), 0.0,
-             R.get(), dim);
+             R.get(), dim, nullptr, "gy_cross");
     DBuf<int32_t> st(c, 1);
     spd_solve_slices(c, G.get(), dim, dim, std::vector<int>{dim}, 1, 1, nullptr, 1e-8, Gi.get(),
                      dim, (int64_t)dim * dim, nullptr, st.get());
@@ -234,7 +234,7 @@ extern "C" int fpls_gaussian_y_decode(fpls_ctx* ctx, const fpls_mat* Zm, const d
     to_device(c, cd.get(), center, m);
     if (n > 0) {
       gemm_f64(c, false, false, n, m, dim, 1.0, Zd.get(), n, nullptr, Dd.get(), dim, nullptr, 0.0,
-               Y.get(), n);
+               Y.get(), n, nullptr, "gy_decode");
       launch(c, "gy_center", k_gy_add_center, dim3(gyg(n * m)), dim3(256), 0, Y.get(), n,
              (int64_t)m, (const double*)cd.get());
     }
