@@ -125,15 +125,23 @@ extern "C" int fpls_gaussian_y_encode(fpls_ctx* ctx, const fpls_mat* Ym, int32_t
     FPLS_REQUIRE(n >= 1 && m >= 1, FPLS_ERR_ARG, "gaussian_y_encode: Y is empty");
     FPLS_REQUIRE(dim >= 1, FPLS_ERR_ARG, "gaussian_y_dim must be a positive integer");
     FPLS_REQUIRE(center && decoder, FPLS_ERR_ARG, "gaussian_y_encode: center / decoder missing");
-    DBuf<double> Yd = as_f64(c, Y);
+    // FP64 Y is read in place; FP32 Y is widened once
+    DBuf<double> Yd;
+    const double* yd = Y.d();
+    int64_t ldy = Y.ld;
+    if (Y.dtype != FPLS_F64) {
+      Yd = as_f64(c, Y);
+      yd = Yd.get();
+      ldy = n;
+    }
     // y_center and Yc
     DBuf<double> cs(c, m);
     DMat Yv;
-    Yv.p = Yd.get();
+    Yv.p = yd;
     Yv.dtype = FPLS_F64;
     Yv.rows = n;
     Yv.cols = m;
-    Yv.ld = n;
+    Yv.ld = ldy;
     col_sums(c, Yv, cs.get());
     std::vector<double> hc(m);
     to_host(c, hc.data(), cs.get(), m);
@@ -150,13 +158,19 @@ extern "C" int fpls_gaussian_y_encode(fpls_ctx* ctx, const fpls_mat* Ym, int32_t
         R(c, (size_t)dim * m), Gi(c, (size_t)dim * dim), D(c, (size_t)dim * m);
     to_device(c, Pd.get(), P.data(), P.size());
     // Z = (Y - 1 center^T) P
-    gemm_f64(c, false, false, n, dim, m, 1.0, Yd.get(), n, cs.get(), Pd.get(), m, nullptr, 0.0,
+    gemm_f64(c, false, false, n, dim, m, 1.0, yd, ldy, cs.get(), Pd.get(), m, nullptr, 0.0,
              Z.get(), n, nullptr, "gy_encode");
     // decoder = (Z^T Z + ridge I)^{-1} Z^T Yc
     gemm_f64(c, true, false, dim, dim, n, 1.0, Z.get(), n, nullptr, Z.get(), n, nullptr, 0.0,
              G.get(), dim);
-    gemm_f64(c, true, false, dim, m, n, 1.0, Z.get(), n, nullptr, Yd.get(), n, cs.get(), 0.0,
-             R.get(), dim, nullptr, "gy_cross");
+    {
+      // R = Z^T Yc as (Yc^T Z)^T: the long side (m) on the rows of the product
+      DBuf<double> Rt(c, (size_t)m * dim);
+      gemm_f64(c, true, false, m, dim, n, 1.0, yd, ldy, cs.get(), Z.get(), n, nullptr, 0.0,
+               Rt.get(), m, nullptr, "gy_cross");
+      transpose_f64(c, Rt.get(), m, m, dim, R.get(), dim);
+      sync(c);
+    }
     DBuf<int32_t> st(c, 1);
     spd_solve_slices(c, G.get(), dim, dim, std::vector<int>{dim}, 1, 1, nullptr, 1e-8, Gi.get(),
                      dim, (int64_t)dim * dim, nullptr, st.get());
