@@ -190,17 +190,3 @@ def test_lda_top_k_from_scores(ctx):
     np.testing.assert_array_equal(r["Ypred_top"], ri)
     np.testing.assert_array_equal(r["Ypred_top_score"], rs)
     np.testing.assert_array_equal(r["Ypred_index"], base["Ypred_index"])
-
-
-def test_class_bias_argument_rules(ctx):
-    import fastpls_amd as F
-    lev = ["a", "b", "c"]
-    np.testing.assert_array_equal(F.class_bias_matrix({"c": 3, "a": 1, "b": 2}, lev, 2),
-                                  [[1, 1], [2, 2], [3, 3]])
-    np.testing.assert_array_equal(F.class_bias_matrix(np.ones((2, 3)), lev, 2), np.ones((3, 2)))
-    with pytest.raises(ValueError, match="one numeric value per class"):
-        F.class_bias_matrix([1.0, 2.0], lev, 2)
-    with pytest.raises(ValueError, match="one row per class"):
-        F.class_bias_matrix(np.ones((4, 2)), lev, 2)
-    with pytest.raises(ValueError, match="one column or one column per ncomp"):
-        F.class_bias_matrix(np.ones((3, 3)), lev, 2)

@@ -1,0 +1,37 @@
+"""Host-side argument handling of the R-API mirror (no GPU): the class-bias
+matrix rules of .class_bias_matrix (R/main.R:377-415), .resolve_top_k
+(R/main.R:368-375) and the error paths of the OPLS / kernel PLS wrappers that
+are decided before any device call."""
+import numpy as np
+import pytest
+
+import fastpls_amd as F
+
+
+def test_class_bias_matrix_rules():
+    lev = ["a", "b", "c"]
+    np.testing.assert_array_equal(F.class_bias_matrix(None, lev, 2), np.zeros((3, 2)))
+    np.testing.assert_array_equal(F.class_bias_matrix({"c": 3, "a": 1, "b": 2}, lev, 2),
+                                  [[1, 1], [2, 2], [3, 3]])
+    np.testing.assert_array_equal(F.class_bias_matrix([1.0, 2.0, 3.0], lev, 1), [[1], [2], [3]])
+    # a slices x classes matrix is transposed; one column broadcasts
+    np.testing.assert_array_equal(F.class_bias_matrix(np.ones((2, 3)), lev, 2), np.ones((3, 2)))
+    np.testing.assert_array_equal(F.class_bias_matrix(np.arange(3.0)[:, None], lev, 4),
+                                  np.repeat(np.arange(3.0)[:, None], 4, axis=1))
+    with pytest.raises(ValueError, match="one numeric value per class"):
+        F.class_bias_matrix([1.0, 2.0], lev, 2)
+    with pytest.raises(ValueError, match="one numeric value per class"):
+        F.class_bias_matrix({"a": 1.0, "b": 2.0}, lev, 2)
+    with pytest.raises(ValueError, match="one row per class"):
+        F.class_bias_matrix(np.ones((4, 2)), lev, 2)
+    with pytest.raises(ValueError, match="one column or one column per ncomp"):
+        F.class_bias_matrix(np.ones((3, 3)), lev, 2)
+
+
+def test_resolve_top_k():
+    assert F.resolve_top_k(1) == 1
+    assert F.resolve_top_k(3, top5=True) == 5
+    assert F.resolve_top_k(7, top5=True) == 7
+    for bad in (0, -2):
+        with pytest.raises(ValueError, match="top must be a positive integer"):
+            F.resolve_top_k(bad)
