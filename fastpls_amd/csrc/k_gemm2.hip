@@ -690,7 +690,7 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
     if (Cd && !offset && !c_rowmajor && tiles < 4 * (int64_t)ctx->num_cus && K >= 512) {
       // few row tiles: split K into slabs (FP32 partials, summed in FP64)
       int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
-      splits = std::min<int64_t>(splits, K / 256);
+      splits = std::min<int64_t>(splits, K / 128);
       splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N)));
       int64_t kchunk = (K + splits - 1) / splits;
       kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
