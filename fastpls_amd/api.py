@@ -149,14 +149,23 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
             model.update(predict(model, Xtest, Ytest, top=top))
         return model
     if gaussian_y:
-        if classifier != "argmax":
-            raise ValueError("gaussian_y supports the argmax classifier")
+        if classifier not in ("argmax", "lda", "cknn"):
+            raise ValueError("classifier must be 'argmax', 'lda' or 'cknn'")
         yprep = prepare_gaussian_y(Ytrain, Xtrain, True, gaussian_y_dim, seed, device)
         model = pls(Xtrain, yprep["Ytrain"], None, None, ncomp, scaling, method, svd_method,
                     "argmax", lda_ridge, candidate_knn_k, candidate_tau, candidate_alpha,
                     candidate_top_m, fit, backend, seed, rsvd_oversample, rsvd_power,
                     precision, False, fetch, device, top)
         _attach_gaussian_y(model, yprep)
+        if yprep["classification"] and classifier != "argmax":
+            # .attach_lda_classifier with the original labels (R/main.R:6197-6203):
+            # the head is trained on the latent scores of the model of the codes
+            keep_h: list = []
+            _, codes, C = _encode_factor(Ytrain, n_classes)
+            yv = as_ivec(codes, keep_h)
+            Xm = as_mat(Xtrain, keep_h)
+            _attach_head(model, classifier, yv, C, lda_ridge, candidate_knn_k, candidate_tau,
+                         candidate_alpha, candidate_top_m, Xm)
         if fit and fetch and not yprep["classification"] and "Yfit" in model:
             Yf = decode_gaussian_y_cube(model["gaussian_y"], model["Yfit"], device)
             model["Yfit"] = Yf
@@ -202,18 +211,9 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     model["ncomp"] = np.array(inf.ncomp[:inf.n_ncomp], dtype=np.int64)
     if fetch:
         _fetch_fields(model, bool(fit))
-    if classification and classifier == "lda":
-        ncomp_eff = sorted(set(int(min(max(c, 1), inf.kmax)) for c in model["ncomp"]))
-        ks, kp = int_array(ncomp_eff)
-        ctx.check(ctx.lib.fpls_lda_train_prefix(ctx.h, h, None, ctypes.byref(yv), C, kp, ks.size,
-                                                float(lda_ridge)))
-        model["classification_rule"] = "lda_hip"
-        model["lda"] = {"ncomp": ncomp_eff, "ridge": float(lda_ridge)}
-    elif classification and classifier == "cknn":
-        ctx.check(ctx.lib.fpls_cknn_fit(ctx.h, h, None, ctypes.byref(yv), C, int(candidate_knn_k),
-                                        float(candidate_tau), float(candidate_alpha),
-                                        int(candidate_top_m)))
-        model["classification_rule"] = "candidate_knn_hip"
+    if classification and classifier in ("lda", "cknn"):
+        _attach_head(model, classifier, yv, C, lda_ridge, candidate_knn_k, candidate_tau,
+                     candidate_alpha, candidate_top_m)
     else:
         model["classification_rule"] = "argmax"
     if Xtest is not None:
@@ -329,6 +329,28 @@ def _q2_classes(Ytest, lev, cube: np.ndarray) -> np.ndarray:
     tss = float(((onehot - onehot.mean(axis=0)) ** 2).sum())
     return np.array([1.0 - float(((onehot - cube[:, :, a]) ** 2).sum()) / tss
                      for a in range(cube.shape[2])])
+
+
+def _attach_head(model: FastPLS, classifier: str, yv, C: int, lda_ridge, candidate_knn_k,
+                 candidate_tau, candidate_alpha, candidate_top_m, Xm=None) -> None:
+    """.attach_lda_classifier (R/main.R:1098-1290): the LDA or candidate-kNN
+    head trained on the model's resident training scores, or on the
+    projection of ``Xm`` (the training rows) when given."""
+    ctx = model.ctx
+    inf = model.info()
+    xp = ctypes.byref(Xm) if Xm is not None else None
+    if classifier == "lda":
+        ncomp_eff = sorted(set(int(min(max(c, 1), inf.kmax)) for c in model["ncomp"]))
+        ks, kp = int_array(ncomp_eff)
+        ctx.check(ctx.lib.fpls_lda_train_prefix(ctx.h, model.handle, xp, ctypes.byref(yv), C,
+                                                kp, ks.size, float(lda_ridge)))
+        model["classification_rule"] = "lda_hip"
+        model["lda"] = {"ncomp": ncomp_eff, "ridge": float(lda_ridge)}
+    else:
+        ctx.check(ctx.lib.fpls_cknn_fit(ctx.h, model.handle, xp, ctypes.byref(yv), C,
+                                        int(candidate_knn_k), float(candidate_tau),
+                                        float(candidate_alpha), int(candidate_top_m)))
+        model["classification_rule"] = "candidate_knn_hip"
 
 
 def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 1,

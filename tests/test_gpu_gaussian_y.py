@@ -117,7 +117,7 @@ def test_gaussian_y_argument_errors(ctx):
     with pytest.raises(ValueError):
         F.prepare_gaussian_y(np.ones((10, 3)), X, True, 0)
     with pytest.raises(ValueError):
-        F.pls(X, np.array(list("ab") * 5), ncomp=1, gaussian_y=True, classifier="lda")
+        F.pls(X, np.array(list("ab") * 5), ncomp=1, gaussian_y=True, classifier="knn")
 
 
 def test_gaussian_y_through_opls_and_kernelpls(ctx):
@@ -143,3 +143,29 @@ def test_gaussian_y_through_opls_and_kernelpls(ctx):
                                                 filt["P_orth"]))
     ref = O.decode_gaussian_y_regression(zt[:, :, 0], enc["decoder"], enc["y_center"])
     np.testing.assert_allclose(of["Ypred"][:, :, 0], ref, rtol=1e-7, atol=1e-8)
+
+
+def test_gaussian_y_with_the_lda_head(ctx):
+    """.attach_lda_classifier after a Gaussian-Y classification fit
+    (R/main.R:6197-6203): the LDA head trained with the original labels on the
+    latent scores of the model of the codes; labels equal the oracle's."""
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    Xtr, ytr, Xte, _ = synth.classes_on_sphere(n_classes=6, per_class_train=60,
+                                               per_class_test=15, p=30, seed=9)
+    Xtr, Xte = Xtr.astype(np.float64), Xte.astype(np.float64)
+    fit = F.pls(Xtr, ytr, Xte, ncomp=[2, 4], method="simpls", gaussian_y=True,
+                gaussian_y_dim=5, classifier="lda", seed=4)
+    assert fit["classification_rule"] == "lda_hip" and fit["gaussian_y"]
+    codes = np.unique(ytr, return_inverse=True)[1] + 1
+    enc = O.prepare_gaussian_y_classes(codes, 6, 5, 4)
+    mod = O.pls_model2_fast(Xtr, enc["Z"], [2, 4], 1, False, "rsvd", 10, 1, 4)
+    O.attach_latent_projection_cache(mod)
+    Rp = mod["R_predict"]
+    lda = O.lda_project_train_prefix(Xtr, Rp[:, :4], mod["R_offset"][:4], codes, 6, [2, 4], 1e-8)
+    for s_, kk in enumerate([2, 4]):
+        lab, sc = O.lda_project_predict_labels(Xte, Rp[:, :kk], mod["R_offset"][:kk], lda[s_],
+                                               return_scores=True)
+        srt = np.sort(sc, axis=1)
+        tie = (srt[:, -1] - srt[:, -2]) < 1e-8 * np.abs(sc).max()
+        assert np.all((fit["Ypred_index"][:, s_] == lab) | tie)
