@@ -131,6 +131,8 @@ SIGNATURES = {
                                                ctypes.c_int32, _vp, _vp]),
     "fpls_class_topk_scores": (ctypes.c_int, [_vp, _P(Mat), ctypes.c_int32, ctypes.c_int32, _vp,
                                               ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
+    "fpls_gaussian_y_predict_classes": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, ctypes.c_int32,
+                                                       ctypes.c_int32, _vp, _vp]),
     "fpls_opls_filter": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32,
                                         ctypes.c_int32, ctypes.c_int32, _P(Mat), _vp, _vp, _vp,
                                         _vp, _vp]),

@@ -419,6 +419,20 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
             and not model.get("gaussian_y")):
         res.update(class_predict_topk(model, newdata, None, top))
         return res
+    gspec = model.get("gaussian_y")
+    if (model.get("classification") and Ytest is None and not raw_scores and gspec
+            and gspec.get("task") == "classification"):
+        # nearest-code classes ranked on the device from the compact factors
+        codes = np.asfortranarray(gspec["codes"], dtype=np.float64)
+        C = codes.shape[0]
+        k = max(1, min(int(top), C))
+        ti = np.empty((w, k, L), dtype=np.int32, order="F")
+        ts = np.empty((w, k, L), dtype=np.float64, order="F")
+        ctx.check(ctx.lib.fpls_gaussian_y_predict_classes(ctx.h, model.handle, ctypes.byref(X),
+                                                          codes.ctypes.data, C, int(top),
+                                                          ti.ctypes.data, ts.ctypes.data))
+        res.update(_topk_result(ti, ts, lev))
+        return res
     m = inf.m
     yp = _host_f64(w, m, L)
     ctx.check(ctx.lib.fpls_predict(ctx.h, model.handle, ctypes.byref(X), yp.ctypes.data))

@@ -364,3 +364,84 @@ extern "C" int fpls_class_topk_scores(fpls_ctx* ctx, const fpls_mat* scores, int
     if (top_score) to_host(c, top_score, os.get(), (size_t)w * k * L);
   });
 }
+
+// Nearest-code classes of a Gaussian-Y classification model, ranked on the
+// device from the compact factors: with z = T W_a + mY the code score
+// 2 z c - |c|^2 (nearest_code_classes, src/fastPLS.cpp:4203-4215, applied to
+// the compact prediction of pls_predict_code_classes_compact_cpu, :4217-4270)
+// is T (2 W_a codes^T) + (2 codes mY - |c|^2), so the per-slice weights are
+// 2 W_a codes^T and the offsets 2 codes mY - |c|^2; the ranking is that of
+// .class_topk_from_score_cube on the decoded scores (R/main.R:2744-2750).
+extern "C" int fpls_gaussian_y_predict_classes(fpls_ctx* ctx, const fpls_model* model,
+                                               const fpls_mat* Xm, const double* codes,
+                                               int32_t n_classes, int32_t top_k,
+                                               int32_t* top_index, double* top_score) {
+  if (!model || !top_index || !codes) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    const Model* M = &model->m;
+    DBuf<unsigned char> hx;
+    DMat X = stage(c, Xm, hx, "gaussian_y_predict_classes: X");
+    FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG,
+                 "gaussian_y_predict_classes: Xtest has the wrong number of columns");
+    FPLS_REQUIRE(top_k >= 1, FPLS_ERR_ARG, "top must be a positive integer");
+    FPLS_REQUIRE(n_classes >= 1, FPLS_ERR_ARG, "gaussian_y_predict_classes: no classes");
+    const int64_t w = X.rows;
+    const int dim = (int)M->m, C = n_classes;
+    const int L = (int)M->ncomp.size();
+    const int kmax = M->kmax;
+    const int k = clamp_topk(top_k, C);
+    // offsets 2 codes mY - |c|^2 (host, FP64)
+    std::vector<double> hmy(dim), hoff(C);
+    to_host(c, hmy.data(), M->mY.get(), dim);
+    for (int q = 0; q < C; ++q) {
+      double dot = 0.0, nn = 0.0;
+      for (int j = 0; j < dim; ++j) {
+        const double v = codes[q + (size_t)j * C];
+        dot += v * hmy[j];
+        nn += v * v;
+      }
+      hoff[q] = 2.0 * dot - nn;
+    }
+    DBuf<double> off(c, C), dcodes(c, (size_t)C * dim);
+    h2d(c, off.get(), hoff.data(), sizeof(double) * C);
+    h2d(c, dcodes.get(), codes, sizeof(double) * (size_t)C * dim);
+    // per-slice weights 2 W_a codes^T (first mc rows of a kmax x C block)
+    DBuf<double> Wc(c, (size_t)kmax * C * L);
+    for (int a = 0; a < L; ++a) {
+      const int mc = M->ncomp[a];
+      double* Wa = Wc.get() + (size_t)a * kmax * C;
+      if (M->method == FPLS_METHOD_SIMPLS)  // W_a = Q[:, :mc]^T
+        gemm_f64(c, true, true, mc, C, dim, 2.0, M->Q.get(), dim, nullptr, dcodes.get(), C,
+                 nullptr, 0.0, Wa, kmax);
+      else
+        gemm_f64(c, false, true, mc, C, dim, 2.0, M->W_latent.get() + (size_t)a * kmax * dim,
+                 kmax, nullptr, dcodes.get(), C, nullptr, 0.0, Wa, kmax);
+    }
+    if (w == 0) return;
+    DBuf<int32_t> oi(c, (size_t)w * k * L);
+    DBuf<double> os(c, top_score ? (size_t)w * k * L : 0);
+    DBuf<double> T(c, (size_t)w * kmax);
+    if (X.dtype == FPLS_F32)
+      gemm_nn_f32(c, M->precision, w, kmax, X.cols, X.f(), X.ld, M->mX.get(),
+                  M->R_predict_f32.get(), M->p, nullptr, nullptr, T.get(), w);
+    else
+      project_scores(c, M, X, kmax, nullptr, T.get(), w);
+    const int64_t cap = std::max<int64_t>(4096, (int64_t(1) << 28) / std::max(1, C));
+    const int64_t rb = std::min<int64_t>(w, cap);
+    DBuf<double> Sb(c, (size_t)rb * C);
+    for (int a = 0; a < L; ++a) {
+      const int mc = M->ncomp[a];
+      for (int64_t r0 = 0; r0 < w; r0 += rb) {
+        const int64_t nr = std::min<int64_t>(rb, w - r0);
+        gemm_f64(c, false, false, nr, C, mc, 1.0, T.get() + r0, w, nullptr,
+                 Wc.get() + (size_t)a * kmax * C, kmax, nullptr, 0.0, Sb.get(), nr);
+        const size_t base = (size_t)a * w * k + r0;
+        class_topk_dev(c, Sb.get(), nr, nr, C, off.get(), k, w, oi.get() + base,
+                       top_score ? os.get() + base : nullptr);
+      }
+    }
+    to_host_i32(c, top_index, oi.get(), (size_t)w * k * L);
+    if (top_score) to_host(c, top_score, os.get(), (size_t)w * k * L);
+  });
+}

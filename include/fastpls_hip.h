@@ -53,6 +53,9 @@
  *                             :3838, fill_topk_from_yblock :3863)
  *   fpls_class_topk_scores    class_bias_topk_from_cube  src/fastPLS.cpp:3908-3952,
  *                             .class_topk_from_score_cube R/main.R:453-474
+ *   fpls_gaussian_y_predict_classes  pls_predict_code_classes_compact_cpu
+ *                             src/fastPLS.cpp:4217-4270 with the ranking of
+ *                             .class_topk_from_score_cube R/main.R:2744-2750
  *   fpls_opls_filter          opls_filter_cpp            src/fastPLS.cpp:4377-4461
  *   fpls_opls_apply           opls_apply_filter_cpp      src/fastPLS.cpp:4463-4483
  *   fpls_kernel_matrix        kernel_matrix_cpp          src/fastPLS.cpp:4314-4341
@@ -290,6 +293,16 @@ int fpls_class_predict_topk(fpls_ctx* ctx, const fpls_model* model, const fpls_m
 int fpls_class_topk_scores(fpls_ctx* ctx, const fpls_mat* scores, int32_t m, int32_t n_slices,
                            const double* class_bias, int32_t bias_cols, int32_t top_k,
                            int32_t* top_index, double* top_score);
+
+/* Nearest-code classes of a Gaussian-Y classification model (a fit on the
+   code matrix Z = codes[y]): the ranking above of the code scores
+   2 z c - |c|^2 of the compact prediction z, straight from the factors.
+   codes: host FP64 n_classes x m (column-major, m = the model's response
+   dimension); outputs as fpls_class_predict_topk with k = max(1, min(top_k,
+   n_classes)). */
+int fpls_gaussian_y_predict_classes(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                                    const double* codes, int32_t n_classes, int32_t top_k,
+                                    int32_t* top_index, double* top_score);
 
 /* ---- OPLS filter and kernel matrices (pre-stages of the SIMPLS fit) ---------- */
 /* Orthogonal-signal filter: X is centred (scaling 1), centred and divided by

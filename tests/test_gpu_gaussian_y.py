@@ -169,3 +169,24 @@ def test_gaussian_y_with_the_lda_head(ctx):
         srt = np.sort(sc, axis=1)
         tie = (srt[:, -1] - srt[:, -2]) < 1e-8 * np.abs(sc).max()
         assert np.all((fit["Ypred_index"][:, s_] == lab) | tie)
+
+
+@pytest.mark.parametrize("method", ["simpls", "plssvd"])
+def test_nearest_code_ranking_from_the_factors(ctx, method):
+    """fpls_gaussian_y_predict_classes (the code scores ranked straight from the
+    compact factors) against the decoded score cube ranked by
+    .class_topk_from_score_cube (the raw-score path)."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(17)
+    X = rng.standard_normal((400, 20))
+    y = rng.choice([f"k{i}" for i in range(9)], 400)
+    Xt = rng.standard_normal((120, 20))
+    fit = F.pls(X, y, ncomp=[2, 5], method=method, gaussian_y=True, gaussian_y_dim=6, seed=5)
+    fast = F.predict(fit, Xt, top=4)
+    full = F.predict(fit, Xt, top=4, raw_scores=True)
+    np.testing.assert_allclose(fast["Ypred_top_score"], full["Ypred_top_score"], rtol=0,
+                               atol=1e-10)
+    gap = np.abs(np.diff(full["Ypred_top_score"], axis=1)).min(axis=1)
+    assert np.all(np.all(fast["Ypred_top"] == full["Ypred_top"], axis=1) | (gap < 1e-9))
+    np.testing.assert_array_equal(fast["Ypred"][gap.min(axis=1) > 1e-9],
+                                  full["Ypred"][gap.min(axis=1) > 1e-9])
