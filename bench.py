@@ -392,7 +392,7 @@ def main_sketch(args):
     X, Y = c5_data(synth.SEED_BASE + 5)
     n, p, m, k = C5_N, C5_P, C5_M, C5_K
     l = k + C5_OS
-    # FP64 judge: S = X^T Y in FP64 and its top-k left singular subspace
+    # FP64 reference: S = X^T Y in FP64 and its top-k left singular subspace
     X64 = X.double()
     S64 = X64.t() @ Y.double()
     del X64
