@@ -674,10 +674,13 @@ def model_from_fields(fields: Dict, device: int = 0) -> FastPLS:
     lev), e.g. a model fitted by the reference's CPU path: predict() then runs
     on the device (fpls_model_import; compact_prediction_weights,
     src/fastPLS.cpp:3737-3836)."""
-    ctx = _lib.context(device)
     method = fields.get("pls_method", "simpls")
     if method not in ("simpls", "simpls_fast", "plssvd"):
         raise ValueError("pls_method must be 'simpls' or 'plssvd'")
+    for name in ("R", "ncomp", "mX", "vX", "mY"):
+        if fields.get(name) is None:
+            raise ValueError(f"model field {name} is required")
+    ctx = _lib.context(device)
     keep: list = []
 
     def arr(name, shape=None, required=False):

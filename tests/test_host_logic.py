@@ -35,3 +35,17 @@ def test_resolve_top_k():
     for bad in (0, -2):
         with pytest.raises(ValueError, match="top must be a positive integer"):
             F.resolve_top_k(bad)
+
+
+def test_model_import_field_checks():
+    """model_from_fields rejects an unknown method or missing fields before
+    any device call (the reference's list must carry R, ncomp, mX, vX, mY)."""
+    f = {"R": np.eye(3), "ncomp": [1], "mX": np.zeros(3), "vX": np.ones(3), "mY": np.zeros(2),
+         "pls_method": "pcr"}
+    with pytest.raises(ValueError, match="pls_method"):
+        F.model_from_fields(f)
+    for missing in ("R", "mY"):
+        g = dict(f, pls_method="simpls")
+        g.pop(missing)
+        with pytest.raises(ValueError, match=missing):
+            F.model_from_fields(g)
