@@ -102,6 +102,7 @@ SIGNATURES = {
                                              ctypes.c_int32, ctypes.c_double]),
     "fpls_lda_read": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int64]),
     "fpls_lda_predict": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, _vp]),
+    "fpls_lda_predict_noalloc": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, _vp]),
     "fpls_lda_train_scores": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                              ctypes.c_int32, ctypes.c_double, _vp, _vp, _vp, _vp,
                                              _vp, _vp]),

@@ -113,8 +113,31 @@ struct Ctx {
     checks.clear();
     nflags = 0;
   }
-  void* alloc(size_t bytes) { return pool.get(bytes); }
-  void free(void* p) { pool.put(p); }
+  // Fixed-arena mode of the allocation-free entry points: while arena_base is
+  // set, alloc() carves 256-byte aligned blocks from the device range
+  // [arena_base, arena_base + arena_cap) as a stack instead of the pool (a
+  // freed block returns to the arena once every block above it is freed too),
+  // and running out raises FPLS_ERR_ARG.
+  struct ArenaBlock {
+    size_t off, bytes;
+    bool live;
+  };
+  unsigned char* arena_base = nullptr;
+  size_t arena_cap = 0, arena_used = 0, arena_peak = 0;
+  std::vector<ArenaBlock> arena_blocks;
+  // Per-device address of the library's static scratch array (set up when
+  // the context is created; null if that failed, with the reason below).
+  void* scratch_ptr = nullptr;
+  size_t scratch_bytes = 0;
+  std::string scratch_err;
+  void* alloc(size_t bytes);
+  void free(void* p);
+  // Byte budget for the split-K FP32 partials of one product (smaller inside
+  // the fixed arena, so that the slab count, not the call, gives way).
+  size_t partial_budget(size_t normal = size_t(256) << 20) const {
+    if (!arena_base) return normal;
+    return (arena_cap - arena_used) / 2;
+  }
   hipEvent_t event();
   void collect_timing();
 };
@@ -289,6 +312,11 @@ void chol_inv_lower(Ctx* ctx, const double* G, int64_t ldg, int l, int64_t rows,
                     double* Linv, int32_t* status);
 void scale_rows_div(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols,
                     const double* f);
+
+// ---------------------------------------------------------------- scratch (noalloc.hip)
+// Looks up (once per device, with ctx->device current) the address of the
+// library's static scratch array and records it in ctx->scratch_ptr.
+void scratch_setup(Ctx* ctx);
 
 // ---------------------------------------------------------------- RNG (host)
 // std::mt19937(seed) + std::normal_distribution<double>(0,1), column-major.

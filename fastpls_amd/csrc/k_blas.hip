@@ -410,7 +410,8 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     const int64_t want = 2 * ctx->num_cus;
     if (tiles < want && K >= 512) {
       splits = std::min<int64_t>((want + tiles - 1) / tiles, K / 256);
-      const int64_t cap = std::max<int64_t>(1, (int64_t(64) << 20) / std::max<int64_t>(1, M * N));
+      const int64_t cap = std::max<int64_t>(
+          1, (int64_t)(ctx->partial_budget(size_t(512) << 20) / 8) / std::max<int64_t>(1, M * N));
       splits = std::max<int64_t>(1, std::min(splits, cap));
     }
     int64_t kchunk = (K + splits - 1) / splits;
@@ -452,7 +453,8 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     // least), latency rather than FLOPs bounds these products
     splits = std::min<int64_t>((want + tiles - 1) / tiles, (K + 2 * G64_BK - 1) / (2 * G64_BK));
     // keep the partial buffer modest
-    const int64_t cap = std::max<int64_t>(1, (int64_t(64) << 20) / std::max<int64_t>(1, M * N));
+    const int64_t cap = std::max<int64_t>(
+          1, (int64_t)(ctx->partial_budget(size_t(512) << 20) / 8) / std::max<int64_t>(1, M * N));
     splits = std::max<int64_t>(1, std::min(splits, cap));
   }
   int64_t kchunk = (K + splits - 1) / splits;

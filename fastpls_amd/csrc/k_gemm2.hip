@@ -691,7 +691,7 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       // few row tiles: split K into slabs (FP32 partials, summed in FP64)
       int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
       splits = std::min<int64_t>(splits, K / 128);
-      splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N)));
+      splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N)));
       int64_t kchunk = (K + splits - 1) / splits;
       kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
       splits = (K + kchunk - 1) / kchunk;
@@ -1438,7 +1438,7 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
     const int64_t tiles = (M + 4 * TL_BM - 1) / (4 * TL_BM);
     int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
     splits = std::min<int64_t>(splits, std::max<int64_t>(1, K / 256));
-    splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N)));
+    splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N)));
     tn_skinny(ctx, precision, M, (int)N, K, A, lda, shiftA, Bf.get(), Bh.get(), Bl.get(), Kp,
               splits, C, ldc);
     return;
@@ -1459,7 +1459,7 @@ void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, con
                    const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc,
                    int64_t splits) {
   if (M <= 0 || N <= 0) return;
-  const int64_t cap = std::max<int64_t>(1, (int64_t(256) << 20) / (4 * M * N));
+  const int64_t cap = std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N));
   splits = std::max<int64_t>(1, std::min(splits, cap));
   int64_t kchunk = (K + splits - 1) / splits;
   kchunk = (kchunk + G2_BK - 1) / G2_BK * G2_BK;

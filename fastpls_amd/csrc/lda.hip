@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <functional>
 #include <set>
 #include <vector>
 
@@ -381,6 +382,57 @@ extern "C" int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t sli
   });
 }
 
+namespace fpls {
+
+// Labels (w x L, device, 1-based) of every ncomp slice for the test rows X;
+// `on_scores(a, sc)` receives slice a's discriminant scores plus constants
+// (w x C, device) when set.
+void lda_predict_core(Ctx* c, const Model* M, const DMat& X, int32_t* lab,
+                      const std::function<void(int, const double*)>& on_scores) {
+  FPLS_REQUIRE(!M->lda.empty(), FPLS_ERR_STATE,
+               "this model does not contain fitted LDA classifier parameters");
+  FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "lda_predict: X has the wrong number of columns");
+  const int64_t w = X.rows, C = M->n_classes;
+  const int L = (int)M->ncomp.size();
+  const int lda_kmax = M->lda.back().k;
+  // projected test scores for the largest slice; prefixes serve the rest
+  DMat T;
+  T.rows = w;
+  T.cols = lda_kmax;
+  T.ld = w;
+  DBuf<float> Tf;
+  DBuf<double> Td;
+  if (X.dtype == FPLS_F32) {
+    Tf = DBuf<float>(c, (size_t)w * lda_kmax);
+    project_scores(c, M, X, lda_kmax, Tf.get(), nullptr, w);
+    T.p = Tf.get();
+    T.dtype = FPLS_F32;
+  } else {
+    Td = DBuf<double>(c, (size_t)w * lda_kmax);
+    project_scores(c, M, X, lda_kmax, nullptr, Td.get(), w);
+    T.p = Td.get();
+    T.dtype = FPLS_F64;
+  }
+  DBuf<double> sc(c, (size_t)w * C);
+  for (int a = 0; a < L; ++a) {
+    const int kk = std::min(std::max(M->ncomp[a], 1), lda_kmax);
+    const LdaSlice* sl = nullptr;
+    for (const auto& s : M->lda)
+      if (s.k == kk) sl = &s;
+    FPLS_REQUIRE(sl != nullptr, FPLS_ERR_STATE, "no fitted LDA classifier for this ncomp");
+    lda_scores(c, M, *sl, X, T, sc.get());
+    launch(c, "lda_argmax", k_argmax_rows, dim3((unsigned)((w + 255) / 256)), dim3(256), 0,
+           sc.get(), w, w, C, sl->constants.get(), lab + (size_t)a * w);
+    if (on_scores) {
+      launch(c, "lda_consts", k_add_consts, dim3(g1(w * C)), dim3(256), 0, sc.get(), w, w, C,
+             sl->constants.get());
+      on_scores(a, sc.get());
+    }
+  }
+}
+
+}  // namespace fpls
+
 extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
                                 int32_t* labels, double* scores_out) {
   if (!model) return FPLS_ERR_ARG;
@@ -391,45 +443,15 @@ extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fp
                  "this model does not contain fitted LDA classifier parameters");
     DBuf<unsigned char> hx;
     DMat X = stage(c, Xm, hx, "lda_predict: X");
-    FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "lda_predict: X has the wrong number of columns");
     const int64_t w = X.rows, C = M->n_classes;
     const int L = (int)M->ncomp.size();
-    const int lda_kmax = M->lda.back().k;
-    // projected test scores for the largest slice; prefixes serve the rest
-    DMat T;
-    T.rows = w;
-    T.cols = lda_kmax;
-    T.ld = w;
-    DBuf<float> Tf;
-    DBuf<double> Td;
-    if (X.dtype == FPLS_F32) {
-      Tf = DBuf<float>(c, (size_t)w * lda_kmax);
-      project_scores(c, M, X, lda_kmax, Tf.get(), nullptr, w);
-      T.p = Tf.get();
-      T.dtype = FPLS_F32;
-    } else {
-      Td = DBuf<double>(c, (size_t)w * lda_kmax);
-      project_scores(c, M, X, lda_kmax, nullptr, Td.get(), w);
-      T.p = Td.get();
-      T.dtype = FPLS_F64;
-    }
-    DBuf<double> sc(c, (size_t)w * C);
     DBuf<int32_t> lab(c, (size_t)w * L);
-    for (int a = 0; a < L; ++a) {
-      const int kk = std::min(std::max(M->ncomp[a], 1), lda_kmax);
-      const LdaSlice* sl = nullptr;
-      for (const auto& s : M->lda)
-        if (s.k == kk) sl = &s;
-      FPLS_REQUIRE(sl != nullptr, FPLS_ERR_STATE, "no fitted LDA classifier for this ncomp");
-      lda_scores(c, M, *sl, X, T, sc.get());
-      launch(c, "lda_argmax", k_argmax_rows, dim3((unsigned)((w + 255) / 256)), dim3(256), 0,
-             sc.get(), w, w, C, sl->constants.get(), lab.get() + (size_t)a * w);
-      if (scores_out) {
-        launch(c, "lda_consts", k_add_consts, dim3(g1(w * C)), dim3(256), 0, sc.get(), w, w, C,
-               sl->constants.get());
-        to_host(c, scores_out + (size_t)a * w * C, sc.get(), (size_t)w * C);
-      }
-    }
+    std::function<void(int, const double*)> cb;
+    if (scores_out)
+      cb = [&](int a, const double* sc) {
+        to_host(c, scores_out + (size_t)a * w * C, sc, (size_t)w * C);
+      };
+    lda_predict_core(c, M, X, lab.get(), cb);
     to_host_i32(c, labels, lab.get(), (size_t)w * L);
   });
 }

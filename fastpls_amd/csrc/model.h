@@ -3,6 +3,8 @@
 // (src/fastPLS.cpp:2759-2777, 4966-4980; R/main.R:1019-1041, 1236-1250, 858-871).
 #pragma once
 
+#include <functional>
+
 #include "fpls_internal.h"
 
 namespace fpls {
@@ -113,6 +115,10 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
                     int32_t C, const std::vector<int>& ks, double ridge, int precision,
                     DBuf<double>& counts_out, std::vector<LdaSlice>& slices);
+// LDA labels (w x n_ncomp, device, 1-based) of the test rows X; on_scores(a,
+// scores) receives slice a's discriminant scores (w x C, device) when set.
+void lda_predict_core(Ctx* ctx, const Model* M, const DMat& X, int32_t* labels,
+                      const std::function<void(int, const double*)>& on_scores);
 // labels[i] = 1 + argmax_c(scores[i,c] + cst[c]) (first maximum wins)
 void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t C,
                  const double* cst, int32_t* labels);

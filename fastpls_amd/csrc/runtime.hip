@@ -62,12 +62,48 @@ size_t Pool::total_bytes() const {
   return t;
 }
 
+// ---------------------------------------------------------------- context allocation
+void* Ctx::alloc(size_t bytes) {
+  if (!arena_base) return pool.get(bytes);
+  const size_t need = std::max<size_t>(256, (bytes + 255) & ~size_t(255));
+  FPLS_REQUIRE(need <= arena_cap - arena_used, FPLS_ERR_ARG,
+               "allocation-free call: its temporaries need more than the " +
+                   std::to_string(arena_cap >> 20) + " MiB scratch array (" +
+                   std::to_string((arena_used + need) >> 20) + " MiB asked so far)");
+  void* p = arena_base + arena_used;
+  arena_blocks.push_back({arena_used, need, true});
+  arena_used += need;
+  arena_peak = std::max(arena_peak, arena_used);
+  return p;
+}
+
+void Ctx::free(void* p) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  if (arena_base && b >= arena_base && b < arena_base + arena_cap) {
+    const size_t off = static_cast<size_t>(b - arena_base);
+    for (size_t i = arena_blocks.size(); i-- > 0;) {
+      if (arena_blocks[i].off == off && arena_blocks[i].live) {
+        arena_blocks[i].live = false;
+        break;
+      }
+    }
+    while (!arena_blocks.empty() && !arena_blocks.back().live) {
+      arena_used = arena_blocks.back().off;
+      arena_blocks.pop_back();
+    }
+    return;
+  }
+  pool.put(p);
+}
+
 // ---------------------------------------------------------------- deferred checks
 constexpr int kFlagSlots = 1024;
 
 int32_t* Ctx::defer_check(int count, int code, const std::string& msg, int fail_value) {
   FPLS_REQUIRE(count >= 1 && count <= kFlagSlots, FPLS_ERR_ARG, "too many deferred checks");
   if (!flags_dev) {
+    FPLS_REQUIRE(!arena_base, FPLS_ERR_STATE,
+                 "allocation-free call reached a deferred check before any other call");
     FPLS_HIP(hipMalloc(&flags_dev, kFlagSlots * sizeof(int32_t)));
     FPLS_HIP(hipHostMalloc(&flags_host, kFlagSlots * sizeof(int32_t), hipHostMallocDefault));
   }
@@ -252,6 +288,7 @@ int fpls_ctx_create(int device, fpls_ctx** out) {
     delete c;
     return FPLS_ERR_HIP;
   }
+  scratch_setup(&c->c);
   *out = c;
   return FPLS_OK;
 }

@@ -254,6 +254,18 @@ int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t slice, int32_t
    (w x C x n_ncomp, may be NULL). */
 int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
                      int32_t* labels, double* scores);
+/* No counterpart in the reference.  fpls_lda_predict for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc): Xtest (device), labels (w x n_ncomp
+   int32, device) and scores (w x C x n_ncomp FP64, device, may be NULL) as
+   fpls_lda_predict, with every temporary carved from one static 64 MiB
+   scratch array per GPU.  Rows beyond what that array holds for the model
+   (about 64 MiB / (4 k + 12 C) bytes per row for FP32 input) are rejected
+   with FPLS_ERR_ARG before any work starts.  Returns after the device work
+   has finished.  Calls that overlap on one GPU (several threads or contexts)
+   share the array and race. */
+int fpls_lda_predict_noalloc(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                             int32_t* labels, double* scores);
 /* Array-level trainer on given scores T (n x kmax). Outputs per slice s with
    k_s = ncomp[s]: means/linear (C x k_s), inv_cov (k_s x k_s), constants (C),
    packed one slice after another; priors (C) and ridge (n_ncomp). */
