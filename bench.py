@@ -123,21 +123,80 @@ def run_gpu(args, data):
     return out
 
 
+# ---------------------------------------------------------------- CPU baseline
+def host_cpu_info():
+    """The host cores this process may use (its affinity set, bounded by
+    OMP_NUM_THREADS when that is set) and the CPU model string."""
+    try:
+        share = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        share = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(share, int(omp)) if omp.isdigit() and int(omp) > 0 else share
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": share, "threads_all": threads, "cpu_model": model,
+            "os_cpu_count": os.cpu_count()}
+
+
+def time_cpu(fn, repeats=5, warmup=1, one_thread=True):
+    """Median wall time of ``repeats`` runs of ``fn`` after ``warmup`` runs,
+    at all the process's host cores and (``one_thread``) at 1 thread, with the
+    BLAS thread pool limited accordingly.  Returns (timings, last result, info)."""
+    from threadpoolctl import threadpool_limits
+    info = host_cpu_info()
+    out = None
+    tim = {}
+    legs = [("all", info["threads_all"])] + ([("one", 1)] if one_thread else [])
+    for label, nt in legs:
+        with threadpool_limits(limits=nt):
+            for _ in range(warmup):
+                out = fn()
+            runs = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                out = fn()
+                runs.append(time.perf_counter() - t0)
+        tim[label] = {"threads": nt, "median_s": float(np.median(runs)), "runs_s": runs}
+    return tim, out, info
+
+
+def cpu_baseline_obj(tim, info, units_per_run, unit, sample, kind="port", extra=None):
+    """The cpu_baseline object: the all-cores median is ``value``; the 1-thread
+    median, the core count and the CPU model are reported beside it."""
+    a = tim["all"]
+    obj = {"value": units_per_run / a["median_s"], "unit": unit, "cores": a["threads"],
+           "kind": kind, "sample": sample, "median_s": a["median_s"],
+           "runs": len(a["runs_s"]), "warmup_runs": 1, "nproc": info["nproc"],
+           "cpu_model": info["cpu_model"]}
+    if "one" in tim:
+        o = tim["one"]
+        obj["one_thread"] = {"value": units_per_run / o["median_s"], "cores": 1,
+                             "median_s": o["median_s"], "runs": len(o["runs_s"])}
+    if extra:
+        obj.update(extra)
+    return obj
+
+
 def oracle_run(data):
     """FP64 oracle (numpy/LAPACK restatement of the reference CPU path) on the
-    host cores, timed as the CPU baseline."""
-    from threadpoolctl import threadpool_limits
+    host cores, timed as the CPU baseline (median of 5 after a warm-up, all
+    cores and 1 thread)."""
     from oracle import fastpls_oracle as O
     Xtr, ytr, Xte, yte = data
-    cores = min(16, os.cpu_count() or 1)
-    with threadpool_limits(limits=cores):
-        t0 = time.perf_counter()
-        ref = O.pls_plssvd_lda(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [NCOMP])
-        dt = time.perf_counter() - t0
-    _, sc = O.lda_project_predict_labels(Xte.astype(np.float64), ref["R_predict"][:, :NCOMP],
+    X64, Xt64 = Xtr.astype(np.float64), Xte.astype(np.float64)
+    tim, ref, info = time_cpu(lambda: O.pls_plssvd_lda(X64, ytr, Xt64, [NCOMP]))
+    _, sc = O.lda_project_predict_labels(Xt64, ref["R_predict"][:, :NCOMP],
                                          ref["R_offset"][:NCOMP], ref["lda"][NCOMP],
                                          return_scores=True)
-    return ref, sc, dt, cores
+    return ref, sc, tim, info
 
 
 # ---------------------------------------------------------------- nested CV
@@ -1053,16 +1112,126 @@ def c3_oracle_check(Xtr, ytr, Xte, res):
     return errors, cpu
 
 
+# ---------------------------------------------------------------- SIMPLS regression
+C1_N, C1_P, C1_M, C1_NT, C1_K = 200, 2000, 3, 100, 10
+# Algorithmic work per fit (FP64): bytes 8 [n p (1 + 2K) + n m + n_t p] (X read
+# for S and twice per component, Y once, Xtest once); flops 2npm + K (4np +
+# 2nm + 14pm) + 4p sum_a a + 2pmK + 2 n_t p m.
+C1_BYTES = 8.0 * (C1_N * C1_P * (1 + 2 * C1_K) + C1_N * C1_M + C1_NT * C1_P)
+C1_FLOPS = (2.0 * C1_N * C1_P * C1_M
+            + C1_K * (4.0 * C1_N * C1_P + 2.0 * C1_N * C1_M + 14.0 * C1_P * C1_M)
+            + 4.0 * C1_P * (C1_K * (C1_K - 1) / 2) + 2.0 * C1_P * C1_M * C1_K
+            + 2.0 * C1_NT * C1_P * C1_M)
+
+
+def main_simpls_reg(args):
+    """SIMPLS regression fit + predict (n = 200, p = 2,000, m = 3, ncomp = 10,
+    100 test rows, FP64 inputs as the reference's arma::mat): fits/s with X, Y
+    and Xtest resident in HBM.  The chosen form is the FP64 path (the
+    reference's own precision); the plain form takes FP32 copies of X and
+    Xtest.  Both are checked against the FP64 oracle."""
+    import torch
+    import fastpls_amd as F
+    from fastpls_amd import synth
+    from oracle import fastpls_oracle as O
+    X, Y, Xt, Yt = synth.regression_lowrank(n=C1_N, p=C1_P, m=C1_M, n_test=C1_NT)
+    ctx = F.context(0)
+    dev = {"fp64": (F.device_colmajor(X), F.device_colmajor(Xt)),
+           "fp32": (F.device_colmajor(X.astype(np.float32)),
+                    F.device_colmajor(Xt.astype(np.float32)))}
+    dY = F.device_colmajor(Y)
+    torch.cuda.synchronize()
+    res = {}
+    knames = ("gemv_n", "gemv_n_reduce", "gemv_t", "simpls_refresh", "simpls_orth",
+              "simpls_normalize", "div_vec", "gemm_f64", "gemm_f64_reduce", "gemm_tn_f32",
+              "gemm_tn_reduce", "col_sums", "div", "fill", "project_f32", "gemm_nn_f32",
+              "gemm_nn_reduce", "add_row")
+    for form, (dX, dXt) in dev.items():
+        def step(fetch=False):
+            m = F.pls(dX, dY, dXt, method="simpls", ncomp=C1_K, fetch=fetch)
+            out = (m["Ypred"], m.get("B"))
+            m.free()
+            return out
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        yp, B = step(fetch=True)
+        ctx.timing(True)
+        tsteps = 5
+        for _ in range(tsteps):
+            step()
+        kern = {}
+        for nm in knames:
+            n, ms = ctx.kernel_time(nm)
+            if n:
+                kern[nm] = {"launches_per_step": n / tsteps, "ms_per_step": ms / tsteps}
+        ctx.timing(False)
+        res[form] = {"fits_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
+                     "Ypred": yp, "B": B, "kernels": kern}
+    ref = O.pls_model2_fast(X, Y, [C1_K])
+    ref_pred = O.pls_predict(ref, Xt)
+    errors = {f: {"B_rel_err": float(np.linalg.norm(r["B"] - ref["B"]) / np.linalg.norm(ref["B"])),
+                  "Ypred_rel_err": float(np.linalg.norm(r["Ypred"] - ref_pred)
+                                         / np.linalg.norm(ref_pred))}
+              for f, r in res.items()}
+    chosen = "fp64"
+    r = res[chosen]
+    # dominant data-parallel kernels: the two passes over X per component
+    roof = None
+    gk = [k for k in ("gemv_n", "gemv_t") if k in r["kernels"]]
+    if gk:
+        kn = max(gk, key=lambda k: r["kernels"][k]["ms_per_step"])
+        kt = r["kernels"][kn]
+        avg_s = kt["ms_per_step"] * 1e-3 / kt["launches_per_step"]
+        per_launch = 8.0 * C1_N * C1_P
+        roof = {"bound": "hbm", "kernel": kn, "achieved": per_launch / avg_s / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": per_launch / avg_s / 1e9 / HBM_PEAK_GBS,
+                "traffic": measured_traffic(kn, kt["launches_per_step"]),
+                "avg_launch_us": avg_s * 1e6, "launches_per_step": kt["launches_per_step"],
+                "algorithmic_bytes_per_launch": per_launch,
+                "fit_algorithmic_bytes": C1_BYTES, "fit_algorithmic_flops": C1_FLOPS,
+                "fit_GBps": C1_BYTES / (r["ms_per_step"] * 1e-3) / 1e9,
+                "note": "a 3.2 MB X per pass: each launch is latency bound, not HBM bound"}
+    out = {
+        "metric": METRIC, "value": r["fits_per_s"], "unit": "fits/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded; rank-20 X = U diag(10 0.8^i) V^T + 0.1 N(0,1), Y = X B + "
+                "0.05 N(0,1))",
+        "config": {"workload": f"simpls_reg n={C1_N} p={C1_P} m={C1_M} ntest={C1_NT} "
+                               f"ncomp={C1_K}", "chosen_form": chosen,
+                   "inputs": "resident in HBM"},
+        "plain_fp32_fits_per_s": res["fp32"]["fits_per_s"],
+        "chosen_vs_plain": r["fits_per_s"] / res["fp32"]["fits_per_s"],
+        "roofline": roof, "oracle_errors": errors,
+        "kernels": {f: v["kernels"] for f, v in res.items()},
+    }
+    if not args.no_cpu_baseline:
+        tim, _, info = time_cpu(lambda: O.pls_predict(O.pls_model2_fast(X, Y, [C1_K]), Xt),
+                                repeats=9)
+        out["cpu_baseline"] = cpu_baseline_obj(
+            tim, info, 1.0, "fits/s", "full FP64 SIMPLS fit + predict of the same workload "
+            "(numpy/LAPACK oracle); median of 9 runs after one warm-up")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=["plssvd_lda", "cv", "sketch", "cknn", "topk",
-                                           "kernelpls", "gaussian"],
+    ap.add_argument("--workload", choices=["plssvd_lda", "simpls_reg", "cv", "sketch", "cknn",
+                                           "topk", "kernelpls", "gaussian"],
                     default="plssvd_lda",
-                    help="plssvd_lda: PLSSVD + LDA fit/predict (default); cv: batched nested "
+                    help="plssvd_lda: PLSSVD + LDA fit/predict (default); simpls_reg: SIMPLS "
+                         "regression fit/predict (n=200, p=2000); cv: batched nested "
                          "cross-validation of SIMPLS; sketch: rSVD of X^T Y with learned vs "
                          "Gaussian sketches; cknn: label-aware PLSSVD + candidate kNN at 1000 "
                          "classes")
@@ -1076,6 +1245,9 @@ def main():
         sys.exit(2)
     if args.copy_mib > 0:
         calibration_copy(args.copy_mib)
+    if args.workload == "simpls_reg":
+        main_simpls_reg(args)
+        return
     if args.workload == "cv":
         main_cv(args)
         return
@@ -1102,7 +1274,7 @@ def main():
     cpu = None
     ok = {"fp32": True, "bf16x3": True}
     if not args.no_cpu_baseline:
-        ref, sc, dt, cores = oracle_run(data)
+        ref, sc, tim, info = oracle_run(data)
         s = np.sort(sc, axis=1)
         margin = s[:, -1] - s[:, -2]
         tie_tol = 1e-3 * float(np.abs(sc).max())
@@ -1118,10 +1290,11 @@ def main():
                 "accuracy": float(np.mean(lab == data[3])),
             }
             ok[prec] = st <= SIN_THETA_BOUND and errors[prec]["label_mismatch_non_tie"] == 0
-        cpu = {"value": 1.0 / dt, "unit": "fits/s", "cores": cores, "kind": "port",
-               "sample": "one full FP64 fit + predict of the same workload (numpy/LAPACK oracle, "
-                         "threads = cores)", "seconds": dt,
-               "oracle_accuracy": float(np.mean(ref["Ypred_labels"][:, 0] == data[3]))}
+        cpu = cpu_baseline_obj(
+            tim, info, 1.0, "fits/s",
+            "full FP64 fit + predict of the same workload (numpy/LAPACK oracle); median of 5 "
+            "runs after one warm-up, at all the process's cores and at 1 thread",
+            extra={"oracle_accuracy": float(np.mean(ref["Ypred_labels"][:, 0] == data[3]))})
     chosen = "bf16x3" if ok["bf16x3"] else "fp32"
     r = res[chosen]
     kname = "project_bf16x3" if chosen == "bf16x3" else "project_f32"

@@ -12,12 +12,14 @@ from .api import (FastPLS, candidate_knn_predict, class_sums, cv_batched, fastsv
                   decode_gaussian_y_matrix, decode_gaussian_y_cube, gaussian_y_default_dim,
                   class_predict_topk, class_topk_from_score_cube, class_bias_matrix,
                   resolve_top_k, opls_filter, opls_apply_filter, kernel_matrix,
-                  center_kernel_train, center_kernel_test, model_from_fields)
+                  center_kernel_train, center_kernel_test, model_from_fields, compiled_kfold,
+                  cv_folds)
 
 __all__ = ["pls", "predict", "fastsvd", "optim_pls_cv", "pls_single_cv", "pls_cv_predict",
            "pls_double_cv", "cv_batched", "sketch_train", "sketch_eval", "prepare_gaussian_y",
            "decode_gaussian_y_matrix", "decode_gaussian_y_cube", "gaussian_y_default_dim",
            "class_predict_topk", "class_topk_from_score_cube", "class_bias_matrix",
            "resolve_top_k", "opls_filter", "opls_apply_filter", "kernel_matrix",
-           "center_kernel_train", "center_kernel_test", "model_from_fields", "FastPLS", "Context", "FplsError", "available",
+           "center_kernel_train", "center_kernel_test", "model_from_fields", "compiled_kfold",
+           "cv_folds", "FastPLS", "Context", "FplsError", "available",
            "context", "device_colmajor", "load"]

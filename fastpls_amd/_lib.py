@@ -151,6 +151,8 @@ SIGNATURES = {
     "fpls_cv_batched": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                        ctypes.c_int32, _vp, _vp, ctypes.c_int32, _P(Opts), _vp,
                                        _vp, _vp, _vp]),
+    "fpls_cv_folds": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32, _vp, ctypes.c_int32,
+                                     ctypes.c_uint32, _vp, _P(ctypes.c_int32)]),
     "fpls_gaussian_streams": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int64, _vp]),
 }
 

@@ -283,7 +283,9 @@ def class_predict_topk(model: FastPLS, newdata, class_bias=None, top: int = 1) -
     X = as_mat(newdata, keep)
     inf = model.info()
     L, m = inf.n_ncomp, inf.m
-    lev = model.get("lev") or [str(i + 1) for i in range(m)]
+    lev = model.get("lev")
+    if lev is None:
+        lev = [str(i + 1) for i in range(m)]
     bias = class_bias_matrix(class_bias, lev, L)
     k = max(1, min(int(top), m))
     ti = np.empty((X.rows, k, L), dtype=np.int32, order="F")
@@ -680,7 +682,6 @@ def model_from_fields(fields: Dict, device: int = 0) -> FastPLS:
     for name in ("R", "ncomp", "mX", "vX", "mY"):
         if fields.get(name) is None:
             raise ValueError(f"model field {name} is required")
-    ctx = _lib.context(device)
     keep: list = []
 
     def arr(name, shape=None, required=False):
@@ -713,12 +714,17 @@ def model_from_fields(fields: Dict, device: int = 0) -> FastPLS:
         v = fields.get(name)
         if v is not None and np.size(v):
             a = np.asarray(v, dtype=np.float64)
-            if name == "W_latent" and a.ndim == 3 and a.shape[2] > L:
-                a = a[:, :, :L]          # W_dim[2] >= length(ncomp) in the reference
+            if name in ("W_latent", "C_latent") and a.ndim == 3 and a.shape[2] > L:
+                a = a[:, :, :L]          # dim[3] >= length(ncomp) in the reference
             setattr(f, name, arr_from(a, shape, keep).ctypes.data)
-    f.mX = arr("mX", (-1,), True).ctypes.data
-    f.vX = arr("vX", (-1,), True).ctypes.data
+    mX = arr("mX", (-1,), True)
+    vX = arr("vX", (-1,), True)
+    if mX.size != p or vX.size != p:
+        raise ValueError("Model `mX` and `vX` must have one value per predictor (nrow(R))")
+    f.mX = mX.ctypes.data
+    f.vX = vX.ctypes.data
     f.mY = mY.ctypes.data
+    ctx = _lib.context(device)
     h = ctypes.c_void_p()
     ctx.check(ctx.lib.fpls_model_import(ctx.h, ctypes.byref(f), ctypes.byref(h)))
     model = FastPLS(ctx, h)
@@ -985,13 +991,61 @@ def _batched_eligible(Xdata, method, classifier, scaling, ngroups) -> bool:
     return ok and len(shape) == 2 and shape[1] > 5
 
 
+_LOOCV_KEYS = ("loocv", "loo", "leave-one-out", "leave_one_out", "leave one out")
+
+
+def compiled_kfold(kfold, constrain) -> int:
+    """.compiled_cv_kfold_arg (R/main.R:4003-4009, .cv_kfold_int 3982-3996):
+    -1 (leave one group out) for "loocv" or kfold >= number of groups, else
+    max(2, kfold)."""
+    n_groups = int(np.unique(np.asarray(constrain).ravel()).size)
+    if isinstance(kfold, str):
+        if kfold.strip().lower() in _LOOCV_KEYS:
+            return -1
+        raise ValueError("cross-validation: kfold must be a single integer or 'loocv'.")
+    if np.ndim(kfold) != 0 or kfold is None or not np.isfinite(float(kfold)):
+        raise ValueError("cross-validation: kfold must be a finite integer or 'loocv'.")
+    k = int(kfold)
+    if k < 0:
+        return -1
+    if k >= n_groups:
+        return -1
+    return max(2, k)
+
+
+def cv_folds(constrain, kfold=-1, labels=None, n_classes: int = 0, seed: int = 1):
+    """Row -> fold map (1-based) and fold count of pls_cv_predict_compiled
+    (src/fastPLS.cpp:6666-6775) through ``fpls_cv_folds``: leave one group out
+    for kfold < 0 or kfold >= #groups, otherwise a permutation of the groups
+    (stratified by the class of each group's first row when ``labels`` are
+    given) from R's uniforms after set.seed(seed)."""
+    lib = _lib.load()
+    cons = np.ascontiguousarray(np.asarray(constrain).ravel().astype(np.int32))
+    n = cons.size
+    fold = np.zeros(n, dtype=np.int32)
+    nf = ctypes.c_int32()
+    lab = None
+    if labels is not None:
+        lab = np.ascontiguousarray(np.asarray(labels).ravel().astype(np.int32))
+        if lab.size != n:
+            raise ValueError("labels must have one value per sample")
+    rc = lib.fpls_cv_folds(cons.ctypes.data, n, int(kfold), lab.ctypes.data if lab is not None
+                           else None, int(n_classes), int(seed) & 0xFFFFFFFF, fold.ctypes.data,
+                           ctypes.byref(nf))
+    if rc != _lib.FPLS_OK:
+        raise _lib.FplsError(rc, "fold assignment failed (labels must be 1..n_classes)")
+    return fold, nf.value
+
+
 def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method="simpls",
                    classifier="argmax", lda_ridge=1e-8, candidate_knn_k=10, candidate_tau=0.2,
                    candidate_alpha=0.75, candidate_top_m=20, seed=1, rsvd_oversample=10,
                    rsvd_power=1, precision="fp32", return_scores=False, device=0,
-                   engine="auto", north=1) -> Dict:
-    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151), leave-one-group-out
-    over the distinct values of ``constrain``.  ``method``: "plssvd",
+                   engine="auto", north=1, kfold=-1) -> Dict:
+    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) over the folds of
+    ``cv_folds``: ``kfold`` = -1 (or >= the number of groups) leaves one
+    distinct value of ``constrain`` out at a time, otherwise ``kfold`` folds
+    of a seeded permutation of the groups.  ``method``: "plssvd",
     "simpls", "opls" (the filter with ``north`` components per training fold,
     then SIMPLS with scaling "none") or "kernelpls" (the direct SIMPLS core,
     as the compiled CV fits it).
@@ -1003,13 +1057,20 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     rounding."""
     cons0 = np.asarray(constrain).ravel()
     ngroups = int(np.unique(cons0).size)
+    kfold = compiled_kfold(kfold, cons0) if isinstance(kfold, str) else int(kfold)
     if engine not in ("auto", "batched", "folds"):
         raise ValueError("engine must be 'auto', 'batched' or 'folds'")
+    nfold_est = ngroups if (kfold < 0 or kfold >= ngroups) else max(2, kfold)
     use_batched = engine == "batched" or (
-        engine == "auto" and _batched_eligible(Xdata, method, classifier, scaling, ngroups))
+        engine == "auto" and _batched_eligible(Xdata, method, classifier, scaling, nfold_est))
     if use_batched:
-        return _pls_cv_predict_batched(Xdata, Ydata, cons0, ncomp, scaling, seed, precision,
-                                       return_scores, device, method, classifier)
+        if _is_factor(Ydata):
+            _, codes0, C0 = _encode_factor(Ydata)
+            fold1, nfold = cv_folds(cons0, kfold, codes0, C0, seed)
+        else:
+            fold1, nfold = cv_folds(cons0, kfold, None, 0, seed)
+        return _pls_cv_predict_batched(Xdata, Ydata, fold1 - 1, nfold, ncomp, scaling, seed,
+                                       precision, return_scores, device, method, classifier)
     ctx = _lib.context(device)
     keep: list = []
     X = as_mat(Xdata, keep)
@@ -1037,7 +1098,7 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     opts.opls_north = int(north)
     ngroups = int(np.unique(cons).size)
     fold = np.empty(n, dtype=np.int32)
-    status = np.zeros(max(ngroups, 1), dtype=np.int32)
+    status = np.zeros(max(ngroups, kfold, 2), dtype=np.int32)
     nf = ctypes.c_int32()
     cls = np.zeros((n, L), dtype=np.int32, order="F") if classification else None
     want_scores = (not classification) or return_scores
@@ -1050,7 +1111,7 @@ def pls_cv_predict(Xdata, Ydata, constrain, ncomp=2, scaling="centering", method
     meth = methods[method]
     ctx.check(ctx.lib.fpls_cv_predict(
         ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
-        ctypes.byref(yv) if yv is not None else None, C, cons.ctypes.data, nc_ptr, L, -1, meth,
+        ctypes.byref(yv) if yv is not None else None, C, cons.ctypes.data, nc_ptr, L, kfold, meth,
         ctypes.byref(opts), CLASSIFIERS[classifier], float(lda_ridge), int(candidate_knn_k),
         float(candidate_tau), float(candidate_alpha), int(candidate_top_m), fold.ctypes.data,
         status.ctypes.data, status.size, ctypes.byref(nf),
@@ -1116,14 +1177,14 @@ def _cv_summary(out, Ydata, cls, yp, classification, lev, n, L):
     return out
 
 
-def _pls_cv_predict_batched(Xdata, Ydata, cons, ncomp, scaling, seed, precision, return_scores,
-                            device, method, classifier):
+def _pls_cv_predict_batched(Xdata, Ydata, fold, F, ncomp, scaling, seed, precision,
+                            return_scores, device, method, classifier):
+    """The folds (0-based row -> fold, F folds) as one batched call; fold f
+    fits with seed + f like the fold-by-fold driver."""
     if method != "simpls" or classifier != "argmax":
         raise ValueError("the batched engine runs SIMPLS with the argmax classifier")
-    groups = np.unique(cons)  # sorted, as arma::unique / std::set
-    fold = np.searchsorted(groups, cons).astype(np.int32)
-    n = cons.size
-    F = groups.size
+    fold = np.asarray(fold, dtype=np.int32)
+    n = fold.size
     masks = np.ones((n, F), dtype=np.int8, order="F")
     masks[np.arange(n), fold] = 2
     res = cv_batched(Xdata, Ydata, masks, ncomp, None, scaling, seed, precision,
@@ -1214,16 +1275,31 @@ def pls_double_cv(Xdata, Ydata, ncomp=2, outer_fold=None, inner_fold=None, scali
             "fold": ofold, "backend": "hip", "method": "simpls"}
 
 
+def _check_finite(Xdata):
+    """The reference's CV entry points stop on missing values
+    (R/main.R:6426-6428); non-finite X is refused on the host."""
+    if type(Xdata).__module__.startswith("torch"):
+        import torch
+        ok = bool(torch.isfinite(Xdata).all().item())
+    else:
+        ok = bool(np.isfinite(np.asarray(Xdata, dtype=np.float64)).all())
+    if not ok:
+        raise ValueError("Missing values are present")
+
+
 def optim_pls_cv(Xdata, Ydata, ncomp=2, constrain=None, scaling="centering", method="simpls",
-                 backend="hip", seed=1, classifier="argmax", **kw) -> Dict:
-    """optim.pls.cv (R/main.R:6399-6537) with leave-one-group-out folds given by
-    ``constrain`` (default: one group per sample, i.e. LOOCV)."""
+                 backend="hip", seed=1, kfold=10, classifier="argmax", **kw) -> Dict:
+    """optim.pls.cv (R/main.R:6399-6537): ``kfold`` folds (default 10; "loocv"
+    or kfold >= #groups leaves one group out) over the groups of
+    ``constrain`` (default one group per sample), stratified by class for a
+    factor response, drawn after set.seed(seed) as the compiled CV does."""
     _check_backend(backend)
+    _check_finite(Xdata)
     n = np.asarray(Xdata).shape[0] if not type(Xdata).__module__.startswith("torch") else Xdata.shape[0]
     if constrain is None:
         constrain = np.arange(1, n + 1)
     res = pls_cv_predict(Xdata, Ydata, constrain, ncomp, scaling, method, classifier, seed=seed,
-                         **kw)
+                         kfold=compiled_kfold(kfold, constrain), **kw)
     vals = res["metrics"]["metric_value"]
     bi = _cv_best_index(res["metrics"]["metric_name"], vals)
     res["optim_comp"] = int(res["ncomp"][bi])
@@ -1236,11 +1312,12 @@ def optim_pls_cv(Xdata, Ydata, ncomp=2, constrain=None, scaling="centering", met
     return res
 
 
-def pls_single_cv(Xdata, Ydata, constrain=None, ncomp=2, scaling="centering", method="simpls",
-                  backend="hip", seed=1, classifier="argmax", **kw) -> Dict:
+def pls_single_cv(Xdata, Ydata, constrain=None, ncomp=2, kfold=10, scaling="centering",
+                  method="simpls", backend="hip", seed=1, classifier="argmax", **kw) -> Dict:
     """pls.single.cv (R/main.R:4316-4435): cross-validated predictions for the
-    given ncomp values with leave-one-group-out folds."""
-    return optim_pls_cv(Xdata, Ydata, ncomp, constrain, scaling, method, backend, seed,
+    given ncomp values over ``kfold`` folds (default 10) of the groups of
+    ``constrain``."""
+    return optim_pls_cv(Xdata, Ydata, ncomp, constrain, scaling, method, backend, seed, kfold,
                         classifier, **kw)
 
 

@@ -1,5 +1,8 @@
 // Cross-validated prediction: pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151)
-// with leave-one-group-out folds (group_fold(j) = j, :6733-6736).  X stays
+// with the reference's group -> fold map (:6719-6770): leave one group out, or
+// k folds over a permutation of the groups (stratified by the class of each
+// group's first row for classification) drawn from R's uniform generator as
+// set.seed(seed) leaves it (R/main.R:3883).  X stays
 // resident; each fold gathers its train / test rows on the device, fits with
 // seed + f, and predicts every ncomp slice from the one fit: compact argmax
 // classes (:4115-4168), the LDA head (:6975-7004) or the candidate-kNN head
@@ -26,6 +29,110 @@ __global__ void k_gather_rows(const T* __restrict__ X, int64_t ld, int64_t cols,
 
 static unsigned g1(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
+}
+
+// ---------------------------------------------------------------- folds
+// R's default uniform generator after set.seed(seed) (R's RNG.c: RNG_Init,
+// MT_genrand, fixup): the seed is scrambled by 50 steps of s <- 69069 s + 1,
+// the next 625 steps fill the position slot and the 624 Mersenne-Twister
+// words, and each draw is a tempered MT19937 word times 2^-32, kept inside
+// (0, 1).
+struct RUnif {
+  uint32_t mt[624];
+  int mti = 624;
+  explicit RUnif(int32_t seed) {
+    uint32_t s = (uint32_t)seed;
+    for (int j = 0; j < 50; ++j) s = 69069u * s + 1u;
+    s = 69069u * s + 1u;  // the position slot, reset to 624
+    for (int j = 0; j < 624; ++j) {
+      s = 69069u * s + 1u;
+      mt[j] = s;
+    }
+  }
+  double next() {
+    if (mti >= 624) {
+      for (int k = 0; k < 624; ++k) {
+        const uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+        mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      mti = 0;
+    }
+    uint32_t y = mt[mti++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    const double x = (double)y * 2.3283064365386963e-10;
+    const double half = 0.5 * 2.328306437080797e-10;
+    if (x <= 0.0) return half;
+    if (1.0 - x <= 0.0) return 1.0 - half;
+    return x;
+  }
+};
+
+// samplewithoutreplace (src/fastPLS.cpp:4498-4509): size draws, each taking
+// the element at floor(u * rest) out of the remaining ones.
+static std::vector<int> sample_without_replace(std::vector<int> yy, int size, RUnif& rng) {
+  std::vector<int> xx(size);
+  int rest = (int)yy.size();
+  for (int ii = 0; ii < size; ++ii) {
+    const int it = (int)(rng.next() * rest);
+    xx[ii] = yy[it];
+    yy.erase(yy.begin() + it);
+    --rest;
+  }
+  return xx;
+}
+
+// Row -> fold (0-based) of pls_cv_predict_compiled (src/fastPLS.cpp:6666-6775);
+// labels (1-based, may be null) stratify the permutation by the class of each
+// group's first row.  Returns the fold count.
+static int assign_folds(const int32_t* constrain, int64_t n, int kfold, const int32_t* labels,
+                        int n_classes, int32_t seed, std::vector<int>& fold) {
+  const bool loo_requested = kfold < 0;
+  if (!loo_requested && kfold < 2) kfold = 2;
+  std::set<int32_t> uniq(constrain, constrain + n);
+  std::map<int32_t, int> gidx;
+  for (int32_t g : uniq) gidx.emplace(g, (int)gidx.size());
+  const int ngroups = (int)uniq.size();
+  std::vector<int> group_of(n);
+  for (int64_t i = 0; i < n; ++i) group_of[i] = gidx[constrain[i]];
+  std::vector<int> group_fold(ngroups, 0);
+  int nfold;
+  if (loo_requested || kfold >= ngroups) {
+    nfold = std::max(ngroups, 1);
+    for (int j = 0; j < ngroups; ++j) group_fold[j] = j;
+  } else {
+    nfold = kfold;
+    RUnif rng(seed);
+    if (labels && n_classes > 1) {
+      std::vector<int64_t> first(ngroups, -1);
+      for (int64_t i = 0; i < n; ++i)
+        if (first[group_of[i]] < 0) first[group_of[i]] = i;
+      std::vector<std::vector<int>> by_class(n_classes);
+      for (int j = 0; j < ngroups; ++j) {
+        int cls = labels[first[j]];
+        cls = std::min(std::max(cls, 1), n_classes);
+        by_class[cls - 1].push_back(j);
+      }
+      for (int cls = 0; cls < n_classes; ++cls) {
+        const int ng = (int)by_class[cls].size();
+        if (ng < 1) continue;
+        std::vector<int> frame(ng);
+        for (int q = 0; q < ng; ++q) frame[q] = q + 1;
+        const std::vector<int> perm = sample_without_replace(frame, ng, rng);
+        for (int pos = 0; pos < ng; ++pos) group_fold[by_class[cls][perm[pos] - 1]] = pos % kfold;
+      }
+    } else {
+      std::vector<int> frame(ngroups);
+      for (int q = 0; q < ngroups; ++q) frame[q] = q + 1;
+      const std::vector<int> perm = sample_without_replace(frame, ngroups, rng);
+      for (int j = 0; j < ngroups; ++j) group_fold[j] = (perm[j] - 1) % kfold;
+    }
+  }
+  fold.resize(n);
+  for (int64_t i = 0; i < n; ++i) fold[i] = group_fold[group_of[i]];
+  return nfold;
 }
 
 // Gathered copy of the rows idx of A (same dtype, packed ld = nr).
@@ -139,19 +246,11 @@ extern "C" int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       for (int& v : nc) v = std::max(v, 1);
     }
     // groups -> folds (arma::unique sorts ascending)
-    std::set<int32_t> uniq(constrain, constrain + n);
-    std::map<int32_t, int> gidx;
-    for (int32_t g : uniq) gidx.emplace(g, (int)gidx.size());
-    const int ngroups = (int)uniq.size();
-    const bool logo = kfold < 0 || kfold >= ngroups;
-    FPLS_REQUIRE(logo, FPLS_ERR_ARG,
-                 "randomised fold assignment draws from R's RNG; pass the folds as constrain "
-                 "groups with kfold = -1 (leave one group out)");
-    const int nfold = std::max(ngroups, 1);
+    std::vector<int> fold;
+    const int nfold = assign_folds(constrain, n, kfold, classification ? yh.data() : nullptr,
+                                   (int)n_classes, (int32_t)o.seed, fold);
     FPLS_REQUIRE(status_out && status_cap >= nfold, FPLS_ERR_ARG,
                  "status buffer smaller than the number of folds");
-    std::vector<int> fold(n);
-    for (int64_t i = 0; i < n; ++i) fold[i] = gidx[constrain[i]];
     if (fold_out)
       for (int64_t i = 0; i < n; ++i) fold_out[i] = fold[i] + 1;
     if (n_folds_out) *n_folds_out = nfold;
@@ -393,4 +492,23 @@ extern "C" int fpls_cv_predict(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
     if (classification && class_pred) std::copy(cp.begin(), cp.end(), class_pred);
     if (store_scores && ypred) std::copy(yp.begin(), yp.end(), ypred);
   });
+}
+
+extern "C" int fpls_cv_folds(const int32_t* constrain, int64_t n, int32_t kfold,
+                             const int32_t* labels, int32_t n_classes, uint32_t seed,
+                             int32_t* fold_out, int32_t* n_folds_out) {
+  if (!constrain || !fold_out || n < 1) return FPLS_ERR_ARG;
+  if (labels)
+    for (int64_t i = 0; i < n; ++i)
+      if (labels[i] < 1 || labels[i] > n_classes) return FPLS_ERR_LABEL_RANGE;
+  try {
+    std::vector<int> fold;
+    const int nfold = assign_folds(constrain, n, kfold, labels, (int)n_classes, (int32_t)seed,
+                                   fold);
+    for (int64_t i = 0; i < n; ++i) fold_out[i] = fold[i] + 1;
+    if (n_folds_out) *n_folds_out = nfold;
+  } catch (const std::exception&) {
+    return FPLS_ERR_ARG;
+  }
+  return FPLS_OK;
 }

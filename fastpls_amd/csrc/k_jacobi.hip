@@ -197,14 +197,19 @@ __global__ __launch_bounds__(256) void k_jac_nearid(const int* __restrict__ ns,
         }
       }
       const int j = cj;
+      // free rows rank by |entry| with NaN below every number (-0.5) and
+      // held rows below that (-1): a total order on (value, row), so every
+      // lane's butterfly ends on the same free row
       double v = -1.0;
       int ix = JB_PAIR;
       if (r0 < s && !tk0) {
-        v = fabs(Zs[r0 + j * s]);
+        const double a = fabs(Zs[r0 + j * s]);
+        v = a >= 0.0 ? a : -0.5;
         ix = r0;
       }
       if (r1 < s && !tk1) {
-        const double a = fabs(Zs[r1 + j * s]);
+        double a = fabs(Zs[r1 + j * s]);
+        a = a >= 0.0 ? a : -0.5;
         if (a > v) {
           v = a;
           ix = r1;
@@ -231,7 +236,7 @@ __global__ __launch_bounds__(256) void k_jac_nearid(const int* __restrict__ ns,
   }
   __syncthreads();
   for (int e = t; e < s * s; e += 256) {
-    const int i = e % s, j = e / s, d = dest[j];
+    const int i = e % s, j = e / s, d = min(max(dest[j], 0), s - 1);
     const double sg = Zs[d + j * s] >= 0.0 ? 1.0 : -1.0;
     Zq[i + d * s] = sg * Zs[i + j * s];
   }
@@ -340,6 +345,15 @@ void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, dou
          Vw.get());
   double* Vcur = Vw.get();
   double* Vnext = Vt.get();
+  {
+    // refuse a non-finite matrix before the first sweep
+    launch(ctx, "jacobi_offnorm", k_jac_offnorm, dim3(1), dim3(1024), 0, n,
+           (const double*)Aw.get(), norms.get());
+    double h0[2];
+    to_host(ctx, h0, norms.get(), 2);
+    FPLS_REQUIRE(std::isfinite(h0[1]), FPLS_ERR_ARG,
+                 "symmetric eigensolver: the matrix holds non-finite values (non-finite X or Y?)");
+  }
   const dim3 cgrid((unsigned)((n + 63) / 64), (unsigned)(JB_PAIR / 64), (unsigned)q);
   const int max_sweeps = 40;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {

@@ -28,11 +28,14 @@ __global__ void k_sqrt_clamp(const double* __restrict__ w, double* __restrict__ 
 
 void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l) {
   DBuf<double> G(ctx, (size_t)l * l), Linv(ctx, (size_t)l * l), tmp(ctx, (size_t)rows * l);
-  DBuf<int32_t> st(ctx, 1);
+  // a Cholesky that still breaks down after its shifted retries (status -1,
+  // e.g. a non-finite Gram) fails the entry point when it returns
+  int32_t* st = ctx->defer_check(3, FPLS_ERR_SOLVE,
+                                 "Cholesky-QR of the range finder failed (non-finite sketch?)", -1);
   for (int pass = 0; pass < 3; ++pass) {
     gemm_f64(ctx, true, false, l, l, rows, 1.0, Y, rows, nullptr, Y, rows, nullptr, 0.0, G.get(),
              l);
-    chol_inv_lower(ctx, G.get(), l, l, rows, pass == 0 ? -1.0 : 0.0, Linv.get(), st.get());
+    chol_inv_lower(ctx, G.get(), l, l, rows, pass == 0 ? -1.0 : 0.0, Linv.get(), st + pass);
     // Y <- Y L^{-T}
     gemm_f64(ctx, false, true, rows, l, l, 1.0, Y, rows, nullptr, Linv.get(), l, nullptr, 0.0,
              tmp.get(), rows);
@@ -86,12 +89,13 @@ void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols
   const int64_t max_rank = std::min(rows, cols);
   const int target = (int)std::min<int64_t>(max_rank, std::max(k, 1));
   const int l = (int)std::min<int64_t>(max_rank, (int64_t)target + std::max(oversample, 0));
-  // dispatch rules of src/svd_iface.cpp:64-70 and src/svd_cpu_rsvd.cpp:100-105
-  if (use_full || max_rank < 6 || l >= max_rank) {
+  // dispatch rules of src/svd_iface.cpp:64-70 and src/svd_cpu_rsvd.cpp:100-105;
+  // a sketch wider than the one-workgroup factorisations (l > 128) takes the
+  // exact SVD, whose leading factors the range finder approximates
+  if (use_full || max_rank < 6 || l >= max_rank || l > kJacobiMax) {
     svd_exact_dev(ctx, A, lda, rows, cols, target, out, left_only);
     return;
   }
-  FPLS_REQUIRE(l <= kJacobiMax, FPLS_ERR_ARG, "rSVD sketch width k + oversample must be <= 128");
   std::vector<double> om((size_t)cols * l);
   if (omega_host) std::copy(omega_host, omega_host + om.size(), om.begin());
   else gaussian_host(om.data(), (int64_t)om.size(), seed);
@@ -288,10 +292,12 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       const double* src = static_cast<const double*>(omega->ptr);
       for (int64_t j = 0; j < l; ++j)
         for (int64_t i = 0; i < m; ++i) om[i + j * m] = src[i + j * omega->ld];
-    } else {
-      gaussian_host(om.data(), (int64_t)om.size(), o.seed);
     }
-    const bool exact = o.svd_method == FPLS_SVD_EXACT || l >= max_rank;
+    // wider sketches than the one-workgroup factorisations hold take the
+    // exact SVD of the formed S (as rsvd_dev does)
+    const bool exact = o.svd_method == FPLS_SVD_EXACT || l >= max_rank || l > kJacobiMax;
+    // the exact route never reads the sketch: draw it only for the range finder
+    if (!omega && !exact) gaussian_host(om.data(), (int64_t)om.size(), o.seed);
     if (exact || o.xprod_route == 1) {
       // S formed once (one pass over X), then the exact SVD or the range
       // finder on S with the same sketch and power schedule
@@ -300,7 +306,6 @@ extern "C" int fpls_rsvd_xprod(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       rsvd_dev(c, S.get(), p, p, m, target, o.oversample, o.power, o.seed, om.data(),
                Vt == nullptr, exact, r);
     } else {
-      FPLS_REQUIRE(l <= kJacobiMax, FPLS_ERR_ARG, "rSVD sketch width k + oversample must be <= 128");
       const int64_t n = X.rows;
       // Y and X operands in FP64 scratch for the small products
       DBuf<double> Yd(c, (size_t)n * m);

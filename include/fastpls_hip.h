@@ -419,10 +419,19 @@ int fpls_gaussian_y_classes(fpls_ctx* ctx, const fpls_mat* Z, const double* code
                             int32_t n_classes, int32_t* labels, double* scores);
 
 /* ---- cross-validation ----------------------------------------------------------- */
-/* pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out
-   folds: the distinct values of constrain (n, host) are the folds (kfold = -1,
-   or kfold >= number of groups).  Randomised fold assignment draws from R's
-   RNG and is rejected (FPLS_ERR_ARG).  method: FPLS_METHOD_PLSSVD or
+/* The row -> fold map of pls_cv_predict_compiled (src/fastPLS.cpp:6666-6775):
+   the distinct values of constrain (n, host) are the groups; kfold < 0 or
+   kfold >= number of groups leaves one group out (fold j = group j in
+   ascending order); otherwise kfold (at least 2) folds take a permutation of
+   the groups (samplewithoutreplace, src/fastPLS.cpp:4498-4509), per class of
+   each group's first row when labels (1-based, host, may be NULL) and
+   n_classes > 1 are given, drawn from R's Mersenne-Twister uniforms as
+   set.seed(seed) leaves them (R/main.R:3883).  fold_out (n, 1-based), and the
+   fold count.  Host only: needs no device. */
+int fpls_cv_folds(const int32_t* constrain, int64_t n, int32_t kfold, const int32_t* labels,
+                  int32_t n_classes, uint32_t seed, int32_t* fold_out, int32_t* n_folds_out);
+/* pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with the folds of
+   fpls_cv_folds (kfold, opts->seed).  method: FPLS_METHOD_PLSSVD or
    FPLS_METHOD_SIMPLS; classifier: 0 argmax, 1 LDA, 2 candidate kNN (labels
    required).  Each fold fits with seed + fold.  Outputs: fold_out (n, 1-based),
    status_out (>= number of folds: 1 ok, 2 empty, 3 no training rows,

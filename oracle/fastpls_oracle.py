@@ -20,10 +20,12 @@ The Gaussian sketch uses the libstdc++ ``std::mt19937`` +
 ``oracle/rng.cpp`` (built by ``oracle/Makefile``), so sketches are bit-equal to
 the reference's on the same seed.
 
-Pinned by: tests/test_oracle_*.py (closed forms, brute force, library
-routines, the reference's own tolerance tests restated, and the committed
-golden fixtures).  Functions without such a pin are marked
-"parity unpinned" in their docstring.
+Pinned by: tests/test_oracle*.py (closed forms, brute force, library
+routines, the reference's own tolerance tests restated, hand-worked cases) and
+the golden fixtures under tests/golden/ (textbook SIMPLS / PLSSVD written
+independently of this module, and the values R prints for its uniform
+generator).  Functions without such a pin are marked "parity unpinned" in
+their docstring.
 """
 from __future__ import annotations
 
@@ -911,11 +913,104 @@ def nearest_code_classes(Z, codes) -> np.ndarray:
 # end-to-end pls() flows used by the tests and the CPU baseline (R/main.R:5860-6314)
 # ---------------------------------------------------------------------------
 
+class RUniform:
+    """R's default uniform generator after ``set.seed(seed)`` (R's RNG.c:
+    RNG_Init scrambles the seed by 50 steps of s <- 69069 s + 1 and fills the
+    Mersenne-Twister position slot and 624 state words with the next 625 steps;
+    MT_genrand returns the tempered MT19937 word times 2^-32; fixup keeps the
+    value inside (0, 1)).  The reference draws its CV permutations from it
+    (unif_rand in samplewithoutreplace, src/fastPLS.cpp:4498-4509, after
+    set.seed(seed) in R/main.R:3883).  Pinned by R's known runif() outputs."""
+
+    _M32 = 0xFFFFFFFF
+
+    def __init__(self, seed: int):
+        s = int(seed) & self._M32
+        for _ in range(50):
+            s = (69069 * s + 1) & self._M32
+        words = []
+        for _ in range(625):
+            s = (69069 * s + 1) & self._M32
+            words.append(s)
+        self.mt = words[1:]
+        self.mti = 624
+
+    def _twist(self):
+        mt = self.mt
+        for k in range(624):
+            y = (mt[k] & 0x80000000) | (mt[(k + 1) % 624] & 0x7FFFFFFF)
+            mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ (0x9908B0DF if y & 1 else 0)
+        self.mti = 0
+
+    def unif_rand(self) -> float:
+        if self.mti >= 624:
+            self._twist()
+        y = self.mt[self.mti]
+        self.mti += 1
+        y ^= y >> 11
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= y >> 18
+        x = y * 2.3283064365386963e-10
+        i2_32m1 = 2.328306437080797e-10
+        if x <= 0.0:
+            return 0.5 * i2_32m1
+        if 1.0 - x <= 0.0:
+            return 1.0 - 0.5 * i2_32m1
+        return x
+
+
+def samplewithoutreplace(yy, size: int, rng: RUniform):
+    """samplewithoutreplace (src/fastPLS.cpp:4498-4509)."""
+    yy = list(yy)
+    out = []
+    for _ in range(size):
+        it = int(rng.unif_rand() * len(yy))
+        out.append(yy.pop(it))
+    return out
+
+
+def cv_folds(constrain, kfold: int, labels=None, n_classes: int = 0, seed: int = 1):
+    """Row -> fold map (0-based) and fold count of pls_cv_predict_compiled
+    (src/fastPLS.cpp:6666-6775): kfold < 0 or kfold >= #groups leaves one
+    group out; otherwise each class's groups (class of the group's first row)
+    are permuted and dealt round-robin, or all groups when unstratified."""
+    cons = np.asarray(constrain).ravel()
+    groups = np.unique(cons)
+    g_of = np.searchsorted(groups, cons)
+    ng = groups.size
+    kfold = int(kfold)
+    loo = kfold < 0
+    if not loo and kfold < 2:
+        kfold = 2
+    group_fold = np.zeros(ng, dtype=np.int64)
+    if loo or kfold >= ng:
+        return g_of.astype(np.int64), max(ng, 1)
+    rng = RUniform(seed)
+    if labels is not None and n_classes > 1:
+        lab = np.asarray(labels, dtype=np.int64).ravel()
+        first = np.array([np.nonzero(g_of == j)[0][0] for j in range(ng)])
+        cls = np.clip(lab[first], 1, n_classes)
+        for c in range(1, n_classes + 1):
+            members = [j for j in range(ng) if cls[j] == c]
+            if not members:
+                continue
+            perm = samplewithoutreplace(range(1, len(members) + 1), len(members), rng)
+            for pos, pv in enumerate(perm):
+                group_fold[members[pv - 1]] = pos % kfold
+    else:
+        perm = samplewithoutreplace(range(1, ng + 1), ng, rng)
+        for j in range(ng):
+            group_fold[j] = (perm[j] - 1) % kfold
+    return group_fold[g_of], kfold
+
+
 def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False,
                    n_classes=None, classifier="argmax", scaling=1, seed=1, lda_ridge=1e-8,
-                   knn_k=10, tau=0.2, alpha=0.75, top_m=20, north=1):
-    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with leave-one-group-out
-    folds (group_fold(j) = j over the sorted distinct constrain values, :6720-6736).
+                   knn_k=10, tau=0.2, alpha=0.75, top_m=20, north=1, kfold=-1):
+    """pls_cv_predict_compiled (src/fastPLS.cpp:6615-7151) with the folds of
+    ``cv_folds`` (kfold < 0: leave one group out, group_fold(j) = j over the
+    sorted distinct constrain values, :6720-6736).
     Y holds 1-based labels when ``classification``.  Returns fold (1-based),
     status, class_pred (n x L) and Ypred (n x m x L, argmax / regression).
     method "opls" filters each training fold (opls_filter_cpp with ``north``
@@ -925,8 +1020,6 @@ def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False
     X = np.asarray(X, dtype=np.float64)
     n = X.shape[0]
     cons = np.asarray(constrain).ravel()
-    groups = np.unique(cons)
-    fold = np.searchsorted(groups, cons)
     L = len(np.atleast_1d(ncomp))
     nc = np.array(np.atleast_1d(ncomp), dtype=np.int64)
     if classification:
@@ -938,12 +1031,22 @@ def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False
     else:
         Yd = np.asarray(Y, dtype=np.float64).reshape(n, -1)
         m = Yd.shape[1]
+    fold, nfold = cv_folds(cons, kfold, y if classification else None,
+                           C if classification else 0, seed)
     if method == "plssvd":
         nc = np.minimum(np.maximum(nc, 1), min(n, X.shape[1], m))
     class_pred = np.zeros((n, L), dtype=np.int32)
     Ypred = np.zeros((n, m, L))
-    status = np.zeros(len(groups), dtype=np.int32)
-    for f in range(len(groups)):
+    # decision margin per row and slice (inf where no decision is made): the
+    # gap between the best and the second class score, for the tie log
+    margin = np.full((n, L), np.inf)
+
+    def gap(scores):
+        srt = np.sort(scores, axis=1)
+        return srt[:, -1] - srt[:, -2] if scores.shape[1] > 1 else np.full(scores.shape[0], np.inf)
+
+    status = np.zeros(nfold, dtype=np.int32)
+    for f in range(nfold):
         te = np.nonzero(fold == f)[0]
         tr = np.nonzero(fold != f)[0]
         if te.size == 0:
@@ -982,23 +1085,33 @@ def pls_cv_predict(X, Y, constrain, ncomp, method="simpls", classification=False
                 ks = [min(int(k), kmax) for k in nc]
                 lda = lda_train_prefix(Ttr, y[tr], C, ks, lda_ridge)
                 for s, k in enumerate(ks):
-                    class_pred[te, s] = lda_labels_from_scores(Tte[:, :k] @ lda[s]["linear"].T,
-                                                               lda[s]["constants"])
+                    sc = Tte[:, :k] @ lda[s]["linear"].T
+                    class_pred[te, s] = lda_labels_from_scores(sc, lda[s]["constants"])
+                    margin[te, s] = gap(sc + lda[s]["constants"])
             else:
                 for s, k in enumerate(nc):
                     k = min(int(k), kmax)
                     a = candidate_row_l2(candidate_score_space(mod, Ttr, k))
                     b = candidate_row_l2(candidate_score_space(mod, Tte, k))
                     cen = candidate_centroids(a, y[tr], C)
-                    r = candidate_knn_predict(b, a, y[tr], cen, None, 1, top_m, knn_k, tau, alpha)
+                    r = candidate_knn_predict(b, a, y[tr], cen, None, 2, top_m, knn_k, tau, alpha)
                     class_pred[te, s] = r["top_index"][:, 0]
+                    mg = r["top_score"][:, 0] - r["top_score"][:, 1]
+                    tm = max(2, min(int(top_m), C))
+                    if C > tm:  # a class at the candidate boundary may swap in
+                        bs = -np.sort(-(b @ cen.T), axis=1)
+                        mg = np.minimum(mg, bs[:, tm - 1] - bs[:, tm])
+                    margin[te, s] = mg
         else:
             yp = pls_predict(mod, Xte)
             Ypred[te] = yp
             if classification:
                 class_pred[te] = (np.argmax(yp, axis=1) + 1).astype(np.int32)
+                for s in range(L):
+                    margin[te, s] = gap(yp[:, :, s])
         status[f] = 1
-    return {"fold": fold + 1, "status": status, "class_pred": class_pred, "Ypred": Ypred}
+    return {"fold": fold + 1, "status": status, "class_pred": class_pred, "Ypred": Ypred,
+            "margin": margin}
 
 
 def pls_plssvd_lda(Xtrain, ytrain, Xtest, ncomp, lda_ridge=1e-8, scaling=1, seed=1,

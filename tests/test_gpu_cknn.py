@@ -85,11 +85,12 @@ def test_pls_cknn_pipeline(ctx):
     assert np.mean(res["Ypred_index"][:, 0] == yte) > 0.9
 
 
-def test_label_aware_cknn_many_classes_large_ncomp(ctx):
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_label_aware_cknn_many_classes_large_ncomp(ctx, precision):
     """The large-class-count label-aware PLSSVD + candidate-kNN path at reduced
     size: Zipf class sizes, ncomp above 128 (exact SVD and the latent solve
-    take the block-Jacobi eigensolver).  Against the oracle's label-aware
-    model and candidate-kNN predictions."""
+    take the block-Jacobi eigensolver), in both contraction forms.  Against
+    the oracle's label-aware model and candidate-kNN predictions."""
     import fastpls_amd as F
     rng = np.random.default_rng(41)
     C, p, n, nt = 180, 256, 9000, 400
@@ -104,7 +105,8 @@ def test_label_aware_cknn_many_classes_large_ncomp(ctx):
     ytr = ytr.astype(np.int32)
     ncomp = 150
     m = F.pls(Xtr, ytr, method="plssvd", ncomp=ncomp, classifier="cknn", label_aware=True,
-              candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20)
+              candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
+              precision=precision)
     res = F.predict(m, Xte, top=5)
     ref = O.plssvd_label_aware(Xtr.astype(np.float64), ytr, [ncomp])
     assert sin_theta(m["R"][:, :ncomp], ref["R"][:, :ncomp]) < 1e-4

@@ -143,8 +143,11 @@ def test_model_topk_matches_oracle(ctx, method, dtype):
     tol = 1e-8 if dtype == np.float64 else 1e-4
     np.testing.assert_allclose(res["Ypred_top_score"], os_, rtol=0,
                                atol=tol * float(np.abs(os_).max()))
-    agree = np.mean(res["Ypred_top"][:, 0, :] == oi[:, 0, :])
-    assert agree > (0.999 if dtype == np.float64 else 0.99)
+    # the first-ranked class equals the oracle's except where its FP64 top-2
+    # gap is within the contraction's error (documented near-ties)
+    g12 = os_[:, 0, :] - os_[:, 1, :]
+    bad = (res["Ypred_top"][:, 0, :] != oi[:, 0, :]) & (g12 > tol * float(np.abs(os_).max()))
+    assert bad.sum() == 0, f"{bad.sum()} non-tie mismatches"
 
 
 def test_predict_routing_like_reference_tests(ctx):

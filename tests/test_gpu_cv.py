@@ -74,8 +74,9 @@ def test_cv_classifier_heads(ctx, classifier):
                            candidate_knn_k=5, candidate_top_m=4)
     ref = O.pls_cv_predict(Xtr, ytr, g, ncomp, method="plssvd", classification=True,
                            n_classes=6, classifier=classifier, knn_k=5, top_m=4)
-    agree = np.mean(res["class_pred"] == ref["class_pred"])
-    assert agree > 0.99, agree
+    # labels equal the oracle's except documented near-ties (FP64 margin)
+    bad = (res["class_pred"] != ref["class_pred"]) & (ref["margin"] > 1e-7)
+    assert bad.sum() == 0, f"{bad.sum()} non-tie mismatches"
 
 
 def test_cv_degenerate_fold_status(ctx):

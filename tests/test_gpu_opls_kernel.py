@@ -182,4 +182,5 @@ def test_cv_with_opls_and_kernelpls_methods(ctx, method):
         rr = O.pls_cv_predict(X, y, g, [2, 3], method="opls", north=1, classification=True,
                               n_classes=3, classifier="lda")
         np.testing.assert_array_equal(rc["status"], rr["status"])
-        assert np.mean(rc["class_pred"] == rr["class_pred"]) > 0.99
+        bad = (rc["class_pred"] != rr["class_pred"]) & (rr["margin"] > 1e-7)
+        assert bad.sum() == 0, f"{bad.sum()} non-tie mismatches"

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import fastpls_oracle as O
-from tests.helpers import align_signs, rel_err
+from tests.helpers import label_mismatches_excused, align_signs, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -72,4 +72,8 @@ def test_simpls_classification_fp32_labels(ctx):
     assert rel_err(m["B"], ref["B"]) < 1e-4
     yp_ref = O.pls_predict(ref, Xte.astype(np.float64))
     lab_ref = np.argmax(yp_ref, axis=1) + 1
-    assert np.mean(m["Ypred_index"] == lab_ref) > 0.99
+    # FP32 contractions: rows whose FP64 top-2 margin is under 1e-4 are ties
+    for a in range(2):
+        _, unexcused = label_mismatches_excused(m["Ypred_index"][:, a], lab_ref[:, a],
+                                                yp_ref[:, :, a], 1e-4)
+        assert not unexcused, unexcused

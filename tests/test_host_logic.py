@@ -49,3 +49,33 @@ def test_model_import_field_checks():
         g.pop(missing)
         with pytest.raises(ValueError, match=missing):
             F.model_from_fields(g)
+    # centring vectors must have one value per predictor (host over-read otherwise)
+    g = dict(f, pls_method="simpls", Q=np.zeros((2, 3)), mX=np.zeros(2))
+    with pytest.raises(ValueError, match="one value per predictor"):
+        F.model_from_fields(g)
+    g = dict(f, pls_method="simpls", Q=np.zeros((2, 3)), vX=np.ones(5))
+    with pytest.raises(ValueError, match="one value per predictor"):
+        F.model_from_fields(g)
+
+
+def test_cv_kfold_argument_rules():
+    """.compiled_cv_kfold_arg / .cv_kfold_int (R/main.R:3982-4009)."""
+    cons = np.repeat(np.arange(6), 2)
+    assert F.compiled_kfold("loocv", cons) == -1
+    assert F.compiled_kfold(" LOO ", cons) == -1
+    assert F.compiled_kfold(6, cons) == -1          # kfold >= number of groups
+    assert F.compiled_kfold(12, cons) == -1
+    assert F.compiled_kfold(3, cons) == 3
+    assert F.compiled_kfold(1, cons) == 2           # at least two folds
+    assert F.compiled_kfold(-1, cons) == -1
+    with pytest.raises(ValueError, match="single integer or 'loocv'"):
+        F.compiled_kfold("ten", cons)
+    with pytest.raises(ValueError, match="finite integer"):
+        F.compiled_kfold(float("nan"), cons)
+
+
+def test_cv_rejects_missing_values():
+    X = np.ones((6, 3))
+    X[2, 1] = np.nan
+    with pytest.raises(ValueError, match="Missing values are present"):
+        F.optim_pls_cv(X, np.arange(6.0), ncomp=1)
