@@ -199,9 +199,16 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     fn = ctx.lib.fpls_plssvd_fit if method == "plssvd" else ctx.lib.fpls_simpls_fit
     if method not in ("plssvd", "simpls"):
         raise ValueError("method must be 'plssvd' or 'simpls'")
-    ctx.check(fn(ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
-                 ctypes.byref(yv) if yv is not None else None, C, nc_ptr, nc_arr.size,
-                 ctypes.byref(opts), ctypes.byref(h)))
+    fused_lda = method == "plssvd" and classification and classifier == "lda"
+    if fused_lda:
+        # the fit and the LDA head in one call (one launch for their solves)
+        ctx.check(ctx.lib.fpls_plssvd_lda_fit(ctx.h, ctypes.byref(X), ctypes.byref(yv), C,
+                                              nc_ptr, nc_arr.size, ctypes.byref(opts),
+                                              float(lda_ridge), ctypes.byref(h)))
+    else:
+        ctx.check(fn(ctx.h, ctypes.byref(X), ctypes.byref(Ym) if Ym is not None else None,
+                     ctypes.byref(yv) if yv is not None else None, C, nc_ptr, nc_arr.size,
+                     ctypes.byref(opts), ctypes.byref(h)))
     model = FastPLS(ctx, h)
     model["classification"] = classification
     model["lev"] = lev
@@ -211,7 +218,12 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     model["ncomp"] = np.array(inf.ncomp[:inf.n_ncomp], dtype=np.int64)
     if fetch:
         _fetch_fields(model, bool(fit))
-    if classification and classifier in ("lda", "cknn"):
+    if fused_lda:
+        model["classification_rule"] = "lda_hip"
+        model["lda"] = {"ncomp": sorted(set(int(min(max(c, 1), inf.kmax))
+                                            for c in model["ncomp"])),
+                        "ridge": float(lda_ridge)}
+    elif classification and classifier in ("lda", "cknn"):
         _attach_head(model, classifier, yv, C, lda_ridge, candidate_knn_k, candidate_tau,
                      candidate_alpha, candidate_top_m)
     else:

@@ -288,6 +288,13 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                      int nmax, int count, double* w, int64_t strideW, double* V,
                      int64_t strideV);
+// Eigenpairs of the symmetric tridiagonal (tri: d, e, tau, scale of a
+// one-workgroup Householder reduction, reflectors in refl, ld n) by
+// multisection bisection and inverse iteration, one wave per eigenvalue
+// (k_eig_ii.hip): eigenvalues descending in w, eigenvectors back-transformed
+// into V (ld n).  Sets *fallback when a cluster is wider than it takes.
+void tridiag_eig_ii(Ctx* ctx, int n, const double* tri, const double* refl, double* w, double* V,
+                    int* fallback);
 // Any order: block Jacobi over pairs of <= 64-row blocks, each pair
 // diagonalised by the one-workgroup solver (k_jacobi.hip).  Same outputs as
 // sym_eig (eigenvalues descending in w, eigenvectors in V, ld n).
@@ -305,6 +312,31 @@ void spd_solve_large(Ctx* ctx, const double* A, int64_t lda, const std::vector<i
 void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
                       int n_slices, int mode, const double* dvals, double ridge, double* X,
                       int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status);
+// One solve of spd_solve_slices' kind, on its own matrix: X (k x k, ld ldx) =
+// (A_k + shift I)^{-1} RHS by mode; lambda_out / status single entries.
+struct SpdJob {
+  const double* A;
+  int64_t lda;
+  int k, mode;
+  const double* d;
+  double ridge;
+  double* X;
+  int64_t ldx;
+  double* lambda_out;
+  int32_t* status;
+};
+constexpr int kSpdJobMax = 16;
+struct SpdJobs {
+  SpdJob j[kSpdJobMax];
+};
+// Independent solves (orders <= 128) in as few launches as possible, one
+// workgroup each.
+void spd_solve_jobs(Ctx* ctx, const std::vector<SpdJob>& jobs);
+// The jobs spd_solve_slices would run (orders <= 128), appended to jobs.
+void spd_slice_jobs(const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
+                    int n_slices, int mode, const double* dvals, double ridge, double* X,
+                    int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status,
+                    std::vector<SpdJob>& jobs);
 void scale_cols(Ctx* ctx, double* X, int64_t ld, int64_t rows, int64_t cols, const double* f,
                 bool divide);
 // Linv = chol(G + shift I)^{-1} (lower), with shift escalation on breakdown.

@@ -22,11 +22,7 @@ namespace fpls {
 
 constexpr int JMAX = kJacobiMax;  // 128
 
-// orders of the ncomp slices of one launch, passed by value
-constexpr int kSliceMax = 64;
-struct SliceOrders {
-  int k[kSliceMax];
-};
+
 constexpr int JLD = JMAX + 1;
 
 // --------------------------------------------------------------- slice solves
@@ -69,24 +65,27 @@ __device__ __forceinline__ double spd_bcast(double v, int src) {
   return __longlong_as_double(((unsigned long long)hi << 32) | lo);
 }
 
+// One workgroup per job of the launch (jobs passed by value).
 template <int KMAX>
-__global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ Ain, int64_t lda,
-                                                     SliceOrders ks, int mode,
-                                                     const double* __restrict__ dvals,
-                                                     double ridge, double* __restrict__ X,
-                                                     int64_t ldx, int64_t slice_stride,
-                                                     double* __restrict__ lambda_out,
-                                                     int32_t* __restrict__ status) {
+__global__ __launch_bounds__(SPD_T) void k_spd_solve(SpdJobs jobs) {
   static_assert(KMAX == 128, "k_spd_solve: 16 x 16 grid of 8 x 8 blocks");
+  const SpdJob& J = jobs.j[blockIdx.x];
+  const double* __restrict__ Ain = J.A;
+  const int64_t lda = J.lda;
+  const int mode = J.mode;
+  const double* __restrict__ dvals = J.d;
+  const double ridge = J.ridge;
+  const int64_t ldx = J.ldx;
+  double* __restrict__ lambda_out = J.lambda_out;
+  int32_t* __restrict__ status = J.status;
   __shared__ __align__(16) double L[KMAX * (KMAX + 1)];  // LU fallback; panels of the sweep
   __shared__ int piv[KMAX];
   __shared__ double shift;
   __shared__ int chol_ok;
   constexpr int LD = KMAX + 1;
-  const int s = blockIdx.x;
-  const int k = ks.k[s];
+  const int k = J.k;
   const int t = threadIdx.x, nt = blockDim.x;
-  double* Xs = X + (int64_t)s * slice_stride;
+  double* Xs = J.X;
   // trace of A_k for the shift rules: one diagonal entry per thread, summed in
   // a fixed order (a serial loop of dependent global loads in one thread
   // costs about a microsecond per entry)
@@ -109,7 +108,7 @@ __global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ 
       double lam = ridge;
       if (!isfinite(lam) || lam < 0.0) lam = 1e-8;
       sh = lam * scale;
-      if (lambda_out) lambda_out[s] = sh;
+      if (lambda_out) *lambda_out = sh;
     } else if (mode == 2) {
       const double tr = trace;
       double r = 1e-10 * (tr / (double)k);
@@ -269,7 +268,7 @@ __global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ 
         Xs[i + (int64_t)j * ldx] = (mode == 1) ? x : x * dvals[j];
       }
       FPLS_SPD_STAMP(5);
-      if (t == 0 && status) status[s] = 0;
+      if (t == 0 && status) *status = 0;
       return;
     }
   }
@@ -317,7 +316,7 @@ __global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ 
     __syncthreads();
   }
   if (singular) {
-    if (t == 0 && status) status[s] = 1;
+    if (t == 0 && status) *status = 1;
     return;
   }
   for (int c = t; c < k; c += nt) {
@@ -339,7 +338,7 @@ __global__ __launch_bounds__(SPD_T) void k_spd_solve(const double* __restrict__ 
       x[i] = v / L[i + i * LD];
     }
   }
-  if (t == 0 && status) status[s] = 2;  // solved through the LU fallback
+  if (t == 0 && status) *status = 2;  // solved through the LU fallback
 }
 
 void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
@@ -353,20 +352,38 @@ void spd_solve_slices(Ctx* ctx, const double* A, int64_t lda, int kmax, const st
                     lambda_out, status);
     return;
   }
-  // slice orders travel in the kernel arguments, kSliceMax per launch
-  for (int c0 = 0; c0 < n_slices; c0 += kSliceMax) {
-    const int cnt = std::min(kSliceMax, n_slices - c0);
-    SliceOrders so{};
+  std::vector<SpdJob> jobs;
+  spd_slice_jobs(A, lda, kmax, ks, n_slices, mode, dvals, ridge, X, ldx, slice_stride,
+                 lambda_out, status, jobs);
+  spd_solve_jobs(ctx, jobs);
+}
+
+void spd_slice_jobs(const double* A, int64_t lda, int kmax, const std::vector<int>& ks,
+                    int n_slices, int mode, const double* dvals, double ridge, double* X,
+                    int64_t ldx, int64_t slice_stride, double* lambda_out, int32_t* status,
+                    std::vector<SpdJob>& jobs) {
+  FPLS_REQUIRE(kmax >= 1 && kmax <= kJacobiMax, FPLS_ERR_ARG,
+               "spd_slice_jobs: order must be in 1..128");
+  FPLS_REQUIRE((int)ks.size() >= n_slices, FPLS_ERR_ARG, "spd_slice_jobs: missing orders");
+  for (int s = 0; s < n_slices; ++s) {
+    FPLS_REQUIRE(ks[s] >= 1 && ks[s] <= kmax, FPLS_ERR_ARG,
+                 "spd_solve_slices: order out of range");
+    jobs.push_back(SpdJob{A, lda, ks[s], mode, dvals, ridge, X + (int64_t)s * slice_stride, ldx,
+                          lambda_out ? lambda_out + s : nullptr, status ? status + s : nullptr});
+  }
+}
+
+void spd_solve_jobs(Ctx* ctx, const std::vector<SpdJob>& jobs) {
+  // jobs travel in the kernel arguments, kSpdJobMax per launch
+  for (size_t c0 = 0; c0 < jobs.size(); c0 += kSpdJobMax) {
+    const int cnt = (int)std::min<size_t>(kSpdJobMax, jobs.size() - c0);
+    SpdJobs js{};
     for (int i = 0; i < cnt; ++i) {
-      FPLS_REQUIRE(ks[c0 + i] >= 1 && ks[c0 + i] <= kmax, FPLS_ERR_ARG,
-                   "spd_solve_slices: order out of range");
-      so.k[i] = ks[c0 + i];
+      FPLS_REQUIRE(jobs[c0 + i].k >= 1 && jobs[c0 + i].k <= JMAX, FPLS_ERR_ARG,
+                   "spd_solve_jobs: order out of range");
+      js.j[i] = jobs[c0 + i];
     }
-    double* Xc = X + (int64_t)c0 * slice_stride;
-    double* lc = lambda_out ? lambda_out + c0 : nullptr;
-    int32_t* sc = status ? status + c0 : nullptr;
-    launch(ctx, "spd_solve", k_spd_solve<128>, dim3(cnt), dim3(SPD_T), 0, A, lda, so, mode,
-           dvals, ridge, Xc, ldx, slice_stride, lc, sc);
+    launch(ctx, "spd_solve", k_spd_solve<128>, dim3(cnt), dim3(SPD_T), 0, js);
   }
 }
 

@@ -70,12 +70,18 @@ __device__ __forceinline__ int bound(const double* x, int lo, int hi, double v, 
 // Batched: workgroup b solves problem b (A + b strideA, order ns[b] or n0,
 // results at w + b strideW and V + b strideV with ld = its order, workspace of
 // 3 nmax^2 doubles at ws + b strideWs).
+// tri_out (when set): stop after stage 1 and write d (n), e (n), tau (n) and
+// the scale of the problem at tri_out; the reflectors stay in the workspace
+// (Vr, ld n).  run_flag (when set): return at once unless *run_flag != 0.
 __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ Ain0, int64_t lda,
                                                     int64_t strideA, const int* __restrict__ ns,
                                                     int n0, int nmax, double* __restrict__ w0,
                                                     int64_t strideW, double* __restrict__ Vout0,
                                                     int64_t strideV, double* __restrict__ ws,
-                                                    int64_t strideWs) {
+                                                    int64_t strideWs,
+                                                    double* __restrict__ tri_out,
+                                                    const int* __restrict__ run_flag) {
+  if (run_flag && *run_flag == 0) return;
   const int64_t bi = blockIdx.x;
   const int n = ns ? ns[bi] : n0;
   const double* __restrict__ Ain = Ain0 + bi * strideA;
@@ -223,6 +229,15 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
     }
   }
   __syncthreads();
+  if (tri_out) {
+    for (int i = t; i < n; i += ENT) {
+      tri_out[i] = dv[i];
+      tri_out[n + i] = i + 1 < n ? ev[i] : 0.0;
+      tri_out[2 * n + i] = tauv[i];
+    }
+    if (t == 0) tri_out[3 * n] = scale;
+    return;
+  }
 
   FPLS_EIG_STAMP(1);
   // ------------------------------------------------ stage 2: divide and conquer
@@ -680,9 +695,27 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
   FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
                "sym_eig: order must be in 1..128 for the one-workgroup solver");
-  DBuf<double> ws(ctx, (size_t)3 * n * n);
+  DBuf<double> ws(ctx, (size_t)3 * n * n + 4 * n + 8);
+  if (n < 3) {
+    launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
+           (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
+           (double*)nullptr, (const int*)nullptr);
+    return;
+  }
+  // tridiagonalisation on one workgroup, then every eigenpair by bisection
+  // and inverse iteration on its own wave (k_eig_ii.hip); spectra with a
+  // cluster wider than the inverse iteration takes fall back to divide and
+  // conquer (the flag at the end of the workspace)
+  double* tri = ws.get() + (size_t)3 * n * n;
+  int* flag = reinterpret_cast<int*>(tri + 4 * n + 4);
+  fill_i32(ctx, flag, 0, 1);
+  launch(ctx, "sym_eig_trd", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
+         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0, tri,
+         (const int*)nullptr);
+  tridiag_eig_ii(ctx, n, tri, ws.get() + (size_t)2 * n * n, w, V, flag);
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
-         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0);
+         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
+         (double*)nullptr, (const int*)flag);
 }
 
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
@@ -692,7 +725,8 @@ void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, co
                "sym_eig_batched: orders must be in 1..128");
   DBuf<double> ws(ctx, (size_t)3 * nmax * nmax * count);
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda, strideA,
-         ns_dev, nmax, nmax, w, strideW, V, strideV, ws.get(), (int64_t)3 * nmax * nmax);
+         ns_dev, nmax, nmax, w, strideW, V, strideV, ws.get(), (int64_t)3 * nmax * nmax,
+         (double*)nullptr, (const int*)nullptr);
 }
 
 }  // namespace fpls

@@ -132,11 +132,17 @@ __global__ void k_counts_checked(const unsigned long long* __restrict__ cnt, int
   }
 }
 
-// Train prefix-LDA models for every k in ks on scores T (n x >= max ks).
-// G (optional): T^T T already computed (ld ldg).
-void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
-                    int32_t C, const std::vector<int>& ks, double ridge, int precision,
-                    DBuf<double>& counts_out, std::vector<LdaSlice>& slices) {
+// Prefix-LDA training (src/fastPLS.cpp:1817-1927) in two halves around the
+// inverse-covariance solves, so that a caller can batch those solves with
+// its own (lda_train_core runs all three steps).
+//
+// lda_prepare: counts, class means and the pooled covariance on scores T
+// (n x >= max ks; G = T^T T optional), the per-k slices with their buffers,
+// and one SpdJob per slice appended to jobs.
+void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
+                 int32_t C, const std::vector<int>& ks, double ridge, int precision,
+                 DBuf<double>& counts_out, std::vector<LdaSlice>& slices, LdaPending& pend,
+                 std::vector<SpdJob>& jobs) {
   const int64_t n = T.rows;
   FPLS_REQUIRE(n > 0 && T.cols > 0, FPLS_ERR_ARG, "LDA requires a non-empty score matrix");
   FPLS_REQUIRE(C >= 2, FPLS_ERR_ARG, "LDA requires at least two classes");
@@ -156,7 +162,8 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
   launch(ctx, "lda_counts", k_counts_checked, dim3((C + 255) / 256), dim3(256), 0,
          reinterpret_cast<const unsigned long long*>(cnt.get()), C, counts_out.get(), empty);
   // class sums of the leading kmax score columns: k x C
-  DBuf<double> sums(ctx, (size_t)kmax * C), means(ctx, (size_t)C * kmax);
+  DBuf<double> sums(ctx, (size_t)kmax * C);
+  pend.means = DBuf<double>(ctx, (size_t)C * kmax);
   if (T.dtype == FPLS_F32)
     gemm_tn_f32(ctx, precision, kmax, C, n, T.f(), T.ld, nullptr, nullptr, 0, y, nullptr,
                 sums.get(), kmax);
@@ -164,9 +171,10 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
     gemm_f64(ctx, true, false, kmax, C, n, 1.0, T.d(), T.ld, nullptr, nullptr, n, nullptr, 0.0,
              sums.get(), kmax, y);
   launch(ctx, "lda_means", k_means_from_sums, dim3(g1(kmax * C)), dim3(256), 0, sums.get(),
-         (int64_t)kmax, (int64_t)C, counts_out.get(), means.get());
+         (int64_t)kmax, (int64_t)C, counts_out.get(), pend.means.get());
   // pooled = (symmatu(T^T T) - sum_c n_c mu_c mu_c^T) / max(1, n - C)
-  DBuf<double> Gl, pooled(ctx, (size_t)kmax * kmax), msc(ctx, (size_t)C * kmax);
+  DBuf<double> Gl, msc(ctx, (size_t)C * kmax);
+  pend.pooled = DBuf<double>(ctx, (size_t)kmax * kmax);
   if (!G) {
     Gl = DBuf<double>(ctx, (size_t)kmax * kmax);
     if (T.dtype == FPLS_F32)
@@ -179,27 +187,42 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
     ldg = kmax;
   }
   launch(ctx, "symmatu", k_symmatu, dim3(g1(kmax * kmax)), dim3(256), 0, G, ldg, (int64_t)kmax,
-         pooled.get());
-  copy_f64(ctx, msc.get(), means.get(), (int64_t)C * kmax);
+         pend.pooled.get());
+  copy_f64(ctx, msc.get(), pend.means.get(), (int64_t)C * kmax);
   launch(ctx, "scale_rows", k_scale_rows_by, dim3(g1(C * kmax)), dim3(256), 0, msc.get(),
          (int64_t)C, (int64_t)C, (int64_t)kmax, counts_out.get());
-  gemm_f64(ctx, true, false, kmax, kmax, C, -1.0, means.get(), C, nullptr, msc.get(), C, nullptr,
-           1.0, pooled.get(), kmax);
+  gemm_f64(ctx, true, false, kmax, kmax, C, -1.0, pend.means.get(), C, nullptr, msc.get(), C,
+           nullptr, 1.0, pend.pooled.get(), kmax);
   const double df = std::max(1.0, (double)n - (double)C);
-  launch(ctx, "scale", k_scale_all, dim3(g1(kmax * kmax)), dim3(256), 0, pooled.get(),
+  launch(ctx, "scale", k_scale_all, dim3(g1(kmax * kmax)), dim3(256), 0, pend.pooled.get(),
          (int64_t)kmax * kmax, 1.0 / df);
-  // per k: ridge, inverse, linear, constants
+  pend.n = n;
+  pend.C = C;
+  pend.kmax = kmax;
+  pend.counts = counts_out.get();
+  // per k: the ridge rule and the inverse (mode 1 of the slice solves)
   slices.clear();
   for (int k : ks) {
     LdaSlice sl;
     sl.k = k;
     sl.means = DBuf<double>(ctx, (size_t)C * k);
-    copy2d_f64(ctx, sl.means.get(), C, means.get(), C, C, k);
+    copy2d_f64(ctx, sl.means.get(), C, pend.means.get(), C, C, k);
     sl.inv_cov = DBuf<double>(ctx, (size_t)k * k);
     sl.ridge_d = DBuf<double>(ctx, 1);
     sl.status_d = DBuf<int32_t>(ctx, 1);
-    spd_solve_slices(ctx, pooled.get(), kmax, k, std::vector<int>{k}, 1, 1, nullptr, ridge,
-                     sl.inv_cov.get(), k, (int64_t)k * k, sl.ridge_d.get(), sl.status_d.get());
+    slices.push_back(std::move(sl));
+  }
+  for (auto& sl : slices)
+    jobs.push_back(SpdJob{pend.pooled.get(), (int64_t)kmax, sl.k, 1, nullptr, ridge,
+                          sl.inv_cov.get(), (int64_t)sl.k, sl.ridge_d.get(), sl.status_d.get()});
+}
+
+// lda_finish: after the solves, the singular-covariance checks, linear terms
+// means inv_cov and the constants -1/2 mu linear + log(max(prior, DBL_MIN)).
+void lda_finish(Ctx* ctx, const LdaPending& pend, std::vector<LdaSlice>& slices) {
+  const int64_t C = pend.C;
+  for (auto& sl : slices) {
+    const int k = sl.k;
     int32_t* sing = ctx->defer_check(1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular", 1);
     copy_i32(ctx, sing, sl.status_d.get(), 1);
     sl.linear = DBuf<double>(ctx, (size_t)C * k);
@@ -207,10 +230,19 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
              nullptr, 0.0, sl.linear.get(), C);
     sl.constants = DBuf<double>(ctx, C);
     launch(ctx, "lda_constants", k_lda_constants, dim3((C + 3) / 4), dim3(256), 0,
-           sl.means.get(), (int64_t)C, sl.linear.get(), (int64_t)C, (int64_t)C, (int64_t)k,
-           counts_out.get(), (double)n, sl.constants.get());
-    slices.push_back(std::move(sl));
+           sl.means.get(), C, sl.linear.get(), C, C, (int64_t)k, pend.counts, (double)pend.n,
+           sl.constants.get());
   }
+}
+
+void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
+                    int32_t C, const std::vector<int>& ks, double ridge, int precision,
+                    DBuf<double>& counts_out, std::vector<LdaSlice>& slices) {
+  LdaPending pend;
+  std::vector<SpdJob> jobs;
+  lda_prepare(ctx, T, G, ldg, y, C, ks, ridge, precision, counts_out, slices, pend, jobs);
+  spd_solve_jobs(ctx, jobs);
+  lda_finish(ctx, pend, slices);
 }
 
 static std::vector<int> unique_sorted(const int32_t* ncomp, int32_t L, int kcap) {

@@ -106,11 +106,32 @@ void opls_apply_dev(Ctx* c, const DMat& X, const double* mX, const double* vX, c
                     const double* Po, int north, double* Xout, int64_t ldo);
 
 // Fits on device-resident inputs (plssvd.hip, simpls.hip).
+// With lda (labels y required), the LDA head is trained on the training
+// scores inside the fit, its inverse covariances solved in the same launch as
+// the latent coefficients (pls(..., classifier = "lda")).
+struct LdaFuse {
+  double ridge = 1e-8;
+  const int32_t* ncomp = nullptr;
+  int32_t n_ncomp = 0;
+};
 void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
-                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o);
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o,
+                    const LdaFuse* lda = nullptr);
 void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
                     const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o);
 
+// LDA training state between lda_prepare and lda_finish (lda.hip).
+struct LdaPending {
+  int64_t n = 0;
+  int C = 0, kmax = 0;
+  DBuf<double> means, pooled;     // C x kmax, kmax x kmax
+  const double* counts = nullptr; // C class counts (the model's)
+};
+void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
+                 int32_t C, const std::vector<int>& ks, double ridge, int precision,
+                 DBuf<double>& counts_out, std::vector<LdaSlice>& slices, LdaPending& pend,
+                 std::vector<SpdJob>& jobs);
+void lda_finish(Ctx* ctx, const LdaPending& pend, std::vector<LdaSlice>& slices);
 // LDA (lda.hip): prefix models for every k in ks on scores T; G = T^T T optional.
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
                     int32_t C, const std::vector<int>& ks, double ridge, int precision,

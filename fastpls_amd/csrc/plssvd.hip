@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <string>
 #include <cmath>
+#include <set>
 #include <vector>
 
 #include "model.h"
@@ -321,7 +322,8 @@ namespace fpls {
 
 // pls_model1 on device-resident inputs (dense Y or labels with m classes).
 void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_t* y, int64_t m,
-                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o) {
+                    const int32_t* ncomp, int32_t n_ncomp, const fpls_opts& o,
+                    const LdaFuse* lda) {
   {
     M->ctx = c;
     M->method = FPLS_METHOD_PLSSVD;
@@ -368,8 +370,26 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
     fill_f64(c, M->W_latent.get(), 0.0, (int64_t)kmax * m * L);
     int32_t* st = c->defer_check(L, FPLS_ERR_SOLVE, "plssvd latent solve failed", 1);
-    spd_solve_slices(c, M->G.get(), kmax, kmax, nc, L, M->label_aware ? 2 : 0, M->s.get(), 0.0,
-                     M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st);
+    const bool fuse = lda != nullptr && y != nullptr && kmax <= kJacobiMax;
+    LdaPending lpend;
+    if (fuse) {
+      // the LDA head on the training scores (reusing G = T^T T); its inverse
+      // covariances join the latent solves in one launch
+      std::set<int> kset;
+      for (int i = 0; i < lda->n_ncomp; ++i)
+        kset.insert(std::min(std::max<int>(lda->ncomp[i], 1), kmax));
+      FPLS_REQUIRE(!kset.empty(), FPLS_ERR_ARG, "LDA requires at least one component count");
+      const std::vector<int> ks(kset.begin(), kset.end());
+      std::vector<SpdJob> jobs;
+      lda_prepare(c, M->T(), M->G.get(), kmax, y, (int32_t)m, ks, lda->ridge, M->precision,
+                  M->counts, M->lda, lpend, jobs);
+      spd_slice_jobs(M->G.get(), kmax, kmax, nc, L, M->label_aware ? 2 : 0, M->s.get(), 0.0,
+                     M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st, jobs);
+      spd_solve_jobs(c, jobs);
+    } else {
+      spd_solve_slices(c, M->G.get(), kmax, kmax, nc, L, M->label_aware ? 2 : 0, M->s.get(), 0.0,
+                       M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st);
+    }
     M->b_stored = !M->label_aware && store_b_policy(o.store_b, p, m, L);
     if (M->b_stored) M->B = DBuf<double>(c, (size_t)p * m * L);
     DBuf<double> Yc;
@@ -402,6 +422,17 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
       }
     }
     M->n_components = kmax;
+    if (fuse) {
+      lda_finish(c, lpend, M->lda);
+    } else if (lda != nullptr) {
+      // orders above the one-workgroup solves: the LDA head on its own
+      std::set<int> kset;
+      for (int i = 0; i < lda->n_ncomp; ++i)
+        kset.insert(std::min(std::max<int>(lda->ncomp[i], 1), kmax));
+      lda_train_core(c, M->T(), M->G.get(), kmax, y, (int32_t)m,
+                     std::vector<int>(kset.begin(), kset.end()), lda->ridge, M->precision,
+                     M->counts, M->lda);
+    }
   }
 }
 
@@ -434,6 +465,34 @@ extern "C" int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_mat
       m = Y.cols;
     }
     plssvd_fit_dev(c, &mdl->m, X, Y, y, m, ncomp, n_ncomp, o);
+  });
+  if (rc == FPLS_OK) *out = mdl.release();
+  return rc;
+}
+
+extern "C" int fpls_plssvd_lda_fit(fpls_ctx* ctx, const fpls_mat* Xm, const fpls_ivec* yv,
+                                   int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                                   const fpls_opts* opts, double lda_ridge, fpls_model** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<fpls_model> mdl(new fpls_model());
+  int rc = fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    FPLS_REQUIRE(out != nullptr, FPLS_ERR_ARG, "out is NULL");
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hx;
+    DBuf<int32_t> hl;
+    DMat X = stage(c, Xm, hx, "plssvd_lda: X");
+    FPLS_REQUIRE(yv && yv->ptr, FPLS_ERR_ARG, "plssvd_lda: class labels are required");
+    FPLS_REQUIRE(n_classes >= 2, FPLS_ERR_ARG, "LDA requires at least two classes");
+    const int32_t* y = stage_labels(c, yv, X.rows, hl, "plssvd_lda: y");
+    LdaFuse lf;
+    lf.ridge = lda_ridge;
+    lf.ncomp = ncomp;
+    lf.n_ncomp = n_ncomp;
+    DMat Y;
+    plssvd_fit_dev(c, &mdl->m, X, Y, y, n_classes, ncomp, n_ncomp, o, &lf);
   });
   if (rc == FPLS_OK) *out = mdl.release();
   return rc;

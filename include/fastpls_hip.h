@@ -21,6 +21,7 @@
  *   fpls_available            has_cuda()                 src/fastPLS.cpp:1607-1609
  *   fpls_plssvd_fit           pls_model1                 src/fastPLS.cpp:4780-4986
  *                             .plssvd_label_aware_stream_model (labels) R/main.R:513-655
+ *   fpls_plssvd_lda_fit       pls_model1 + .attach_lda_classifier R/main.R:1216-1251
  *   fpls_simpls_fit           pls_model2_fast            src/fastPLS.cpp:2454-2778
  *   fpls_model_read           the Rcpp::List fields returned by the fits
  *   fpls_model_import         those fields back into a device model (predict()
@@ -154,6 +155,17 @@ int fpls_kernel_time(fpls_ctx* ctx, const char* kernel, int64_t* launches, doubl
 int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
                     int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
                     const fpls_opts* opts, fpls_model** out);
+
+/* PLSSVD with the LDA head: pls(method = "plssvd", classifier = "lda")'s fit
+   (pls_model1 src/fastPLS.cpp:4780-4986 on the one-hot of y, then
+   .attach_lda_classifier on the training scores R/main.R:1216-1251 with
+   lda_train_prefix_cpp src/fastPLS.cpp:1817-1927) in one call: the same
+   model as fpls_plssvd_fit followed by fpls_lda_train_prefix(model, NULL, y,
+   n_classes, ncomp, n_ncomp, lda_ridge), with the latent solves and the LDA
+   inverse covariances run in one launch. */
+int fpls_plssvd_lda_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y, int32_t n_classes,
+                        const int32_t* ncomp, int32_t n_ncomp, const fpls_opts* opts,
+                        double lda_ridge, fpls_model** out);
 
 /* SIMPLS ("fast" incremental core).  Dense Y or labels (one-hot response). */
 int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,

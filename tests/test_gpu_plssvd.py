@@ -181,3 +181,43 @@ def test_class_sums_exact_counts(ctx):
     np.testing.assert_allclose(r["col_sums"], X.astype(np.float64).sum(0), rtol=1e-9, atol=1e-6)
     with pytest.raises(F.FplsError, match="LABEL_RANGE"):
         F.class_sums(X, np.where(y == 3, C + 1, y).astype(np.int32), C)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_fused_plssvd_lda_equals_separate_calls(ctx, precision):
+    """fpls_plssvd_lda_fit is fpls_plssvd_fit + fpls_lda_train_prefix with the
+    solves batched: every model and LDA field equal to the last bit."""
+    import ctypes
+    import fastpls_amd as F
+    from fastpls_amd import _lib, synth
+    Xtr, ytr, Xte, _ = synth.classes_on_sphere(n_classes=9, per_class_train=120,
+                                               per_class_test=30, p=64, seed=12)
+    keep = []
+    Xm = _lib.as_mat(Xtr, keep)
+    yv = _lib.as_ivec(ytr, keep)
+    nc, ncp = _lib.int_array([3, 8])
+    opts = _lib.default_opts(precision=_lib.PREC_BF16X3 if precision == "bf16x3" else 0)
+    h1, h2 = ctypes.c_void_p(), ctypes.c_void_p()
+    ctx.check(ctx.lib.fpls_plssvd_lda_fit(ctx.h, ctypes.byref(Xm), ctypes.byref(yv), 9, ncp, 2,
+                                          ctypes.byref(opts), 1e-8, ctypes.byref(h1)))
+    ctx.check(ctx.lib.fpls_plssvd_fit(ctx.h, ctypes.byref(Xm), None, ctypes.byref(yv), 9, ncp, 2,
+                                      ctypes.byref(opts), ctypes.byref(h2)))
+    ctx.check(ctx.lib.fpls_lda_train_prefix(ctx.h, h2, None, ctypes.byref(yv), 9, ncp, 2, 1e-8))
+    m1, m2 = F.FastPLS(ctx, h1), F.FastPLS(ctx, h2)
+    p, k = 64, 8
+    for name, shape in (("R", (p, k)), ("C_latent", (k, k, 2)), ("W_latent", (k, 9, 2)),
+                        ("B", (p, 9, 2)), ("mX", (p,)), ("mY", (9,))):
+        np.testing.assert_array_equal(m1.read(name, shape), m2.read(name, shape), err_msg=name)
+    for s, kk in enumerate([3, 8]):
+        for field, shape in (("means", (9, kk)), ("inv_cov", (kk, kk)), ("linear", (9, kk)),
+                             ("constants", (9,)), ("priors", (9,)), ("ridge", (1,))):
+            np.testing.assert_array_equal(m1.lda_read(s, field, shape),
+                                          m2.lda_read(s, field, shape), err_msg=field)
+    Xt = _lib.as_mat(Xte, keep)
+    l1 = np.empty((Xte.shape[0], 2), dtype=np.int32, order="F")
+    l2 = np.empty_like(l1)
+    ctx.check(ctx.lib.fpls_lda_predict(ctx.h, h1, ctypes.byref(Xt), l1.ctypes.data, None))
+    ctx.check(ctx.lib.fpls_lda_predict(ctx.h, h2, ctypes.byref(Xt), l2.ctypes.data, None))
+    np.testing.assert_array_equal(l1, l2)
+    m1.free()
+    m2.free()
