@@ -94,6 +94,9 @@ SIGNATURES = {
                                        ctypes.c_int32, _P(Opts), _P(_vp)]),
     "fpls_plssvd_lda_fit": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                            ctypes.c_int32, _P(Opts), ctypes.c_double, _P(_vp)]),
+    "fpls_plssvd_lda_fit_predict": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp,
+                                                   ctypes.c_int32, _P(Opts), ctypes.c_double,
+                                                   _P(Mat), _vp, _P(_vp)]),
     "fpls_model_info_get": (ctypes.c_int, [_vp, _P(ModelInfo)]),
     "fpls_model_import": (ctypes.c_int, [_vp, _P(ModelFields), _P(_vp)]),
     "fpls_model_read": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int64]),

@@ -200,7 +200,17 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     if method not in ("plssvd", "simpls"):
         raise ValueError("method must be 'plssvd' or 'simpls'")
     fused_lda = method == "plssvd" and classification and classifier == "lda"
-    if fused_lda:
+    # with test rows and top-1 labels, the prediction joins the same call (one
+    # host round trip at its end)
+    fused_pred = None
+    if fused_lda and Xtest is not None and resolve_top_k(top) == 1:
+        Xt = as_mat(Xtest, keep)
+        fused_pred = np.empty((Xt.rows, nc_arr.size), dtype=np.int32, order="F")
+        ctx.check(ctx.lib.fpls_plssvd_lda_fit_predict(
+            ctx.h, ctypes.byref(X), ctypes.byref(yv), C, nc_ptr, nc_arr.size,
+            ctypes.byref(opts), float(lda_ridge), ctypes.byref(Xt), fused_pred.ctypes.data,
+            ctypes.byref(h)))
+    elif fused_lda:
         # the fit and the LDA head in one call (one launch for their solves)
         ctx.check(ctx.lib.fpls_plssvd_lda_fit(ctx.h, ctypes.byref(X), ctypes.byref(yv), C,
                                               nc_ptr, nc_arr.size, ctypes.byref(opts),
@@ -228,7 +238,16 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
                      candidate_alpha, candidate_top_m)
     else:
         model["classification_rule"] = "argmax"
-    if Xtest is not None:
+    if fused_pred is not None:
+        res = {"Ypred_index": fused_pred,
+               "Ypred": np.asarray(lev, dtype=object)[fused_pred - 1]}
+        if Ytest is not None:
+            L = fused_pred.shape[1]
+            res["Q2Y"] = _q2_classes(Ytest, lev, np.stack(
+                [(fused_pred[:, a][:, None] == np.arange(1, C + 1)[None, :]).astype(np.float64)
+                 for a in range(L)], axis=2))
+        model.update(res)
+    elif Xtest is not None:
         model.update(predict(model, Xtest, Ytest, top=top))
     return model
 

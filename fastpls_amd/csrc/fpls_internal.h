@@ -13,6 +13,7 @@
 #include <stdexcept>
 #include <string>
 #include <utility>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/fastpls_hip.h"
@@ -42,6 +43,9 @@ struct Error : public std::runtime_error {
 // ---------------------------------------------------------------- pool
 // HBM pool: blocks are cached by size so that repeated fits allocate nothing
 // after the first call.
+// Caching HBM pool: blocks are kept after their first hipMalloc and handed
+// out again (smallest free block that fits without wasting more than half of
+// it); lookups are O(log blocks).
 struct Pool {
   struct Block {
     void* ptr;
@@ -49,6 +53,8 @@ struct Pool {
     bool used;
   };
   std::vector<Block> blocks;
+  std::unordered_map<void*, size_t> index;  // ptr -> blocks[i]
+  std::multimap<size_t, size_t> free_by_size;
   void* get(size_t bytes);
   void put(void* p);
   void release_all();
@@ -101,6 +107,10 @@ struct Ctx {
     std::string msg;
   };
   int32_t* flags_dev = nullptr;
+  // Ticket counter of the last-block reductions (zero between launches: the
+  // last block of each launch resets it).
+  unsigned* ticket_dev = nullptr;
+  unsigned* ticket();
   int32_t* flags_host = nullptr;
   int nflags = 0;
   std::vector<Check> checks;
@@ -109,10 +119,23 @@ struct Ctx {
   std::shared_ptr<void> cv_plan;
   int32_t* defer_check(int count, int code, const std::string& msg, int fail_value = 0);
   void flush_checks();
-  void drop_checks() {
-    checks.clear();
-    nflags = 0;
-  }
+  // Device -> host copies completed by the next flush (the entry point's one
+  // synchronisation): staged in pinned memory, copied to dst after the sync.
+  struct HostCopy {
+    void* dst;
+    size_t off, bytes;
+  };
+  unsigned char* stage_host = nullptr;
+  size_t stage_cap = 0, stage_used = 0;
+  std::vector<HostCopy> host_copies;
+  void to_host_later(void* dst, const void* src, size_t bytes);
+  void complete_host_copies(bool synced);
+  // Persistent zero buffer of integer class counts (zero between launches:
+  // the kernel that uses it drains it).
+  unsigned long long* zcounts_dev = nullptr;
+  static constexpr int kZeroCounts = 16384;
+  unsigned long long* zero_counts();
+  void drop_checks();
   // Fixed-arena mode of the allocation-free entry points: while arena_base is
   // set, alloc() carves 256-byte aligned blocks from the device range
   // [arena_base, arena_base + arena_cap) as a stack instead of the pool (a
@@ -234,6 +257,19 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const double* offset, float* Cf, double* Cd, int64_t ldc,
                  bool c_rowmajor = false);
+// The same with its operands prepared once by the caller: B zero-padded to
+// nn_pad_k(K) x nn_pad_n(N) (ld nn_pad_k(K), from nn_pad_operand) and, with
+// shiftA, its FP32 copy shift32 (nn_shift_operand).  A model keeps these for
+// the projections it repeats.
+void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K,
+                        const float* A, int64_t lda, const double* shiftA,
+                        const float* shift32, const float* B, const double* offset, float* Cf,
+                        double* Cd, int64_t ldc, bool c_rowmajor = false);
+int64_t nn_pad_k(int64_t K);
+int64_t nn_pad_n(int64_t N);
+DBuf<float> nn_pad_operand(Ctx* ctx, const double* B, int64_t ldb, bool trans, int64_t K,
+                           int64_t N);
+DBuf<float> nn_shift_operand(Ctx* ctx, const double* shift, int64_t K);
 // C (M x N col-major FP32, ld M) = A B^T, A: M x K and B: N x K both row-major.
 void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const float* B, int64_t ldb, float* C);
@@ -273,9 +309,20 @@ void gemv_t(Ctx* ctx, const DMat& A, const double* shift, const double* x, doubl
 void col_sums(Ctx* ctx, const DMat& X, double* sums);
 // Column sums of squared deviations from `center`.
 void col_ssd(Ctx* ctx, const DMat& X, const double* center, double* ssd);
-// Class counts from 1-based labels; *bad counts labels outside 1..C.
+// One pass of either (center null: sums; else squared deviations) finished in
+// the same kernel: out = sum / div, then post 1: sqrt, post 2: sqrt with zero
+// or non-finite mapped to 1; ones (optional) is set to 1 in every column.
+void col_stat(Ctx* ctx, const DMat& X, const double* center, double* out, double div, int post,
+              double* ones = nullptr);
+// Class counts from 1-based labels; *bad counts labels outside 1..C (cleared
+// first unless bad_zeroed: the slots of deferred checks are zero already).
 void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* counts,
-                  int32_t* bad);
+                  int32_t* bad, bool bad_zeroed = false);
+// FP64 class proportions prop[c] = count_c / n and counts (optional) in one
+// launch; *bad (zero on entry) counts labels outside 1..C.  Returns false
+// (nothing launched) when the per-block partials would be too large.
+bool label_counts_prop(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, double* prop,
+                       double* counts, int32_t* bad);
 
 // ---------------------------------------------------------------- kernels (k_dense.hip)
 // Symmetric eigen-decomposition (one workgroup, cyclic parallel Jacobi, FP64),

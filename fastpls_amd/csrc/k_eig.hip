@@ -73,6 +73,7 @@ __device__ __forceinline__ int bound(const double* x, int lo, int hi, double v, 
 // tri_out (when set): stop after stage 1 and write d (n), e (n), tau (n) and
 // the scale of the problem at tri_out; the reflectors stay in the workspace
 // (Vr, ld n).  run_flag (when set): return at once unless *run_flag != 0.
+// zero_flag (when set) is cleared (the fallback flag the later kernels set).
 __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ Ain0, int64_t lda,
                                                     int64_t strideA, const int* __restrict__ ns,
                                                     int n0, int nmax, double* __restrict__ w0,
@@ -80,8 +81,10 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
                                                     int64_t strideV, double* __restrict__ ws,
                                                     int64_t strideWs,
                                                     double* __restrict__ tri_out,
-                                                    const int* __restrict__ run_flag) {
+                                                    const int* __restrict__ run_flag,
+                                                    int* __restrict__ zero_flag) {
   if (run_flag && *run_flag == 0) return;
+  if (zero_flag && threadIdx.x == 0 && blockIdx.x == 0) *zero_flag = 0;
   const int64_t bi = blockIdx.x;
   const int n = ns ? ns[bi] : n0;
   const double* __restrict__ Ain = Ain0 + bi * strideA;
@@ -699,7 +702,7 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
   if (n < 3) {
     launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
            (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
-           (double*)nullptr, (const int*)nullptr);
+           (double*)nullptr, (const int*)nullptr, (int*)nullptr);
     return;
   }
   // tridiagonalisation on one workgroup, then every eigenpair by bisection
@@ -708,14 +711,13 @@ void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V
   // conquer (the flag at the end of the workspace)
   double* tri = ws.get() + (size_t)3 * n * n;
   int* flag = reinterpret_cast<int*>(tri + 4 * n + 4);
-  fill_i32(ctx, flag, 0, 1);
   launch(ctx, "sym_eig_trd", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
          (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0, tri,
-         (const int*)nullptr);
+         (const int*)nullptr, flag);
   tridiag_eig_ii(ctx, n, tri, ws.get() + (size_t)2 * n * n, w, V, flag);
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
          (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
-         (double*)nullptr, (const int*)flag);
+         (double*)nullptr, (const int*)flag, (int*)nullptr);
 }
 
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
@@ -726,7 +728,7 @@ void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, co
   DBuf<double> ws(ctx, (size_t)3 * nmax * nmax * count);
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda, strideA,
          ns_dev, nmax, nmax, w, strideW, V, strideV, ws.get(), (int64_t)3 * nmax * nmax,
-         (double*)nullptr, (const int*)nullptr);
+         (double*)nullptr, (const int*)nullptr, (int*)nullptr);
 }
 
 }  // namespace fpls

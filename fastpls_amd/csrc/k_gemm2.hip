@@ -668,20 +668,67 @@ static void launch_proj(Ctx* ctx, const char* name, int N, dim3 grid, Args... ar
     launch(ctx, name, k_proj<BF, CEN, 8>, grid, dim3(P_NT), 0, args...);
 }
 
+// B (K x N, FP64, ld ldb; or its transpose, B(k, n) = src[n + k * ldb]) as the
+// zero-padded FP32 operand of gemm_nn_f32_padded, in one pass.
+__global__ void k_g2_pad_f64(const double* __restrict__ B, int64_t ldb, int trans, int64_t K,
+                             int64_t N, float* __restrict__ out, int64_t Kp, int64_t Np) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < Kp * Np;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k = t % Kp, n = t / Kp;
+    out[t] = (k < K && n < N) ? (float)(trans ? B[n + k * ldb] : B[k + n * ldb]) : 0.f;
+  }
+}
+
+__global__ void k_g2_shift_f32(const double* __restrict__ s, int64_t K, float* __restrict__ out) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < K;
+       t += (int64_t)gridDim.x * blockDim.x)
+    out[t] = (float)s[t];
+}
+
+int64_t nn_pad_k(int64_t K) { return (K + G2_BK - 1) / G2_BK * G2_BK; }
+int64_t nn_pad_n(int64_t N) { return (N + G2_BN - 1) / G2_BN * G2_BN; }
+
+DBuf<float> nn_pad_operand(Ctx* ctx, const double* B, int64_t ldb, bool trans, int64_t K,
+                           int64_t N) {
+  const int64_t Kp = nn_pad_k(K), Np = nn_pad_n(N);
+  DBuf<float> out(ctx, (size_t)Kp * Np);
+  launch(ctx, "pad_b", k_g2_pad_f64, dim3(g1(Kp * Np)), dim3(256), 0, B, ldb, trans ? 1 : 0, K,
+         N, out.get(), Kp, Np);
+  return out;
+}
+
+DBuf<float> nn_shift_operand(Ctx* ctx, const double* shift, int64_t K) {
+  DBuf<float> out(ctx, (size_t)K);
+  launch(ctx, "shift_f32", k_g2_shift_f32, dim3(g1(K)), dim3(256), 0, shift, K, out.get());
+  return out;
+}
+
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
   if (M <= 0 || N <= 0) return;
-  FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
   FPLS_REQUIRE(K >= 1, FPLS_ERR_ARG, "gemm_nn_f32: inner dimension must be positive");
-  dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN));
-  const int crow = c_rowmajor ? 1 : 0;
   // zero-padded copy of the (small) B operand: Kp x Np, ld Kp
-  const int64_t Kp = (K + G2_BK - 1) / G2_BK * G2_BK, Np = (N + G2_BN - 1) / G2_BN * G2_BN;
+  const int64_t Kp = nn_pad_k(K), Np = nn_pad_n(N);
   DBuf<float> Bp(ctx, (size_t)Kp * Np);
   launch(ctx, "pad_b", k_g2_pad, dim3(g1(Kp * Np)), dim3(256), 0, B, ldb, K, N, Bp.get(), Kp, Np);
-  B = Bp.get();
-  ldb = Kp;
+  DBuf<float> shf;
+  if (shiftA) shf = nn_shift_operand(ctx, shiftA, K);
+  gemm_nn_f32_padded(ctx, precision, M, N, K, A, lda, shiftA, shf.get(), Bp.get(), offset, Cf,
+                     Cd, ldc, c_rowmajor);
+}
+
+void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K,
+                        const float* A, int64_t lda, const double* shiftA,
+                        const float* shift32, const float* B, const double* offset, float* Cf,
+                        double* Cd, int64_t ldc, bool c_rowmajor) {
+  if (M <= 0 || N <= 0) return;
+  FPLS_REQUIRE((Cf != nullptr) != (Cd != nullptr), FPLS_ERR_ARG, "gemm_nn_f32: one output");
+  FPLS_REQUIRE(K >= 1, FPLS_ERR_ARG, "gemm_nn_f32: inner dimension must be positive");
+  FPLS_REQUIRE(!shiftA || shift32, FPLS_ERR_ARG, "gemm_nn_f32: the FP32 shift is missing");
+  dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN));
+  const int crow = c_rowmajor ? 1 : 0;
+  const int64_t ldb = nn_pad_k(K);
   // centred contractions (projections of X) are timed as "project_*"
   const bool bf = precision == FPLS_PREC_BF16X3;
   if (N <= 128) {
@@ -695,13 +742,9 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       int64_t kchunk = (K + splits - 1) / splits;
       kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
       splits = (K + kchunk - 1) / kchunk;
-      DBuf<float> part(ctx, (size_t)splits * M * N), shf;
-      if (shiftA) {
-        shf = DBuf<float>(ctx, (size_t)K);
-        f64_to_f32(ctx, shf.get(), shiftA, K);
-      }
+      DBuf<float> part(ctx, (size_t)splits * M * N);
       dim3 sgrid((unsigned)tiles, (unsigned)splits);
-      const float* sh32 = shf.get();
+      const float* sh32 = shift32;
       if (shiftA) {
         if (bf)
           launch_proj<true, true>(ctx, "project_bf16x3", n32, sgrid, M, n32, K, A, lda, sh32, B,
@@ -727,14 +770,12 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
     }
     dim3 pgrid((unsigned)tiles);
     if (shiftA) {
-      DBuf<float> shf(ctx, (size_t)K);
-      f64_to_f32(ctx, shf.get(), shiftA, K);
       if (bf)
-        launch_proj<true, true>(ctx, "project_bf16x3", n32, pgrid, M, n32, K, A, lda,
-               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
+        launch_proj<true, true>(ctx, "project_bf16x3", n32, pgrid, M, n32, K, A, lda, shift32, B,
+                                ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
       else
-        launch_proj<false, true>(ctx, "project_f32", n32, pgrid, M, n32, K, A, lda,
-               (const float*)shf.get(), B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
+        launch_proj<false, true>(ctx, "project_f32", n32, pgrid, M, n32, K, A, lda, shift32, B,
+                                 ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
     } else {
       if (bf)
         launch_proj<true, false>(ctx, "gemm_nn_bf16x3", n32, pgrid, M, n32, K, A,

@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <functional>
+#include <memory>
 #include <set>
 #include <vector>
 
@@ -24,56 +25,6 @@ void fit_scores_and_gram(Ctx* ctx, Model* M, const DMat& X);
 
 static unsigned g1(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
-}
-
-// means (C x k) = sums^T / counts, sums: k x C (ld k)
-__global__ void k_means_from_sums(const double* sums, int64_t k, int64_t C, const double* counts,
-                                  double* means) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < k * C;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = t % C, j = t / C;
-    means[c + j * C] = sums[j + c * k] / counts[c];
-  }
-}
-
-// pooled (k x k) = symmatu(G) (upper triangle mirrored), ld k
-__global__ void k_symmatu(const double* G, int64_t ldg, int64_t k, double* P) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < k * k;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t % k, j = t / k;
-    P[i + j * k] = (i <= j) ? G[i + j * ldg] : G[j + i * ldg];
-  }
-}
-
-__global__ void k_scale_rows_by(double* X, int64_t ld, int64_t rows, int64_t cols, const double* f) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < rows * cols;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = t % rows, j = t / rows;
-    X[i + j * ld] *= f[i];
-  }
-}
-
-__global__ void k_scale_all(double* X, int64_t n, double f) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
-       t += (int64_t)gridDim.x * blockDim.x)
-    X[t] *= f;
-}
-
-// constants[c] = -1/2 means_c . linear_c + log(max(count_c / n, DBL_MIN))
-// one wave per class: q = sum_j means[c, j] linear[c, j] (fixed-order wave sum)
-__global__ void k_lda_constants(const double* means, int64_t ldm, const double* linear,
-                                int64_t ldl, int64_t C, int64_t k, const double* counts, double n,
-                                double* constants) {
-  const int64_t c = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (c >= C) return;
-  double q = 0.0;
-  for (int64_t j = lane; j < k; j += 64) q = fma(means[c + j * ldm], linear[c + j * ldl], q);
-  q = wave_sum_d(q);
-  if (lane == 0) {
-    const double prior = fmax(counts[c] / n, DBL_MIN);
-    constants[c] = -0.5 * q + log(prior);
-  }
 }
 
 // labels[i] = 1 + argmax_c (scores[i,c] + cst[c]); first maximum wins.
@@ -132,6 +83,134 @@ __global__ void k_counts_checked(const unsigned long long* __restrict__ cnt, int
   }
 }
 
+// One pass over the class sums (k x C, ld kmax) for the training state:
+//   means (C x kmax, ld C)  mu[c, a] = sums[a, c] / n_c, and the leading k_s
+//                           columns again into each slice's means,
+//   pooled (kmax x kmax)    (symmatu(G) - sum_c mu[c, a] (mu[c, b] n_c)) / df,
+//   empty[0]                the number of empty classes (when set).
+// 16 x 16 tiles of pooled, the class terms staged through LDS 64 at a time.
+struct LdaMeanCopies {
+  double* p[kSpdJobMax];
+  int k[kSpdJobMax];
+  int count;
+};
+constexpr int LP_CH = 64;
+__global__ __launch_bounds__(256) void k_lda_pooled(const double* __restrict__ sums,
+                                                    int64_t kmax, int64_t C,
+                                                    const double* __restrict__ counts,
+                                                    const double* __restrict__ G, int64_t ldg,
+                                                    double inv_df, double* __restrict__ pooled,
+                                                    double* __restrict__ means,
+                                                    LdaMeanCopies copies,
+                                                    int32_t* __restrict__ empty) {
+  __shared__ double mua[16][LP_CH + 1], msb[16][LP_CH + 1];
+  const int t = threadIdx.x, tx = t & 15, ty = t >> 4;
+  const int64_t a0 = 16 * (int64_t)blockIdx.x, b0 = 16 * (int64_t)blockIdx.y;
+  double acc = 0.0;
+  for (int64_t c0 = 0; c0 < C; c0 += LP_CH) {
+    __syncthreads();
+    for (int idx = t; idx < 16 * LP_CH; idx += 256) {
+      const int i = idx & 15, cc = idx >> 4;
+      const int64_t c = c0 + cc;
+      double va = 0.0, vb = 0.0;
+      if (c < C) {
+        const double nc = counts[c];
+        if (a0 + i < kmax) va = sums[(a0 + i) + c * kmax] / nc;
+        if (b0 + i < kmax) vb = (sums[(b0 + i) + c * kmax] / nc) * nc;
+      }
+      mua[i][cc] = va;
+      msb[i][cc] = vb;
+    }
+    __syncthreads();
+    const int cn = C - c0 < LP_CH ? (int)(C - c0) : LP_CH;
+    // zero-padded chunk: four independent accumulators over all LP_CH terms
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int cc = 0; cc < LP_CH; cc += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) q[u] = fma(mua[tx][cc + u], msb[ty][cc + u], q[u]);
+    }
+    (void)cn;
+    acc += (q[0] + q[1]) + (q[2] + q[3]);
+  }
+  const int64_t a = a0 + tx, b = b0 + ty;
+  if (a < kmax && b < kmax) {
+    const double g = (a <= b) ? G[a + b * ldg] : G[b + a * ldg];
+    pooled[a + b * kmax] = (g - acc) * inv_df;
+  }
+  if (blockIdx.y == 0) {
+    for (int64_t idx = t; idx < 16 * C; idx += 256) {
+      const int64_t i = idx & 15, c = idx >> 4, aa = a0 + i;
+      if (aa >= kmax) continue;
+      const double mu = sums[aa + c * kmax] / counts[c];
+      means[c + aa * C] = mu;
+      for (int s = 0; s < copies.count; ++s)
+        if (aa < copies.k[s]) copies.p[s][c + aa * C] = mu;
+    }
+  }
+  if (empty && blockIdx.x == 0 && blockIdx.y == 0)
+    for (int64_t c = t; c < C; c += 256)
+      if (counts[c] == 0.0) atomicAdd(empty, 1);
+}
+
+// Per slice and class c: linear[c, :] = means[c, :] inv_cov (k x k, staged
+// in LDS), constants[c] = -1/2 means_c . linear_c + log(max(n_c / n, DBL_MIN))
+// (the dot product in the lane order of one wave, fixed).  Block: one class,
+// 256 threads; thread pair (t, t + 128) splits the k-long product of output
+// column t into two interleaved halves with four accumulators each.
+struct LdaLinJob {
+  const double* means;  // C x k
+  const double* inv;    // k x k
+  double* linear;       // C x k
+  double* constants;    // C
+  int k;
+};
+struct LdaLinJobs {
+  LdaLinJob j[kSpdJobMax];
+  int count;
+};
+__global__ __launch_bounds__(256) void k_lda_linear(LdaLinJobs jobs, int64_t C,
+                                                    const double* __restrict__ counts, double n) {
+  __shared__ double inv[kJacobiMax * (kJacobiMax + 1)];
+  __shared__ double mu[kJacobiMax], half[kJacobiMax], lin[kJacobiMax];
+  const LdaLinJob J = jobs.j[blockIdx.y];
+  const int k = J.k, t = threadIdx.x, lane = t & 63, col = t & 127, hb = t >> 7;
+  const int64_t c = blockIdx.x;
+  const int ld = k + 1;  // odd: column reads of lanes spread over the banks
+  for (int idx = t; idx < k * k; idx += 256) inv[(idx % k) + (idx / k) * ld] = J.inv[idx];
+  if (t < k) mu[t] = J.means[c + t * C];
+  __syncthreads();
+  double s = 0.0;
+  if (col < k) {
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* ic = inv + col * ld;
+    int b = hb;
+    for (; b + 6 < k; b += 8) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = fma(mu[b + 2 * u], ic[b + 2 * u], a[u]);
+    }
+    for (; b < k; b += 2) a[0] = fma(mu[b], ic[b], a[0]);
+    s = (a[0] + a[1]) + (a[2] + a[3]);
+  }
+  if (hb == 1 && col < k) half[col] = s;
+  __syncthreads();
+  if (hb == 0 && col < k) {
+    const double v = s + half[col];
+    J.linear[c + col * C] = v;
+    lin[col] = v;
+  }
+  __syncthreads();
+  if (t < 64) {
+    double qv = 0.0;
+    for (int j = lane; j < k; j += 64) qv = fma(mu[j], lin[j], qv);
+    qv = wave_sum_d(qv);
+    if (lane == 0) {
+      const double prior = fmax(counts[c] / n, DBL_MIN);
+      J.constants[c] = -0.5 * qv + log(prior);
+    }
+  }
+}
+
 // Prefix-LDA training (src/fastPLS.cpp:1817-1927) in two halves around the
 // inverse-covariance solves, so that a caller can batch those solves with
 // its own (lda_train_core runs all three steps).
@@ -142,7 +221,7 @@ __global__ void k_counts_checked(const unsigned long long* __restrict__ cnt, int
 void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
                  int32_t C, const std::vector<int>& ks, double ridge, int precision,
                  DBuf<double>& counts_out, std::vector<LdaSlice>& slices, LdaPending& pend,
-                 std::vector<SpdJob>& jobs) {
+                 std::vector<SpdJob>& jobs, bool counts_ready) {
   const int64_t n = T.rows;
   FPLS_REQUIRE(n > 0 && T.cols > 0, FPLS_ERR_ARG, "LDA requires a non-empty score matrix");
   FPLS_REQUIRE(C >= 2, FPLS_ERR_ARG, "LDA requires at least two classes");
@@ -150,17 +229,22 @@ void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const in
   FPLS_REQUIRE(kmax >= 1 && kmax <= T.cols, FPLS_ERR_ARG,
                "LDA component counts must be in 1..ncol(Ttrain)");
   FPLS_REQUIRE(kmax <= kJacobiMax, FPLS_ERR_ARG, "LDA on the device supports up to 128 components");
+  FPLS_REQUIRE((int)ks.size() <= kSpdJobMax, FPLS_ERR_ARG,
+               "LDA supports up to 16 component counts per fit");
   // counts + label validation (src/fastPLS.cpp:1853-1867): codes outside 1..C
   // and empty classes are counted on the device and checked when the entry
   // point returns (kernels below count such rows in no class)
-  DBuf<int64_t> cnt(ctx, C);
-  int32_t* bad =
-      ctx->defer_check(1, FPLS_ERR_LABEL_RANGE, "LDA labels must be encoded as 1..n_classes");
-  label_counts(ctx, y, n, C, cnt.get(), bad);
-  counts_out = DBuf<double>(ctx, C);
   int32_t* empty = ctx->defer_check(1, FPLS_ERR_EMPTY_CLASS, "LDA received an empty class");
-  launch(ctx, "lda_counts", k_counts_checked, dim3((C + 255) / 256), dim3(256), 0,
-         reinterpret_cast<const unsigned long long*>(cnt.get()), C, counts_out.get(), empty);
+  if (!counts_ready) {
+    DBuf<int64_t> cnt(ctx, C);
+    int32_t* bad =
+        ctx->defer_check(1, FPLS_ERR_LABEL_RANGE, "LDA labels must be encoded as 1..n_classes");
+    label_counts(ctx, y, n, C, cnt.get(), bad, true);
+    counts_out = DBuf<double>(ctx, C);
+    launch(ctx, "lda_counts", k_counts_checked, dim3((C + 255) / 256), dim3(256), 0,
+           reinterpret_cast<const unsigned long long*>(cnt.get()), C, counts_out.get(), empty);
+    empty = nullptr;  // counted already
+  }
   // class sums of the leading kmax score columns: k x C
   DBuf<double> sums(ctx, (size_t)kmax * C);
   pend.means = DBuf<double>(ctx, (size_t)C * kmax);
@@ -170,10 +254,7 @@ void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const in
   else
     gemm_f64(ctx, true, false, kmax, C, n, 1.0, T.d(), T.ld, nullptr, nullptr, n, nullptr, 0.0,
              sums.get(), kmax, y);
-  launch(ctx, "lda_means", k_means_from_sums, dim3(g1(kmax * C)), dim3(256), 0, sums.get(),
-         (int64_t)kmax, (int64_t)C, counts_out.get(), pend.means.get());
-  // pooled = (symmatu(T^T T) - sum_c n_c mu_c mu_c^T) / max(1, n - C)
-  DBuf<double> Gl, msc(ctx, (size_t)C * kmax);
+  DBuf<double> Gl;
   pend.pooled = DBuf<double>(ctx, (size_t)kmax * kmax);
   if (!G) {
     Gl = DBuf<double>(ctx, (size_t)kmax * kmax);
@@ -186,53 +267,59 @@ void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const in
     G = Gl.get();
     ldg = kmax;
   }
-  launch(ctx, "symmatu", k_symmatu, dim3(g1(kmax * kmax)), dim3(256), 0, G, ldg, (int64_t)kmax,
-         pend.pooled.get());
-  copy_f64(ctx, msc.get(), pend.means.get(), (int64_t)C * kmax);
-  launch(ctx, "scale_rows", k_scale_rows_by, dim3(g1(C * kmax)), dim3(256), 0, msc.get(),
-         (int64_t)C, (int64_t)C, (int64_t)kmax, counts_out.get());
-  gemm_f64(ctx, true, false, kmax, kmax, C, -1.0, pend.means.get(), C, nullptr, msc.get(), C,
-           nullptr, 1.0, pend.pooled.get(), kmax);
-  const double df = std::max(1.0, (double)n - (double)C);
-  launch(ctx, "scale", k_scale_all, dim3(g1(kmax * kmax)), dim3(256), 0, pend.pooled.get(),
-         (int64_t)kmax * kmax, 1.0 / df);
-  pend.n = n;
-  pend.C = C;
-  pend.kmax = kmax;
-  pend.counts = counts_out.get();
-  // per k: the ridge rule and the inverse (mode 1 of the slice solves)
+  // per k: the means prefix and the buffers of the ridge rule and the inverse
   slices.clear();
+  LdaMeanCopies copies{};
   for (int k : ks) {
     LdaSlice sl;
     sl.k = k;
     sl.means = DBuf<double>(ctx, (size_t)C * k);
-    copy2d_f64(ctx, sl.means.get(), C, pend.means.get(), C, C, k);
     sl.inv_cov = DBuf<double>(ctx, (size_t)k * k);
     sl.ridge_d = DBuf<double>(ctx, 1);
-    sl.status_d = DBuf<int32_t>(ctx, 1);
+    copies.p[copies.count] = sl.means.get();
+    copies.k[copies.count] = k;
+    ++copies.count;
     slices.push_back(std::move(sl));
   }
-  for (auto& sl : slices)
+  // means and pooled = (symmatu(T^T T) - sum_c n_c mu_c mu_c^T) / max(1, n - C)
+  const double df = std::max(1.0, (double)n - (double)C);
+  const unsigned tiles = (unsigned)((kmax + 15) / 16);
+  launch(ctx, "lda_pooled", k_lda_pooled, dim3(tiles, tiles), dim3(256), 0,
+         (const double*)sums.get(), (int64_t)kmax, (int64_t)C, (const double*)counts_out.get(),
+         G, ldg, 1.0 / df, pend.pooled.get(), pend.means.get(), copies, empty);
+  pend.n = n;
+  pend.C = C;
+  pend.kmax = kmax;
+  pend.counts = counts_out.get();
+  pend.f32_scores = T.dtype == FPLS_F32;
+  // the ridge rule and the inverse (mode 1 of the slice solves); the solve
+  // writes its status (1: singular) straight into the deferred check
+  for (auto& sl : slices) {
+    int32_t* sing = ctx->defer_check(1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular", 1);
     jobs.push_back(SpdJob{pend.pooled.get(), (int64_t)kmax, sl.k, 1, nullptr, ridge,
-                          sl.inv_cov.get(), (int64_t)sl.k, sl.ridge_d.get(), sl.status_d.get()});
+                          sl.inv_cov.get(), (int64_t)sl.k, sl.ridge_d.get(), sing});
+  }
 }
 
-// lda_finish: after the solves, the singular-covariance checks, linear terms
-// means inv_cov and the constants -1/2 mu linear + log(max(prior, DBL_MIN)).
+// lda_finish: after the solves, the linear terms means inv_cov and the
+// constants -1/2 mu linear + log(max(prior, DBL_MIN)) of every slice in one
+// launch, and (FP32 scores) linear^T as the padded operand of the latent scores.
 void lda_finish(Ctx* ctx, const LdaPending& pend, std::vector<LdaSlice>& slices) {
   const int64_t C = pend.C;
+  LdaLinJobs lj{};
+  int kbig = 1;
   for (auto& sl : slices) {
-    const int k = sl.k;
-    int32_t* sing = ctx->defer_check(1, FPLS_ERR_SOLVE, "LDA pooled covariance is singular", 1);
-    copy_i32(ctx, sing, sl.status_d.get(), 1);
-    sl.linear = DBuf<double>(ctx, (size_t)C * k);
-    gemm_f64(ctx, false, false, C, k, k, 1.0, sl.means.get(), C, nullptr, sl.inv_cov.get(), k,
-             nullptr, 0.0, sl.linear.get(), C);
+    sl.linear = DBuf<double>(ctx, (size_t)C * sl.k);
     sl.constants = DBuf<double>(ctx, C);
-    launch(ctx, "lda_constants", k_lda_constants, dim3((C + 3) / 4), dim3(256), 0,
-           sl.means.get(), C, sl.linear.get(), C, C, (int64_t)k, pend.counts, (double)pend.n,
-           sl.constants.get());
+    lj.j[lj.count++] = LdaLinJob{sl.means.get(), sl.inv_cov.get(), sl.linear.get(),
+                                 sl.constants.get(), sl.k};
+    kbig = std::max(kbig, sl.k);
   }
+  FPLS_REQUIRE(kbig <= kJacobiMax, FPLS_ERR_ARG, "LDA supports up to 128 components");
+  launch(ctx, "lda_linear", k_lda_linear, dim3((unsigned)C, (unsigned)lj.count), dim3(256), 0,
+         lj, C, pend.counts, (double)pend.n);
+  if (pend.f32_scores)
+    for (auto& sl : slices) sl.linT_pad = nn_pad_operand(ctx, sl.linear.get(), C, true, sl.k, C);
 }
 
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
@@ -276,7 +363,10 @@ void lda_scores(Ctx* ctx, const Model* M, const LdaSlice& sl, const DMat& X, con
     }
   } else {
     // latent form: T lin^T with T = Xs R_predict (src/fastPLS.cpp:2139-2146)
-    if (Tpre.dtype == FPLS_F32) {
+    if (Tpre.dtype == FPLS_F32 && sl.linT_pad.get()) {
+      gemm_nn_f32_padded(ctx, M->precision, w, C, kk, Tpre.f(), Tpre.ld, nullptr, nullptr,
+                         sl.linT_pad.get(), nullptr, nullptr, scores, w);
+    } else if (Tpre.dtype == FPLS_F32) {
       DBuf<double> linT(ctx, (size_t)kk * C);
       transpose_f64(ctx, sl.linear.get(), C, C, kk, linT.get(), kk);
       DBuf<float> linTf(ctx, (size_t)kk * C);
@@ -464,6 +554,48 @@ void lda_predict_core(Ctx* c, const Model* M, const DMat& X, int32_t* lab,
 }
 
 }  // namespace fpls
+
+extern "C" int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* Xm,
+                                           const fpls_ivec* yv, int32_t n_classes,
+                                           const int32_t* ncomp, int32_t n_ncomp,
+                                           const fpls_opts* opts, double lda_ridge,
+                                           const fpls_mat* Xtest, int32_t* labels,
+                                           fpls_model** out) {
+  if (out) *out = nullptr;
+  std::unique_ptr<fpls_model> mdl(new fpls_model());
+  int rc = fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    FPLS_REQUIRE(out != nullptr && labels != nullptr, FPLS_ERR_ARG, "out or labels is NULL");
+    FPLS_REQUIRE(Xtest != nullptr, FPLS_ERR_ARG, "plssvd_lda_fit_predict: Xtest is NULL");
+    fpls_opts o;
+    fpls_opts_default(&o);
+    if (opts) o = *opts;
+    DBuf<unsigned char> hx, ht;
+    DBuf<int32_t> hl;
+    DMat X = stage(c, Xm, hx, "plssvd_lda: X");
+    FPLS_REQUIRE(yv && yv->ptr, FPLS_ERR_ARG, "plssvd_lda: class labels are required");
+    FPLS_REQUIRE(n_classes >= 2, FPLS_ERR_ARG, "LDA requires at least two classes");
+    const int32_t* y = stage_labels(c, yv, X.rows, hl, "plssvd_lda: y");
+    DMat Xt = stage(c, Xtest, ht, "plssvd_lda: Xtest");
+    FPLS_REQUIRE(Xt.cols == X.cols, FPLS_ERR_ARG,
+                 "plssvd_lda: Xtest has the wrong number of columns");
+    LdaFuse lf;
+    lf.ridge = lda_ridge;
+    lf.ncomp = ncomp;
+    lf.n_ncomp = n_ncomp;
+    DMat Y;
+    Model* M = &mdl->m;
+    plssvd_fit_dev(c, M, X, Y, y, n_classes, ncomp, n_ncomp, o, &lf);
+    FPLS_REQUIRE(!M->lda.empty(), FPLS_ERR_STATE, "plssvd_lda: the LDA head was not fitted");
+    const int L = (int)M->ncomp.size();
+    DBuf<int32_t> lab(c, (size_t)Xt.rows * L);
+    lda_predict_core(c, M, Xt, lab.get(), nullptr);
+    // completed by the entry point's one synchronisation (the checks' read-back)
+    c->to_host_later(labels, lab.get(), (size_t)Xt.rows * L * sizeof(int32_t));
+  });
+  if (rc == FPLS_OK) *out = mdl.release();
+  return rc;
+}
 
 extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
                                 int32_t* labels, double* scores_out) {

@@ -17,6 +17,8 @@ struct LdaSlice {
   DBuf<double> constants;   // C
   DBuf<double> ridge_d;     // applied ridge (1), read back on request
   DBuf<int32_t> status_d;   // 0 Cholesky, 1 singular, 2 LU fallback (1)
+  DBuf<float> linT_pad;     // linear^T (k x C) as the padded FP32 operand of the
+                            // latent scores (optional; built by the fit)
 };
 
 struct CknnSlice {
@@ -45,6 +47,8 @@ struct Model {
   // projection cache (R/main.R:1019-1041): R_predict = R / vX, offset = mX R_predict
   DBuf<double> R_predict, R_offset;
   DBuf<float> R_predict_f32;    // for the MFMA projection of FP32 inputs
+  DBuf<float> R_pad_f32;        // R_predict (kmax columns) as the padded operand
+  DBuf<float> mX_f32;           // mX in FP32, the shift of that projection
   // training scores T = Xc R (n x kmax): FP32 for FP32 inputs, FP64 otherwise
   DBuf<float> T_f32;
   DBuf<double> T_f64;
@@ -76,7 +80,7 @@ struct Model {
 };
 
 // R_predict / R_offset / FP32 copy from R, vX, mX (k columns).
-void build_projection_cache(Ctx* ctx, Model* M);
+void build_projection_cache(Ctx* ctx, Model* M, bool unit_scale = false);
 // T = (X - 1 mX^T) R_predict[:, :k]  (FP32 MFMA for FP32 X, FP64 otherwise),
 // written into an n x k buffer of the input's dtype (Tf or Td).
 void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, double* Td,
@@ -85,6 +89,10 @@ void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, d
 // Exact / randomized truncated SVD of a device FP64 matrix (rsvd.hip).
 struct SvdOut {
   DBuf<double> U, s, Vt;        // U rows x k, s k, Vt k x cols
+  // want_v: the exact path of a tall A (cols <= rows) returns V (cols x >= k,
+  // ld cols; the eigenvector matrix itself) instead of Vt
+  DBuf<double> V;
+  bool want_v = false;
   int k = 0;
 };
 void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
@@ -124,13 +132,16 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
 struct LdaPending {
   int64_t n = 0;
   int C = 0, kmax = 0;
+  bool f32_scores = false;        // T was FP32: keep the padded FP32 linear^T
   DBuf<double> means, pooled;     // C x kmax, kmax x kmax
   const double* counts = nullptr; // C class counts (the model's)
 };
+// counts_ready: counts_out already holds the FP64 class counts of y (its
+// producer checked the label range), so they are not counted again.
 void lda_prepare(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,
                  int32_t C, const std::vector<int>& ks, double ridge, int precision,
                  DBuf<double>& counts_out, std::vector<LdaSlice>& slices, LdaPending& pend,
-                 std::vector<SpdJob>& jobs);
+                 std::vector<SpdJob>& jobs, bool counts_ready = false);
 void lda_finish(Ctx* ctx, const LdaPending& pend, std::vector<LdaSlice>& slices);
 // LDA (lda.hip): prefix models for every k in ks on scores T; G = T^T T optional.
 void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const int32_t* y,

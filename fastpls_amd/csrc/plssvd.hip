@@ -21,25 +21,9 @@
 #include <vector>
 
 #include "model.h"
+#include "fpls_wave.h"
 
 namespace fpls {
-
-__global__ void k_div_scalar(double* x, int64_t n, double d) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    x[i] /= d;
-}
-
-// sd from sums of squared deviations: sqrt(ssd / (n-1)); guard != 0 maps
-// zero / non-finite to 1 (label-aware rule, R/main.R:553-557).
-__global__ void k_sd_from_ssd(double* v, int64_t p, int64_t n, int guard) {
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < p;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    double s = sqrt(v[j] / (double)(n > 1 ? n - 1 : 1));
-    if (guard && (!isfinite(s) || s == 0.0)) s = 1.0;
-    v[j] = s;
-  }
-}
 
 __global__ void k_counts_to_prop(const unsigned long long* counts, double* mY, double* cnt_d,
                                  int C, int64_t n) {
@@ -53,18 +37,59 @@ __global__ void k_counts_to_prop(const unsigned long long* counts, double* mY, d
 
 static unsigned g1(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536)); }
 
-void build_projection_cache(Ctx* ctx, Model* M) {
+// The projection cache in one pass, block j per column of the padded FP32
+// operand (Kp x Np, ld Kp; j >= k and rows >= p are zero):
+//   R_predict(:, j) = R(:, j) / vX (or R when unit; zero or non-finite vX
+//   count as 1), its FP32 copy (ld p) and
+//   padded copy, offset[j] = mX . R_predict(:, j) (fixed-order block sum), and
+//   from block 0 the FP32 mX.
+__global__ __launch_bounds__(256) void k_proj_cache(const double* __restrict__ R, int64_t p,
+                                                    int64_t k, const double* __restrict__ vX,
+                                                    const double* __restrict__ mX, int unit,
+                                                    double* __restrict__ Rp,
+                                                    float* __restrict__ Rf,
+                                                    float* __restrict__ Rpad, int64_t Kp,
+                                                    double* __restrict__ offset,
+                                                    float* __restrict__ mXf) {
+  __shared__ double red[4];
+  const int64_t j = blockIdx.x;
+  const int t = threadIdx.x;
+  double acc = 0.0;
+  for (int64_t i = t; i < Kp; i += 256) {
+    float f = 0.f;
+    if (j < k && i < p) {
+      double g = unit ? 1.0 : vX[i];
+      if (!isfinite(g) || g == 0.0) g = 1.0;  // the rule of scale_rows_div
+      const double x = unit ? R[i + j * p] : R[i + j * p] / g;
+      Rp[i + j * p] = x;
+      f = (float)x;
+      Rf[i + j * p] = f;
+      acc = fma(mX[i], x, acc);
+    }
+    Rpad[i + j * Kp] = f;
+    if (j == 0 && i < p) mXf[i] = (float)mX[i];
+  }
+  if (j >= k) return;  // uniform
+  acc = wave_sum_d(acc);
+  if ((t & 63) == 0) red[t >> 6] = acc;
+  __syncthreads();
+  if (t == 0) offset[j] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+void build_projection_cache(Ctx* ctx, Model* M, bool unit_scale) {
   const int64_t p = M->p;
   const int k = M->kmax;
+  const int64_t Kp = nn_pad_k(p), Np = nn_pad_n(k);
   M->R_predict = DBuf<double>(ctx, (size_t)p * k);
   M->R_offset = DBuf<double>(ctx, k);
-  copy_f64(ctx, M->R_predict.get(), M->R.get(), p * k);
-  scale_rows_div(ctx, M->R_predict.get(), p, p, k, M->vX.get());
-  // offset = mX R_predict  (k)
-  gemm_f64(ctx, true, false, k, 1, p, 1.0, M->R_predict.get(), p, nullptr, M->mX.get(), p,
-           nullptr, 0.0, M->R_offset.get(), k);
   M->R_predict_f32 = DBuf<float>(ctx, (size_t)p * k);
-  f64_to_f32(ctx, M->R_predict_f32.get(), M->R_predict.get(), p * k);
+  M->R_pad_f32 = DBuf<float>(ctx, (size_t)Kp * Np);
+  M->mX_f32 = DBuf<float>(ctx, (size_t)p);
+  // vX == 1 (no autoscaling): the division would return R itself
+  launch(ctx, "proj_cache", k_proj_cache, dim3((unsigned)Np), dim3(256), 0,
+         (const double*)M->R.get(), p, (int64_t)k, (const double*)M->vX.get(),
+         (const double*)M->mX.get(), unit_scale ? 1 : 0, M->R_predict.get(),
+         M->R_predict_f32.get(), M->R_pad_f32.get(), Kp, M->R_offset.get(), M->mX_f32.get());
 }
 
 void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, double* Td,
@@ -74,8 +99,12 @@ void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, d
   // (X - 1 mX^T) R_predict == X R_predict - 1 offset^T (R/main.R:1003-1016);
   // centring on load keeps FP32 accuracy for uncentred data.
   if (X.dtype == FPLS_F32) {
-    gemm_nn_f32(ctx, M->precision, X.rows, k, X.cols, X.f(), X.ld, M->mX.get(),
-                M->R_predict_f32.get(), M->p, nullptr, Tf, Td, ldt);
+    if (k == M->kmax && M->R_pad_f32.get() && M->mX_f32.get())
+      gemm_nn_f32_padded(ctx, M->precision, X.rows, k, X.cols, X.f(), X.ld, M->mX.get(),
+                         M->mX_f32.get(), M->R_pad_f32.get(), nullptr, Tf, Td, ldt);
+    else
+      gemm_nn_f32(ctx, M->precision, X.rows, k, X.cols, X.f(), X.ld, M->mX.get(),
+                  M->R_predict_f32.get(), M->p, nullptr, Tf, Td, ldt);
   } else {
     FPLS_REQUIRE(Td != nullptr, FPLS_ERR_ARG, "projection: FP64 input needs an FP64 output");
     gemm_f64(ctx, false, false, X.rows, k, X.cols, 1.0, X.d(), X.ld, M->mX.get(),
@@ -89,36 +118,34 @@ void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int3
   const int64_t n = X.rows, p = X.cols;
   M->mX = DBuf<double>(ctx, p);
   M->vX = DBuf<double>(ctx, p);
+  // means (and unit scales) in the column pass itself; sd = sqrt(ssd / (n-1))
   if (scaling < 3) {
-    col_sums(ctx, X, M->mX.get());
-    launch(ctx, "div", k_div_scalar, dim3(g1(p)), dim3(256), 0, M->mX.get(), p, (double)n);
+    col_stat(ctx, X, nullptr, M->mX.get(), (double)n, 0, scaling == 2 ? nullptr : M->vX.get());
   } else {
     fill_f64(ctx, M->mX.get(), 0.0, p);
   }
   if (scaling == 2) {
-    col_ssd(ctx, X, M->mX.get(), M->vX.get());
-    launch(ctx, "sd", k_sd_from_ssd, dim3(g1(p)), dim3(256), 0, M->vX.get(), p, n,
-           guard_sd ? 1 : 0);
-  } else {
+    col_stat(ctx, X, M->mX.get(), M->vX.get(), (double)(n > 1 ? n - 1 : 1), guard_sd ? 2 : 1);
+  } else if (scaling >= 3) {
     fill_f64(ctx, M->vX.get(), 1.0, p);
   }
   if (y) {
     M->mY = DBuf<double>(ctx, C);
     M->counts = DBuf<double>(ctx, C);
-    DBuf<int64_t> cnt(ctx, C);
     // rows with codes outside 1..C are counted in no class by every kernel of
     // the fit; the count is checked when the entry point returns
     int32_t* bad =
         ctx->defer_check(1, FPLS_ERR_LABEL_RANGE, "labels must be encoded as 1..n_classes");
-    label_counts(ctx, y, n, C, cnt.get(), bad);
-    launch(ctx, "counts", k_counts_to_prop, dim3((C + 255) / 256), dim3(256), 0,
-           reinterpret_cast<const unsigned long long*>(cnt.get()), M->mY.get(), M->counts.get(),
-           C, n);
+    if (!label_counts_prop(ctx, y, n, C, M->mY.get(), M->counts.get(), bad)) {
+      DBuf<int64_t> cnt(ctx, C);
+      label_counts(ctx, y, n, C, cnt.get(), bad, true);
+      launch(ctx, "counts", k_counts_to_prop, dim3((C + 255) / 256), dim3(256), 0,
+             reinterpret_cast<const unsigned long long*>(cnt.get()), M->mY.get(),
+             M->counts.get(), C, n);
+    }
   } else {
     M->mY = DBuf<double>(ctx, Y->cols);
-    col_sums(ctx, *Y, M->mY.get());
-    launch(ctx, "div", k_div_scalar, dim3(g1(Y->cols)), dim3(256), 0, M->mY.get(), Y->cols,
-           (double)n);
+    col_stat(ctx, *Y, nullptr, M->mY.get(), (double)n, 0);
   }
 }
 
@@ -349,26 +376,41 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision);
     // truncated SVD of S (src/fastPLS.cpp:4843-4856, plssvd_use_small_exact_svd :318-324)
     SvdOut sv;
+    sv.want_v = true;
     const bool full = M->label_aware || o.svd_method == FPLS_SVD_EXACT || max_rank <= 32;
     rsvd_dev(c, S.get(), p, p, m, kmax, o.oversample, o.power, o.seed, nullptr, false, full, sv);
     kmax = std::min(kmax, sv.k);
     FPLS_REQUIRE(kmax >= 1, FPLS_ERR_RANK, "plssvd effective rank is < 1 after SVD");
     M->kmax = kmax;
-    M->R = DBuf<double>(c, (size_t)p * kmax);
-    copy_f64(c, M->R.get(), sv.U.get(), p * kmax);
-    M->s = DBuf<double>(c, kmax);
-    copy_f64(c, M->s.get(), sv.s.get(), kmax);
-    M->Q = DBuf<double>(c, (size_t)m * kmax);  // V = Vt^T (m x kmax)
-    transpose_f64(c, sv.Vt.get(), sv.k, kmax, m, M->Q.get(), m);
-    build_projection_cache(c, M);
+    // the factors move into the model when the SVD kept exactly kmax of them
+    if (sv.k == kmax) {
+      M->R = std::move(sv.U);
+      M->s = std::move(sv.s);
+    } else {
+      M->R = DBuf<double>(c, (size_t)p * kmax);
+      copy_f64(c, M->R.get(), sv.U.get(), p * kmax);
+      M->s = DBuf<double>(c, kmax);
+      copy_f64(c, M->s.get(), sv.s.get(), kmax);
+    }
+    if (sv.V.get()) {
+      M->Q = std::move(sv.V);  // V (m x >= kmax, ld m)
+    } else {
+      M->Q = DBuf<double>(c, (size_t)m * kmax);  // V = Vt^T (m x kmax)
+      transpose_f64(c, sv.Vt.get(), sv.k, kmax, m, M->Q.get(), m);
+    }
+    build_projection_cache(c, M, o.scaling != 2);
     fit_scores_and_gram(c, M, X);
     // per-slice latent coefficients
     for (int& v : nc) v = std::min(v, kmax);
     M->ncomp = nc;
     M->C_latent = DBuf<double>(c, (size_t)kmax * kmax * L);
     M->W_latent = DBuf<double>(c, (size_t)kmax * m * L);
-    fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
-    fill_f64(c, M->W_latent.get(), 0.0, (int64_t)kmax * m * L);
+    // slices below kmax leave the trailing rows and columns of their blocks
+    // unwritten; at kmax the solves and products write every entry
+    if (std::any_of(nc.begin(), nc.end(), [&](int v) { return v < kmax; })) {
+      fill_f64(c, M->C_latent.get(), 0.0, (int64_t)kmax * kmax * L);
+      fill_f64(c, M->W_latent.get(), 0.0, (int64_t)kmax * m * L);
+    }
     int32_t* st = c->defer_check(L, FPLS_ERR_SOLVE, "plssvd latent solve failed", 1);
     const bool fuse = lda != nullptr && y != nullptr && kmax <= kJacobiMax;
     LdaPending lpend;
@@ -382,7 +424,7 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
       const std::vector<int> ks(kset.begin(), kset.end());
       std::vector<SpdJob> jobs;
       lda_prepare(c, M->T(), M->G.get(), kmax, y, (int32_t)m, ks, lda->ridge, M->precision,
-                  M->counts, M->lda, lpend, jobs);
+                  M->counts, M->lda, lpend, jobs, true);
       spd_slice_jobs(M->G.get(), kmax, kmax, nc, L, M->label_aware ? 2 : 0, M->s.get(), 0.0,
                      M->C_latent.get(), kmax, (int64_t)kmax * kmax, nullptr, st, jobs);
       spd_solve_jobs(c, jobs);

@@ -26,6 +26,32 @@ __global__ void k_sqrt_clamp(const double* __restrict__ w, double* __restrict__ 
   if (i < n) s[i] = sqrt(fmax(w[i], 0.0));
 }
 
+// Left factor of the exact SVD from the eigenpairs of A^T A:
+//   s_j = sqrt(max(w_j, 0)),  U(:, j) = A E(:, j) / s_j  (s_j = 0 leaves A E),
+// one output per thread (four accumulators over the cols-long product), the
+// columns of E broadcast per block; block (0, j) writes s_j.
+__global__ __launch_bounds__(256) void k_svd_left(const double* __restrict__ A, int64_t lda,
+                                                  int64_t rows, int64_t cols,
+                                                  const double* __restrict__ E, int64_t lde,
+                                                  const double* __restrict__ w,
+                                                  double* __restrict__ U,
+                                                  double* __restrict__ s) {
+  const int64_t j = blockIdx.y, i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const double sj = sqrt(fmax(w[j], 0.0));
+  if (blockIdx.x == 0 && threadIdx.x == 0) s[j] = sj;
+  if (i >= rows) return;
+  const double* e = E + j * lde;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t c = 0;
+  for (; c + 4 <= cols; c += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = fma(A[i + (c + u) * lda], e[c + u], a[u]);
+  }
+  for (; c < cols; ++c) a[0] = fma(A[i + c * lda], e[c], a[0]);
+  const double v = (a[0] + a[1]) + (a[2] + a[3]);
+  U[i + j * rows] = sj != 0.0 ? v / sj : v;
+}
+
 void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l) {
   DBuf<double> G(ctx, (size_t)l * l), Linv(ctx, (size_t)l * l), tmp(ctx, (size_t)rows * l);
   // a Cholesky that still breaks down after its shifted retries (status -1,
@@ -53,19 +79,24 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
   DBuf<double> Gm(ctx, (size_t)g * g), w(ctx, g), E(ctx, (size_t)g * g);
   out.s = DBuf<double>(ctx, kk);
   out.U = DBuf<double>(ctx, (size_t)rows * kk);
-  if (!left_only) out.Vt = DBuf<double>(ctx, (size_t)kk * cols);
+  if (!left_only && !(out.want_v && cols <= rows)) out.Vt = DBuf<double>(ctx, (size_t)kk * cols);
   out.k = kk;
   if (cols <= rows) {
     gemm_f64(ctx, true, false, cols, cols, rows, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);
     sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get());
-    launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
-           kk);
-    // U = A V / s
-    gemm_f64(ctx, false, false, rows, kk, cols, 1.0, A, lda, nullptr, E.get(), g, nullptr, 0.0,
-             out.U.get(), rows);
-    scale_cols(ctx, out.U.get(), rows, rows, kk, out.s.get(), true);
-    if (!left_only) transpose_f64(ctx, E.get(), g, cols, kk, out.Vt.get(), kk);
+    // s = sqrt(w), U = A V / s in one pass (the product is rows x cols x kk)
+    launch(ctx, "svd_left", k_svd_left, dim3((unsigned)((rows + 255) / 256), (unsigned)kk),
+           dim3(256), 0, A, lda, rows, cols, (const double*)E.get(), (int64_t)g,
+           (const double*)w.get(), out.U.get(), out.s.get());
+    if (!left_only) {
+      if (out.want_v) {
+        out.V = std::move(E);
+        out.Vt = DBuf<double>();
+      } else {
+        transpose_f64(ctx, E.get(), g, cols, kk, out.Vt.get(), kk);
+      }
+    }
   } else {
     gemm_f64(ctx, false, true, rows, rows, cols, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);

@@ -11,49 +11,49 @@ namespace fpls {
 void* Pool::get(size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
   if (bytes == 0) bytes = 256;
-  // Smallest free block that fits without wasting more than half of it.
-  int best = -1;
-  for (size_t i = 0; i < blocks.size(); ++i) {
-    const Block& b = blocks[i];
-    if (!b.used && b.bytes >= bytes && b.bytes <= 2 * bytes + (1 << 20)) {
-      if (best < 0 || b.bytes < blocks[best].bytes) best = static_cast<int>(i);
-    }
-  }
-  if (best >= 0) {
-    blocks[best].used = true;
-    return blocks[best].ptr;
+  // smallest free block that fits without wasting more than half of it
+  auto it = free_by_size.lower_bound(bytes);
+  if (it != free_by_size.end() && it->first <= 2 * bytes + (1 << 20)) {
+    const size_t i = it->second;
+    free_by_size.erase(it);
+    blocks[i].used = true;
+    return blocks[i].ptr;
   }
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     // Drop cached blocks and retry once.
     (void)hipGetLastError();
-    for (auto it = blocks.begin(); it != blocks.end();) {
-      if (!it->used) {
-        (void)hipFree(it->ptr);
-        it = blocks.erase(it);
-      } else {
-        ++it;
-      }
+    std::vector<Block> keep;
+    for (const Block& b : blocks) {
+      if (b.used) keep.push_back(b);
+      else (void)hipFree(b.ptr);
     }
+    blocks.swap(keep);
+    index.clear();
+    free_by_size.clear();
+    for (size_t i = 0; i < blocks.size(); ++i) index[blocks[i].ptr] = i;
     FPLS_HIP(hipMalloc(&p, bytes));
   }
+  index[p] = blocks.size();
   blocks.push_back({p, bytes, true});
   return p;
 }
 
 void Pool::put(void* p) {
-  for (auto& b : blocks) {
-    if (b.ptr == p) {
-      b.used = false;
-      return;
-    }
-  }
+  auto it = index.find(p);
+  if (it == index.end()) return;
+  Block& b = blocks[it->second];
+  if (!b.used) return;
+  b.used = false;
+  free_by_size.emplace(b.bytes, it->second);
 }
 
 void Pool::release_all() {
   for (auto& b : blocks) (void)hipFree(b.ptr);
   blocks.clear();
+  index.clear();
+  free_by_size.clear();
 }
 
 size_t Pool::total_bytes() const {
@@ -99,6 +99,14 @@ void Ctx::free(void* p) {
 // ---------------------------------------------------------------- deferred checks
 constexpr int kFlagSlots = 1024;
 
+__global__ void k_fill_flags(int32_t* f, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = 0;
+}
+
+// The flag slots are zero between entry points: they are cleared once when
+// allocated and again (one fill over the slots used) whenever a batch of
+// checks is read back or dropped, so handing out a slot launches nothing.
 int32_t* Ctx::defer_check(int count, int code, const std::string& msg, int fail_value) {
   FPLS_REQUIRE(count >= 1 && count <= kFlagSlots, FPLS_ERR_ARG, "too many deferred checks");
   if (!flags_dev) {
@@ -106,23 +114,85 @@ int32_t* Ctx::defer_check(int count, int code, const std::string& msg, int fail_
                  "allocation-free call reached a deferred check before any other call");
     FPLS_HIP(hipMalloc(&flags_dev, kFlagSlots * sizeof(int32_t)));
     FPLS_HIP(hipHostMalloc(&flags_host, kFlagSlots * sizeof(int32_t), hipHostMallocDefault));
+    fill_i32(this, flags_dev, 0, kFlagSlots);
   }
   if (nflags + count > kFlagSlots) flush_checks();
   int32_t* p = flags_dev + nflags;
-  fill_i32(this, p, 0, count);
   checks.push_back(Check{nflags, count, code, fail_value, msg});
   nflags += count;
   return p;
 }
 
+unsigned* Ctx::ticket() {
+  if (!ticket_dev) {
+    FPLS_REQUIRE(!arena_base, FPLS_ERR_STATE,
+                 "allocation-free call reached a ticket counter before any other call");
+    FPLS_HIP(hipMalloc(&ticket_dev, 64 * sizeof(unsigned)));
+    fill_i32(this, reinterpret_cast<int32_t*>(ticket_dev), 0, 64);
+  }
+  return ticket_dev;
+}
+
+void Ctx::drop_checks() {
+  // an entry point that failed half way: its kernels may have set slots
+  if (flags_dev && nflags > 0) {
+    hipLaunchKernelGGL(k_fill_flags, dim3((nflags + 255) / 256), dim3(256), 0, stream, flags_dev,
+                       nflags);
+    (void)hipGetLastError();
+  }
+  checks.clear();
+  nflags = 0;
+  host_copies.clear();
+  stage_used = 0;
+}
+
+void Ctx::to_host_later(void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (stage_used + need > stage_cap) {
+    complete_host_copies(false);
+    if (need > stage_cap) {
+      if (stage_host) FPLS_HIP(hipHostFree(stage_host));
+      stage_host = nullptr;
+      stage_cap = std::max<size_t>(need, std::max<size_t>(2 * stage_cap, 1 << 20));
+      FPLS_HIP(hipHostMalloc(reinterpret_cast<void**>(&stage_host), stage_cap,
+                             hipHostMallocDefault));
+    }
+  }
+  FPLS_HIP(hipMemcpyAsync(stage_host + stage_used, src, bytes, hipMemcpyDeviceToHost, stream));
+  host_copies.push_back(HostCopy{dst, stage_used, bytes});
+  stage_used += need;
+}
+
+void Ctx::complete_host_copies(bool synced) {
+  if (host_copies.empty()) return;
+  if (!synced) FPLS_HIP(hipStreamSynchronize(stream));
+  for (const HostCopy& h : host_copies) std::memcpy(h.dst, stage_host + h.off, h.bytes);
+  host_copies.clear();
+  stage_used = 0;
+}
+
+unsigned long long* Ctx::zero_counts() {
+  if (!zcounts_dev) {
+    FPLS_REQUIRE(!arena_base, FPLS_ERR_STATE,
+                 "allocation-free call reached the count buffer before any other call");
+    FPLS_HIP(hipMalloc(&zcounts_dev, kZeroCounts * sizeof(unsigned long long)));
+    fill_f64(this, reinterpret_cast<double*>(zcounts_dev), 0.0, kZeroCounts);
+  }
+  return zcounts_dev;
+}
+
 void Ctx::flush_checks() {
   if (checks.empty()) {
+    complete_host_copies(false);
     nflags = 0;
     return;
   }
   FPLS_HIP(hipMemcpyAsync(flags_host, flags_dev, nflags * sizeof(int32_t), hipMemcpyDeviceToHost,
                           stream));
   FPLS_HIP(hipStreamSynchronize(stream));
+  complete_host_copies(true);
+  fill_i32(this, flags_dev, 0, nflags);
   std::vector<Check> cs;
   cs.swap(checks);
   nflags = 0;
@@ -303,6 +373,9 @@ void fpls_ctx_destroy(fpls_ctx* ctx) {
   }
   for (auto e : ctx->c.spare_events) (void)hipEventDestroy(e);
   if (ctx->c.flags_dev) (void)hipFree(ctx->c.flags_dev);
+  if (ctx->c.ticket_dev) (void)hipFree(ctx->c.ticket_dev);
+  if (ctx->c.zcounts_dev) (void)hipFree(ctx->c.zcounts_dev);
+  if (ctx->c.stage_host) (void)hipHostFree(ctx->c.stage_host);
   if (ctx->c.flags_host) (void)hipHostFree(ctx->c.flags_host);
   ctx->c.cv_plan.reset();
   ctx->c.pool.release_all();

@@ -457,7 +457,7 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
   int32_t done[2] = {0, 0};
   to_host_i32(c, done, fl, 2);  // the fit's one round trip (waits for the whole chain)
   M->n_components = done[1];
-  build_projection_cache(c, M);
+  build_projection_cache(c, M, M->scaling != 2);
 }
 
 }  // namespace fpls

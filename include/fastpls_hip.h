@@ -22,6 +22,7 @@
  *   fpls_plssvd_fit           pls_model1                 src/fastPLS.cpp:4780-4986
  *                             .plssvd_label_aware_stream_model (labels) R/main.R:513-655
  *   fpls_plssvd_lda_fit       pls_model1 + .attach_lda_classifier R/main.R:1216-1251
+ *   fpls_plssvd_lda_fit_predict  pls(..., Xtest) fit-then-predict tail R/main.R:5640-5646
  *   fpls_simpls_fit           pls_model2_fast            src/fastPLS.cpp:2454-2778
  *   fpls_model_read           the Rcpp::List fields returned by the fits
  *   fpls_model_import         those fields back into a device model (predict()
@@ -166,6 +167,17 @@ int fpls_plssvd_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const f
 int fpls_plssvd_lda_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y, int32_t n_classes,
                         const int32_t* ncomp, int32_t n_ncomp, const fpls_opts* opts,
                         double lda_ridge, fpls_model** out);
+
+/* pls(Xtrain, Ytrain, Xtest, method = "plssvd", classifier = "lda"): the
+   fit of fpls_plssvd_lda_fit and predict(model, Xtest) of the LDA head
+   (R/main.R:5640-5646, the fit-then-predict tail of pls(); prediction
+   lda_project_predict_labels_cpp src/fastPLS.cpp:2100-2147) in one call with
+   one host round trip at its end.  labels: Xtest rows x n_ncomp int32 (host,
+   1-based), as fpls_lda_predict writes them; *out receives the model. */
+int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y,
+                                int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
+                                const fpls_opts* opts, double lda_ridge, const fpls_mat* Xtest,
+                                int32_t* labels, fpls_model** out);
 
 /* SIMPLS ("fast" incremental core).  Dense Y or labels (one-hot response). */
 int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,

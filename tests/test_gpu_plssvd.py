@@ -219,5 +219,16 @@ def test_fused_plssvd_lda_equals_separate_calls(ctx, precision):
     ctx.check(ctx.lib.fpls_lda_predict(ctx.h, h1, ctypes.byref(Xt), l1.ctypes.data, None))
     ctx.check(ctx.lib.fpls_lda_predict(ctx.h, h2, ctypes.byref(Xt), l2.ctypes.data, None))
     np.testing.assert_array_equal(l1, l2)
+    # pls(..., Xtest) with the head: fit and prediction in one call
+    h3 = ctypes.c_void_p()
+    l3 = np.empty_like(l1)
+    ctx.check(ctx.lib.fpls_plssvd_lda_fit_predict(ctx.h, ctypes.byref(Xm), ctypes.byref(yv), 9,
+                                                  ncp, 2, ctypes.byref(opts), 1e-8,
+                                                  ctypes.byref(Xt), l3.ctypes.data,
+                                                  ctypes.byref(h3)))
+    np.testing.assert_array_equal(l3, l1)
+    m3 = F.FastPLS(ctx, h3)
+    np.testing.assert_array_equal(m3.read("R", (p, k)), m1.read("R", (p, k)))
     m1.free()
     m2.free()
+    m3.free()
