@@ -314,6 +314,21 @@ void col_ssd(Ctx* ctx, const DMat& X, const double* center, double* ssd);
 // or non-finite mapped to 1; ones (optional) is set to 1 in every column.
 void col_stat(Ctx* ctx, const DMat& X, const double* center, double* out, double div, int post,
               double* ones = nullptr);
+// Class sums of an FP32 X (n x p, ld) from labels y in 1..C in one pass (any
+// C up to 1663): S (p x C, ld lds) = X^T onehot(y), or with shiftB the centred
+// X^T (onehot(y) - 1 shiftB^T) (independent of any centring of X); mean (p,
+// optional) gets the column means of X and ones (p, optional) the value 1.
+// Returns false (nothing launched) when C is too large.
+bool class_sums_f32(Ctx* ctx, int64_t n, int64_t p, const float* X, int64_t ld, const int32_t* y,
+                    int C, const double* shiftB, double* S, int64_t lds, double* mean,
+                    double* ones);
+// The centred class cross-product S (p x C, ld lds) = (X - 1 m^T)^T
+// (onehot(y) - 1 mY^T) of an FP32 X for C <= 127 on the one-hot MFMA tiles in
+// one pass over X, which also yields the column means (mean, optional) and
+// ones (p, optional, set to 1).  Returns false when C is too large.
+bool class_cross_cov_f32(Ctx* ctx, int precision, int64_t p, int64_t C, int64_t n,
+                         const float* X, int64_t ld, const int32_t* y, const double* mY,
+                         double* S, int64_t lds, double* mean, double* ones);
 // Class counts from 1-based labels; *bad counts labels outside 1..C (cleared
 // first unless bad_zeroed: the slots of deferred checks are zero already).
 void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* counts,

@@ -386,6 +386,38 @@ __global__ void k_reduce_partials_f64(int64_t M, int64_t N, int splits,
   }
 }
 
+// C(i, j) = alpha sum_k A(i, k) op(B)(k, j) + beta C(i, j) for short K.  Block
+// (., j): 64 rows i over the lanes, the four waves split k four ways
+// (k = w + 4u), four accumulators each, and the wave sums are added in wave
+// order through LDS.  Column j of op(B) is the same for the whole block
+// (broadcast loads), A's column k is read coalesced.
+__global__ __launch_bounds__(256) void k_gemm_small(int transB, int64_t M, int64_t K,
+                                                    double alpha, const double* __restrict__ A,
+                                                    int64_t lda, const double* __restrict__ B,
+                                                    int64_t ldb, double beta,
+                                                    double* __restrict__ C, int64_t ldc) {
+  __shared__ double part[4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j = blockIdx.y, i = blockIdx.x * (int64_t)64 + lane;
+  const int64_t bs = transB ? ldb : 1, b0 = transB ? j : j * ldb;
+  const int64_t ir = i < M ? i : M - 1;
+  double a[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t k = w;
+  for (; k + 12 < K; k += 16) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      a[u] = fma(A[ir + (k + 4 * u) * lda], B[b0 + (k + 4 * u) * bs], a[u]);
+  }
+  for (; k < K; k += 4) a[0] = fma(A[ir + k * lda], B[b0 + k * bs], a[0]);
+  part[w][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+  __syncthreads();
+  if (w == 0 && i < M) {
+    const double v = alpha * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+    double* c = C + i + j * ldc;
+    *c = beta == 0.0 ? v : fma(beta, *c, v);
+  }
+}
+
 void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t K, double alpha,
               const double* A, int64_t lda, const double* shiftA, const double* B, int64_t ldb,
               const double* shiftB, double beta, double* C, int64_t ldc,
@@ -398,6 +430,15 @@ void gemm_f64(Ctx* ctx, bool transA, bool transB, int64_t M, int64_t N, int64_t 
     fill_f64(ctx, zero.get(), 0.0, 1);
     launch(ctx, "gemm_f64_reduce", k_reduce_partials_f64, dim3(grid1d(M * N)), dim3(256), 0, M,
            N, 0, zero.get(), alpha, beta, C, ldc);
+    return;
+  }
+  // short products (K <= 256) of a small plain A (every block re-reads all of
+  // A from L2, so A stays within 2 MiB and N within 512): one output per
+  // thread in one launch, no split-K partials to reduce
+  if (!transA && K <= 256 && M >= 32 && M * K <= (int64_t(1) << 18) && N <= 512 && !labelsB &&
+      !shiftA && !shiftB) {
+    launch(ctx, kname, k_gemm_small, dim3((unsigned)((M + 63) / 64), (unsigned)N), dim3(256), 0,
+           transB ? 1 : 0, M, K, alpha, A, lda, B, ldb, beta, C, ldc);
     return;
   }
   // the large tile pays when there are enough of them (or a long K to split);

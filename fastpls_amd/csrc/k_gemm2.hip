@@ -14,6 +14,7 @@
 // B[l/16][l%16]; 16x16x32 bf16 -- lane l supplies A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16]; both accumulate D[4(l/16)+r][l%16].
 #include <algorithm>
+#include <type_traits>
 
 #include "fpls_internal.h"
 
@@ -447,6 +448,35 @@ __global__ __launch_bounds__(256) void k_g2_reduce(int64_t M, int64_t N, int spl
   }
 }
 
+// The split-K sum with the epilogue of k_proj: C = sum_z P_z - 1 offset^T
+// into the FP32 or FP64 output, column- or row-major; one output per thread,
+// the slabs in order (coalesced reads of every partial).
+__global__ __launch_bounds__(256) void k_g2_reduce_out(int64_t M, int64_t N, int splits,
+                                                       const float* __restrict__ partial,
+                                                       const double* __restrict__ off,
+                                                       float* __restrict__ Cf,
+                                                       double* __restrict__ Cd, int64_t ldc,
+                                                       int crow) {
+  const int64_t n = M * N, t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double r = 0.0;
+  int z = 0;
+  for (; z + 4 <= splits; z += 4) {
+    const float p0 = partial[(int64_t)z * n + t], p1 = partial[(int64_t)(z + 1) * n + t];
+    const float p2 = partial[(int64_t)(z + 2) * n + t], p3 = partial[(int64_t)(z + 3) * n + t];
+    r += static_cast<double>(p0);
+    r += static_cast<double>(p1);
+    r += static_cast<double>(p2);
+    r += static_cast<double>(p3);
+  }
+  for (; z < splits; ++z) r += static_cast<double>(partial[(int64_t)z * n + t]);
+  const int64_t row = t % M, col = t / M;
+  if (off) r -= off[col];
+  const int64_t idx = crow ? row * ldc + col : row + col * ldc;
+  if (Cd) Cd[idx] = r;
+  else Cf[idx] = static_cast<float>(r);
+}
+
 // ---------------------------------------------------------------- skinny NN
 // T (M x N, N <= 128) = (X - 1 shA^T) B with X streamed straight from HBM into
 // MFMA operand registers (no LDS for the big operand).  Workgroup = 4 waves x
@@ -497,7 +527,15 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   auto kof = [&](int j) { return BF ? 8 * (lane / 16) + j : 4 * j + lane / 16; };
   // the centring shifts arrive as FP32 (converted once per call): any
   // arithmetic on a value at load time would wait on every outstanding load
-  float ac0[8], ac1[8], an0[8], an1[8], mc[8], mn[8];
+  // three register sets (chunks c, c + 1, c + 2) used in a fixed rotation by
+  // the loop unrolled three times: loads run two chunks ahead of the MFMAs
+  // (a workgroup is bound by the bytes it has in flight, not by arithmetic),
+  // and no register that is still the target of a load is ever copied
+  struct ASet {
+    float a[8], b[8], m[8];
+    float4 q[KPT / 4];
+  };
+  ASet R0, R1, R2;
   auto fetchA = [&](int64_t k0, float (&x0)[8], float (&x1)[8], float (&mx)[8]) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -509,21 +547,21 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   };
   // B chunk staging: thread -> column n = t / TPC, KPT consecutive k
   const int bn = t / TPC, bk = (t % TPC) * KPT;
-  float4 bq[KPT / 4];
-  auto fetchB = [&](int64_t k0) {
+  auto fetchB = [&](int64_t k0, float4 (&dst)[KPT / 4]) {
     const float* src = B + (k0 + bk) + (int64_t)bn * ldb;
 #pragma unroll
-    for (int q = 0; q < KPT / 4; ++q) bq[q] = *reinterpret_cast<const float4*>(src + 4 * q);
+    for (int q = 0; q < KPT / 4; ++q) dst[q] = *reinterpret_cast<const float4*>(src + 4 * q);
   };
-  auto storeB = [&](int buf) {
+  auto storeB = [&](int buf, const float4 (&src)[KPT / 4]) {
     if (BF) {
       __bf16* base = reinterpret_cast<__bf16*>(smem);
       __bf16* Bh = base + (0 * 2 + buf) * BW * P_LDB;
       __bf16* Bl = base + (1 * 2 + buf) * BW * P_LDB;
 #pragma unroll
       for (int h = 0; h < KPT / 8; ++h) {
-        const float v[8] = {bq[2 * h].x, bq[2 * h].y, bq[2 * h].z, bq[2 * h].w,
-                            bq[2 * h + 1].x, bq[2 * h + 1].y, bq[2 * h + 1].z, bq[2 * h + 1].w};
+        const float v[8] = {src[2 * h].x, src[2 * h].y, src[2 * h].z, src[2 * h].w,
+                            src[2 * h + 1].x, src[2 * h + 1].y, src[2 * h + 1].z,
+                            src[2 * h + 1].w};
         v8bf hv, lv;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
@@ -539,14 +577,15 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       float* Bs = smem + buf * P_BK * (BW + 4);
 #pragma unroll
       for (int q = 0; q < KPT / 4; ++q) {
-        Bs[(bk + 4 * q + 0) * (BW + 4) + bn] = bq[q].x;
-        Bs[(bk + 4 * q + 1) * (BW + 4) + bn] = bq[q].y;
-        Bs[(bk + 4 * q + 2) * (BW + 4) + bn] = bq[q].z;
-        Bs[(bk + 4 * q + 3) * (BW + 4) + bn] = bq[q].w;
+        Bs[(bk + 4 * q + 0) * (BW + 4) + bn] = src[q].x;
+        Bs[(bk + 4 * q + 1) * (BW + 4) + bn] = src[q].y;
+        Bs[(bk + 4 * q + 2) * (BW + 4) + bn] = src[q].z;
+        Bs[(bk + 4 * q + 3) * (BW + 4) + bn] = src[q].w;
       }
     }
   };
-  auto compute = [&](int buf) {
+  auto compute = [&](int buf, const float (&ac0)[8], const float (&ac1)[8],
+                     const float (&mc)[8]) {
     if (BF) {
       const __bf16* base = reinterpret_cast<const __bf16*>(smem);
       const __bf16* Bh = base + (0 * 2 + buf) * BW * P_LDB;
@@ -597,29 +636,29 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
     }
   };
   const int nk = (int)((kend - kbeg + P_BK - 1) / P_BK);
-  fetchA(kbeg, ac0, ac1, mc);
-  fetchB(kbeg);
-  storeB(0);
-  __syncthreads();
-  for (int c = 0; c < nk; ++c) {
-    const bool more = c + 1 < nk;
-    if (more) {
-      fetchA(kbeg + (int64_t)(c + 1) * P_BK, an0, an1, mn);
-      fetchB(kbeg + (int64_t)(c + 1) * P_BK);
+  // chunk c uses register set c % 3 and LDS buffer c & 1
+  auto step = [&](int c, ASet& cur, ASet& nxt, ASet& far) {
+    if (c + 2 < nk) {
+      fetchA(kbeg + (int64_t)(c + 2) * P_BK, far.a, far.b, far.m);
+      fetchB(kbeg + (int64_t)(c + 2) * P_BK, far.q);
     }
-    // keep the next chunk's loads ahead of this chunk's MFMAs
     __builtin_amdgcn_sched_barrier(0);
-    compute(c & 1);
-    if (more) {
-      storeB((c + 1) & 1);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        ac0[j] = an0[j];
-        ac1[j] = an1[j];
-        mc[j] = mn[j];
-      }
-    }
+    compute(c & 1, cur.a, cur.b, cur.m);
+    if (c + 1 < nk) storeB((c + 1) & 1, nxt.q);
     __syncthreads();
+  };
+  fetchA(kbeg, R0.a, R0.b, R0.m);
+  fetchB(kbeg, R0.q);
+  if (nk > 1) {
+    fetchA(kbeg + P_BK, R1.a, R1.b, R1.m);
+    fetchB(kbeg + P_BK, R1.q);
+  }
+  storeB(0, R0.q);
+  __syncthreads();
+  for (int c = 0; c < nk; c += 3) {
+    step(c, R0, R1, R2);
+    if (c + 1 < nk) step(c + 1, R1, R2, R0);
+    if (c + 2 < nk) step(c + 2, R2, R0, R1);
   }
   // epilogue: lane holds D[4(l/16) + r][l%16] of each 16 x 16 block
   if (part) {  // split-K: raw FP32 slab partial, M x N column-major
@@ -734,9 +773,10 @@ void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K
   if (N <= 128) {
     const int64_t tiles = (M + P_BM - 1) / P_BM;
     const int n32 = (int)N;
-    if (Cd && !offset && !c_rowmajor && tiles < 4 * (int64_t)ctx->num_cus && K >= 512) {
-      // few row tiles: split K into slabs (FP32 partials, summed in FP64)
-      int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
+    if (tiles < (int64_t)ctx->num_cus && K >= 256) {
+      // fewer row tiles than CUs: split K into slabs (FP32 partials, summed in
+      // FP64; the reduction applies the offset and writes either output)
+      int64_t splits = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
       splits = std::min<int64_t>(splits, K / 128);
       splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N)));
       int64_t kchunk = (K + splits - 1) / splits;
@@ -764,8 +804,8 @@ void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K
                                     ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
                                     (int64_t)0, 0, kchunk, part.get());
       }
-      launch(ctx, "gemm_nn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0,
-             M, N, (int)splits, (const float*)part.get(), Cd, ldc);
+      launch(ctx, "gemm_nn_reduce", k_g2_reduce_out, dim3((unsigned)((M * N + 255) / 256)),
+             dim3(256), 0, M, N, (int)splits, (const float*)part.get(), offset, Cf, Cd, ldc, crow);
       return;
     }
     dim3 pgrid((unsigned)tiles);
@@ -852,11 +892,44 @@ __global__ __launch_bounds__(256) void k_tl_reduce(int64_t M, int64_t N, int spl
   }
 }
 
+// k_tl_reduce (with shB) that also finishes the column statistics of the
+// shifted A: threads past M * N write mean[m] = shA[m] + sum_z P_z(m, N) / n
+// and ones[m] = 1 (either optional).
+__global__ __launch_bounds__(256) void k_tl_reduce_stats(int64_t M, int64_t N, int splits,
+                                                         const float* __restrict__ partial,
+                                                         const double* __restrict__ shB,
+                                                         double* __restrict__ C, int64_t ldc,
+                                                         const double* __restrict__ shA,
+                                                         int64_t nrows, double* __restrict__ mean,
+                                                         double* __restrict__ ones) {
+  const int64_t NC = N + 1, stride = M * NC;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  auto slab_sum = [&](int64_t off) {
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) a[q] += (double)partial[(int64_t)(z + q) * stride + off];
+    }
+    for (int q = 0; z + q < splits; ++q) a[q] += (double)partial[(int64_t)(z + q) * stride + off];
+    return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  };
+  if (t < M * N) {
+    const int64_t m = t % M, n = t / M;
+    C[m + n * ldc] = slab_sum(m + n * M) - shB[n] * slab_sum(m + N * M);
+  } else if (t < M * N + M) {
+    const int64_t m = t - M * N;
+    if (mean) mean[m] = shA[m] + slab_sum(m + N * M) / (double)nrows;
+    if (ones) ones[m] = 1.0;
+  }
+}
+
 template <bool BF>
 __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, int64_t kchunk,
                                                    const float* A, int64_t lda,
                                                    const double* shA, const int32_t* lab,
-                                                   int ones, float* partial) {
+                                                   int ones, float* partial,
+                                                   double* __restrict__ self_shift) {
   __shared__ float red[4][TL_BM][128 + 1];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int64_t m0 = (int64_t)blockIdx.x * TL_BM;
@@ -870,8 +943,20 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
   const int64_t mr1 = std::min<int64_t>(m0 + 16 + lane % 16, M - 1);
   const float* col0 = A + mr0 * lda;
   const float* col1 = A + mr1 * lda;
-  const float s0 = shA ? static_cast<float>(shA[mr0]) : 0.f;
-  const float s1 = shA ? static_cast<float>(shA[mr1]) : 0.f;
+  // self_shift: each lane shifts its columns by their first row (a sample of
+  // the column, the classic shifted-data rule; the same in every workgroup,
+  // recorded by slab 0)
+  double d0 = 0.0, d1 = 0.0;
+  if (self_shift) {
+    d0 = (double)col0[0];
+    d1 = (double)col1[0];
+    if (blockIdx.y == 0 && lane < 16) {
+      if (m0 + lane < M) self_shift[m0 + lane] = d0;
+      if (m0 + 16 + lane < M) self_shift[m0 + 16 + lane] = d1;
+    }
+  }
+  const float s0 = shA ? static_cast<float>(shA[mr0]) : static_cast<float>(d0);
+  const float s1 = shA ? static_cast<float>(shA[mr1]) : static_cast<float>(d1);
   const bool vec = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
                    ((reinterpret_cast<uintptr_t>(lab) & 15) == 0) && ((kbeg & 3) == 0);
   v4f acc[2][8];
@@ -880,8 +965,6 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
 #pragma unroll
     for (int b = 0; b < 8; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
   const int ko = 8 * (lane / 16);
-  float x0[8], x1[8], y0[8], y1[8];
-  int lb[8], lc[8];
   auto fetch = [&](int64_t k0, float (&a0)[8], float (&a1)[8], int (&l8)[8]) {
     const int64_t kb = k0 + ko;
     if (vec && kb + 8 <= kend) {
@@ -958,25 +1041,28 @@ __global__ __launch_bounds__(TL_NT) void k_tns_lab(int64_t M, int N, int64_t K, 
       }
     }
   };
-  // wave w takes K steps w, w + 4, w + 8, ... of the slab; the loads of step
-  // i + 1 are issued before the MFMAs of step i
-  int64_t k0 = kbeg + (int64_t)w * TL_BK;
-  if (k0 < kend) fetch(k0, x0, x1, lb);
-  while (k0 < kend) {
-    const int64_t kn = k0 + 4 * TL_BK;
-    const bool more = kn < kend;
-    if (more) fetch(kn, y0, y1, lc);
+  // wave w takes K steps w, w + 4, w + 8, ... of the slab; three register
+  // sets in a fixed rotation (the step loop unrolled three times) keep the
+  // loads of steps i + 1 and i + 2 in flight during the MFMAs of step i, and
+  // no register still waiting for its load is copied
+  struct Set {
+    float a[8], b[8];
+    int l[8];
+  };
+  Set S0, S1, S2;
+  const int64_t kst = kbeg + (int64_t)w * TL_BK, kstr = 4 * TL_BK;
+  const int ns = kst < kend ? (int)((kend - kst + kstr - 1) / kstr) : 0;
+  auto step = [&](int i, Set& cur, Set& far) {
+    if (i + 2 < ns) fetch(kst + (int64_t)(i + 2) * kstr, far.a, far.b, far.l);
     __builtin_amdgcn_sched_barrier(0);
-    compute(x0, x1, lb);
-    if (more) {
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        x0[j] = y0[j];
-        x1[j] = y1[j];
-        lb[j] = lc[j];
-      }
-    }
-    k0 = kn;
+    compute(cur.a, cur.b, cur.l);
+  };
+  if (ns > 0) fetch(kst, S0.a, S0.b, S0.l);
+  if (ns > 1) fetch(kst + kstr, S1.a, S1.b, S1.l);
+  for (int i = 0; i < ns; i += 3) {
+    step(i, S0, S2);
+    if (i + 1 < ns) step(i + 1, S1, S0);
+    if (i + 2 < ns) step(i + 2, S2, S1);
   }
   // sum the four waves: lane holds D[4(l/16) + r][l%16] of each 16 x 16 block
 #pragma unroll
@@ -1285,8 +1371,14 @@ __global__ __launch_bounds__(P_NT) void k_gram(int64_t M, int64_t K, int64_t kch
   const float shv = (sh && mok) ? static_cast<float>(sh[sm]) : 0.f;
   const bool vec = ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0) &&
                    ((kbeg & 3) == 0);
-  float v[16];
-  auto fetch = [&](int64_t k0) {
+  // three register sets (chunks c, c + 1, c + 2) in a fixed rotation of the
+  // loop unrolled three times: the loads run two chunks ahead and no register
+  // still waiting for its load is copied
+  struct GSet {
+    float v[16];
+  };
+  GSet V0, V1, V2;
+  auto fetch = [&](int64_t k0, float (&v)[16]) {
     const int64_t kb = k0 + sk;
     if (vec && kb + 16 <= kend) {
 #pragma unroll
@@ -1302,7 +1394,7 @@ __global__ __launch_bounds__(P_NT) void k_gram(int64_t M, int64_t K, int64_t kch
       for (int e = 0; e < 16; ++e) v[e] = scol[std::min<int64_t>(kb + e, K - 1)];
     }
   };
-  auto store = [&](int64_t k0, int buf) {
+  auto store = [&](int64_t k0, int buf, const float (&v)[16]) {
     const int64_t kb = k0 + sk;
     float c[16];
 #pragma unroll
@@ -1380,19 +1472,23 @@ __global__ __launch_bounds__(P_NT) void k_gram(int64_t M, int64_t K, int64_t kch
     }
   };
   const int nk = (int)((kend - kbeg + P_BK - 1) / P_BK);
-  if (nk > 0) {
-    fetch(kbeg);
-    store(kbeg, 0);
-  }
-  __syncthreads();
-  for (int c = 0; c < nk; ++c) {
-    const bool more = c + 1 < nk;
-    const int64_t kn = kbeg + (int64_t)(c + 1) * P_BK;
-    if (more) fetch(kn);
+  auto step = [&](int c, GSet& nxt, GSet& far) {
+    if (c + 2 < nk) fetch(kbeg + (int64_t)(c + 2) * P_BK, far.v);
     __builtin_amdgcn_sched_barrier(0);
     compute(c & 1);
-    if (more) store(kn, (c + 1) & 1);
+    if (c + 1 < nk) store(kbeg + (int64_t)(c + 1) * P_BK, (c + 1) & 1, nxt.v);
     __syncthreads();
+  };
+  if (nk > 0) {
+    fetch(kbeg, V0.v);
+    if (nk > 1) fetch(kbeg + P_BK, V1.v);
+    store(kbeg, 0, V0.v);
+  }
+  __syncthreads();
+  for (int c = 0; c < nk; c += 3) {
+    step(c, V1, V2);
+    if (c + 1 < nk) step(c + 1, V2, V0);
+    if (c + 2 < nk) step(c + 2, V0, V1);
   }
   float* P = partial + (int64_t)blockIdx.x * M * M;
 #pragma unroll
@@ -1437,6 +1533,12 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
            (const float*)part.get(), C, ldc);
     return;
   }
+  // more classes than one 128-column one-hot tile: class sums in one pass over
+  // A (the centred product, shiftB set, does not depend on the centring of A,
+  // and raw class sums take none)
+  if (labelsB && N + (shiftB ? 1 : 0) > 128 && (shiftB || !shiftA) &&
+      class_sums_f32(ctx, K, M, A, lda, labelsB, (int)N, shiftB, C, ldc, nullptr, nullptr))
+    return;
   if (labelsB && N + (shiftB ? 1 : 0) <= 128) {
     // class sums from the labels; the centring of the one-hot by shiftB uses
     // an extra all-ones column (the column sums of the centred A)
@@ -1452,10 +1554,10 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
     const int n32 = (int)N, ones = shiftB ? 1 : 0;
     if (precision == FPLS_PREC_BF16X3)
       launch(ctx, "gemm_tn_bf16x3", k_tns_lab<true>, grid, dim3(TL_NT), 0, M, n32, K, kchunk, A,
-             lda, shiftA, labelsB, ones, part.get());
+             lda, shiftA, labelsB, ones, part.get(), (double*)nullptr);
     else
       launch(ctx, "gemm_tn_f32", k_tns_lab<false>, grid, dim3(TL_NT), 0, M, n32, K, kchunk, A,
-             lda, shiftA, labelsB, ones, part.get());
+             lda, shiftA, labelsB, ones, part.get(), (double*)nullptr);
     launch(ctx, "gemm_tn_reduce", k_tl_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
            (const float*)part.get(), shiftB, C, ldc);
     return;
@@ -1486,6 +1588,35 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   }
   gemm_tn_dense(ctx, precision, M, N, K, A, lda, shiftA, B, ldb, labelsB, shiftB, C, ldc,
                 tn_default_splits(ctx, M, N, K));
+}
+
+bool class_cross_cov_f32(Ctx* ctx, int precision, int64_t p, int64_t C, int64_t n,
+                         const float* X, int64_t ld, const int32_t* y, const double* mY,
+                         double* S, int64_t lds, double* mean, double* ones) {
+  if (C + 1 > 128 || n <= 0 || p <= 0) return false;
+  // the shift: any value gives the same centred product; a sample of each
+  // column (its first row) keeps the FP32 partials of data far from zero
+  // accurate
+  DBuf<double> sh(ctx, (size_t)p);
+  const int64_t mt = (p + TL_BM - 1) / TL_BM, NC = C + 1;
+  int64_t splits = std::max<int64_t>(1, (2 * ctx->num_cus + mt - 1) / mt);
+  splits = std::min<int64_t>(splits, (n + 4 * TL_BK * 4 - 1) / (4 * TL_BK * 4));
+  int64_t kchunk = (n + splits - 1) / splits;
+  kchunk = (kchunk + TL_BK - 1) / TL_BK * TL_BK;
+  splits = (n + kchunk - 1) / kchunk;
+  DBuf<float> part(ctx, (size_t)splits * p * NC);
+  dim3 grid((unsigned)mt, (unsigned)splits);
+  if (precision == FPLS_PREC_BF16X3)
+    launch(ctx, "gemm_tn_bf16x3", k_tns_lab<true>, grid, dim3(TL_NT), 0, p, (int)C, n, kchunk, X,
+           ld, (const double*)nullptr, y, 1, part.get(), sh.get());
+  else
+    launch(ctx, "gemm_tn_f32", k_tns_lab<false>, grid, dim3(TL_NT), 0, p, (int)C, n, kchunk, X,
+           ld, (const double*)nullptr, y, 1, part.get(), sh.get());
+  const int64_t outs = p * C + p;
+  launch(ctx, "gemm_tn_reduce", k_tl_reduce_stats, dim3((unsigned)((outs + 255) / 256)),
+         dim3(256), 0, p, C, (int)splits, (const float*)part.get(), mY, S, lds,
+         (const double*)sh.get(), n, mean, ones);
+  return true;
 }
 
 int64_t tn_default_splits(const Ctx* ctx, int64_t M, int64_t N, int64_t K) {

@@ -185,7 +185,8 @@ void label_counts(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, int64_t* cou
 bool label_counts_prop(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, double* prop,
                        double* counts, int32_t* bad) {
   if (n <= 0 || C <= 0 || C > Ctx::kZeroCounts) return false;
-  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+  // about 2048 rows per block keeps the global atomics (C per block) few
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 2047) / 2048, 256));
   launch(ctx, "label_counts", k_label_counts_prop, dim3(blocks), dim3(256), C * sizeof(unsigned),
          y, n, C, ctx->zero_counts(), ctx->ticket(), prop, counts, bad);
   return true;

@@ -114,17 +114,21 @@ void project_scores(Ctx* ctx, const Model* M, const DMat& X, int k, float* Tf, d
 
 // mX, vX, mY (+ counts) for a fit; shared by PLSSVD and SIMPLS.
 void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int32_t* y,
-                    int32_t C, int scaling, bool guard_sd) {
+                    int32_t C, int scaling, bool guard_sd, bool means_later) {
   const int64_t n = X.rows, p = X.cols;
   M->mX = DBuf<double>(ctx, p);
   M->vX = DBuf<double>(ctx, p);
-  // means (and unit scales) in the column pass itself; sd = sqrt(ssd / (n-1))
-  if (scaling < 3) {
+  // means (and unit scales) in the column pass itself; sd = sqrt(ssd / (n-1));
+  // means_later: the one-pass class sums of cross_cov take mX (and vX = 1 or
+  // the sd pass after it)
+  if (means_later) {
+  } else if (scaling < 3) {
     col_stat(ctx, X, nullptr, M->mX.get(), (double)n, 0, scaling == 2 ? nullptr : M->vX.get());
   } else {
     fill_f64(ctx, M->mX.get(), 0.0, p);
   }
-  if (scaling == 2) {
+  if (means_later) {
+  } else if (scaling == 2) {
     col_stat(ctx, X, M->mX.get(), M->vX.get(), (double)(n > 1 ? n - 1 : 1), guard_sd ? 2 : 1);
   } else if (scaling >= 3) {
     fill_f64(ctx, M->vX.get(), 1.0, p);
@@ -149,10 +153,31 @@ void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int3
   }
 }
 
-// S = ((X - mX)/vX)^T (Y - mY)  (p x m, FP64); Y dense or labels.
+// Whether the PLSSVD fit of labels takes the column means from the one-pass
+// class sums (FP32 X, centring or autoscaling, at most 1663 classes).
+bool fit_means_from_class_sums(const DMat& X, const int32_t* y, int64_t C, int scaling) {
+  return y != nullptr && X.dtype == FPLS_F32 && scaling < 3 && C >= 1 && C <= 1663;
+}
+
+// S = ((X - mX)/vX)^T (Y - mY)  (p x m, FP64); Y dense or labels.  With
+// means_later (fit_means_from_class_sums) the class-sum pass also writes mX,
+// and vX (1, or the sd pass that follows it).
 void cross_cov(Ctx* ctx, const Model* M, const DMat& X, const DMat* Y, const int32_t* y, int64_t m,
-               double* S, int precision) {
+               double* S, int precision, bool means_later, bool guard_sd) {
   const int64_t n = X.rows, p = X.cols;
+  if (means_later) {
+    double* ones = M->scaling == 2 ? nullptr : M->vX.get();
+    FPLS_REQUIRE(class_cross_cov_f32(ctx, precision, p, m, n, X.f(), X.ld, y, M->mY.get(), S, p,
+                                     M->mX.get(), ones) ||
+                     class_sums_f32(ctx, n, p, X.f(), X.ld, y, (int)m, M->mY.get(), S, p,
+                                    M->mX.get(), ones),
+                 FPLS_ERR_ARG, "cross_cov: too many classes for the one-pass class sums");
+    if (M->scaling == 2) {
+      col_stat(ctx, X, M->mX.get(), M->vX.get(), (double)(n > 1 ? n - 1 : 1), guard_sd ? 2 : 1);
+      scale_rows_div(ctx, S, p, p, m, M->vX.get());
+    }
+    return;
+  }
   if (X.dtype == FPLS_F32) {
     const float* Bp = nullptr;
     int64_t ldb = 0;
@@ -370,10 +395,12 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     for (int& v : nc) v = std::min(std::max(v, 1), max_rank);
     int kmax = *std::max_element(nc.begin(), nc.end());
     const int L = (int)nc.size();
-    fit_preprocess(c, M, X, y ? nullptr : &Y, y, (int32_t)m, o.scaling, M->label_aware);
+    const bool one_pass = fit_means_from_class_sums(X, y, m, o.scaling);
+    fit_preprocess(c, M, X, y ? nullptr : &Y, y, (int32_t)m, o.scaling, M->label_aware,
+                   one_pass);
     if (y) M->n_classes = (int)m;
     DBuf<double> S(c, (size_t)p * m);
-    cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision);
+    cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision, one_pass, M->label_aware);
     // truncated SVD of S (src/fastPLS.cpp:4843-4856, plssvd_use_small_exact_svd :318-324)
     SvdOut sv;
     sv.want_v = true;

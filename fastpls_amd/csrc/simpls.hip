@@ -20,9 +20,9 @@
 namespace fpls {
 
 void fit_preprocess(Ctx* ctx, Model* M, const DMat& X, const DMat* Y, const int32_t* y, int32_t C,
-                    int scaling, bool guard_sd);
+                    int scaling, bool guard_sd, bool means_later = false);
 void cross_cov(Ctx* ctx, const Model* M, const DMat& X, const DMat* Y, const int32_t* y, int64_t m,
-               double* S, int precision);
+               double* S, int precision, bool means_later = false, bool guard_sd = false);
 bool store_b_policy(int store_b, int64_t p, int64_t m, int n_slices);
 DBuf<double> centered_response(Ctx* ctx, const Model* M, const DMat* Y, const int32_t* y,
                                int64_t n, int64_t m);
