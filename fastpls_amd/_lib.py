@@ -77,6 +77,24 @@ class ModelFields(ctypes.Structure):
                 ("mY", ctypes.c_void_p)]
 
 
+class LdaFields(ctypes.Structure):
+    _fields_ = [("n_classes", ctypes.c_int32), ("n_models", ctypes.c_int32),
+                ("ncomp", ctypes.c_void_p), ("means", ctypes.c_void_p),
+                ("inv_cov", ctypes.c_void_p), ("linear", ctypes.c_void_p),
+                ("constants", ctypes.c_void_p), ("priors", ctypes.c_void_p),
+                ("ridge", ctypes.c_void_p)]
+
+
+class CknnFields(ctypes.Structure):
+    _fields_ = [("n_classes", ctypes.c_int32), ("n_models", ctypes.c_int32),
+                ("ncomp", ctypes.c_void_p), ("centroids", ctypes.c_void_p),
+                ("n_train", ctypes.c_int64), ("ttrain_cols", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("Ttrain", ctypes.c_void_p),
+                ("y_codes", ctypes.c_void_p), ("knn_k", ctypes.c_int32),
+                ("top_m", ctypes.c_int32), ("tau", ctypes.c_double),
+                ("alpha", ctypes.c_double)]
+
+
 _P = ctypes.POINTER
 _vp = ctypes.c_void_p
 SIGNATURES = {
@@ -99,6 +117,8 @@ SIGNATURES = {
                                                    _P(Mat), _vp, _P(_vp)]),
     "fpls_model_info_get": (ctypes.c_int, [_vp, _P(ModelInfo)]),
     "fpls_model_import": (ctypes.c_int, [_vp, _P(ModelFields), _P(_vp)]),
+    "fpls_model_import_lda": (ctypes.c_int, [_vp, _vp, _P(LdaFields)]),
+    "fpls_model_import_cknn": (ctypes.c_int, [_vp, _vp, _P(CknnFields)]),
     "fpls_model_read": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _vp, ctypes.c_int64]),
     "fpls_model_free": (None, [_vp]),
     "fpls_predict": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp]),

@@ -771,7 +771,120 @@ def model_from_fields(fields: Dict, device: int = 0) -> FastPLS:
     inf = model.info()
     model["B_stored"] = bool(inf.b_stored)
     model["compact_prediction"] = not bool(inf.b_stored)
+    _import_heads(model, fields)
     return model
+
+
+def _head_models(sub: Dict, ks) -> list:
+    """The per-k entries of a classifier sub-list's `models`, named by k (the
+    reference's named list) or given in the order of `ncomp`."""
+    models = sub["models"]
+    if isinstance(models, dict):
+        out = []
+        for k in ks:
+            e = models.get(k, models.get(str(k)))
+            if e is None:
+                raise ValueError(f"No fitted classifier for ncomp={k}")
+            out.append(e)
+        return out
+    models = list(models)
+    if len(models) != len(ks):
+        raise ValueError("classifier models must have one entry per ncomp")
+    return models
+
+
+def _import_heads(model: FastPLS, fields: Dict) -> None:
+    """The `lda` (R/main.R:1244-1250) and `candidate_knn` (R/main.R:861-872)
+    sub-lists of a reference model onto the device model
+    (fpls_model_import_lda / fpls_model_import_cknn); the classification rule
+    follows the model's own, else the head that is present (LDA first)."""
+    ctx = model.ctx
+    lev = fields.get("lev")
+    keep: list = []
+
+    def f64(v, shape=None):
+        a = np.asfortranarray(np.asarray(v, dtype=np.float64))
+        if shape is not None:
+            a = np.asfortranarray(a.reshape(shape, order="F"))
+        keep.append(a)
+        return a
+
+    def packed(arrs):
+        a = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64)
+                                                 .ravel(order="F") for x in arrs]))
+        keep.append(a)
+        return a.ctypes.data
+    rule = fields.get("classification_rule")
+    heads: list = []
+    lda = fields.get("lda")
+    if lda is not None and lda.get("models") is not None:
+        ks = [int(k) for k in np.atleast_1d(lda["ncomp"])]
+        ms = _head_models(lda, ks)
+        lin = [f64(e["linear"]) for e in ms]
+        C = len(lev) if lev is not None else lin[0].shape[0]
+        for e, k, li in zip(ms, ks, lin):
+            if li.shape != (C, k) or np.size(e["constants"]) != C:
+                raise ValueError(f"LDA model for ncomp={k} has inconsistent linear / constants")
+        lf = _lib.LdaFields()
+        lf.n_classes, lf.n_models = C, len(ks)
+        kv, lf.ncomp = int_array(ks)
+        keep.append(kv)
+        lf.linear = packed(lin)
+        lf.constants = packed([e["constants"] for e in ms])
+        if all(e.get("means") is not None for e in ms):
+            lf.means = packed([f64(e["means"], (C, k)) for e, k in zip(ms, ks)])
+        if all(e.get("inv_cov") is not None for e in ms):
+            lf.inv_cov = packed([f64(e["inv_cov"], (k, k)) for e, k in zip(ms, ks)])
+        if ms[0].get("priors") is not None:
+            lf.priors = packed([ms[0]["priors"]])
+        if all(e.get("ridge") is not None for e in ms):
+            lf.ridge = packed([[float(np.atleast_1d(e["ridge"])[0])] for e in ms])
+        ctx.check(ctx.lib.fpls_model_import_lda(ctx.h, model.handle, ctypes.byref(lf)))
+        model["lda"] = {"ncomp": ks, "ridge": lda.get("ridge")}
+        heads.append("lda_hip")
+    ck = fields.get("candidate_knn")
+    if ck is not None and ck.get("models") is not None:
+        ks = [int(k) for k in np.atleast_1d(ck["ncomp"])]
+        ms = _head_models(ck, ks)
+        cen = [f64(e["centroids"]) for e in ms]
+        C = len(lev) if lev is not None else cen[0].shape[0]
+        Tt = f64(ck["Ttrain"])
+        if Tt.ndim != 2:
+            raise ValueError("candidate_knn Ttrain must be a matrix")
+        yc = np.ascontiguousarray(np.asarray(ck["y_codes"]).ravel(), dtype=np.int32)
+        keep.append(yc)
+        if yc.size != Tt.shape[0]:
+            raise ValueError("candidate_knn y_codes must hold one label per Ttrain row")
+        par = ck.get("parameters", {})
+        cf = _lib.CknnFields()
+        cf.n_classes, cf.n_models = C, len(ks)
+        kv, cf.ncomp = int_array(ks)
+        keep.append(kv)
+        for c_, k in zip(cen, ks):
+            if c_.shape != (C, k):
+                raise ValueError(f"candidate-kNN centroids for ncomp={k} must be C x {k}")
+        cf.centroids = packed(cen)
+        cf.n_train, cf.ttrain_cols = Tt.shape
+        cf.Ttrain = Tt.ctypes.data
+        cf.y_codes = yc.ctypes.data
+        cf.knn_k = int(np.atleast_1d(par.get("knn_k", 3))[0])
+        cf.top_m = int(np.atleast_1d(par.get("top_m", 20))[0])
+        cf.tau = float(np.atleast_1d(par.get("tau", 0.2))[0])
+        cf.alpha = float(np.atleast_1d(par.get("alpha", 0.5))[0])
+        ctx.check(ctx.lib.fpls_model_import_cknn(ctx.h, model.handle, ctypes.byref(cf)))
+        heads.append("candidate_knn_hip")
+    if rule is not None and str(rule).startswith("lda") and "lda_hip" in heads:
+        rule = "lda_hip"
+    elif rule is not None and "candidate_knn" in str(rule) and "candidate_knn_hip" in heads:
+        rule = "candidate_knn_hip"
+    elif rule is None and heads:
+        rule = heads[0]
+    if rule in ("lda_hip", "candidate_knn_hip"):
+        model["classification"] = True
+        model["classification_rule"] = rule
+        if model.get("lev") is None:
+            C = model.info().n_classes
+            model["lev"] = list(range(1, C + 1))
 
 
 def arr_from(a: np.ndarray, shape, keep: list) -> np.ndarray:

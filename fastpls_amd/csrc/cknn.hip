@@ -672,6 +672,52 @@ void score_space(Ctx* ctx, const Model* M, const DMat& T, int k, float* out) {
               nullptr, k, true);
 }
 
+// The candidate-kNN head for every k of ks on training scores T (rows x >=
+// max(ks)) with checked labels y (device, 1..C): the class-sorted,
+// row-normalised score space per k (R/main.R:845-848, 911-914) and the class
+// centroids -- computed from it (R/main.R:686-698), or, when cen_host is set,
+// taken from it (C x k column-major per k, packed in the order of ks).
+static void cknn_attach(Ctx* c, Model* M, const DMat& T, const int32_t* y, int C,
+                        const std::vector<int>& ks, const double* cen_host) {
+  const int64_t n = T.rows;
+  CkClassIndex ci;
+  stable_class_sort(c, y, n, C, ci);
+  std::vector<CknnSlice> slices;
+  size_t oc = 0;
+  for (int k : ks) {
+    CknnSlice sl;
+    sl.k = k;
+    DBuf<float> S(c, (size_t)n * k);
+    score_space(c, M, T, k, S.get());
+    sl.Tn = DBuf<float>(c, (size_t)n * k);
+    rownorm_rows(c, S.get(), n, k, ci.d_rows.get(), sl.Tn.get());
+    if (cen_host) {
+      // column-major C x k -> the row-major layout of the device head
+      std::vector<double> rm((size_t)C * k);
+      for (int j = 0; j < k; ++j)
+        for (int r = 0; r < C; ++r) rm[(size_t)r * k + j] = cen_host[oc + (size_t)j * C + r];
+      sl.centroids = DBuf<double>(c, rm.size());
+      to_device(c, sl.centroids.get(), rm.data(), rm.size());
+      sl.centroids_f = DBuf<float>(c, rm.size());
+      f64_to_f32(c, sl.centroids_f.get(), sl.centroids.get(), (int64_t)rm.size());
+      oc += rm.size();
+    } else {
+      CkClassIndex cix;
+      cix.off = ci.off;
+      cix.d_off = DBuf<int>(c, C + 1);
+      h2d(c, cix.d_off.get(), ci.off.data(), (C + 1) * sizeof(int));
+      class_centroids(c, sl.Tn.get(), k, cix, C, sl.centroids, sl.centroids_f);
+    }
+    slices.push_back(std::move(sl));
+  }
+  sync(c);
+  M->cknn = std::move(slices);
+  M->class_offsets_host.assign(ci.off.begin(), ci.off.end());
+  M->class_offsets = std::move(ci.d_off);
+  M->class_rows = std::move(ci.d_rows);
+  M->n_classes = C;
+}
+
 }  // namespace fpls
 
 using namespace fpls;
@@ -721,28 +767,9 @@ extern "C" int fpls_cknn_fit(fpls_ctx* ctx, fpls_model* model, const fpls_mat* X
       FPLS_REQUIRE(M->has_T(), FPLS_ERR_STATE, "model holds no training scores; pass Xtrain");
       T = M->T();
     }
-    CkClassIndex ci;
-    stable_class_sort(c, y, M->n, n_classes, ci);
-    M->cknn.clear();
-    M->class_offsets_host.assign(ci.off.begin(), ci.off.end());
-    M->class_offsets = std::move(ci.d_off);
-    M->class_rows = std::move(ci.d_rows);
-    M->n_classes = n_classes;
-    for (int k : ks) {
-      CknnSlice sl;
-      sl.k = k;
-      DBuf<float> S(c, (size_t)M->n * k);
-      score_space(c, M, T, k, S.get());
-      sl.Tn = DBuf<float>(c, (size_t)M->n * k);
-      launch(c, "cknn_rownorm", k_rownorm_permute, dim3((unsigned)((M->n + 3) / 4)), dim3(256), 0,
-             (const float*)S.get(), M->n, k, (const int*)M->class_rows.get(), sl.Tn.get());
-      CkClassIndex cix;
-      cix.off = ci.off;
-      cix.d_off = DBuf<int>(c, n_classes + 1);
-      h2d(c, cix.d_off.get(), ci.off.data(), (n_classes + 1) * sizeof(int));
-      class_centroids(c, sl.Tn.get(), k, cix, n_classes, sl.centroids, sl.centroids_f);
-      M->cknn.push_back(std::move(sl));
-    }
+    FPLS_REQUIRE(M->lda.empty() || M->n_classes == n_classes, FPLS_ERR_ARG,
+                 "the LDA and candidate-kNN heads disagree on the number of classes");
+    cknn_attach(c, M, T, y, n_classes, std::vector<int>(ks.begin(), ks.end()), nullptr);
     M->knn_k = std::max(1, (int)knn_k);
     M->tau = tau;
     M->alpha = alpha;
@@ -859,5 +886,67 @@ extern "C" int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_m,
                       bias.get(), topc, top_m, knn_k, tau, alpha, ti.get(), ts.get(), Te.rows);
     to_host_i32(c, top_index, ti.get(), (size_t)Te.rows * topc);
     if (top_score) to_host(c, top_score, ts.get(), (size_t)Te.rows * topc);
+  });
+}
+
+// The reference's model$candidate_knn list (R/main.R:861-872) as the device
+// candidate-kNN head: Ttrain, y_codes, the per-model centroids and the
+// parameters.  The score spaces are formed from Ttrain with the device model's
+// C_latent, as .candidate_knn_predictions does at every call (R/main.R:903-918).
+extern "C" int fpls_model_import_cknn(fpls_ctx* ctx, fpls_model* model,
+                                      const fpls_cknn_fields* f) {
+  if (!model || !f) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    Model* M = &model->m;
+    const int C = f->n_classes, L = f->n_models;
+    const int64_t n = f->n_train;
+    FPLS_REQUIRE(C >= 1, FPLS_ERR_ARG, "candidate kNN requires at least one class");
+    FPLS_REQUIRE(L >= 1 && f->ncomp, FPLS_ERR_ARG,
+                 "candidate-kNN import requires at least one model");
+    FPLS_REQUIRE(f->centroids && f->Ttrain && f->y_codes && n >= 1, FPLS_ERR_ARG,
+                 "candidate-kNN import requires centroids, Ttrain and y_codes");
+    FPLS_REQUIRE(std::isfinite(f->tau) && f->tau > 0.0, FPLS_ERR_ARG,
+                 "candidate_tau must be positive");
+    FPLS_REQUIRE(std::isfinite(f->alpha), FPLS_ERR_ARG, "candidate_alpha must be finite");
+    FPLS_REQUIRE(M->lda.empty() || M->n_classes == C, FPLS_ERR_ARG,
+                 "the LDA and candidate-kNN heads disagree on the number of classes");
+    std::vector<int> ks(f->ncomp, f->ncomp + L);
+    for (int a = 0; a < L; ++a) {
+      FPLS_REQUIRE(ks[a] >= 1 && ks[a] <= M->kmax, FPLS_ERR_ARG,
+                   "candidate-kNN score dimension does not match the model");
+      FPLS_REQUIRE(a == 0 || ks[a] > ks[a - 1], FPLS_ERR_ARG,
+                   "candidate-kNN ncomp must be strictly ascending");
+    }
+    const int kmax = ks.back();
+    FPLS_REQUIRE(f->ttrain_cols >= kmax, FPLS_ERR_ARG,
+                 "candidate-kNN Ttrain has fewer columns than max(ncomp)");
+    // labels: the range check of .fit_candidate_knn (R/main.R:840-843)
+    DBuf<int32_t> y(c, n);
+    FPLS_HIP(hipMemcpyAsync(y.get(), f->y_codes, sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                            c->stream));
+    DBuf<int64_t> cnt(c, C);
+    DBuf<int32_t> bad(c, 1);
+    label_counts(c, y.get(), n, C, cnt.get(), bad.get());
+    int32_t hbad = 0;
+    to_host_i32(c, &hbad, bad.get(), 1);
+    FPLS_REQUIRE(hbad == 0, FPLS_ERR_LABEL_RANGE,
+                 "candidate-kNN received labels outside the training levels");
+    // Ttrain[:, :kmax] in FP32, the precision of the device head's scores
+    DBuf<double> T64(c, (size_t)n * kmax);
+    to_device(c, T64.get(), f->Ttrain, (size_t)n * kmax);
+    DBuf<float> T32(c, (size_t)n * kmax);
+    f64_to_f32(c, T32.get(), T64.get(), n * (int64_t)kmax);
+    DMat T;
+    T.p = T32.get();
+    T.dtype = FPLS_F32;
+    T.rows = n;
+    T.cols = kmax;
+    T.ld = n;
+    cknn_attach(c, M, T, y.get(), C, ks, f->centroids);
+    M->knn_k = std::max(1, (int)f->knn_k);
+    M->tau = f->tau;
+    M->alpha = f->alpha;
+    M->top_m = std::max(1, (int)f->top_m);
   });
 }

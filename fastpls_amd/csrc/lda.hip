@@ -427,6 +427,7 @@ extern "C" int fpls_lda_train_prefix(fpls_ctx* ctx, fpls_model* model, const fpl
       }
     }
     M->n_classes = n_classes;
+    M->priors.reset();
     lda_train_core(c, T, G, ldg, y, n_classes, ks, ridge, M->precision, M->counts, M->lda);
   });
 }
@@ -489,16 +490,20 @@ extern "C" int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t sli
       case FPLS_LDA_CONSTANTS: src = s.constants.get(); need = C; break;
       case FPLS_LDA_PRIORS: {
         FPLS_REQUIRE(host_out && n_elem >= C, FPLS_ERR_ARG, "output buffer too small");
+        if (M.priors.get()) {
+          to_host(c, host_out, M.priors.get(), C);
+          return;
+        }
+        FPLS_REQUIRE(M.counts.get() && M.n > 0, FPLS_ERR_STATE,
+                     "this LDA head holds no priors");
         to_host(c, host_out, M.counts.get(), C);
         for (int64_t i = 0; i < C; ++i) host_out[i] /= (double)M.n;
         return;
       }
-      case FPLS_LDA_RIDGE:
-        FPLS_REQUIRE(host_out && n_elem >= 1, FPLS_ERR_ARG, "output buffer too small");
-        to_host(c, host_out, s.ridge_d.get(), 1);
-        return;
+      case FPLS_LDA_RIDGE: src = s.ridge_d.get(); need = 1; break;
       default: fail(FPLS_ERR_ARG, "unknown LDA field");
     }
+    FPLS_REQUIRE(src != nullptr, FPLS_ERR_STATE, "this LDA head holds no such field");
     FPLS_REQUIRE(host_out && n_elem >= need, FPLS_ERR_ARG, "output buffer too small");
     to_host(c, host_out, src, need);
   });
@@ -595,6 +600,61 @@ extern "C" int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* Xm,
   });
   if (rc == FPLS_OK) *out = mdl.release();
   return rc;
+}
+
+// The reference's model$lda list (R/main.R:1244-1250) as the device LDA head:
+// each model's linear / constants (what lda_project_predict_labels_cpp reads,
+// src/fastPLS.cpp:2100-2147) and the optional means, inv_cov, priors and
+// applied ridge, plus the padded FP32 linear^T of the latent score product.
+extern "C" int fpls_model_import_lda(fpls_ctx* ctx, fpls_model* model,
+                                     const fpls_lda_fields* f) {
+  if (!model || !f) return FPLS_ERR_ARG;
+  return fpls_guard(ctx, [&] {
+    Ctx* c = &ctx->c;
+    Model* M = &model->m;
+    const int C = f->n_classes, L = f->n_models;
+    FPLS_REQUIRE(C >= 1, FPLS_ERR_ARG, "LDA import requires at least one class");
+    FPLS_REQUIRE(L >= 1 && f->ncomp, FPLS_ERR_ARG, "LDA import requires at least one model");
+    FPLS_REQUIRE(f->linear && f->constants, FPLS_ERR_ARG,
+                 "LDA import requires linear and constants");
+    FPLS_REQUIRE(M->cknn.empty() || M->n_classes == C, FPLS_ERR_ARG,
+                 "the LDA and candidate-kNN heads disagree on the number of classes");
+    for (int a = 0; a < L; ++a) {
+      FPLS_REQUIRE(f->ncomp[a] >= 1 && f->ncomp[a] <= M->kmax, FPLS_ERR_ARG,
+                   "LDA score dimension does not match the model");
+      FPLS_REQUIRE(a == 0 || f->ncomp[a] > f->ncomp[a - 1], FPLS_ERR_ARG,
+                   "LDA ncomp must be strictly ascending");
+    }
+    std::vector<LdaSlice> slices(L);
+    size_t om = 0, oi = 0;
+    for (int a = 0; a < L; ++a) {
+      LdaSlice& sl = slices[a];
+      const int k = f->ncomp[a];
+      sl.k = k;
+      auto up = [&](DBuf<double>& dst, const double* src, size_t count) {
+        dst = DBuf<double>(c, count);
+        to_device(c, dst.get(), src, count);
+      };
+      up(sl.linear, f->linear + om, (size_t)C * k);
+      up(sl.constants, f->constants + (size_t)a * C, (size_t)C);
+      if (f->means) up(sl.means, f->means + om, (size_t)C * k);
+      if (f->inv_cov) up(sl.inv_cov, f->inv_cov + oi, (size_t)k * k);
+      if (f->ridge) up(sl.ridge_d, f->ridge + a, 1);
+      sl.linT_pad = nn_pad_operand(c, sl.linear.get(), C, true, k, C);
+      om += (size_t)C * k;
+      oi += (size_t)k * k;
+    }
+    DBuf<double> pri;
+    if (f->priors) {
+      pri = DBuf<double>(c, C);
+      to_device(c, pri.get(), f->priors, C);
+    }
+    sync(c);
+    M->lda = std::move(slices);
+    M->priors = std::move(pri);
+    M->counts.reset();
+    M->n_classes = C;
+  });
 }
 
 extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,

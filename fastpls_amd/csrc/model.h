@@ -57,6 +57,7 @@ struct Model {
   // LDA head
   int n_classes = 0;
   DBuf<double> counts;          // C (as double, like the reference)
+  DBuf<double> priors;          // C, held only by an imported head (else counts / n)
   std::vector<LdaSlice> lda;
   // candidate-kNN head
   std::vector<CknnSlice> cknn;

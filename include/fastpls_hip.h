@@ -28,6 +28,9 @@
  *   fpls_model_import         those fields back into a device model (predict()
  *                             on a model fitted by the reference; the B_stored
  *                             policy src/fastPLS.cpp:63-90)
+ *   fpls_model_import_lda     the model$lda list back onto a device model
+ *                             (R/main.R:1244-1250; per model src/fastPLS.cpp:1914-1921)
+ *   fpls_model_import_cknn    the model$candidate_knn list likewise (R/main.R:861-872)
  *   fpls_predict              pls_predict                src/fastPLS.cpp:3211-3445
  *   fpls_project              .fastpls_latent_scores     R/main.R:965-1017
  *   fpls_lda_train_prefix     lda_project_train_prefix_cpp src/fastPLS.cpp:1930-1957
@@ -250,6 +253,51 @@ typedef struct fpls_model_fields {
 int fpls_model_import(fpls_ctx* ctx, const fpls_model_fields* fields, fpls_model** out);
 int fpls_model_read(fpls_ctx* ctx, const fpls_model* model, int32_t field, double* host_out,
                     int64_t n_elem);
+
+/* The reference's model$lda list (R/main.R:1244-1250): ncomp, and per entry of
+   models the fields of lda_train_prefix_cpp (src/fastPLS.cpp:1914-1921).  Host
+   FP64, column-major, packed one model after another in the order of ncomp --
+   the layout fpls_lda_train_scores writes. */
+typedef struct fpls_lda_fields {
+  int32_t n_classes;        /* C = length(lev)                                   */
+  int32_t n_models;         /* length(lda$ncomp)                                 */
+  const int32_t* ncomp;     /* lda$ncomp: strictly ascending, each in 1..kmax    */
+  const double* means;      /* C x k per model, or NULL                          */
+  const double* inv_cov;    /* k x k per model, or NULL                          */
+  const double* linear;     /* C x k per model (required)                        */
+  const double* constants;  /* C per model (required)                            */
+  const double* priors;     /* C, or NULL                                        */
+  const double* ridge;      /* n_models applied ridges, or NULL                  */
+} fpls_lda_fields;
+/* Attach that LDA head to a device model (one from fpls_model_import or a
+   fit), replacing any LDA head it holds: fpls_lda_predict then scores with
+   the given linear / constants exactly as lda_project_predict_labels_cpp
+   (src/fastPLS.cpp:2100-2147).  fpls_lda_read returns the imported fields;
+   one that was passed as NULL reads as FPLS_ERR_STATE. */
+int fpls_model_import_lda(fpls_ctx* ctx, fpls_model* model, const fpls_lda_fields* fields);
+
+/* The reference's model$candidate_knn list (R/main.R:861-872). */
+typedef struct fpls_cknn_fields {
+  int32_t n_classes;        /* C = length(lev)                                   */
+  int32_t n_models;         /* length(candidate_knn$ncomp)                       */
+  const int32_t* ncomp;     /* candidate_knn$ncomp: strictly ascending, 1..kmax  */
+  const double* centroids;  /* C x k per model (models[[k]]$centroids), packed   */
+  int64_t n_train;          /* nrow(Ttrain)                                      */
+  int32_t ttrain_cols;      /* ncol(Ttrain) >= max(ncomp)                        */
+  int32_t reserved;
+  const double* Ttrain;     /* n_train x ttrain_cols training scores             */
+  const int32_t* y_codes;   /* n_train training labels, 1..C                     */
+  int32_t knn_k, top_m;     /* parameters$knn_k, parameters$top_m                */
+  double tau, alpha;        /* parameters$tau, parameters$alpha                  */
+} fpls_cknn_fields;
+/* Attach that candidate-kNN head to a device model, replacing any it holds:
+   the training scores are mapped into each model's score space with the
+   device model's C_latent and row-normalised as at prediction time
+   (R/main.R:903-918), the centroids are used as given, and
+   fpls_cknn_predict then ranks as .candidate_knn_predictions.  Errors as
+   .fit_candidate_knn (R/main.R:813-815, 840-843); labels outside 1..C are
+   rejected with FPLS_ERR_LABEL_RANGE. */
+int fpls_model_import_cknn(fpls_ctx* ctx, fpls_model* model, const fpls_cknn_fields* fields);
 void fpls_model_free(fpls_model* model);
 
 /* ---- prediction ------------------------------------------------------------ */

@@ -36,3 +36,16 @@ def label_mismatches_excused(labels, ref_labels, scores_ref, tol):
     margin = s[:, -1] - s[:, -2]
     unexcused = [int(i) for i in bad if margin[i] > tol]
     return bad, unexcused
+
+
+def topk_mismatches_excused(ti, ri, rs, tol):
+    """Rows whose ranked class lists differ from the reference's, excused only
+    where the reference's own ranked scores hold a gap below `tol` (a
+    documented near-tie); returns the unexcused rows."""
+    bad = []
+    for i in range(ri.shape[0]):
+        if not np.array_equal(ti[i], ri[i]):
+            gaps = np.abs(np.diff(rs[i]))
+            if not (gaps.size and np.min(gaps) < tol):
+                bad.append(i)
+    return bad

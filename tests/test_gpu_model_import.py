@@ -3,7 +3,10 @@ B_stored policy (src/fastPLS.cpp:63-90, 3737-3836).  A model fitted by the FP64
 oracle (the reference's algorithm on the CPU) is imported and predicted on the
 device: predictions agree with the oracle's pls_predict to FP64 rounding, and
 ranked classes with its pls_class_predict_topk.  A device fit exported through
-its fields and imported again predicts bit-identically."""
+its fields and imported again predicts bit-identically.  The classifier
+sub-lists (lda, R/main.R:1244-1250; candidate_knn, R/main.R:861-872) of an
+oracle-fitted model are imported onto it and predict with the oracle's labels,
+barring near-ties of the oracle's own scores."""
 import os
 
 import numpy as np
@@ -122,3 +125,140 @@ def test_b_storage_policy_environment(ctx):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
+
+
+# ---- classifier heads (R/main.R:1244-1250, 861-872) --------------------------
+
+def _class_data(seed, n=360, p=28, C=6, nt=90):
+    rng = np.random.default_rng(seed)
+    mu = 1.5 * rng.standard_normal((C, p))
+    y = rng.integers(1, C + 1, n)
+    y[:C] = np.arange(1, C + 1)
+    X = mu[y - 1] + rng.standard_normal((n, p))
+    yt = rng.integers(1, C + 1, nt)
+    Xt = mu[yt - 1] + rng.standard_normal((nt, p))
+    return X, y, Xt
+
+
+def _oracle_plssvd_lda(X, y, C, ks):
+    """A PLSSVD model with its lda list, fitted by the oracle (the reference's
+    CPU path: pls_model1 + .attach_lda_classifier's lda_train_prefix_cpp)."""
+    mod = O.pls_model1(X, O.transformy(y), ks, 1, False, "rsvd", 10, 1, 9)
+    mod["pls_method"] = "plssvd"
+    O.attach_latent_projection_cache(mod)
+    T = O.latent_scores(mod, X, max(ks))
+    models = O.lda_train_prefix(T, y, C, ks, 1e-8)
+    mod["lda"] = {"ncomp": ks, "models": {m_["ncomp"]: m_ for m_ in models}, "ridge": 1e-8}
+    mod["classification_rule"] = "lda_cpp"
+    mod["lev"] = [f"c{i}" for i in range(1, C + 1)]
+    return mod
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 1e-9), (np.float32, 2e-3)])
+def test_import_oracle_lda_head_predicts_like_the_oracle(ctx, dtype, tol):
+    """lda_project_predict_labels_cpp on the imported head: labels equal the
+    oracle's except where its top-2 discriminant margin is below tol (FP64
+    input: FP64 products end to end; FP32 input: the FP32 MFMA projection)."""
+    import fastpls_amd as F
+    from tests.helpers import label_mismatches_excused
+    C, ks = 6, [2, 4, 5]
+    X, y, Xt = _class_data(11, C=C)
+    mod = _oracle_plssvd_lda(X, y, C, ks)
+    dm = F.model_from_fields(mod)
+    assert dm["classification_rule"] == "lda_hip"
+    res = F.predict(dm, Xt.astype(dtype), raw_scores=True)
+    for a, k in enumerate(ks):
+        lda = mod["lda"]["models"][k]
+        lab, sc = O.lda_project_predict_labels(Xt, mod["R_predict"][:, :k], mod["R_offset"],
+                                               lda, return_scores=True)
+        bad, unexcused = label_mismatches_excused(res["Ypred_index"][:, a], lab, sc, tol)
+        assert not unexcused, f"ncomp={k}: non-tie label mismatches at rows {unexcused[:10]}"
+        assert len(bad) <= 2
+        np.testing.assert_allclose(res["LDA_scores"][:, :, a], sc, rtol=0,
+                                   atol=tol * max(1.0, np.abs(sc).max()))
+        assert list(res["Ypred"][:, a]) == [mod["lev"][i - 1] for i in res["Ypred_index"][:, a]]
+    # the imported fields read back unchanged
+    for a, k in enumerate(ks):
+        lda = mod["lda"]["models"][k]
+        np.testing.assert_array_equal(dm.lda_read(a, "linear", (C, k)), lda["linear"])
+        np.testing.assert_array_equal(dm.lda_read(a, "constants", (C,)), lda["constants"])
+        np.testing.assert_array_equal(dm.lda_read(a, "inv_cov", (k, k)), lda["inv_cov"])
+        np.testing.assert_array_equal(dm.lda_read(a, "means", (C, k)), lda["means"])
+        np.testing.assert_array_equal(dm.lda_read(a, "priors", (C,)), lda["priors"])
+
+
+def test_device_lda_head_round_trips_through_import(ctx):
+    """A device-fitted LDA head exported with fpls_lda_read and imported onto
+    the imported model predicts labels and scores bit-identically (FP64)."""
+    import fastpls_amd as F
+    C = 5
+    X, y, Xt = _class_data(12, C=C)
+    m = F.pls(X, y, ncomp=[2, 3], method="plssvd", classifier="lda", seed=3)
+    ks = m["lda"]["ncomp"]
+    models = {k: {"linear": m.lda_read(a, "linear", (C, k)),
+                  "constants": m.lda_read(a, "constants", (C,))} for a, k in enumerate(ks)}
+    fields = dict(m)
+    fields["lda"] = {"ncomp": ks, "models": models}
+    fields["classification_rule"] = "lda_cpp"
+    dm = F.model_from_fields(fields)
+    a = F.predict(m, Xt, raw_scores=True)
+    b = F.predict(dm, Xt, raw_scores=True)
+    np.testing.assert_array_equal(a["Ypred_index"], b["Ypred_index"])
+    np.testing.assert_array_equal(a["LDA_scores"], b["LDA_scores"])
+    with pytest.raises(F.FplsError, match="no such field"):
+        dm.lda_read(0, "means", (C, ks[0]))
+
+
+@pytest.mark.parametrize("method", ["plssvd", "simpls"])
+def test_import_oracle_cknn_head_predicts_like_the_oracle(ctx, method):
+    """.candidate_knn_predictions on the imported candidate_knn list: the
+    ranked classes equal the oracle's except at near-ties of its scores
+    (1e-5: FP32 similarities), scores to 1e-5.  PLSSVD maps the scores
+    through C_latent (the .candidate_score_space slice rule)."""
+    import fastpls_amd as F
+    from tests.helpers import topk_mismatches_excused
+    C, ks = 6, [2, 3, 5]
+    X, y, Xt = _class_data(13, C=C)
+    if method == "plssvd":
+        mod = O.pls_model1(X, O.transformy(y), ks, 1, False, "rsvd", 10, 1, 9)
+    else:
+        mod = O.pls_model2_fast(X, O.transformy(y), ks, 1, False, "rsvd", 10, 1, 9)
+    mod["pls_method"] = method
+    mod["lev"] = list(range(1, C + 1))
+    O.fit_candidate_knn(mod, X, y, knn_k=4, tau=0.2, alpha=0.75, top_m=5)
+    mod["classification_rule"] = "candidate_knn_cpp"
+    dm = F.model_from_fields(mod)
+    assert dm["classification_rule"] == "candidate_knn_hip"
+    top = 3
+    res = F.predict(dm, Xt, top=top)
+    ref = O.candidate_knn_predictions(mod, Xt, top=top)
+    for a in range(len(ks)):
+        ri, rs = ref[a]["top_index"], ref[a]["top_score"]
+        ti = res["Ypred_top"][:, :, a]
+        bad = topk_mismatches_excused(ti, ri, rs, 1e-5)
+        assert not bad, f"ncomp={ks[a]}: non-tie top-k mismatches at rows {bad[:10]}"
+        np.testing.assert_allclose(res["Ypred_top_score"][:, :, a], rs, rtol=0, atol=1e-5)
+
+
+def test_import_heads_errors(ctx):
+    import fastpls_amd as F
+    C, ks = 4, [2, 3]
+    X, y, _ = _class_data(14, C=C)
+    mod = _oracle_plssvd_lda(X, y, C, ks)
+    bad = dict(mod, lda=dict(mod["lda"], ncomp=[3, 2]))
+    with pytest.raises(F.FplsError, match="strictly ascending"):
+        F.model_from_fields(bad)
+    bad = dict(mod, lda=dict(mod["lda"], ncomp=[2, 99],
+                             models={2: mod["lda"]["models"][2], 99: mod["lda"]["models"][3]}))
+    with pytest.raises(ValueError, match="inconsistent"):
+        F.model_from_fields(bad)
+    ck = dict(mod)
+    ck.pop("lda")
+    O.fit_candidate_knn(ck, X, y)
+    yc = ck["candidate_knn"]["y_codes"].copy()
+    yc[5] = C + 1
+    with pytest.raises(F.FplsError, match="outside the training levels"):
+        F.model_from_fields(dict(ck, candidate_knn=dict(ck["candidate_knn"], y_codes=yc)))
+    par = dict(ck["candidate_knn"]["parameters"], tau=0.0)
+    with pytest.raises(F.FplsError, match="candidate_tau"):
+        F.model_from_fields(dict(ck, candidate_knn=dict(ck["candidate_knn"], parameters=par)))

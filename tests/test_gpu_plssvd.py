@@ -232,3 +232,31 @@ def test_fused_plssvd_lda_equals_separate_calls(ctx, precision):
     m1.free()
     m2.free()
     m3.free()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("scaling", ["centering", "autoscaling"])
+@pytest.mark.parametrize("C", [9, 150])
+def test_label_fit_offset_columns(ctx, dtype, scaling, C):
+    """Columns centred near 1000 with unit-scale spread: the one-pass
+    labelled cross-product shifts X by its first row before the products
+    (C <= 127: the MFMA class cross-product; C > 127: the segmented class
+    sums).  mX, vX and the coefficients match the FP64 oracle on the same
+    (rounded) data: FP64 inputs to 1e-9, FP32 inputs mX / vX to 1e-7 and B to
+    1e-4 (FP32 products of exactly shifted values)."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(70 + C + (scaling == "autoscaling"))
+    n, p, k = 3000, 60, 5
+    mu = rng.standard_normal((C, p))
+    y = rng.integers(1, C + 1, n).astype(np.int32)
+    y[:C] = np.arange(1, C + 1)
+    X = (1000.0 + rng.uniform(-50, 50, p)) + mu[y - 1] + rng.standard_normal((n, p))
+    X = X.astype(dtype)
+    m = F.pls(X, y, method="plssvd", ncomp=[k], scaling=scaling, seed=5)
+    sc = 1 if scaling == "centering" else 2
+    ref = O.pls_model1(X.astype(np.float64), O.transformy(y), [k], sc, False, "rsvd", 10, 1, 5)
+    tight = dtype == np.float64
+    assert rel_err(m["mX"], ref["mX"]) < (1e-12 if tight else 1e-7)
+    assert rel_err(m["vX"], ref["vX"]) < (1e-9 if tight else 1e-7)
+    assert rel_err(m["B"], ref["B"]) < (1e-9 if tight else 1e-4)
+    assert rel_err(align_signs(m["R"], ref["R"]), ref["R"]) < (1e-9 if tight else 1e-4)
