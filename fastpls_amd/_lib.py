@@ -114,7 +114,7 @@ SIGNATURES = {
                                            ctypes.c_int32, _P(Opts), ctypes.c_double, _P(_vp)]),
     "fpls_plssvd_lda_fit_predict": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                                    ctypes.c_int32, _P(Opts), ctypes.c_double,
-                                                   _P(Mat), _vp, _P(_vp)]),
+                                                   _P(Mat), _vp, _vp, _P(_vp)]),
     "fpls_model_info_get": (ctypes.c_int, [_vp, _P(ModelInfo)]),
     "fpls_model_import": (ctypes.c_int, [_vp, _P(ModelFields), _P(_vp)]),
     "fpls_model_import_lda": (ctypes.c_int, [_vp, _vp, _P(LdaFields)]),
@@ -127,6 +127,7 @@ SIGNATURES = {
                                              ctypes.c_int32, ctypes.c_double]),
     "fpls_lda_read": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int64]),
     "fpls_lda_predict": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, _vp]),
+    "fpls_lda_predict_margins": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, _vp, _vp]),
     "fpls_lda_predict_noalloc": (ctypes.c_int, [_vp, _vp, _P(Mat), _vp, _vp]),
     "fpls_lda_train_scores": (ctypes.c_int, [_vp, _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                              ctypes.c_int32, ctypes.c_double, _vp, _vp, _vp, _vp,
@@ -134,9 +135,15 @@ SIGNATURES = {
     "fpls_cknn_fit": (ctypes.c_int, [_vp, _vp, _P(Mat), _P(IVec), ctypes.c_int32, ctypes.c_int32,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_int32]),
     "fpls_cknn_predict": (ctypes.c_int, [_vp, _vp, _P(Mat), ctypes.c_int32, _vp, _vp]),
+    "fpls_cknn_predict_margins": (ctypes.c_int, [_vp, _vp, _P(Mat), ctypes.c_int32, _vp, _vp,
+                                                 _vp]),
     "fpls_cknn_predict_scores": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), _P(Mat), _vp,
                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                 ctypes.c_double, ctypes.c_double, _vp, _vp]),
+    "fpls_cknn_predict_scores_margins": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), _P(Mat),
+                                                        _vp, ctypes.c_int32, ctypes.c_int32,
+                                                        ctypes.c_int32, ctypes.c_double,
+                                                        ctypes.c_double, _vp, _vp, _vp]),
     "fpls_rsvd": (ctypes.c_int, [_vp, _P(Mat), ctypes.c_int32, _P(Opts), _P(Mat), _vp, _vp, _vp,
                                  _P(ctypes.c_int32)]),
     "fpls_rsvd_xprod": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), ctypes.c_int32, _P(Opts), _P(Mat),

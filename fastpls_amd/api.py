@@ -123,14 +123,16 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
         precision="fp32", label_aware=False, fetch=True, device=0, top=1,
         n_classes=None, gaussian_y=False, gaussian_y_dim=None, north=1, kernel="linear",
-        gamma=None, degree=3, coef0=1.0) -> FastPLS:
+        gamma=None, degree=3, coef0=1.0, tie_log=False) -> FastPLS:
     """pls() (R/main.R:5860-6314) with backend = "hip".  ``n_classes``: level
     count of device-resident integer labels (optional).  ``gaussian_y``: fit a
     Gaussian compression of the response and decode the predictions
     (.prepare_gaussian_y, R/main.R:6057-6068; argmax classifier).
     ``method="opls"`` (``north`` orthogonal components) and
     ``method="kernelpls"`` (``kernel``, ``gamma``, ``degree``, ``coef0``) fit
-    SIMPLS on the filtered X / the centred training kernel (R/main.R:5956-6006)."""
+    SIMPLS on the filtered X / the centred training kernel (R/main.R:5956-6006).
+    ``tie_log``: with Xtest and an LDA / candidate-kNN head, also return the
+    device's per-row winning margins as ``tie_margin`` (see predict())."""
     _check_backend(backend)
     if method in ("opls", "kernelpls"):
         inner_kw = dict(ncomp=ncomp, classifier=classifier, lda_ridge=lda_ridge,
@@ -146,7 +148,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
             model = _kernel_pls_fit(Xtrain, Ytrain, scaling, kernel, gamma, degree, coef0,
                                     inner_kw)
         if Xtest is not None:
-            model.update(predict(model, Xtest, Ytest, top=top))
+            model.update(predict(model, Xtest, Ytest, top=top, tie_log=tie_log))
         return model
     if gaussian_y:
         if classifier not in ("argmax", "lda", "cknn"):
@@ -174,7 +176,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
             model["R2Y"] = np.array([1.0 - float(((Yo - Yf[:, :, a]) ** 2).sum()) / tss
                                      for a in range(Yf.shape[2])])
         if Xtest is not None:
-            model.update(predict(model, Xtest, Ytest, top=top))
+            model.update(predict(model, Xtest, Ytest, top=top, tie_log=tie_log))
         return model
     ctx = _lib.context(device)
     keep: list = []
@@ -206,10 +208,12 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     if fused_lda and Xtest is not None and resolve_top_k(top) == 1:
         Xt = as_mat(Xtest, keep)
         fused_pred = np.empty((Xt.rows, nc_arr.size), dtype=np.int32, order="F")
+        fused_margin = (np.empty((Xt.rows, nc_arr.size), dtype=np.float64, order="F")
+                        if tie_log else None)
         ctx.check(ctx.lib.fpls_plssvd_lda_fit_predict(
             ctx.h, ctypes.byref(X), ctypes.byref(yv), C, nc_ptr, nc_arr.size,
             ctypes.byref(opts), float(lda_ridge), ctypes.byref(Xt), fused_pred.ctypes.data,
-            ctypes.byref(h)))
+            fused_margin.ctypes.data if tie_log else None, ctypes.byref(h)))
     elif fused_lda:
         # the fit and the LDA head in one call (one launch for their solves)
         ctx.check(ctx.lib.fpls_plssvd_lda_fit(ctx.h, ctypes.byref(X), ctypes.byref(yv), C,
@@ -241,6 +245,8 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     if fused_pred is not None:
         res = {"Ypred_index": fused_pred,
                "Ypred": np.asarray(lev, dtype=object)[fused_pred - 1]}
+        if tie_log:
+            res["tie_margin"] = fused_margin
         if Ytest is not None:
             L = fused_pred.shape[1]
             res["Q2Y"] = _q2_classes(Ytest, lev, np.stack(
@@ -248,7 +254,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
                  for a in range(L)], axis=2))
         model.update(res)
     elif Xtest is not None:
-        model.update(predict(model, Xtest, Ytest, top=top))
+        model.update(predict(model, Xtest, Ytest, top=top, tie_log=tie_log))
     return model
 
 
@@ -387,7 +393,7 @@ def _attach_head(model: FastPLS, classifier: str, yv, C: int, lda_ridge, candida
 
 
 def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 1,
-            top5: bool = False, raw_scores: bool = False) -> Dict:
+            top5: bool = False, raw_scores: bool = False, tie_log: bool = False) -> Dict:
     """predict.fastPLS (R/main.R:2577-2762) for a HIP model.
 
     Classification results follow .class_topk_to_labels: ``Ypred`` (labels,
@@ -395,7 +401,13 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
     class indices, w x top x L), ``Ypred_top_score`` and ``Ypred_top_label``.
     The argmax head without Ytest / raw_scores ranks on the device straight
     from the compact factors (fpls_class_predict_topk); otherwise the score
-    cube is formed and ranked (fpls_class_topk_scores)."""
+    cube is formed and ranked (fpls_class_topk_scores).
+
+    ``tie_log`` (LDA and candidate-kNN heads): ``tie_margin`` holds the
+    device's per-row margins -- LDA w x L (winning minus runner-up
+    discriminant value), candidate kNN w x 2 x L (rank-1 minus rank-2
+    reranked score; the candidate-set boundary gap of the base scores) --
+    by which a label is judged a near-tie."""
     ctx = model.ctx
     top = resolve_top_k(top, top5)
     if model.get("opls") is not None:
@@ -421,9 +433,17 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
         labels = np.empty((w, L), dtype=np.int32, order="F")
         want = raw_scores or top > 1
         sc = np.empty((w, C, L), dtype=np.float64, order="F") if want else None
-        ctx.check(ctx.lib.fpls_lda_predict(ctx.h, model.handle, ctypes.byref(X),
-                                           labels.ctypes.data,
-                                           sc.ctypes.data if want else None))
+        if tie_log:
+            mg = np.empty((w, L), dtype=np.float64, order="F")
+            ctx.check(ctx.lib.fpls_lda_predict_margins(ctx.h, model.handle, ctypes.byref(X),
+                                                       labels.ctypes.data,
+                                                       sc.ctypes.data if want else None,
+                                                       mg.ctypes.data))
+            res["tie_margin"] = mg
+        else:
+            ctx.check(ctx.lib.fpls_lda_predict(ctx.h, model.handle, ctypes.byref(X),
+                                               labels.ctypes.data,
+                                               sc.ctypes.data if want else None))
         res["Ypred_index"] = labels
         res["Ypred"] = np.asarray(lev, dtype=object)[labels - 1]
         if raw_scores:
@@ -439,9 +459,17 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
         k = max(1, min(int(top), len(lev)))
         ti = np.empty((w, k, L), dtype=np.int32, order="F")
         ts = np.empty((w, k, L), dtype=np.float64, order="F")
-        ctx.check(ctx.lib.fpls_cknn_predict(ctx.h, model.handle, ctypes.byref(X), k,
-                                            ti.ctypes.data, ts.ctypes.data))
+        if tie_log:
+            mg = np.empty((w, 2, L), dtype=np.float64, order="F")
+            ctx.check(ctx.lib.fpls_cknn_predict_margins(ctx.h, model.handle, ctypes.byref(X), k,
+                                                        ti.ctypes.data, ts.ctypes.data,
+                                                        mg.ctypes.data))
+        else:
+            ctx.check(ctx.lib.fpls_cknn_predict(ctx.h, model.handle, ctypes.byref(X), k,
+                                                ti.ctypes.data, ts.ctypes.data))
         res.update(_topk_result(ti, ts, lev))
+        if tie_log:
+            res["tie_margin"] = mg
         if Ytest is not None:
             C = len(lev)
             res["Q2Y"] = _q2_classes(Ytest, lev, np.stack(
@@ -1585,12 +1613,13 @@ def candidate_knn_predict(Ttest_norm, Ttrain_norm, y_codes, centroids, candidate
     w = Te.rows
     ti = np.empty((w, top), dtype=np.int32, order="F")
     ts = np.empty((w, top), dtype=np.float64, order="F")
+    mg = np.empty((w, 2), dtype=np.float64, order="F")
     bias = None
     if candidate_bias is not None and len(candidate_bias):
         bias = np.ascontiguousarray(np.asarray(candidate_bias, dtype=np.float64))
         keep.append(bias)
-    ctx.check(ctx.lib.fpls_cknn_predict_scores(
+    ctx.check(ctx.lib.fpls_cknn_predict_scores_margins(
         ctx.h, ctypes.byref(Te), ctypes.byref(Tr), ctypes.byref(yv), ctypes.byref(Ce),
         bias.ctypes.data if bias is not None else None, top, int(top_m), int(knn_k), float(tau),
-        float(alpha), ti.ctypes.data, ts.ctypes.data))
-    return {"top_index": ti, "top_score": ts}
+        float(alpha), ti.ctypes.data, ts.ctypes.data, mg.ctypes.data))
+    return {"top_index": ti, "top_score": ts, "tie_margin": mg}

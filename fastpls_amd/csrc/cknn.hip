@@ -170,16 +170,22 @@ __global__ __launch_bounds__(256) void k_centroids(const double* __restrict__ pa
 // ---------------------------------------------------------------- top-m candidates
 // One wavefront per test row: m rounds of a wave argmax with the ordering
 // (value descending, class ascending), i.e. repeated strict-'>' insertion in
-// ascending class order.  base: w x C row-major.
+// ascending class order.  base: w x C row-major.  With boundary set, one more
+// round gives the best class left out: boundary[i] = the m-th candidate's base
+// score minus that class's (+inf when no class is left out), the tie-log entry
+// of the candidate set.
 __global__ __launch_bounds__(256) void k_topm(const float* __restrict__ base, int64_t w, int C,
                                               int m, int32_t* __restrict__ cand,
-                                              double* __restrict__ cand_base) {
+                                              double* __restrict__ cand_base,
+                                              double* __restrict__ boundary) {
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= w) return;
   const float* row = base + i * C;
   unsigned long long taken = 0ull;  // bit b: class lane + 64 b already selected
-  for (int r = 0; r < m; ++r) {
+  float last = -INFINITY;           // base score of the latest selected candidate
+  const int rounds = boundary ? m + 1 : m;
+  for (int r = 0; r < rounds; ++r) {
     float bv = -INFINITY;
     int bc = -1;
     for (int b = 0, c = lane; c < C; ++b, c += 64) {
@@ -202,7 +208,13 @@ __global__ __launch_bounds__(256) void k_topm(const float* __restrict__ base, in
         bc = oc;
       }
     }
+    if (r == m) {  // the boundary round (boundary is set)
+      if (lane == 0)
+        boundary[i] = (bc >= 0 && last > -INFINITY) ? (double)last - (double)bv : INFINITY;
+      break;
+    }
     if (bc >= 0 && (bc % 64) == lane) taken |= 1ull << (bc / 64);
+    last = bc >= 0 ? bv : -INFINITY;
     if (lane == 0) {
       cand[i * m + r] = bc + 1;
       cand_base[i * m + r] = (double)bv;
@@ -439,9 +451,12 @@ __global__ void k_cand_merge(const int* __restrict__ bucket, int nslots,
 
 // rerank: strict '>' insertion of the m candidates in candidate order
 // (src/fastPLS.cpp:1428-1457); one thread per test row.
+// With margin set, margin[i] = rank-1 score - rank-2 score (+inf with a single
+// ranked class): the tie-log entry of the winning class.
 __global__ void k_rerank(const int32_t* __restrict__ cand, const double* __restrict__ score,
                          int64_t w, int m, int top, int C, int32_t* __restrict__ top_index,
-                         double* __restrict__ top_score, int64_t ld_out) {
+                         double* __restrict__ top_score, int64_t ld_out,
+                         double* __restrict__ margin) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= w) return;
   double sv[64];
@@ -470,6 +485,7 @@ __global__ void k_rerank(const int32_t* __restrict__ cand, const double* __restr
     top_index[i + j * ld_out] = si[j];
     if (top_score) top_score[i + j * ld_out] = sv[j];
   }
+  if (margin) margin[i] = (m > 1 && sv[1] > -INFINITY) ? sv[0] - sv[1] : INFINITY;
 }
 
 // ---------------------------------------------------------------- host core
@@ -536,7 +552,8 @@ void class_centroids(Ctx* ctx, const float* Tn, int k, const CkClassIndex& ci, i
 void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int k, const float* Tn,
                        const std::vector<int>& off, const float* cen_f, int C, const double* bias,
                        int top, int top_m, int knn_k, double tau, double alpha,
-                       int32_t* top_index, double* top_score, int64_t ld_out) {
+                       int32_t* top_index, double* top_score, int64_t ld_out,
+                       double* margins) {
   FPLS_REQUIRE(k >= 1 && k <= 1024, FPLS_ERR_ARG, "candidate kNN supports up to 1024 components");
   FPLS_REQUIRE(C <= 4096, FPLS_ERR_ARG, "candidate selection supports up to 4096 classes");
   const int top_k = std::max(1, std::min(top, C));
@@ -552,7 +569,8 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
   DBuf<int32_t> cand(ctx, npairs);
   DBuf<double> cbase(ctx, npairs), score(ctx, npairs);
   launch(ctx, "cknn_topm", k_topm, dim3((unsigned)((w + 3) / 4)), dim3(256), 0,
-         (const float*)base.get(), w, C, m, cand.get(), cbase.get());
+         (const float*)base.get(), w, C, m, cand.get(), cbase.get(),
+         margins ? margins + w : (double*)nullptr);
   // bucket the pairs by class
   DBuf<int> cnt(ctx, C), cursor(ctx, C), bucket(ctx, npairs);
   fill_i32(ctx, cnt.get(), 0, C);
@@ -618,7 +636,7 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
            (const double*)cbase.get(), score.get());
   launch(ctx, "cknn_rerank", k_rerank, dim3((unsigned)((w + 255) / 256)), dim3(256), 0,
          (const int32_t*)cand.get(), (const double*)score.get(), w, m, top_k, C, top_index,
-         top_score, ld_out);
+         top_score, ld_out, margins);
   sync(ctx);  // host vectors above must outlive the async copies
 }
 
@@ -778,8 +796,9 @@ extern "C" int fpls_cknn_fit(fpls_ctx* ctx, fpls_model* model, const fpls_mat* X
   });
 }
 
-extern "C" int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
-                                 int32_t top, int32_t* top_index, double* top_score) {
+static int cknn_predict_entry(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                              int32_t top, int32_t* top_index, double* top_score,
+                              double* margins_out) {
   if (!model) return FPLS_ERR_ARG;
   return fpls_guard(ctx, [&] {
     Ctx* c = &ctx->c;
@@ -814,6 +833,8 @@ extern "C" int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const f
     const int L = (int)M->ncomp.size();
     DBuf<int32_t> ti(c, (size_t)w * topc * L);
     DBuf<double> ts(c, (size_t)w * topc * L);
+    DBuf<double> mg;
+    if (margins_out) mg = DBuf<double>(c, (size_t)w * 2 * L);
     for (int a = 0; a < L; ++a) {
       const int kk = std::max(1, std::min(M->ncomp[a], kmax));
       const CknnSlice* sl = nullptr;
@@ -827,18 +848,31 @@ extern "C" int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const f
       cknn_predict_core(c, M->precision, Ttn.get(), w, kk, sl->Tn.get(), M->class_offsets_host,
                         sl->centroids_f.get(), C, nullptr, topc, top_m_eff, M->knn_k, M->tau,
                         M->alpha, ti.get() + (size_t)a * w * topc, ts.get() + (size_t)a * w * topc,
-                        w);
+                        w, margins_out ? mg.get() + (size_t)a * w * 2 : nullptr);
     }
     to_host_i32(c, top_index, ti.get(), (size_t)w * topc * L);
+    if (margins_out) to_host(c, margins_out, mg.get(), (size_t)w * 2 * L);
     if (top_score) to_host(c, top_score, ts.get(), (size_t)w * topc * L);
   });
 }
 
-extern "C" int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_m,
-                                        const fpls_mat* Ttrain_m, const fpls_ivec* yv,
-                                        const fpls_mat* cen_m, const double* candidate_bias,
-                                        int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
-                                        double alpha, int32_t* top_index, double* top_score) {
+extern "C" int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                                 int32_t top, int32_t* top_index, double* top_score) {
+  return cknn_predict_entry(ctx, model, Xm, top, top_index, top_score, nullptr);
+}
+
+extern "C" int fpls_cknn_predict_margins(fpls_ctx* ctx, const fpls_model* model,
+                                         const fpls_mat* Xm, int32_t top, int32_t* top_index,
+                                         double* top_score, double* margins) {
+  if (!margins) return FPLS_ERR_ARG;
+  return cknn_predict_entry(ctx, model, Xm, top, top_index, top_score, margins);
+}
+
+static int cknn_scores_entry(fpls_ctx* ctx, const fpls_mat* Ttest_m, const fpls_mat* Ttrain_m,
+                             const fpls_ivec* yv, const fpls_mat* cen_m,
+                             const double* candidate_bias, int32_t top_k, int32_t top_m,
+                             int32_t knn_k, double tau, double alpha, int32_t* top_index,
+                             double* top_score, double* margins_out) {
   return fpls_guard(ctx, [&] {
     Ctx* c = &ctx->c;
     DBuf<unsigned char> h1, h2, h3;
@@ -882,11 +916,34 @@ extern "C" int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_m,
     const int topc = std::max(1, std::min((int)top_k, C));
     DBuf<int32_t> ti(c, (size_t)Te.rows * topc);
     DBuf<double> ts(c, (size_t)Te.rows * topc);
+    DBuf<double> mg;
+    if (margins_out) mg = DBuf<double>(c, (size_t)Te.rows * 2);
     cknn_predict_core(c, FPLS_PREC_FP32, te.get(), Te.rows, k, sorted.get(), ci.off, ce.get(), C,
-                      bias.get(), topc, top_m, knn_k, tau, alpha, ti.get(), ts.get(), Te.rows);
+                      bias.get(), topc, top_m, knn_k, tau, alpha, ti.get(), ts.get(), Te.rows,
+                      mg.get());
     to_host_i32(c, top_index, ti.get(), (size_t)Te.rows * topc);
     if (top_score) to_host(c, top_score, ts.get(), (size_t)Te.rows * topc);
+    if (margins_out) to_host(c, margins_out, mg.get(), (size_t)Te.rows * 2);
   });
+}
+
+extern "C" int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_m,
+                                        const fpls_mat* Ttrain_m, const fpls_ivec* yv,
+                                        const fpls_mat* cen_m, const double* candidate_bias,
+                                        int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
+                                        double alpha, int32_t* top_index, double* top_score) {
+  return cknn_scores_entry(ctx, Ttest_m, Ttrain_m, yv, cen_m, candidate_bias, top_k, top_m,
+                           knn_k, tau, alpha, top_index, top_score, nullptr);
+}
+
+extern "C" int fpls_cknn_predict_scores_margins(
+    fpls_ctx* ctx, const fpls_mat* Ttest_m, const fpls_mat* Ttrain_m, const fpls_ivec* yv,
+    const fpls_mat* cen_m, const double* candidate_bias, int32_t top_k, int32_t top_m,
+    int32_t knn_k, double tau, double alpha, int32_t* top_index, double* top_score,
+    double* margins) {
+  if (!margins) return FPLS_ERR_ARG;
+  return cknn_scores_entry(ctx, Ttest_m, Ttrain_m, yv, cen_m, candidate_bias, top_k, top_m,
+                           knn_k, tau, alpha, top_index, top_score, margins);
 }
 
 // The reference's model$candidate_knn list (R/main.R:861-872) as the device

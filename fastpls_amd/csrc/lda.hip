@@ -27,14 +27,27 @@ static unsigned g1(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 65536));
 }
 
-// labels[i] = 1 + argmax_c (scores[i,c] + cst[c]); first maximum wins.
+// labels[i] = 1 + argmax_c (scores[i,c] + cst[c]); first maximum wins.  With
+// MARGIN, margin[i] = the winning value minus the runner-up (0 at an exact
+// tie, +inf with one class): the row's entry in the tie log.
+template <bool MARGIN>
 __global__ void k_argmax_rows(const double* __restrict__ scores, int64_t lds, int64_t n,
                               int64_t C, const double* __restrict__ cst,
-                              int32_t* __restrict__ labels) {
+                              int32_t* __restrict__ labels, double* __restrict__ margin) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= n) return;
   int best = 0;
   double bv = scores[i] + cst[0];
+  double sv = -INFINITY;  // runner-up value
+  auto take = [&](double v, int c) {
+    if (v > bv) {
+      if (MARGIN) sv = bv;
+      bv = v;
+      best = c;
+    } else if (MARGIN && v > sv) {
+      sv = v;
+    }
+  };
   int64_t c = 1;
   // eight loads in flight, then the comparisons in column order (first max)
   for (; c + 8 <= C; c += 8) {
@@ -42,20 +55,11 @@ __global__ void k_argmax_rows(const double* __restrict__ scores, int64_t lds, in
 #pragma unroll
     for (int u = 0; u < 8; ++u) v[u] = scores[i + (c + u) * lds] + cst[c + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u)
-      if (v[u] > bv) {
-        bv = v[u];
-        best = (int)(c + u);
-      }
+    for (int u = 0; u < 8; ++u) take(v[u], (int)(c + u));
   }
-  for (; c < C; ++c) {
-    const double v = scores[i + c * lds] + cst[c];
-    if (v > bv) {
-      bv = v;
-      best = (int)c;
-    }
-  }
+  for (; c < C; ++c) take(scores[i + c * lds] + cst[c], (int)c);
   labels[i] = best + 1;
+  if (MARGIN) margin[i] = bv - sv;
 }
 
 __global__ void k_add_consts(double* S, int64_t lds, int64_t n, int64_t C, const double* cst) {
@@ -67,10 +71,14 @@ __global__ void k_add_consts(double* S, int64_t lds, int64_t n, int64_t C, const
 }
 
 void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t C,
-                 const double* cst, int32_t* labels) {
-  if (n > 0)
-    launch(ctx, "lda_argmax", k_argmax_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-           scores, lds, n, C, cst, labels);
+                 const double* cst, int32_t* labels, double* margin) {
+  if (n <= 0) return;
+  if (margin)
+    launch(ctx, "lda_argmax", k_argmax_rows<true>, dim3((unsigned)((n + 255) / 256)), dim3(256),
+           0, scores, lds, n, C, cst, labels, margin);
+  else
+    launch(ctx, "lda_argmax", k_argmax_rows<false>, dim3((unsigned)((n + 255) / 256)), dim3(256),
+           0, scores, lds, n, C, cst, labels, (double*)nullptr);
 }
 
 // FP64 class counts; empty[0] = number of empty classes
@@ -515,7 +523,8 @@ namespace fpls {
 // `on_scores(a, sc)` receives slice a's discriminant scores plus constants
 // (w x C, device) when set.
 void lda_predict_core(Ctx* c, const Model* M, const DMat& X, int32_t* lab,
-                      const std::function<void(int, const double*)>& on_scores) {
+                      const std::function<void(int, const double*)>& on_scores,
+                      double* margins) {
   FPLS_REQUIRE(!M->lda.empty(), FPLS_ERR_STATE,
                "this model does not contain fitted LDA classifier parameters");
   FPLS_REQUIRE(X.cols == M->p, FPLS_ERR_ARG, "lda_predict: X has the wrong number of columns");
@@ -548,8 +557,8 @@ void lda_predict_core(Ctx* c, const Model* M, const DMat& X, int32_t* lab,
       if (s.k == kk) sl = &s;
     FPLS_REQUIRE(sl != nullptr, FPLS_ERR_STATE, "no fitted LDA classifier for this ncomp");
     lda_scores(c, M, *sl, X, T, sc.get());
-    launch(c, "lda_argmax", k_argmax_rows, dim3((unsigned)((w + 255) / 256)), dim3(256), 0,
-           sc.get(), w, w, C, sl->constants.get(), lab + (size_t)a * w);
+    argmax_rows(c, sc.get(), w, w, C, sl->constants.get(), lab + (size_t)a * w,
+                margins ? margins + (size_t)a * w : nullptr);
     if (on_scores) {
       launch(c, "lda_consts", k_add_consts, dim3(g1(w * C)), dim3(256), 0, sc.get(), w, w, C,
              sl->constants.get());
@@ -565,7 +574,7 @@ extern "C" int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* Xm,
                                            const int32_t* ncomp, int32_t n_ncomp,
                                            const fpls_opts* opts, double lda_ridge,
                                            const fpls_mat* Xtest, int32_t* labels,
-                                           fpls_model** out) {
+                                           double* margins, fpls_model** out) {
   if (out) *out = nullptr;
   std::unique_ptr<fpls_model> mdl(new fpls_model());
   int rc = fpls_guard(ctx, [&] {
@@ -594,9 +603,12 @@ extern "C" int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* Xm,
     FPLS_REQUIRE(!M->lda.empty(), FPLS_ERR_STATE, "plssvd_lda: the LDA head was not fitted");
     const int L = (int)M->ncomp.size();
     DBuf<int32_t> lab(c, (size_t)Xt.rows * L);
-    lda_predict_core(c, M, Xt, lab.get(), nullptr);
+    DBuf<double> mg;
+    if (margins) mg = DBuf<double>(c, (size_t)Xt.rows * L);
+    lda_predict_core(c, M, Xt, lab.get(), nullptr, mg.get());
     // completed by the entry point's one synchronisation (the checks' read-back)
     c->to_host_later(labels, lab.get(), (size_t)Xt.rows * L * sizeof(int32_t));
+    if (margins) c->to_host_later(margins, mg.get(), (size_t)Xt.rows * L * sizeof(double));
   });
   if (rc == FPLS_OK) *out = mdl.release();
   return rc;
@@ -657,8 +669,8 @@ extern "C" int fpls_model_import_lda(fpls_ctx* ctx, fpls_model* model,
   });
 }
 
-extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
-                                int32_t* labels, double* scores_out) {
+static int lda_predict_entry(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                             int32_t* labels, double* scores_out, double* margins_out) {
   if (!model) return FPLS_ERR_ARG;
   return fpls_guard(ctx, [&] {
     Ctx* c = &ctx->c;
@@ -675,7 +687,22 @@ extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fp
       cb = [&](int a, const double* sc) {
         to_host(c, scores_out + (size_t)a * w * C, sc, (size_t)w * C);
       };
-    lda_predict_core(c, M, X, lab.get(), cb);
+    DBuf<double> mg;
+    if (margins_out) mg = DBuf<double>(c, (size_t)w * L);
+    lda_predict_core(c, M, X, lab.get(), cb, mg.get());
     to_host_i32(c, labels, lab.get(), (size_t)w * L);
+    if (margins_out) to_host(c, margins_out, mg.get(), (size_t)w * L);
   });
+}
+
+extern "C" int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xm,
+                                int32_t* labels, double* scores_out) {
+  return lda_predict_entry(ctx, model, Xm, labels, scores_out, nullptr);
+}
+
+extern "C" int fpls_lda_predict_margins(fpls_ctx* ctx, const fpls_model* model,
+                                        const fpls_mat* Xm, int32_t* labels, double* scores_out,
+                                        double* margins) {
+  if (!margins) return FPLS_ERR_ARG;
+  return lda_predict_entry(ctx, model, Xm, labels, scores_out, margins);
 }

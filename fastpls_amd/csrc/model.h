@@ -149,12 +149,15 @@ void lda_train_core(Ctx* ctx, const DMat& T, const double* G, int64_t ldg, const
                     int32_t C, const std::vector<int>& ks, double ridge, int precision,
                     DBuf<double>& counts_out, std::vector<LdaSlice>& slices);
 // LDA labels (w x n_ncomp, device, 1-based) of the test rows X; on_scores(a,
-// scores) receives slice a's discriminant scores (w x C, device) when set.
+// scores) receives slice a's discriminant scores (w x C, device) when set;
+// margins (w x n_ncomp, device, optional) the rows' winning margins.
 void lda_predict_core(Ctx* ctx, const Model* M, const DMat& X, int32_t* labels,
-                      const std::function<void(int, const double*)>& on_scores);
-// labels[i] = 1 + argmax_c(scores[i,c] + cst[c]) (first maximum wins)
+                      const std::function<void(int, const double*)>& on_scores,
+                      double* margins = nullptr);
+// labels[i] = 1 + argmax_c(scores[i,c] + cst[c]) (first maximum wins);
+// margin[i] (optional) = winning value - runner-up value
 void argmax_rows(Ctx* ctx, const double* scores, int64_t lds, int64_t n, int64_t C,
-                 const double* cst, int32_t* labels);
+                 const double* cst, int32_t* labels, double* margin = nullptr);
 
 // Candidate kNN building blocks (cknn.hip).
 struct CkClassIndex {
@@ -170,7 +173,8 @@ void rownorm_rows(Ctx* ctx, const float* S, int64_t n, int k, const int* rows, f
 void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int k,
                        const float* Tn, const std::vector<int>& off, const float* cen_f, int C,
                        const double* bias, int top, int top_m, int knn_k, double tau,
-                       double alpha, int32_t* top_index, double* top_score, int64_t ld_out);
+                       double alpha, int32_t* top_index, double* top_score, int64_t ld_out,
+                       double* margins = nullptr);  // w x 2: rerank gap, candidate boundary
 
 }  // namespace fpls
 

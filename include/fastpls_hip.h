@@ -176,11 +176,13 @@ int fpls_plssvd_lda_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y, in
    (R/main.R:5640-5646, the fit-then-predict tail of pls(); prediction
    lda_project_predict_labels_cpp src/fastPLS.cpp:2100-2147) in one call with
    one host round trip at its end.  labels: Xtest rows x n_ncomp int32 (host,
-   1-based), as fpls_lda_predict writes them; *out receives the model. */
+   1-based), as fpls_lda_predict writes them; margins (same shape, FP64, host,
+   may be NULL): the tie log of fpls_lda_predict_margins; *out receives the
+   model. */
 int fpls_plssvd_lda_fit_predict(fpls_ctx* ctx, const fpls_mat* X, const fpls_ivec* y,
                                 int32_t n_classes, const int32_t* ncomp, int32_t n_ncomp,
                                 const fpls_opts* opts, double lda_ridge, const fpls_mat* Xtest,
-                                int32_t* labels, fpls_model** out);
+                                int32_t* labels, double* margins, fpls_model** out);
 
 /* SIMPLS ("fast" incremental core).  Dense Y or labels (one-hot response). */
 int fpls_simpls_fit(fpls_ctx* ctx, const fpls_mat* X, const fpls_mat* Y, const fpls_ivec* y,
@@ -326,6 +328,14 @@ int fpls_lda_read(fpls_ctx* ctx, const fpls_model* model, int32_t slice, int32_t
    (w x C x n_ncomp, may be NULL). */
 int fpls_lda_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
                      int32_t* labels, double* scores);
+/* fpls_lda_predict with the tie log (the per-row diagnostic the reference
+   does not emit; its labels come from lda_labels_from_scores,
+   src/fastPLS.cpp:1718-1734): margins (w x n_ncomp FP64, host, required) =
+   the winning discriminant value minus the runner-up (score + constant; 0 at
+   an exact tie, +inf with one class).  A label is a near-tie when its margin
+   is below the error bound of the precision mode. */
+int fpls_lda_predict_margins(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                             int32_t* labels, double* scores, double* margins);
 /* No counterpart in the reference.  fpls_lda_predict for callers that cannot
    allocate device memory (code under HIP graph capture, tight serving loops,
    harnesses that forbid hipMalloc): Xtest (device), labels (w x n_ncomp
@@ -352,12 +362,28 @@ int fpls_cknn_fit(fpls_ctx* ctx, fpls_model* model, const fpls_mat* Xtrain, cons
 /* top_index (w x top x n_ncomp, 1-based class), top_score (same, may be NULL). */
 int fpls_cknn_predict(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
                       int32_t top, int32_t* top_index, double* top_score);
+/* fpls_cknn_predict with the tie log (no counterpart in the reference, whose
+   ranking is candidate_insert_top, src/fastPLS.cpp:1249-1267, 1389-1457):
+   margins (w x 2 x n_ncomp FP64, host, required): [., 0, a] the rank-1 minus
+   the rank-2 reranked score (+inf with one ranked class), [., 1, a] the
+   top_m-th candidate's base score minus the best base score left out of the
+   candidate set (+inf when every class is a candidate). */
+int fpls_cknn_predict_margins(fpls_ctx* ctx, const fpls_model* model, const fpls_mat* Xtest,
+                              int32_t top, int32_t* top_index, double* top_score,
+                              double* margins);
 /* Array-level scorer on given normalised scores (candidate_knn_predict_cpp). */
 int fpls_cknn_predict_scores(fpls_ctx* ctx, const fpls_mat* Ttest_norm,
                              const fpls_mat* Ttrain_norm, const fpls_ivec* y_codes,
                              const fpls_mat* centroids, const double* candidate_bias,
                              int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
                              double alpha, int32_t* top_index, double* top_score);
+/* The same with the tie log of fpls_cknn_predict_margins (margins: ntest x 2). */
+int fpls_cknn_predict_scores_margins(fpls_ctx* ctx, const fpls_mat* Ttest_norm,
+                                     const fpls_mat* Ttrain_norm, const fpls_ivec* y_codes,
+                                     const fpls_mat* centroids, const double* candidate_bias,
+                                     int32_t top_k, int32_t top_m, int32_t knn_k, double tau,
+                                     double alpha, int32_t* top_index, double* top_score,
+                                     double* margins);
 
 /* ---- argmax-head class ranking ------------------------------------------------ */
 /* Top-k classes of the argmax head with per-class offsets: value(i, j) =

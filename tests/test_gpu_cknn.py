@@ -9,21 +9,16 @@ import numpy as np
 import pytest
 
 from oracle import fastpls_oracle as O
-from tests.helpers import sin_theta
+from tests.helpers import sin_theta, topk_mismatches_excused
 
 pytestmark = pytest.mark.gpu
 
 
-def _check_topk(ti, ts, ref, tol=1e-5):
+def _check_topk(ti, ts, ref, tol=1e-5, gpu_margin=None):
     ri, rs = ref["top_index"], ref["top_score"]
     assert ti.shape == ri.shape
-    bad = []
-    for i in range(ri.shape[0]):
-        if not np.array_equal(ti[i], ri[i]):
-            # excuse only near-ties in the oracle's ranked scores
-            gaps = np.abs(np.diff(rs[i]))
-            if not (np.min(gaps) < tol if gaps.size else False):
-                bad.append(i)
+    # excuse only near-ties in both the oracle's and the device's scores
+    bad = topk_mismatches_excused(ti, ri, rs, tol, ts, gpu_margin)
     assert not bad, f"rows with non-tie top-k mismatches: {bad[:10]}"
     fin = np.isfinite(rs)
     np.testing.assert_allclose(np.sort(ts, axis=1)[fin.all(1)], np.sort(rs, axis=1)[fin.all(1)],
@@ -47,7 +42,7 @@ def test_cknn_scores_match_oracle(ctx, C, k, ntr, nte, knn_k, top_m, top):
     cen = O.candidate_centroids(Ttr, ytr, C)
     got = F.candidate_knn_predict(Tte, Ttr, ytr, cen, None, top, top_m, knn_k, 0.2, 0.75)
     ref = O.candidate_knn_predict(Tte, Ttr, ytr, cen, None, top, top_m, knn_k, 0.2, 0.75)
-    _check_topk(got["top_index"], got["top_score"], ref)
+    _check_topk(got["top_index"], got["top_score"], ref, gpu_margin=got["tie_margin"])
 
 
 def test_cknn_large_class_spans_chunks(ctx):
@@ -64,7 +59,7 @@ def test_cknn_large_class_spans_chunks(ctx):
     cen = O.candidate_centroids(Ttr, ytr, C)
     got = F.candidate_knn_predict(Tte, Ttr, ytr, cen, None, 3, 3, 10, 0.2, 0.75)
     ref = O.candidate_knn_predict(Tte, Ttr, ytr, cen, None, 3, 3, 10, 0.2, 0.75)
-    _check_topk(got["top_index"], got["top_score"], ref)
+    _check_topk(got["top_index"], got["top_score"], ref, gpu_margin=got["tie_margin"])
 
 
 def test_pls_cknn_pipeline(ctx):
@@ -76,12 +71,13 @@ def test_pls_cknn_pipeline(ctx):
     ncomp = 10
     m = F.pls(Xtr, ytr, method="plssvd", ncomp=ncomp, classifier="cknn", candidate_knn_k=10,
               candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=8)
-    res = F.predict(m, Xte, top=5)
+    res = F.predict(m, Xte, top=5, tie_log=True)
     ref = O.pls_model1(Xtr.astype(np.float64), O.transformy(ytr), [ncomp])
     ref["lev"] = list(range(1, 16))
     O.fit_candidate_knn(ref, Xtr.astype(np.float64), ytr, 10, 0.2, 0.75, 8)
     rr = O.candidate_knn_predictions(ref, Xte.astype(np.float64), top=5)[0]
-    _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-4)
+    _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-4,
+                gpu_margin=res["tie_margin"][:, :, 0])
     assert np.mean(res["Ypred_index"][:, 0] == yte) > 0.9
 
 
@@ -107,10 +103,35 @@ def test_label_aware_cknn_many_classes_large_ncomp(ctx, precision):
     m = F.pls(Xtr, ytr, method="plssvd", ncomp=ncomp, classifier="cknn", label_aware=True,
               candidate_knn_k=10, candidate_tau=0.2, candidate_alpha=0.75, candidate_top_m=20,
               precision=precision)
-    res = F.predict(m, Xte, top=5)
+    res = F.predict(m, Xte, top=5, tie_log=True)
     ref = O.plssvd_label_aware(Xtr.astype(np.float64), ytr, [ncomp])
     assert sin_theta(m["R"][:, :ncomp], ref["R"][:, :ncomp]) < 1e-4
     ref["lev"] = list(range(1, C + 1))
     O.fit_candidate_knn(ref, Xtr.astype(np.float64), ytr, 10, 0.2, 0.75, 20)
     rr = O.candidate_knn_predictions(ref, Xte.astype(np.float64), top=5)[0]
-    _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-3)
+    _check_topk(res["Ypred_top"][:, :, 0], res["Ypred_top_score"][:, :, 0], rr, tol=1e-3,
+                gpu_margin=res["tie_margin"][:, :, 0])
+
+
+def test_cknn_tie_log(ctx):
+    """The device tie log: the rank-1 minus rank-2 reranked score equals the
+    gap of its own ranked scores; the candidate-set boundary gap matches the
+    FP64 base scores to the FP32 similarity error, and is +inf when every
+    class is a candidate."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(8)
+    C, k, ntr, nte = 10, 12, 800, 150
+    mu = rng.standard_normal((C, k))
+    ytr = rng.integers(1, C + 1, ntr).astype(np.int32)
+    ytr[:C] = np.arange(1, C + 1)
+    Ttr = O.candidate_row_l2(mu[ytr - 1] + 0.8 * rng.standard_normal((ntr, k)))
+    Tte = O.candidate_row_l2(mu[rng.integers(0, C, nte)] + 0.8 * rng.standard_normal((nte, k)))
+    cen = O.candidate_centroids(Ttr, ytr, C)
+    got = F.candidate_knn_predict(Tte, Ttr, ytr, cen, None, 3, 4, 5, 0.2, 0.75)
+    mg = got["tie_margin"]
+    np.testing.assert_array_equal(mg[:, 0], got["top_score"][:, 0] - got["top_score"][:, 1])
+    base = np.sort(Tte @ cen.T, axis=1)[:, ::-1]
+    np.testing.assert_allclose(mg[:, 1], base[:, 3] - base[:, 4], rtol=0, atol=1e-5)
+    assert np.all(mg[:, 1] >= 0)
+    full = F.candidate_knn_predict(Tte, Ttr, ytr, cen, None, 3, C, 5, 0.2, 0.75)
+    assert np.all(np.isinf(full["tie_margin"][:, 1]))

@@ -166,17 +166,21 @@ def test_import_oracle_lda_head_predicts_like_the_oracle(ctx, dtype, tol):
     mod = _oracle_plssvd_lda(X, y, C, ks)
     dm = F.model_from_fields(mod)
     assert dm["classification_rule"] == "lda_hip"
-    res = F.predict(dm, Xt.astype(dtype), raw_scores=True)
+    res = F.predict(dm, Xt.astype(dtype), raw_scores=True, tie_log=True)
     for a, k in enumerate(ks):
         lda = mod["lda"]["models"][k]
         lab, sc = O.lda_project_predict_labels(Xt, mod["R_predict"][:, :k], mod["R_offset"],
                                                lda, return_scores=True)
-        bad, unexcused = label_mismatches_excused(res["Ypred_index"][:, a], lab, sc, tol)
+        bad, unexcused = label_mismatches_excused(res["Ypred_index"][:, a], lab, sc, tol,
+                                                  res["tie_margin"][:, a])
         assert not unexcused, f"ncomp={k}: non-tie label mismatches at rows {unexcused[:10]}"
         assert len(bad) <= 2
         np.testing.assert_allclose(res["LDA_scores"][:, :, a], sc, rtol=0,
                                    atol=tol * max(1.0, np.abs(sc).max()))
         assert list(res["Ypred"][:, a]) == [mod["lev"][i - 1] for i in res["Ypred_index"][:, a]]
+        # the tie log is the device's own winning margin, to the last bit
+        srt = np.sort(res["LDA_scores"][:, :, a], axis=1)
+        np.testing.assert_array_equal(res["tie_margin"][:, a], srt[:, -1] - srt[:, -2])
     # the imported fields read back unchanged
     for a, k in enumerate(ks):
         lda = mod["lda"]["models"][k]
@@ -230,12 +234,13 @@ def test_import_oracle_cknn_head_predicts_like_the_oracle(ctx, method):
     dm = F.model_from_fields(mod)
     assert dm["classification_rule"] == "candidate_knn_hip"
     top = 3
-    res = F.predict(dm, Xt, top=top)
+    res = F.predict(dm, Xt, top=top, tie_log=True)
     ref = O.candidate_knn_predictions(mod, Xt, top=top)
     for a in range(len(ks)):
         ri, rs = ref[a]["top_index"], ref[a]["top_score"]
         ti = res["Ypred_top"][:, :, a]
-        bad = topk_mismatches_excused(ti, ri, rs, 1e-5)
+        bad = topk_mismatches_excused(ti, ri, rs, 1e-5, res["Ypred_top_score"][:, :, a],
+                                      res["tie_margin"][:, :, a])
         assert not bad, f"ncomp={ks[a]}: non-tie top-k mismatches at rows {bad[:10]}"
         np.testing.assert_allclose(res["Ypred_top_score"][:, :, a], rs, rtol=0, atol=1e-5)
 

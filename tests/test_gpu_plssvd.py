@@ -81,7 +81,7 @@ def test_plssvd_lda_fp32_labels(ctx):
     Xtr, ytr, Xte, yte = synth.classes_on_sphere(n_classes=12, per_class_train=150,
                                                  per_class_test=40, p=96, seed=5)
     ncomp = 11
-    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=ncomp, classifier="lda")
+    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=ncomp, classifier="lda", tie_log=True)
     ref = O.pls_plssvd_lda(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [ncomp])
     # coefficients within the FP32 budget
     assert rel_err(align_signs(m["R"], ref["R"]), ref["R"]) < 1e-4
@@ -97,7 +97,8 @@ def test_plssvd_lda_fp32_labels(ctx):
     _, sc = O.lda_project_predict_labels(Xte.astype(np.float64), ref["R_predict"][:, :ncomp],
                                          ref["R_offset"][:ncomp], lda_ref, return_scores=True)
     scale = float(np.abs(sc).max())
-    bad, unexcused = label_mismatches_excused(labels, ref["Ypred_labels"][:, 0], sc, 1e-3 * scale)
+    bad, unexcused = label_mismatches_excused(labels, ref["Ypred_labels"][:, 0], sc, 1e-3 * scale,
+                                              m["tie_margin"][:, 0])
     assert not unexcused, f"label mismatches beyond ties: {unexcused}"
 
 
@@ -110,7 +111,8 @@ def test_precision_modes_subspace_error(ctx, precision):
     Xtr, ytr, Xte, yte = synth.classes_on_sphere(n_classes=20, per_class_train=200,
                                                  per_class_test=30, p=256, seed=17)
     k = 19
-    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=k, classifier="lda", precision=precision)
+    m = F.pls(Xtr, ytr, Xte, method="plssvd", ncomp=k, classifier="lda", precision=precision,
+              tie_log=True)
     ref = O.pls_plssvd_lda(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [k])
     assert sin_theta(m["R"], ref["R"]) < 1e-4
     assert rel_err(m["B"], ref["B"]) < (1e-4 if precision == "fp32" else 5e-4)
@@ -118,7 +120,8 @@ def test_precision_modes_subspace_error(ctx, precision):
     _, sc = O.lda_project_predict_labels(Xte.astype(np.float64), ref["R_predict"][:, :k],
                                          ref["R_offset"][:k], lda_ref, return_scores=True)
     bad, unexcused = label_mismatches_excused(m["Ypred_index"][:, 0], ref["Ypred_labels"][:, 0],
-                                              sc, 1e-3 * float(np.abs(sc).max()))
+                                              sc, 1e-3 * float(np.abs(sc).max()),
+                                              m["tie_margin"][:, 0])
     assert not unexcused
 
 
@@ -224,7 +227,7 @@ def test_fused_plssvd_lda_equals_separate_calls(ctx, precision):
     l3 = np.empty_like(l1)
     ctx.check(ctx.lib.fpls_plssvd_lda_fit_predict(ctx.h, ctypes.byref(Xm), ctypes.byref(yv), 9,
                                                   ncp, 2, ctypes.byref(opts), 1e-8,
-                                                  ctypes.byref(Xt), l3.ctypes.data,
+                                                  ctypes.byref(Xt), l3.ctypes.data, None,
                                                   ctypes.byref(h3)))
     np.testing.assert_array_equal(l3, l1)
     m3 = F.FastPLS(ctx, h3)
