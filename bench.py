@@ -142,8 +142,18 @@ def host_cpu_info():
                     break
     except OSError:
         pass
+    lscpu = ""
+    try:
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            if line.strip().startswith("Model name"):
+                lscpu = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
     return {"nproc": share, "threads_all": threads, "cpu_model": model,
-            "os_cpu_count": os.cpu_count()}
+            "lscpu_model": lscpu, "os_cpu_count": os.cpu_count()}
 
 
 def time_cpu(fn, repeats=5, warmup=1, one_thread=True):
@@ -160,10 +170,12 @@ def time_cpu(fn, repeats=5, warmup=1, one_thread=True):
             for _ in range(warmup):
                 out = fn()
             runs = []
-            for _ in range(repeats):
+            for r in range(repeats):
                 t0 = time.perf_counter()
                 out = fn()
                 runs.append(time.perf_counter() - t0)
+                print(f"cpu baseline: {nt} thread(s) run {r + 1}/{repeats} {runs[-1]:.3f} s",
+                      file=sys.stderr, flush=True)
         tim[label] = {"threads": nt, "median_s": float(np.median(runs)), "runs_s": runs}
     return tim, out, info
 
@@ -174,8 +186,9 @@ def cpu_baseline_obj(tim, info, units_per_run, unit, sample, kind="port", extra=
     a = tim["all"]
     obj = {"value": units_per_run / a["median_s"], "unit": unit, "cores": a["threads"],
            "kind": kind, "sample": sample, "median_s": a["median_s"],
-           "runs": len(a["runs_s"]), "warmup_runs": 1, "nproc": info["nproc"],
-           "cpu_model": info["cpu_model"]}
+           "units_per_run": units_per_run, "runs": len(a["runs_s"]), "warmup_runs": 1,
+           "nproc": info["nproc"], "cpu_model": info["cpu_model"],
+           "lscpu_model": info.get("lscpu_model", "")}
     if "one" in tim:
         o = tim["one"]
         obj["one_thread"] = {"value": units_per_run / o["median_s"], "cores": 1,
@@ -183,6 +196,18 @@ def cpu_baseline_obj(tim, info, units_per_run, unit, sample, kind="port", extra=
     if extra:
         obj.update(extra)
     return obj
+
+
+def cpu_sample(fn, units_per_run, unit, sample, repeats=5, extra=None):
+    """The plan's CPU baseline for one workload: ``fn`` (one bounded sample of
+    it) timed by time_cpu -- median of ``repeats`` runs after a warm-up, at all
+    the process's cores and at 1 thread -- as a cpu_baseline object whose
+    ``value`` is ``units_per_run`` over the all-cores median.  Returns
+    (object, fn's last result)."""
+    tim, out, info = time_cpu(fn, repeats=repeats)
+    desc = (f"{sample}; median of {repeats} runs after one warm-up, at all "
+            f"{info['threads_all']} of the process's cores and at 1 thread")
+    return cpu_baseline_obj(tim, info, units_per_run, unit, desc, extra=extra), out
 
 
 def oracle_run(data):
@@ -293,22 +318,23 @@ def run_cv(args):
 def cv_oracle_check(X, y, outer, inner, masks, seeds, out, n_sample):
     """FP64 oracle fits of the first ``n_sample`` inner fits (timed: the CPU
     baseline) and the errors of every GPU form on those fits."""
-    from threadpoolctl import threadpool_limits
     from oracle import fastpls_oracle as O
     n = X.shape[0]
     Yd = np.zeros((n, 5))
     Yd[np.arange(n), y - 1] = 1.0
-    cores = min(16, os.cpu_count() or 1)
-    refs = []
-    with threadpool_limits(limits=cores):
-        t0 = time.perf_counter()
-        for f in range(n_sample):
-            tr = masks[:, f] == 1
-            te = masks[:, f] == 2
-            mod = O.pls_model2_fast(X[tr].astype(np.float64), Yd[tr], CV_NCOMP, 1, False,
-                                    "rsvd", 10, 1, int(seeds[f]))
-            refs.append((te, O.pls_predict(mod, X[te].astype(np.float64))))
-        dt = time.perf_counter() - t0
+    X64 = X.astype(np.float64)
+
+    def one_fit(f):
+        tr = masks[:, f] == 1
+        te = masks[:, f] == 2
+        mod = O.pls_model2_fast(X64[tr], Yd[tr], CV_NCOMP, 1, False, "rsvd", 10, 1,
+                                int(seeds[f]))
+        return te, O.pls_predict(mod, X64[te])
+    cpu, r0 = cpu_sample(lambda: one_fit(0), 1.0, "fits/s",
+                         "inner fit 0 (ncomp 1..30: FP64 numpy oracle SIMPLS fit + predict of "
+                         "its held-out rows) of the same workload")
+    refs = [r0] + [one_fit(f) for f in range(1, n_sample)]
+    del X64
     errors = {}
     for form, r in out.items():
         num = den = 0.0
@@ -333,9 +359,6 @@ def cv_oracle_check(X, y, outer, inner, masks, seeds, out, n_sample):
             tot += diff.size
         errors[form] = {"Ypred_rel_err": (num / den) ** 0.5, "label_agreement": agree / tot,
                         "label_mismatch_non_tie": nontie}
-    cpu = {"value": n_sample / dt, "unit": "fits/s", "cores": cores, "kind": "port",
-           "sample": f"the first {n_sample} inner fits (ncomp 1..30) of the same workload, "
-                     "FP64 numpy oracle, threads = cores", "seconds": dt}
     return errors, cpu
 
 
@@ -584,21 +607,25 @@ def main_sketch(args):
     print(json.dumps(out))
 
 
+C5_CPU_ROWS = 5_000
+
+
 def c5_cpu_baseline(X, Y):
     """The oracle's matrix-free rSVD (FP64, q = 1: the reference default) of
-    the same S = X^T Y on the host cores: one rSVD is the sample."""
-    from threadpoolctl import threadpool_limits
+    S = X^T Y over the first C5_CPU_ROWS rows on the host cores.  Its passes
+    over X and Y are linear in the rows and the l x l factorisations are
+    row-independent, so the rate is scaled by C5_CPU_ROWS / n (which counts
+    the small row-independent part n / C5_CPU_ROWS times: a conservative
+    CPU rate)."""
     from oracle import fastpls_oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    Xh = X.cpu().numpy().astype(np.float64)
-    Yh = Y.cpu().numpy().astype(np.float64)
-    with threadpool_limits(limits=cores):
-        t0 = time.perf_counter()
-        O.truncated_rsvd_crossprod(Xh, Yh, C5_K, C5_OS, 1, 1)
-        dt = time.perf_counter() - t0
-    return {"value": 1.0 / dt, "unit": "rSVD/s", "cores": cores, "kind": "port",
-            "sample": "one matrix-free rSVD (q=1, FP64 numpy oracle) of the full workload",
-            "seconds": dt}
+    r = min(C5_CPU_ROWS, X.shape[0])
+    Xh = X[:r].cpu().numpy().astype(np.float64)
+    Yh = Y[:r].cpu().numpy().astype(np.float64)
+    cpu, _ = cpu_sample(lambda: O.truncated_rsvd_crossprod(Xh, Yh, C5_K, C5_OS, 1, 1),
+                        r / X.shape[0], "rSVD/s",
+                        f"one matrix-free rSVD (q=1, FP64 numpy oracle) of X^T Y over the "
+                        f"first {r} of the {X.shape[0]} rows, rate scaled by {r}/{X.shape[0]}")
+    return cpu
 
 
 # Label-aware PLSSVD + candidate kNN at 1000 classes: train 1,200,000 x 1024
@@ -735,7 +762,7 @@ def main_cknn(args):
 # Argmax-head class ranking on the same large-class-count data: the label-aware
 # PLSSVD model (ncomp 500, 1000 classes) fitted once, then one step ranks the
 # 50,000 test rows (top-5 with per-class offsets mY, no class bias).
-TK_SAMPLE = 30000
+TK_SAMPLE, TK_CPU_ROWS = 30000, 5000
 
 
 def main_topk(args):
@@ -831,19 +858,22 @@ def main_topk(args):
 
 def topk_oracle_check(m, Xte, r):
     """The oracle's pls_class_predict_topk on the device model's fields for the
-    first TK_SAMPLE test rows: timed (the CPU baseline, per row) and compared."""
-    from threadpoolctl import threadpool_limits
+    first TK_SAMPLE test rows, compared with the device's; its first
+    TK_CPU_ROWS rows are the timed CPU baseline (per row)."""
     from oracle import fastpls_oracle as O
     inf = m.info()
     p, k, C, L = Xte.shape[1], inf.kmax, inf.m, inf.n_ncomp
     mod = {"R": m.read("R", (p, k)), "W_latent": m.read("W_latent", (k, C, L)),
            "mX": m.read("mX", (p,)), "vX": m.read("vX", (p,)), "mY": m.read("mY", (C,)),
            "ncomp": np.array([inf.ncomp[i] for i in range(L)]), "m": C, "pls_method": "plssvd"}
-    cores = min(16, os.cpu_count() or 1)
-    with threadpool_limits(limits=cores):
-        t0 = time.perf_counter()
-        ri, rs = O.pls_class_predict_topk(mod, Xte[:TK_SAMPLE].astype(np.float64), None, C3_TOP)
-        t = time.perf_counter() - t0
+    Xs = Xte[:TK_CPU_ROWS].astype(np.float64)
+    cpu, (ri0, rs0) = cpu_sample(
+        lambda: O.pls_class_predict_topk(mod, Xs, None, C3_TOP), float(TK_CPU_ROWS), "rows/s",
+        f"FP64 numpy oracle pls_class_predict_topk on the first {TK_CPU_ROWS} test rows "
+        "(NumPy BLAS products + the insertion loop)")
+    ri1, rs1 = O.pls_class_predict_topk(mod, Xte[TK_CPU_ROWS:TK_SAMPLE].astype(np.float64),
+                                        None, C3_TOP)
+    ri, rs = np.concatenate([ri0, ri1]), np.concatenate([rs0, rs1])
     ti, ts = r["Ypred_top"][:TK_SAMPLE, :, 0], r["Ypred_top_score"][:TK_SAMPLE, :, 0]
     ri, rs = ri[:, :, 0], rs[:, :, 0]
     bad = 0
@@ -856,9 +886,6 @@ def topk_oracle_check(m, Xte, r):
               "top5_mismatch_non_tie": bad,
               "max_abs_score_err": float(np.max(np.abs(ts - rs))),
               "max_abs_score": float(np.max(np.abs(rs)))}
-    cpu = {"value": TK_SAMPLE / t, "unit": "rows/s", "cores": cores, "kind": "port",
-           "sample": f"FP64 numpy oracle pls_class_predict_topk on the first {TK_SAMPLE} test rows "
-                     "(NumPy BLAS products + the insertion loop)", "seconds": t}
     return errors, cpu
 
 
@@ -933,14 +960,11 @@ def main_kernel(args):
     odt = (time.perf_counter() - t0) / args.steps
     errors, cpu = {}, None
     if not args.no_cpu_baseline:
-        from threadpoolctl import threadpool_limits
         from oracle import fastpls_oracle as O
-        cores = min(16, os.cpu_count() or 1)
-        with threadpool_limits(limits=cores):
-            t0 = time.perf_counter()
-            ref = O.kernel_pls(Xtr.astype(np.float64), ytr, Xte.astype(np.float64), [KP_K],
-                               kernel=2, classes=KP_C)
-            t = time.perf_counter() - t0
+        X64, Xt64 = Xtr.astype(np.float64), Xte.astype(np.float64)
+        cpu, ref = cpu_sample(
+            lambda: O.kernel_pls(X64, ytr, Xt64, [KP_K], kernel=2, classes=KP_C), 1.0,
+            "fits/s", "one full FP64 kernel-PLS fit + predict of the same workload (numpy oracle)")
         rl = np.argmax(ref["Ypred"][:, :, 0], axis=1) + 1
         s = np.sort(ref["Ypred"][:, :, 0], axis=1)
         margin = s[:, -1] - s[:, -2]
@@ -949,9 +973,6 @@ def main_kernel(args):
                   "label_mismatch_non_tie": int((diff & (margin > 1e-6)).sum()),
                   "accuracy": float(np.mean(lab[:, 0] == yte)),
                   "oracle_accuracy": float(np.mean(rl == yte))}
-        cpu = {"value": 1.0 / t, "unit": "fits/s", "cores": cores, "kind": "port",
-               "sample": "one full FP64 kernel-PLS fit + predict of the same workload (numpy "
-                         "oracle)", "seconds": t}
     out = {
         "metric": "kernel PLS (rbf) fits/s", "value": 1.0 / dt, "unit": "fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
@@ -1040,21 +1061,18 @@ def main_gaussian(args):
     q2 = float(1.0 - ((Yt - yp[:, :, 0]) ** 2).sum() / ((Yt - Yt.mean(axis=0)) ** 2).sum())
     errors, cpu = {}, None
     if not args.no_cpu_baseline:
-        from threadpoolctl import threadpool_limits
         from oracle import fastpls_oracle as O
-        cores = min(16, os.cpu_count() or 1)
         X, Y, Xt = (dX.double().cpu().numpy(), dY.cpu().numpy(), dXt.double().cpu().numpy())
-        with threadpool_limits(limits=cores):
-            t0 = time.perf_counter()
+
+        def gy_run():
             enc = O.prepare_gaussian_y_regression(Y, GY_DIM, 7)
             mod = O.pls_model2_fast(X, enc["Z"], [GY_K], 1, False, "rsvd", 10, 1, 7)
             zt = O.pls_predict(mod, Xt)[:, :, 0]
-            ref = O.decode_gaussian_y_regression(zt, enc["decoder"], enc["y_center"])
-            t = time.perf_counter() - t0
+            return O.decode_gaussian_y_regression(zt, enc["decoder"], enc["y_center"])
+        cpu, ref = cpu_sample(gy_run, 1.0, "fits/s",
+                              "the full workload through the FP64 numpy oracle (encode, fit, "
+                              "predict, decode)")
         errors = {"Ypred_rel_err": float(np.linalg.norm(yp[:, :, 0] - ref) / np.linalg.norm(ref))}
-        cpu = {"value": 1.0 / t, "unit": "fits/s", "cores": cores, "kind": "port",
-               "sample": "the full workload through the FP64 numpy oracle (encode, fit, predict, "
-                         "decode)", "seconds": t}
     out = {
         "metric": "gaussian_y SIMPLS fits/s", "value": 1.0 / dt, "unit": "fits/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * dt,
@@ -1071,24 +1089,26 @@ def main_gaussian(args):
     print(json.dumps(out))
 
 
+C3_CPU_FRAC = 20
+
+
 def c3_oracle_check(Xtr, ytr, Xte, res):
-    """FP64 oracle: full fit on the training rows, prediction of the first
-    C3_SAMPLE test rows (timed: the CPU baseline, prediction extrapolated to
-    all test rows), and the top-5 agreement of each GPU form on those rows."""
-    from threadpoolctl import threadpool_limits
+    """FP64 oracle: full fit on the training rows and prediction of the first
+    C3_SAMPLE test rows, against which each GPU form's top-5 lists are
+    compared.  The CPU baseline is a 1/C3_CPU_FRAC sample of the workload --
+    the oracle fit on the first n/C3_CPU_FRAC training rows and the
+    prediction of the first ntest/C3_CPU_FRAC test rows -- with the rate
+    scaled by 1/C3_CPU_FRAC (its row-independent part, the S factorisation,
+    is counted C3_CPU_FRAC times: a conservative CPU rate)."""
     from oracle import fastpls_oracle as O
-    cores = min(16, os.cpu_count() or 1)
-    with threadpool_limits(limits=cores):
-        X64 = Xtr.astype(np.float64)
-        t0 = time.perf_counter()
-        ref = O.plssvd_label_aware(X64, ytr, [C3_K])
-        ref["lev"] = list(range(1, C3_C + 1))
-        O.fit_candidate_knn(ref, X64, ytr, 10, 0.2, 0.75, 20)
-        t_fit = time.perf_counter() - t0
-        del X64
-        t0 = time.perf_counter()
-        rr = O.candidate_knn_predictions(ref, Xte[:C3_SAMPLE].astype(np.float64), top=C3_TOP)[0]
-        t_pred = time.perf_counter() - t0
+    X64 = Xtr.astype(np.float64)
+    t0 = time.perf_counter()
+    ref = O.plssvd_label_aware(X64, ytr, [C3_K])
+    ref["lev"] = list(range(1, C3_C + 1))
+    O.fit_candidate_knn(ref, X64, ytr, 10, 0.2, 0.75, 20)
+    t_full = time.perf_counter() - t0
+    del X64
+    rr = O.candidate_knn_predictions(ref, Xte[:C3_SAMPLE].astype(np.float64), top=C3_TOP)[0]
     errors = {}
     for p, v in res.items():
         ti, ts = v["top"][:C3_SAMPLE], v["top_score"][:C3_SAMPLE]
@@ -1103,17 +1123,25 @@ def c3_oracle_check(Xtr, ytr, Xte, res):
                      "top5_equal_rows": float(np.mean(np.all(ti == ri, axis=1))),
                      "top5_mismatch_non_tie": bad,
                      "max_abs_score_err": float(np.max(np.abs(np.sort(ts, 1) - np.sort(rs, 1))))}
-    n_test = Xte.shape[0]
-    total = t_fit + t_pred * n_test / C3_SAMPLE
-    cpu = {"value": 1.0 / total, "unit": "fits/s", "cores": cores, "kind": "port",
-           "sample": f"FP64 numpy oracle: full fit, prediction of the first {C3_SAMPLE} test rows "
-                     f"extrapolated to {n_test}", "fit_seconds": t_fit,
-           "predict_seconds_sample": t_pred}
+    ns, nts = Xtr.shape[0] // C3_CPU_FRAC, Xte.shape[0] // C3_CPU_FRAC
+    Xs, ys, Xts = Xtr[:ns].astype(np.float64), ytr[:ns], Xte[:nts].astype(np.float64)
+
+    def sample():
+        mod = O.plssvd_label_aware(Xs, ys, [C3_K])
+        mod["lev"] = list(range(1, C3_C + 1))
+        O.fit_candidate_knn(mod, Xs, ys, 10, 0.2, 0.75, 20)
+        return O.candidate_knn_predictions(mod, Xts, top=C3_TOP)[0]
+    cpu, _ = cpu_sample(sample, 1.0 / C3_CPU_FRAC, "fits/s",
+                        f"FP64 numpy oracle fit on the first {ns} training rows and prediction "
+                        f"of the first {nts} test rows (1/{C3_CPU_FRAC} of the workload), rate "
+                        f"scaled by 1/{C3_CPU_FRAC}",
+                        extra={"full_fit_seconds_once": t_full})
     return errors, cpu
 
 
 # ---------------------------------------------------------------- SIMPLS regression
 C1_N, C1_P, C1_M, C1_NT, C1_K = 200, 2000, 3, 100, 10
+C1_CPU_BATCH = 50
 # Algorithmic work per fit (FP64): bytes 8 [n p (1 + 2K) + n m + n_t p] (X read
 # for S and twice per component, Y once, Xtest once); flops 2npm + K (4np +
 # 2nm + 14pm) + 4p sum_a a + 2pmK + 2 n_t p m.
@@ -1213,11 +1241,14 @@ def main_simpls_reg(args):
         "kernels": {f: v["kernels"] for f, v in res.items()},
     }
     if not args.no_cpu_baseline:
-        tim, _, info = time_cpu(lambda: O.pls_predict(O.pls_model2_fast(X, Y, [C1_K]), Xt),
-                                repeats=9)
-        out["cpu_baseline"] = cpu_baseline_obj(
-            tim, info, 1.0, "fits/s", "full FP64 SIMPLS fit + predict of the same workload "
-            "(numpy/LAPACK oracle); median of 9 runs after one warm-up")
+        def c1_batch():
+            for _ in range(C1_CPU_BATCH):
+                r = O.pls_predict(O.pls_model2_fast(X, Y, [C1_K]), Xt)
+            return r
+        out["cpu_baseline"], _ = cpu_sample(
+            c1_batch, float(C1_CPU_BATCH), "fits/s",
+            f"{C1_CPU_BATCH} full FP64 SIMPLS fits + predictions of the same workload per run "
+            "(numpy/LAPACK oracle)")
     print(json.dumps(out))
 
 
