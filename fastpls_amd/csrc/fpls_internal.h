@@ -206,6 +206,16 @@ struct DMat {
   const float* f() const { return static_cast<const float*>(p); }
   const double* d() const { return static_cast<const double*>(p); }
 };
+// A column-major FP64 device array as a DMat.
+inline DMat dview(const double* p, int64_t rows, int64_t cols, int64_t ld) {
+  DMat d;
+  d.p = p;
+  d.dtype = FPLS_F64;
+  d.rows = rows;
+  d.cols = cols;
+  d.ld = ld;
+  return d;
+}
 
 // Stage a caller matrix on the device (host buffers are copied into `hold`).
 DMat stage(Ctx* ctx, const fpls_mat* m, DBuf<unsigned char>& hold, const char* what);

@@ -1321,13 +1321,15 @@ void tn_skinny(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float
 // with ld Kp (K rounded up to 32) and zero rows K .. Kp - 1, as FP32 (Bf) or
 // as bf16 hi / lo planes (Bh, Bl).
 __global__ void k_g2_bsplit(const float* __restrict__ B, int64_t ldb, int64_t K, int N,
-                            int64_t Kp, float* __restrict__ Bf, __bf16* __restrict__ Bh,
-                            __bf16* __restrict__ Bl) {
+                            int64_t Kp, const double* __restrict__ shift, float* __restrict__ Bf,
+                            __bf16* __restrict__ Bh, __bf16* __restrict__ Bl) {
   const int64_t tot = Kp * N;
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < tot;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int64_t k = e % Kp, n = e / Kp;
-    const float v = k < K ? B[k + n * ldb] : 0.f;
+    const float v = k >= K ? 0.f
+                    : shift ? (float)((double)B[k + n * ldb] - shift[n])
+                            : B[k + n * ldb];
     if (Bf) {
       Bf[e] = v;
     } else {
@@ -1562,10 +1564,10 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
            (const float*)part.get(), shiftB, C, ldc);
     return;
   }
-  if (!labelsB && !shiftB && N <= 128) {
+  if (!labelsB && N <= 128) {
     // skinny dense B: the B tile of each K step is shared through LDS by the
     // workgroup's four waves (k_tns_dense); split K so that the grid holds
-    // about five workgroups per CU
+    // about five workgroups per CU.  B's shift is applied as it is padded.
     const bool bf = precision == FPLS_PREC_BF16X3;
     const int64_t Kp = (K + TL_BK - 1) / TL_BK * TL_BK;
     DBuf<float> Bf;
@@ -1577,7 +1579,7 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       Bf = DBuf<float>(ctx, (size_t)Kp * N);
     }
     launch(ctx, "pad_b", k_g2_bsplit, dim3(g1(Kp * N)), dim3(256), 0, B, ldb, K, (int)N, Kp,
-           Bf.get(), Bh.get(), Bl.get());
+           shiftB, Bf.get(), Bh.get(), Bl.get());
     const int64_t tiles = (M + 4 * TL_BM - 1) / (4 * TL_BM);
     int64_t splits = std::max<int64_t>(1, (5 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
     splits = std::min<int64_t>(splits, std::max<int64_t>(1, K / 256));

@@ -297,16 +297,6 @@ void prep_into(Ctx* c, const DMat& X, const double* mX, const double* vX, double
            X.ld, X.rows, X.cols, mX, vX, out, ldo);
 }
 
-DMat dview(const double* p, int64_t rows, int64_t cols, int64_t ld) {
-  DMat d;
-  d.p = p;
-  d.dtype = FPLS_F64;
-  d.rows = rows;
-  d.cols = cols;
-  d.ld = ld;
-  return d;
-}
-
 }  // namespace
 
 // The filter on a device matrix: X (n x p, FP32 / FP64) preprocessed and

@@ -82,22 +82,101 @@ __device__ double block_norm(const double* x, int64_t n, double* red) {
   return amax * sqrt(block_sum(s, red));
 }
 
-// dst = src / |src| (and also /= |src|); a zero or non-finite norm stops the
-// fit.  Nothing is written once the fit has stopped.
-__global__ __launch_bounds__(SP_NT) void k_sp_normalize(const double* src, double* dst, int64_t n,
-                                                        double* also, int64_t n2,
-                                                        int32_t* flags) {
-  __shared__ double red[SP_NT / 64];
+// ---- chip-wide normalisation for the large path: the scaled norm of
+// x = src - sub combined from per-chunk partials (max |x|, sum (x / max)^2)
+// as in a two-level dnrm2, then dst = x / |x| (and also /= |x|).
+constexpr int SP_NB = 256, SP_NE = 8;
+constexpr int64_t SP_NCH = (int64_t)SP_NB * SP_NE;  // elements per partial
+
+template <int NT>
+__device__ __forceinline__ double blk_reduce(double v, double* red, bool is_max) {
+  v = is_max ? wave_max_d(v) : wave_sum_d(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = red[0];
+#pragma unroll
+  for (int w = 1; w < NT / 64; ++w) t = is_max ? fmax(t, red[w]) : t + red[w];
+  return t;
+}
+
+__global__ __launch_bounds__(SP_NB) void k_sp_norm_part(const double* __restrict__ src,
+                                                        const double* __restrict__ sub, int64_t n,
+                                                        double2* __restrict__ part,
+                                                        const int32_t* __restrict__ flags) {
+  __shared__ double red[SP_NB / 64];
   if (flags[0]) return;
-  const double nr = block_norm(src, n, red);
-  if (!(nr > 0.0 && nr <= DBL_MAX)) {
-    if (threadIdx.x == 0) flags[0] = 1;
+  const int64_t b = (int64_t)blockIdx.x * SP_NCH + threadIdx.x;
+  double v[SP_NE], amax = 0.0, bad = 0.0;
+#pragma unroll
+  for (int e = 0; e < SP_NE; ++e) {
+    const int64_t i = b + (int64_t)e * SP_NB;
+    v[e] = i < n ? src[i] - (sub ? sub[i] : 0.0) : 0.0;
+    const double a = fabs(v[e]);
+    if (!(a <= DBL_MAX)) bad = 1.0;
+    else amax = fmax(amax, a);
+  }
+  bad = blk_reduce<SP_NB>(bad, red, true);
+  amax = blk_reduce<SP_NB>(amax, red, true);
+  double ssq = 0.0;
+  if (bad == 0.0 && amax > 0.0) {
+    const double inv = 1.0 / amax;
+#pragma unroll
+    for (int e = 0; e < SP_NE; ++e) {
+      const double u = v[e] * inv;
+      ssq = fma(u, u, ssq);
+    }
+  }
+  ssq = blk_reduce<SP_NB>(ssq, red, false);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(bad != 0.0 ? __builtin_nan("") : amax, ssq);
+}
+
+__global__ __launch_bounds__(SP_NB) void k_sp_norm_apply(const double* src, const double* sub,
+                                                         double* dst, int64_t n, double* also,
+                                                         int64_t n2,
+                                                         const double2* __restrict__ part,
+                                                         int nparts, int32_t* flags) {
+  __shared__ double red[SP_NB / 64];
+  if (flags[0]) return;
+  double am = 0.0, bad = 0.0;
+  for (int q = threadIdx.x; q < nparts; q += SP_NB) {
+    const double a = part[q].x;
+    if (!(a <= DBL_MAX)) bad = 1.0;
+    else am = fmax(am, a);
+  }
+  bad = blk_reduce<SP_NB>(bad, red, true);
+  am = blk_reduce<SP_NB>(am, red, true);
+  double ss = 0.0;
+  if (bad == 0.0 && am > 0.0)
+    for (int q = threadIdx.x; q < nparts; q += SP_NB) {
+      const double r = part[q].x / am;
+      ss = fma(r * r, part[q].y, ss);
+    }
+  ss = blk_reduce<SP_NB>(ss, red, false);
+  const double nr = bad != 0.0 ? __builtin_nan("") : am * sqrt(ss);
+  if (!(nr > 0.0 && nr <= DBL_MAX)) {  // every block computes the same nr
+    if (blockIdx.x == 0 && threadIdx.x == 0) flags[0] = 1;
     return;
   }
-  __syncthreads();
-  for (int64_t i = threadIdx.x; i < n; i += SP_NT) dst[i] = src[i] / nr;
+  const int64_t t0 = (int64_t)blockIdx.x * SP_NB + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * SP_NB;
+  for (int64_t i = t0; i < n; i += stride) dst[i] = (src[i] - (sub ? sub[i] : 0.0)) / nr;
   if (also)
-    for (int64_t i = threadIdx.x; i < n2; i += SP_NT) also[i] /= nr;
+    for (int64_t i = t0; i < n2; i += stride) also[i] /= nr;
+}
+
+// dst = (src - sub) / |src - sub| (sub may be NULL; dst may be src), also /=
+// the same norm; a zero or non-finite norm stops the fit.  part holds
+// ceil(n / SP_NCH) partials.
+static void sp_normalize(Ctx* c, const double* src, const double* sub, double* dst, int64_t n,
+                         double* also, int64_t n2, double2* part, int32_t* flags) {
+  const int nparts = (int)((n + SP_NCH - 1) / SP_NCH);
+  launch(c, "simpls_norm", k_sp_norm_part, dim3(nparts), dim3(SP_NB), 0, src, sub, n, part,
+         (const int32_t*)flags);
+  const int64_t work = std::max(n, also ? n2 : 0);
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>((work + 1023) / 1024, 1024));
+  launch(c, "simpls_norm", k_sp_norm_apply, dim3(g), dim3(SP_NB), 0, src, sub, dst, n, also, n2,
+         (const double2*)part, nparts, flags);
 }
 
 // X^T X branch: tn = sqrt(r . pp) from the device scalar, r /= tn, pp /= tn.
@@ -127,7 +206,8 @@ __global__ void k_sp_rank1(double* S, int64_t ld, int64_t rows, int64_t cols, co
   }
 }
 
-// Component a kept: R[:, a] = r, Q[:, a] = q, V[:, a] = v, Bcur += r q^T.
+// Component a kept: R[:, a] = r, Q[:, a] = q, V[:, a] = v, and Bcur += r q^T
+// when Bcur is set.
 __global__ void k_sp_commit(int a, const double* r, const double* q, const double* v, int64_t p,
                             int64_t m, double* R, double* Q, double* V, double* Bcur,
                             int32_t* flags) {
@@ -139,10 +219,11 @@ __global__ void k_sp_commit(int a, const double* r, const double* q, const doubl
     V[i + a * p] = v[i];
   }
   for (int64_t j = t0; j < m; j += stride) Q[j + a * m] = q[j];
-  for (int64_t e = t0; e < p * m; e += stride) {
-    const int64_t i = e % p, j = e / p;
-    Bcur[e] += r[i] * q[j];
-  }
+  if (Bcur)
+    for (int64_t e = t0; e < p * m; e += stride) {
+      const int64_t i = e % p, j = e / p;
+      Bcur[e] += r[i] * q[j];
+    }
   if (t0 == 0) flags[1] = a + 1;
 }
 
@@ -339,7 +420,9 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     copy_f64(c, S0.get(), S.get(), p * m);
   }
   DBuf<double> yv_(c, p), zv(c, m), rr(c, p), rs(c, p), tt(c, n), pp(c, p), qq(c, m), vv(c, p),
-      cv(c, K), vS(c, m), tn2(c, 1), om(c, std::max<int64_t>(p, 1));
+      cv(c, K), vS(c, m), tn2(c, 1), om(c, std::max<int64_t>(p, 1)), vc(c, p);
+  DBuf<double2> npart(c, (size_t)((std::max(n, p) + SP_NCH - 1) / SP_NCH));
+  const DMat Sv = dview(S.get(), p, m, p);
   DBuf<int32_t> flags(c, 2);
   fill_i32(c, flags.get(), 0, 2);
   if (can_incremental)
@@ -362,15 +445,14 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
       launch(c, "simpls_refresh", k_sp_refresh_small, dim3(1), dim3(SP_NT), 0,
              (const double*)S.get(), p, m, std::max(o.inc_iters, 0), warm, rr.get(), fl);
     } else if (can_incremental) {
-      copy_f64(c, yv_.get(), warm, p);
+      // power steps y <- S (S^T y) from the warm start, then normalised
+      const double* yin = warm;
       for (int it = 0; it < std::max(o.inc_iters, 0); ++it) {
-        gemm_f64(c, true, false, m, 1, p, 1.0, S.get(), p, nullptr, yv_.get(), p, nullptr, 0.0,
-                 zv.get(), m);
-        gemm_f64(c, false, false, p, 1, m, 1.0, S.get(), p, nullptr, zv.get(), m, nullptr, 0.0,
-                 yv_.get(), p);
+        gemv_t(c, Sv, nullptr, yin, zv.get());
+        gemv_n(c, Sv, nullptr, zv.get(), yv_.get());
+        yin = yv_.get();
       }
-      launch(c, "simpls_normalize", k_sp_normalize, dim3(1), dim3(SP_NT), 0,
-             (const double*)yv_.get(), rr.get(), p, (double*)nullptr, (int64_t)0, fl);
+      sp_normalize(c, yin, nullptr, rr.get(), p, nullptr, 0, npart.get(), fl);
     } else {
       SvdOut sv;
       svd_exact_dev(c, S.get(), p, p, m, 1, sv, true);
@@ -397,8 +479,7 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
       launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, rr.get(), M->vX.get(), p,
              rs.get());
       gemv_n(c, X, M->mX.get(), rs.get(), tt.get());
-      launch(c, "simpls_normalize", k_sp_normalize, dim3(1), dim3(SP_NT), 0,
-             (const double*)tt.get(), tt.get(), n, rr.get(), p, fl);
+      sp_normalize(c, tt.get(), nullptr, tt.get(), n, rr.get(), p, npart.get(), fl);
       // pp = Xs^T tt = (Xc^T tt) / vX
       gemv_t(c, X, M->mX.get(), tt.get(), pp.get());
       launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, pp.get(), M->vX.get(), p,
@@ -416,29 +497,33 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
              (const double*)qq.get(), M->R.get(), M->Q.get(), M->V.get(), Bcur.get(), fl);
     } else {
       // vv = pp - V_a (V_a^T pp); |v| (stop test); vv /= |v|
-      copy_f64(c, vv.get(), pp.get(), p);
       if (a > 0) {
-        gemm_f64(c, true, false, a, 1, p, 1.0, M->V.get(), p, nullptr, pp.get(), p, nullptr, 0.0,
-                 cv.get(), a);
-        gemm_f64(c, false, false, p, 1, a, -1.0, M->V.get(), p, nullptr, cv.get(), a, nullptr,
-                 1.0, vv.get(), p);
+        const DMat Va = dview(M->V.get(), p, a, p);
+        gemv_t(c, Va, nullptr, pp.get(), cv.get());
+        gemv_n(c, Va, nullptr, cv.get(), vc.get());
       }
-      launch(c, "simpls_normalize", k_sp_normalize, dim3(1), dim3(SP_NT), 0,
-             (const double*)vv.get(), vv.get(), p, (double*)nullptr, (int64_t)0, fl);
+      sp_normalize(c, pp.get(), a > 0 ? vc.get() : nullptr, vv.get(), p, nullptr, 0, npart.get(),
+                   fl);
       // S -= vv (vv^T S)
-      gemm_f64(c, true, false, 1, m, p, 1.0, vv.get(), p, nullptr, S.get(), p, nullptr, 0.0,
-               vS.get(), 1);
+      gemv_t(c, Sv, nullptr, vv.get(), vS.get());
       launch(c, "simpls_deflate", k_sp_rank1, dim3(g1(p * m)), dim3(256), 0, S.get(), p, p, m,
              (const double*)vv.get(), (const double*)vS.get(), -1.0, (const int32_t*)fl);
-      launch(c, "simpls_commit", k_sp_commit, dim3(g1(p * m)), dim3(256), 0, a,
+      // the large path forms B at the requested orders from R and Q instead
+      // of a running sum (no per-component pass over a p x m matrix)
+      launch(c, "simpls_commit", k_sp_commit, dim3(g1(std::max(p, m))), dim3(256), 0, a,
              (const double*)rr.get(), (const double*)qq.get(), (const double*)vv.get(), p, m,
-             M->R.get(), M->Q.get(), M->V.get(), Bcur.get(), fl);
+             M->R.get(), M->Q.get(), M->V.get(), (double*)nullptr, fl);
     }
     while (i_out < L && a == nc[i_out] - 1) {
-      if (M->b_stored)
+      if (M->b_stored) {
+        // B_a = R[:, :a+1] Q[:, :a+1]^T, stored unless the fit has stopped
+        if (!small)
+          gemm_f64(c, false, true, p, m, a + 1, 1.0, M->R.get(), p, nullptr, M->Q.get(), m,
+                   nullptr, 0.0, Bcur.get(), p);
         launch(c, "simpls_store", k_sp_copy, dim3(g1(p * m)), dim3(256), 0,
                M->B.get() + (size_t)i_out * p * m, (const double*)Bcur.get(), p * m, (int64_t)1,
                (const double*)nullptr, (const int32_t*)fl);
+      }
       if (o.fit) {
         // reference quirk kept: Yfit_cur only accumulates at requested ncomp
         launch(c, "simpls_yfit", k_sp_rank1, dim3(g1(n * m)), dim3(256), 0, Yfit_cur.get(), n, n,
