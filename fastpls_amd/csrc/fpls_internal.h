@@ -350,23 +350,26 @@ bool label_counts_prop(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, double*
                        double* counts, int32_t* bad);
 
 // ---------------------------------------------------------------- kernels (k_dense.hip)
-// Symmetric eigen-decomposition (one workgroup, cyclic parallel Jacobi, FP64),
-// n <= kJacobiMax.  Eigenvalues descending in w, eigenvectors in V (ld n).
+// Symmetric eigen-decomposition (FP64), n <= kJacobiMax: tridiagonal path
+// with the divide-and-conquer solver as fallback.  Eigenvalues descending in
+// w, eigenvectors in V (ld n).
 constexpr int kJacobiMax = 128;
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
-// count independent problems (order ns_dev[b] <= nmax, A + b strideA with ld
-// lda), one workgroup each; eigenvalues descending at w + b strideW,
-// eigenvectors at V + b strideV (ld = the problem's order).
+// count independent problems (order ns_dev[b] <= nmax, or n0 for every
+// problem when ns_dev is null; A + b strideA with ld lda); eigenvalues
+// descending at w + b strideW, eigenvectors at V + b strideV (ld = the
+// problem's order).
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                      int nmax, int count, double* w, int64_t strideW, double* V,
-                     int64_t strideV);
-// Eigenpairs of the symmetric tridiagonal (tri: d, e, tau, scale of a
-// one-workgroup Householder reduction, reflectors in refl, ld n) by
-// multisection bisection and inverse iteration, one wave per eigenvalue
-// (k_eig_ii.hip): eigenvalues descending in w, eigenvectors back-transformed
-// into V (ld n).  Sets *fallback when a cluster is wider than it takes.
-void tridiag_eig_ii(Ctx* ctx, int n, const double* tri, const double* refl, double* w, double* V,
-                    int* fallback);
+                     int64_t strideV, int n0 = 0);
+// The tridiagonal path of sym_eig_batched (k_eig_ii.hip): Householder
+// reduction with the matrix in registers, bisection and twisted-factorisation
+// eigenvectors one workgroup per eigenvalue, Gram-Schmidt inside clusters.
+// flags[b] (device) is set for a problem whose clusters or residuals need the
+// divide-and-conquer solver.
+void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
+                 int n0, int nmax, int count, double* w, int64_t strideW, double* V,
+                 int64_t strideV, int* flags);
 // Any order: block Jacobi over pairs of <= 64-row blocks, each pair
 // diagonalised by the one-workgroup solver (k_jacobi.hip).  Same outputs as
 // sym_eig (eigenvalues descending in w, eigenvectors in V, ld n).

@@ -72,7 +72,7 @@ __device__ __forceinline__ int bound(const double* x, int lo, int hi, double v, 
 // 3 nmax^2 doubles at ws + b strideWs).
 // tri_out (when set): stop after stage 1 and write d (n), e (n), tau (n) and
 // the scale of the problem at tri_out; the reflectors stay in the workspace
-// (Vr, ld n).  run_flag (when set): return at once unless *run_flag != 0.
+// (Vr, ld n).  run_flag (when set): return at once unless run_flag[b] != 0.
 // zero_flag (when set) is cleared (the fallback flag the later kernels set).
 __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ Ain0, int64_t lda,
                                                     int64_t strideA, const int* __restrict__ ns,
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
                                                     double* __restrict__ tri_out,
                                                     const int* __restrict__ run_flag,
                                                     int* __restrict__ zero_flag) {
-  if (run_flag && *run_flag == 0) return;
+  if (run_flag && run_flag[blockIdx.x] == 0) return;
   if (zero_flag && threadIdx.x == 0 && blockIdx.x == 0) *zero_flag = 0;
   const int64_t bi = blockIdx.x;
   const int n = ns ? ns[bi] : n0;
@@ -696,39 +696,28 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
 }
 
 void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
-  FPLS_REQUIRE(n >= 1 && n <= kJacobiMax, FPLS_ERR_ARG,
-               "sym_eig: order must be in 1..128 for the one-workgroup solver");
-  DBuf<double> ws(ctx, (size_t)3 * n * n + 4 * n + 8);
-  if (n < 3) {
-    launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
-           (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
-           (double*)nullptr, (const int*)nullptr, (int*)nullptr);
-    return;
-  }
-  // tridiagonalisation on one workgroup, then every eigenpair by bisection
-  // and inverse iteration on its own wave (k_eig_ii.hip); spectra with a
-  // cluster wider than the inverse iteration takes fall back to divide and
-  // conquer (the flag at the end of the workspace)
-  double* tri = ws.get() + (size_t)3 * n * n;
-  int* flag = reinterpret_cast<int*>(tri + 4 * n + 4);
-  launch(ctx, "sym_eig_trd", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
-         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0, tri,
-         (const int*)nullptr, flag);
-  tridiag_eig_ii(ctx, n, tri, ws.get() + (size_t)2 * n * n, w, V, flag);
-  launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3(1), dim3(ENT), 0, A, lda, (int64_t)0,
-         (const int*)nullptr, n, n, w, (int64_t)0, V, (int64_t)0, ws.get(), (int64_t)0,
-         (double*)nullptr, (const int*)flag, (int*)nullptr);
+  sym_eig_batched(ctx, A, lda, 0, nullptr, n, 1, w, 0, V, 0, n);
 }
 
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                      int nmax, int count, double* w, int64_t strideW, double* V,
-                     int64_t strideV) {
+                     int64_t strideV, int n0) {
   FPLS_REQUIRE(nmax >= 1 && nmax <= kJacobiMax && count >= 1, FPLS_ERR_ARG,
                "sym_eig_batched: orders must be in 1..128");
+  if (!ns_dev && n0 == 0) n0 = nmax;
+  if (!ns_dev) FPLS_REQUIRE(n0 >= 1 && n0 <= nmax, FPLS_ERR_ARG, "sym_eig: order out of range");
+  // tridiagonalisation, then every eigenpair from bisection and the twisted
+  // factorisation on its own workgroup (k_eig_ii.hip); a problem whose
+  // spectrum the inverse-free vectors cannot resolve (flag set) is solved
+  // again by divide and conquer
+  DBuf<int> flags(ctx, (size_t)count);
+  const int nfix = ns_dev ? nmax : n0;
+  sym_eig_tri(ctx, A, lda, strideA, ns_dev, nfix, nmax, count, w, strideW, V, strideV,
+              flags.get());
   DBuf<double> ws(ctx, (size_t)3 * nmax * nmax * count);
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda, strideA,
-         ns_dev, nmax, nmax, w, strideW, V, strideV, ws.get(), (int64_t)3 * nmax * nmax,
-         (double*)nullptr, (const int*)nullptr, (int*)nullptr);
+         ns_dev, nfix, nmax, w, strideW, V, strideV, ws.get(),
+         (int64_t)3 * nmax * nmax, (double*)nullptr, (const int*)flags.get(), (int*)nullptr);
 }
 
 }  // namespace fpls

@@ -1,38 +1,56 @@
-// Eigenpairs of a symmetric tridiagonal matrix T (order n <= 128, from the
-// one-workgroup Householder reduction of k_eig.hip) with one wave per
-// eigenvalue, spread over the chip instead of one workgroup:
+// Symmetric eigenpairs of orders <= 128 (Gram matrices of the cross-covariance
+// and of rSVD sketches, Jacobi pair blocks), batched, in three launches:
 //
-//  1. eigenvalues by multisection: each wave brackets eigenvalue k between
-//     points where the Sturm count (number of eigenvalues below x, from the
-//     LDL^T recurrence q_i = d_i - x - e_{i-1}^2 / q_{i-1}) is <= k and > k;
-//     its 64 lanes evaluate 64 interior points per step, so the bracket
-//     shrinks 65-fold per step (Barth, Martin & Wilkinson 1967, bisection on
-//     the Sturm sequence; the pivmin guard and the Gershgorin start of
-//     LAPACK's dstebz);
-//  2. eigenvectors by inverse iteration (LAPACK's dstein: the partially
-//     pivoted factorisation of T - lambda I and the perturbed solve of
-//     dlagtf / dlagts, three solves per vector, eigenvalues closer than
-//     1e-3 |T| form a cluster whose vectors are orthogonalised against each
-//     other by one wave in turn), normalised with the largest entry positive;
-//  3. the back-transformation by the Householder reflectors, per vector.
-//
-// Orthogonality between clusters is then within eps |T| / gap <= 2e-13; a
-// cluster of more than CL_MAX eigenvalues sets *fallback for the divide and
-// conquer solver (k_eig.hip), which handles any clustering.
+//  1. k_trd_reg: Householder reduction to tridiagonal form T = Q^T A Q
+//     (Golub & Van Loan, Algorithm 8.3.1), one workgroup of 256 threads per
+//     matrix with the whole matrix in registers: thread (tr, tc) of a 16 x 16
+//     grid holds the cyclic 8 x 8 set A[tr + 16 i][tc + 16 q], so the live
+//     trailing block stays spread over every thread as it shrinks.  Per column
+//     the owners publish the column to LDS, every wave forms the reflector,
+//     p = tau A v is reduced over the 16 lanes of a thread row by DPP moves
+//     (reduce-scatter: each lane ends with one row's total), and the rank-2
+//     update A -= v w^T + w v^T runs on the registers: two barriers a column.
+//  2. k_trd_pairs: one workgroup per eigenvalue k of T: multisection on the
+//     Sturm count (number of eigenvalues below x from the LDL^T recurrence
+//     q_i = d_i - x - e_{i-1}^2 / q_{i-1}; Barth, Martin & Wilkinson 1967,
+//     with the pivmin guard and Gershgorin start of LAPACK's dstebz), 512
+//     points per step; then the eigenvector from the twisted factorisation
+//     of T - lambda I (top-down L D L^T and bottom-up U D U^T pivots on two
+//     lanes at once, twist index r = argmin |gamma_r|, z_r = 1 and the two
+//     product recurrences outward; Fernando 1997, Parlett & Dhillon 2000),
+//     normalised with its largest entry positive, checked by its residual,
+//     and back-transformed by the reflectors on one wave.
+//  3. k_trd_clorth: eigenvalues closer than 1e-3 |T| form a cluster whose
+//     vectors are made orthogonal by modified Gram-Schmidt (one wave per
+//     cluster); a cluster wider than CL_MAX or tighter than 1e-9 |T|, a
+//     non-finite vector or a residual above the bound sets the problem's flag,
+//     and the divide-and-conquer solver (k_eig.hip) then recomputes that
+//     problem.
 #include <cfloat>
 #include <cmath>
 
 #include "fpls_internal.h"
 #include "fpls_wave.h"
 
+// Timing hook for stage-level measurements; compiled out unless a probe build
+// defines it (it then records a clock value at each call site).
+#ifndef FPLS_TRD_STAMP
+#define FPLS_TRD_STAMP(slot)
+#endif
+
 namespace fpls {
 
 namespace {
 
 constexpr int II_N = 128;   // largest order
-constexpr int II_W = 4;     // waves per workgroup
-constexpr int CL_MAX = 8;   // largest cluster inverse iteration takes
+constexpr int CL_MAX = 8;   // largest cluster the Gram-Schmidt pass takes
 constexpr double II_EPS = DBL_EPSILON * 0.5;  // unit roundoff (dlamch('E'))
+
+constexpr int TD_T = 256;  // tridiagonalisation: 16 x 16 thread grid
+constexpr int TD_R = 8;    // 8 x 8 cyclic register set per thread
+
+// DPP lane moves inside a 16-lane row (row_ror:8 pairs lane i with i ^ 8)
+constexpr int DPP_ROR8 = 0x128;
 
 __device__ __forceinline__ double wmin(double v) { return -wave_max_d(-v); }
 
@@ -46,27 +64,8 @@ __device__ __forceinline__ double rcp2(double d) {
   return fma(y, e, y);
 }
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-}
-
-// Number of eigenvalues of T below x (LDL^T signs, with the pivmin guard).
-__device__ __forceinline__ int sturm_count(const double* d, const double* ee, int n, double x,
-                                           double pivmin) {
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  int c = q < 0.0 ? 1 : 0;
-  for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - ee[i - 1] * rcp2(q);
-    if (fabs(q) < pivmin) q = -pivmin;
-    c += q < 0.0 ? 1 : 0;
-  }
-  return c;
-}
-
-// Gershgorin interval, |T| bound and pivmin shared by both kernels (every
-// lane of a wave gets the same values).
+// Gershgorin interval, |T| bounds and pivmin (every lane of a wave gets the
+// same values).
 struct TriBounds {
   double gl, gu, tnorm, pivmin, onenrm;
 };
@@ -92,243 +91,452 @@ __device__ TriBounds tri_bounds(const double* d, const double* e, const double* 
   return b;
 }
 
-// ---------------------------------------------------------------- eigenvalues
-// One workgroup per eigenvalue k: its 4 waves evaluate 256 interior points of
-// the bracket per step (257-fold shrink), combined through LDS ballots.
-constexpr int BI_T = 256, BI_PTS = BI_T;
-__global__ __launch_bounds__(BI_T) void k_trd_eigval(int n, const double* __restrict__ tri,
-                                                     double* __restrict__ lam) {
+// ---------------------------------------------------------------- stage 1
+// Shared state of the column loop (LDS buffers and the thread's roles).
+struct TrdState {
+  double (*xs)[II_N];  // published column, by parity of j
+  double* ps;          // tau A v
+  double* tri;
+  double* refl;
+  int n, t, tr, tc, lane, wv;
+  int hb3, hb2, hb1, hb0, slot;
+};
+
+// Columns j in [16 jb, jend) with L = 8 - jb live register rows / columns:
+// register (i, q) holds A[tr + 16 (i + jb)][tc + 16 (q + jb)].  Rows and
+// columns at or past n hold zeros throughout (their v and w are zero), so
+// only the diagonal position relative to j needs a test.
+template <int L>
+__device__ __forceinline__ void trd_block(double (&a)[TD_R][TD_R], int jb, int jend,
+                                          const TrdState& S) {
+  const int n = S.n, tr = S.tr, tc = S.tc, lane = S.lane;
+  for (int j = 16 * jb; j < jend; ++j) {
+    double* __restrict__ x = S.xs[j & 1];
+    if (j == 20) FPLS_TRD_STAMP(2);
+    // column j from its owners (column register 0 of the threads with
+    // tc == j % 16); rows below j are never read back
+    if (tc == (j & 15)) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) x[tr + 16 * (i + jb)] = a[i][0];
+    }
+    __syncthreads();
+    if (j == 20) FPLS_TRD_STAMP(3);
+    const double alpha = x[j + 1];
+    double s = 0.0;
+    {
+      const int r0 = j + 2 + lane, r1 = r0 + 64;
+      if (r0 < n) s = x[r0] * x[r0];
+      if (r1 < n) s = fma(x[r1], x[r1], s);
+    }
+    const double xn2 = wave_sum_d(s);
+    double beta = alpha, tau = 0.0, sv = 0.0;
+    if (xn2 > 0.0) {
+      beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
+      tau = (beta - alpha) / beta;
+      sv = 1.0 / (alpha - beta);
+    }
+    if (S.t == 0) {
+      S.tri[j] = x[j];
+      S.tri[n + j] = beta;
+      S.tri[2 * n + j] = tau;
+    }
+    if (S.wv == 0) {
+      for (int r = lane; r < n; r += 64)
+        S.refl[r + (int64_t)j * n] = (r == j + 1) ? 1.0 : (r > j + 1 ? x[r] * sv : 0.0);
+    }
+    if (j == 20) FPLS_TRD_STAMP(4);
+    if (tau == 0.0) continue;  // uniform: H_j = I leaves A unchanged
+    double vr[L], vc[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
+      const double xr = x[r], xc = x[c];
+      vr[i] = (r == j + 1) ? 1.0 : (r > j + 1 ? xr * sv : 0.0);
+      vc[i] = (c == j + 1) ? 1.0 : (c > j + 1 ? xc * sv : 0.0);
+    }
+    double pr[TD_R];
+#pragma unroll
+    for (int i = 0; i < TD_R; ++i) {
+      double acc = 0.0;
+      if (i < L) {
+#pragma unroll
+        for (int q = 0; q < L; ++q) acc = fma(a[i][q], vc[q], acc);
+      }
+      pr[i] = acc;
+    }
+    // reduce-scatter over the 16 lanes of the thread row: partners by
+    // row_ror:8 (bit 3), row_half_mirror (bit 2), quad_perm xor 2 (bit 1),
+    // quad_perm xor 1 (bit 0); each exchange sends the half the partner keeps
+    double h4[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double keep = S.hb3 ? pr[4 + u] : pr[u];
+      const double send = S.hb3 ? pr[u] : pr[4 + u];
+      h4[u] = keep + dpp_mov<DPP_ROR8>(send);
+    }
+    double h2[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const double keep = S.hb2 ? h4[2 + u] : h4[u];
+      const double send = S.hb2 ? h4[u] : h4[2 + u];
+      h2[u] = keep + dpp_mov<dpp::HALF_MIRROR>(send);
+    }
+    double h1;
+    {
+      const double keep = S.hb1 ? h2[1] : h2[0];
+      const double send = S.hb1 ? h2[0] : h2[1];
+      h1 = keep + dpp_mov<dpp::QUAD_XOR2>(send);
+    }
+    h1 += dpp_mov<dpp::QUAD_XOR1>(h1);
+    if (j == 20) FPLS_TRD_STAMP(5);
+    if (S.hb0 == 0 && S.slot < L) S.ps[tr + 16 * (S.slot + jb)] = tau * h1;
+    __syncthreads();
+    if (j == 20) FPLS_TRD_STAMP(6);
+    // K = tau / 2 p^T v over rows j + 1 .. n - 1 (ps holds tau p)
+    double sk = 0.0;
+    {
+      const int r0 = j + 1 + lane, r1 = r0 + 64;
+      if (r0 < n) sk = S.ps[r0] * ((r0 == j + 1) ? 1.0 : x[r0] * sv);
+      if (r1 < n) sk = fma(S.ps[r1], x[r1] * sv, sk);
+    }
+    const double K = 0.5 * tau * wave_sum_d(sk);
+    double wr[L], wc[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
+      const double pr_ = S.ps[r], pc_ = S.ps[c];
+      wr[i] = r > j ? fma(-K, vr[i], pr_) : 0.0;
+      wc[i] = c > j ? fma(-K, vc[i], pc_) : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+#pragma unroll
+      for (int q = 0; q < L; ++q) a[i][q] = fma(-vr[i], wc[q], fma(-wr[i], vc[q], a[i][q]));
+    if (j == 20) FPLS_TRD_STAMP(7);
+  }
+}
+
+// tri (3 n + 1 doubles): d, e (e[n-1] = 0), tau, then the scale (max |A|) the
+// matrix was divided by.  refl (n x n, ld n): column j is reflector j's vector
+// v_j (zero above row j + 1, v_j(j + 1) = 1); columns n - 2, n - 1 are zero.
+// flags[b] is cleared for the later stages.
+__global__ __launch_bounds__(TD_T) void k_trd_reg(const double* __restrict__ A0, int64_t lda,
+                                                  int64_t strideA, const int* __restrict__ ns,
+                                                  int n0, double* __restrict__ tri0,
+                                                  int64_t strideTri, double* __restrict__ refl0,
+                                                  int64_t strideRefl, int* __restrict__ flags) {
+  const int64_t bi = blockIdx.x;
+  const int n = ns ? ns[bi] : n0;
+  const double* __restrict__ A = A0 + bi * strideA;
+  double* __restrict__ tri = tri0 + bi * strideTri;
+  double* __restrict__ refl = refl0 + bi * strideRefl;
+  __shared__ double xs[2][II_N];
+  __shared__ double ps[II_N];
+  __shared__ double red[TD_T / 64];
+  __shared__ double tail[3];
+  const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, wv = t >> 6;
+  if (t == 0 && flags) flags[bi] = 0;
+  if (n < 1 || n > II_N) return;
+  FPLS_TRD_STAMP(0);
+
+  // symmetric part of A, cyclic register set, and its largest entry
+  double a[TD_R][TD_R];
+  double mx = 0.0;
+#pragma unroll
+  for (int i = 0; i < TD_R; ++i)
+#pragma unroll
+    for (int q = 0; q < TD_R; ++q) {
+      const int r = tr + 16 * i, c = tc + 16 * q;
+      double x = 0.0;
+      if (r < n && c < n) x = 0.5 * (A[r + (int64_t)c * lda] + A[c + (int64_t)r * lda]);
+      a[i][q] = x;
+      mx = fmax(mx, fabs(x));
+    }
+  mx = wave_max_d(mx);
+  if (lane == 0) red[wv] = mx;
+  __syncthreads();
+  const double mall = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  const double scale = (mall > 0.0 && isfinite(mall)) ? mall : 1.0;
+  const double iscale = 1.0 / scale;
+#pragma unroll
+  for (int i = 0; i < TD_R; ++i)
+#pragma unroll
+    for (int q = 0; q < TD_R; ++q) a[i][q] *= iscale;
+  if (t == 0) tri[3 * n] = scale;
+
+  TrdState S;
+  S.xs = xs;
+  S.ps = ps;
+  S.tri = tri;
+  S.refl = refl;
+  S.n = n;
+  S.t = t;
+  S.tr = tr;
+  S.tc = tc;
+  S.lane = lane;
+  S.wv = wv;
+  S.hb3 = (tc >> 3) & 1;
+  S.hb2 = (tc >> 2) & 1;
+  S.hb1 = (tc >> 1) & 1;
+  S.hb0 = tc & 1;
+  S.slot = S.hb3 * 4 + S.hb2 * 2 + S.hb1;
+
+  FPLS_TRD_STAMP(1);
+  // reflectors j = 0 .. n - 3, sixteen columns per register block
+  int jb = 0;
+  for (;;) {
+    const int jend = min(16 * jb + 16, n - 2);
+    if (16 * jb < jend) {
+      switch (jb) {
+        case 0: trd_block<8>(a, jb, jend, S); break;
+        case 1: trd_block<7>(a, jb, jend, S); break;
+        case 2: trd_block<6>(a, jb, jend, S); break;
+        case 3: trd_block<5>(a, jb, jend, S); break;
+        case 4: trd_block<4>(a, jb, jend, S); break;
+        case 5: trd_block<3>(a, jb, jend, S); break;
+        case 6: trd_block<2>(a, jb, jend, S); break;
+        default: trd_block<1>(a, jb, jend, S); break;
+      }
+    }
+    if (16 * (jb + 1) >= n - 2) break;
+    // block jb of rows and columns is finished: drop it
+#pragma unroll
+    for (int i = 0; i < TD_R - 1; ++i)
+#pragma unroll
+      for (int q = 0; q < TD_R - 1; ++q) a[i][q] = a[i + 1][q + 1];
+#pragma unroll
+    for (int i = 0; i < TD_R; ++i) {
+      a[i][TD_R - 1] = 0.0;
+      a[TD_R - 1][i] = 0.0;
+    }
+    ++jb;
+  }
+  // the last 2 x 2 block (or the 1 x 1 matrix): d[n-2], e[n-2], d[n-1]
+  {
+    const int live = TD_R - jb;
+#pragma unroll
+    for (int i = 0; i < TD_R; ++i)
+#pragma unroll
+      for (int q = 0; q < TD_R; ++q) {
+        if (i < live && q < live) {
+          const int r = tr + 16 * (i + jb), c = tc + 16 * (q + jb);
+          if (n >= 2 && r == n - 2 && c == n - 2) tail[0] = a[i][q];
+          if (n >= 2 && r == n - 1 && c == n - 2) tail[1] = a[i][q];
+          if (r == n - 1 && c == n - 1) tail[2] = a[i][q];
+        }
+      }
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (n >= 2) {
+      tri[n - 2] = tail[0];
+      tri[n + n - 2] = tail[1];
+      tri[2 * n + n - 2] = 0.0;
+    }
+    tri[n - 1] = tail[2];
+    tri[n + n - 1] = 0.0;
+    tri[2 * n + n - 1] = 0.0;
+  }
+  for (int idx = t; idx < 2 * n; idx += TD_T) {
+    const int j = n - 2 + idx / n, r = idx % n;
+    if (j >= 0) refl[r + (int64_t)j * n] = 0.0;
+  }
+  FPLS_TRD_STAMP(8);
+}
+
+// ---------------------------------------------------------------- stage 2
+constexpr int PR_T = 256;              // threads per eigenvalue
+constexpr int PR_PTS = 2 * PR_T;       // multisection points per step
+
+// Number of eigenvalues of T below x0 and below x1 (two interleaved chains).
+__device__ __forceinline__ void sturm_count2(const double* d, const double* ee, int n, double x0,
+                                             double x1, double pivmin, int& c0, int& c1) {
+  double q0 = d[0] - x0, q1 = d[0] - x1;
+  if (fabs(q0) < pivmin) q0 = -pivmin;
+  if (fabs(q1) < pivmin) q1 = -pivmin;
+  int n0 = q0 < 0.0 ? 1 : 0, n1 = q1 < 0.0 ? 1 : 0;
+  for (int i = 1; i < n; ++i) {
+    const double di = d[i], ei = ee[i - 1];
+    q0 = (di - x0) - ei * rcp2(q0);
+    q1 = (di - x1) - ei * rcp2(q1);
+    if (fabs(q0) < pivmin) q0 = -pivmin;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    n0 += q0 < 0.0 ? 1 : 0;
+    n1 += q1 < 0.0 ? 1 : 0;
+  }
+  c0 = n0;
+  c1 = n1;
+}
+
+// lam: ascending eigenvalues of the scaled T (stride nmax per problem);
+// w / V: descending eigenvalues times the scale and the back-transformed
+// eigenvectors (ld n); flags[b] set when a vector fails its checks.
+__global__ __launch_bounds__(PR_T) void k_trd_pairs(
+    const int* __restrict__ ns, int n0, const double* __restrict__ tri0, int64_t strideTri,
+    const double* __restrict__ refl0, int64_t strideRefl, double* __restrict__ w0,
+    int64_t strideW, double* __restrict__ V0, int64_t strideV, double* __restrict__ lam0,
+    int64_t strideLam, int* __restrict__ flags) {
+  const int64_t bi = blockIdx.y;
+  const int n = ns ? ns[bi] : n0;
+  const int k = blockIdx.x;
+  if (k >= n || n > II_N) return;
+  const double* __restrict__ tri = tri0 + bi * strideTri;
+  const double* __restrict__ refl = refl0 + bi * strideRefl;
+  double* __restrict__ w = w0 + bi * strideW;
+  double* __restrict__ V = V0 + bi * strideV;
+  double* __restrict__ lam = lam0 + bi * strideLam;
   __shared__ double d[II_N], e[II_N], ee[II_N];
-  __shared__ unsigned long long above_w[BI_T / 64];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  __shared__ double fac[2][II_N];
+  __shared__ double zv[II_N];
+  __shared__ double R[II_N * II_N];  // the reflectors, staged for the back-transformation
+  __shared__ unsigned long long above_w[PR_T / 64][2];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int i = t; i < n; i += PR_T) {
     d[i] = tri[i];
     e[i] = tri[n + i];
     ee[i] = tri[n + i] * tri[n + i];
   }
+  if (n >= 3) {
+    const int nr = n * (n - 2);  // columns 0 .. n - 3
+    for (int base = t; base < nr; base += 4 * PR_T) {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = base + u * PR_T;
+        v[u] = idx < nr ? refl[idx] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int idx = base + u * PR_T;
+        if (idx < nr) R[idx] = v[u];
+      }
+    }
+  }
   __syncthreads();
-  const int k = blockIdx.x, t = threadIdx.x, wv = t >> 6;
+  FPLS_TRD_STAMP(16);
+  const double scale = tri[3 * n];
   const TriBounds b = tri_bounds(d, e, ee, n);
-  // count(lo) <= k < count(hi)
+
+  // ---- eigenvalue k (ascending): count(lo) <= k < count(hi)
   double lo = b.gl, hi = b.gu;
   for (int it = 0; it < 12; ++it) {
     const double width = hi - lo;
-    // dstebz's tolerance: absolute eps |T|, relative 2 eps, at least pivmin
     const double tol = fmax(II_EPS * b.tnorm,
                             fmax(b.pivmin, 2.0 * II_EPS * fmax(fabs(lo), fabs(hi))));
-    if (width <= tol) break;  // uniform over the workgroup
-    const double step = width / (double)(BI_PTS + 1);
-    const double x = lo + step * (double)(t + 1);
-    const int c = sturm_count(d, ee, n, x, b.pivmin);
-    const unsigned long long above = __ballot(c > k);
-    if ((t & 63) == 0) above_w[wv] = above;
-    __syncthreads();
-    // first point with more than k eigenvalues below it; the one before it
-    int f = BI_PTS;
-    for (int q = 0; q < BI_T / 64; ++q) {
-      const unsigned long long m = above_w[q];
-      if (m) {
-        f = 64 * q + __ffsll((long long)m) - 1;
-        break;
-      }
+    if (!(width > tol)) break;  // uniform over the workgroup
+    const double step = width / (double)(PR_PTS + 1);
+    // thread t evaluates points t + 1 and t + 1 + PR_T
+    const double xa = lo + step * (double)(t + 1);
+    const double xb = lo + step * (double)(t + 1 + PR_T);
+    int ca, cb;
+    sturm_count2(d, ee, n, xa, xb, b.pivmin, ca, cb);
+    const unsigned long long ma = __ballot(ca > k), mb = __ballot(cb > k);
+    if (lane == 0) {
+      above_w[wv][0] = ma;
+      above_w[wv][1] = mb;
     }
     __syncthreads();
-    const double xf = f < BI_PTS ? lo + step * (double)(f + 1) : hi;
-    const double xb = f > 0 ? lo + step * (double)f : lo;
-    lo = xb;
+    // first point (in point order) with more than k eigenvalues below it
+    int f = PR_PTS;
+    for (int h = 0; h < 2 && f == PR_PTS; ++h)
+      for (int q = 0; q < PR_T / 64; ++q) {
+        const unsigned long long m = above_w[q][h];
+        if (m) {
+          f = h * PR_T + 64 * q + __ffsll((long long)m) - 1;
+          break;
+        }
+      }
+    __syncthreads();
+    const double xf = f < PR_PTS ? lo + step * (double)(f + 1) : hi;
+    const double xb0 = f > 0 ? lo + step * (double)f : lo;
+    lo = xb0;
     hi = xf;
   }
-  if (t == 0) lam[k] = 0.5 * (lo + hi);
-}
+  const double lambda = 0.5 * (lo + hi);
+  FPLS_TRD_STAMP(17);
+  if (wv != 0) return;  // one wave finishes the vector
+  if (lane == 0) lam[k] = lambda;
 
-// ---------------------------------------------------------------- eigenvectors
-// Cheap deterministic start vector in (-1, 1) for eigenvalue index k.
-__device__ __forceinline__ double start_entry(int k, int i) {
-  unsigned h = (unsigned)(k * 7919 + i * 104729 + 12345);
-  h ^= h >> 16;
-  h *= 0x7feb352du;
-  h ^= h >> 15;
-  h *= 0x846ca68bu;
-  h ^= h >> 16;
-  return (double)(h >> 8) * (2.0 / 16777216.0) - 1.0;
-}
-
-__global__ __launch_bounds__(64 * II_W) void k_trd_eigvec(int n, const double* __restrict__ tri,
-                                                          const double* __restrict__ lam,
-                                                          const double* __restrict__ refl,
-                                                          double* __restrict__ w,
-                                                          double* __restrict__ V,
-                                                          int* __restrict__ fallback) {
-  __shared__ double d[II_N], e[II_N], ee[II_N], tau[II_N];
-  __shared__ double fa[II_W][II_N], fb[II_W][II_N], fc[II_W][II_N], fd[II_W][II_N];
-  __shared__ double xv[II_W][II_N];
-  __shared__ int fin[II_W][II_N];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    d[i] = tri[i];
-    e[i] = tri[n + i];
-    ee[i] = tri[n + i] * tri[n + i];
-    tau[i] = tri[2 * n + i];
-  }
-  __syncthreads();
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int k0 = blockIdx.x * II_W + wv;
-  if (k0 >= n) return;
-  const TriBounds b = tri_bounds(d, e, ee, n);
-  const double ortol = 1e-3 * b.onenrm, tight = 1e-9 * b.onenrm;
-  // clusters (gaps <= 1e-3 |T|) are orthogonalised afterwards by their first
-  // member (k_trd_clorth); a tight or wide cluster goes to divide and conquer
-  {
-    int lo = k0, hi = k0;
-    while (lo > 0 && lam[lo] - lam[lo - 1] <= ortol) --lo;
-    while (hi + 1 < n && lam[hi + 1] - lam[hi] <= ortol) ++hi;
-    bool bad = hi - lo + 1 > CL_MAX;
-    for (int q = lo; q < hi && !bad; ++q) bad = lam[q + 1] - lam[q] <= tight;
-    if (bad) {
-      if (lane == 0) *fallback = 1;
-      return;
-    }
-  }
-  const double scale = tri[3 * n];
-  double* a = fa[wv];
-  double* bb = fb[wv];
-  double* c = fc[wv];
-  double* d2 = fd[wv];
-  double* x = xv[wv];
-  int* in = fin[wv];
-  const double dtpcrt = sqrt(0.1 / (double)n);
-  {
-    const int k = k0;
-    const double xj = lam[k];
-    // factorise T - xj I = P L U (dlagtf) on lane 0: the running diagonal
-    // and super-diagonal entries carried in registers, factors to LDS; the
-    // pivot test |c| / scale2 <= |a| / scale1 compared without divisions
-    if (lane == 0) {
-      double aq = d[0] - xj;
-      double bq = n > 1 ? e[0] : 0.0;
-      double scale1 = fabs(aq) + fabs(bq);
-      for (int q = 0; q + 1 < n; ++q) {
-        const double cq = e[q];
-        double a1 = d[q + 1] - xj;
-        const double bq1 = q + 2 < n ? e[q + 1] : 0.0;
-        const double scale2 = fabs(cq) + fabs(a1) + fabs(bq1);
-        double aout = aq, bout = bq, cout = 0.0, d2q = 0.0, bnext = bq1;
-        int inq = 0;
-        if (cq == 0.0) {
-          scale1 = scale2;
-        } else if (fabs(cq) * scale1 <= fabs(aq) * scale2) {
-          scale1 = scale2;
-          cout = cq * rcp2(aq);
-          a1 = fma(-cout, bq, a1);
-        } else {
-          inq = 1;
-          const double mult = aq * rcp2(cq);
-          aout = cq;
-          const double temp = a1;
-          a1 = fma(-mult, temp, bq);
-          d2q = bq1;
-          bnext = -mult * bq1;
-          bout = temp;
-          cout = mult;
-        }
-        a[q] = aout;
-        bb[q] = bout;
-        c[q] = cout;
-        d2[q] = d2q;
-        in[q] = inq;
-        aq = a1;
-        bq = bnext;
+  bool bad = false;
+  double z0 = 0.0, z1 = 0.0;  // this lane's entries lane, lane + 64
+  if (n == 1) {
+    z0 = lane == 0 ? 1.0 : 0.0;
+  } else {
+    // top-down (lane 0) and bottom-up (lane 1) pivots of T - lambda I
+    if (lane < 2) {
+      const bool up = lane == 1;
+      int ii = up ? n - 1 : 0;
+      double q = d[ii] - lambda;
+      if (fabs(q) < b.pivmin) q = -b.pivmin;
+      fac[lane][ii] = q;
+      for (int s = 1; s < n; ++s) {
+        ii = up ? n - 1 - s : s;
+        const double e2 = up ? ee[ii] : ee[ii - 1];
+        q = (d[ii] - lambda) - e2 * rcp2(q);
+        if (fabs(q) < b.pivmin) q = -b.pivmin;
+        fac[lane][ii] = q;
       }
-      a[n - 1] = aq;
-      bb[n - 1] = 0.0;
-      c[n - 1] = 0.0;
-      d2[n - 1] = 0.0;
-      in[n - 1] = 0;
     }
-    wave_sync();
-    // perturbation size of dlagts (job -1)
-    double tl = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      tl = fmax(tl, fabs(a[i]));
-      if (i >= 1) tl = fmax(tl, fabs(bb[i - 1]));
-      if (i >= 2) tl = fmax(tl, fabs(d2[i - 2]));
-    }
-    tl = wave_max_d(tl) * II_EPS;
-    if (tl == 0.0) tl = II_EPS;
-    const double an = wave_max_d(lane == 0 ? fabs(a[n - 1]) : 0.0);
-    for (int i = lane; i < n; i += 64) x[i] = start_entry(k, i);
-    wave_sync();
-    int nrmchk = 0;
-    for (int its = 0; its < 5; ++its) {
-      // x *= n |T|_1 max(eps, |u_nn|) / |x|_1
-      double s1 = 0.0;
-      for (int i = lane; i < n; i += 64) s1 += fabs(x[i]);
-      s1 = wave_sum_d(s1);
-      const double scl = (double)n * b.onenrm * fmax(II_EPS, an) / s1;
-      for (int i = lane; i < n; i += 64) x[i] *= scl;
-      wave_sync();
-      if (lane == 0) {
-        // solve (T - xj I) y = x (dlagts, job -1: tiny pivots perturbed);
-        // the last solved entries are carried in registers
-        double cur = x[0];
-        for (int q = 1; q < n; ++q) {
-          const double xq = x[q], cm = c[q - 1];
-          if (in[q - 1] == 0) {
-            x[q - 1] = cur;
-            cur = fma(-cm, cur, xq);
-          } else {
-            x[q - 1] = xq;
-            cur = fma(-cm, xq, cur);
-          }
-        }
-        x[n - 1] = cur;
-        double x1 = 0.0, x2 = 0.0;
-        for (int q = n - 1; q >= 0; --q) {
-          double temp = fma(-d2[q], x2, fma(-bb[q], x1, x[q]));
-          double ak = a[q];
-          double pert = copysign(tl, ak);
-          for (int guard = 0; guard < 200; ++guard) {
-            const double absak = fabs(ak);
-            if (absak < 1.0) {
-              if (absak < DBL_MIN) {
-                if (absak == 0.0 || fabs(temp) * DBL_MIN > absak) {
-                  ak += pert;
-                  pert *= 2.0;
-                  continue;
-                }
-                temp *= 1.0 / DBL_MIN;
-                ak *= 1.0 / DBL_MIN;
-              } else if (fabs(temp) > absak * (1.0 / DBL_MIN)) {
-                ak += pert;
-                pert *= 2.0;
-                continue;
-              }
-            }
-            break;
-          }
-          const double y = temp * rcp2(ak);
-          x[q] = y;
-          x2 = x1;
-          x1 = y;
-        }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    // twist index: smallest |gamma_r|, lowest index on ties
+    double gbest = DBL_MAX;
+    int rbest = n;
+    for (int r = lane; r < n; r += 64) {
+      const double g = fabs(fac[0][r] + fac[1][r] - (d[r] - lambda));
+      if (g < gbest) {
+        gbest = g;
+        rbest = r;
       }
-      wave_sync();
-      double xm = 0.0;
-      for (int i = lane; i < n; i += 64) xm = fmax(xm, fabs(x[i]));
-      xm = wave_max_d(xm);
-      // one more solve after dstein's growth criterion is met (its EXTRA)
-      if (xm < dtpcrt) continue;
-      if (++nrmchk >= 2) break;
     }
-    // unit length, largest entry positive
-    double amax = -1.0, av = 0.0, s2 = 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double og = __shfl_xor(gbest, o, 64);
+      const int orr = __shfl_xor(rbest, o, 64);
+      if (og < gbest || (og == gbest && orr < rbest)) {
+        gbest = og;
+        rbest = orr;
+      }
+    }
+    const int rt = rbest < n ? rbest : 0;  // a NaN gamma everywhere: any index
+    FPLS_TRD_STAMP(18);
+    // z_rt = 1; lane 0 walks up (z_i = -(e_i / D+_i) z_{i+1}), lane 1 down
+    // (z_i = -(e_{i-1} / D-_i) z_{i-1}); when the neighbour is exactly zero the
+    // equation of the row between gives z_i from the entry two steps back
+    if (lane < 2) {
+      const bool down = lane == 1;
+      if (lane == 0) zv[rt] = 1.0;
+      const int steps = down ? n - 1 - rt : rt;
+      double zp = 1.0, zpp = 0.0;
+      for (int s = 1; s <= steps; ++s) {
+        const int i = down ? rt + s : rt - s;
+        const double ei = down ? e[i - 1] : e[i];
+        double z = -(ei * rcp2(fac[lane][i])) * zp;
+        if (zp == 0.0 && s >= 2) {
+          const double efar = down ? e[i - 2] : e[i + 1];
+          z = ei != 0.0 ? -(efar / ei) * zpp : 0.0;
+        }
+        zv[i] = z;
+        zpp = zp;
+        zp = z;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    z0 = lane < n ? zv[lane] : 0.0;
+    z1 = lane + 64 < n ? zv[lane + 64] : 0.0;
+    // unit length, largest entry positive (lowest index on ties)
+    double amax = -1.0, av = 0.0;
     int ai = n;
-    for (int i = lane; i < n; i += 64) {
-      const double v = x[i], u = fabs(v);
-      s2 = fma(v, v, s2);
-      if (u > amax) {
-        amax = u;
-        av = v;
-        ai = i;
-      }
+    if (lane < n && fabs(z0) > amax) {
+      amax = fabs(z0);
+      av = z0;
+      ai = lane;
+    }
+    if (lane + 64 < n && fabs(z1) > amax) {
+      amax = fabs(z1);
+      av = z1;
+      ai = lane + 64;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -340,27 +548,72 @@ __global__ __launch_bounds__(64 * II_W) void k_trd_eigvec(int n, const double* _
         ai = oi;
       }
     }
-    s2 = wave_sum_d(s2);
-    const double inv = (av < 0.0 ? -1.0 : 1.0) / sqrt(s2);
-    for (int i = lane; i < n; i += 64) x[i] *= inv;
-    wave_sync();
-    // tridiagonal eigenvector k (ascending) is output column n - 1 - k
-    const int oc = n - 1 - k;
-    for (int i = lane; i < n; i += 64) V[i + (int64_t)oc * n] = x[i];
-    if (lane == 0) w[oc] = lam[k] * scale;
+    const double nrm2 = wave_sum_d(fma(z0, z0, z1 * z1));
+    if (!(nrm2 > 0.0) || !isfinite(nrm2)) bad = true;
+    const double inv = (av < 0.0 ? -1.0 : 1.0) / sqrt(nrm2);
+    z0 *= inv;
+    z1 *= inv;
+    // residual |(T - lambda I) z|_inf against 1000 n eps |T|
+    zv[lane] = z0;
+    if (lane + 64 < II_N) zv[lane + 64] = z1;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    double res = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      double ri = (d[i] - lambda) * zv[i];
+      if (i > 0) ri = fma(e[i - 1], zv[i - 1], ri);
+      if (i + 1 < n) ri = fma(e[i], zv[i + 1], ri);
+      res = fmax(res, fabs(ri));
+    }
+    res = wave_max_d(res);
+    if (!(res <= 1000.0 * (double)n * II_EPS * fmax(b.tnorm, DBL_MIN))) bad = true;
+    FPLS_TRD_STAMP(19);
+    // back-transformation z <- H_0 ... H_{n-3} z (reflectors from the last)
+    if (n >= 3) {
+      const double* __restrict__ tauv = tri + 2 * n;
+      int j = n - 3;
+      double r0 = lane < n ? R[lane + j * n] : 0.0;
+      double r1 = lane + 64 < n ? R[lane + 64 + j * n] : 0.0;
+      for (; j >= 0; --j) {
+        const int jn = j > 0 ? j - 1 : 0;
+        const double n0v = lane < n ? R[lane + jn * n] : 0.0;
+        const double n1v = lane + 64 < n ? R[lane + 64 + jn * n] : 0.0;
+        const double sj = tauv[j] * wave_sum_d(fma(r0, z0, r1 * z1));
+        z0 = fma(-sj, r0, z0);
+        z1 = fma(-sj, r1, z1);
+        r0 = n0v;
+        r1 = n1v;
+      }
+    }
+  }
+  FPLS_TRD_STAMP(20);
+  const int oc = n - 1 - k;
+  if (lane < n) V[lane + (int64_t)oc * n] = z0;
+  if (lane + 64 < n) V[lane + 64 + (int64_t)oc * n] = z1;
+  if (lane == 0) {
+    w[oc] = lambda * scale;
+    if (bad || !isfinite(lambda)) flags[bi] = 1;
   }
 }
 
+// ---------------------------------------------------------------- stage 3
 // Modified Gram-Schmidt inside each cluster of eigenvalues closer than
 // 1e-3 |T| (one wave per cluster, led by its first member), in ascending
 // eigenvalue order: the members were computed independently from their own
 // eigenvalues, each accurate to eps |T| / gap, and are made orthogonal here.
-__global__ __launch_bounds__(64 * II_W) void k_trd_clorth(int n, const double* __restrict__ tri,
-                                                          const double* __restrict__ lam,
-                                                          double* __restrict__ V,
-                                                          const int* __restrict__ fallback) {
+// Orthogonality between clusters is then within eps |T| / gap <= 2e-13.
+constexpr int CL_W = 4;  // waves per workgroup
+__global__ __launch_bounds__(64 * CL_W) void k_trd_clorth(
+    const int* __restrict__ ns, int n0, const double* __restrict__ tri0, int64_t strideTri,
+    const double* __restrict__ lam0, int64_t strideLam, double* __restrict__ V0,
+    int64_t strideV, int* __restrict__ flags) {
+  const int64_t bi = blockIdx.y;
+  const int n = ns ? ns[bi] : n0;
+  if (n > II_N) return;
+  const double* __restrict__ tri = tri0 + bi * strideTri;
+  const double* __restrict__ lam = lam0 + bi * strideLam;
+  double* __restrict__ V = V0 + bi * strideV;
   __shared__ double d[II_N], e[II_N], ee[II_N];
-  if (*fallback) return;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     d[i] = tri[i];
     e[i] = tri[n + i];
@@ -368,14 +621,20 @@ __global__ __launch_bounds__(64 * II_W) void k_trd_clorth(int n, const double* _
   }
   __syncthreads();
   const int lane = threadIdx.x & 63;
-  const int k0 = blockIdx.x * II_W + (threadIdx.x >> 6);
+  const int k0 = blockIdx.x * CL_W + (threadIdx.x >> 6);
   if (k0 >= n) return;
   const TriBounds b = tri_bounds(d, e, ee, n);
-  const double ortol = 1e-3 * b.onenrm;
+  const double ortol = 1e-3 * b.onenrm, tight = 1e-9 * b.onenrm;
   if (k0 > 0 && lam[k0] - lam[k0 - 1] <= ortol) return;
   int cl = 1;
   while (k0 + cl < n && lam[k0 + cl] - lam[k0 + cl - 1] <= ortol) ++cl;
   if (cl == 1) return;
+  bool badc = cl > CL_MAX;
+  for (int q = k0; q + 1 < k0 + cl && !badc; ++q) badc = lam[q + 1] - lam[q] <= tight;
+  if (badc) {
+    if (lane == 0) flags[bi] = 1;
+    return;
+  }
   for (int m = 1; m < cl; ++m) {
     double* zm = V + (int64_t)(n - 1 - (k0 + m)) * n;
     double u0 = lane < n ? zm[lane] : 0.0, u1 = lane + 64 < n ? zm[lane + 64] : 0.0;
@@ -394,92 +653,24 @@ __global__ __launch_bounds__(64 * II_W) void k_trd_clorth(int n, const double* _
   }
 }
 
-// Back-transformation V <- H_0 ... H_{n-3} V for every eigenvector at once:
-// the reflectors (column j of refl: support j + 1 .. n - 1, v(j + 1) = 1)
-// staged in LDS, one column of V per 8 lanes (16 rows each) in registers,
-// reflectors applied from the last to the first.
-constexpr int BT_T = 128, BT_COLS = BT_T / 8, BT_RPL = II_N / 8;
-__global__ __launch_bounds__(BT_T) void k_trd_backtf(int n, const double* __restrict__ tri,
-                                                     const double* __restrict__ refl,
-                                                     double* __restrict__ V) {
-  __shared__ double R[II_N * II_N];
-  __shared__ double tau[II_N];
-  {
-    // eight loads in flight per thread
-    const int nn = n * n;
-    for (int base = threadIdx.x; base < nn; base += 8 * BT_T) {
-      double v[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int idx = base + u * BT_T;
-        v[u] = idx < nn ? refl[idx] : 0.0;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int idx = base + u * BT_T;
-        if (idx < nn) R[idx] = v[u];
-      }
-    }
-  }
-  for (int i = threadIdx.x; i < n; i += BT_T) tau[i] = tri[2 * n + i];
-  __syncthreads();
-  const int col = blockIdx.x * BT_COLS + (threadIdx.x >> 3), g = threadIdx.x & 7;
-  const bool live = col < n;
-  double z[BT_RPL];
-#pragma unroll
-  for (int ii = 0; ii < BT_RPL; ++ii) {
-    const int r = g + 8 * ii;
-    z[ii] = live && r < n ? V[r + (int64_t)col * n] : 0.0;
-  }
-  // reflector j + 1's entries are loaded while reflector j is applied; a
-  // zero tau applies as a zero update
-  double rv[BT_RPL];
-  int j = n - 3;
-  if (j >= 0) {
-#pragma unroll
-    for (int ii = 0; ii < BT_RPL; ++ii) rv[ii] = g + 8 * ii < n ? R[j * n + g + 8 * ii] : 0.0;
-  }
-  for (; j >= 0; --j) {
-    const double tj = tau[j];
-    double rn[BT_RPL];
-    const int jn = j > 0 ? j - 1 : 0;
-#pragma unroll
-    for (int ii = 0; ii < BT_RPL; ++ii) rn[ii] = g + 8 * ii < n ? R[jn * n + g + 8 * ii] : 0.0;
-    double s4[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int ii = 0; ii < BT_RPL; ++ii) s4[ii & 3] = fma(rv[ii], z[ii], s4[ii & 3]);
-    const double sj = tj * grp8_sum((s4[0] + s4[1]) + (s4[2] + s4[3]));
-#pragma unroll
-    for (int ii = 0; ii < BT_RPL; ++ii) {
-      z[ii] = fma(-rv[ii], sj, z[ii]);
-      rv[ii] = rn[ii];
-    }
-  }
-  if (live) {
-#pragma unroll
-    for (int ii = 0; ii < BT_RPL; ++ii) {
-      const int r = g + 8 * ii;
-      if (r < n) V[r + (int64_t)col * n] = z[ii];
-    }
-  }
-}
-
 }  // namespace
 
-void tridiag_eig_ii(Ctx* ctx, int n, const double* tri, const double* refl, double* w, double* V,
-                    int* fallback) {
-  FPLS_REQUIRE(n >= 1 && n <= II_N, FPLS_ERR_ARG, "tridiag_eig_ii: order must be 1..128");
-  DBuf<double> lam(ctx, n);
-  const unsigned blocks = (unsigned)((n + II_W - 1) / II_W);
-  launch(ctx, "sym_eig_bisect", k_trd_eigval, dim3((unsigned)n), dim3(BI_T), 0, n, tri,
-         lam.get());
-  launch(ctx, "sym_eig_invit", k_trd_eigvec, dim3(blocks), dim3(64 * II_W), 0, n, tri,
-         (const double*)lam.get(), refl, w, V, fallback);
-  launch(ctx, "sym_eig_clorth", k_trd_clorth, dim3(blocks), dim3(64 * II_W), 0, n, tri,
-         (const double*)lam.get(), V, (const int*)fallback);
-  if (n >= 3)
-    launch(ctx, "sym_eig_backtf", k_trd_backtf, dim3((unsigned)((n + BT_COLS - 1) / BT_COLS)),
-           dim3(BT_T), 0, n, tri, refl, V);
+void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
+                 int n0, int nmax, int count, double* w, int64_t strideW, double* V,
+                 int64_t strideV, int* flags) {
+  FPLS_REQUIRE(nmax >= 1 && nmax <= II_N && count >= 1, FPLS_ERR_ARG,
+               "sym_eig_tri: orders must be in 1..128");
+  const int64_t sTri = 3 * (int64_t)nmax + 1, sRefl = (int64_t)nmax * nmax, sLam = nmax;
+  DBuf<double> tri(ctx, (size_t)sTri * count), refl(ctx, (size_t)sRefl * count),
+      lam(ctx, (size_t)sLam * count);
+  launch(ctx, "sym_eig_trd", k_trd_reg, dim3((unsigned)count), dim3(TD_T), 0, A, lda, strideA,
+         ns_dev, n0, tri.get(), sTri, refl.get(), sRefl, flags);
+  launch(ctx, "sym_eig_pairs", k_trd_pairs, dim3((unsigned)nmax, (unsigned)count), dim3(PR_T),
+         0, ns_dev, n0, (const double*)tri.get(), sTri, (const double*)refl.get(), sRefl, w,
+         strideW, V, strideV, lam.get(), sLam, flags);
+  launch(ctx, "sym_eig_clorth", k_trd_clorth,
+         dim3((unsigned)((nmax + CL_W - 1) / CL_W), (unsigned)count), dim3(64 * CL_W), 0, ns_dev,
+         n0, (const double*)tri.get(), sTri, (const double*)lam.get(), sLam, V, strideV, flags);
 }
 
 }  // namespace fpls
