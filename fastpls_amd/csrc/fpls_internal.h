@@ -352,16 +352,21 @@ bool label_counts_prop(Ctx* ctx, const int32_t* y, int64_t n, int32_t C, double*
 // ---------------------------------------------------------------- kernels (k_dense.hip)
 // Symmetric eigen-decomposition (FP64), n <= kJacobiMax: tridiagonal path
 // with the divide-and-conquer solver as fallback.  Eigenvalues descending in
-// w, eigenvectors in V (ld n).
+// w, eigenvectors in V (ld n).  kwant (> 0): only the kwant largest pairs
+// are used by the caller, so only their accuracy decides the fallback (the
+// others are returned too, possibly less orthogonal); <= 0: all of them.
 constexpr int kJacobiMax = 128;
-void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
+void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V,
+             int kwant = 0);
 // count independent problems (order ns_dev[b] <= nmax, or n0 for every
 // problem when ns_dev is null; A + b strideA with ld lda); eigenvalues
 // descending at w + b strideW, eigenvectors at V + b strideV (ld = the
 // problem's order).
+// dc_only: straight to the divide-and-conquer solver (spectra known to be
+// clustered, such as the pair blocks of the block Jacobi).
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                      int nmax, int count, double* w, int64_t strideW, double* V,
-                     int64_t strideV, int n0 = 0);
+                     int64_t strideV, int n0 = 0, int kwant = 0, bool dc_only = false);
 // The tridiagonal path of sym_eig_batched (k_eig_ii.hip): Householder
 // reduction with the matrix in registers, bisection and twisted-factorisation
 // eigenvectors one workgroup per eigenvalue, Gram-Schmidt inside clusters.
@@ -369,11 +374,12 @@ void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, co
 // divide-and-conquer solver.
 void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                  int n0, int nmax, int count, double* w, int64_t strideW, double* V,
-                 int64_t strideV, int* flags);
+                 int64_t strideV, int kwant, int* flags);
 // Any order: block Jacobi over pairs of <= 64-row blocks, each pair
 // diagonalised by the one-workgroup solver (k_jacobi.hip).  Same outputs as
 // sym_eig (eigenvalues descending in w, eigenvectors in V, ld n).
-void sym_eig_large(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V);
+void sym_eig_large(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V,
+                   int kwant = 0);
 // spd_solve_slices for orders above 128, through the eigendecomposition.
 void spd_solve_large(Ctx* ctx, const double* A, int64_t lda, const std::vector<int>& ks,
                      int n_slices, int mode, const double* dvals, double ridge, double* X,

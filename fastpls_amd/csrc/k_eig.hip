@@ -695,13 +695,13 @@ __global__ __launch_bounds__(ENT) void k_sym_eig_dc(const double* __restrict__ A
   FPLS_EIG_STAMP(10);
 }
 
-void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V) {
-  sym_eig_batched(ctx, A, lda, 0, nullptr, n, 1, w, 0, V, 0, n);
+void sym_eig(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V, int kwant) {
+  sym_eig_batched(ctx, A, lda, 0, nullptr, n, 1, w, 0, V, 0, n, kwant);
 }
 
 void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                      int nmax, int count, double* w, int64_t strideW, double* V,
-                     int64_t strideV, int n0) {
+                     int64_t strideV, int n0, int kwant, bool dc_only) {
   FPLS_REQUIRE(nmax >= 1 && nmax <= kJacobiMax && count >= 1, FPLS_ERR_ARG,
                "sym_eig_batched: orders must be in 1..128");
   if (!ns_dev && n0 == 0) n0 = nmax;
@@ -710,11 +710,17 @@ void sym_eig_batched(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, co
   // factorisation on its own workgroup (k_eig_ii.hip); a problem whose
   // spectrum the inverse-free vectors cannot resolve (flag set) is solved
   // again by divide and conquer
-  DBuf<int> flags(ctx, (size_t)count);
   const int nfix = ns_dev ? nmax : n0;
-  sym_eig_tri(ctx, A, lda, strideA, ns_dev, nfix, nmax, count, w, strideW, V, strideV,
-              flags.get());
   DBuf<double> ws(ctx, (size_t)3 * nmax * nmax * count);
+  if (dc_only) {
+    launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda,
+           strideA, ns_dev, nfix, nmax, w, strideW, V, strideV, ws.get(),
+           (int64_t)3 * nmax * nmax, (double*)nullptr, (const int*)nullptr, (int*)nullptr);
+    return;
+  }
+  DBuf<int> flags(ctx, (size_t)count);
+  sym_eig_tri(ctx, A, lda, strideA, ns_dev, nfix, nmax, count, w, strideW, V, strideV, kwant,
+              flags.get());
   launch(ctx, "sym_eig_dc", k_sym_eig_dc, dim3((unsigned)count), dim3(ENT), 0, A, lda, strideA,
          ns_dev, nfix, nmax, w, strideW, V, strideV, ws.get(),
          (int64_t)3 * nmax * nmax, (double*)nullptr, (const int*)flags.get(), (int*)nullptr);

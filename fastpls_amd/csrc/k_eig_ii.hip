@@ -14,12 +14,15 @@
 //     Sturm count (number of eigenvalues below x from the LDL^T recurrence
 //     q_i = d_i - x - e_{i-1}^2 / q_{i-1}; Barth, Martin & Wilkinson 1967,
 //     with the pivmin guard and Gershgorin start of LAPACK's dstebz), 512
-//     points per step; then the eigenvector from the twisted factorisation
+//     points per step, the division by the hardware reciprocal and one Newton
+//     step (about 11 ulp, so the counts are those of a matrix perturbed by a
+//     few ulp); then the eigenvector from the twisted factorisation
 //     of T - lambda I (top-down L D L^T and bottom-up U D U^T pivots on two
 //     lanes at once, twist index r = argmin |gamma_r|, z_r = 1 and the two
 //     product recurrences outward; Fernando 1997, Parlett & Dhillon 2000),
 //     normalised with its largest entry positive, checked by its residual,
-//     and back-transformed by the reflectors on one wave.
+//     and back-transformed on one wave by blocks of four reflectors in
+//     compact-WY form (their T factors built by the other three waves).
 //  3. k_trd_clorth: eigenvalues closer than 1e-3 |T| form a cluster whose
 //     vectors are made orthogonal by modified Gram-Schmidt (one wave per
 //     cluster); a cluster wider than CL_MAX or tighter than 1e-9 |T|, a
@@ -64,6 +67,12 @@ __device__ __forceinline__ double rcp2(double d) {
   return fma(y, e, y);
 }
 
+// 1/d from the hardware reciprocal and one Newton step (within about 11 ulp)
+__device__ __forceinline__ double rcp1(double d) {
+  const double y = __builtin_amdgcn_rcp(d);
+  return fma(y, fma(-d, y, 1.0), y);
+}
+
 // Gershgorin interval, |T| bounds and pivmin (every lane of a wave gets the
 // same values).
 struct TriBounds {
@@ -96,6 +105,7 @@ __device__ TriBounds tri_bounds(const double* d, const double* e, const double* 
 struct TrdState {
   double (*xs)[II_N];  // published column, by parity of j
   double* ps;          // tau A v
+  double* kred;        // per-wave shares of v^T A v
   double* tri;
   double* refl;
   int n, t, tr, tc, lane, wv;
@@ -105,14 +115,23 @@ struct TrdState {
 // Columns j in [16 jb, jend) with L = 8 - jb live register rows / columns:
 // register (i, q) holds A[tr + 16 (i + jb)][tc + 16 (q + jb)].  Rows and
 // columns at or past n hold zeros throughout (their v and w are zero), so
-// only the diagonal position relative to j needs a test.
+// only the diagonal position relative to j needs a test.  The rank-2 update
+// of column j is applied lazily: register column 0 (which holds the next
+// column's owners' entries) before the first barrier of column j + 1, the
+// other columns after it, where the FMAs overlap the latency of the next
+// reflector's scalars.
 template <int L>
 __device__ __forceinline__ void trd_block(double (&a)[TD_R][TD_R], int jb, int jend,
                                           const TrdState& S) {
   const int n = S.n, tr = S.tr, tc = S.tc, lane = S.lane;
+  double vr[L], vc[L], wr[L], wc[L];  // the pending update A -= v w^T + w v^T
+#pragma unroll
+  for (int i = 0; i < L; ++i) vr[i] = vc[i] = wr[i] = wc[i] = 0.0;
   for (int j = 16 * jb; j < jend; ++j) {
     double* __restrict__ x = S.xs[j & 1];
     if (j == 20) FPLS_TRD_STAMP(2);
+#pragma unroll
+    for (int i = 0; i < L; ++i) a[i][0] = fma(-vr[i], wc[0], fma(-wr[i], vc[0], a[i][0]));
     // column j from its owners (column register 0 of the threads with
     // tc == j % 16); rows below j are never read back
     if (tc == (j & 15)) {
@@ -128,12 +147,95 @@ __device__ __forceinline__ void trd_block(double (&a)[TD_R][TD_R], int jb, int j
       if (r0 < n) s = x[r0] * x[r0];
       if (r1 < n) s = fma(x[r1], x[r1], s);
     }
+    double xr[L], xc[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      xr[i] = x[tr + 16 * (i + jb)];
+      xc[i] = x[tc + 16 * (i + jb)];
+    }
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+#pragma unroll
+      for (int q = 1; q < L; ++q) a[i][q] = fma(-vr[i], wc[q], fma(-wr[i], vc[q], a[i][q]));
+    // reflector: beta = -sign(alpha) sqrt(alpha^2 + |x|^2), tau = 1 + |alpha| / |beta|,
+    // v = x / (alpha - beta); r = 1 / sqrt(.) from rsq and two Newton steps
     const double xn2 = wave_sum_d(s);
     double beta = alpha, tau = 0.0, sv = 0.0;
     if (xn2 > 0.0) {
-      beta = -copysign(sqrt(fma(alpha, alpha, xn2)), alpha);
-      tau = (beta - alpha) / beta;
-      sv = 1.0 / (alpha - beta);
+      const double q2 = fma(alpha, alpha, xn2);
+      double r = __builtin_amdgcn_rsq(q2);
+      double h = fma(-q2 * r, r, 1.0);
+      r = fma(0.5 * r, h, r);
+      h = fma(-q2 * r, r, 1.0);
+      r = fma(0.5 * r, h, r);
+      const double sq = q2 * r;
+      beta = -copysign(sq, alpha);
+      tau = fma(fabs(alpha), r, 1.0);
+      sv = copysign(rcp2(fabs(alpha) + sq), alpha);
+    }
+    if (j == 20) FPLS_TRD_STAMP(4);
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
+      vr[i] = (r == j + 1) ? 1.0 : (r > j + 1 ? xr[i] * sv : 0.0);
+      vc[i] = (c == j + 1) ? 1.0 : (c > j + 1 ? xc[i] * sv : 0.0);
+    }
+    if (tau != 0.0) {  // uniform: H_j = I leaves A unchanged
+      double pr[TD_R];
+      double kp = 0.0;  // this thread's share of v^T A v
+#pragma unroll
+      for (int i = 0; i < TD_R; ++i) {
+        double acc = 0.0;
+        if (i < L) {
+#pragma unroll
+          for (int q = 0; q < L; ++q) acc = fma(a[i][q], vc[q], acc);
+          kp = fma(vr[i], acc, kp);
+        }
+        pr[i] = acc;
+      }
+      // reduce-scatter over the 16 lanes of the thread row: partners by
+      // row_ror:8 (bit 3), row_half_mirror (bit 2), quad_perm xor 2 (bit 1),
+      // quad_perm xor 1 (bit 0); each exchange sends the half the partner keeps
+      double h4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double keep = S.hb3 ? pr[4 + u] : pr[u];
+        const double send = S.hb3 ? pr[u] : pr[4 + u];
+        h4[u] = keep + dpp_mov<DPP_ROR8>(send);
+      }
+      double h2[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const double keep = S.hb2 ? h4[2 + u] : h4[u];
+        const double send = S.hb2 ? h4[u] : h4[2 + u];
+        h2[u] = keep + dpp_mov<dpp::HALF_MIRROR>(send);
+      }
+      double h1;
+      {
+        const double keep = S.hb1 ? h2[1] : h2[0];
+        const double send = S.hb1 ? h2[0] : h2[1];
+        h1 = keep + dpp_mov<dpp::QUAD_XOR2>(send);
+      }
+      h1 += dpp_mov<dpp::QUAD_XOR1>(h1);
+      const double kw = wave_sum_d(kp);
+      if (j == 20) FPLS_TRD_STAMP(5);
+      if (S.hb0 == 0 && S.slot < L) S.ps[tr + 16 * (S.slot + jb)] = tau * h1;
+      if (lane == 0) S.kred[S.wv] = kw;
+      __syncthreads();
+      if (j == 20) FPLS_TRD_STAMP(6);
+      // p = tau A v (in ps), K = tau / 2 p^T v = tau^2 / 2 v^T A v, w = p - K v
+      const double K =
+          0.5 * tau * tau * ((S.kred[0] + S.kred[1]) + (S.kred[2] + S.kred[3]));
+#pragma unroll
+      for (int i = 0; i < L; ++i) {
+        const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
+        const double pr_ = S.ps[r], pc_ = S.ps[c];
+        wr[i] = r > j ? fma(-K, vr[i], pr_) : 0.0;
+        wc[i] = c > j ? fma(-K, vc[i], pc_) : 0.0;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < L; ++i) vr[i] = vc[i] = wr[i] = wc[i] = 0.0;
     }
     if (S.t == 0) {
       S.tri[j] = x[j];
@@ -144,76 +246,13 @@ __device__ __forceinline__ void trd_block(double (&a)[TD_R][TD_R], int jb, int j
       for (int r = lane; r < n; r += 64)
         S.refl[r + (int64_t)j * n] = (r == j + 1) ? 1.0 : (r > j + 1 ? x[r] * sv : 0.0);
     }
-    if (j == 20) FPLS_TRD_STAMP(4);
-    if (tau == 0.0) continue;  // uniform: H_j = I leaves A unchanged
-    double vr[L], vc[L];
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-      const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
-      const double xr = x[r], xc = x[c];
-      vr[i] = (r == j + 1) ? 1.0 : (r > j + 1 ? xr * sv : 0.0);
-      vc[i] = (c == j + 1) ? 1.0 : (c > j + 1 ? xc * sv : 0.0);
-    }
-    double pr[TD_R];
-#pragma unroll
-    for (int i = 0; i < TD_R; ++i) {
-      double acc = 0.0;
-      if (i < L) {
-#pragma unroll
-        for (int q = 0; q < L; ++q) acc = fma(a[i][q], vc[q], acc);
-      }
-      pr[i] = acc;
-    }
-    // reduce-scatter over the 16 lanes of the thread row: partners by
-    // row_ror:8 (bit 3), row_half_mirror (bit 2), quad_perm xor 2 (bit 1),
-    // quad_perm xor 1 (bit 0); each exchange sends the half the partner keeps
-    double h4[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const double keep = S.hb3 ? pr[4 + u] : pr[u];
-      const double send = S.hb3 ? pr[u] : pr[4 + u];
-      h4[u] = keep + dpp_mov<DPP_ROR8>(send);
-    }
-    double h2[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const double keep = S.hb2 ? h4[2 + u] : h4[u];
-      const double send = S.hb2 ? h4[u] : h4[2 + u];
-      h2[u] = keep + dpp_mov<dpp::HALF_MIRROR>(send);
-    }
-    double h1;
-    {
-      const double keep = S.hb1 ? h2[1] : h2[0];
-      const double send = S.hb1 ? h2[0] : h2[1];
-      h1 = keep + dpp_mov<dpp::QUAD_XOR2>(send);
-    }
-    h1 += dpp_mov<dpp::QUAD_XOR1>(h1);
-    if (j == 20) FPLS_TRD_STAMP(5);
-    if (S.hb0 == 0 && S.slot < L) S.ps[tr + 16 * (S.slot + jb)] = tau * h1;
-    __syncthreads();
-    if (j == 20) FPLS_TRD_STAMP(6);
-    // K = tau / 2 p^T v over rows j + 1 .. n - 1 (ps holds tau p)
-    double sk = 0.0;
-    {
-      const int r0 = j + 1 + lane, r1 = r0 + 64;
-      if (r0 < n) sk = S.ps[r0] * ((r0 == j + 1) ? 1.0 : x[r0] * sv);
-      if (r1 < n) sk = fma(S.ps[r1], x[r1] * sv, sk);
-    }
-    const double K = 0.5 * tau * wave_sum_d(sk);
-    double wr[L], wc[L];
-#pragma unroll
-    for (int i = 0; i < L; ++i) {
-      const int r = tr + 16 * (i + jb), c = tc + 16 * (i + jb);
-      const double pr_ = S.ps[r], pc_ = S.ps[c];
-      wr[i] = r > j ? fma(-K, vr[i], pr_) : 0.0;
-      wc[i] = c > j ? fma(-K, vc[i], pc_) : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < L; ++i)
-#pragma unroll
-      for (int q = 0; q < L; ++q) a[i][q] = fma(-vr[i], wc[q], fma(-wr[i], vc[q], a[i][q]));
     if (j == 20) FPLS_TRD_STAMP(7);
   }
+  // the last pending update before the caller drops a block
+#pragma unroll
+  for (int i = 0; i < L; ++i)
+#pragma unroll
+    for (int q = 0; q < L; ++q) a[i][q] = fma(-vr[i], wc[q], fma(-wr[i], vc[q], a[i][q]));
 }
 
 // tri (3 n + 1 doubles): d, e (e[n-1] = 0), tau, then the scale (max |A|) the
@@ -233,6 +272,7 @@ __global__ __launch_bounds__(TD_T) void k_trd_reg(const double* __restrict__ A0,
   __shared__ double xs[2][II_N];
   __shared__ double ps[II_N];
   __shared__ double red[TD_T / 64];
+  __shared__ double kred[TD_T / 64];
   __shared__ double tail[3];
   const int t = threadIdx.x, tr = t >> 4, tc = t & 15, lane = t & 63, wv = t >> 6;
   if (t == 0 && flags) flags[bi] = 0;
@@ -267,6 +307,7 @@ __global__ __launch_bounds__(TD_T) void k_trd_reg(const double* __restrict__ A0,
   TrdState S;
   S.xs = xs;
   S.ps = ps;
+  S.kred = kred;
   S.tri = tri;
   S.refl = refl;
   S.n = n;
@@ -347,6 +388,8 @@ __global__ __launch_bounds__(TD_T) void k_trd_reg(const double* __restrict__ A0,
 // ---------------------------------------------------------------- stage 2
 constexpr int PR_T = 256;              // threads per eigenvalue
 constexpr int PR_PTS = 2 * PR_T;       // multisection points per step
+constexpr int BT_B = 4;                // reflectors per back-transformation block
+constexpr int BT_TS = 10;              // packed upper-triangular T of one block
 
 // Number of eigenvalues of T below x0 and below x1 (two interleaved chains).
 __device__ __forceinline__ void sturm_count2(const double* d, const double* ee, int n, double x0,
@@ -357,8 +400,8 @@ __device__ __forceinline__ void sturm_count2(const double* d, const double* ee, 
   int n0 = q0 < 0.0 ? 1 : 0, n1 = q1 < 0.0 ? 1 : 0;
   for (int i = 1; i < n; ++i) {
     const double di = d[i], ei = ee[i - 1];
-    q0 = (di - x0) - ei * rcp2(q0);
-    q1 = (di - x1) - ei * rcp2(q1);
+    q0 = (di - x0) - ei * rcp1(q0);
+    q1 = (di - x1) - ei * rcp1(q1);
     if (fabs(q0) < pivmin) q0 = -pivmin;
     if (fabs(q1) < pivmin) q1 = -pivmin;
     n0 += q0 < 0.0 ? 1 : 0;
@@ -375,7 +418,7 @@ __global__ __launch_bounds__(PR_T) void k_trd_pairs(
     const int* __restrict__ ns, int n0, const double* __restrict__ tri0, int64_t strideTri,
     const double* __restrict__ refl0, int64_t strideRefl, double* __restrict__ w0,
     int64_t strideW, double* __restrict__ V0, int64_t strideV, double* __restrict__ lam0,
-    int64_t strideLam, int* __restrict__ flags) {
+    int64_t strideLam, int kwant, int* __restrict__ flags) {
   const int64_t bi = blockIdx.y;
   const int n = ns ? ns[bi] : n0;
   const int k = blockIdx.x;
@@ -389,6 +432,7 @@ __global__ __launch_bounds__(PR_T) void k_trd_pairs(
   __shared__ double fac[2][II_N];
   __shared__ double zv[II_N];
   __shared__ double R[II_N * II_N];  // the reflectors, staged for the back-transformation
+  __shared__ double Tm[BT_TS * ((II_N + BT_B - 1) / BT_B)];
   __shared__ unsigned long long above_w[PR_T / 64][2];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   for (int i = t; i < n; i += PR_T) {
@@ -454,137 +498,208 @@ __global__ __launch_bounds__(PR_T) void k_trd_pairs(
   }
   const double lambda = 0.5 * (lo + hi);
   FPLS_TRD_STAMP(17);
-  if (wv != 0) return;  // one wave finishes the vector
-  if (lane == 0) lam[k] = lambda;
-
+  const int m = n - 2;                          // reflectors 0 .. n - 3
+  const int nblk = m > 0 ? (m + BT_B - 1) / BT_B : 0;
+  const double* __restrict__ tauv = tri + 2 * n;
   bool bad = false;
-  double z0 = 0.0, z1 = 0.0;  // this lane's entries lane, lane + 64
-  if (n == 1) {
-    z0 = lane == 0 ? 1.0 : 0.0;
+  double z0 = 0.0, z1 = 0.0;  // wave 0: this lane's entries lane, lane + 64
+  if (wv != 0) {
+    // compact-WY factors of the blocks of BT_B reflectors (dlarft, forward,
+    // columnwise): H_j0 ... H_j0+3 = I - V T V^T, T upper triangular with
+    // T[:a, a] = -tau_a T[:a, :a] (V[:, :a]^T v_a); blocks shared round-robin
+    // by waves 1 .. 3 while wave 0 forms the eigenvector
+    for (int bk = wv - 1; bk < nblk; bk += PR_T / 64 - 1) {
+      const int j0 = BT_B * bk;
+      double va0[BT_B], va1[BT_B];
+#pragma unroll
+      for (int u = 0; u < BT_B; ++u) {
+        const bool on = j0 + u < m;
+        va0[u] = (on && lane < n) ? R[lane + (j0 + u) * n] : 0.0;
+        va1[u] = (on && lane + 64 < n) ? R[lane + 64 + (j0 + u) * n] : 0.0;
+      }
+      // G01 G02 G12 G03 G13 G23
+      const double g01 = wave_sum_d(fma(va0[0], va0[1], va1[0] * va1[1]));
+      const double g02 = wave_sum_d(fma(va0[0], va0[2], va1[0] * va1[2]));
+      const double g12 = wave_sum_d(fma(va0[1], va0[2], va1[1] * va1[2]));
+      const double g03 = wave_sum_d(fma(va0[0], va0[3], va1[0] * va1[3]));
+      const double g13 = wave_sum_d(fma(va0[1], va0[3], va1[1] * va1[3]));
+      const double g23 = wave_sum_d(fma(va0[2], va0[3], va1[2] * va1[3]));
+      if (lane == 0) {
+        const double t0 = j0 < m ? tauv[j0] : 0.0, t1 = j0 + 1 < m ? tauv[j0 + 1] : 0.0;
+        const double t2 = j0 + 2 < m ? tauv[j0 + 2] : 0.0, t3 = j0 + 3 < m ? tauv[j0 + 3] : 0.0;
+        const double T00 = t0, T11 = t1, T22 = t2, T33 = t3;
+        const double T01 = -t1 * (T00 * g01);
+        const double T02 = -t2 * (T00 * g02 + T01 * g12), T12 = -t2 * (T11 * g12);
+        const double T03 = -t3 * (T00 * g03 + T01 * g13 + T02 * g23);
+        const double T13 = -t3 * (T11 * g13 + T12 * g23), T23 = -t3 * (T22 * g23);
+        double* Tb = Tm + BT_TS * bk;
+        Tb[0] = T00; Tb[1] = T01; Tb[2] = T11; Tb[3] = T02; Tb[4] = T12;
+        Tb[5] = T22; Tb[6] = T03; Tb[7] = T13; Tb[8] = T23; Tb[9] = T33;
+      }
+    }
   } else {
-    // top-down (lane 0) and bottom-up (lane 1) pivots of T - lambda I
-    if (lane < 2) {
-      const bool up = lane == 1;
-      int ii = up ? n - 1 : 0;
-      double q = d[ii] - lambda;
-      if (fabs(q) < b.pivmin) q = -b.pivmin;
-      fac[lane][ii] = q;
-      for (int s = 1; s < n; ++s) {
-        ii = up ? n - 1 - s : s;
-        const double e2 = up ? ee[ii] : ee[ii - 1];
-        q = (d[ii] - lambda) - e2 * rcp2(q);
+    if (lane == 0) lam[k] = lambda;
+    if (n == 1) {
+      z0 = lane == 0 ? 1.0 : 0.0;
+    } else {
+      // top-down (lane 0) and bottom-up (lane 1) pivots of T - lambda I
+      if (lane < 2) {
+        const bool up = lane == 1;
+        int ii = up ? n - 1 : 0;
+        double q = d[ii] - lambda;
         if (fabs(q) < b.pivmin) q = -b.pivmin;
         fac[lane][ii] = q;
-      }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    // twist index: smallest |gamma_r|, lowest index on ties
-    double gbest = DBL_MAX;
-    int rbest = n;
-    for (int r = lane; r < n; r += 64) {
-      const double g = fabs(fac[0][r] + fac[1][r] - (d[r] - lambda));
-      if (g < gbest) {
-        gbest = g;
-        rbest = r;
-      }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double og = __shfl_xor(gbest, o, 64);
-      const int orr = __shfl_xor(rbest, o, 64);
-      if (og < gbest || (og == gbest && orr < rbest)) {
-        gbest = og;
-        rbest = orr;
-      }
-    }
-    const int rt = rbest < n ? rbest : 0;  // a NaN gamma everywhere: any index
-    FPLS_TRD_STAMP(18);
-    // z_rt = 1; lane 0 walks up (z_i = -(e_i / D+_i) z_{i+1}), lane 1 down
-    // (z_i = -(e_{i-1} / D-_i) z_{i-1}); when the neighbour is exactly zero the
-    // equation of the row between gives z_i from the entry two steps back
-    if (lane < 2) {
-      const bool down = lane == 1;
-      if (lane == 0) zv[rt] = 1.0;
-      const int steps = down ? n - 1 - rt : rt;
-      double zp = 1.0, zpp = 0.0;
-      for (int s = 1; s <= steps; ++s) {
-        const int i = down ? rt + s : rt - s;
-        const double ei = down ? e[i - 1] : e[i];
-        double z = -(ei * rcp2(fac[lane][i])) * zp;
-        if (zp == 0.0 && s >= 2) {
-          const double efar = down ? e[i - 2] : e[i + 1];
-          z = ei != 0.0 ? -(efar / ei) * zpp : 0.0;
+        for (int s = 1; s < n; ++s) {
+          ii = up ? n - 1 - s : s;
+          const double e2 = up ? ee[ii] : ee[ii - 1];
+          q = (d[ii] - lambda) - e2 * rcp1(q);
+          if (fabs(q) < b.pivmin) q = -b.pivmin;
+          fac[lane][ii] = q;
         }
-        zv[i] = z;
-        zpp = zp;
-        zp = z;
       }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // twist index: smallest |gamma_r|, lowest index on ties; then the walk
+      // ratios e_i / D+_i (up) and e_{i-1} / D-_i (down) in place
+      double gbest = DBL_MAX;
+      int rbest = n;
+      for (int r = lane; r < n; r += 64) {
+        const double g0 = fac[0][r], g1 = fac[1][r];
+        const double g = fabs(g0 + g1 - (d[r] - lambda));
+        if (g < gbest) {
+          gbest = g;
+          rbest = r;
+        }
+        fac[0][r] = r + 1 < n ? e[r] * rcp1(g0) : 0.0;
+        fac[1][r] = r >= 1 ? e[r - 1] * rcp1(g1) : 0.0;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double og = __shfl_xor(gbest, o, 64);
+        const int orr = __shfl_xor(rbest, o, 64);
+        if (og < gbest || (og == gbest && orr < rbest)) {
+          gbest = og;
+          rbest = orr;
+        }
+      }
+      const int rt = rbest < n ? rbest : 0;  // a NaN gamma everywhere: any index
+      FPLS_TRD_STAMP(18);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      // z_rt = 1; lane 0 walks up (z_i = -(e_i / D+_i) z_{i+1}), lane 1 down
+      // (z_i = -(e_{i-1} / D-_i) z_{i-1}); when the neighbour is exactly zero
+      // the equation of the row between gives z_i from the entry two steps back
+      if (lane < 2) {
+        const bool down = lane == 1;
+        const double* rat = fac[lane];
+        if (lane == 0) zv[rt] = 1.0;
+        const int steps = down ? n - 1 - rt : rt;
+        double zp = 1.0, zpp = 0.0;
+        int s = 1;
+        for (; s + 3 <= steps; s += 4) {
+          int ii[4];
+          double rr[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ii[u] = down ? rt + s + u : rt - s - u;
+            rr[u] = rat[ii[u]];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            double z = -rr[u] * zp;
+            if (zp == 0.0 && s + u >= 2) {
+              const int i = ii[u];
+              const double ei = down ? e[i - 1] : e[i], efar = down ? e[i - 2] : e[i + 1];
+              z = ei != 0.0 ? -(efar / ei) * zpp : 0.0;
+            }
+            zv[ii[u]] = z;
+            zpp = zp;
+            zp = z;
+          }
+        }
+        for (; s <= steps; ++s) {
+          const int i = down ? rt + s : rt - s;
+          double z = -rat[i] * zp;
+          if (zp == 0.0 && s >= 2) {
+            const double ei = down ? e[i - 1] : e[i], efar = down ? e[i - 2] : e[i + 1];
+            z = ei != 0.0 ? -(efar / ei) * zpp : 0.0;
+          }
+          zv[i] = z;
+          zpp = zp;
+          zp = z;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      z0 = lane < n ? zv[lane] : 0.0;
+      z1 = lane + 64 < n ? zv[lane + 64] : 0.0;
+      // unit length, largest entry positive (lowest index on ties)
+      double amax = -1.0, av = 0.0;
+      int ai = n;
+      if (lane < n && fabs(z0) > amax) {
+        amax = fabs(z0);
+        av = z0;
+        ai = lane;
+      }
+      if (lane + 64 < n && fabs(z1) > amax) {
+        amax = fabs(z1);
+        av = z1;
+        ai = lane + 64;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ou = __shfl_xor(amax, o, 64), ov = __shfl_xor(av, o, 64);
+        const int oi = __shfl_xor(ai, o, 64);
+        if (ou > amax || (ou == amax && oi < ai)) {
+          amax = ou;
+          av = ov;
+          ai = oi;
+        }
+      }
+      const double nrm2 = wave_sum_d(fma(z0, z0, z1 * z1));
+      if (!(nrm2 > 0.0) || !isfinite(nrm2)) bad = true;
+      const double inv = (av < 0.0 ? -1.0 : 1.0) / sqrt(nrm2);
+      z0 *= inv;
+      z1 *= inv;
+      // residual |(T - lambda I) z|_inf against 1000 n eps |T|
+      zv[lane] = z0;
+      if (lane + 64 < II_N) zv[lane + 64] = z1;
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      double res = 0.0;
+      for (int i = lane; i < n; i += 64) {
+        double ri = (d[i] - lambda) * zv[i];
+        if (i > 0) ri = fma(e[i - 1], zv[i - 1], ri);
+        if (i + 1 < n) ri = fma(e[i], zv[i + 1], ri);
+        res = fmax(res, fabs(ri));
+      }
+      res = wave_max_d(res);
+      if (!(res <= 1000.0 * (double)n * II_EPS * fmax(b.tnorm, DBL_MIN))) bad = true;
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    z0 = lane < n ? zv[lane] : 0.0;
-    z1 = lane + 64 < n ? zv[lane + 64] : 0.0;
-    // unit length, largest entry positive (lowest index on ties)
-    double amax = -1.0, av = 0.0;
-    int ai = n;
-    if (lane < n && fabs(z0) > amax) {
-      amax = fabs(z0);
-      av = z0;
-      ai = lane;
-    }
-    if (lane + 64 < n && fabs(z1) > amax) {
-      amax = fabs(z1);
-      av = z1;
-      ai = lane + 64;
+  }
+  __syncthreads();  // the T factors are in LDS
+  if (wv != 0) return;
+  FPLS_TRD_STAMP(19);
+  // back-transformation z <- H_0 ... H_{n-3} z, block by block from the last:
+  // z <- z - V (T (V^T z))
+  for (int bk = nblk - 1; bk >= 0; --bk) {
+    const int j0 = BT_B * bk;
+    double va0[BT_B], va1[BT_B], sa[BT_B];
+#pragma unroll
+    for (int u = 0; u < BT_B; ++u) {
+      const bool on = j0 + u < m;
+      va0[u] = (on && lane < n) ? R[lane + (j0 + u) * n] : 0.0;
+      va1[u] = (on && lane + 64 < n) ? R[lane + 64 + (j0 + u) * n] : 0.0;
     }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ou = __shfl_xor(amax, o, 64), ov = __shfl_xor(av, o, 64);
-      const int oi = __shfl_xor(ai, o, 64);
-      if (ou > amax || (ou == amax && oi < ai)) {
-        amax = ou;
-        av = ov;
-        ai = oi;
-      }
-    }
-    const double nrm2 = wave_sum_d(fma(z0, z0, z1 * z1));
-    if (!(nrm2 > 0.0) || !isfinite(nrm2)) bad = true;
-    const double inv = (av < 0.0 ? -1.0 : 1.0) / sqrt(nrm2);
-    z0 *= inv;
-    z1 *= inv;
-    // residual |(T - lambda I) z|_inf against 1000 n eps |T|
-    zv[lane] = z0;
-    if (lane + 64 < II_N) zv[lane + 64] = z1;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    double res = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      double ri = (d[i] - lambda) * zv[i];
-      if (i > 0) ri = fma(e[i - 1], zv[i - 1], ri);
-      if (i + 1 < n) ri = fma(e[i], zv[i + 1], ri);
-      res = fmax(res, fabs(ri));
-    }
-    res = wave_max_d(res);
-    if (!(res <= 1000.0 * (double)n * II_EPS * fmax(b.tnorm, DBL_MIN))) bad = true;
-    FPLS_TRD_STAMP(19);
-    // back-transformation z <- H_0 ... H_{n-3} z (reflectors from the last)
-    if (n >= 3) {
-      const double* __restrict__ tauv = tri + 2 * n;
-      int j = n - 3;
-      double r0 = lane < n ? R[lane + j * n] : 0.0;
-      double r1 = lane + 64 < n ? R[lane + 64 + j * n] : 0.0;
-      for (; j >= 0; --j) {
-        const int jn = j > 0 ? j - 1 : 0;
-        const double n0v = lane < n ? R[lane + jn * n] : 0.0;
-        const double n1v = lane + 64 < n ? R[lane + 64 + jn * n] : 0.0;
-        const double sj = tauv[j] * wave_sum_d(fma(r0, z0, r1 * z1));
-        z0 = fma(-sj, r0, z0);
-        z1 = fma(-sj, r1, z1);
-        r0 = n0v;
-        r1 = n1v;
-      }
-    }
+    for (int u = 0; u < BT_B; ++u) sa[u] = wave_sum_d(fma(va0[u], z0, va1[u] * z1));
+    const double* Tb = Tm + BT_TS * bk;
+    const double u0 = fma(Tb[0], sa[0], fma(Tb[1], sa[1], fma(Tb[3], sa[2], Tb[6] * sa[3])));
+    const double u1 = fma(Tb[2], sa[1], fma(Tb[4], sa[2], Tb[7] * sa[3]));
+    const double u2 = fma(Tb[5], sa[2], Tb[8] * sa[3]);
+    const double u3 = Tb[9] * sa[3];
+    z0 = fma(-va0[0], u0, fma(-va0[1], u1, fma(-va0[2], u2, fma(-va0[3], u3, z0))));
+    z1 = fma(-va1[0], u0, fma(-va1[1], u1, fma(-va1[2], u2, fma(-va1[3], u3, z1))));
   }
   FPLS_TRD_STAMP(20);
   const int oc = n - 1 - k;
@@ -592,7 +707,10 @@ __global__ __launch_bounds__(PR_T) void k_trd_pairs(
   if (lane + 64 < n) V[lane + 64 + (int64_t)oc * n] = z1;
   if (lane == 0) {
     w[oc] = lambda * scale;
-    if (bad || !isfinite(lambda)) flags[bi] = 1;
+    // only the wanted (largest kwant) eigenpairs can send the problem to the
+    // fallback solver
+    const bool wanted = kwant <= 0 || oc < kwant;
+    if (wanted && (bad || !isfinite(lambda))) flags[bi] = 1;
   }
 }
 
@@ -606,7 +724,7 @@ constexpr int CL_W = 4;  // waves per workgroup
 __global__ __launch_bounds__(64 * CL_W) void k_trd_clorth(
     const int* __restrict__ ns, int n0, const double* __restrict__ tri0, int64_t strideTri,
     const double* __restrict__ lam0, int64_t strideLam, double* __restrict__ V0,
-    int64_t strideV, int* __restrict__ flags) {
+    int64_t strideV, int kwant, int* __restrict__ flags) {
   const int64_t bi = blockIdx.y;
   const int n = ns ? ns[bi] : n0;
   if (n > II_N) return;
@@ -632,7 +750,9 @@ __global__ __launch_bounds__(64 * CL_W) void k_trd_clorth(
   bool badc = cl > CL_MAX;
   for (int q = k0; q + 1 < k0 + cl && !badc; ++q) badc = lam[q + 1] - lam[q] <= tight;
   if (badc) {
-    if (lane == 0) flags[bi] = 1;
+    // a cluster among the unwanted (smallest) eigenvalues keeps its vectors
+    const bool wanted = kwant <= 0 || k0 + cl - 1 >= n - kwant;
+    if (wanted && lane == 0) flags[bi] = 1;
     return;
   }
   for (int m = 1; m < cl; ++m) {
@@ -657,7 +777,7 @@ __global__ __launch_bounds__(64 * CL_W) void k_trd_clorth(
 
 void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const int* ns_dev,
                  int n0, int nmax, int count, double* w, int64_t strideW, double* V,
-                 int64_t strideV, int* flags) {
+                 int64_t strideV, int kwant, int* flags) {
   FPLS_REQUIRE(nmax >= 1 && nmax <= II_N && count >= 1, FPLS_ERR_ARG,
                "sym_eig_tri: orders must be in 1..128");
   const int64_t sTri = 3 * (int64_t)nmax + 1, sRefl = (int64_t)nmax * nmax, sLam = nmax;
@@ -667,10 +787,11 @@ void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const 
          ns_dev, n0, tri.get(), sTri, refl.get(), sRefl, flags);
   launch(ctx, "sym_eig_pairs", k_trd_pairs, dim3((unsigned)nmax, (unsigned)count), dim3(PR_T),
          0, ns_dev, n0, (const double*)tri.get(), sTri, (const double*)refl.get(), sRefl, w,
-         strideW, V, strideV, lam.get(), sLam, flags);
+         strideW, V, strideV, lam.get(), sLam, kwant, flags);
   launch(ctx, "sym_eig_clorth", k_trd_clorth,
          dim3((unsigned)((nmax + CL_W - 1) / CL_W), (unsigned)count), dim3(64 * CL_W), 0, ns_dev,
-         n0, (const double*)tri.get(), sTri, (const double*)lam.get(), sLam, V, strideV, flags);
+         n0, (const double*)tri.get(), sTri, (const double*)lam.get(), sLam, V, strideV, kwant,
+         flags);
 }
 
 }  // namespace fpls

@@ -305,10 +305,11 @@ unsigned jgrid(int64_t n) {
 
 }  // namespace
 
-void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, double* V) {
+void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, double* V,
+                   int kwant) {
   FPLS_REQUIRE(n >= 1, FPLS_ERR_ARG, "sym_eig_large: empty matrix");
   if (n <= kJacobiMax) {
-    sym_eig(ctx, S, lds, n, w, V);
+    sym_eig(ctx, S, lds, n, w, V, kwant);
     return;
   }
   const int nb = 2 * ((n + JB_PAIR - 1) / JB_PAIR);  // even, blocks of <= 64 rows
@@ -363,7 +364,7 @@ void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, dou
       launch(ctx, "jacobi_gather", k_jac_gather, dim3(16, (unsigned)q), dim3(256), 0,
              (const double*)Aw.get(), (int64_t)n, ix, nsr, M.get());
       sym_eig_batched(ctx, M.get(), JB_PAIR, (int64_t)JB_PAIR * JB_PAIR, nsr, JB_PAIR, q,
-                      wb.get(), JB_PAIR, Z.get(), (int64_t)JB_PAIR * JB_PAIR);
+                      wb.get(), JB_PAIR, Z.get(), (int64_t)JB_PAIR * JB_PAIR, 0, 0, true);
       launch(ctx, "jacobi_nearid", k_jac_nearid, dim3((unsigned)q), dim3(256), 0, nsr,
              (const double*)Z.get(), Zn.get());
       // B = A J; A <- B^T J; V <- V J

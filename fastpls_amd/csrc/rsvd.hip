@@ -84,7 +84,7 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
   if (cols <= rows) {
     gemm_f64(ctx, true, false, cols, cols, rows, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);
-    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get());
+    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get(), kk);
     // s = sqrt(w), U = A V / s in one pass (the product is rows x cols x kk)
     launch(ctx, "svd_left", k_svd_left, dim3((unsigned)((rows + 255) / 256), (unsigned)kk),
            dim3(256), 0, A, lda, rows, cols, (const double*)E.get(), (int64_t)g,
@@ -100,7 +100,7 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
   } else {
     gemm_f64(ctx, false, true, rows, rows, cols, 1.0, A, lda, nullptr, A, lda, nullptr, 0.0,
              Gm.get(), g);
-    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get());
+    sym_eig_large(ctx, Gm.get(), g, g, w.get(), E.get(), kk);
     launch(ctx, "sqrt", k_sqrt_clamp, dim3((kk + 255) / 256), dim3(256), 0, w.get(), out.s.get(),
            kk);
     copy2d_f64(ctx, out.U.get(), rows, E.get(), g, rows, kk);
@@ -158,7 +158,7 @@ void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols
     DBuf<double> BB(ctx, (size_t)l * l), w(ctx, l), E(ctx, (size_t)l * l);
     gemm_f64(ctx, false, true, l, l, cols, 1.0, Bm.get(), l, nullptr, Bm.get(), l, nullptr, 0.0,
              BB.get(), l);
-    sym_eig(ctx, BB.get(), l, l, w.get(), E.get());
+    sym_eig(ctx, BB.get(), l, l, w.get(), E.get(), target);
     std::vector<double> wh(l);
     to_host(ctx, wh.data(), w.get(), l);
     const double tol = DBL_EPSILON * (double)std::max<int64_t>(l, cols) * std::max(wh[0], 1.0);
