@@ -112,7 +112,8 @@ def run_gpu(args, data):
         kern = {}
         for nm in ("project_f32", "project_bf16x3", "gemm_nn_f32", "gemm_nn_bf16x3",
                    "gemm_tn_f32", "gemm_tn_bf16x3",
-                   "sym_eig_dc", "spd_solve", "col_sums", "lda_argmax", "gemm_f64",
+                   "sym_eig_dc", "sym_eig_trd", "sym_eig_pairs", "sym_eig_clorth",
+                   "spd_solve", "col_sums", "lda_argmax", "gemm_f64",
                    "gemm_tn_reduce", "label_counts"):
             n, ms = ctx.kernel_time(nm)
             if n:
@@ -678,7 +679,9 @@ def main_cknn(args):
              "gemm_tn_f32", "gemm_tn_reduce", "gemm_nn_bf16x3", "gemm_nn_f32", "project_bf16x3",
              "project_f32", "gemm_rr_bf16x3", "gemm_rr_f32", "sym_eig_dc", "jacobi_cols",
              "jacobi_nearid", "jacobi_offnorm", "jacobi_gather", "gemm_f64", "gemm_f64_reduce",
-             "col_sums", "spd_solve", "transpose", "copy", "fill")
+             "col_sums", "spd_solve", "transpose", "copy", "fill", "eig_chip_col",
+             "eig_chip_start", "eig_chip_pairs", "eig_chip_orth", "eig_chip_tfac",
+             "eig_chip_sym", "eig_chip_end")
     res = {}
     model_keep = None
     for prec in ("fp32", "bf16x3"):

@@ -380,6 +380,14 @@ void sym_eig_tri(Ctx* ctx, const double* A, int64_t lda, int64_t strideA, const 
 // sym_eig (eigenvalues descending in w, eigenvectors in V, ld n).
 void sym_eig_large(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V,
                    int kwant = 0);
+// Orders 129 .. 2048 on the whole chip (k_eig_large.hip): blocked Householder
+// tridiagonalisation, the K = kwant (or n) largest eigenpairs by bisection and
+// twisted factorisation, symmetric re-orthogonalisation and a blocked
+// back-transformation.  Writes w[0 .. K) descending and V[:, 0 .. K) (ld n);
+// returns false (outputs unspecified) when a check fails and the caller must
+// use another method.
+bool sym_eig_chip(Ctx* ctx, const double* A, int64_t lda, int n, double* w, double* V,
+                  int kwant);
 // spd_solve_slices for orders above 128, through the eigendecomposition.
 void spd_solve_large(Ctx* ctx, const double* A, int64_t lda, const std::vector<int>& ks,
                      int n_slices, int mode, const double* dvals, double ridge, double* X,

@@ -312,6 +312,8 @@ void sym_eig_large(Ctx* ctx, const double* S, int64_t lds, int n, double* w, dou
     sym_eig(ctx, S, lds, n, w, V, kwant);
     return;
   }
+  // the tridiagonal path on the whole chip; block Jacobi when it declines
+  if (sym_eig_chip(ctx, S, lds, n, w, V, kwant)) return;
   const int nb = 2 * ((n + JB_PAIR - 1) / JB_PAIR);  // even, blocks of <= 64 rows
   const int q = nb / 2;
   std::vector<int> lo(nb + 1);
