@@ -395,6 +395,187 @@ __global__ __launch_bounds__(256) void k_cand_gemm(const float* __restrict__ Tte
   }
 }
 
+// Split-FP16 form of k_cand_gemm.  Both operands are unit-normalised rows
+// (|x| <= 1), so x 2^12 splits exactly enough into an FP16 pair
+// x 2^12 = h + l + r with h = fp16(x 2^12), l = fp16(x 2^12 - h),
+// |r| <= 2^-22 |x 2^12| (h, l normal for every entry above 2^-13), and
+// x . y = 2^-24 (h_x h_y + h_x l_y + l_x h_y) up to 3 2^-22 sum |x_i y_i|:
+// FP32-level similarities from three FP16 MFMA products (16x16x32) at the
+// FP16 matrix-core rate.  Operands arrive pre-split, row-major with ld kp (a
+// multiple of 32, zero padded); tiles and lists as in k_cand_gemm, K streamed
+// in 32-deep steps (hi and lo of both operands, double buffered).
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+constexpr int CKH_KS = 32;                      // halves per K step
+constexpr int CKH_LD = CKH_KS + 8;              // LDS row stride (80 bytes)
+constexpr int CKH_ROWS = CKG_P + CKG_R;         // operand rows per buffer (A then B)
+constexpr int CKH_BUF = 2 * CKH_ROWS * CKH_LD;  // halves per buffer (hi, lo)
+constexpr int CKH_OPS = 2 * CKH_BUF * 2;        // bytes, both buffers
+constexpr int CKH_SMEM = CKH_OPS > CKG_SIM * 4 ? CKH_OPS : CKG_SIM * 4;
+constexpr float CKH_SCALE = 4096.0f;
+
+__global__ void k_split_f16(const float* __restrict__ X, int64_t rows, int k, int kp,
+                            _Float16* __restrict__ Xh, _Float16* __restrict__ Xl) {
+  const int cpr = kp / 8;
+  const int64_t nchunk = rows * cpr;
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nchunk;
+       q += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = q / cpr;
+    const int c0 = (int)(q % cpr) * 8;
+    h8v hv, lv;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int c = c0 + e;
+      const float x = c < k ? X[r * k + c] * CKH_SCALE : 0.0f;
+      const _Float16 h = (_Float16)x;
+      hv[e] = h;
+      lv[e] = (_Float16)(x - (float)h);
+    }
+    *reinterpret_cast<h8v*>(Xh + r * kp + c0) = hv;
+    *reinterpret_cast<h8v*>(Xl + r * kp + c0) = lv;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cand_gemm_h(
+    const _Float16* __restrict__ Tth, const _Float16* __restrict__ Ttl, int kp,
+    const _Float16* __restrict__ Tnh, const _Float16* __restrict__ Tnl,
+    const int* __restrict__ bucket, int m, const CkItem* __restrict__ items, int knn_k,
+    float* __restrict__ part) {
+  __shared__ __align__(16) unsigned char smem_raw[CKH_SMEM];
+  _Float16* hs = reinterpret_cast<_Float16*>(smem_raw);
+  const CkItem it = items[blockIdx.x];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int np = it.pair_end - it.pair_beg;
+  const int ph = w & 1, th = w >> 1;
+  // staging: thread -> operand row t / 2, 16 consecutive k at 16 (t % 2)
+  const int srow = t >> 1, sk = (t & 1) * 16;
+  // rows past the item's pairs (training rows) stage a valid row whose
+  // similarities are never read, so the staging loads need no branches
+  const int64_t tr = srow < np ? bucket[it.pair_beg + srow] / m : 0;
+  const _Float16* ah = Tth + tr * kp;
+  const _Float16* al = Ttl + tr * kp;
+  // hi at [buf][0][row][k], lo at [buf][1][row][k]; A rows 0..127, B rows 128..255
+  auto lds = [&](int buf, int lo, int row) {
+    return hs + buf * CKH_BUF + lo * CKH_ROWS * CKH_LD + row * CKH_LD;
+  };
+  float L[CK_KNN_MAX];
+#pragma unroll
+  for (int j = 0; j < CK_KNN_MAX; ++j) L[j] = -INFINITY;
+  float thr = -INFINITY;
+  const int nks = kp / CKH_KS;
+  for (int r0 = it.row_beg; r0 < it.row_end; r0 += CKG_R) {
+    const int64_t brow = r0 + srow < it.row_end ? r0 + srow : r0;
+    const _Float16* bh = Tnh + brow * kp;
+    const _Float16* bl = Tnl + brow * kp;
+    v4f acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
+    // staged 16-byte pieces: A hi, A lo, B hi, B lo (two each)
+    uint4 q0, q1, q2, q3, q4, q5, q6, q7;
+#define CKH_LOAD(k0)                                                                   \
+  do {                                                                                 \
+    const int o_ = (k0) + sk;                                                          \
+    q0 = *reinterpret_cast<const uint4*>(ah + o_);                                 \
+    q1 = *reinterpret_cast<const uint4*>(ah + o_ + 8);                         \
+    q2 = *reinterpret_cast<const uint4*>(al + o_);                                 \
+    q3 = *reinterpret_cast<const uint4*>(al + o_ + 8);                         \
+    q4 = *reinterpret_cast<const uint4*>(bh + o_);                                 \
+    q5 = *reinterpret_cast<const uint4*>(bh + o_ + 8);                         \
+    q6 = *reinterpret_cast<const uint4*>(bl + o_);                                 \
+    q7 = *reinterpret_cast<const uint4*>(bl + o_ + 8);                         \
+  } while (0)
+#define CKH_STORE(buf)                                                                 \
+  do {                                                                                 \
+    *reinterpret_cast<uint4*>(lds((buf), 0, srow) + sk) = q0;                          \
+    *reinterpret_cast<uint4*>(lds((buf), 0, srow) + sk + 8) = q1;                      \
+    *reinterpret_cast<uint4*>(lds((buf), 1, srow) + sk) = q2;                          \
+    *reinterpret_cast<uint4*>(lds((buf), 1, srow) + sk + 8) = q3;                      \
+    *reinterpret_cast<uint4*>(lds((buf), 0, CKG_P + srow) + sk) = q4;                  \
+    *reinterpret_cast<uint4*>(lds((buf), 0, CKG_P + srow) + sk + 8) = q5;              \
+    *reinterpret_cast<uint4*>(lds((buf), 1, CKG_P + srow) + sk) = q6;                  \
+    *reinterpret_cast<uint4*>(lds((buf), 1, CKG_P + srow) + sk + 8) = q7;              \
+  } while (0)
+    CKH_LOAD(0);
+    __syncthreads();  // the previous tile's Sim reads are done
+    CKH_STORE(0);
+    __syncthreads();
+    const int g8 = 8 * (lane >> 4), li = lane & 15;
+    for (int st = 0; st < nks; ++st) {
+      const bool more = st + 1 < nks;
+      if (more) CKH_LOAD((st + 1) * CKH_KS);
+      const int buf = st & 1;
+      h8v ahf[4], alf[4], bhf[4], blf[4];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int row = ph * 64 + a * 16 + li;
+        ahf[a] = *reinterpret_cast<const h8v*>(lds(buf, 0, row) + g8);
+        alf[a] = *reinterpret_cast<const h8v*>(lds(buf, 1, row) + g8);
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int row = CKG_P + th * 64 + b * 16 + li;
+        bhf[b] = *reinterpret_cast<const h8v*>(lds(buf, 0, row) + g8);
+        blf[b] = *reinterpret_cast<const h8v*>(lds(buf, 1, row) + g8);
+      }
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahf[a], blf[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alf[a], bhf[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahf[a], bhf[b], acc[a][b], 0, 0, 0);
+        }
+      if (more) CKH_STORE((st + 1) & 1);
+      __syncthreads();
+    }
+#undef CKH_LOAD
+#undef CKH_STORE
+    // Sim tile through LDS (aliasing the operands): lane holds
+    // D[4 (l / 16) + r][l % 16] of each block, scaled by 2^24
+    float* Sim = reinterpret_cast<float*>(smem_raw);
+    const int r4 = 4 * (lane >> 4);
+    constexpr float unscale = 1.0f / (CKH_SCALE * CKH_SCALE);
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Sim[(ph * 64 + a * 16 + r4 + r) * (CKG_R + 1) + th * 64 + b * 16 + li] =
+              acc[a][b][r] * unscale;
+    __syncthreads();
+    if (srow < np) {
+      const int cend = min(CKG_R, it.row_end - r0);
+      const float* srow_p = Sim + srow * (CKG_R + 1);
+      for (int c = (t & 1) * 64; c < (t & 1) * 64 + 64 && c < cend; ++c) {
+        float v = srow_p[c];
+        if (v > thr) {
+#pragma unroll
+          for (int j = 0; j < CK_KNN_MAX; ++j) {
+            if (j < knn_k && v > L[j]) {
+              const float o = L[j];
+              L[j] = v;
+              v = o;
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < CK_KNN_MAX; ++j)
+            if (j == knn_k - 1) thr = L[j];
+        }
+      }
+    }
+  }
+  if (srow < np) {
+    const int slot = it.pair_beg + srow;
+    float* dst = part + (int64_t)it.part_base +
+                 ((int64_t)slot * it.nchunks + it.chunk) * 2 * knn_k + (t & 1) * knn_k;
+#pragma unroll
+    for (int j = 0; j < CK_KNN_MAX; ++j)
+      if (j < knn_k) dst[j] = L[j];
+  }
+}
+
 // merge partial lists of one pair and form the smoothed score (FP64)
 __global__ void k_cand_merge(const int* __restrict__ bucket, int nslots,
                              const int* __restrict__ slot_class, const int* __restrict__ class_nchunks,
@@ -625,8 +806,25 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
   h2d(ctx, d_csize.get(), csize.data(), C * sizeof(int));
   if (!items.empty()) {
     h2d(ctx, d_items.get(), items.data(), items.size() * sizeof(CkItem));
-    launch(ctx, "cknn_partial", k_cand_gemm, dim3((unsigned)items.size()), dim3(256), 0, Ttn, k,
-           Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k, part.get());
+    if (precision == FPLS_PREC_BF16X3) {
+      // split form: FP16 pairs of both operands, three FP16 MFMA products
+      const int kp = (k + CKH_KS - 1) / CKH_KS * CKH_KS;
+      const int64_t ntr = off[C];
+      DBuf<_Float16> th(ctx, (size_t)w * kp), tl(ctx, (size_t)w * kp);
+      DBuf<_Float16> nh(ctx, (size_t)ntr * kp), nl(ctx, (size_t)ntr * kp);
+      launch(ctx, "cknn_split", k_split_f16, dim3(g1(w * (kp / 8))), dim3(256), 0, Ttn, w, k, kp,
+             th.get(), tl.get());
+      launch(ctx, "cknn_split", k_split_f16, dim3(g1(ntr * (kp / 8))), dim3(256), 0, Tn, ntr, k,
+             kp, nh.get(), nl.get());
+      launch(ctx, "cknn_partial", k_cand_gemm_h, dim3((unsigned)items.size()), dim3(256), 0,
+             (const _Float16*)th.get(), (const _Float16*)tl.get(), kp, (const _Float16*)nh.get(),
+             (const _Float16*)nl.get(), (const int*)bucket.get(), m,
+             (const CkItem*)d_items.get(), knn_k, part.get());
+    } else {
+      launch(ctx, "cknn_partial", k_cand_gemm, dim3((unsigned)items.size()), dim3(256), 0, Ttn,
+             k, Tn, (const int*)bucket.get(), m, (const CkItem*)d_items.get(), knn_k,
+             part.get());
+    }
   }
   if (nslots > 0)
     launch(ctx, "cknn_merge", k_cand_merge, dim3((nslots + 255) / 256), dim3(256), 0,
