@@ -57,7 +57,7 @@ class Opts(ctypes.Structure):
                 ("precision", ctypes.c_int32), ("fit", ctypes.c_int32),
                 ("store_b", ctypes.c_int32), ("label_aware", ctypes.c_int32),
                 ("xprod_route", ctypes.c_int32), ("opls_north", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("xprod", ctypes.c_int32), ("reserved", ctypes.c_int32 * 3)]
 
 
 class ModelInfo(ctypes.Structure):
@@ -65,7 +65,8 @@ class ModelInfo(ctypes.Structure):
                 ("m", ctypes.c_int64), ("kmax", ctypes.c_int32), ("n_ncomp", ctypes.c_int32),
                 ("n_components", ctypes.c_int32), ("b_stored", ctypes.c_int32),
                 ("has_lda", ctypes.c_int32), ("has_cknn", ctypes.c_int32),
-                ("n_classes", ctypes.c_int32), ("ncomp", ctypes.c_int32 * 64)]
+                ("n_classes", ctypes.c_int32), ("ncomp", ctypes.c_int32 * 64),
+                ("xprod_mode", ctypes.c_int32)]
 
 
 class ModelFields(ctypes.Structure):

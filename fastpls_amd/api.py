@@ -61,6 +61,29 @@ class FastPLS(dict):
         return out.reshape(shape, order="F")
 
 
+def should_use_xprod_default(p, q, ncomp) -> bool:
+    """.should_use_xprod_default (R/main.R:1883-1892): the matrix-free route
+    when S = X^T Y would exceed 32 MiB, or for q >= 100 responses and at most
+    10 components."""
+    try:
+        p, q = float(p), float(q)
+        nc = max(int(v) for v in np.atleast_1d(ncomp))
+    except (TypeError, ValueError):
+        return False
+    if not (np.isfinite(p) and np.isfinite(q)):
+        return False
+    s_mb = p * q * 8 / 1024.0 ** 2
+    return bool(s_mb > 32 or (q >= 100 and nc <= 10))
+
+
+def _n_levels(y) -> int:
+    """Level count of host labels (0 when it cannot be read on the host)."""
+    try:
+        return int(len(np.unique(np.asarray(y))))
+    except (TypeError, ValueError, RuntimeError):
+        return 0
+
+
 def _is_factor(y) -> bool:
     if type(y).__module__.startswith("torch"):
         import torch
@@ -123,7 +146,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         fit=False, backend="hip", seed=1, rsvd_oversample=10, rsvd_power=1,
         precision="fp32", label_aware=False, fetch=True, device=0, top=1,
         n_classes=None, gaussian_y=False, gaussian_y_dim=None, north=1, kernel="linear",
-        gamma=None, degree=3, coef0=1.0, tie_log=False) -> FastPLS:
+        gamma=None, degree=3, coef0=1.0, tie_log=False, xprod=None) -> FastPLS:
     """pls() (R/main.R:5860-6314) with backend = "hip".  ``n_classes``: level
     count of device-resident integer labels (optional).  ``gaussian_y``: fit a
     Gaussian compression of the response and decode the predictions
@@ -132,7 +155,13 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     ``method="kernelpls"`` (``kernel``, ``gamma``, ``degree``, ``coef0``) fit
     SIMPLS on the filtered X / the centred training kernel (R/main.R:5956-6006).
     ``tie_log``: with Xtest and an LDA / candidate-kNN head, also return the
-    device's per-row winning margins as ``tie_margin`` (see predict())."""
+    device's per-row winning margins as ``tie_margin`` (see predict()).
+    ``xprod``: None applies .should_use_xprod_default with the response width
+    (R/main.R:6080-6085, the rSVD routes of PLSSVD and SIMPLS); True / False
+    force the route.  A dense-response PLSSVD fit on the route takes the
+    matrix-free range finder of S = X^T Y (pls.model1.rsvd.xprod.precision,
+    R/main.R:6091-6108); label responses keep the one-pass class sums and
+    SIMPLS keeps S resident, so their ``xprod_mode`` stays "materialized"."""
     _check_backend(backend)
     if method in ("opls", "kernelpls"):
         inner_kw = dict(ncomp=ncomp, classifier=classifier, lda_ridge=lda_ridge,
@@ -182,11 +211,20 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     keep: list = []
     X = as_mat(Xtrain, keep)
     classification = _is_factor(Ytrain)
+    if xprod is None:
+        if classification:
+            q_resp = int(n_classes) if n_classes else _n_levels(Ytrain)
+        else:
+            q_resp = 1 if len(getattr(Ytrain, "shape", np.shape(Ytrain))) == 1 \
+                else int(Ytrain.shape[1] if hasattr(Ytrain, "shape") else np.shape(Ytrain)[1])
+        xprod = svd_method != "exact" and method in ("plssvd", "simpls") and \
+            should_use_xprod_default(X.cols, q_resp, ncomp)
     opts = _lib.default_opts(scaling=SCALINGS[scaling], seed=int(seed) & 0xFFFFFFFF,
                              oversample=int(rsvd_oversample), power=int(rsvd_power),
                              svd_method=_lib.SVD_EXACT if svd_method == "exact" else _lib.SVD_RSVD,
                              precision=_lib.PREC_BF16X3 if precision == "bf16x3" else _lib.PREC_FP32,
-                             fit=int(bool(fit)), label_aware=int(bool(label_aware)))
+                             fit=int(bool(fit)), label_aware=int(bool(label_aware)),
+                             xprod=int(bool(xprod)))
     nc_arr, nc_ptr = int_array(np.atleast_1d(ncomp))
     h = ctypes.c_void_p()
     yv = None
@@ -226,9 +264,11 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
     model = FastPLS(ctx, h)
     model["classification"] = classification
     model["lev"] = lev
-    model["xprod_mode"] = "label_aware_stream" if (classification and label_aware) else "dense"
     model["predict_backend"] = "hip"
     inf = model.info()
+    model["xprod_mode"] = {1: "implicit", 2: "label_aware_stream"}.get(int(inf.xprod_mode),
+                                                                      "materialized")
+    model["xprod_default"] = bool(xprod)
     model["ncomp"] = np.array(inf.ncomp[:inf.n_ncomp], dtype=np.int64)
     if fetch:
         _fetch_fields(model, bool(fit))

@@ -40,6 +40,7 @@ struct Model {
   std::vector<int> ncomp;       // effective ncomp per slice
   bool b_stored = false;
   bool label_aware = false;
+  int xprod_mode = 0;           // 0 S formed, 1 matrix-free rSVD of X^T Y, 2 label-aware
 
   DBuf<double> R, Q, V, s, mX, vX, mY;
   DBuf<double> C_latent, W_latent, B;
@@ -101,6 +102,13 @@ void svd_exact_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t
 void rsvd_dev(Ctx* ctx, const double* A, int64_t lda, int64_t rows, int64_t cols, int k,
               int oversample, int power, uint32_t seed, const double* omega_host,
               bool left_only, bool use_full, SvdOut& out);
+// The matrix-free range finder of S = Xs^T Yc (truncated_rsvd_crossprod_double):
+// Xs = (X - 1 mX^T) diag(1 / vX) (vX null: unscaled), Yc = Yd - 1 mY^T on
+// load; Yd FP64 n x m (ld n); om_host the m x l sketch; target <= l factors.
+void rsvd_xprod_core(Ctx* c, const DMat& X, const double* mX, const double* vX,
+                     const double* Yd, int64_t m, const double* mY, int target, int l,
+                     int power, const double* om_host, int precision, bool want_v,
+                     SvdOut& r);
 // Orthonormal basis of the columns of Y (rows x l) by shifted CholeskyQR3, in place.
 void orth_cholqr(Ctx* ctx, double* Y, int64_t rows, int l);
 

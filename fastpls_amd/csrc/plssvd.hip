@@ -399,13 +399,32 @@ void plssvd_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
     fit_preprocess(c, M, X, y ? nullptr : &Y, y, (int32_t)m, o.scaling, M->label_aware,
                    one_pass);
     if (y) M->n_classes = (int)m;
-    DBuf<double> S(c, (size_t)p * m);
-    cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision, one_pass, M->label_aware);
     // truncated SVD of S (src/fastPLS.cpp:4843-4856, plssvd_use_small_exact_svd :318-324)
     SvdOut sv;
     sv.want_v = true;
     const bool full = M->label_aware || o.svd_method == FPLS_SVD_EXACT || max_rank <= 32;
-    rsvd_dev(c, S.get(), p, p, m, kmax, o.oversample, o.power, o.seed, nullptr, false, full, sv);
+    const int64_t xrank = std::min(p, m);
+    const int l_sk = (int)std::min<int64_t>(xrank, (int64_t)kmax + std::max(o.oversample, 0));
+    // matrix-free route (truncated_rsvd_crossprod_double): dense Y, the range
+    // finder, a sketch narrower than S (the reference forms S otherwise)
+    const bool implicit = o.xprod && !y && !full && l_sk < xrank && l_sk <= kJacobiMax;
+    M->xprod_mode = M->label_aware ? 2 : (implicit ? 1 : 0);
+    if (implicit) {
+      DBuf<double> Yd(c, (size_t)n * m);
+      if (Y.dtype == FPLS_F64) copy2d_f64(c, Yd.get(), n, Y.d(), Y.ld, n, m);
+      else
+        for (int64_t j = 0; j < m; ++j) f32_to_f64(c, Yd.get() + j * n, Y.f() + j * Y.ld, n);
+      std::vector<double> om((size_t)m * l_sk);
+      gaussian_host(om.data(), (int64_t)om.size(), o.seed);
+      rsvd_xprod_core(c, X, M->mX.get(), o.scaling == 2 ? M->vX.get() : nullptr, Yd.get(), m,
+                      M->mY.get(), kmax, l_sk, o.power, om.data(), o.precision, true, sv);
+    } else {
+      DBuf<double> S(c, (size_t)p * m);
+      cross_cov(c, M, X, y ? nullptr : &Y, y, m, S.get(), o.precision, one_pass,
+                M->label_aware);
+      rsvd_dev(c, S.get(), p, p, m, kmax, o.oversample, o.power, o.seed, nullptr, false, full,
+               sv);
+    }
     kmax = std::min(kmax, sv.k);
     FPLS_REQUIRE(kmax >= 1, FPLS_ERR_RANK, "plssvd effective rank is < 1 after SVD");
     M->kmax = kmax;
@@ -584,6 +603,7 @@ extern "C" int fpls_model_info_get(const fpls_model* model, fpls_model_info* inf
   info->has_cknn = M.cknn.empty() ? 0 : 1;
   info->n_classes = M.n_classes;
   for (size_t i = 0; i < M.ncomp.size() && i < 64; ++i) info->ncomp[i] = M.ncomp[i];
+  info->xprod_mode = M.xprod_mode;
   return FPLS_OK;
 }
 

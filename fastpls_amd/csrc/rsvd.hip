@@ -249,32 +249,94 @@ extern "C" int fpls_rsvd(fpls_ctx* ctx, const fpls_mat* Am, int32_t k, const fpl
 
 namespace fpls {
 
-// out (rows x l, FP64) = A M, A the data matrix (FP32 on MFMA or FP64), M cols x l
+// out (rows x l, FP64) = (A - 1 shift^T) M, A the data matrix (FP32 on MFMA or
+// FP64), M cols x l; shift (per column of A, FP64) may be null
 static void data_times(Ctx* ctx, const DMat& A, const double* M, int l, double* out,
-                       int precision) {
+                       int precision, const double* shift = nullptr) {
   if (A.dtype == FPLS_F32) {
     DBuf<float> Mf(ctx, (size_t)A.cols * l);
     f64_to_f32(ctx, Mf.get(), M, A.cols * (int64_t)l);
-    gemm_nn_f32(ctx, precision, A.rows, l, A.cols, A.f(), A.ld, nullptr, Mf.get(), A.cols,
+    gemm_nn_f32(ctx, precision, A.rows, l, A.cols, A.f(), A.ld, shift, Mf.get(), A.cols,
                 nullptr, nullptr, out, A.rows);
   } else {
-    gemm_f64(ctx, false, false, A.rows, l, A.cols, 1.0, A.d(), A.ld, nullptr, M, A.cols, nullptr,
+    gemm_f64(ctx, false, false, A.rows, l, A.cols, 1.0, A.d(), A.ld, shift, M, A.cols, nullptr,
              0.0, out, A.rows);
   }
 }
 
-// out (cols x l, FP64) = A^T N, N rows x l
+// out (cols x l, FP64) = (A - 1 shift^T)^T N, N rows x l
 static void data_t_times(Ctx* ctx, const DMat& A, const double* N, int l, double* out,
-                         int precision) {
+                         int precision, const double* shift = nullptr) {
   if (A.dtype == FPLS_F32) {
     DBuf<float> Nf(ctx, (size_t)A.rows * l);
     f64_to_f32(ctx, Nf.get(), N, A.rows * (int64_t)l);
-    gemm_tn_f32(ctx, precision, A.cols, l, A.rows, A.f(), A.ld, nullptr, Nf.get(), A.rows,
+    gemm_tn_f32(ctx, precision, A.cols, l, A.rows, A.f(), A.ld, shift, Nf.get(), A.rows,
                 nullptr, nullptr, out, A.cols);
   } else {
-    gemm_f64(ctx, true, false, A.cols, l, A.rows, 1.0, A.d(), A.ld, nullptr, N, A.rows, nullptr,
+    gemm_f64(ctx, true, false, A.cols, l, A.rows, 1.0, A.d(), A.ld, shift, N, A.rows, nullptr,
              0.0, out, A.cols);
   }
+}
+
+// truncated_rsvd_crossprod_double (src/fastPLS.cpp:550-623) and
+// finalize_rsvd_from_q_b_double (:434-475) for S = Xs^T Yc without forming
+// S: Xs = (X - 1 mX^T) diag(1 / vX), Yc = Yd - 1 mY^T, both applied on load
+// (vX null: no scaling), a_times(M) = Xs^T (Yc M), at_times(M) = Yc^T (Xs M);
+// Yd FP64 (n x m, ld n), om_host the m x l sketch (column-major).
+void rsvd_xprod_core(Ctx* c, const DMat& X, const double* mX, const double* vX,
+                     const double* Yd, int64_t m, const double* mY, int target, int l,
+                     int power, const double* om_host, int precision, bool want_v,
+                     SvdOut& r) {
+  const int64_t n = X.rows, p = X.cols;
+  DBuf<double> Om(c, (size_t)m * l), Ys(c, (size_t)p * l), Z(c, (size_t)m * l),
+      Nn(c, (size_t)n * l), Ms;
+  if (vX) Ms = DBuf<double>(c, (size_t)p * l);
+  to_device(c, Om.get(), om_host, (size_t)m * l);
+  auto xs_times = [&](const double* M, double* out) {  // Xs M (n x l)
+    const double* Mx = M;
+    if (vX) {
+      copy_f64(c, Ms.get(), M, p * (int64_t)l);
+      scale_rows_div(c, Ms.get(), p, p, l, vX);
+      Mx = Ms.get();
+    }
+    data_times(c, X, Mx, l, out, precision, mX);
+  };
+  auto a_times = [&](const double* M, double* out) {  // Xs^T (Yc M)
+    gemm_f64(c, false, false, n, l, m, 1.0, Yd, n, mY, M, m, nullptr, 0.0, Nn.get(), n);
+    data_t_times(c, X, Nn.get(), l, out, precision, mX);
+    if (vX) scale_rows_div(c, out, p, p, l, vX);
+  };
+  auto at_times = [&](const double* M, double* out) {  // Yc^T (Xs M)
+    xs_times(M, Nn.get());
+    gemm_f64(c, true, false, m, l, n, 1.0, Yd, n, mY, Nn.get(), n, nullptr, 0.0, out, m);
+  };
+  a_times(Om.get(), Ys.get());
+  const int q = std::max(power, 0);
+  if (q == 1) {
+    at_times(Ys.get(), Z.get());
+    a_times(Z.get(), Ys.get());
+  } else {
+    for (int it = 0; it < q; ++it) {
+      at_times(Ys.get(), Z.get());
+      orth_cholqr(c, Z.get(), m, l);
+      a_times(Z.get(), Ys.get());
+    }
+  }
+  orth_cholqr(c, Ys.get(), p, l);  // Q
+  // B = (Xs Q)^T Yc  (l x m)
+  xs_times(Ys.get(), Nn.get());
+  DBuf<double> Bm(c, (size_t)l * m);
+  gemm_f64(c, true, false, l, m, n, 1.0, Nn.get(), n, nullptr, Yd, n, mY, 0.0, Bm.get(), l);
+  SvdOut sb;
+  sb.want_v = want_v;
+  svd_exact_dev(c, Bm.get(), l, l, m, target, sb, false);
+  r.k = sb.k;
+  r.s = std::move(sb.s);
+  r.U = DBuf<double>(c, (size_t)p * sb.k);
+  gemm_f64(c, false, false, p, sb.k, l, 1.0, Ys.get(), p, nullptr, sb.U.get(), l, nullptr, 0.0,
+           r.U.get(), p);
+  r.V = std::move(sb.V);
+  r.Vt = std::move(sb.Vt);
 }
 
 // S = X^T Y materialised in FP64 (p x m)

@@ -132,7 +132,14 @@ typedef struct fpls_opts {
                           and sketch S (one pass over X)                      */
   int32_t opls_north;  /* fpls_cv_predict with FPLS_METHOD_OPLS: orthogonal
                           components removed per training fold (default 1)   */
-  int32_t reserved[4];
+  int32_t xprod;       /* fpls_plssvd_fit with a dense Y and the rSVD: 1 takes
+                          the matrix-free FP64 range finder of S = Xs^T Yc
+                          (pls.model1.rsvd.xprod.precision "implicit64",
+                          R/main.R:6080-6108, src/fastPLS.cpp:5518-5590,
+                          550-623); S is never formed.  0 forms S (pls_model1).
+                          pls() sets it by .should_use_xprod_default
+                          (R/main.R:1883-1892)                                */
+  int32_t reserved[3];
 } fpls_opts;
 
 void fpls_opts_default(fpls_opts* opts);
@@ -209,6 +216,8 @@ typedef struct fpls_model_info {
   int32_t has_cknn;
   int32_t n_classes;
   int32_t ncomp[64];   /* effective ncomp per slice (first 64)             */
+  int32_t xprod_mode;  /* how S was treated: 0 formed (dense or class sums),
+                          1 matrix-free ("implicit"), 2 label-aware stream  */
 } fpls_model_info;
 
 /* Field ids for fpls_model_read; sizes in FP64 elements. */

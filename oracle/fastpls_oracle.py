@@ -310,8 +310,12 @@ def _store_B(p: int, m: int, n_slices: int, max_mb: float = 256.0) -> bool:
 
 def pls_model1(Xtrain, Ytrain, ncomp, scaling: int = 1, fit: bool = False,
                svd_method: str = "rsvd", oversample: int = 10, power: int = 1,
-               seed: int = 1) -> Dict:
-    """pls_model1 (src/fastPLS.cpp:4780-4986), optimized branch (default)."""
+               seed: int = 1, xprod: bool = False) -> Dict:
+    """pls_model1 (src/fastPLS.cpp:4780-4986), optimized branch (default).
+    ``xprod``: pls_model1_rsvd_xprod_precision with precision 3 ("implicit64",
+    src/fastPLS.cpp:5518-5690): the same preprocessing and latent
+    coefficients, the SVD of S from the matrix-free range finder
+    truncated_rsvd_crossprod_double (:550-623) without forming S."""
     Xtrain = np.asarray(Xtrain, dtype=np.float64)
     Ytrain = np.asarray(Ytrain, dtype=np.float64)
     if Ytrain.ndim == 1:
@@ -322,10 +326,14 @@ def pls_model1(Xtrain, Ytrain, ncomp, scaling: int = 1, fit: bool = False,
     ncomp = np.array([min(max(int(c), 1), max_rank) for c in np.atleast_1d(ncomp)], dtype=np.int64)
     max_ncomp_eff = min(int(ncomp.max()), max_rank)
     X, Y, mX, vX, mY = _preprocess(Xtrain, Ytrain, scaling)
-    S = X.T @ Y
     use_full = max_rank <= 32  # plssvd_use_small_exact_svd (src/fastPLS.cpp:318-324)
-    svd = truncated_svd(S, max_ncomp_eff, "exact" if svd_method == "exact" else "rsvd",
-                        oversample, power, seed, False, use_full)
+    if xprod and svd_method != "exact":
+        svd = truncated_rsvd_crossprod(X, Y, max_ncomp_eff, oversample, power, seed, False,
+                                       use_full)
+    else:
+        S = X.T @ Y
+        svd = truncated_svd(S, max_ncomp_eff, "exact" if svd_method == "exact" else "rsvd",
+                            oversample, power, seed, False, use_full)
     U, s, V = svd["U"], svd["s"], svd["Vt"].T
     L = ncomp.size
     C_latent = np.zeros((max_ncomp_eff, max_ncomp_eff, L))
@@ -355,6 +363,8 @@ def pls_model1(Xtrain, Ytrain, ncomp, scaling: int = 1, fit: bool = False,
     out = {"C_latent": C_latent, "W_latent": W_latent, "Q": V, "Ttrain": T, "R": U,
            "mX": mX, "vX": vX, "mY": mY, "p": p, "m": m, "ncomp": ncomp, "Yfit": Yfit,
            "R2Y": R2Y, "s": s[:max_ncomp_eff], "pls_method": "plssvd", "B_stored": store_B}
+    if xprod and svd_method != "exact":
+        out["xprod_precision"], out["xprod_mode"] = 3, "implicit"
     if store_B:
         out["B"] = B
     return out

@@ -263,3 +263,33 @@ def test_label_fit_offset_columns(ctx, dtype, scaling, C):
     assert rel_err(m["vX"], ref["vX"]) < (1e-9 if tight else 1e-7)
     assert rel_err(m["B"], ref["B"]) < (1e-9 if tight else 1e-4)
     assert rel_err(align_signs(m["R"], ref["R"]), ref["R"]) < (1e-9 if tight else 1e-4)
+
+
+@pytest.mark.parametrize("dtype,scaling,tol", [(np.float64, "centering", 1e-9),
+                                                (np.float64, "autoscaling", 1e-9),
+                                                (np.float32, "centering", 1e-4)])
+def test_plssvd_xprod_route_matches_oracle(ctx, dtype, scaling, tol):
+    """pls() takes the matrix-free route of .should_use_xprod_default for a
+    dense response with q >= 100 and ncomp <= 10 (R/main.R:6080-6108): S is
+    never formed, xprod_mode is "implicit", and the fit equals the oracle's
+    pls_model1_rsvd_xprod_precision (implicit64) and the device's own
+    formed-S fit."""
+    import fastpls_amd as F
+    rng = np.random.default_rng(23)
+    n, p, q = 400, 250, 120
+    X = _lowrank(rng, n, p, 20, 0.05) + 1.5
+    Y = X[:, :q] @ rng.standard_normal((q, q)) * 0.1 + 0.1 * rng.standard_normal((n, q))
+    ncomp = [3, 6]
+    sc = {"centering": 1, "autoscaling": 2}[scaling]
+    m = F.pls(X.astype(dtype), Y.astype(dtype), method="plssvd", ncomp=ncomp, fit=True,
+              scaling=scaling, seed=4)
+    assert m["xprod_mode"] == "implicit" and m["xprod_default"]
+    ref = O.pls_model1(X.astype(dtype), Y.astype(dtype), ncomp, sc, True, "rsvd", 10, 1, 4,
+                       xprod=True)
+    assert sin_theta(m["R"], ref["R"]) < tol
+    assert rel_err(m["B"], ref["B"]) < tol
+    assert rel_err(m["Yfit"], ref["Yfit"]) < tol
+    formed = F.pls(X.astype(dtype), Y.astype(dtype), method="plssvd", ncomp=ncomp, fit=True,
+                   scaling=scaling, seed=4, xprod=False)
+    assert formed["xprod_mode"] == "materialized"
+    assert rel_err(formed["B"], m["B"]) < tol

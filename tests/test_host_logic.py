@@ -79,3 +79,17 @@ def test_cv_rejects_missing_values():
     X[2, 1] = np.nan
     with pytest.raises(ValueError, match="Missing values are present"):
         F.optim_pls_cv(X, np.arange(6.0), ncomp=1)
+
+
+def test_should_use_xprod_default_rule():
+    """.should_use_xprod_default (R/main.R:1883-1892): S above 32 MiB (strict),
+    or q >= 100 with at most 10 components; non-finite sizes give FALSE."""
+    from fastpls_amd.api import should_use_xprod_default as rule
+    assert rule(4096, 1024, 50) is False          # exactly 32 MiB
+    assert rule(4097, 1024, 50) is True
+    assert rule(50000, 100, 10) is True           # the rSVD-of-X^T Y workload
+    assert rule(768, 100, 99) is False            # the classification workload
+    assert rule(300, 100, 10) is True and rule(300, 99, 10) is False
+    assert rule(300, 100, 11) is False
+    assert rule(300, 100, [3, 12]) is False       # max(ncomp) decides
+    assert rule(float("nan"), 100, 2) is False

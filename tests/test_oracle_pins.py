@@ -420,3 +420,30 @@ def test_candidate_centroids_empty_class_and_fit_by_loops():
         np.testing.assert_allclose(ck["models"][k]["centroids"],
                                    O.candidate_centroids(Sn, yy, 4), rtol=1e-10)
     assert ck["parameters"] == {"knn_k": 3, "tau": 0.2, "alpha": 0.75, "top_m": 20}
+
+
+def test_plssvd_xprod_route_equals_formed_s():
+    """pls_model1_rsvd_xprod_precision ("implicit64", src/fastPLS.cpp:5518-5690)
+    against pls_model1 on the formed S with the same Gaussian sketch: the
+    range finder sees S Omega either way, so R, s, B and Yfit agree to
+    rounding; a missing centring or scaling term in the matrix-free products
+    (a_times = Xs^T (Yc M)) would not."""
+    rng = np.random.default_rng(41)
+    n, p, q = 160, 70, 40
+    X = rng.standard_normal((n, p)) @ rng.standard_normal((p, p)) * 0.3 + 2.0
+    Y = X[:, :q] * 0.5 + rng.standard_normal((n, q)) + 1.0
+    for scaling in (1, 2):
+        a = O.pls_model1(X, Y, [2, 5], scaling, True, "rsvd", 6, 1, 3, xprod=True)
+        b = O.pls_model1(X, Y, [2, 5], scaling, True, "rsvd", 6, 1, 3)
+        assert a["xprod_mode"] == "implicit" and "xprod_mode" not in b
+        np.testing.assert_allclose(a["s"], b["s"], rtol=1e-10)
+        assert rel_err(align_signs(a["R"], b["R"]), b["R"]) < 1e-9
+        assert rel_err(a["B"], b["B"]) < 1e-9
+        assert rel_err(a["Yfit"], b["Yfit"]) < 1e-10
+    # with one operand centred the other's centring cancels (1^T Yc = 0), but
+    # dropping the autoscaling of X in the products changes the result
+    Xs, Ys, *_ = O._preprocess(X, Y, 2)
+    Xc, *_ = O._preprocess(X, Y, 1)
+    bad = O.truncated_rsvd_crossprod(Xc, Ys, 5, 6, 1, 3)
+    good = O.truncated_rsvd_crossprod(Xs, Ys, 5, 6, 1, 3)
+    assert np.abs(bad["s"] / bad["s"][0] - good["s"] / good["s"][0]).max() > 1e-3
