@@ -120,7 +120,9 @@ def run_gpu(args, data):
                 kern[nm] = {"launches_per_step": n / tsteps, "ms_per_step": ms / tsteps}
         ctx.timing(False)
         out[prec] = {"fits_per_s": args.steps / dt, "ms_per_step": 1e3 * dt / args.steps,
-                     "labels": labels, "R": R, "B": B, "kernels": kern}
+                     "labels": labels, "R": R, "B": B, "kernels": kern,
+                     "autotune": {k: v for k, v in ctx.autotune_report().items()
+                                  if ("bf16x3" in k) == (prec == "bf16x3")}}
     return out
 
 
@@ -1374,6 +1376,7 @@ def main():
         "roofline": roof,
         "oracle_errors": errors,
         "kernels": {p: v["kernels"] for p, v in res.items()},
+        "autotune": res[chosen].get("autotune"),
     }
     if cpu is not None:
         out["cpu_baseline"] = cpu

@@ -107,6 +107,8 @@ SIGNATURES = {
     "fpls_ctx_synchronize": (ctypes.c_int, [_vp]),
     "fpls_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "fpls_kernel_time": (ctypes.c_int, [_vp, ctypes.c_char_p, _P(ctypes.c_int64), _P(ctypes.c_double)]),
+    "fpls_autotune_set": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "fpls_autotune_report": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int64]),
     "fpls_plssvd_fit": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp,
                                        ctypes.c_int32, _P(Opts), _P(_vp)]),
     "fpls_simpls_fit": (ctypes.c_int, [_vp, _P(Mat), _P(Mat), _P(IVec), ctypes.c_int32, _vp,
@@ -323,6 +325,25 @@ class Context:
         ms = ctypes.c_double()
         self.check(self.lib.fpls_kernel_time(self.h, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
         return n.value, ms.value
+
+    def autotune(self, mode: int = 1, force_mb: int = 0, force_splits: int = 0):
+        """The projection-tile autotuner: mode 1 times candidates on a shape's
+        first call, 0 keeps the built-in rule; force_* pin a configuration."""
+        self.check(self.lib.fpls_autotune_set(self.h, int(mode), int(force_mb),
+                                              int(force_splits)))
+
+    def autotune_report(self) -> dict:
+        """The tuning table: {shape key: {mb, rows, splits, us, tuned}}."""
+        import json
+        cap = 1 << 16
+        while True:
+            buf = ctypes.create_string_buffer(cap)
+            rc = self.lib.fpls_autotune_report(self.h, buf, cap)
+            if rc == 0:
+                return json.loads(buf.value.decode())
+            if rc <= cap:
+                raise RuntimeError("fpls_autotune_report failed")
+            cap = rc
 
     def synchronize(self):
         self.check(self.lib.fpls_ctx_synchronize(self.h))

@@ -117,6 +117,21 @@ struct Ctx {
   // Captured launch plan of the batched cross-validation (cvb.hip), kept while
   // the shapes and X stay the same.
   std::shared_ptr<void> cv_plan;
+  // Autotuner of the skinny projection tiles (k_gemm2.hip): per shape key the
+  // chosen (tile height mb, split-K count) and its time in microseconds.
+  // tune_mode 1: the candidates are timed on a shape's first call (not while
+  // the stream is captured or in the fixed-arena mode); 0: the built-in rule.
+  // force_mb / force_splits (> 0) pin a configuration.
+  struct TuneChoice {
+    int mb = 2;
+    int splits = 1;
+    double us = 0.0;
+    bool tuned = false;
+  };
+  int tune_mode = 1;
+  bool tune_log = false;
+  int force_mb = 0, force_splits = 0;
+  std::map<std::string, TuneChoice> tune_table;
   int32_t* defer_check(int count, int code, const std::string& msg, int fail_value = 0);
   void flush_checks();
   // Device -> host copies completed by the next flush (the entry point's one

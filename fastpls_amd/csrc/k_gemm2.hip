@@ -14,7 +14,10 @@
 // B[l/16][l%16]; 16x16x32 bf16 -- lane l supplies A[l%16][8(l/16)..+7],
 // B[8(l/16)..+7][l%16]; both accumulate D[4(l/16)+r][l%16].
 #include <algorithm>
+#include <cstdio>
 #include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "fpls_internal.h"
 
@@ -480,23 +483,26 @@ __global__ __launch_bounds__(256) void k_g2_reduce_out(int64_t M, int64_t N, int
 // ---------------------------------------------------------------- skinny NN
 // T (M x N, N <= 128) = (X - 1 shA^T) B with X streamed straight from HBM into
 // MFMA operand registers (no LDS for the big operand).  Workgroup = 4 waves x
-// 32 rows; each wave holds 2 x NB accumulator blocks (NB 16-column blocks of
-// B staged: 4 for N <= 64, else 8).  B (K x N, the zero-padded copy) goes
+// 16 MB rows; each wave holds MB x NB accumulator blocks (NB 16-column blocks
+// of B staged: 4 for N <= 64, else 8; MB, the tile height, chosen per shape by
+// the autotuner below).  B (K x N, the zero-padded copy) goes
 // through LDS one 32-deep K chunk at a time (double buffered, one barrier per
 // chunk); the X operand of chunk c+1 is loaded into registers while the MFMAs
 // of chunk c run.  With split-K (part != NULL) each workgroup writes its
 // slab's FP32 partial.  The kernel is latency bound: loading two consecutive
 // rows per lane as one vector costs registers and, at the lower occupancy,
 // time (measured), so rows go one per lane.
-constexpr int P_BM = 128, P_BK = 32, P_NT = 256;
+constexpr int P_BK = 32, P_NT = 256;
+constexpr int p_bm(int mb) { return 64 * mb; }  // rows of a workgroup tile
 constexpr int P_LDB = P_BK + 8;  // bf16 row length of the [n][k] B tiles
 template <bool BF, int NB>
 struct PSmem {
   static constexpr int value = BF ? (2 * 2 * 16 * NB * P_LDB) / 2 : 2 * P_BK * (16 * NB + 4);
 };
 
-// NB: 16-column blocks of B staged and multiplied (N <= 16 NB)
-template <bool BF, bool CEN, int NB>
+// NB: 16-column blocks of B staged and multiplied (N <= 16 NB); MB: 16-row
+// blocks per wave
+template <bool BF, bool CEN, int NB, int MB>
 __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
                                                const float* A, int64_t lda,
                                                const float* shA,
@@ -510,19 +516,20 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   constexpr int TPC = P_NT / BW;           // staging threads per column
   constexpr int KPT = P_BK / TPC;          // k values per staging thread (8 or 16)
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int64_t row0 = (int64_t)blockIdx.x * P_BM + w * 32;
+  const int64_t row0 = (int64_t)blockIdx.x * p_bm(MB) + w * 16 * MB;
   // K slab of this workgroup (kchunk is a multiple of P_BK; B's zero rows pad
   // the last slab up to a whole chunk)
   const int64_t kbeg = (int64_t)blockIdx.y * kchunk;
   const int64_t kend = std::min<int64_t>(K, kbeg + kchunk);
   const int nb = (N + 15) >> 4;
-  v4f acc[2][NB];
+  v4f acc[MB][NB];
 #pragma unroll
-  for (int a = 0; a < 2; ++a)
+  for (int a = 0; a < MB; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) acc[a][b] = v4f{0.f, 0.f, 0.f, 0.f};
-  const float* arow0 = A + std::min<int64_t>(row0 + lane % 16, M - 1);
-  const float* arow1 = A + std::min<int64_t>(row0 + 16 + lane % 16, M - 1);
+  const float* arow[MB];
+#pragma unroll
+  for (int a = 0; a < MB; ++a) arow[a] = A + std::min<int64_t>(row0 + 16 * a + lane % 16, M - 1);
   // k offset (within a chunk) of operand element j of this lane
   auto kof = [&](int j) { return BF ? 8 * (lane / 16) + j : 4 * j + lane / 16; };
   // the centring shifts arrive as FP32 (converted once per call): any
@@ -532,16 +539,16 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   // (a workgroup is bound by the bytes it has in flight, not by arithmetic),
   // and no register that is still the target of a load is ever copied
   struct ASet {
-    float a[8], b[8], m[8];
+    float a[MB][8], m[8];
     float4 q[KPT / 4];
   };
   ASet R0, R1, R2;
-  auto fetchA = [&](int64_t k0, float (&x0)[8], float (&x1)[8], float (&mx)[8]) {
+  auto fetchA = [&](int64_t k0, float (&x)[MB][8], float (&mx)[8]) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int64_t gk = std::min<int64_t>(k0 + kof(j), K - 1);
-      x0[j] = arow0[gk * lda];
-      x1[j] = arow1[gk * lda];
+#pragma unroll
+      for (int a = 0; a < MB; ++a) x[a][j] = arow[a][gk * lda];
       mx[j] = CEN ? shA[gk] : 0.f;
     }
   };
@@ -584,23 +591,22 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
       }
     }
   };
-  auto compute = [&](int buf, const float (&ac0)[8], const float (&ac1)[8],
-                     const float (&mc)[8]) {
+  auto compute = [&](int buf, const float (&ac)[MB][8], const float (&mc)[8]) {
     if (BF) {
       const __bf16* base = reinterpret_cast<const __bf16*>(smem);
       const __bf16* Bh = base + (0 * 2 + buf) * BW * P_LDB;
       const __bf16* Bl = base + (1 * 2 + buf) * BW * P_LDB;
-      v8bf ah0, al0, ah1, al1;
+      v8bf ah[MB], al[MB];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
-        __bf16 hh, ll;
         const float me = mc[e];
-        split_bf(ac0[e] - me, hh, ll);
-        ah0[e] = hh;
-        al0[e] = ll;
-        split_bf(ac1[e] - me, hh, ll);
-        ah1[e] = hh;
-        al1[e] = ll;
+#pragma unroll
+        for (int a = 0; a < MB; ++a) {
+          __bf16 hh, ll;
+          split_bf(ac[a][e] - me, hh, ll);
+          ah[a][e] = hh;
+          al[a][e] = ll;
+        }
       }
       const int ko = 8 * (lane / 16);
 #pragma unroll
@@ -609,12 +615,12 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
           const int n = b * 16 + lane % 16;
           const v8bf bh = *reinterpret_cast<const v8bf*>(Bh + n * P_LDB + ko);
           const v8bf bl = *reinterpret_cast<const v8bf*>(Bl + n * P_LDB + ko);
-          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al0, bh, acc[0][b], 0, 0, 0);
-          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah0, bl, acc[0][b], 0, 0, 0);
-          acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah0, bh, acc[0][b], 0, 0, 0);
-          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al1, bh, acc[1][b], 0, 0, 0);
-          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah1, bl, acc[1][b], 0, 0, 0);
-          acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah1, bh, acc[1][b], 0, 0, 0);
+#pragma unroll
+          for (int a = 0; a < MB; ++a) {
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl, acc[a][b], 0, 0, 0);
+            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh, acc[a][b], 0, 0, 0);
+          }
         }
       }
     } else {
@@ -622,14 +628,17 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
 #pragma unroll
       for (int s = 0; s < 8; ++s) {
         const float ms = mc[s];
-        const float x0 = ac0[s] - ms, x1 = ac1[s] - ms;
+        float x[MB];
+#pragma unroll
+        for (int a = 0; a < MB; ++a) x[a] = ac[a][s] - ms;
         const int kr = 4 * s + lane / 16;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           if (b < nb) {
             const float bv = Bs[kr * (BW + 4) + b * 16 + lane % 16];
-            acc[0][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x0, bv, acc[0][b], 0, 0, 0);
-            acc[1][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x1, bv, acc[1][b], 0, 0, 0);
+#pragma unroll
+            for (int a = 0; a < MB; ++a)
+              acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(x[a], bv, acc[a][b], 0, 0, 0);
           }
         }
       }
@@ -639,18 +648,18 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   // chunk c uses register set c % 3 and LDS buffer c & 1
   auto step = [&](int c, ASet& cur, ASet& nxt, ASet& far) {
     if (c + 2 < nk) {
-      fetchA(kbeg + (int64_t)(c + 2) * P_BK, far.a, far.b, far.m);
+      fetchA(kbeg + (int64_t)(c + 2) * P_BK, far.a, far.m);
       fetchB(kbeg + (int64_t)(c + 2) * P_BK, far.q);
     }
     __builtin_amdgcn_sched_barrier(0);
-    compute(c & 1, cur.a, cur.b, cur.m);
+    compute(c & 1, cur.a, cur.m);
     if (c + 1 < nk) storeB((c + 1) & 1, nxt.q);
     __syncthreads();
   };
-  fetchA(kbeg, R0.a, R0.b, R0.m);
+  fetchA(kbeg, R0.a, R0.m);
   fetchB(kbeg, R0.q);
   if (nk > 1) {
-    fetchA(kbeg + P_BK, R1.a, R1.b, R1.m);
+    fetchA(kbeg + P_BK, R1.a, R1.m);
     fetchB(kbeg + P_BK, R1.q);
   }
   storeB(0, R0.q);
@@ -664,7 +673,7 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   if (part) {  // split-K: raw FP32 slab partial, M x N column-major
     float* P = part + (int64_t)blockIdx.y * M * N;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
+    for (int a = 0; a < MB; ++a) {
       const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
 #pragma unroll
       for (int b = 0; b < NB; ++b) {
@@ -678,7 +687,7 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
     return;
   }
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
+  for (int a = 0; a < MB; ++a) {
     const int64_t rbase = row0 + 16 * a + 4 * (lane / 16);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -698,13 +707,17 @@ __global__ __launch_bounds__(P_NT) void k_proj(int64_t M, int N, int64_t K,
   }
 }
 
-// k_proj with the column-block count picked from N.
+// k_proj with the column-block count picked from N and the tile height mb.
 template <bool BF, bool CEN, typename... Args>
-static void launch_proj(Ctx* ctx, const char* name, int N, dim3 grid, Args... args) {
-  if (N <= 64)
-    launch(ctx, name, k_proj<BF, CEN, 4>, grid, dim3(P_NT), 0, args...);
-  else
-    launch(ctx, name, k_proj<BF, CEN, 8>, grid, dim3(P_NT), 0, args...);
+static void launch_proj(Ctx* ctx, const char* name, int N, int mb, dim3 grid, Args... args) {
+  if (N <= 64) {
+    if (mb == 1) launch(ctx, name, k_proj<BF, CEN, 4, 1>, grid, dim3(P_NT), 0, args...);
+    else if (mb == 4) launch(ctx, name, k_proj<BF, CEN, 4, 4>, grid, dim3(P_NT), 0, args...);
+    else launch(ctx, name, k_proj<BF, CEN, 4, 2>, grid, dim3(P_NT), 0, args...);
+  } else {
+    if (mb == 1) launch(ctx, name, k_proj<BF, CEN, 8, 1>, grid, dim3(P_NT), 0, args...);
+    else launch(ctx, name, k_proj<BF, CEN, 8, 2>, grid, dim3(P_NT), 0, args...);
+  }
 }
 
 // B (K x N, FP64, ld ldb; or its transpose, B(k, n) = src[n + k * ldb]) as the
@@ -742,6 +755,181 @@ DBuf<float> nn_shift_operand(Ctx* ctx, const double* shift, int64_t K) {
   return out;
 }
 
+// ---------------------------------------------------------------- projection autotuner
+// One skinny projection (N <= 128) and its launch configurations: tile height
+// mb (64 mb rows per workgroup) and split-K count (FP32 slab partials summed
+// in FP64 by the epilogue kernel).
+struct ProjShape {
+  bool bf, cen;
+  int64_t M;
+  int N;
+  int64_t K;
+  const float* A;
+  int64_t lda;
+  const float* sh32;
+  const float* B;
+  int64_t ldb;
+  const double* offset;
+  float* Cf;
+  double* Cd;
+  int64_t ldc;
+  int crow;
+};
+
+// The built-in split rule: fewer row tiles than CUs splits K into slabs so
+// that about two workgroups run per CU, each slab at least 128 deep.
+static int proj_rule_splits(const Ctx* ctx, const ProjShape& s, int mb) {
+  const int64_t tiles = (s.M + p_bm(mb) - 1) / p_bm(mb);
+  if (!(tiles < (int64_t)ctx->num_cus && s.K >= 256)) return 1;
+  int64_t splits = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
+  splits = std::min<int64_t>(splits, s.K / 128);
+  splits = std::min<int64_t>(
+      splits, std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * s.M * s.N)));
+  return (int)std::max<int64_t>(1, splits);
+}
+
+static void proj_run(Ctx* ctx, const ProjShape& s, int mb, int splits_req) {
+  const int64_t tiles = (s.M + p_bm(mb) - 1) / p_bm(mb);
+  const char* name = s.cen ? (s.bf ? "project_bf16x3" : "project_f32")
+                           : (s.bf ? "gemm_nn_bf16x3" : "gemm_nn_f32");
+  if (splits_req > 1) {
+    int64_t kchunk = (s.K + splits_req - 1) / splits_req;
+    kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
+    const int64_t splits = (s.K + kchunk - 1) / kchunk;
+    DBuf<float> part(ctx, (size_t)splits * s.M * s.N);
+    const dim3 g((unsigned)tiles, (unsigned)splits);
+    const double* noff = nullptr;
+    float* ncf = nullptr;
+    double* ncd = nullptr;
+    if (s.cen) {
+      if (s.bf)
+        launch_proj<true, true>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                                s.ldb, noff, ncf, ncd, (int64_t)0, 0, kchunk, part.get());
+      else
+        launch_proj<false, true>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                                 s.ldb, noff, ncf, ncd, (int64_t)0, 0, kchunk, part.get());
+    } else {
+      if (s.bf)
+        launch_proj<true, false>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                                 s.ldb, noff, ncf, ncd, (int64_t)0, 0, kchunk, part.get());
+      else
+        launch_proj<false, false>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                                  s.ldb, noff, ncf, ncd, (int64_t)0, 0, kchunk, part.get());
+    }
+    launch(ctx, "gemm_nn_reduce", k_g2_reduce_out, dim3((unsigned)((s.M * s.N + 255) / 256)),
+           dim3(256), 0, s.M, (int64_t)s.N, (int)splits, (const float*)part.get(), s.offset, s.Cf,
+           s.Cd, s.ldc, s.crow);
+    return;
+  }
+  const dim3 g((unsigned)tiles);
+  if (s.cen) {
+    if (s.bf)
+      launch_proj<true, true>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                              s.ldb, s.offset, s.Cf, s.Cd, s.ldc, s.crow, s.K, (float*)nullptr);
+    else
+      launch_proj<false, true>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda, s.sh32, s.B,
+                               s.ldb, s.offset, s.Cf, s.Cd, s.ldc, s.crow, s.K, (float*)nullptr);
+  } else {
+    if (s.bf)
+      launch_proj<true, false>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda,
+                               (const float*)nullptr, s.B, s.ldb, s.offset, s.Cf, s.Cd, s.ldc,
+                               s.crow, s.K, (float*)nullptr);
+    else
+      launch_proj<false, false>(ctx, name, s.N, mb, g, s.M, s.N, s.K, s.A, s.lda,
+                                (const float*)nullptr, s.B, s.ldb, s.offset, s.Cf, s.Cd, s.ldc,
+                                s.crow, s.K, (float*)nullptr);
+  }
+}
+
+// The configuration of a shape: pinned, cached, timed now, or the rule.
+// Timing first runs the rule's configuration three times (clocks and caches
+// settle), then times every candidate three times, each run between its own
+// HIP events on the context stream, and keeps its fastest run; a candidate
+// replaces the rule only when it is at least 3 % faster.  Every candidate
+// writes the same output, and the caller's launch with the winner writes it
+// last.
+static Ctx::TuneChoice proj_choose(Ctx* ctx, const ProjShape& s) {
+  Ctx::TuneChoice ch;
+  if (ctx->force_mb > 0 || ctx->force_splits > 0) {
+    ch.mb = ctx->force_mb > 0 ? ctx->force_mb : 2;
+    if (s.N > 64 && ch.mb == 4) ch.mb = 2;  // NB = 8 tiles exist for mb 1 and 2
+    ch.splits = ctx->force_splits > 0 ? ctx->force_splits : proj_rule_splits(ctx, s, ch.mb);
+    return ch;
+  }
+  char key[160];
+  std::snprintf(key, sizeof(key), "proj:%s%s:M%lld:N%d:K%lld", s.bf ? "bf16x3" : "f32",
+                s.cen ? ":centred" : "", (long long)s.M, s.N, (long long)s.K);
+  auto it = ctx->tune_table.find(key);
+  if (it != ctx->tune_table.end()) return it->second;
+  // the rule: 64-row tiles for the split-BF16x3 form (its per-chunk split
+  // work favours more workgroups), 128-row tiles for FP32 (measured on the
+  // 50,000 x 768 projection)
+  ch.mb = s.bf ? 1 : 2;
+  ch.splits = proj_rule_splits(ctx, s, ch.mb);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  const bool capturing = hipStreamIsCapturing(ctx->stream, &cap) == hipSuccess &&
+                         cap != hipStreamCaptureStatusNone;
+  if (ctx->tune_mode == 0 || capturing || ctx->arena_base || s.M * (int64_t)s.N < 4096) {
+    if (!capturing && !ctx->arena_base) ctx->tune_table[key] = ch;
+    return ch;
+  }
+  std::vector<std::pair<int, int>> cand;
+  for (int mb : {1, 2, 4}) {
+    if (mb == 4 && s.N > 64) continue;
+    const int r = proj_rule_splits(ctx, s, mb);
+    for (int sp : {1, r, 2 * r}) {
+      if (sp > 1 && (sp > s.K / 128 || (int64_t)sp * 4 * s.M * s.N > (int64_t)ctx->partial_budget()))
+        continue;
+      if (std::find(cand.begin(), cand.end(), std::make_pair(mb, sp)) == cand.end())
+        cand.emplace_back(mb, sp);
+    }
+  }
+  hipEvent_t e0, e1;
+  FPLS_HIP(hipEventCreate(&e0));
+  FPLS_HIP(hipEventCreate(&e1));
+  auto time_one = [&](int mb, int sp) {
+    double us = 1e30;
+    for (int r = 0; r < 3; ++r) {
+      FPLS_HIP(hipEventRecord(e0, ctx->stream));
+      proj_run(ctx, s, mb, sp);
+      FPLS_HIP(hipEventRecord(e1, ctx->stream));
+      FPLS_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FPLS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      us = std::min(us, 1e3 * (double)ms);
+    }
+    return us;
+  };
+  for (int r = 0; r < 3; ++r) proj_run(ctx, s, ch.mb, ch.splits);
+  const double rule_us = time_one(ch.mb, ch.splits);
+  double best = rule_us;
+  int bmb = ch.mb, bsp = ch.splits;
+  for (const auto& c : cand) {
+    const double us = (c.first == ch.mb && c.second == ch.splits) ? rule_us
+                                                                    : time_one(c.first, c.second);
+    if (ctx->tune_log)
+      std::fprintf(stderr, "fpls autotune %s: rows %d splits %d  %.2f us\n", key, 64 * c.first,
+                   c.second, us);
+    if (us < best) {
+      best = us;
+      bmb = c.first;
+      bsp = c.second;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (best < 0.97 * rule_us) {
+    ch.mb = bmb;
+    ch.splits = bsp;
+  } else {
+    best = rule_us;
+  }
+  ch.us = best;
+  ch.tuned = true;
+  ctx->tune_table[key] = ch;
+  return ch;
+}
+
 void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const float* A,
                  int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                  const double* offset, float* Cf, double* Cd, int64_t ldc, bool c_rowmajor) {
@@ -771,59 +959,10 @@ void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K
   // centred contractions (projections of X) are timed as "project_*"
   const bool bf = precision == FPLS_PREC_BF16X3;
   if (N <= 128) {
-    const int64_t tiles = (M + P_BM - 1) / P_BM;
-    const int n32 = (int)N;
-    if (tiles < (int64_t)ctx->num_cus && K >= 256) {
-      // fewer row tiles than CUs: split K into slabs (FP32 partials, summed in
-      // FP64; the reduction applies the offset and writes either output)
-      int64_t splits = std::max<int64_t>(1, (2 * (int64_t)ctx->num_cus + tiles - 1) / tiles);
-      splits = std::min<int64_t>(splits, K / 128);
-      splits = std::min<int64_t>(splits, std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N)));
-      int64_t kchunk = (K + splits - 1) / splits;
-      kchunk = (kchunk + P_BK - 1) / P_BK * P_BK;
-      splits = (K + kchunk - 1) / kchunk;
-      DBuf<float> part(ctx, (size_t)splits * M * N);
-      dim3 sgrid((unsigned)tiles, (unsigned)splits);
-      const float* sh32 = shift32;
-      if (shiftA) {
-        if (bf)
-          launch_proj<true, true>(ctx, "project_bf16x3", n32, sgrid, M, n32, K, A, lda, sh32, B,
-                                  ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
-                                  (int64_t)0, 0, kchunk, part.get());
-        else
-          launch_proj<false, true>(ctx, "project_f32", n32, sgrid, M, n32, K, A, lda, sh32, B,
-                                   ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
-                                   (int64_t)0, 0, kchunk, part.get());
-      } else {
-        if (bf)
-          launch_proj<true, false>(ctx, "gemm_nn_bf16x3", n32, sgrid, M, n32, K, A, lda, sh32, B,
-                                   ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
-                                   (int64_t)0, 0, kchunk, part.get());
-        else
-          launch_proj<false, false>(ctx, "gemm_nn_f32", n32, sgrid, M, n32, K, A, lda, sh32, B,
-                                    ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
-                                    (int64_t)0, 0, kchunk, part.get());
-      }
-      launch(ctx, "gemm_nn_reduce", k_g2_reduce_out, dim3((unsigned)((M * N + 255) / 256)),
-             dim3(256), 0, M, N, (int)splits, (const float*)part.get(), offset, Cf, Cd, ldc, crow);
-      return;
-    }
-    dim3 pgrid((unsigned)tiles);
-    if (shiftA) {
-      if (bf)
-        launch_proj<true, true>(ctx, "project_bf16x3", n32, pgrid, M, n32, K, A, lda, shift32, B,
-                                ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
-      else
-        launch_proj<false, true>(ctx, "project_f32", n32, pgrid, M, n32, K, A, lda, shift32, B,
-                                 ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
-    } else {
-      if (bf)
-        launch_proj<true, false>(ctx, "gemm_nn_bf16x3", n32, pgrid, M, n32, K, A,
-               lda, (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
-      else
-        launch_proj<false, false>(ctx, "gemm_nn_f32", n32, pgrid, M, n32, K, A, lda,
-               (const float*)nullptr, B, ldb, offset, Cf, Cd, ldc, crow, K, (float*)nullptr);
-    }
+    const ProjShape sh{bf, shiftA != nullptr, M, (int)N, K, A, lda, shift32, B, ldb, offset,
+                       Cf, Cd, ldc, crow};
+    const Ctx::TuneChoice ch = proj_choose(ctx, sh);
+    proj_run(ctx, sh, ch.mb, ch.splits);
     return;
   }
   if (shiftA) {
@@ -1661,24 +1800,24 @@ void proj_splitk(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const flo
   FPLS_REQUIRE(N >= 1 && N <= 128 && kchunk % P_BK == 0 && splits >= 1 &&
                    (int64_t)(splits - 1) * kchunk < K && (int64_t)splits * kchunk >= K,
                FPLS_ERR_ARG, "proj_splitk: bad shape");
-  dim3 grid((unsigned)((M + P_BM - 1) / P_BM), (unsigned)splits);
+  dim3 grid((unsigned)((M + p_bm(2) - 1) / p_bm(2)), (unsigned)splits);
   const bool bf = precision == FPLS_PREC_BF16X3;
   if (shA32) {
     if (bf)
-      launch_proj<true, true>(ctx, "cv_project_bf16x3", N, grid, M, N, K, A, lda,
+      launch_proj<true, true>(ctx, "cv_project_bf16x3", N, 2, grid, M, N, K, A, lda,
              shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
              (int64_t)0, 0, kchunk, part);
     else
-      launch_proj<false, true>(ctx, "cv_project_f32", N, grid, M, N, K, A, lda,
+      launch_proj<false, true>(ctx, "cv_project_f32", N, 2, grid, M, N, K, A, lda,
              shA32, Bpad, ldb, (const double*)nullptr, (float*)nullptr, (double*)nullptr,
              (int64_t)0, 0, kchunk, part);
   } else {
     if (bf)
-      launch_proj<true, false>(ctx, "cv_project_bf16x3", N, grid, M, N, K, A,
+      launch_proj<true, false>(ctx, "cv_project_bf16x3", N, 2, grid, M, N, K, A,
              lda, (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
              (double*)nullptr, (int64_t)0, 0, kchunk, part);
     else
-      launch_proj<false, false>(ctx, "cv_project_f32", N, grid, M, N, K, A, lda,
+      launch_proj<false, false>(ctx, "cv_project_f32", N, 2, grid, M, N, K, A, lda,
              (const float*)nullptr, Bpad, ldb, (const double*)nullptr, (float*)nullptr,
              (double*)nullptr, (int64_t)0, 0, kchunk, part);
   }

@@ -1,6 +1,9 @@
 // Context, HBM pool, staging, timing and the small host utilities of the
 // gfx950 fastPLS library.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <random>
 
 #include "fpls_internal.h"
@@ -359,7 +362,39 @@ int fpls_ctx_create(int device, fpls_ctx** out) {
     return FPLS_ERR_HIP;
   }
   scratch_setup(&c->c);
+  if (const char* e = std::getenv("FPLS_AUTOTUNE")) c->c.tune_mode = std::atoi(e) != 0 ? 1 : 0;
+  if (const char* e = std::getenv("FPLS_AUTOTUNE_LOG")) c->c.tune_log = std::atoi(e) != 0;
   *out = c;
+  return FPLS_OK;
+}
+
+int fpls_autotune_set(fpls_ctx* ctx, int32_t mode, int32_t force_mb, int32_t force_splits) {
+  if (!ctx) return FPLS_ERR_ARG;
+  if (force_mb != 0 && force_mb != 1 && force_mb != 2 && force_mb != 4) return FPLS_ERR_ARG;
+  if (force_splits < 0) return FPLS_ERR_ARG;
+  ctx->c.tune_mode = mode != 0 ? 1 : 0;
+  ctx->c.force_mb = force_mb;
+  ctx->c.force_splits = force_splits;
+  ctx->c.tune_table.clear();
+  return FPLS_OK;
+}
+
+int fpls_autotune_report(fpls_ctx* ctx, char* buf, int64_t cap) {
+  if (!ctx || (!buf && cap > 0)) return FPLS_ERR_ARG;
+  std::string js = "{";
+  bool first = true;
+  for (const auto& kv : ctx->c.tune_table) {
+    char one[256];
+    std::snprintf(one, sizeof(one), "%s\"%s\": {\"mb\": %d, \"rows\": %d, \"splits\": %d, "
+                  "\"us\": %.2f, \"tuned\": %s}", first ? "" : ", ", kv.first.c_str(),
+                  kv.second.mb, 64 * kv.second.mb, kv.second.splits, kv.second.us,
+                  kv.second.tuned ? "true" : "false");
+    js += one;
+    first = false;
+  }
+  js += "}";
+  if ((int64_t)js.size() + 1 > cap) return (int)std::min<size_t>(js.size() + 1, 1u << 30);
+  std::memcpy(buf, js.c_str(), js.size() + 1);
   return FPLS_OK;
 }
 

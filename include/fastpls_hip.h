@@ -157,6 +157,19 @@ int fpls_ctx_synchronize(fpls_ctx* ctx);
    by HIP events on the context stream; fpls_kernel_time returns the count
    and the summed milliseconds since the last reset. */
 int fpls_timing_enable(fpls_ctx* ctx, int enable);
+/* Extension (no reference counterpart; the reference's CUDA kernels use fixed
+   launch shapes): the autotuner of the skinny projection tiles (T = X R and
+   the other thin products of the data matrix).  mode 1 (the default, also
+   FPLS_AUTOTUNE=1 in the environment) times the candidate tile heights (64,
+   128, 256 rows per workgroup) and split-K counts on a shape's first call
+   and keeps the fastest per context; mode 0 uses the built-in rule.
+   force_mb (1, 2, 4; 0 = none) and force_splits (>= 1; 0 = none) pin a
+   configuration.  Clears the table. */
+int fpls_autotune_set(fpls_ctx* ctx, int32_t mode, int32_t force_mb, int32_t force_splits);
+/* The tuning table as JSON text ({"key": {"mb", "rows", "splits", "us",
+   "tuned"}, ...}); returns FPLS_OK, or the buffer size it needs when cap is
+   too small. */
+int fpls_autotune_report(fpls_ctx* ctx, char* buf, int64_t cap);
 int fpls_kernel_time(fpls_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
 
 /* ---- fits ---------------------------------------------------------------- */
