@@ -70,6 +70,71 @@ def calibration_copy(mib):
     del src, dst
 
 
+def pmc_summary(fetch_csv, write_csv, copy_mib, keys, workload):
+    """Per-dispatch HBM bytes from two rocprofv3 counter passes (FETCH_SIZE,
+    then WRITE_SIZE; ``*_counter_collection.csv``), each scaled by what the
+    same counter reads for the calibration copy of ``copy_mib`` MiB.  Prints a
+    markdown table and records ``keys`` ({name: kernel-name prefix}) in
+    profiles/hbm_traffic.json."""
+    import csv
+    import collections
+
+    def load(path, counter):
+        per = collections.defaultdict(list)
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name", counter) != counter:
+                    continue
+                per[row["Kernel_Name"]].append(float(row["Counter_Value"]))
+        return per
+
+    fe, wr = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    kib = copy_mib * 1024.0
+    cf = max((v for k, vs in fe.items() if "copyBuffer" in k for v in vs), default=0.0)
+    cw = max((v for k, vs in wr.items() if "copyBuffer" in k for v in vs), default=0.0)
+    if cf <= 0 or cw <= 0:
+        raise SystemExit("no calibration copy in the counter passes (run with --copy-mib)")
+    fs, ws = kib / cf, kib / cw
+    print(f"Calibration: the {copy_mib} MiB copy reads FETCH_SIZE = {cf:.0f} KiB and "
+          f"WRITE_SIZE = {cw:.0f} KiB, so FETCH_SIZE is scaled by {fs:.3f} and WRITE_SIZE "
+          f"by {ws:.3f}.\n")
+    print("| kernel | dispatches | FETCH KiB (raw avg) | WRITE KiB (raw avg) | scaled MB read "
+          "| scaled MB written |")
+    print("|---|---|---|---|---|---|")
+    rows = []
+    for k in sorted(set(fe) | set(wr)):
+        f_ = fe.get(k, [0.0])
+        w_ = wr.get(k, [0.0])
+        af, aw = sum(f_) / len(f_), sum(w_) / len(w_)
+        rows.append((k, len(f_), af, aw))
+    rows.sort(key=lambda r: -(r[2] * fs + r[3] * ws) * r[1])
+    for k, cnt, af, aw in rows[:40]:
+        print(f"| `{k[:90]}` | {cnt} | {af:.0f} | {aw:.0f} | {af * fs * 1.024e-3:.2f} | "
+              f"{aw * ws * 1.024e-3:.2f} |")
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        table = {"kernels": {}}
+    for key, prefix in keys.items():
+        hit = [r for r in rows if r[0].startswith(prefix)]
+        if not hit:
+            print(f"\nno dispatch named {prefix!r}")
+            continue
+        cnt = sum(r[1] for r in hit)
+        rd = sum(r[2] * r[1] for r in hit) / cnt * fs * 1024.0
+        wb = sum(r[3] * r[1] for r in hit) / cnt * ws * 1024.0
+        table["kernels"][key] = {"symbol": prefix, "workload": workload,
+                                 "fetch_scale": fs, "write_scale": ws,
+                                 "hbm_bytes_per_launch": rd + wb,
+                                 "read_bytes_per_launch": rd, "write_bytes_per_launch": wb}
+        print(f"\n{key}: {cnt} dispatches, {(rd + wb) / 1e6:.2f} MB per launch "
+              f"({rd / 1e6:.2f} read, {wb / 1e6:.2f} written)")
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1)
+
+
 def sin_theta(U, Uref):
     Q1, _ = np.linalg.qr(U)
     Q2, _ = np.linalg.qr(Uref)
@@ -744,7 +809,7 @@ def main_cknn(args):
             peak, note = PEAK_TFLOPS["fp32"], "similarities on FP32 MFMA (k_cand_gemm)"
         roof = {"bound": "mfma", "kernel": "cknn_partial", "achieved": achieved,
                 "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": measured_traffic("cknn_partial_c3", launches),
+                "traffic": measured_traffic("cknn_partial_c3_" + chosen, launches),
                 "avg_launch_us": avg_s * 1e6, "launches_per_step": launches,
                 "algorithmic_flops_per_launch": flops,
                 "candidate_pair_rows": work, "note": note}
@@ -1281,7 +1346,19 @@ def main():
                     help="cv workload: 'batched' skips the fold-by-fold plain path (profiling)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB first (counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="no GPU run: summarise two rocprofv3 counter passes (FETCH_SIZE, "
+                         "WRITE_SIZE) of a run made with --copy-mib into a per-dispatch table "
+                         "and profiles/hbm_traffic.json")
+    ap.add_argument("--pmc-key", action="append", default=[], metavar="NAME=KERNEL_PREFIX",
+                    help="with --pmc-summary: record the kernels named KERNEL_PREFIX... as NAME")
+    ap.add_argument("--pmc-workload", default="", help="with --pmc-summary: workload label")
     args = ap.parse_args()
+    if args.pmc_summary:
+        keys = dict(kv.split("=", 1) for kv in args.pmc_key)
+        pmc_summary(args.pmc_summary[0], args.pmc_summary[1], args.copy_mib or 1024, keys,
+                    args.pmc_workload)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
