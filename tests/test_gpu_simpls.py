@@ -26,6 +26,42 @@ def test_simpls_regression_workload_fp64(ctx):
     assert rel_err(yp, O.pls_predict(ref, Xt)) < 1e-8
 
 
+def test_simpls_rsvd_workload_full_shape_fp64(ctx):
+    """SIMPLS with ncomp = 50 at the full shape of the rSVD-of-X^T Y workload
+    (n = 20,000, p = 50,000, m = 100; X rank-80 signal s_i = 50 0.93^i under
+    N(0, 1) noise, Y = Y0c G / 10), X resident on the device in FP64 so the
+    device runs its FP64 kernels; the oracle draws the same seeded Gaussian
+    sketches.  R (sign-aligned), B and the fitted values agree to 1e-8."""
+    import torch
+    import fastpls_amd as F
+    n, p, m0, m, k, r = 20000, 50000, 20, 100, 50, 80
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(20505)
+    U = torch.linalg.qr(torch.randn(n, r, generator=g, device=dev, dtype=torch.float64))[0]
+    V = torch.linalg.qr(torch.randn(p, r, generator=g, device=dev, dtype=torch.float64))[0]
+    s = 50.0 * 0.93 ** torch.arange(r, device=dev, dtype=torch.float64)
+    Xt = torch.randn(p, n, generator=g, device=dev, dtype=torch.float64)
+    for j0 in range(0, p, 5000):
+        Xt[j0:j0 + 5000] += V[j0:j0 + 5000] @ (U * s).T
+    X = Xt.t()  # n x p, column-major on the device
+    Bt = torch.randn(p, m0, generator=g, device=dev, dtype=torch.float64) / p ** 0.5
+    Y0 = X @ Bt + torch.randn(n, m0, generator=g, device=dev, dtype=torch.float64)
+    Y0 -= Y0.mean(0)
+    Y = ((Y0 @ torch.randn(m0, m, generator=g, device=dev, dtype=torch.float64)) / 10.0)
+    Y = Y.t().contiguous().t()
+    del U, V, Bt, Y0
+    mod = F.pls(X, Y, method="simpls", ncomp=k, fit=True)
+    Xh = X.cpu().numpy()
+    Yh = Y.cpu().numpy()
+    del X, Xt, Y
+    torch.cuda.empty_cache()
+    ref = O.pls_model2_fast(Xh, Yh, [k], fit=True)
+    assert mod["n_components"] == ref["n_components"] == k
+    assert rel_err(align_signs(mod["R"], ref["R"]), ref["R"]) < 1e-8
+    assert rel_err(mod["B"], ref["B"]) < 1e-8
+    assert rel_err(mod["Yfit"], ref["Yfit"]) < 1e-8
+
+
 @pytest.mark.parametrize("ncomp", [[1, 3, 6], [4]])
 def test_simpls_fit_r2_fp64(ctx, ncomp):
     import fastpls_amd as F
