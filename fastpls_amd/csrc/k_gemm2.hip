@@ -303,7 +303,9 @@ __global__ __launch_bounds__(G2_NT) void k_g2_tn(int64_t M, int64_t N, int64_t K
                                                  const float* __restrict__ B, int64_t ldb,
                                                  const int32_t* __restrict__ labB,
                                                  const double* __restrict__ shB,
-                                                 float* __restrict__ partial) {
+                                                 float* __restrict__ partial, int sym) {
+  // sym: C is symmetric (A == B); tiles below the diagonal are left to the reduction
+  if (sym && blockIdx.y < blockIdx.x) return;
   __shared__ __align__(16) float smem[G2Smem<BF>::value];
   __shared__ float sha[G2_BM], shb[G2_BN];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -429,17 +431,25 @@ __global__ void k_g2_pad(const float* __restrict__ B, int64_t ldb, int64_t K, in
 // C(m, n) = sum_z P_z(m, n) in FP64, fixed order.  Block: 32 outputs x 8
 // split groups; group g sums splits g, g + 8, ... and the 8 group sums are
 // added in order, so many loads are in flight per output.
+// sym (M == N): the partials hold only the 128 x 128 tiles on and above the
+// diagonal; outputs below it read their mirror.
 __global__ __launch_bounds__(256) void k_g2_reduce(int64_t M, int64_t N, int splits,
                                                    const float* __restrict__ partial,
-                                                   double* __restrict__ C, int64_t ldc) {
+                                                   double* __restrict__ C, int64_t ldc,
+                                                   int sym) {
   __shared__ double red[8][32];
   const int64_t n = M * N;
   const int o = threadIdx.x & 31, g = threadIdx.x >> 5;
   const int64_t t = (int64_t)blockIdx.x * 32 + o;
   double s = 0.0;
   if (t < n) {
+    int64_t src = t;
+    if (sym) {
+      const int64_t i = t % M, j = t / M;
+      if (j / G2_BN < i / G2_BM) src = j + i * M;
+    }
 #pragma unroll 4
-    for (int z = g; z < splits; z += 8) s += static_cast<double>(partial[(int64_t)z * n + t]);
+    for (int z = g; z < splits; z += 8) s += static_cast<double>(partial[(int64_t)z * n + src]);
   }
   red[g][o] = s;
   __syncthreads();
@@ -945,6 +955,8 @@ void gemm_nn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
                      Cd, ldc, c_rowmajor);
 }
 
+constexpr int64_t kProjSliceRows = 32768, kProjSliceMaxN = 1024;
+
 void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K,
                         const float* A, int64_t lda, const double* shiftA,
                         const float* shift32, const float* B, const double* offset, float* Cf,
@@ -963,6 +975,22 @@ void gemm_nn_f32_padded(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K
                        Cf, Cd, ldc, crow};
     const Ctx::TuneChoice ch = proj_choose(ctx, sh);
     proj_run(ctx, sh, ch.mb, ch.splits);
+    return;
+  }
+  if (M >= kProjSliceRows && N <= kProjSliceMaxN) {
+    // tall A, a few hundred columns: one streaming pass of the skinny
+    // projection per 128 columns of B (B's padded columns make every slice
+    // whole) reads A once per slice at the streaming kernel's rate; the
+    // 128 x 128 tiles would read it as often, far slower
+    for (int64_t j0 = 0; j0 < N; j0 += 128) {
+      const int nj = (int)std::min<int64_t>(128, N - j0);
+      const int64_t co = crow ? j0 : j0 * ldc;
+      const ProjShape sh{bf, shiftA != nullptr, M, nj, K, A, lda, shift32, B + j0 * ldb, ldb,
+                         offset ? offset + j0 : nullptr, Cf ? Cf + co : nullptr,
+                         Cd ? Cd + co : nullptr, ldc, crow};
+      const Ctx::TuneChoice ch = proj_choose(ctx, sh);
+      proj_run(ctx, sh, ch.mb, ch.splits);
+    }
     return;
   }
   if (shiftA) {
@@ -1453,7 +1481,7 @@ void tn_skinny(Ctx* ctx, int precision, int64_t M, int N, int64_t K, const float
              A, lda, shA, Bf, Bh, Bl, ldb, part.get());
   }
   launch(ctx, "tn_skinny_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0,
-         M, (int64_t)N, (int)splits, (const float*)part.get(), C, ldc);
+         M, (int64_t)N, (int)splits, (const float*)part.get(), C, ldc, 0);
 }
 
 // The B operand of k_tns_dense from a column-major B (K x N, ld ldb): [n][k]
@@ -1671,7 +1699,7 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
       launch(ctx, "gemm_tn_f32", k_gram<false>, dim3((unsigned)splits), dim3(P_NT), 0, M, K,
              kchunk, A, lda, shiftA, part.get());
     launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
-           (const float*)part.get(), C, ldc);
+           (const float*)part.get(), C, ldc, 0);
     return;
   }
   // more classes than one 128-column one-hot tile: class sums in one pass over
@@ -1727,8 +1755,9 @@ void gemm_tn_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
               splits, C, ldc);
     return;
   }
+  const int64_t splits = tn_default_splits(ctx, M, N, K);
   gemm_tn_dense(ctx, precision, M, N, K, A, lda, shiftA, B, ldb, labelsB, shiftB, C, ldc,
-                tn_default_splits(ctx, M, N, K));
+                splits);
 }
 
 bool class_cross_cov_f32(Ctx* ctx, int precision, int64_t p, int64_t C, int64_t n,
@@ -1771,6 +1800,8 @@ void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, con
                    int64_t lda, const double* shiftA, const float* B, int64_t ldb,
                    const int32_t* labelsB, const double* shiftB, double* C, int64_t ldc,
                    int64_t splits) {
+  // the Gram matrix of A: only the tiles on and above the diagonal
+  const int sym = (!labelsB && A == B && lda == ldb && shiftA == shiftB && M == N) ? 1 : 0;
   if (M <= 0 || N <= 0) return;
   const int64_t cap = std::max<int64_t>(1, (int64_t)ctx->partial_budget() / (4 * M * N));
   splits = std::max<int64_t>(1, std::min(splits, cap));
@@ -1782,12 +1813,12 @@ void gemm_tn_dense(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, con
             (unsigned)splits);
   if (precision == FPLS_PREC_BF16X3)
     launch(ctx, "gemm_tn_bf16x3", k_g2_tn<true>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
-           shiftA, B, ldb, labelsB, shiftB, part.get());
+           shiftA, B, ldb, labelsB, shiftB, part.get(), sym);
   else
     launch(ctx, "gemm_tn_f32", k_g2_tn<false>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
-           shiftA, B, ldb, labelsB, shiftB, part.get());
+           shiftA, B, ldb, labelsB, shiftB, part.get(), sym);
   launch(ctx, "gemm_tn_reduce", k_g2_reduce, dim3((unsigned)((M * N + 31) / 32)), dim3(256), 0, M, N, (int)splits,
-         (const float*)part.get(), C, ldc);
+         (const float*)part.get(), C, ldc, sym);
 }
 
 // Split-K skinny NN (N <= 128): part[z] (M x N FP32, column-major) is slab z
@@ -1831,10 +1862,10 @@ void gemm_rr_f32(Ctx* ctx, int precision, int64_t M, int64_t N, int64_t K, const
   dim3 grid((unsigned)((M + G2_BM - 1) / G2_BM), (unsigned)((N + G2_BN - 1) / G2_BN), 1);
   if (precision == FPLS_PREC_BF16X3)
     launch(ctx, "gemm_rr_bf16x3", k_g2_tn<true>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
-           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C, 0);
   else
     launch(ctx, "gemm_rr_f32", k_g2_tn<false>, grid, dim3(G2_NT), 0, M, N, K, kchunk, A, lda,
-           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C);
+           (const double*)nullptr, B, ldb, (const int32_t*)nullptr, (const double*)nullptr, C, 0);
 }
 
 }  // namespace fpls
