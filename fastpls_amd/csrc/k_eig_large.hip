@@ -136,37 +136,42 @@ __global__ __launch_bounds__(LC_T) void k_lt_col(double* __restrict__ A, int n, 
   const double* __restrict__ vp = Rf + (int64_t)(pend ? j - 1 : 0) * n;
   const double* __restrict__ wp = Wc + (int64_t)((j - 1) & 1) * n;
   {
-    const int clo = max(c0, j), chi = min(c0 + LT_CB, n);
+    // wave wv takes columns c0 + wv + NW i; every load of the thread's
+    // LT_CB / NW columns is issued before the first is used, so a tile costs
+    // about one memory latency rather than one per pair of columns
+    // (the tile's entries of v_j, v_{j-1} and w_{j-1} go through LDS)
+    constexpr int CPT = LT_CB / NW;
+    __shared__ double sv[LT_CB], svp[LT_CB], swp[LT_CB];
     const int r = r0 + lane;
+    const bool row_ok = r < n && r >= j;
+    const double vr = (pend && row_ok) ? vp[r] : 0.0, wr = (pend && row_ok) ? wp[r] : 0.0;
+    double a[CPT];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int c = c0 + wv + NW * i;
+      a[i] = (row_ok && c >= j && c < n) ? A[r + (int64_t)c * n] : 0.0;
+    }
+    if (t < LT_CB) {
+      const int c = c0 + t;
+      sv[t] = c < n ? v[c] : 0.0;
+      svp[t] = (pend && c < n) ? vp[c] : 0.0;
+      swp[t] = (pend && c < n) ? wp[c] : 0.0;
+    }
+    __syncthreads();
     double acc0 = 0.0, acc1 = 0.0;
-    if (r < n && r >= j) {
-      const double vr = pend ? vp[r] : 0.0, wr = pend ? wp[r] : 0.0;
-      int c = clo + wv;
-      for (; c + NW < chi; c += 2 * NW) {
-        const int c1 = c + NW;
-        double a0 = A[r + (int64_t)c * n], a1 = A[r + (int64_t)c1 * n];
+#pragma unroll
+    for (int i = 0; i < CPT; ++i) {
+      const int c = c0 + wv + NW * i, cl = wv + NW * i;
+      if (row_ok && c >= j && c < n) {
+        double x = a[i];
         if (pend) {
-          a0 = fma(-vr, wp[c], fma(-wr, vp[c], a0));
-          a1 = fma(-vr, wp[c1], fma(-wr, vp[c1], a1));
-          A[r + (int64_t)c * n] = a0;
-          A[r + (int64_t)c1 * n] = a1;
+          x = fma(-vr, swp[cl], fma(-wr, svp[cl], x));
+          A[r + (int64_t)c * n] = x;
         }
-        acc0 = fma(a0, v[c], acc0);
-        acc1 = fma(a1, v[c1], acc1);
+        if (i & 1) acc1 = fma(x, sv[cl], acc1);
+        else acc0 = fma(x, sv[cl], acc0);
         if (c == j + 1 && r > j)
-          __hip_atomic_store(&xcol[r], a0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (c1 == j + 1 && r > j)
-          __hip_atomic_store(&xcol[r], a1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-      for (; c < chi; c += NW) {
-        double a0 = A[r + (int64_t)c * n];
-        if (pend) {
-          a0 = fma(-vr, wp[c], fma(-wr, vp[c], a0));
-          A[r + (int64_t)c * n] = a0;
-        }
-        acc0 = fma(a0, v[c], acc0);
-        if (c == j + 1 && r > j)
-          __hip_atomic_store(&xcol[r], a0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          __hip_atomic_store(&xcol[r], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
     }
     part[wv][lane] = acc0 + acc1;
