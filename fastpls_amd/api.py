@@ -106,6 +106,19 @@ def _encode_factor(y, n_classes=None):
     return list(lev), (codes + 1).astype(np.int32), len(lev)
 
 
+def _labels_from_codes(lev, codes):
+    """R's factor labels of 1-based class codes.  Levels that are the integers
+    1..C (device labels, or host labels coded that way) are the codes
+    themselves, returned without a per-row lookup."""
+    C = len(lev)
+    if C and isinstance(lev[0], (int, np.integer)) and lev[0] == 1 and lev[-1] == C:
+        return codes  # sorted unique integers from 1 to C: exactly 1..C
+    levs = np.asarray(lev)
+    if levs.dtype.kind not in "iufUS":
+        levs = np.asarray(lev, dtype=object)
+    return levs[codes - 1]
+
+
 def _check_backend(backend: str):
     if backend != "hip":
         raise ValueError("this package implements backend='hip' only")
@@ -284,7 +297,7 @@ def pls(Xtrain, Ytrain, Xtest=None, Ytest=None, ncomp=2, scaling="centering",
         model["classification_rule"] = "argmax"
     if fused_pred is not None:
         res = {"Ypred_index": fused_pred,
-               "Ypred": np.asarray(lev, dtype=object)[fused_pred - 1]}
+               "Ypred": _labels_from_codes(lev, fused_pred)}
         if tie_log:
             res["tie_margin"] = fused_margin
         if Ytest is not None:
@@ -485,7 +498,7 @@ def predict(model: FastPLS, newdata, Ytest=None, proj: bool = False, top: int = 
                                                labels.ctypes.data,
                                                sc.ctypes.data if want else None))
         res["Ypred_index"] = labels
-        res["Ypred"] = np.asarray(lev, dtype=object)[labels - 1]
+        res["Ypred"] = _labels_from_codes(lev, labels)
         if raw_scores:
             res["LDA_scores"] = sc
         if top > 1:
