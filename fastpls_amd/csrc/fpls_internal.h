@@ -327,6 +327,12 @@ void gaussian_dev(Ctx* ctx, const uint32_t* seeds_dev, int n_streams, int64_t le
 //  y(M) = (A - 1 shift^T) x      (A: M x K col-major; FP32 or FP64)
 //  y(K) = (A - 1 shift^T)^T x
 void gemv_n(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y);
+// y = (A - 1 shift^T) x, then y /= |y| and also (n2 entries) /= |y| in the same
+// launch (a zero or non-finite norm sets flags[0]; nothing is normalised once
+// it is set).  False when the shape does not fit the one-launch form
+// (rows <= 2048, 16-byte vector rows) and nothing was launched.
+bool gemv_n_normalize(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y,
+                      double* also, int64_t n2, int32_t* flags);
 void gemv_t(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y);
 
 // ---------------------------------------------------------------- kernels (k_stats.hip)

@@ -7,6 +7,7 @@
 //
 // Layouts are column-major throughout.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 
 #include "fpls_internal.h"
@@ -563,8 +564,20 @@ __device__ __forceinline__ V load_stream(const T* p) {
   return r;
 }
 
+// max or sum over a 256-thread workgroup (every thread gets the result)
+__device__ __forceinline__ double gn_block_reduce(double v, double* red, bool is_max) {
+  v = is_max ? wave_max_d(v) : wave_sum_d(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = red[0];
+#pragma unroll
+  for (int w = 1; w < 4; ++w) r = is_max ? fmax(r, red[w]) : r + red[w];
+  return r;
+}
+
 // One lane's VW consecutive rows of the K slab [kb, ke) of (A - 1 sh^T) x.
-template <typename T, int VW, int KU, typename Vec>
+template <typename T, int VW, int KU, typename Vec, bool AGENT_OUT = false>
 __device__ __forceinline__ void gemv_nv_part(int64_t kb, int64_t ke,
                                              const T* __restrict__ a0, int64_t lda,
                                              const double* __restrict__ sh,
@@ -592,7 +605,12 @@ __device__ __forceinline__ void gemv_nv_part(int64_t kb, int64_t ke,
     for (int r = 0; r < VW; ++r) s[r] = fma(static_cast<double>(a.v[r]) - sv, xv, s[r]);
   }
 #pragma unroll
-  for (int r = 0; r < VW; ++r) out[r] = s[r];
+  for (int r = 0; r < VW; ++r) {
+    if (AGENT_OUT)
+      __hip_atomic_store(&out[r], s[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      out[r] = s[r];
+  }
 }
 
 // The same partials with VW = 16 / sizeof(T) consecutive rows per lane (one
@@ -630,6 +648,118 @@ __global__ void k_sum_parts(int64_t M, int parts, const double* __restrict__ par
     for (; z < parts; ++z) s[0] += part[(int64_t)z * M + i];
     y[i] = (s[0] + s[1]) + (s[2] + s[3]);
   }
+}
+
+// y = (A - 1 sh^T) x over K slabs as k_gemv_nv + k_sum_parts, then, in the
+// last workgroup to finish (atomic ticket; partials written and read with
+// agent-scope atomics), y /= |y| and also /= |y| with the scaled norm of a
+// one-partial normalisation (a zero or non-finite norm sets flags[0] and
+// leaves y unnormalised; nothing is normalised once flags[0] is set).  One
+// launch for M <= GN_MMAX rows.
+constexpr int GN_MMAX = 2048;
+template <typename T>
+__global__ __launch_bounds__(256) void k_gemv_nv_norm(int64_t M, int64_t K, int64_t kchunk,
+                                                      const T* __restrict__ A, int64_t lda,
+                                                      const double* __restrict__ sh,
+                                                      const double* x, double* ypart, double* y,
+                                                      double* also, int64_t n2,
+                                                      int32_t* flags, unsigned* counter) {
+  constexpr int VW = 16 / sizeof(T);
+  constexpr int KU = 8;
+  struct alignas(16) Vec {
+    T v[VW];
+  };
+  __shared__ unsigned ticket;
+  __shared__ double red[4];
+  const int t = threadIdx.x;
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + t) * VW;
+  if (i0 < M) {
+    const int64_t kb = (int64_t)blockIdx.y * kchunk;
+    const int64_t ke = std::min<int64_t>(K, kb + kchunk);
+    gemv_nv_part<T, VW, KU, Vec, true>(kb, ke, A + i0, lda, sh, x,
+                                       ypart + (int64_t)blockIdx.y * M + i0);
+  }
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's partial stores have completed
+  __syncthreads();
+  if (t == 0)
+    ticket = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  if (ticket != gridDim.x * gridDim.y - 1) return;
+  if (t == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const int parts = (int)gridDim.y;
+  constexpr int NE = GN_MMAX / 256;
+  double v[NE], amax = 0.0, bad = 0.0;
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = t + (int64_t)e * 256;
+    v[e] = 0.0;
+    if (i < M) {
+      double s4[4] = {0.0, 0.0, 0.0, 0.0};
+      int z = 0;
+      for (; z + 4 <= parts; z += 4)
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          s4[u] += __hip_atomic_load(&ypart[(int64_t)(z + u) * M + i], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      for (; z < parts; ++z)
+        s4[0] += __hip_atomic_load(&ypart[(int64_t)z * M + i], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+      v[e] = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+      y[i] = v[e];
+    }
+    const double a = fabs(v[e]);
+    if (!(a <= DBL_MAX)) bad = 1.0;
+    else amax = fmax(amax, a);
+  }
+  if (flags[0]) return;  // uniform over the workgroup
+  bad = gn_block_reduce(bad, red, true);
+  amax = gn_block_reduce(amax, red, true);
+  double ssq = 0.0;
+  if (bad == 0.0 && amax > 0.0) {
+    const double inv = 1.0 / amax;
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const double u = v[e] * inv;
+      ssq = fma(u, u, ssq);
+    }
+  }
+  ssq = gn_block_reduce(ssq, red, false);
+  const double nr = bad != 0.0 ? __builtin_nan("") : amax * sqrt(ssq);
+  if (!(nr > 0.0 && nr <= DBL_MAX)) {
+    if (t == 0) flags[0] = 1;
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < NE; ++e) {
+    const int64_t i = t + (int64_t)e * 256;
+    if (i < M) y[i] = v[e] / nr;
+  }
+  if (also)
+    for (int64_t i = t; i < n2; i += 256) also[i] /= nr;
+}
+
+bool gemv_n_normalize(Ctx* ctx, const DMat& A, const double* shift, const double* x, double* y,
+                      double* also, int64_t n2, int32_t* flags) {
+  const int64_t M = A.rows, K = A.cols;
+  const int vw = A.dtype == FPLS_F32 ? 4 : 2;
+  if (M <= 0 || M > GN_MMAX || K <= 0 || M % vw != 0 || A.ld % vw != 0 ||
+      (reinterpret_cast<uintptr_t>(A.p) & 15) != 0 || ctx->arena_base)
+    return false;
+  // the K slabs of gemv_n's vector path
+  const int64_t rb = (M / vw + 255) / 256;
+  int64_t parts = std::max<int64_t>(1, std::min<int64_t>((8 * ctx->num_cus + rb - 1) / rb,
+                                                         (K + 15) / 16));
+  const int64_t kchunk = (K + parts - 1) / parts;
+  parts = std::max<int64_t>(1, (K + kchunk - 1) / kchunk);
+  DBuf<double> part(ctx, (size_t)parts * M);
+  dim3 grid((unsigned)rb, (unsigned)parts);
+  if (A.dtype == FPLS_F32)
+    launch(ctx, "gemv_n_norm", k_gemv_nv_norm<float>, grid, dim3(256), 0, M, K, kchunk, A.f(),
+           A.ld, shift, x, part.get(), y, also, n2, flags, ctx->ticket() + 1);
+  else
+    launch(ctx, "gemv_n_norm", k_gemv_nv_norm<double>, grid, dim3(256), 0, M, K, kchunk, A.d(),
+           A.ld, shift, x, part.get(), y, also, n2, flags, ctx->ticket() + 1);
+  return true;
 }
 
 // y(K): one wavefront per column, lanes stride the rows.

@@ -165,12 +165,60 @@ __global__ __launch_bounds__(SP_NB) void k_sp_norm_apply(const double* src, cons
     for (int64_t i = t0; i < n2; i += stride) also[i] /= nr;
 }
 
+// One-workgroup form of k_sp_norm_part + k_sp_norm_apply for n <= SP_NCH
+// (one partial): the same arithmetic, so the same result, in one launch.
+__global__ __launch_bounds__(SP_NB) void k_sp_norm_one(const double* src, const double* sub,
+                                                       double* dst, int64_t n, double* also,
+                                                       int64_t n2, int32_t* flags) {
+  __shared__ double red[SP_NB / 64];
+  if (flags[0]) return;
+  double v[SP_NE], amax = 0.0, bad = 0.0;
+#pragma unroll
+  for (int e = 0; e < SP_NE; ++e) {
+    const int64_t i = threadIdx.x + (int64_t)e * SP_NB;
+    v[e] = i < n ? src[i] - (sub ? sub[i] : 0.0) : 0.0;
+    const double a = fabs(v[e]);
+    if (!(a <= DBL_MAX)) bad = 1.0;
+    else amax = fmax(amax, a);
+  }
+  bad = blk_reduce<SP_NB>(bad, red, true);
+  amax = blk_reduce<SP_NB>(amax, red, true);
+  double ssq = 0.0;
+  if (bad == 0.0 && amax > 0.0) {
+    const double inv = 1.0 / amax;
+#pragma unroll
+    for (int e = 0; e < SP_NE; ++e) {
+      const double u = v[e] * inv;
+      ssq = fma(u, u, ssq);
+    }
+  }
+  ssq = blk_reduce<SP_NB>(ssq, red, false);
+  // k_sp_norm_apply with one partial: r = amax / amax = 1, ss = fma(1, ssq, 0)
+  const double nr = bad != 0.0 ? __builtin_nan("") : amax * sqrt(ssq);
+  if (!(nr > 0.0 && nr <= DBL_MAX)) {
+    if (threadIdx.x == 0) flags[0] = 1;
+    return;
+  }
+#pragma unroll
+  for (int e = 0; e < SP_NE; ++e) {
+    const int64_t i = threadIdx.x + (int64_t)e * SP_NB;
+    if (i < n) dst[i] = v[e] / nr;
+  }
+  if (also)
+    for (int64_t i = threadIdx.x; i < n2; i += SP_NB) also[i] /= nr;
+}
+
 // dst = (src - sub) / |src - sub| (sub may be NULL; dst may be src), also /=
 // the same norm; a zero or non-finite norm stops the fit.  part holds
 // ceil(n / SP_NCH) partials.
 static void sp_normalize(Ctx* c, const double* src, const double* sub, double* dst, int64_t n,
                          double* also, int64_t n2, double2* part, int32_t* flags) {
   const int nparts = (int)((n + SP_NCH - 1) / SP_NCH);
+  if (nparts == 1 && (!also || n2 <= 16 * SP_NCH)) {
+    launch(c, "simpls_norm", k_sp_norm_one, dim3(1), dim3(SP_NB), 0, src, sub, dst, n, also, n2,
+           flags);
+    return;
+  }
   launch(c, "simpls_norm", k_sp_norm_part, dim3(nparts), dim3(SP_NB), 0, src, sub, n, part,
          (const int32_t*)flags);
   const int64_t work = std::max(n, also ? n2 : 0);
@@ -430,6 +478,7 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
                             c->stream));
   int32_t* fl = flags.get();
   const bool small = can_incremental && p <= SP_PMAX && m <= SP_MMAX && K <= SP_MMAX;
+  const bool unit_scale = M->scaling != 2;  // vX = 1: centring only, or none
   int i_out = 0;
   for (int a = 0; a < K; ++a) {
     // an occasional look at the stop flag spares a long tail of dead
@@ -476,14 +525,20 @@ void simpls_fit_dev(Ctx* c, Model* M, const DMat& X, const DMat& Y, const int32_
       }
     } else {
       // tt = Xs rr = Xc (rr / vX); |t| (stop test); tt, rr /= |t|
-      launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, rr.get(), M->vX.get(), p,
-             rs.get());
-      gemv_n(c, X, M->mX.get(), rs.get(), tt.get());
-      sp_normalize(c, tt.get(), nullptr, tt.get(), n, rr.get(), p, npart.get(), fl);
+      // (without autoscaling vX is all ones and the divisions are identities)
+      if (!unit_scale)
+        launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, rr.get(), M->vX.get(), p,
+               rs.get());
+      const double* xin = unit_scale ? rr.get() : rs.get();
+      if (!gemv_n_normalize(c, X, M->mX.get(), xin, tt.get(), rr.get(), p, fl)) {
+        gemv_n(c, X, M->mX.get(), xin, tt.get());
+        sp_normalize(c, tt.get(), nullptr, tt.get(), n, rr.get(), p, npart.get(), fl);
+      }
       // pp = Xs^T tt = (Xc^T tt) / vX
       gemv_t(c, X, M->mX.get(), tt.get(), pp.get());
-      launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, pp.get(), M->vX.get(), p,
-             pp.get());
+      if (!unit_scale)
+        launch(c, "div_vec", k_div_vec, dim3(g1(p)), dim3(256), 0, pp.get(), M->vX.get(), p,
+               pp.get());
       // qq = Yc^T tt
       if (y)
         gemm_f64(c, true, false, 1, m, n, 1.0, tt.get(), n, nullptr, nullptr, n, M->mY.get(), 0.0,
