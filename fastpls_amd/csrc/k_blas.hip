@@ -745,10 +745,12 @@ bool gemv_n_normalize(Ctx* ctx, const DMat& A, const double* shift, const double
   if (M <= 0 || M > GN_MMAX || K <= 0 || M % vw != 0 || A.ld % vw != 0 ||
       (reinterpret_cast<uintptr_t>(A.p) & 15) != 0 || ctx->arena_base)
     return false;
-  // the K slabs of gemv_n's vector path
+  // K slabs of at least 128 columns: the last workgroup's sum over the
+  // partials is a chain of dependent L2 round trips, so fewer, longer slabs
+  // (the matrices that fit this form are at most 2048 rows) finish sooner
   const int64_t rb = (M / vw + 255) / 256;
   int64_t parts = std::max<int64_t>(1, std::min<int64_t>((8 * ctx->num_cus + rb - 1) / rb,
-                                                         (K + 15) / 16));
+                                                         (K + 127) / 128));
   const int64_t kchunk = (K + parts - 1) / parts;
   parts = std::max<int64_t>(1, (K + kchunk - 1) / kchunk);
   DBuf<double> part(ctx, (size_t)parts * M);
