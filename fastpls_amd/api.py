@@ -1439,7 +1439,9 @@ def pls_double_cv(Xdata, Ydata, ncomp=2, outer_fold=None, inner_fold=None, scali
     seed + 1000 run + f + inner fold (R/main.R:6708 and src/fastPLS.cpp:6906;
     f counts outer folds from 1); the optimal ncomp of each outer fold
     maximises inner accuracy (first maximum, R/main.R:6527-6536); the outer
-    fits (seed + 2000 run + f, R/main.R:6738) run as a second batched call."""
+    fits (seed + 2000 run + f, R/main.R:6738) run as a second batched call.
+    Missing values in Xdata stop it, as pls.double.cv does (R/main.R:6610-6612)."""
+    _check_finite(Xdata)
     keep_shape = np.asarray(Xdata).shape if not type(Xdata).__module__.startswith("torch") \
         else tuple(Xdata.shape)
     n = keep_shape[0]

@@ -79,6 +79,10 @@ def test_cv_rejects_missing_values():
     X[2, 1] = np.nan
     with pytest.raises(ValueError, match="Missing values are present"):
         F.optim_pls_cv(X, np.arange(6.0), ncomp=1)
+    with pytest.raises(ValueError, match="Missing values are present"):
+        F.pls_double_cv(X, np.array([1, 2, 1, 2, 1, 2]), ncomp=1,
+                        outer_fold=np.array([1, 1, 2, 2, 3, 3]),
+                        inner_fold=np.ones((6, 3), dtype=int))
 
 
 def test_should_use_xprod_default_rule():
