@@ -194,33 +194,55 @@ __global__ __launch_bounds__(LC_T) void k_lt_col(double* __restrict__ A, int n, 
   // ---- the last workgroup
   if (t == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const int ncc = gridDim.x;
-  double* ys = lt_sm;       // n: w_j
   double* xs = lt_sm + n;   // n: column j + 1
+  __shared__ double wj1s;
   const double tau = tri[2 * n + j];
+  const bool more = j + 1 <= n - 3;
+  // this thread's rows r = j + 1 + t + LC_T q: the partials of (A v_j)(r) and
+  // column j + 1 are loaded together (one round of memory latency)
+  constexpr int NQ = LT_NMAX / LC_T;
+  static_assert(LT_NMAX % LC_T == 0, "rows per thread");
+  double ysr[NQ], xr[NQ], vq[NQ];
   double dv = 0.0;
-  for (int r = j + 1 + t; r < n; r += LC_T) {
-    double s = 0.0;
-    for (int q = 0; q < ncc; ++q)
-      s += __hip_atomic_load(&Yp[(int64_t)q * n + r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s *= tau;
-    ys[r] = s;
-    dv = fma(s, v[r], dv);
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = j + 1 + t + LC_T * q;
+    ysr[q] = 0.0;
+    xr[q] = 0.0;
+    vq[q] = 0.0;
+    if (r < n) {
+      double s = 0.0;
+      for (int c = 0; c < ncc; ++c)
+        s += __hip_atomic_load(&Yp[(int64_t)c * n + r], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      if (more)
+        xr[q] = __hip_atomic_load(&xcol[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      vq[q] = v[r];
+      s *= tau;
+      ysr[q] = s;
+      dv = fma(s, vq[q], dv);
+    }
   }
   const double alpha = -0.5 * tau * lt_block_sum<LC_T>(dv, red);
   double* __restrict__ wj = Wc + (int64_t)(j & 1) * n;
-  for (int r = t; r < n; r += LC_T) {
-    double wr = 0.0;
-    if (r > j) wr = fma(alpha, v[r], ys[r]);
-    wj[r] = wr;
-    ys[r] = wr;
+  for (int r = t; r <= j && r < n; r += LC_T) wj[r] = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = j + 1 + t + LC_T * q;
+    if (r < n) {
+      ysr[q] = fma(alpha, vq[q], ysr[q]);  // w_j(r)
+      wj[r] = ysr[q];
+      if (r == j + 1) wj1s = ysr[q];
+    }
   }
-  if (j + 1 > n - 3) return;
+  if (!more) return;
   __syncthreads();
   // column j + 1 after the update by (v_j, w_j)
-  const double wj1 = ys[j + 1], vj1 = v[j + 1];
-  for (int r = j + 1 + t; r < n; r += LC_T) {
-    const double x = __hip_atomic_load(&xcol[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    xs[r] = fma(-v[r], wj1, fma(-ys[r], vj1, x));
+  const double wj1 = wj1s, vj1 = v[j + 1];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = j + 1 + t + LC_T * q;
+    if (r < n) xs[r] = fma(-vq[q], wj1, fma(-ysr[q], vj1, xr[q]));
   }
   __syncthreads();
   lt_reflector<LC_T>(xs, j + 1, n, Rf, tri, red);
