@@ -409,8 +409,9 @@ constexpr int CKH_KS = 32;                      // halves per K step
 constexpr int CKH_LD = CKH_KS + 8;              // LDS row stride (80 bytes)
 constexpr int CKH_ROWS = CKG_P + CKG_R;         // operand rows per buffer (A then B)
 constexpr int CKH_BUF = 2 * CKH_ROWS * CKH_LD;  // halves per buffer (hi, lo)
-constexpr int CKH_OPS = 2 * CKH_BUF * 2;        // bytes, both buffers
-constexpr int CKH_SMEM = CKH_OPS > CKG_SIM * 4 ? CKH_OPS : CKG_SIM * 4;
+constexpr int CKH_OPS = CKH_BUF * 2;            // bytes, one buffer
+constexpr int CKH_SIMH = (CKG_P / 2) * (CKG_R + 1) * 4;  // bytes, half a Sim tile
+constexpr int CKH_SMEM = CKH_OPS > CKH_SIMH ? CKH_OPS : CKH_SIMH;
 constexpr float CKH_SCALE = 4096.0f;
 
 __global__ void k_split_f16(const float* __restrict__ X, int64_t rows, int k, int kp,
@@ -435,7 +436,8 @@ __global__ void k_split_f16(const float* __restrict__ X, int64_t rows, int k, in
   }
 }
 
-__global__ __launch_bounds__(256) void k_cand_gemm_h(
+template <int KM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_cand_gemm_h(
     const _Float16* __restrict__ Tth, const _Float16* __restrict__ Ttl, int kp,
     const _Float16* __restrict__ Tnh, const _Float16* __restrict__ Tnl,
     const int* __restrict__ bucket, int m, const CkItem* __restrict__ items, int knn_k,
@@ -454,12 +456,10 @@ __global__ __launch_bounds__(256) void k_cand_gemm_h(
   const _Float16* ah = Tth + tr * kp;
   const _Float16* al = Ttl + tr * kp;
   // hi at [buf][0][row][k], lo at [buf][1][row][k]; A rows 0..127, B rows 128..255
-  auto lds = [&](int buf, int lo, int row) {
-    return hs + buf * CKH_BUF + lo * CKH_ROWS * CKH_LD + row * CKH_LD;
-  };
-  float L[CK_KNN_MAX];
+  auto lds = [&](int lo, int row) { return hs + lo * CKH_ROWS * CKH_LD + row * CKH_LD; };
+  float L[KM];
 #pragma unroll
-  for (int j = 0; j < CK_KNN_MAX; ++j) L[j] = -INFINITY;
+  for (int j = 0; j < KM; ++j) L[j] = -INFINITY;
   float thr = -INFINITY;
   const int nks = kp / CKH_KS;
   for (int r0 = it.row_beg; r0 < it.row_end; r0 += CKG_R) {
@@ -485,38 +485,37 @@ __global__ __launch_bounds__(256) void k_cand_gemm_h(
     q6 = *reinterpret_cast<const uint4*>(bl + o_);                                 \
     q7 = *reinterpret_cast<const uint4*>(bl + o_ + 8);                         \
   } while (0)
-#define CKH_STORE(buf)                                                                 \
+#define CKH_STORE()                                                                    \
   do {                                                                                 \
-    *reinterpret_cast<uint4*>(lds((buf), 0, srow) + sk) = q0;                          \
-    *reinterpret_cast<uint4*>(lds((buf), 0, srow) + sk + 8) = q1;                      \
-    *reinterpret_cast<uint4*>(lds((buf), 1, srow) + sk) = q2;                          \
-    *reinterpret_cast<uint4*>(lds((buf), 1, srow) + sk + 8) = q3;                      \
-    *reinterpret_cast<uint4*>(lds((buf), 0, CKG_P + srow) + sk) = q4;                  \
-    *reinterpret_cast<uint4*>(lds((buf), 0, CKG_P + srow) + sk + 8) = q5;              \
-    *reinterpret_cast<uint4*>(lds((buf), 1, CKG_P + srow) + sk) = q6;                  \
-    *reinterpret_cast<uint4*>(lds((buf), 1, CKG_P + srow) + sk + 8) = q7;              \
+    *reinterpret_cast<uint4*>(lds(0, srow) + sk) = q0;                          \
+    *reinterpret_cast<uint4*>(lds(0, srow) + sk + 8) = q1;                      \
+    *reinterpret_cast<uint4*>(lds(1, srow) + sk) = q2;                          \
+    *reinterpret_cast<uint4*>(lds(1, srow) + sk + 8) = q3;                      \
+    *reinterpret_cast<uint4*>(lds(0, CKG_P + srow) + sk) = q4;                  \
+    *reinterpret_cast<uint4*>(lds(0, CKG_P + srow) + sk + 8) = q5;              \
+    *reinterpret_cast<uint4*>(lds(1, CKG_P + srow) + sk) = q6;                  \
+    *reinterpret_cast<uint4*>(lds(1, CKG_P + srow) + sk + 8) = q7;              \
   } while (0)
     CKH_LOAD(0);
     __syncthreads();  // the previous tile's Sim reads are done
-    CKH_STORE(0);
+    CKH_STORE();
     __syncthreads();
     const int g8 = 8 * (lane >> 4), li = lane & 15;
     for (int st = 0; st < nks; ++st) {
       const bool more = st + 1 < nks;
       if (more) CKH_LOAD((st + 1) * CKH_KS);
-      const int buf = st & 1;
       h8v ahf[4], alf[4], bhf[4], blf[4];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
         const int row = ph * 64 + a * 16 + li;
-        ahf[a] = *reinterpret_cast<const h8v*>(lds(buf, 0, row) + g8);
-        alf[a] = *reinterpret_cast<const h8v*>(lds(buf, 1, row) + g8);
+        ahf[a] = *reinterpret_cast<const h8v*>(lds(0, row) + g8);
+        alf[a] = *reinterpret_cast<const h8v*>(lds(1, row) + g8);
       }
 #pragma unroll
       for (int b = 0; b < 4; ++b) {
         const int row = CKG_P + th * 64 + b * 16 + li;
-        bhf[b] = *reinterpret_cast<const h8v*>(lds(buf, 0, row) + g8);
-        blf[b] = *reinterpret_cast<const h8v*>(lds(buf, 1, row) + g8);
+        bhf[b] = *reinterpret_cast<const h8v*>(lds(0, row) + g8);
+        blf[b] = *reinterpret_cast<const h8v*>(lds(1, row) + g8);
       }
 #pragma unroll
       for (int a = 0; a < 4; ++a)
@@ -526,44 +525,53 @@ __global__ __launch_bounds__(256) void k_cand_gemm_h(
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alf[a], bhf[b], acc[a][b], 0, 0, 0);
           acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahf[a], bhf[b], acc[a][b], 0, 0, 0);
         }
-      if (more) CKH_STORE((st + 1) & 1);
-      __syncthreads();
+      __syncthreads();  // every wave is done reading this step's operands
+      if (more) {
+        CKH_STORE();
+        __syncthreads();
+      }
     }
 #undef CKH_LOAD
 #undef CKH_STORE
-    // Sim tile through LDS (aliasing the operands): lane holds
-    // D[4 (l / 16) + r][l % 16] of each block, scaled by 2^24
+    // Sim through LDS (aliasing the operands) in two halves of 64 pairs, so
+    // the workgroup needs half the LDS: lane holds D[4 (l / 16) + r][l % 16]
+    // of each block, scaled by 2^24
     float* Sim = reinterpret_cast<float*>(smem_raw);
     const int r4 = 4 * (lane >> 4);
     constexpr float unscale = 1.0f / (CKH_SCALE * CKH_SCALE);
+    for (int h = 0; h < 2; ++h) {
+      if (ph == h) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 4; ++a)
 #pragma unroll
-      for (int b = 0; b < 4; ++b)
+          for (int b = 0; b < 4; ++b)
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
-          Sim[(ph * 64 + a * 16 + r4 + r) * (CKG_R + 1) + th * 64 + b * 16 + li] =
-              acc[a][b][r] * unscale;
-    __syncthreads();
-    if (srow < np) {
-      const int cend = min(CKG_R, it.row_end - r0);
-      const float* srow_p = Sim + srow * (CKG_R + 1);
-      for (int c = (t & 1) * 64; c < (t & 1) * 64 + 64 && c < cend; ++c) {
-        float v = srow_p[c];
-        if (v > thr) {
+            for (int r = 0; r < 4; ++r)
+              Sim[(a * 16 + r4 + r) * (CKG_R + 1) + th * 64 + b * 16 + li] =
+                  acc[a][b][r] * unscale;
+      }
+      __syncthreads();
+      if (srow < np && (srow >> 6) == h) {
+        const int cend = min(CKG_R, it.row_end - r0);
+        const float* srow_p = Sim + (srow - 64 * h) * (CKG_R + 1);
+        for (int c = (t & 1) * 64; c < (t & 1) * 64 + 64 && c < cend; ++c) {
+          float v = srow_p[c];
+          if (v > thr) {
 #pragma unroll
-          for (int j = 0; j < CK_KNN_MAX; ++j) {
-            if (j < knn_k && v > L[j]) {
-              const float o = L[j];
-              L[j] = v;
-              v = o;
+            for (int j = 0; j < KM; ++j) {
+              if (j < knn_k && v > L[j]) {
+                const float o = L[j];
+                L[j] = v;
+                v = o;
+              }
             }
-          }
 #pragma unroll
-          for (int j = 0; j < CK_KNN_MAX; ++j)
-            if (j == knn_k - 1) thr = L[j];
+            for (int j = 0; j < KM; ++j)
+              if (j == knn_k - 1) thr = L[j];
+          }
         }
       }
+      __syncthreads();  // the half's reads are done before the next write
     }
   }
   if (srow < np) {
@@ -571,7 +579,7 @@ __global__ __launch_bounds__(256) void k_cand_gemm_h(
     float* dst = part + (int64_t)it.part_base +
                  ((int64_t)slot * it.nchunks + it.chunk) * 2 * knn_k + (t & 1) * knn_k;
 #pragma unroll
-    for (int j = 0; j < CK_KNN_MAX; ++j)
+    for (int j = 0; j < KM; ++j)
       if (j < knn_k) dst[j] = L[j];
   }
 }
@@ -816,7 +824,9 @@ void cknn_predict_core(Ctx* ctx, int precision, const float* Ttn, int64_t w, int
              th.get(), tl.get());
       launch(ctx, "cknn_split", k_split_f16, dim3(g1(ntr * (kp / 8))), dim3(256), 0, Tn, ntr, k,
              kp, nh.get(), nl.get());
-      launch(ctx, "cknn_partial", k_cand_gemm_h, dim3((unsigned)items.size()), dim3(256), 0,
+      // list capacity: 16 registers for knn_k <= 16, else 32
+      auto kern = knn_k <= 16 ? k_cand_gemm_h<16> : k_cand_gemm_h<CK_KNN_MAX>;
+      launch(ctx, "cknn_partial", kern, dim3((unsigned)items.size()), dim3(256), 0,
              (const _Float16*)th.get(), (const _Float16*)tl.get(), kp, (const _Float16*)nh.get(),
              (const _Float16*)nl.get(), (const int*)bucket.get(), m,
              (const CkItem*)d_items.get(), knn_k, part.get());
