@@ -295,15 +295,18 @@ def test_plssvd_xprod_route_matches_oracle(ctx, dtype, scaling, tol):
     assert rel_err(formed["B"], m["B"]) < tol
 
 
-def test_plssvd_fp32_wide_gram_fits(ctx):
+@pytest.mark.parametrize("n", [3000, 40000])
+def test_plssvd_fp32_wide_gram_fits(ctx, n):
     """FP32 dense-response PLSSVD with ncomp = 150: the training-score Gram
     T^T T (150 x 150) takes the tiled TN product that computes only the tiles
-    on and above the diagonal and mirrors the rest in its reduction.  Fitted
-    values and coefficients against the FP64 oracle (exact SVD on both sides)
-    to the FP32 input rounding (rel 1e-3; a wrong mirror gives O(1))."""
+    on and above the diagonal and mirrors the rest in its reduction; at
+    n = 40,000 rows the scores T = Xc R (150 columns) run as 128-column
+    passes of the streaming projection.  Fitted values and coefficients
+    against the FP64 oracle (exact SVD on both sides) to the FP32 input
+    rounding (rel 1e-3; a wrong mirror or slice gives O(1))."""
     import fastpls_amd as F
     rng = np.random.default_rng(23)
-    n, p, m, k = 3000, 300, 200, 150
+    p, m, k = 300, 200, 150
     X = (rng.standard_normal((n, p)) * np.linspace(2.0, 0.5, p)).astype(np.float32)
     W = rng.standard_normal((p, m)) / np.sqrt(p)
     Y = (X.astype(np.float64) @ W + 0.1 * rng.standard_normal((n, m))).astype(np.float32)
